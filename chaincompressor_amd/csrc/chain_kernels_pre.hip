@@ -1,0 +1,779 @@
+// Pre-entropy kernels of the Chain compressor for gfx950 (CDNA4).
+//
+// Stage map to the reference CPU path (cpp/chain/lcps_cpp.cpp):
+//   k_bbox / k_bbox_finish   minima                         :2072-2079
+//   k_quantize               quantise, split, corrections   :2091-2110, 1494-1523
+//   k_sort_*                 L1 grouping by block (stable)  :2114-2138
+//   k_link                   successor + chain decision     :2155-2200
+//   k_explicit               explicit block records         :2196-2199
+//   k_l2_stats / k_l2_split  L2 diff/keep/choice2           :1305-1454, 1456-1478
+// All work is 64-wide-wavefront native: ballots for ranking/compaction,
+// LDS-staged tiles of 4096 items, single-pass decoupled look-back scans.
+#include "chain_common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace chain {
+
+// ------------------------------------------------------------------------
+// small device helpers
+
+__device__ inline uint64_t okey(double x) {      // order-preserving key, -0 < +0
+    uint64_t b = __double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+__device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// inclusive scan of a u32 inside one 64-lane wavefront
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+        uint32_t t = __shfl_up(v, k, 64);
+        if (lane >= (uint32_t)k) v += t;
+    }
+    return v;
+}
+
+// exclusive scan over a 256-thread workgroup; returns the exclusive prefix,
+// *total = sum over the workgroup. s_tmp must hold 4 entries.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) s_tmp[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < CH_WG / 64; ++k) {
+        uint32_t t = s_tmp[k];
+        before += k < w ? t : 0u;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+// largest f with begin[f] <= x (begin ascending, begin[0] = 0)
+__device__ inline uint32_t upper_find(const uint32_t* begin, uint32_t nframes, uint32_t x) {
+    uint32_t lo = 0, hi = nframes;   // answer in [lo, hi)
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (begin[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------------
+// single-pass decoupled look-back over tiles (status word: 2-bit flag | 62-bit value)
+
+#define LB_AGG (1ull << 62)
+#define LB_INC (2ull << 62)
+#define LB_MASK ((1ull << 62) - 1ull)
+#define LB_SPIN_LIMIT (1u << 24)
+
+__device__ inline void lb_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t lb_load(uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by all lanes of wave 0. Tiles are numbered by an atomic ticket, so
+// every predecessor is already resident or finished and publishes its
+// aggregate before it looks back itself: the wait always ends. A bounded
+// spin sets *err instead of waiting forever.
+__device__ inline uint64_t lookback(uint64_t* status, uint32_t tile, uint64_t aggregate, uint32_t* err) {
+    const uint32_t lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) lb_store(&status[0], LB_INC | aggregate);
+        return 0;
+    }
+    if (lane == 0) lb_store(&status[tile], LB_AGG | aggregate);
+    uint64_t excl = 0;
+    int64_t t = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (t >= 0) {
+        // each lane inspects one predecessor, nearest first
+        const int64_t mine = t - (int64_t)lane;
+        uint64_t s = mine >= 0 ? lb_load(&status[mine]) : LB_INC;
+        const uint64_t not_ready = __ballot((s >> 62) == 0);
+        const uint64_t inc_mask = __ballot((s >> 62) == 2);
+        // lanes up to (and including) the first inclusive one, stopping at
+        // the first not-ready one
+        const uint64_t first_inc = inc_mask ? (inc_mask & (~inc_mask + 1)) : 0;
+        const uint64_t first_nr = not_ready ? (not_ready & (~not_ready + 1)) : 0;
+        if (first_nr && (!first_inc || first_nr < first_inc)) {
+            // sum the ready aggregates before the first not-ready lane, then wait
+            const uint64_t take = first_nr - 1;
+            uint64_t v = ((take >> lane) & 1ull) ? (s & LB_MASK) : 0ull;
+            for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, 64);
+            excl += v;
+            t -= __popcll(take);
+            if (++spins > LB_SPIN_LIMIT) { if (lane == 0) atomicOr(err, 1u); break; }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const uint64_t take = first_inc ? (first_inc | (first_inc - 1)) : ~0ull;
+        uint64_t v = ((take >> lane) & 1ull) ? (s & LB_MASK) : 0ull;
+        for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, 64);
+        excl += v;
+        if (first_inc) break;
+        t -= 64;
+    }
+    if (lane == 0) lb_store(&status[tile], LB_INC | (excl + aggregate));
+    return excl;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------------------------
+// generic single-pass exclusive scan of u32 -> u32 (used for radix offsets
+// and per-tile counts)
+
+__global__ __launch_bounds__(CH_WG) void k_scan_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                    uint64_t n, uint64_t* status, uint32_t* ticket, uint32_t* err,
+                                                    uint64_t* total_out) {
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
+    uint32_t v[CH_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t tot;
+    uint32_t pre = block_excl_scan(sum, s_tmp, &tot);
+    if (threadIdx.x < 64) {
+        uint64_t e = lookback(status, tile, tot, err);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    __syncthreads();
+    uint64_t run = s_excl + pre;
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = (uint32_t)run;
+        run += v[k];
+    }
+    if (total_out && base < n && base + CH_ITEMS >= n) *total_out = run;
+}
+
+// ------------------------------------------------------------------------
+// K1: per-tile minima/maxima of every dimension; a second kernel folds the
+// tiles of each frame. The first +0 and -0 index are kept so the host can
+// reproduce std::min's "first of equal values" choice for a zero minimum.
+
+template <typename T, int D, bool SOA>
+__device__ inline double load_coord(const T* pts, const FrameDev& F, uint64_t i, int d) {
+    if (SOA) return (double)pts[F.pt_off * D + (uint64_t)d * F.n + i];
+    return (double)pts[(F.pt_off + i) * D + d];
+}
+
+template <typename T, int D, bool SOA>
+__global__ __launch_bounds__(CH_WG) void k_bbox(const T* __restrict__ pts, const FrameDev* __restrict__ fr,
+                                                const uint32_t* __restrict__ tile_begin, uint32_t nframes,
+                                                uint64_t* __restrict__ part, uint32_t* __restrict__ bad) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t f = upper_find(tile_begin, nframes, tile);
+    const FrameDev& F = fr[f];
+    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
+    const uint64_t end = start + CH_TILE < F.n ? start + CH_TILE : F.n;
+    uint64_t mn[D], mx[D], zp[D], zn[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) { mn[d] = ~0ull; mx[d] = 0; zp[d] = ~0ull; zn[d] = ~0ull; }
+    bool nonfinite = false;
+    for (uint64_t i = start + threadIdx.x; i < end; i += CH_WG) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const double x = load_coord<T, D, SOA>(pts, F, i, d);
+            nonfinite |= !isfinite(x);
+            const uint64_t k = okey(x);
+            mn[d] = k < mn[d] ? k : mn[d];
+            mx[d] = k > mx[d] ? k : mx[d];
+            if (x == 0.0) {
+                if (signbit(x)) zn[d] = i < zn[d] ? i : zn[d];
+                else zp[d] = i < zp[d] ? i : zp[d];
+            }
+        }
+    }
+    __shared__ uint64_t s_red[4][CH_MAXD][4];
+    const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        uint64_t a = mn[d], b = mx[d], c = zp[d], e = zn[d];
+        for (int k = 32; k >= 1; k >>= 1) {
+            uint64_t ta = __shfl_xor(a, k, 64), tb = __shfl_xor(b, k, 64);
+            uint64_t tc = __shfl_xor(c, k, 64), te = __shfl_xor(e, k, 64);
+            a = ta < a ? ta : a; b = tb > b ? tb : b; c = tc < c ? tc : c; e = te < e ? te : e;
+        }
+        if (lane_id() == 0) { s_red[w][d][0] = a; s_red[w][d][1] = b; s_red[w][d][2] = c; s_red[w][d][3] = e; }
+    }
+    if (nonfinite) atomicOr(&bad[f], 1u);
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)D) {
+        const int d = threadIdx.x;
+        uint64_t a = ~0ull, b = 0, c = ~0ull, e = ~0ull;
+        for (int k = 0; k < 4; ++k) {
+            a = s_red[k][d][0] < a ? s_red[k][d][0] : a;
+            b = s_red[k][d][1] > b ? s_red[k][d][1] : b;
+            c = s_red[k][d][2] < c ? s_red[k][d][2] : c;
+            e = s_red[k][d][3] < e ? s_red[k][d][3] : e;
+        }
+        uint64_t* o = part + ((uint64_t)tile * D + d) * 4;
+        o[0] = a; o[1] = b; o[2] = c; o[3] = e;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(CH_WG) void k_bbox_finish(const uint64_t* __restrict__ part,
+                                                       const uint32_t* __restrict__ tile_begin,
+                                                       uint64_t* __restrict__ frame_box) {
+    const uint32_t f = blockIdx.x;
+    const uint32_t t0 = tile_begin[f], t1 = tile_begin[f + 1];
+    __shared__ uint64_t s_red[4][CH_MAXD][4];
+    const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        uint64_t a = ~0ull, b = 0, c = ~0ull, e = ~0ull;
+        for (uint32_t t = t0 + threadIdx.x; t < t1; t += CH_WG) {
+            const uint64_t* o = part + ((uint64_t)t * D + d) * 4;
+            a = o[0] < a ? o[0] : a; b = o[1] > b ? o[1] : b; c = o[2] < c ? o[2] : c; e = o[3] < e ? o[3] : e;
+        }
+        for (int k = 32; k >= 1; k >>= 1) {
+            uint64_t ta = __shfl_xor(a, k, 64), tb = __shfl_xor(b, k, 64);
+            uint64_t tc = __shfl_xor(c, k, 64), te = __shfl_xor(e, k, 64);
+            a = ta < a ? ta : a; b = tb > b ? tb : b; c = tc < c ? tc : c; e = te < e ? te : e;
+        }
+        if (lane_id() == 0) { s_red[w][d][0] = a; s_red[w][d][1] = b; s_red[w][d][2] = c; s_red[w][d][3] = e; }
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)D) {
+        const int d = threadIdx.x;
+        uint64_t a = ~0ull, b = 0, c = ~0ull, e = ~0ull;
+        for (int k = 0; k < 4; ++k) {
+            a = s_red[k][d][0] < a ? s_red[k][d][0] : a;
+            b = s_red[k][d][1] > b ? s_red[k][d][1] : b;
+            c = s_red[k][d][2] < c ? s_red[k][d][2] : c;
+            e = s_red[k][d][3] < e ? s_red[k][d][3] : e;
+        }
+        uint64_t* o = frame_box + ((uint64_t)f * D + d) * 4;
+        o[0] = a; o[1] = b; o[2] = c; o[3] = e;
+    }
+}
+
+// ------------------------------------------------------------------------
+// K2: FP64 quantisation, block split, concatenated block key and exact
+// corrections (cpp/chain/lcps_cpp.cpp:2091-2110 with :1494-1523).
+
+__device__ inline bool within_eb(double restored, double original, double eb) {
+    if (!isfinite(restored)) return false;
+    const double difference = restored - original;
+    const double magnitude = fabs(difference);
+    if (magnitude < eb) return true;
+    if (!(magnitude == eb)) return false;
+    const double virtual_original = difference - restored;
+    const double remainder = (restored - (difference - virtual_original)) + (-original - virtual_original);
+    return difference >= 0.0 ? remainder <= 0.0 : remainder >= 0.0;
+}
+
+template <typename T, int D, bool SOA, typename KeyT, typename RelT>
+__global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, const FrameDev* __restrict__ fr,
+                                                    const uint32_t* __restrict__ tile_begin, uint32_t nframes,
+                                                    KeyT* __restrict__ key_out, RelT* __restrict__ rel_out,
+                                                    uint64_t* __restrict__ corr_idx, double* __restrict__ corr_val,
+                                                    uint32_t* __restrict__ tile_ncorr) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t f = upper_find(tile_begin, nframes, tile);
+    const FrameDev& F = fr[f];
+    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
+    const uint64_t end = start + CH_TILE < F.n ? start + CH_TILE : F.n;
+    const double denom = F.denom, eb = F.eb;
+    const uint64_t p = F.p, magic = F.p_magic;
+    const uint32_t l = F.p_l;
+    double mins[D];
+    uint32_t shift[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) { mins[d] = F.mins[d]; shift[d] = F.key_shift[d]; }
+    __shared__ uint32_t s_tmp[4];
+    uint32_t run = 0;
+    const uint64_t cbase = (uint64_t)tile * CH_TILE * D;
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
+        const bool valid = i < end;
+        uint32_t cflags = 0;
+        double xs[D];
+        if (valid) {
+            KeyT key = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const double x = load_coord<T, D, SOA>(pts, F, i, d);
+                xs[d] = x;
+                const double qv = floor((x - mins[d]) / denom);
+                const uint64_t q = (uint64_t)qv;
+                const double restored = fma((double)q + 0.5, denom, mins[d]);
+                if (!within_eb(restored, x, eb)) cflags |= 1u << d;
+                const uint64_t b = div_p(q, p, magic, l);
+                rel_out[(F.pt_off + i) * D + d] = (RelT)(q - b * p);
+                key |= (KeyT)b << shift[d];
+            }
+            key_out[F.pt_off + i] = key;
+        }
+        const uint32_t cnt = __popc(cflags);
+        if (__syncthreads_or(cnt != 0)) {
+            uint32_t tot;
+            uint32_t pos = run + block_excl_scan(cnt, s_tmp, &tot);
+            if (valid) {
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+                    if (cflags & (1u << d)) {
+                        corr_idx[cbase + pos] = i * D + d;
+                        corr_val[cbase + pos] = xs[d];
+                        ++pos;
+                    }
+            }
+            run += tot;
+        }
+    }
+    if (threadIdx.x == 0) tile_ncorr[tile] = run;
+}
+
+// gather per-tile correction lists into one ordered list
+template <int D>
+__global__ __launch_bounds__(CH_WG) void k_corr_gather(const uint64_t* __restrict__ corr_idx,
+                                                       const double* __restrict__ corr_val,
+                                                       const uint32_t* __restrict__ tile_ncorr,
+                                                       const uint32_t* __restrict__ tile_corr_off,
+                                                       uint64_t* __restrict__ out_idx, double* __restrict__ out_val) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t c = tile_ncorr[tile];
+    const uint64_t src = (uint64_t)tile * CH_TILE * D;
+    const uint64_t dst = tile_corr_off[tile];
+    for (uint32_t k = threadIdx.x; k < c; k += CH_WG) {
+        out_idx[dst + k] = corr_idx[src + k];
+        out_val[dst + k] = corr_val[src + k];
+    }
+}
+
+// ------------------------------------------------------------------------
+// K3: stable LSD radix sort of (block key, point index), 8-bit digits.
+// Pass = tile histogram -> digit-major exclusive scan -> scatter. The scatter
+// ranks keys with wavefront ballots (8 ballots give each lane the mask of
+// lanes holding its digit), stages the tile in LDS in digit order and writes
+// contiguous runs per digit.
+
+template <typename KeyT>
+__global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ keys, uint64_t T, uint32_t shift,
+                                                     uint32_t ntiles, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_h[256];
+    s_h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint64_t i = base + (uint64_t)j * CH_WG + threadIdx.x;
+        if (i < T) atomicAdd(&s_h[(uint32_t)(keys[i] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+template <typename KeyT, bool FIRST>
+__global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                        KeyT* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                        uint64_t T, uint32_t shift, uint32_t ntiles,
+                                                        const uint32_t* __restrict__ counts,
+                                                        const uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t s_lstart[256], s_goff[256], s_run[256], s_tmp[4];
+    __shared__ uint32_t s_wcnt[4][256];
+    __shared__ KeyT s_key[CH_TILE];
+    __shared__ uint32_t s_val[CH_TILE];
+    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
+    const uint32_t cnt_tile = (uint32_t)(T - base < CH_TILE ? T - base : CH_TILE);
+    {
+        const uint32_t c = counts[(uint64_t)tid * ntiles + blockIdx.x];
+        s_goff[tid] = offsets[(uint64_t)tid * ntiles + blockIdx.x];
+        uint32_t tot;
+        s_lstart[tid] = block_excl_scan(c, s_tmp, &tot);
+        s_run[tid] = 0;
+        for (int k = 0; k < 4; ++k) s_wcnt[k][tid] = 0;
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    for (int r = 0; r < CH_ITEMS; ++r) {
+        const uint32_t pos = r * CH_WG + tid;
+        const bool valid = pos < cnt_tile;
+        KeyT key = 0;
+        uint32_t val = 0, digit = 0;
+        if (valid) {
+            key = kin[base + pos];
+            val = FIRST ? (uint32_t)(base + pos) : vin[base + pos];
+            digit = (uint32_t)(key >> shift) & 0xffu;
+        }
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bb = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t prank = __popcll(peers & lt);
+        if (valid && prank == 0) s_wcnt[w][digit] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t rank = s_run[digit] + prank;
+            for (uint32_t k = 0; k < w; ++k) rank += s_wcnt[k][digit];
+            const uint32_t lp = s_lstart[digit] + rank;
+            s_key[lp] = key;
+            s_val[lp] = val;
+        }
+        __syncthreads();
+        {
+            uint32_t a = 0;
+            for (int k = 0; k < 4; ++k) { a += s_wcnt[k][tid]; s_wcnt[k][tid] = 0; }
+            s_run[tid] += a;
+        }
+        __syncthreads();
+    }
+    for (uint32_t p = tid; p < cnt_tile; p += CH_WG) {
+        const KeyT k = s_key[p];
+        const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
+        const uint32_t dst = s_goff[d] + (p - s_lstart[d]);
+        kout[dst] = k;
+        vout[dst] = s_val[p];
+    }
+}
+
+// ------------------------------------------------------------------------
+// K4: successor of every point inside its block and the chain decision
+// (cpp/chain/lcps_cpp.cpp:2155-2200), computed in sorted order and scattered
+// to index order.
+
+template <typename KeyT, int D>
+__global__ __launch_bounds__(CH_WG) void k_link(const KeyT* __restrict__ skey, const uint32_t* __restrict__ sval,
+                                                uint64_t T, const FrameDev* __restrict__ fr,
+                                                const uint32_t* __restrict__ frame_pt_begin, uint32_t nframes,
+                                                uint32_t* __restrict__ idrel) {
+    __shared__ uint32_t s_off[CH_MAXF_LDS + 1];
+    const uint32_t* offs = frame_pt_begin;
+    if (nframes <= CH_MAXF_LDS) {
+        for (uint32_t k = threadIdx.x; k <= nframes; k += CH_WG) s_off[k] = frame_pt_begin[k];
+        __syncthreads();
+        offs = s_off;
+    }
+    for (uint64_t s = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; s < T; s += (uint64_t)gridDim.x * CH_WG) {
+        const uint32_t i = sval[s];
+        const KeyT key = skey[s];
+        uint32_t out = 0;
+        if (s + 1 < T && skey[s + 1] == key) {
+            const uint32_t j = sval[s + 1];
+            const uint32_t f = upper_find(offs, nframes, i);
+            const FrameDev& F = fr[f];
+            if ((F.flags & FL_L1) && j < offs[f + 1]) {
+                const uint64_t dist = (uint64_t)(j - i);
+                uint64_t mb = 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const uint32_t wd = F.key_width[d];
+                    const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
+                    const uint64_t b = ((uint64_t)key >> F.key_shift[d]) & mask;
+                    mb = b > mb ? b : mb;
+                }
+                const int64_t local_bits = bitlen64(mb);
+                const int64_t tag_bits = bitlen64((uint64_t)local_bits);
+                const int64_t a = local_bits * D + tag_bits;
+                const int64_t c = (int64_t)D * (int64_t)F.max_bid_bits;
+                const int64_t smaller = a < c ? a : c;
+                const int64_t bl = bitlen64(dist);
+                const int64_t lhs = bl + bitlen64((uint64_t)bl);
+                if (lhs < smaller - F.threshold) out = (uint32_t)dist;
+            }
+        }
+        idrel[i] = out;
+    }
+}
+
+// ------------------------------------------------------------------------
+// K5: ordered compaction of the explicit points' block ids (index order),
+// per-frame point tiles, one decoupled look-back over all tiles.
+
+template <typename KeyT, int D>
+__global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__ idrel, const KeyT* __restrict__ keys,
+                                                    const FrameDev* __restrict__ fr,
+                                                    const uint32_t* __restrict__ tile_begin, uint32_t nframes,
+                                                    uint32_t ntiles, uint64_t* status, uint32_t* ticket,
+                                                    uint32_t* err, uint64_t* __restrict__ blocks,
+                                                    FrameOut* __restrict__ fo, uint64_t* __restrict__ rec_total) {
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t f = upper_find(tile_begin, nframes, tile);
+    const FrameDev& F = fr[f];
+    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
+    uint32_t flags = 0, cnt = 0;
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t i = start + k;
+        if (i < F.n && idrel[F.pt_off + i] == 0) { flags |= 1u << k; ++cnt; }
+    }
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(cnt, s_tmp, &tot);
+    if (threadIdx.x < 64) {
+        const uint64_t e = lookback(status, tile, tot, err);
+        if (threadIdx.x == 0) {
+            s_excl = e;
+            if (tile == F.tile0) fo[f].rec_base = e;
+            if (tile == ntiles - 1) *rec_total = e + tot;
+        }
+    }
+    __syncthreads();
+    uint64_t rec = s_excl + pre;
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        if (flags & (1u << k)) {
+            const uint64_t key = (uint64_t)keys[F.pt_off + start + k];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint32_t wd = F.key_width[d];
+                const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
+                blocks[rec * D + d] = (key >> F.key_shift[d]) & mask;
+            }
+            ++rec;
+        }
+    }
+}
+
+// record counts per frame from the bases (one thread per frame)
+__global__ void k_frame_counts(FrameOut* fo, uint32_t nframes, const uint64_t* rec_total) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    const uint64_t next = f + 1 < nframes ? fo[f + 1].rec_base : *rec_total;
+    fo[f].nrec = next - fo[f].rec_base;
+    fo[f].blk_min = ~0ull;
+    fo[f].blk_max = 0;
+    fo[f].blk_maxdiff = 0;
+}
+
+// ------------------------------------------------------------------------
+// K6: per-frame L2 statistics over the explicit records: minimum, maximum and
+// largest consecutive difference (cpp/chain/lcps_cpp.cpp:1338-1358).
+// Each thread folds a run of records; runs of one frame reduce across the
+// wavefront before one atomic per statistic.
+
+__device__ inline uint32_t frame_of_rec(const uint64_t* s_base, uint32_t nframes, uint64_t r) {
+    uint32_t lo = 0, hi = nframes;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s_base[mid] <= r) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <int D>
+__global__ __launch_bounds__(CH_WG) void k_l2_stats(const uint64_t* __restrict__ blocks,
+                                                    const uint64_t* __restrict__ rec_total, uint32_t nframes,
+                                                    const uint64_t* __restrict__ frame_rec_base,
+                                                    FrameOut* __restrict__ fo) {
+    __shared__ uint64_t s_base[CH_MAXF_LDS];
+    const uint64_t R = *rec_total;
+    const bool lds = nframes <= CH_MAXF_LDS;
+    if (lds) {
+        for (uint32_t k = threadIdx.x; k < nframes; k += CH_WG) s_base[k] = frame_rec_base[k];
+        __syncthreads();
+    }
+    const uint64_t* bases = lds ? s_base : frame_rec_base;
+    const uint64_t gw = (blockIdx.x * (uint64_t)CH_WG + threadIdx.x) >> 6;
+    const uint64_t nw = ((uint64_t)gridDim.x * CH_WG) >> 6;
+    for (uint64_t wb = gw * 64 * CH_ITEMS; wb < R; wb += nw * 64 * CH_ITEMS) {   // wave-uniform trip count
+        const uint64_t r0 = wb + (uint64_t)lane_id() * CH_ITEMS;
+        uint32_t f = r0 < R ? frame_of_rec(bases, nframes, r0) : 0xffffffffu;
+        uint64_t mn = ~0ull, mx = 0, md = 0;
+        uint32_t cur = f;
+        for (int k = 0; k < CH_ITEMS && r0 + k < R; ++k) {
+            const uint64_t r = r0 + k;
+            while (cur + 1 < nframes && bases[cur + 1] <= r) {
+                if (mn != ~0ull) {
+                    atomicMin((unsigned long long*)&fo[cur].blk_min, mn);
+                    atomicMax((unsigned long long*)&fo[cur].blk_max, mx);
+                    atomicMax((unsigned long long*)&fo[cur].blk_maxdiff, md);
+                }
+                mn = ~0ull; mx = 0; md = 0;
+                ++cur;
+            }
+            const bool has_prev = r > bases[cur];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint64_t v = blocks[r * D + d];
+                mn = v < mn ? v : mn;
+                mx = v > mx ? v : mx;
+                if (has_prev) {
+                    const uint64_t u = blocks[(r - 1) * D + d];
+                    const uint64_t ad = u > v ? u - v : v - u;
+                    md = ad > md ? ad : md;
+                }
+            }
+        }
+        // wave-uniform fast path
+        const uint32_t c0 = __shfl(cur, 0, 64);
+        (void)f;
+        if (__ballot(!(cur == c0 || mn == ~0ull)) == 0) {
+            for (int k = 32; k >= 1; k >>= 1) {
+                uint64_t a = __shfl_xor(mn, k, 64), b = __shfl_xor(mx, k, 64), c = __shfl_xor(md, k, 64);
+                mn = a < mn ? a : mn; mx = b > mx ? b : mx; md = c > md ? c : md;
+            }
+            if (lane_id() == 0 && mn != ~0ull && c0 != 0xffffffffu) {
+                atomicMin((unsigned long long*)&fo[c0].blk_min, mn);
+                atomicMax((unsigned long long*)&fo[c0].blk_max, mx);
+                atomicMax((unsigned long long*)&fo[c0].blk_maxdiff, md);
+            }
+        } else if (mn != ~0ull) {
+            atomicMin((unsigned long long*)&fo[cur].blk_min, mn);
+            atomicMax((unsigned long long*)&fo[cur].blk_max, mx);
+            atomicMax((unsigned long long*)&fo[cur].blk_maxdiff, md);
+        }
+    }
+}
+
+// wrap parameters (cpp/chain/lcps_cpp.cpp:1221-1253) and fixed costs
+template <int D>
+__global__ void k_l2_wrap(FrameOut* fo, uint32_t nframes, uint32_t* err) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    FrameOut& o = fo[f];
+    const uint64_t mn = o.blk_min, mx = o.blk_max;
+    if (mn > mx || mx - mn >= (uint64_t)INT64_MAX) { atomicOr(err, 2u); return; }
+    o.wrap_min = (int64_t)mn;
+    o.wrap_max = (int64_t)mx;
+    o.wrap_range = (int64_t)(mx - mn + 1);
+    o.wrap_max_corr = o.wrap_range / 2;
+    o.wrap_min_corr = -o.wrap_max_corr;
+    if ((o.wrap_range & 1) == 0) --o.wrap_max_corr;
+    o.fixed_value_bits = (uint64_t)D * bitlen64(mx);
+    o.fixed_diff_bits = (uint64_t)D * bitlen64(o.blk_maxdiff);
+}
+
+// ------------------------------------------------------------------------
+// K7: L2 choice per record and ordered split into diff / keep streams
+// (cpp/chain/lcps_cpp.cpp:1360-1454; keep-all when L2 is off, :1456-1478).
+
+template <int D>
+__global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__ blocks,
+                                                    const uint64_t* __restrict__ rec_total, uint32_t nframes,
+                                                    const uint64_t* __restrict__ frame_rec_base,
+                                                    const FrameDev* __restrict__ fr, FrameOut* __restrict__ fo,
+                                                    uint64_t* status, uint32_t* ticket, uint32_t* err,
+                                                    uint8_t* __restrict__ choice2, uint64_t* __restrict__ diff,
+                                                    uint64_t* __restrict__ keep, uint64_t* __restrict__ diff_total) {
+    __shared__ uint64_t s_base[CH_MAXF_LDS];
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint64_t s_excl;
+    const uint64_t R = *rec_total;
+    const bool lds = nframes <= CH_MAXF_LDS;
+    if (lds)
+        for (uint32_t k = threadIdx.x; k < nframes; k += CH_WG) s_base[k] = frame_rec_base[k];
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint64_t* bases = lds ? s_base : frame_rec_base;
+    const uint32_t tile = s_tile;
+    // the grid is sized for the largest possible record count; tiles past the
+    // end hold no records and no later tile waits on them
+    if ((uint64_t)tile * CH_TILE >= R) return;
+    const uint64_t r0 = (uint64_t)tile * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
+    uint32_t chosen = 0, cnt = 0;
+    uint32_t f = r0 < R ? frame_of_rec(bases, nframes, r0) : 0;
+    uint32_t fstart = f;
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t r = r0 + k;
+        if (r >= R) break;
+        while (f + 1 < nframes && bases[f + 1] <= r) ++f;
+        if (r == bases[f] || !(fr[f].flags & FL_L2)) continue;
+        const FrameOut& o = fo[f];
+        uint64_t mv = 0, mdf = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const uint64_t v = blocks[r * D + d], u = blocks[(r - 1) * D + d];
+            mv = v > mv ? v : mv;
+            const uint64_t ad = u > v ? u - v : v - u;
+            mdf = ad > mdf ? ad : mdf;
+        }
+        const uint64_t tvw = bitlen64(mv), tdw = bitlen64(mdf);
+        const uint64_t tv = (uint64_t)D * tvw + bitlen64(tvw);
+        const uint64_t td = (uint64_t)D * tdw + bitlen64(tdw);
+        const uint64_t value_bits = tv < o.fixed_value_bits ? tv : o.fixed_value_bits;
+        const uint64_t diff_bits = td > o.fixed_diff_bits ? td : o.fixed_diff_bits;
+        if (value_bits >= diff_bits) { chosen |= 1u << k; ++cnt; }
+    }
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(cnt, s_tmp, &tot);
+    if (threadIdx.x < 64) {
+        const uint64_t e = lookback(status, tile, tot, err);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    __syncthreads();
+    uint64_t dpos = s_excl + pre;   // diffs before record r0
+    f = fstart;
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t r = r0 + k;
+        if (r >= R) break;
+        while (f + 1 < nframes && bases[f + 1] <= r) ++f;
+        if (r == bases[f]) { fo[f].diff_base = dpos; fo[f].keep_base = r - dpos; }
+        const bool c = (chosen >> k) & 1u;
+        choice2[r] = (uint8_t)c;
+        if (!c) {
+            const uint64_t kpos = r - dpos;
+#pragma unroll
+            for (int d = 0; d < D; ++d) keep[kpos * D + d] = blocks[r * D + d];
+        } else {
+            const FrameOut& o = fo[f];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                // prediction = previous record clamped to [wrap_min, wrap_max] (an identity here)
+                int64_t corr = (int64_t)blocks[(r - 1) * D + d] - (int64_t)blocks[r * D + d];
+                if (corr < o.wrap_min_corr) corr += o.wrap_range;
+                else if (corr > o.wrap_max_corr) corr -= o.wrap_range;
+                diff[dpos * D + d] = corr >= 0 ? ((uint64_t)corr << 1) : (((uint64_t)(-(corr + 1)) << 1) | 1ull);
+            }
+            ++dpos;
+        }
+    }
+    if (r0 < R && r0 + CH_ITEMS >= R) *diff_total = dpos;
+}
+
+__global__ void k_copy_rec_base(const FrameOut* fo, uint32_t nframes, uint64_t* rec_base) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nframes) rec_base[f] = fo[f].rec_base;
+}
+
+__global__ void k_frame_corr(FrameOut* fo, uint32_t nframes, const uint32_t* tile_begin,
+                             const uint32_t* tile_corr_off, const uint64_t* corr_total) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    const uint64_t b = tile_corr_off[tile_begin[f]];
+    const uint64_t e = f + 1 < nframes ? tile_corr_off[tile_begin[f + 1]] : *corr_total;
+    fo[f].corr_base = b;
+    fo[f].ncorr = e - b;
+}
+
+__global__ void k_frame_split_counts(FrameOut* fo, uint32_t nframes, const uint64_t* diff_total,
+                                     const uint64_t* rec_total) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    const uint64_t dnext = f + 1 < nframes ? fo[f + 1].diff_base : *diff_total;
+    const uint64_t knext = f + 1 < nframes ? fo[f + 1].keep_base : (*rec_total - *diff_total);
+    fo[f].ndiff = dnext - fo[f].diff_base;
+    fo[f].nkeep = knext - fo[f].keep_base;
+}
+
+}  // namespace chain

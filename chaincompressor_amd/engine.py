@@ -1,0 +1,221 @@
+"""ctypes binding of the C ABI in include/chain_engine.h (libchain_engine.so).
+
+This is the product path: every call runs the HIP kernels of
+chaincompressor_amd/csrc on the GPU. There is no CPU fallback; a missing
+library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchain_engine.so")
+
+CHAIN_F32, CHAIN_F64 = 0, 1
+CHAIN_AOS, CHAIN_SOA = 0, 1
+FMT_LCH1, FMT_LCP3 = 0, 1
+NO_BL, NO_L1, NO_L2 = 1, 2, 4
+
+_FORMATS = {"lch1": FMT_LCH1, "lcp3": FMT_LCP3}
+
+
+class ChainError(RuntimeError):
+    pass
+
+
+class ChainValueError(ValueError):
+    pass
+
+
+class _Buf(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_uint8)), ("size", C.c_size_t)]
+
+
+class _Frame(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("n", C.c_uint64), ("eb", C.c_double), ("p", C.c_int64),
+                ("threshold", C.c_int64), ("flags", C.c_uint32)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("bytes", C.c_uint64), ("block_replacement_size", C.c_uint64), ("idrel_size", C.c_uint64),
+                ("rel_size", C.c_uint64), ("num_chained", C.c_uint64), ("num_corrections", C.c_uint64)]
+
+    def as_list(self):
+        """The tail of lcps_compress2's return list (cpp/chain/lcps_cpp.cpp:2331-2337)."""
+        return [int(self.block_replacement_size), int(self.idrel_size), int(self.rel_size), int(self.num_chained)]
+
+
+EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "chain_engine_stream",
+           "chain_buf_free", "chain_compress", "chain_compress_batch", "chain_decompress",
+           "chain_decompress_batch", "chain_last_stage_times"]
+
+_LIB = None
+
+
+def load_library():
+    """Load libchain_engine.so; raises if it was not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ChainError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.chain_engine_create.argtypes = [C.c_int, C.c_uint64, P(C.c_void_p)]
+    L.chain_engine_destroy.argtypes = [C.c_void_p]
+    L.chain_last_error.argtypes = [C.c_void_p]
+    L.chain_last_error.restype = C.c_char_p
+    L.chain_engine_stream.argtypes = [C.c_void_p]
+    L.chain_engine_stream.restype = C.c_void_p
+    L.chain_buf_free.argtypes = [P(_Buf)]
+    L.chain_compress.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint64, C.c_double,
+                                 C.c_int64, C.c_int64, C.c_uint32, C.c_int, P(_Buf), P(_Stats)]
+    L.chain_compress_batch.argtypes = [C.c_void_p, P(_Frame), C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                       C.c_int, P(_Buf), P(_Stats)]
+    L.chain_decompress.argtypes = [C.c_void_p, P(C.c_uint8), C.c_size_t, C.c_int, C.c_int, P(_Buf),
+                                   P(C.c_uint64), P(C.c_uint32)]
+    L.chain_decompress_batch.argtypes = [C.c_void_p, P(P(C.c_uint8)), P(C.c_size_t), C.c_uint32, C.c_int, C.c_int,
+                                         P(_Buf), P(C.c_uint64), P(C.c_uint32)]
+    L.chain_last_stage_times.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_char_p, C.c_size_t]
+    _LIB = L
+    return L
+
+
+def _take(buf: _Buf) -> bytes:
+    try:
+        return C.string_at(buf.data, buf.size) if buf.size else b""
+    finally:
+        load_library().chain_buf_free(C.byref(buf))
+
+
+def _dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    if dt == np.float32:
+        return CHAIN_F32
+    if dt == np.float64:
+        return CHAIN_F64
+    raise ChainValueError("points must be float32 or float64")
+
+
+class Engine:
+    """One GPU, one HIP stream, one reusable device arena."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        L = load_library()
+        h = C.c_void_p()
+        rc = L.chain_engine_create(device, hbm_budget_bytes, C.byref(h))
+        if rc != 0:
+            raise ChainError(L.chain_last_error(None).decode())
+        self._h = h
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.chain_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc == 0:
+            return
+        msg = self._L.chain_last_error(self._h).decode()
+        if rc == 1:
+            raise ChainValueError(msg)
+        raise ChainError(msg)
+
+    @property
+    def stream(self) -> int:
+        return int(self._L.chain_engine_stream(self._h) or 0)
+
+    def compress(self, points, eb, p, threshold, flags=0, fmt="lch1", layout="aos"):
+        """Compress one frame (host array, (N, D) for AoS or (D, N) for SoA)."""
+        pts = np.asarray(points)
+        dt = _dtype_code(pts.dtype)
+        pts = np.ascontiguousarray(pts)
+        if pts.ndim != 2:
+            raise ChainValueError("points must be a 2-D array")
+        if layout == "aos":
+            n, d = pts.shape
+        else:
+            d, n = pts.shape
+        out = _Buf()
+        st = _Stats()
+        rc = self._L.chain_compress(self._h, pts.ctypes.data_as(C.c_void_p), dt,
+                                    CHAIN_AOS if layout == "aos" else CHAIN_SOA, d, n, float(eb), int(p),
+                                    int(threshold), int(flags), _FORMATS[fmt], C.byref(out), C.byref(st))
+        self._check(rc)
+        return _take(out), st
+
+    def compress_batch(self, frames, eb, p, threshold, flags=0, fmt="lch1", layout="aos", device_ptrs=None):
+        """Compress a list of host arrays (or device pointers with device_ptrs=(ptrs, ns, dtype, dims))."""
+        if device_ptrs is None:
+            arrs = [np.ascontiguousarray(np.asarray(f)) for f in frames]
+            if not arrs:
+                raise ChainValueError("empty batch")
+            dt = _dtype_code(arrs[0].dtype)
+            dims = arrs[0].shape[1] if layout == "aos" else arrs[0].shape[0]
+            ns = [a.shape[0] if layout == "aos" else a.shape[1] for a in arrs]
+            ptrs = [a.ctypes.data for a in arrs]
+            on_dev = 0
+        else:
+            ptrs, ns, dtype, dims = device_ptrs
+            dt = _dtype_code(dtype)
+            arrs = None
+            on_dev = 1
+        F = len(ptrs)
+        ebs = eb if isinstance(eb, (list, tuple, np.ndarray)) else [eb] * F
+        ps = p if isinstance(p, (list, tuple, np.ndarray)) else [p] * F
+        ts = threshold if isinstance(threshold, (list, tuple, np.ndarray)) else [threshold] * F
+        fl = flags if isinstance(flags, (list, tuple, np.ndarray)) else [flags] * F
+        fr = (_Frame * F)()
+        for k in range(F):
+            fr[k] = _Frame(ptrs[k], int(ns[k]), float(ebs[k]), int(ps[k]), int(ts[k]), int(fl[k]))
+        outs = (_Buf * F)()
+        sts = (_Stats * F)()
+        rc = self._L.chain_compress_batch(self._h, fr, F, dt, CHAIN_AOS if layout == "aos" else CHAIN_SOA, dims,
+                                          on_dev, _FORMATS[fmt], outs, sts)
+        self._check(rc)
+        del arrs
+        return [_take(outs[k]) for k in range(F)], [sts[k] for k in range(F)]
+
+    def decompress(self, data: bytes, dtype=np.float64, layout="aos"):
+        buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data if data else b"\0")
+        out = _Buf()
+        n = C.c_uint64()
+        d = C.c_uint32()
+        rc = self._L.chain_decompress(self._h, buf, len(data), _dtype_code(dtype),
+                                      CHAIN_AOS if layout == "aos" else CHAIN_SOA, C.byref(out), C.byref(n),
+                                      C.byref(d))
+        self._check(rc)
+        raw = _take(out)
+        arr = np.frombuffer(raw, dtype=dtype).copy()
+        return arr.reshape((n.value, d.value) if layout == "aos" else (d.value, n.value))
+
+    def decompress_batch(self, streams, dtype=np.float64, layout="aos"):
+        k = len(streams)
+        bufs = [(C.c_uint8 * max(1, len(s))).from_buffer_copy(s if s else b"\0") for s in streams]
+        ptrs = (C.POINTER(C.c_uint8) * k)(*[C.cast(b, C.POINTER(C.c_uint8)) for b in bufs])
+        lens = (C.c_size_t * k)(*[len(s) for s in streams])
+        outs = (_Buf * k)()
+        ns = (C.c_uint64 * k)()
+        ds = (C.c_uint32 * k)()
+        rc = self._L.chain_decompress_batch(self._h, ptrs, lens, k, _dtype_code(dtype),
+                                            CHAIN_AOS if layout == "aos" else CHAIN_SOA, outs, ns, ds)
+        self._check(rc)
+        res = []
+        for j in range(k):
+            arr = np.frombuffer(_take(outs[j]), dtype=dtype).copy()
+            res.append(arr.reshape((ns[j], ds[j]) if layout == "aos" else (ds[j], ns[j])))
+        return res
+
+    def stage_times(self):
+        ms = (C.c_float * 64)()
+        names = C.create_string_buffer(4096)
+        k = self._L.chain_last_stage_times(self._h, ms, 64, names, 4096)
+        labels = names.value.decode().split(";") if k else []
+        return {labels[i]: float(ms[i]) for i in range(k)}

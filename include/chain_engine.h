@@ -1,0 +1,110 @@
+/*
+ * chain_engine.h -- C ABI of the MI355X (gfx950) Chain compressor engine.
+ *
+ * This is the drop-in boundary below the Python modules that replace the
+ * reference's pybind11 extensions:
+ *   lcps_gpu_cpp.lcps_compress2_gpu            cu/chain/lcps_gpu_cpp.cu:1095-1102
+ *   lcps_gpu_decompress_ans.lcps_decompress_gpu cu/decompress/lcps_gpu_decompress_ans.cu:2898-2921
+ * and, for the reference CPU format, lcps_cpp.lcps_compress2
+ * (cpp/chain/lcps_cpp.cpp:2340-2355, ec=False -> LCP3).
+ *
+ * Conventions: plain pointers and sizes, integer status codes (0 = success),
+ * no exceptions across the ABI, library-allocated outputs released with
+ * chain_buf_free(). One engine owns one HIP stream (default priority) on one
+ * device and a reusable device arena; calls on one engine are serialised by
+ * an internal mutex.
+ */
+#ifndef CHAIN_ENGINE_H
+#define CHAIN_ENGINE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chain_engine chain_engine;
+
+enum chain_status {
+    CHAIN_OK = 0,
+    CHAIN_EINVAL = 1,      /* invalid argument (the reference raises ValueError)      */
+    CHAIN_ERUNTIME = 2,    /* data or format error (the reference raises RuntimeError) */
+    CHAIN_EDEVICE = 3,     /* HIP runtime error or device error word                  */
+    CHAIN_ENOMEM = 4,      /* device memory budget exceeded                           */
+};
+
+enum chain_dtype { CHAIN_F32 = 0, CHAIN_F64 = 1 };
+enum chain_layout { CHAIN_AOS = 0, /* (N, D) */ CHAIN_SOA = 1 /* (D, N) */ };
+enum chain_format {
+    CHAIN_FMT_LCH1 = 0,    /* this engine's entropy-coded container (chunked interleaved rANS) */
+    CHAIN_FMT_LCP3 = 1,    /* the reference's raw container, ec=False (lcps_cpp.cpp:2306-2329) */
+};
+/* frame flags: the reference's ablation switches, lcps_cpp.cpp:2023-2026 */
+enum chain_flags { CHAIN_NO_BL = 1u, CHAIN_NO_L1 = 2u, CHAIN_NO_L2 = 4u };
+
+typedef struct {
+    uint8_t* data;   /* malloc'd by the library */
+    size_t size;
+} chain_buf;
+
+/* one frame of a batch */
+typedef struct {
+    const void* points;    /* host pointer (or device pointer, see chain_compress_batch) */
+    uint64_t n;            /* particles */
+    double eb;             /* absolute error bound, > 0 */
+    int64_t p;             /* block size in quantisation steps, > 0 */
+    int64_t threshold;     /* chain threshold in bits */
+    uint32_t flags;        /* chain_flags */
+} chain_frame;
+
+/* per-frame statistics, as lcps_compress2's return list (lcps_cpp.cpp:2331-2337) */
+typedef struct {
+    uint64_t bytes;                  /* container size */
+    uint64_t block_replacement_size; /* diff + keep + choice2 component bytes */
+    uint64_t idrel_size;
+    uint64_t rel_size;
+    uint64_t num_chained;            /* N - B */
+    uint64_t num_corrections;
+} chain_stats;
+
+int chain_engine_create(int device, uint64_t hbm_budget_bytes, chain_engine** out);
+void chain_engine_destroy(chain_engine* e);
+/* message of the last failure on this engine (or of chain_engine_create when e is NULL) */
+const char* chain_last_error(const chain_engine* e);
+/* the engine's hipStream_t, for callers that time or order work around it */
+void* chain_engine_stream(chain_engine* e);
+void chain_buf_free(chain_buf* b);
+
+/* Compress one frame from host memory. Replaces lcps_compress2_gpu
+ * (cu/chain/lcps_gpu_cpp.cu:718-1091) and, with CHAIN_FMT_LCP3,
+ * lcps_compress2(..., ec=False) (cpp/chain/lcps_cpp.cpp:2017-2338). */
+int chain_compress(chain_engine* e, const void* points, int dtype, int layout, uint32_t dims, uint64_t n,
+                   double eb, int64_t p, int64_t threshold, uint32_t flags, int format,
+                   chain_buf* out, chain_stats* stats);
+
+/* Compress a batch of frames (same dtype, layout and dims) through one launch
+ * per stage. points_on_device != 0: every frame's points pointer is a device
+ * pointer. outs/stats have nframes entries. */
+int chain_compress_batch(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
+                         uint32_t dims, int points_on_device, int format, chain_buf* outs, chain_stats* stats);
+
+/* Decompress one LCH1 or LCP3 stream to float64 (or float32) coordinates.
+ * Replaces lcps_decompress_gpu (cu/decompress/lcps_gpu_decompress_ans.cu:2701-2921)
+ * and, for the reference's formats, lcps_decompress_serial
+ * (cpp/decompress/lcps_cpu_decompress_serial.cpp:1377-1381). */
+int chain_decompress(chain_engine* e, const uint8_t* bytes, size_t len, int out_dtype, int out_layout,
+                     chain_buf* out, uint64_t* n, uint32_t* dims);
+
+/* Batch decompression: one launch per stage over all streams. */
+int chain_decompress_batch(chain_engine* e, const uint8_t* const* bytes, const size_t* lens, uint32_t nstreams,
+                           int out_dtype, int out_layout, chain_buf* outs, uint64_t* ns, uint32_t* dims);
+
+/* Per-stage device times (milliseconds, from HIP events on the engine's
+ * stream) of the last call. names is a ';'-separated list. Returns the count. */
+int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, size_t names_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

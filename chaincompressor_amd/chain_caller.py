@@ -1,0 +1,47 @@
+"""Drop-in for the reference's chain_caller.py (chain_caller.py:23-52).
+
+The Chain functions keep the reference's names and argument order and run on
+the MI355X engine:
+
+* chain_gpu(coords (D, N), eb, p, threshold, diff) -> bytes (LCH1)
+  replaces lcps_gpu_cpp.lcps_compress2_gpu (cu/chain/lcps_gpu_cpp.cu:1095)
+* gpu_decompress(bytes) -> float64 (N, D)
+  replaces lcps_gpu_decompress_ans.lcps_decompress_gpu
+  (cu/decompress/lcps_gpu_decompress_ans.cu:2898)
+* chain(coords (N, D), eb, p, threshold, diff, bl, l1, l2, ec) -> [bytes, stats...]
+  mirrors lcps_cpp.lcps_compress2 (cpp/chain/lcps_cpp.cpp:2340): ec=False
+  returns the reference's LCP3 bytes exactly, ec=True this engine's LCH1
+* chain_decompress(bytes) -> float64 (N, D) for LCH1 and LCP3 streams
+
+The LCP-S and Draco wrappers of the reference (lcps_compress,
+lcps_decompress, dracoIds) are competitor baselines outside this engine.
+"""
+from chaincompressor_amd import lcps_gpu_cpp, lcps_gpu_decompress_ans
+
+
+def chain(coords, eb, p, threshold, diff, bl=True, l1=True, l2=True, ec=True):
+    return lcps_gpu_cpp.lcps_compress2(coords, eb, p, threshold, diff, bl, l1, l2, ec)
+
+
+def chain_decompress(compressed_data, timer=False):
+    return lcps_gpu_decompress_ans.lcps_decompress_gpu(compressed_data)
+
+
+def gpu_decompress(compressed_data, timer=False):
+    return lcps_gpu_decompress_ans.lcps_decompress_gpu(compressed_data)
+
+
+def chain_gpu(coords, eb, p, threshold, diff, timer=False):
+    return lcps_gpu_cpp.lcps_compress2_gpu(coords, eb, p, threshold, diff)
+
+
+def lcps_compress(coords, eb, reorder=False):
+    raise NotImplementedError("LCP-S is a competitor baseline of the reference, not part of the Chain engine")
+
+
+def lcps_decompress(compressed):
+    raise NotImplementedError("LCP-S is a competitor baseline of the reference, not part of the Chain engine")
+
+
+def dracoIds(data, qb):
+    raise NotImplementedError("Draco is a competitor baseline of the reference, not part of the Chain engine")

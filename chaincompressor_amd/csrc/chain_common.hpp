@@ -36,8 +36,10 @@ struct FrameDev {
     uint32_t p_l;           // l
     uint32_t max_bid_bits;  // bit length of the largest block id of the frame
     int64_t threshold;
-    uint32_t key_shift[CH_MAXD];
+    uint32_t key_shift[CH_MAXD];  // bit position of the dimension inside its key word
     uint32_t key_width[CH_MAXD];
+    uint32_t key_word[CH_MAXD];   // key word of the dimension (keys wider than 64 bits)
+    uint32_t pad1;
 };
 
 struct FrameOut {

@@ -9,6 +9,8 @@
 #include "chain_common.hpp"
 #include "../../include/chain_engine.h"
 #include "chain_kernels_pre.hip"
+#include "chain_kernels_ent.hip"
+#include "chain_kernels_dec.hip"
 #include "chain_format.hpp"
 
 #include <algorithm>
@@ -106,16 +108,24 @@ struct StageTimer {
     }
 };
 
+// RVC1 parameters of the containers an engine writes (recorded in every section)
+struct RansConfig {
+    uint32_t lanes_max = 64;
+    uint32_t chunk_symbols = 65536;
+};
+
 }  // namespace chain
 
 struct chain_engine {
     int device = 0;
     uint64_t budget = 0;
     hipStream_t stream = nullptr;
-    chain::Arena arena;
+    chain::Arena arena;     // compression
+    chain::Arena arena_d;   // decompression (so either direction's device output survives the other)
     std::mutex mu;
     std::string err;
     chain::StageTimer timer;
+    chain::RansConfig rans;
 };
 
 namespace chain {
@@ -175,10 +185,15 @@ struct FramePlan {
 };
 
 struct CompressResult {
-    // host copies of the streams of every frame (LCP3 path) or containers
+    // host copies of the containers, or (keep_on_device) offsets into the
+    // engine-owned device buffer d_out
+    bool keep_on_device = false;
     std::vector<std::vector<uint8_t>> containers;
     std::vector<chain_stats> stats;
+    uint8_t* d_out = nullptr;
+    std::vector<uint64_t> offsets, sizes;
 };
+
 
 template <class T, int D, bool SOA>
 static void launch_bbox(hipStream_t st, const void* pts, const FrameDev* dfr, const uint32_t* dtb, uint32_t F,
@@ -200,6 +215,84 @@ static void scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t
     fill64(st, status, tiles, 0);
     fill32(st, ticket, 1, 0);
     hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)tiles), dim3(CH_WG), 0, st, in, out, n, status, ticket, err, total);
+}
+
+template <class X>
+static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D, bool rel8, const FrameDev* fdev,
+                          FrameOut* fo, const uint32_t* idrel, const uint64_t* diff, const uint64_t* keep,
+                          const uint8_t* choice2, const void* rel, const uint64_t* corr_idx, const double* corr_val,
+                          const uint32_t* tb, uint32_t ntiles, const std::vector<uint32_t>& tile1k,
+                          uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const RansConfig& rc, uint64_t stride,
+                          uint32_t* err, uint32_t* ticket, X& x) {
+    tm.begin("comp_stats");
+    const uint32_t ninst = 5 * F, nsec = 10 * F;
+    hipLaunchKernelGGL(k_comp_init, dim3(cdiv(ninst, CH_WG)), dim3(CH_WG), 0, st, fdev, fo, F, D, x.ci);
+    fill32(st, x.vhist, 5ull * F * RANS_VALUE_LIMIT, 0);
+    fill32(st, x.thist, 5ull * F * RANS_TAG_LIMIT, 0);
+    HIPCHK(hipMemcpyAsync(x.tb1k, tile1k.data(), (F + 1) * 4, hipMemcpyHostToDevice, st));
+    // every component: a typed view and its launches
+    auto each_comp = [&](auto&& fn) {
+        with_dims(D, [&](auto dd) {
+            constexpr int DD = decltype(dd)::value;
+            fn(0, TypeTag<uint32_t>{}, IC<1>{}, (const void*)idrel);
+            fn(1, TypeTag<uint64_t>{}, IC<DD>{}, (const void*)diff);
+            fn(2, TypeTag<uint64_t>{}, IC<DD>{}, (const void*)keep);
+            fn(3, TypeTag<uint8_t>{}, IC<1>{}, (const void*)choice2);
+            if (rel8) fn(4, TypeTag<uint8_t>{}, IC<DD>{}, rel);
+            else fn(4, TypeTag<uint32_t>{}, IC<DD>{}, rel);
+        });
+    };
+    each_comp([&](int c, auto vt, auto dd, const void* v) {
+        using V = typename decltype(vt)::type;
+        constexpr int DC = decltype(dd)::value;
+        hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
+                           x.vhist, x.thist);
+    });
+    tm.end();
+    tm.begin("rans_table");
+    hipLaunchKernelGGL(k_rans_table, dim3(nsec), dim3(CH_WG), 0, st, x.ci, x.sec, x.vhist, x.thist, F, x.prob, x.cum,
+                       x.tbytes, err);
+    hipLaunchKernelGGL(k_rans_plan, dim3(1), dim3(1), 0, st, x.sec, nsec, F, rc.lanes_max, rc.chunk_symbols,
+                       x.comp_chunk);
+    tm.end();
+    tm.begin("rans_encode");
+    each_comp([&](int c, auto vt, auto dd, const void* v) {
+        using V = typename decltype(vt)::type;
+        constexpr int DC = decltype(dd)::value;
+        if (comp_chunk_bound[c])
+            hipLaunchKernelGGL((k_rans_encode<V, DC>), dim3((unsigned)comp_chunk_bound[c]), dim3(64), 0, st,
+                               (const V*)v, c, F, x.ci, x.sec, x.comp_chunk, x.prob, x.cum, x.scratch, stride,
+                               x.chunk_len, x.chunk_pos, err);
+    });
+    tm.end();
+    tm.begin("assemble");
+    hipLaunchKernelGGL(k_choose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst, x.chunk_len,
+                       x.chunk_off);
+    hipLaunchKernelGGL(k_layout, dim3(1), dim3(1), 0, st, x.ci, x.sec, fo, F, D, x.out_off, x.out_size, x.total);
+    hipLaunchKernelGGL(k_write_frames, dim3(F), dim3(CH_WG), 0, st, fdev, fo, D, x.out_off, x.out_size, corr_idx,
+                       corr_val, x.out);
+    hipLaunchKernelGGL(k_write_components, dim3(ninst), dim3(CH_WG), 0, st, x.ci, x.sec, F, x.tbytes, x.chunk_len,
+                       x.out);
+    const uint64_t chunk_bound = comp_chunk_bound[0] + comp_chunk_bound[1] + comp_chunk_bound[2] +
+                                 comp_chunk_bound[3] + comp_chunk_bound[4];
+    if (chunk_bound)
+        hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunk_bound), dim3(CH_WG), 0, st, x.ci, x.sec, nsec,
+                           x.comp_chunk, x.scratch, stride, x.chunk_len, x.chunk_pos, x.chunk_off, x.out);
+    tm.end();
+    tm.begin("pack");
+    each_comp([&](int c, auto vt, auto dd, const void* v) {
+        using V = typename decltype(vt)::type;
+        constexpr int DC = decltype(dd)::value;
+        hipLaunchKernelGGL((k_pack_fixed<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
+                           x.out);
+        fill64(st, x.status1k, ntiles1k, 0);
+        fill32(st, ticket, 1, 0);
+        hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(ntiles1k), dim3(CH_WG), 0, st, (const V*)v, c, F, x.tb1k,
+                           x.ci, x.status1k, ticket, err, x.head, x.tail, x.head_pos, x.out);
+        hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG)), dim3(CH_WG), 0, st, x.head, x.tail, x.head_pos,
+                           ntiles1k, x.out);
+    });
+    tm.end();
 }
 
 static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
@@ -270,7 +363,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     // phase B sizes depend on the plan; compute an upper bound now so the
     // arena is allocated once (keys as u64, rel as u32 worst case)
     const uint64_t sort_tiles = cdiv(T, CH_TILE);
-    auto carve_b = [&](Carver& c, size_t key_bytes, size_t rel_bytes) {
+    auto carve_b = [&](Carver& c, size_t key_bytes, size_t rel_bytes, uint32_t key_words) {
         struct {
             void* keys_in;
             void* ka;
@@ -297,7 +390,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             uint64_t* rec_base;
             uint64_t* totals;   // [0] rec total, [1] diff total, [2] corr total
         } b;
-        b.keys_in = c.take<char>(T * key_bytes);
+        b.keys_in = c.take<char>(T * key_bytes * key_words);
         b.ka = c.take<char>(T * key_bytes);
         b.kb = c.take<char>(T * key_bytes);
         b.va = c.take<uint32_t>(T);
@@ -324,9 +417,82 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         return b;
     };
     Carver cb0;
-    (void)carve_b(cb0, 8, 4);
+    (void)carve_b(cb0, 8, 4, D);
     const size_t size_b = cb0.off;
-    e->arena.ensure(size_a + size_b, e->budget);
+
+    // LCH1 entropy stage buffers, sized from the particle counts alone
+    const RansConfig rc = e->rans;
+    const uint64_t stride = CHUNK_STATE_BYTES + 3ull * rc.chunk_symbols;
+    uint64_t comp_chunk_bound[5] = {0, 0, 0, 0, 0};
+    uint64_t out_bound = 0;
+    std::vector<uint32_t> tile1k(F + 1, 0);
+    for (uint32_t f = 0; f < F; ++f) {
+        const uint64_t n = plan[f].n;
+        for (int c = 0; c < 5; ++c) {
+            const uint32_t dc = (c == 0 || c == 3) ? 1u : D;
+            comp_chunk_bound[c] += cdiv(n * dc, rc.chunk_symbols) + cdiv(n, rc.chunk_symbols);
+            out_bound += 8 + 16 + 17ull * dc + 8ull * n * dc;
+        }
+        out_bound += 48 + 8ull * D + 16 + 16ull * n * D;
+        tile1k[f + 1] = tile1k[f] + (uint32_t)cdiv(n, TAG_TILE);
+    }
+    const uint64_t chunk_bound = comp_chunk_bound[0] + comp_chunk_bound[1] + comp_chunk_bound[2] +
+                                 comp_chunk_bound[3] + comp_chunk_bound[4];
+    const uint32_t ntiles1k = tile1k[F];
+    auto carve_c = [&](Carver& c) {
+        struct {
+            CompInst* ci;
+            Section* sec;
+            uint32_t* vhist;
+            uint32_t* thist;
+            uint32_t* prob;
+            uint32_t* cum;
+            uint8_t* tbytes;
+            uint32_t* comp_chunk;
+            uint8_t* scratch;
+            uint32_t* chunk_len;
+            uint32_t* chunk_pos;
+            uint32_t* chunk_off;
+            uint64_t* out_off;
+            uint64_t* out_size;
+            uint64_t* total;
+            uint32_t* tb1k;
+            uint8_t* head;
+            uint8_t* tail;
+            uint64_t* head_pos;
+            uint64_t* status1k;
+            uint8_t* out;
+        } x;
+        x.ci = c.take<CompInst>(5 * F);
+        x.sec = c.take<Section>(10 * F);
+        x.vhist = c.take<uint32_t>(5ull * F * RANS_VALUE_LIMIT);
+        x.thist = c.take<uint32_t>(5ull * F * RANS_TAG_LIMIT);
+        x.prob = c.take<uint32_t>(10ull * F * RANS_VALUE_LIMIT);
+        x.cum = c.take<uint32_t>(10ull * F * RANS_VALUE_LIMIT);
+        x.tbytes = c.take<uint8_t>(10ull * F * TABLE_BYTES_MAX);
+        x.comp_chunk = c.take<uint32_t>(8);
+        x.scratch = c.take<uint8_t>(chunk_bound * stride);
+        x.chunk_len = c.take<uint32_t>(chunk_bound);
+        x.chunk_pos = c.take<uint32_t>(chunk_bound);
+        x.chunk_off = c.take<uint32_t>(chunk_bound);
+        x.out_off = c.take<uint64_t>(F);
+        x.out_size = c.take<uint64_t>(F);
+        x.total = c.take<uint64_t>(1);
+        x.tb1k = c.take<uint32_t>(F + 1);
+        x.head = c.take<uint8_t>(ntiles1k);
+        x.tail = c.take<uint8_t>(ntiles1k);
+        x.head_pos = c.take<uint64_t>(ntiles1k);
+        x.status1k = c.take<uint64_t>(ntiles1k + 1);
+        x.out = c.take<uint8_t>(out_bound);
+        return x;
+    };
+    size_t size_c = 0;
+    if (format == CHAIN_FMT_LCH1) {
+        Carver cc0;
+        (void)carve_c(cc0);
+        size_c = cc0.off;
+    }
+    e->arena.ensure(size_a + size_b + size_c, e->budget);
 
     Carver cA;
     cA.base = e->arena.base;
@@ -370,7 +536,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     HIPCHK(hipStreamSynchronize(st));
 
     // ---- plan: minima, quantised range, key layout (lcps_cpp.cpp:2072-2110)
-    uint32_t key_bits_max = 0;
+    uint32_t key_bits_max = 0, nwords = 1;
+    uint32_t word_bits[CH_MAXD] = {0, 0, 0, 0};
     uint64_t relmax = 0;
     for (uint32_t f = 0; f < F; ++f) {
         require(bad[f] == 0, CHAIN_ERUNTIME, "LCPS coordinates must be finite");
@@ -379,7 +546,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         const double denom = 2.0 * P.eb;
         uint64_t bmax[CH_MAXD];
         uint64_t max_bid = 0;
-        uint32_t shift = 0;
+        uint32_t shift = 0, word = 0, total_bits = 0;
         for (uint32_t d = 0; d < D; ++d) {
             const uint64_t* b = &box[((size_t)f * D + d) * 4];
             double mn = bits_double(okey_inv(b[0]));
@@ -392,15 +559,23 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             bmax[d] = P.qmax[d] / P.pe;
             max_bid = std::max(max_bid, bmax[d]);
             const uint32_t w = (uint32_t)bitlen64(bmax[d]);
+            if (shift + w > 64) {   // key wider than 64 bits: start the next word
+                word_bits[word] = std::max(word_bits[word], shift);
+                ++word;
+                shift = 0;
+            }
             Fd.key_width[d] = w;
             Fd.key_shift[d] = w ? shift : 0;
+            Fd.key_word[d] = word;
             shift += w;
+            total_bits += w;
             Fd.mins[d] = mn;
             P.relmax = std::max(P.relmax, std::min<uint64_t>(P.pe - 1, P.qmax[d]));
         }
-        require(shift <= 64, CHAIN_ERUNTIME, "block key exceeds 64 bits on the GPU path");
-        P.key_bits = shift;
-        key_bits_max = std::max(key_bits_max, shift);
+        word_bits[word] = std::max(word_bits[word], shift);
+        nwords = std::max(nwords, word + 1);
+        P.key_bits = total_bits;
+        key_bits_max = std::max(key_bits_max, total_bits);
         relmax = std::max(relmax, P.relmax);
         Fd.flags = ((P.flags & CHAIN_NO_L1) ? 0u : FL_L1) | ((P.flags & CHAIN_NO_L2) ? 0u : FL_L2);
         Fd.eb = P.eb;
@@ -414,14 +589,16 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         Fd.threshold = P.threshold;
     }
     require(relmax <= 0xffffffffull, CHAIN_ERUNTIME, "block offsets of 2^32 or more are not supported on the GPU path");
-    const bool key64 = key_bits_max > 32;
+    const bool key64 = key_bits_max > 32 || nwords > 1;
     const bool rel8 = relmax <= 0xff;
     const size_t kbytes = key64 ? 8 : 4, rbytes = rel8 ? 1 : 4;
-    const uint32_t passes = std::max<uint32_t>(1, (key_bits_max + 7) / 8);
+    uint32_t word_passes[CH_MAXD];
+    for (uint32_t w = 0; w < CH_MAXD; ++w) word_passes[w] = (word_bits[w] + 7) / 8;
+    if (nwords == 1) word_passes[0] = std::max<uint32_t>(1, (key_bits_max + 7) / 8);
 
     Carver cB;
     cB.base = e->arena.base + size_a;
-    auto B = carve_b(cB, kbytes, rbytes);
+    auto B = carve_b(cB, kbytes, rbytes, nwords);
     HIPCHK(hipMemcpyAsync(A.fdev, fdev.data(), F * sizeof(FrameDev), hipMemcpyHostToDevice, st));
 
     auto run_typed = [&](auto key_tag, auto rel_tag) {
@@ -436,43 +613,63 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                 if (layout == CHAIN_SOA)
                     hipLaunchKernelGGL((k_quantize<T_, DD, true, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
                                        (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                       B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr);
+                                       B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
                 else
                     hipLaunchKernelGGL((k_quantize<T_, DD, false, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
                                        (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                       B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr);
+                                       B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
             });
         });
         tm.end();
         // K3 sort
-        tm.begin("sort");
         const KeyT* kin = (const KeyT*)B.keys_in;
         const uint32_t* vin = nullptr;
         KeyT* kout = (KeyT*)B.ka;
         uint32_t* vout = B.va;
-        for (uint32_t ps = 0; ps < passes; ++ps) {
+        uint32_t done_passes = 0;
+        for (uint32_t w = 0; w < nwords; ++w)
+        for (uint32_t ps = 0; ps < word_passes[w]; ++ps) {
+            if (w > 0 && ps == 0) {
+                // values of word w in the order left by the words before it
+                KeyT* g = (KeyT*)((kin == (const KeyT*)B.ka) ? B.kb : B.ka);
+                if (kin == (const KeyT*)B.keys_in) g = (KeyT*)B.ka;
+                hipLaunchKernelGGL(k_gather_word, dim3(grid_for(T)), dim3(CH_WG), 0, st,
+                                   (const uint64_t*)B.keys_in + (uint64_t)w * T, vin, T, (uint64_t*)g);
+                kin = g;
+                kout = (g == (KeyT*)B.ka) ? (KeyT*)B.kb : (KeyT*)B.ka;
+            }
             const uint32_t shift = 8 * ps;
+            const uint32_t ps_all = done_passes++;
+            tm.begin("sort_hist");
             hipLaunchKernelGGL((k_sort_hist<KeyT>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin, T, shift,
                                (uint32_t)sort_tiles, B.counts);
+            tm.end();
+            tm.begin("sort_scan");
             scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status, B.ticket, A.err, nullptr);
-            if (ps == 0)
+            tm.end();
+            tm.begin("sort_scatter");
+            if (ps_all == 0)
                 hipLaunchKernelGGL((k_sort_scatter<KeyT, true>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
                                    vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
             else
                 hipLaunchKernelGGL((k_sort_scatter<KeyT, false>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st,
                                    kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
+            tm.end();
             kin = kout;
             vin = vout;
             kout = (kout == (KeyT*)B.ka) ? (KeyT*)B.kb : (KeyT*)B.ka;
             vout = (vout == B.va) ? B.vb : B.va;
         }
-        tm.end();
         // K4 link + decision
         tm.begin("link");
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
-            hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st, kin, vin, T, A.fdev,
-                               A.ptb, F, B.idrel);
+            if (nwords == 1)
+                hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st, kin, vin, T, A.fdev,
+                                   A.ptb, F, B.idrel);
+            else
+                hipLaunchKernelGGL((k_link_wide<DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st,
+                                   (const uint64_t*)B.keys_in, vin, T, nwords, A.fdev, A.ptb, F, B.idrel);
         });
         tm.end();
         // K5 explicit compaction
@@ -483,7 +680,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             constexpr int DD = decltype(dd)::value;
             hipLaunchKernelGGL((k_explicit<KeyT, DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.idrel,
                                (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, B.status, B.ticket, A.err, B.blocks,
-                               B.fo, &B.totals[0]);
+                               B.fo, &B.totals[0], nwords, T);
         });
         hipLaunchKernelGGL(k_frame_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[0]);
         tm.end();
@@ -496,7 +693,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     else with_rel(TypeTag<uint32_t>{});
 
     // K6/K7 L2
-    tm.begin("l2");
+    tm.begin("l2_stats");
     // frame record bases as a flat array for the record-space kernels
     hipLaunchKernelGGL(k_copy_rec_base, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, B.rec_base);
     const uint64_t rec_tiles_max = cdiv(T, CH_TILE);
@@ -505,6 +702,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         hipLaunchKernelGGL((k_l2_stats<DD>), dim3(grid_for(T, CH_WG * CH_ITEMS)), dim3(CH_WG), 0, st, B.blocks,
                            &B.totals[0], F, B.rec_base, B.fo);
         hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
+        tm.end();
+        tm.begin("l2_split");
         fill64(st, B.status, rec_tiles_max, 0);
         fill32(st, B.ticket, 1, 0);
         hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)rec_tiles_max), dim3(CH_WG), 0, st, B.blocks,
@@ -525,6 +724,51 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     hipLaunchKernelGGL(k_frame_corr, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.tb, B.tile_corr_off,
                        &B.totals[2]);
     tm.end();
+
+    if (format == CHAIN_FMT_LCH1) {
+        Carver cC;
+        cC.base = e->arena.base + size_a + size_b;
+        auto X = carve_c(cC);
+        entropy_stage(st, tm, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
+                      B.corr_val, A.tb, ntiles, tile1k, ntiles1k, comp_chunk_bound, rc, stride, A.err, B.ticket, X);
+        std::vector<uint64_t> offs(F), sizes(F);
+        uint64_t total = 0;
+        uint32_t errw[4];
+        HIPCHK(hipMemcpyAsync(offs.data(), X.out_off, F * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(sizes.data(), X.out_size, F * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&total, X.total, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(errw, A.err, sizeof errw, hipMemcpyDeviceToHost, st));
+        std::vector<CompInst> ci(5 * F);
+        std::vector<FrameOut> fo(F);
+        HIPCHK(hipMemcpyAsync(ci.data(), X.ci, 5 * F * sizeof(CompInst), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(fo.data(), B.fo, F * sizeof(FrameOut), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        require(errw[0] == 0, CHAIN_EDEVICE, "device error word set in the entropy stage");
+        require(total <= out_bound, CHAIN_EDEVICE, "container exceeds its bound");
+        res.stats.resize(F);
+        for (uint32_t f = 0; f < F; ++f) {
+            chain_stats& s = res.stats[f];
+            s.bytes = sizes[f];
+            s.block_replacement_size = ci[1 * F + f].size + ci[2 * F + f].size + ci[3 * F + f].size;
+            s.idrel_size = ci[0 * F + f].size;
+            s.rel_size = ci[4 * F + f].size;
+            s.num_chained = plan[f].n - fo[f].nrec;
+            s.num_corrections = fo[f].ncorr;
+        }
+        res.offsets = offs;
+        res.sizes = sizes;
+        if (res.keep_on_device) {
+            res.d_out = X.out;
+            return;
+        }
+        std::vector<uint8_t> all(total);
+        if (total) HIPCHK(hipMemcpyAsync(all.data(), X.out, total, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        res.containers.resize(F);
+        for (uint32_t f = 0; f < F; ++f)
+            res.containers[f].assign(all.begin() + offs[f], all.begin() + offs[f] + sizes[f]);
+        return;
+    }
 
     // ---- results to the host
     std::vector<FrameOut> fo(F);
@@ -583,6 +827,535 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     }
 }
 
+
+// ------------------------------------------------------------------------
+// decompression of a batch of LCH1 / LCP3 streams
+
+// little-endian reader over one stream inside the pooled input
+struct HostReader {
+    const uint8_t* base;   // pool
+    uint64_t pos, end;     // absolute offsets in the pool
+    const uint8_t* bytes(uint64_t k) {
+        require(k <= end - pos, CHAIN_ERUNTIME, "truncated LCPS bitstream");
+        const uint8_t* q = base + pos;
+        pos += k;
+        return q;
+    }
+    uint8_t u8() { return *bytes(1); }
+    uint32_t u32() { const uint8_t* q = bytes(4); uint32_t v = 0; for (int i = 0; i < 4; ++i) v |= (uint32_t)q[i] << (8 * i); return v; }
+    uint64_t u64() { const uint8_t* q = bytes(8); uint64_t v = 0; for (int i = 0; i < 8; ++i) v |= (uint64_t)q[i] << (8 * i); return v; }
+    int64_t i64() { return (int64_t)u64(); }
+    double f64() { return bits_double(u64()); }
+    uint64_t varuint() {
+        uint64_t v = 0;
+        for (int shift = 0; shift <= 63; shift += 7) {
+            const uint8_t b = u8();
+            require(shift != 63 || (b & 0xfeu) == 0, CHAIN_ERUNTIME, "LCPS varuint overflow");
+            v |= (uint64_t)(b & 0x7fu) << shift;
+            if ((b & 0x80u) == 0) return v;
+        }
+        throw Error(CHAIN_ERUNTIME, "invalid LCPS varuint");
+    }
+    HostReader sub(uint64_t len) {
+        require(len <= end - pos, CHAIN_ERUNTIME, "truncated LCPS bitstream");
+        HostReader r{base, pos, pos + len};
+        pos += len;
+        return r;
+    }
+    void finish() const { require(pos == end, CHAIN_ERUNTIME, "unexpected trailing LCPS bytes"); }
+};
+
+struct DecPlan {
+    std::vector<DecFrame> frames;
+    std::vector<DecSection> secs;
+    std::vector<DecChunk> chunks;
+    std::vector<DecFixed> fixed;
+    std::vector<DecRaw> raws;
+    std::vector<DecTagged> tagged;
+    std::vector<uint32_t> prob, cum;
+    std::vector<uint16_t> slots;
+    uint64_t T = 0, R = 0, ND = 0, NK = 0, tags = 0, tag_tiles = 0, fixed_total = 0, raw_total = 0;
+    uint32_t D = 0;
+};
+
+// RVC1 section (see chain_kernels_ent.hip): table parsed on the host
+static void parse_rvc1(HostReader r, uint64_t expected, int comp, uint64_t dst, DecPlan& P) {
+    require(std::memcmp(r.bytes(4), "RVC1", 4) == 0, CHAIN_ERUNTIME, "invalid RVC1 header");
+    const uint32_t pb = r.u8();
+    require(r.u64() == expected, CHAIN_ERUNTIME, "RVC1 symbol count mismatch");
+    HostReader tr = r.sub(r.u64());
+    const uint32_t L = r.u8();
+    const uint32_t S = r.u32();
+    const uint32_t nchunks = r.u32();
+    if (expected == 0) {
+        require(pb == 0 && tr.end == tr.pos && L == 0 && S == 0 && nchunks == 0, CHAIN_ERUNTIME,
+                "invalid empty RVC1 section");
+        r.finish();
+        return;
+    }
+    require(pb >= 12 && pb <= 20, CHAIN_ERUNTIME, "invalid RVC1 precision");
+    require(L >= 1 && L <= 64 && (L & (L - 1)) == 0 && S >= 1, CHAIN_ERUNTIME, "invalid RVC1 lanes or chunk size");
+    require(nchunks == cdiv(expected, S), CHAIN_ERUNTIME, "RVC1 chunk count mismatch");
+    const uint64_t A = tr.varuint();
+    require(A > 0 && A <= RANS_VALUE_LIMIT, CHAIN_ERUNTIME, "RVC1 alphabet exceeds the GPU decoder's 4096 symbols");
+    std::vector<uint32_t> prob(A, 0);
+    for (uint64_t i = 0; i < A;) {
+        const uint8_t first = tr.u8();
+        const uint32_t token = first & 3u;
+        if (token == 3) {
+            const uint64_t zeros = (first >> 2) + 1u;
+            require(zeros <= A - i, CHAIN_ERUNTIME, "invalid rANS zero run");
+            i += zeros;
+        } else {
+            uint32_t pr = first >> 2;
+            for (uint32_t b = 0; b < token; ++b) pr |= (uint32_t)tr.u8() << (6 + 8 * b);
+            require(pr > 0, CHAIN_ERUNTIME, "invalid rANS probability");
+            prob[i++] = pr;
+        }
+    }
+    tr.finish();
+    const uint32_t M = 1u << pb;
+    DecSection sec;
+    std::memset(&sec, 0, sizeof sec);
+    sec.pb = pb;
+    sec.lanes = L;
+    sec.alphabet = (uint32_t)A;
+    sec.chunk_symbols = S;
+    sec.count = expected;
+    sec.table_off = P.prob.size();
+    sec.comp = comp;
+    sec.dst = dst;
+    uint32_t sum = 0;
+    for (uint64_t k = 0; k < A; ++k) {
+        require(prob[k] <= M - sum, CHAIN_ERUNTIME, "invalid rANS probability sum");
+        P.prob.push_back(prob[k]);
+        P.cum.push_back(sum);
+        sum += prob[k];
+    }
+    require(sum == M, CHAIN_ERUNTIME, "incomplete rANS probability table");
+    if (M <= (1u << 14)) {
+        sec.slot_off = P.slots.size();
+        for (uint64_t k = 0; k < A; ++k)
+            for (uint32_t j = 0; j < prob[k]; ++j) P.slots.push_back((uint16_t)k);
+    } else {
+        sec.slot_off = ~0ull;
+    }
+    const uint32_t si = (uint32_t)P.secs.size();
+    P.secs.push_back(sec);
+    const uint8_t* sizes = r.bytes(4ull * nchunks);
+    for (uint32_t k = 0; k < nchunks; ++k) {
+        uint32_t cb = 0;
+        for (int i = 0; i < 4; ++i) cb |= (uint32_t)sizes[4 * k + i] << (8 * i);
+        require(cb >= 4u * L, CHAIN_ERUNTIME, "RVC1 chunk too small");
+        DecChunk ch;
+        ch.sec = si;
+        ch.bytes = cb;
+        ch.byte_off = r.pos;
+        ch.first = (uint64_t)k * S;
+        (void)r.bytes(cb);
+        P.chunks.push_back(ch);
+    }
+    r.finish();
+}
+
+static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint64_t max_records, uint64_t dst,
+                            DecPlan& P, bool raw_only, uint64_t* n_out) {
+    const uint8_t* magic = r.bytes(4);
+    const uint32_t dims = r.u32();
+    const uint64_t n = r.u64();
+    require(dims == expected_dims && n <= max_records, CHAIN_ERUNTIME, "LCPS component shape mismatch");
+    *n_out = n;
+    if (raw_only) require(std::memcmp(magic, "RAW1", 4) == 0, CHAIN_ERUNTIME, "LCP3 requires raw component streams");
+    if (std::memcmp(magic, "RAW1", 4) == 0) {
+        require(r.end - r.pos == n * dims * 8, CHAIN_ERUNTIME, "RAW1 matrix size mismatch");
+        if (n) {
+            DecRaw j;
+            std::memset(&j, 0, sizeof j);
+            j.byte_off = r.pos;
+            j.count = n * dims;
+            j.dst = dst;
+            j.comp = comp;
+            j.first_thread = P.raw_total;
+            P.raw_total += j.count;
+            P.raws.push_back(j);
+        }
+        return;
+    }
+    if (std::memcmp(magic, "FIXP", 4) == 0) {
+        std::vector<DecFixed> js(dims);
+        uint64_t total = 0;
+        for (uint32_t d = 0; d < dims; ++d) {
+            DecFixed& j = js[d];
+            std::memset(&j, 0, sizeof j);
+            j.minimum = (uint64_t)r.i64();
+            j.width = r.u8();
+            const uint64_t bytes = r.u64();
+            require(j.width >= 1 && j.width <= 64, CHAIN_ERUNTIME, "invalid fixed width");
+            require(bytes == (n * j.width + 7) / 8, CHAIN_ERUNTIME, "invalid fixed payload size");
+            require((int64_t)j.minimum >= 0, CHAIN_ERUNTIME, "fixed value overflow");
+            j.n = n;
+            j.dims = dims;
+            j.dst = dst + d;
+            j.comp = comp;
+            j.byte_off = total;
+            total += bytes;
+        }
+        require(r.end - r.pos == total, CHAIN_ERUNTIME, "fixed raw size mismatch");
+        for (uint32_t d = 0; d < dims; ++d) {
+            js[d].byte_off += r.pos;
+            if (n) {
+                js[d].first_thread = P.fixed_total;
+                P.fixed_total += n;
+                P.fixed.push_back(js[d]);
+            }
+        }
+        return;
+    }
+    if (std::memcmp(magic, "RANS", 4) == 0) {
+        parse_rvc1(r.sub(r.u64()), n * dims, comp, dst, P);
+        r.finish();
+        return;
+    }
+    require(std::memcmp(magic, "TAGR", 4) == 0, CHAIN_ERUNTIME, "unknown LCPS encoding");
+    const uint32_t maxw = r.u8();
+    require(n == 0 ? maxw == 0 : (maxw >= 1 && maxw <= 63), CHAIN_ERUNTIME, "Tagged maximum width mismatch");
+    const uint64_t tag_off = P.tags;
+    parse_rvc1(r.sub(r.u64()), n, -1, tag_off, P);
+    P.tags += n;
+    const uint64_t bytes = r.u64();
+    const uint64_t at = r.pos;
+    (void)r.bytes(bytes);
+    r.finish();
+    if (n) {
+        DecTagged j;
+        std::memset(&j, 0, sizeof j);
+        j.byte_off = at;
+        j.bytes = bytes;
+        j.n = n;
+        j.tag_off = tag_off;
+        j.dst = dst;
+        j.dims = dims;
+        j.tile0 = (uint32_t)P.tag_tiles;
+        j.comp = comp;
+        P.tag_tiles += cdiv(n, CH_TILE);
+        P.tagged.push_back(j);
+    }
+}
+
+static void parse_stream(const uint8_t* pool, uint64_t off, uint64_t len, DecPlan& P) {
+    HostReader r{pool, off, off + len};
+    const uint8_t* magic = r.bytes(4);
+    const bool lch1 = std::memcmp(magic, "LCH1", 4) == 0, lcp3 = std::memcmp(magic, "LCP3", 4) == 0;
+    require(lch1 || lcp3 || std::memcmp(magic, "LCP2", 4) != 0, CHAIN_ERUNTIME,
+            "LCP2 streams carry Zstd sections; decode them with lcps_cpu_decompress");
+    require(lch1 || lcp3, CHAIN_ERUNTIME, "expected a self-contained LCH1 or LCP3 bitstream");
+    DecFrame F;
+    std::memset(&F, 0, sizeof F);
+    const uint32_t D = r.u32();
+    F.n = r.u64();
+    require(D > 0 && F.n > 0, CHAIN_ERUNTIME, "invalid LCP2 shape");
+    require(D <= CH_MAXD, CHAIN_EINVAL, "the GPU engine supports 1 to 4 dimensions");
+    require(P.D == 0 || P.D == D, CHAIN_EINVAL, "every stream of a batch must have the same dimensions");
+    P.D = D;
+    const int64_t p = r.i64();
+    F.eb = r.f64();
+    require(p > 0 && F.eb > 0.0 && std::isfinite(2.0 * F.eb), CHAIN_ERUNTIME, "invalid LCP2 quantization parameters");
+    F.p = (uint64_t)p;
+    F.denom = 2.0 * F.eb;
+    F.wrap_min = r.i64();
+    F.wrap_max = r.i64();
+    require(F.wrap_min >= 0, CHAIN_ERUNTIME, "invalid LCP2 block range");
+    require(F.wrap_min <= F.wrap_max, CHAIN_ERUNTIME, "invalid block-ID wrap range");
+    const uint64_t diffw = (uint64_t)(F.wrap_max - F.wrap_min);
+    require(diffw < (uint64_t)INT64_MAX, CHAIN_ERUNTIME, "block-ID range is too wide for wrapped differences");
+    F.range = diffw + 1;
+    F.max_corr = (int64_t)(F.range / 2);
+    F.min_corr = -F.max_corr;
+    if ((F.range & 1) == 0) --F.max_corr;
+    for (uint32_t d = 0; d < D; ++d) {
+        F.mins[d] = r.f64();
+        require(std::isfinite(F.mins[d]), CHAIN_ERUNTIME, "invalid LCP2 coordinate minimum");
+    }
+    F.pt_off = P.T;
+    F.out_off = P.T * D;
+    const uint32_t cd[5] = {1, D, D, 1, D};
+    uint64_t cn[5];
+    uint64_t dst[5];
+    // decode targets: choice2/diff/keep bases are only known after their
+    // counts, so parse in order and fix the record-space targets as we go
+    dst[0] = F.pt_off;
+    dst[4] = F.pt_off * D;
+    for (int c = 0; c < 5; ++c) {
+        HostReader cr = r.sub(r.u64());
+        if (c == 1) dst[1] = P.ND * D;
+        if (c == 2) dst[2] = P.NK * D;
+        if (c == 3) dst[3] = P.R;
+        parse_component(cr, c, cd[c], F.n, dst[c], P, lcp3, &cn[c]);
+        if (c == 1) { F.diff_base = P.ND; F.ndiff = cn[1]; P.ND += cn[1]; }
+        if (c == 2) { F.keep_base = P.NK; F.nkeep = cn[2]; P.NK += cn[2]; }
+        if (c == 3) { F.rec_base = P.R; F.nrec = cn[3]; P.R += cn[3]; }
+    }
+    require(cn[0] == F.n && cn[4] == F.n, CHAIN_ERUNTIME, "LCP2 point count mismatch");
+    require(F.ndiff + F.nkeep == F.nrec, CHAIN_ERUNTIME, "unused block differences or kept blocks");
+    F.ncorr = r.u64();
+    require(F.ncorr <= F.n * D, CHAIN_ERUNTIME, "invalid LCP2 correction count");
+    HostReader er = r.sub(r.u64());
+    require(er.end - er.pos == 16 * F.ncorr, CHAIN_ERUNTIME, "raw exact correction size mismatch");
+    F.corr_off = er.pos;
+    r.finish();
+    P.T += F.n;
+    P.frames.push_back(F);
+}
+
+struct DecodeResult {
+    bool keep_on_device = false;
+    const uint8_t* d_pool = nullptr;   // device copy of the concatenated streams (skips the upload)
+    std::vector<std::vector<uint8_t>> outs;
+    std::vector<uint64_t> ns, out_off;
+    uint32_t dims = 0;
+    void* d_out = nullptr;
+};
+
+static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, const size_t* lens, uint32_t nstreams,
+                                  int out_dtype, int out_layout, DecodeResult& res) {
+    require(nstreams >= 1, CHAIN_EINVAL, "empty batch");
+    require(out_dtype == CHAIN_F32 || out_dtype == CHAIN_F64, CHAIN_EINVAL, "output dtype must be float32 or float64");
+    require(out_layout == CHAIN_AOS || out_layout == CHAIN_SOA, CHAIN_EINVAL, "layout must be AoS or SoA");
+    hipStream_t st = e->stream;
+    StageTimer& tm = e->timer;
+    tm.reset();
+    // pool the inputs
+    std::vector<uint64_t> in_off(nstreams + 1, 0);
+    for (uint32_t k = 0; k < nstreams; ++k) in_off[k + 1] = in_off[k] + lens[k];
+    std::vector<uint8_t> pool(in_off[nstreams]);
+    for (uint32_t k = 0; k < nstreams; ++k)
+        if (lens[k]) std::memcpy(pool.data() + in_off[k], bytes[k], lens[k]);
+    DecPlan P;
+    for (uint32_t k = 0; k < nstreams; ++k) parse_stream(pool.data(), in_off[k], lens[k], P);
+    const uint32_t F = nstreams, D = P.D;
+    require(P.T < (1ull << 31), CHAIN_EINVAL, "a GPU decode batch holds fewer than 2^31 particles");
+    // tiles: points (chain resolution) and records (L2 inverse), per frame
+    std::vector<uint32_t> ptb(F + 1, 0), rtb(F + 1, 0), ptb32(F + 1, 0);
+    for (uint32_t f = 0; f < F; ++f) {
+        P.frames[f].tile0 = ptb[f];
+        P.frames[f].rtile0 = rtb[f];
+        ptb[f + 1] = ptb[f] + (uint32_t)cdiv(P.frames[f].n, CH_TILE);
+        rtb[f + 1] = rtb[f] + (uint32_t)cdiv(P.frames[f].nrec, CH_TILE);
+        ptb32[f + 1] = (uint32_t)(P.frames[f].pt_off + P.frames[f].n);
+    }
+    const uint32_t npt = ptb[F], nrt = rtb[F];
+    const size_t es = out_dtype == CHAIN_F32 ? 4 : 8;
+    const uint64_t T = P.T;
+    std::vector<uint64_t> seg_base_rec(F), seg_n_rec(F), seg_base_pt(F), seg_n_pt(F);
+    for (uint32_t f = 0; f < F; ++f) {
+        seg_base_rec[f] = P.frames[f].rec_base;
+        seg_n_rec[f] = P.frames[f].nrec;
+        seg_base_pt[f] = P.frames[f].pt_off;
+        seg_n_pt[f] = P.frames[f].n;
+    }
+    auto carve = [&](Carver& c) {
+        struct {
+            uint8_t* in;
+            DecFrame* fr;
+            DecSection* secs;
+            DecChunk* chunks;
+            DecFixed* fixed;
+            DecRaw* raws;
+            DecTagged* tagged;
+            uint32_t* prob;
+            uint32_t* cum;
+            uint16_t* slots;
+            uint32_t *idrel, *rel;
+            uint64_t *diff, *keep;
+            uint8_t *choice, *tags;
+            uint64_t* compact;
+            uint32_t* root;
+            uint32_t *zc, *zoff, *ec, *eoff;
+            uint32_t *ptb, *rtb, *ptb32;
+            uint64_t *sbr, *snr, *sbp, *snp;
+            uint64_t* status;
+            uint32_t* flags;
+            uint64_t* aggv;
+            uint32_t* done;
+            uint32_t* ticket;
+            uint32_t* err;
+            uint64_t* totals;
+            uint8_t* out;
+        } x;
+        x.in = c.take<uint8_t>(pool.size());
+        x.fr = c.take<DecFrame>(F);
+        x.secs = c.take<DecSection>(P.secs.size());
+        x.chunks = c.take<DecChunk>(P.chunks.size());
+        x.fixed = c.take<DecFixed>(P.fixed.size());
+        x.raws = c.take<DecRaw>(P.raws.size());
+        x.tagged = c.take<DecTagged>(P.tagged.size());
+        x.prob = c.take<uint32_t>(P.prob.size());
+        x.cum = c.take<uint32_t>(P.cum.size());
+        x.slots = c.take<uint16_t>(P.slots.size());
+        x.idrel = c.take<uint32_t>(T);
+        x.rel = c.take<uint32_t>(T * D);
+        x.diff = c.take<uint64_t>(P.ND * D);
+        x.keep = c.take<uint64_t>(P.NK * D);
+        x.choice = c.take<uint8_t>(P.R);
+        x.tags = c.take<uint8_t>(P.tags);
+        x.compact = c.take<uint64_t>(P.R * D);
+        x.root = c.take<uint32_t>(T);
+        x.zc = c.take<uint32_t>(nrt + 1);
+        x.zoff = c.take<uint32_t>(nrt + 1);
+        x.ec = c.take<uint32_t>(npt + 1);
+        x.eoff = c.take<uint32_t>(npt + 1);
+        x.ptb = c.take<uint32_t>(F + 1);
+        x.rtb = c.take<uint32_t>(F + 1);
+        x.ptb32 = c.take<uint32_t>(F + 1);
+        x.sbr = c.take<uint64_t>(F);
+        x.snr = c.take<uint64_t>(F);
+        x.sbp = c.take<uint64_t>(F);
+        x.snp = c.take<uint64_t>(F);
+        x.status = c.take<uint64_t>(std::max<uint64_t>({P.tag_tiles, cdiv(npt + 1, CH_TILE), cdiv(nrt + 1, CH_TILE)}) + 1);
+        x.flags = c.take<uint32_t>(nrt + 1);
+        x.aggv = c.take<uint64_t>((uint64_t)(nrt + 1) * 2 * CH_MAXD);
+        x.done = c.take<uint32_t>(npt + 1);
+        x.ticket = c.take<uint32_t>(4);
+        x.err = c.take<uint32_t>(4);
+        x.totals = c.take<uint64_t>(4);
+        x.out = c.take<uint8_t>(T * D * es);
+        return x;
+    };
+    Carver c0;
+    (void)carve(c0);
+    e->arena_d.ensure(c0.off, e->budget);
+    Carver c1;
+    c1.base = e->arena_d.base;
+    auto X = carve(c1);
+    if (res.d_pool) X.in = const_cast<uint8_t*>(res.d_pool);
+    auto h2d = [&](void* dst, const void* src, size_t n) {
+        if (n) HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+    };
+    tm.begin("h2d");
+    if (!res.d_pool) h2d(X.in, pool.data(), pool.size());
+    h2d(X.fr, P.frames.data(), F * sizeof(DecFrame));
+    h2d(X.secs, P.secs.data(), P.secs.size() * sizeof(DecSection));
+    h2d(X.chunks, P.chunks.data(), P.chunks.size() * sizeof(DecChunk));
+    h2d(X.fixed, P.fixed.data(), P.fixed.size() * sizeof(DecFixed));
+    h2d(X.raws, P.raws.data(), P.raws.size() * sizeof(DecRaw));
+    h2d(X.tagged, P.tagged.data(), P.tagged.size() * sizeof(DecTagged));
+    h2d(X.prob, P.prob.data(), P.prob.size() * 4);
+    h2d(X.cum, P.cum.data(), P.cum.size() * 4);
+    h2d(X.slots, P.slots.data(), P.slots.size() * 2);
+    h2d(X.ptb, ptb.data(), (F + 1) * 4);
+    h2d(X.rtb, rtb.data(), (F + 1) * 4);
+    h2d(X.ptb32, ptb32.data(), (F + 1) * 4);
+    h2d(X.sbr, seg_base_rec.data(), F * 8);
+    h2d(X.snr, seg_n_rec.data(), F * 8);
+    h2d(X.sbp, seg_base_pt.data(), F * 8);
+    h2d(X.snp, seg_n_pt.data(), F * 8);
+    fill32(st, X.err, 4, 0);
+    tm.end();
+
+    tm.begin("rans_decode");
+    if (!P.chunks.empty())
+        hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.chunks.size()), dim3(64), 0, st, X.in, X.secs, X.chunks,
+                           X.prob, X.cum, X.slots, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+    tm.end();
+    tm.begin("unpack");
+    if (P.fixed_total)
+        hipLaunchKernelGGL(k_dec_fixed, dim3(grid_for(P.fixed_total)), dim3(CH_WG), 0, st, X.in, X.fixed,
+                           (uint32_t)P.fixed.size(), P.fixed_total, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+    if (P.raw_total)
+        hipLaunchKernelGGL(k_dec_raw, dim3(grid_for(P.raw_total)), dim3(CH_WG), 0, st, X.in, X.raws,
+                           (uint32_t)P.raws.size(), P.raw_total, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+    if (P.tag_tiles) {
+        fill64(st, X.status, P.tag_tiles, 0);
+        fill32(st, X.ticket, 1, 0);
+        hipLaunchKernelGGL(k_dec_tagged, dim3((unsigned)P.tag_tiles), dim3(CH_WG), 0, st, X.in, X.tagged,
+                           (uint32_t)P.tagged.size(), X.tags, X.status, X.ticket, X.idrel, X.diff, X.keep, X.choice,
+                           X.rel, X.err);
+    }
+    tm.end();
+    tm.begin("l2_inverse");
+    if (nrt) {
+        hipLaunchKernelGGL(k_dec_tile_zero_counts, dim3(nrt), dim3(CH_WG), 0, st, X.choice, (const uint32_t*)nullptr,
+                           X.rtb, X.sbr, X.snr, F, X.zc);
+        scan_u32(st, X.zc, X.zoff, nrt, X.status, X.ticket, X.err, &X.totals[0]);
+        fill32(st, X.flags, nrt + 1, 0);
+        fill32(st, X.ticket, 1, 0);
+        with_dims(D, [&](auto dd) {
+            constexpr int DD = decltype(dd)::value;
+            hipLaunchKernelGGL((k_dec_l2<DD>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice, X.diff,
+                               X.keep, X.zoff, X.flags, X.aggv, X.ticket, X.compact, X.err);
+        });
+    }
+    tm.end();
+    tm.begin("chain");
+    hipLaunchKernelGGL(k_dec_tile_zero_counts, dim3(npt), dim3(CH_WG), 0, st, (const uint8_t*)nullptr, X.idrel, X.ptb,
+                       X.sbp, X.snp, F, X.ec);
+    scan_u32(st, X.ec, X.eoff, npt, X.status, X.ticket, X.err, &X.totals[1]);
+    hipLaunchKernelGGL(k_dec_check_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.eoff,
+                       &X.totals[1], X.err);
+    fill32(st, X.done, npt + 1, 0);
+    fill32(st, X.ticket, 1, 0);
+    with_dims(D, [&](auto dd) {
+        constexpr int DD = decltype(dd)::value;
+        hipLaunchKernelGGL((k_dec_chain<DD>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F, X.ptb, npt, X.idrel, X.eoff,
+                           X.ticket, X.done, X.root, X.err);
+    });
+    tm.end();
+    tm.begin("dequantize");
+    with_dims(D, [&](auto dd) {
+        constexpr int DD = decltype(dd)::value;
+        auto go = [&](auto ot) {
+            using O = typename decltype(ot)::type;
+            if (out_layout == CHAIN_SOA) {
+                hipLaunchKernelGGL((k_dec_dequant<DD, O, true>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
+                                   X.ptb32, T, X.root, X.compact, X.rel, (O*)X.out, X.err);
+                hipLaunchKernelGGL((k_dec_corr<O, true>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
+                                   (O*)X.out, X.err);
+            } else {
+                hipLaunchKernelGGL((k_dec_dequant<DD, O, false>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
+                                   X.ptb32, T, X.root, X.compact, X.rel, (O*)X.out, X.err);
+                hipLaunchKernelGGL((k_dec_corr<O, false>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
+                                   (O*)X.out, X.err);
+            }
+        };
+        if (out_dtype == CHAIN_F32) go(TypeTag<float>{});
+        else go(TypeTag<double>{});
+    });
+    tm.end();
+    uint32_t errw[4];
+    HIPCHK(hipMemcpyAsync(errw, X.err, sizeof errw, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (errw[0]) {
+        std::string m = "invalid LCPS bitstream (device checks:";
+        if (errw[0] & DERR_RANS) m += " rANS";
+        if (errw[0] & DERR_VALUE) m += " value";
+        if (errw[0] & DERR_L2) m += " block";
+        if (errw[0] & DERR_CHAIN) m += " chain";
+        if (errw[0] & DERR_QUANT) m += " offset";
+        if (errw[0] & DERR_CORR) m += " correction";
+        if (errw[0] & DERR_COUNT) m += " count";
+        if (errw[0] & (DERR_WAIT | 1u)) m += " wait";
+        throw Error(errw[0] & (DERR_WAIT | 1u) ? CHAIN_EDEVICE : CHAIN_ERUNTIME, m + ")");
+    }
+    res.ns.resize(F);
+    res.out_off.resize(F);
+    res.dims = D;
+    for (uint32_t f = 0; f < F; ++f) {
+        res.ns[f] = P.frames[f].n;
+        res.out_off[f] = P.frames[f].out_off;
+    }
+    if (res.keep_on_device) {
+        res.d_out = X.out;
+        return;
+    }
+    res.outs.resize(F);
+    std::vector<uint8_t> all(T * D * es);
+    HIPCHK(hipMemcpyAsync(all.data(), X.out, all.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (uint32_t f = 0; f < F; ++f) {
+        const uint64_t a = P.frames[f].out_off * es, b = a + P.frames[f].n * D * es;
+        res.outs[f].assign(all.begin() + a, all.begin() + b);
+    }
+}
 }  // namespace chain
 
 // ------------------------------------------------------------------------
@@ -693,14 +1466,94 @@ int chain_compress_batch(chain_engine* e, const chain_frame* frames, uint32_t nf
     });
 }
 
+int chain_compress_batch_device(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
+                                uint32_t dims, void** d_out, uint64_t* offsets, uint64_t* sizes, chain_stats* stats) {
+    return guarded(e, [&] {
+        if (!frames || !d_out || !offsets || !sizes) throw Error(CHAIN_EINVAL, "NULL argument");
+        chain::CompressResult res;
+        res.keep_on_device = true;
+        chain::compress_batch_impl(e, frames, nframes, dtype, layout, dims, true, CHAIN_FMT_LCH1, res);
+        *d_out = res.d_out;
+        for (uint32_t f = 0; f < nframes; ++f) {
+            offsets[f] = res.offsets[f];
+            sizes[f] = res.sizes[f];
+            if (stats) stats[f] = res.stats[f];
+        }
+    });
+}
+
+int chain_engine_set_rans(chain_engine* e, uint32_t lanes_max, uint32_t chunk_symbols) {
+    return guarded(e, [&] {
+        if (lanes_max < 1 || lanes_max > 64 || (lanes_max & (lanes_max - 1)))
+            throw Error(CHAIN_EINVAL, "lanes_max must be a power of two in [1, 64]");
+        if (chunk_symbols < 64 || chunk_symbols > (1u << 20))
+            throw Error(CHAIN_EINVAL, "chunk_symbols must be in [64, 2^20]");
+        e->rans.lanes_max = lanes_max;
+        e->rans.chunk_symbols = chunk_symbols;
+    });
+}
+
 int chain_decompress(chain_engine* e, const uint8_t* bytes, size_t len, int out_dtype, int out_layout,
                      chain_buf* out, uint64_t* n, uint32_t* dims) {
-    return guarded(e, [&] { throw Error(CHAIN_ERUNTIME, "decompression is not built yet"); });
+    return guarded(e, [&] {
+        if (!bytes || !out) throw Error(CHAIN_EINVAL, "NULL argument");
+        chain::DecodeResult res;
+        chain::decompress_batch_impl(e, &bytes, &len, 1, out_dtype, out_layout, res);
+        to_buf(res.outs[0], out);
+        if (n) *n = res.ns[0];
+        if (dims) *dims = res.dims;
+    });
 }
 
 int chain_decompress_batch(chain_engine* e, const uint8_t* const* bytes, const size_t* lens, uint32_t nstreams,
                            int out_dtype, int out_layout, chain_buf* outs, uint64_t* ns, uint32_t* dims) {
-    return guarded(e, [&] { throw Error(CHAIN_ERUNTIME, "decompression is not built yet"); });
+    return guarded(e, [&] {
+        if (!bytes || !lens || !outs) throw Error(CHAIN_EINVAL, "NULL argument");
+        chain::DecodeResult res;
+        chain::decompress_batch_impl(e, bytes, lens, nstreams, out_dtype, out_layout, res);
+        for (uint32_t k = 0; k < nstreams; ++k) {
+            to_buf(res.outs[k], &outs[k]);
+            if (ns) ns[k] = res.ns[k];
+            if (dims) dims[k] = res.dims;
+        }
+    });
+}
+
+int chain_decompress_batch_device(chain_engine* e, const uint8_t* const* bytes, const size_t* lens,
+                                  const void* d_streams, uint32_t nstreams, int out_dtype, int out_layout,
+                                  void** d_out, uint64_t* out_offsets, uint64_t* ns, uint32_t* dims) {
+    return guarded(e, [&] {
+        if (!bytes || !lens || !d_out || !out_offsets) throw Error(CHAIN_EINVAL, "NULL argument");
+        chain::DecodeResult res;
+        res.keep_on_device = true;
+        res.d_pool = (const uint8_t*)d_streams;
+        chain::decompress_batch_impl(e, bytes, lens, nstreams, out_dtype, out_layout, res);
+        *d_out = res.d_out;
+        for (uint32_t k = 0; k < nstreams; ++k) {
+            out_offsets[k] = res.out_off[k];
+            if (ns) ns[k] = res.ns[k];
+            if (dims) dims[k] = res.dims;
+        }
+    });
+}
+
+int chain_device_malloc(chain_engine* e, uint64_t bytes, void** out) {
+    return guarded(e, [&] {
+        if (!out) throw Error(CHAIN_EINVAL, "NULL argument");
+        *out = nullptr;
+        HIPCHK(hipMalloc(out, bytes ? bytes : 1));
+    });
+}
+
+int chain_device_free(chain_engine* e, void* p) {
+    return guarded(e, [&] { HIPCHK(hipFree(p)); });
+}
+
+int chain_copy(chain_engine* e, void* dst, const void* src, uint64_t bytes, int to_device) {
+    return guarded(e, [&] {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+    });
 }
 
 int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, size_t names_len) {

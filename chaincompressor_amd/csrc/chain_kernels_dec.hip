@@ -1,0 +1,635 @@
+// Decoder kernels of the Chain engine (gfx950): LCH1 and LCP3 streams to
+// coordinates. Follows the reference decoder's semantics
+// (cpp/decompress/lcps_cpu_decompress_serial.cpp:1182-1338):
+//   component decode -> L2 inverse (:1230-1275) -> backward chain pass
+//   (:1277-1302) -> fma reconstruction and exact corrections (:1305-1326).
+// Parallel form: rANS chunks per wavefront, a segmented modular scan with
+// decoupled look-back for the L2 inverse, and an O(N) chain resolution that
+// walks tiles from the end (every successor lies at a larger index).
+#include "chain_common.hpp"
+#include "chain_rans.hpp"
+
+#pragma clang fp contract(off)
+
+namespace chain {
+
+enum : uint32_t {
+    DERR_RANS = 1u << 4,        // rANS state / stream mismatch
+    DERR_VALUE = 1u << 5,       // value out of range for its stream
+    DERR_L2 = 1u << 6,          // invalid choice / correction / kept block
+    DERR_CHAIN = 1u << 7,       // successor outside the frame / unresolved chain
+    DERR_QUANT = 1u << 8,       // offset >= p or quantised overflow
+    DERR_CORR = 1u << 9,        // invalid exact correction
+    DERR_COUNT = 1u << 10,      // explicit points != records
+    DERR_WAIT = 1u << 11,       // bounded wait expired
+};
+
+// one decoded component stream of the batch: values go to a typed array
+struct DecFrame {
+    uint64_t n;                 // points
+    uint64_t pt_off;            // global point offset
+    uint64_t rec_base, nrec;    // explicit records (choice2 count)
+    uint64_t diff_base, ndiff, keep_base, nkeep;
+    uint64_t p;
+    double eb, denom;
+    double mins[CH_MAXD];
+    int64_t wrap_min, wrap_max;
+    uint64_t range;             // wrap_max - wrap_min + 1
+    int64_t min_corr, max_corr;
+    uint64_t out_off;           // element offset of the frame in the output
+    uint64_t corr_off, ncorr;   // byte offset of the raw correction records in the input pool
+    uint32_t tile0, rtile0;     // first point tile, first record tile
+};
+
+// destination of a decoded value: component c of frame f, value index k
+__device__ inline void store_value(int c, uint64_t gidx, uint64_t v, uint32_t* idrel, uint64_t* diff, uint64_t* keep,
+                                   uint8_t* choice, uint32_t* rel, uint32_t* err) {
+    switch (c) {
+        case 0: if (v >> 32) atomicOr(err, DERR_VALUE); idrel[gidx] = (uint32_t)v; break;
+        case 1: diff[gidx] = v; break;
+        case 2: keep[gidx] = v; break;
+        case 3: if (v > 1) atomicOr(err, DERR_VALUE); choice[gidx] = (uint8_t)v; break;
+        default: if (v >> 32) atomicOr(err, DERR_VALUE); rel[gidx] = (uint32_t)v; break;
+    }
+}
+
+struct DecSection {
+    uint32_t pb, lanes, alphabet, chunk_symbols;
+    uint64_t count;
+    uint64_t table_off;         // entry offset into the prob/cum pools
+    uint64_t slot_off;          // entry offset into the slot pool (~0: binary search)
+    int32_t comp;               // 0..4 values of that component, -1 widths
+    uint32_t pad;
+    uint64_t dst;               // first global value index (values) or first width index (tags)
+};
+
+struct DecChunk {
+    uint32_t sec;
+    uint32_t bytes;             // chunk bytes (states + stream)
+    uint64_t byte_off;          // chunk start (states) in the input pool
+    uint64_t first;             // first symbol of the chunk within its section
+};
+
+__global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in, const DecSection* __restrict__ secs,
+                                                 const DecChunk* __restrict__ chunks,
+                                                 const uint32_t* __restrict__ prob_pool,
+                                                 const uint32_t* __restrict__ cum_pool,
+                                                 const uint16_t* __restrict__ slot_pool, uint8_t* __restrict__ tags,
+                                                 uint32_t* idrel, uint64_t* diff, uint64_t* keep, uint8_t* choice,
+                                                 uint32_t* rel, uint32_t* err) {
+    const DecChunk ch = chunks[blockIdx.x];
+    const DecSection S = secs[ch.sec];
+    __shared__ uint32_t s_prob[RANS_VALUE_LIMIT], s_cum[RANS_VALUE_LIMIT];
+    __shared__ uint16_t s_slot[1u << 14];
+    const uint32_t lane = threadIdx.x, pb = S.pb, M = 1u << pb, lower = 4u * M;
+    for (uint32_t b = lane; b < S.alphabet; b += 64) {
+        s_prob[b] = prob_pool[S.table_off + b];
+        s_cum[b] = cum_pool[S.table_off + b];
+    }
+    const bool lds_slots = S.slot_off != ~0ull;
+    if (lds_slots)
+        for (uint32_t b = lane; b < M; b += 64) s_slot[b] = slot_pool[S.slot_off + b];
+    __syncthreads();
+    const uint32_t L = S.lanes;
+    const uint64_t Sc = S.count - ch.first < S.chunk_symbols ? S.count - ch.first : S.chunk_symbols;
+    const uint64_t T = (Sc + L - 1) / L;
+    const uint8_t* p = in + ch.byte_off;
+    uint32_t x = lower;
+    if (lane < L) x = (uint32_t)p[4 * lane] | ((uint32_t)p[4 * lane + 1] << 8) | ((uint32_t)p[4 * lane + 2] << 16) |
+                      ((uint32_t)p[4 * lane + 3] << 24);
+    if (lane < L && (x < lower || x >= 256u * lower)) {
+        atomicOr(err, DERR_RANS);
+        x = lower;   // keep a corrupt stream's work bounded
+    }
+    const uint8_t* stream = p + 4 * L;
+    const uint64_t stream_len = ch.bytes - 4ull * L;
+    uint64_t pos = 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint64_t t = 0; t < T; ++t) {
+        const uint64_t k = t * L + lane;
+        const bool active = lane < L && k < Sc;
+        if (active) {
+            const uint32_t slot = x & (M - 1);
+            uint32_t s;
+            if (lds_slots) {
+                s = s_slot[slot];
+            } else {   // largest symbol with cum <= slot
+                uint32_t lo = 0, hi = S.alphabet;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_cum[mid] <= slot) lo = mid; else hi = mid;
+                }
+                s = lo;
+            }
+            x = s_prob[s] * (x >> pb) + slot - s_cum[s];
+            const uint64_t q = ch.first + k;
+            if (S.comp < 0) tags[S.dst + q] = (uint8_t)s;
+            else store_value(S.comp, S.dst + q, s, idrel, diff, keep, choice, rel, err);
+        }
+        // a valid stream needs at most ceil(pb / 8) <= 3 bytes per step
+        for (int round = 0; round < 4; ++round) {
+            const bool need = active && x < lower;
+            const uint64_t m = __ballot(need);
+            if (!m) break;
+            if (need) {
+                const uint64_t at = pos + __popcll(m & lt);
+                x = (x << 8) | (at < stream_len ? stream[at] : 0u);
+            }
+            pos += __popcll(m);
+        }
+        if (active && x < lower) {
+            atomicOr(err, DERR_RANS);
+            x = lower;
+        }
+    }
+    if (lane < L && x != lower) atomicOr(err, DERR_RANS);
+    if (lane == 0 && pos != stream_len) atomicOr(err, DERR_RANS);
+}
+
+// Fixed payload job: one component dimension of one frame
+struct DecFixed {
+    uint64_t byte_off;          // payload of this dimension in the input pool
+    uint64_t n;                 // values
+    uint64_t dst;               // first value index (record r -> dst + r*dims)
+    uint64_t minimum;
+    uint32_t width, dims;
+    int32_t comp;
+    uint32_t pad;
+    uint64_t first_thread;      // prefix of n over jobs
+};
+
+__global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__ in, const DecFixed* __restrict__ jobs,
+                                                     uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
+                                                     uint64_t* keep, uint8_t* choice, uint32_t* rel, uint32_t* err) {
+    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < total; g += (uint64_t)gridDim.x * CH_WG) {
+        uint32_t lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (jobs[mid].first_thread <= g) lo = mid; else hi = mid;
+        }
+        const DecFixed J = jobs[lo];
+        const uint64_t r = g - J.first_thread;
+        const uint64_t bit = r * J.width;
+        const uint8_t* p = in + J.byte_off + (bit >> 3);
+        const uint32_t sh = (uint32_t)(bit & 7);
+        const uint32_t nb = (sh + J.width + 7) >> 3;
+        unsigned __int128 acc = 0;
+        for (uint32_t k = 0; k < nb; ++k) acc = (acc << 8) | p[k];
+        const uint64_t v = (uint64_t)(acc >> (8 * nb - sh - J.width)) &
+                           (J.width >= 64 ? ~0ull : ((1ull << J.width) - 1ull));
+        if (v > (uint64_t)INT64_MAX - J.minimum) atomicOr(err, DERR_VALUE);
+        store_value(J.comp, J.dst + r * J.dims, J.minimum + v, idrel, diff, keep, choice, rel, err);
+    }
+}
+
+// raw int64 components (LCP3's RAW1)
+struct DecRaw {
+    uint64_t byte_off, count, dst;
+    int32_t comp;
+    uint32_t pad;
+    uint64_t first_thread;
+};
+
+__global__ __launch_bounds__(CH_WG) void k_dec_raw(const uint8_t* __restrict__ in, const DecRaw* __restrict__ jobs,
+                                                   uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
+                                                   uint64_t* keep, uint8_t* choice, uint32_t* rel, uint32_t* err) {
+    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < total; g += (uint64_t)gridDim.x * CH_WG) {
+        uint32_t lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (jobs[mid].first_thread <= g) lo = mid; else hi = mid;
+        }
+        const DecRaw J = jobs[lo];
+        const uint64_t k = g - J.first_thread;
+        const uint8_t* p = in + J.byte_off + 8 * k;
+        uint64_t v = 0;
+        for (int b = 0; b < 8; ++b) v |= (uint64_t)p[b] << (8 * b);
+        if ((int64_t)v < 0) atomicOr(err, DERR_VALUE);
+        store_value(J.comp, J.dst + k, v, idrel, diff, keep, choice, rel, err);
+    }
+}
+
+// Tagged payload job (one component of one frame): tiles of 4096 records
+struct DecTagged {
+    uint64_t byte_off;          // packed values in the input pool
+    uint64_t bytes;             // packed bytes
+    uint64_t n;                 // records
+    uint64_t tag_off;           // first width in the width pool
+    uint64_t dst;               // first value index
+    uint32_t dims, tile0;       // first tile of this job
+    int32_t comp;
+    uint32_t pad;
+};
+
+__global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict__ in,
+                                                      const DecTagged* __restrict__ jobs, uint32_t njobs,
+                                                      const uint8_t* __restrict__ tags, uint64_t* status,
+                                                      uint32_t* ticket, uint32_t* idrel, uint64_t* diff,
+                                                      uint64_t* keep, uint8_t* choice, uint32_t* rel,
+                                                      uint32_t* err) {
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].tile0 <= tile) lo = mid; else hi = mid;
+    }
+    const DecTagged J = jobs[lo];
+    const uint64_t r0 = (uint64_t)(tile - J.tile0) * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
+    uint32_t tg[CH_ITEMS];
+    uint32_t bits = 0;
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        tg[k] = r0 + k < J.n ? tags[J.tag_off + r0 + k] : 0u;
+        if (r0 + k < J.n && (tg[k] < 1 || tg[k] > 63)) atomicOr(err, DERR_VALUE);
+        bits += tg[k] * J.dims;
+    }
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(bits, s_tmp, &tot);
+    if (threadIdx.x < 64) {
+        const uint64_t e = lookback_seg(status, tile, J.tile0, tot, err);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    __syncthreads();
+    uint64_t bit = s_excl + pre;
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        if (r0 + k >= J.n) break;
+        const uint32_t w = tg[k];
+        for (uint32_t d = 0; d < J.dims; ++d) {
+            const uint64_t B = bit >> 3;
+            const uint32_t sh = (uint32_t)(bit & 7);
+            const uint32_t nb = (sh + w + 7) >> 3;
+            uint64_t acc_hi = 0, acc = 0;
+            for (uint32_t q = 0; q < nb; ++q) {
+                const uint8_t by = B + q < J.bytes ? in[J.byte_off + B + q] : 0;
+                acc_hi = (acc_hi << 8) | (acc >> 56);
+                acc = (acc << 8) | by;
+            }
+            const uint32_t drop = 8 * nb - sh - w;
+            uint64_t v = drop ? ((acc >> drop) | (drop < 64 ? (acc_hi << (64 - drop)) : 0)) : acc;
+            v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
+            if (B + nb > J.bytes) atomicOr(err, DERR_VALUE);
+            store_value(J.comp, J.dst + (r0 + k) * J.dims + d, v, idrel, diff, keep, choice, rel, err);
+            bit += w;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// per-tile counts: zeros of choice2 (record tiles) and explicit points (point tiles)
+
+__global__ __launch_bounds__(CH_WG) void k_dec_tile_zero_counts(const uint8_t* __restrict__ v8,
+                                                                const uint32_t* __restrict__ v32,
+                                                                const uint32_t* __restrict__ tile_begin,
+                                                                const uint64_t* __restrict__ seg_base,
+                                                                const uint64_t* __restrict__ seg_n, uint32_t F,
+                                                                uint32_t* __restrict__ counts) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t f = upper_find(tile_begin, F, tile);
+    const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+    uint32_t c = 0;
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint64_t r = r0 + (uint64_t)j * CH_WG + threadIdx.x;
+        if (r < seg_n[f]) c += v8 ? (v8[seg_base[f] + r] == 0) : (v32[seg_base[f] + r] == 0);
+    }
+    __shared__ uint32_t s_tmp[4];
+    uint32_t tot;
+    (void)block_excl_scan(c, s_tmp, &tot);
+    if (threadIdx.x == 0) counts[tile] = tot;
+}
+
+// ------------------------------------------------------------------------
+// L2 inverse: segmented modular scan over each frame's records.
+// element = (reset, value): keep records reset to keep - wrap_min, diff
+// records add (-correction) mod range. (r1,v1)+(r2,v2) = (r1|r2, r2 ? v2 : v1+v2 mod R)
+
+struct Agg {
+    uint32_t reset;
+    uint64_t v[CH_MAXD];
+};
+
+__device__ inline uint64_t addmod(uint64_t a, uint64_t b, uint64_t R) {
+    // a, b < R < 2^63
+    const uint64_t s = a + b;
+    return s >= R ? s - R : s;
+}
+
+template <int D>
+__device__ inline void agg_combine(Agg& left, const Agg& right, uint64_t R) {   // left = left (+) right
+    if (right.reset) {
+        left = right;
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) left.v[d] = addmod(left.v[d], right.v[d], R);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ fr, uint32_t F,
+                                                  const uint32_t* __restrict__ rtile_begin,
+                                                  const uint8_t* __restrict__ choice, const uint64_t* __restrict__ diff,
+                                                  const uint64_t* __restrict__ keep,
+                                                  const uint32_t* __restrict__ tile_zero_off,
+                                                  uint32_t* flags, uint64_t* aggv, uint32_t* ticket,
+                                                  uint64_t* __restrict__ compact, uint32_t* err) {
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint64_t s_v[CH_WG][D];
+    __shared__ uint32_t s_r[CH_WG];
+    __shared__ Agg s_in;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t f = upper_find(rtile_begin, F, tile);
+    const DecFrame& Fr = fr[f];
+    const uint64_t R = Fr.range;
+    const uint64_t r0 = (uint64_t)(tile - rtile_begin[f]) * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
+    // zeros before each record: tile prefix (global over all frames) + local
+    uint32_t z = 0;
+    for (int k = 0; k < CH_ITEMS; ++k) z += (r0 + k < Fr.nrec) ? (choice[Fr.rec_base + r0 + k] == 0) : 0u;
+    uint32_t ztot;
+    const uint32_t zpre = block_excl_scan(z, s_tmp, &ztot);
+    uint64_t kidx = (uint64_t)tile_zero_off[tile] + zpre;   // global keep index
+    // thread-local sequential aggregate
+    Agg a;
+    a.reset = 0;
+    for (int d = 0; d < D; ++d) a.v[d] = 0;
+    uint64_t vals[CH_ITEMS][D];
+    uint32_t resets = 0;
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t r = r0 + k;
+        if (r >= Fr.nrec) break;
+        const uint64_t grec = Fr.rec_base + r;
+        const uint8_t c = choice[grec];
+        if (c == 0) {
+            resets |= 1u << k;
+            a.reset = 1;
+            const bool inb = kidx >= Fr.keep_base && kidx < Fr.keep_base + Fr.nkeep;
+            if (!inb) atomicOr(err, DERR_L2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint64_t kv = inb ? keep[kidx * D + d] : (uint64_t)Fr.wrap_min;
+                if ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max) atomicOr(err, DERR_L2);
+                vals[k][d] = kv - (uint64_t)Fr.wrap_min;
+                a.v[d] = vals[k][d];
+            }
+            ++kidx;
+        } else {
+            if (r == 0) atomicOr(err, DERR_L2);
+            const uint64_t didx = grec - kidx;   // global diff index
+            const bool inb = didx >= Fr.diff_base && didx < Fr.diff_base + Fr.ndiff;
+            if (!inb) atomicOr(err, DERR_L2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint64_t sym = inb ? diff[didx * D + d] : 0;
+                if ((int64_t)sym < 0) atomicOr(err, DERR_L2);
+                const int64_t corr = (sym & 1u) ? -(int64_t)(sym >> 1) - 1 : (int64_t)(sym >> 1);
+                if (corr < Fr.min_corr || corr > Fr.max_corr) atomicOr(err, DERR_L2);
+                const uint64_t delta = corr > 0 ? R - (uint64_t)corr : (uint64_t)(-corr);
+                vals[k][d] = delta % R;
+                a.v[d] = addmod(a.v[d], vals[k][d], R);
+            }
+        }
+    }
+    // inclusive scan of thread aggregates over the workgroup (Hillis-Steele in LDS)
+    s_r[threadIdx.x] = a.reset;
+    for (int d = 0; d < D; ++d) s_v[threadIdx.x][d] = a.v[d];
+    __syncthreads();
+    for (uint32_t off = 1; off < CH_WG; off <<= 1) {
+        Agg left;
+        const bool has = threadIdx.x >= off;
+        if (has) {
+            left.reset = s_r[threadIdx.x - off];
+            for (int d = 0; d < D; ++d) left.v[d] = s_v[threadIdx.x - off][d];
+        }
+        __syncthreads();
+        if (has) {
+            Agg cur;
+            cur.reset = s_r[threadIdx.x];
+            for (int d = 0; d < D; ++d) cur.v[d] = s_v[threadIdx.x][d];
+            agg_combine<D>(left, cur, R);
+            s_r[threadIdx.x] = left.reset;
+            for (int d = 0; d < D; ++d) s_v[threadIdx.x][d] = left.v[d];
+        }
+        __syncthreads();
+    }
+    // look-back with multi-word aggregates: flag 1 = tile aggregate, 2 = inclusive
+    if (threadIdx.x == 0) {
+        Agg tot;
+        tot.reset = s_r[CH_WG - 1];
+        for (int d = 0; d < D; ++d) tot.v[d] = s_v[CH_WG - 1][d];
+        Agg incoming;
+        incoming.reset = 0;
+        for (int d = 0; d < D; ++d) incoming.v[d] = 0;
+        const uint32_t seg = rtile_begin[f];
+        if (tile != seg) {
+            for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + d] = tot.v[d];
+            aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + CH_MAXD - 1] |= 0;   // keep layout
+            __hip_atomic_store(&flags[tile], 1u | (tot.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            Agg acc;
+            acc.reset = 0;
+            for (int d = 0; d < D; ++d) acc.v[d] = 0;
+            int64_t t = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            while (t >= (int64_t)seg) {
+                const uint32_t fl = __hip_atomic_load(&flags[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if ((fl & 0xff) == 0) {
+                    if (++spins > LB_SPIN_LIMIT) { atomicOr(err, DERR_WAIT); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                Agg x;
+                x.reset = fl >> 8;
+                const int slot = (fl & 0xff) == 2 ? 1 : 0;
+                for (int d = 0; d < D; ++d) x.v[d] = aggv[((uint64_t)t * 2 + slot) * CH_MAXD + d];
+                agg_combine<D>(x, acc, R);   // x (+) acc
+                acc = x;
+                if ((fl & 0xff) == 2 || x.reset) break;
+                --t;
+            }
+            incoming = acc;
+        }
+        Agg inc = incoming;
+        agg_combine<D>(inc, tot, R);
+        for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 1) * CH_MAXD + d] = inc.v[d];
+        __hip_atomic_store(&flags[tile], 2u | (inc.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        s_in = incoming;
+    }
+    __syncthreads();
+    // exclusive prefix of this thread = incoming (+) inclusive of threads before
+    Agg run = s_in;
+    if (threadIdx.x > 0) {
+        Agg before;
+        before.reset = s_r[threadIdx.x - 1];
+        for (int d = 0; d < D; ++d) before.v[d] = s_v[threadIdx.x - 1][d];
+        agg_combine<D>(run, before, R);
+    }
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t r = r0 + k;
+        if (r >= Fr.nrec) break;
+        const bool rs = (resets >> k) & 1u;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            run.v[d] = rs ? vals[k][d] : addmod(run.v[d], vals[k][d], R);
+            compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + run.v[d];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Chain resolution: the explicit record of every point. Tiles are taken from
+// the end of the batch; a point whose successor lies in a later tile reads
+// that tile's result once its done flag is set (the later tile holds an
+// earlier ticket, so it is running or finished: the wait ends).
+
+#define CH_LOCAL 0x80000000u
+template <int D>
+__global__ __launch_bounds__(CH_WG) void k_dec_chain(const DecFrame* __restrict__ fr, uint32_t F,
+                                                     const uint32_t* __restrict__ tile_begin, uint32_t ntiles,
+                                                     const uint32_t* __restrict__ idrel,
+                                                     const uint32_t* __restrict__ tile_expl_off, uint32_t* ticket,
+                                                     uint32_t* done, uint32_t* __restrict__ root, uint32_t* err) {
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint32_t s_val[CH_TILE];
+    __shared__ uint32_t s_ext[CH_TILE];
+    if (threadIdx.x == 0) s_tile = ntiles - 1 - atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t f = upper_find(tile_begin, F, tile);
+    const DecFrame& Fr = fr[f];
+    const uint64_t p0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+    const uint64_t p1 = p0 + CH_TILE < Fr.n ? p0 + CH_TILE : Fr.n;
+    const uint32_t cnt = (uint32_t)(p1 - p0);
+    // explicit ranks (blocked: thread t owns points t*16 .. t*16+15)
+    uint32_t z = 0;
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint32_t j = threadIdx.x * CH_ITEMS + k;
+        z += (j < cnt && idrel[Fr.pt_off + p0 + j] == 0) ? 1u : 0u;
+    }
+    uint32_t ztot;
+    uint32_t rank = tile_expl_off[tile] + block_excl_scan(z, s_tmp, &ztot);
+    // s_val: LOCAL | j (pointer inside the tile) or a final explicit index;
+    // s_ext: global successor index for pointers leaving the tile (else ~0)
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint32_t j = threadIdx.x * CH_ITEMS + k;
+        if (j >= cnt) break;
+        const uint32_t d = idrel[Fr.pt_off + p0 + j];
+        s_ext[j] = 0xffffffffu;
+        if (d == 0) {
+            s_val[j] = rank++;
+        } else {
+            const uint64_t t = p0 + j + d;
+            if (t >= Fr.n) { atomicOr(err, DERR_CHAIN); s_val[j] = 0; continue; }
+            if (t < p1) s_val[j] = CH_LOCAL | (uint32_t)(t - p0);
+            else { s_val[j] = CH_LOCAL | j; s_ext[j] = (uint32_t)(Fr.pt_off + t); }
+        }
+    }
+    __syncthreads();
+    // pointer jumping inside the tile (12 synchronous rounds cover 4096 links);
+    // an entry pointing to itself with s_ext set is an exit of the tile
+    for (int round = 0; round < 12; ++round) {
+        uint32_t nv[CH_ITEMS], ne[CH_ITEMS];
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const uint32_t j = threadIdx.x * CH_ITEMS + k;
+            nv[k] = 0; ne[k] = 0xffffffffu;
+            if (j >= cnt) continue;
+            nv[k] = s_val[j];
+            ne[k] = s_ext[j];
+            if ((nv[k] & CH_LOCAL) && ne[k] == 0xffffffffu) {
+                const uint32_t tj = nv[k] & ~CH_LOCAL;
+                nv[k] = s_val[tj];
+                ne[k] = s_ext[tj];
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const uint32_t j = threadIdx.x * CH_ITEMS + k;
+            if (j < cnt) { s_val[j] = nv[k]; s_ext[j] = ne[k]; }
+        }
+        __syncthreads();
+    }
+    // resolve exits from the later tiles
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint32_t j = threadIdx.x * CH_ITEMS + k;
+        if (j >= cnt) break;
+        uint32_t v = s_val[j];
+        const uint32_t g2 = s_ext[j];
+        if (g2 != 0xffffffffu) {
+            const uint64_t local2 = g2 - Fr.pt_off;
+            const uint32_t t2 = tile_begin[f] + (uint32_t)(local2 / CH_TILE);
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&done[t2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                if (++spins > LB_SPIN_LIMIT) { atomicOr(err, DERR_WAIT); break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            v = __hip_atomic_load(&root[g2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (v & CH_LOCAL) {
+            atomicOr(err, DERR_CHAIN);
+            v = 0;
+        }
+        root[Fr.pt_off + p0 + j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&done[tile], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// every frame: explicit points == records
+__global__ void k_dec_check_counts(const DecFrame* __restrict__ fr, uint32_t F, const uint32_t* __restrict__ tb,
+                                   const uint32_t* __restrict__ tile_expl_off, const uint64_t* __restrict__ expl_total,
+                                   uint32_t* err) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const uint64_t a = tile_expl_off[tb[f]];
+    const uint64_t b = f + 1 < F ? tile_expl_off[tb[f + 1]] : *expl_total;
+    if (b - a != fr[f].nrec || a != fr[f].rec_base) atomicOr(err, DERR_COUNT);
+}
+
+// q = block * p + offset; x = fma(q + 0.5, 2 eb, min) (serial.cpp:1294-1311)
+template <int D, typename OutT, bool SOA>
+__global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restrict__ fr, uint32_t F,
+                                                       const uint32_t* __restrict__ frame_pt_begin, uint64_t T,
+                                                       const uint32_t* __restrict__ root,
+                                                       const uint64_t* __restrict__ compact,
+                                                       const uint32_t* __restrict__ rel, OutT* __restrict__ out,
+                                                       uint32_t* err) {
+    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < T; g += (uint64_t)gridDim.x * CH_WG) {
+        const uint32_t f = upper_find(frame_pt_begin, F, (uint32_t)g);
+        const DecFrame& Fr = fr[f];
+        const uint64_t i = g - Fr.pt_off;
+        const uint64_t rec = root[g];
+        if (rec - Fr.rec_base >= Fr.nrec) { atomicOr(err, DERR_CHAIN); continue; }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const uint64_t off = rel[g * D + d];
+            const uint64_t block = compact[rec * D + d];
+            if (off >= Fr.p || block > ((uint64_t)INT64_MAX - off) / Fr.p) atomicOr(err, DERR_QUANT);
+            const int64_t q = (int64_t)(block * Fr.p + off);
+            const double x = fma((double)q + 0.5, 2.0 * Fr.eb, Fr.mins[d]);
+            if (SOA) out[Fr.out_off + (uint64_t)d * Fr.n + i] = (OutT)x;
+            else out[Fr.out_off + i * D + d] = (OutT)x;
+        }
+    }
+}
+
+// exact corrections: (u64 index, f64 value) records, ascending, finite
+template <typename OutT, bool SOA>
+__global__ __launch_bounds__(CH_WG) void k_dec_corr(const uint8_t* __restrict__ in, const DecFrame* __restrict__ fr,
+                                                    uint32_t F, uint32_t D, OutT* __restrict__ out, uint32_t* err) {
+    const uint32_t f = blockIdx.y;
+    const DecFrame& Fr = fr[f];
+    for (uint64_t k = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; k < Fr.ncorr; k += (uint64_t)gridDim.x * CH_WG) {
+        const uint8_t* p = in + Fr.corr_off + 16 * k;
+        uint64_t idx = 0, bits = 0, prev = 0;
+        for (int b = 0; b < 8; ++b) idx |= (uint64_t)p[b] << (8 * b);
+        for (int b = 0; b < 8; ++b) bits |= (uint64_t)p[8 + b] << (8 * b);
+        if (k) for (int b = 0; b < 8; ++b) prev |= (uint64_t)p[b - 16] << (8 * b);
+        const double v = __longlong_as_double((long long)bits);
+        if (idx >= Fr.n * D || (k && idx <= prev) || !isfinite(v)) { atomicOr(err, DERR_CORR); continue; }
+        const uint64_t i = idx / D, d = idx % D;
+        if (SOA) out[Fr.out_off + d * Fr.n + i] = (OutT)v;
+        else out[Fr.out_off + idx] = (OutT)v;
+    }
+}
+
+}  // namespace chain

@@ -1,0 +1,786 @@
+// Entropy stage and LCH1 container assembly of the Chain compressor (gfx950).
+//
+// For each frame and each of the five components (idrel, diff, keep, choice2,
+// rel) the engine builds three candidate encodings and keeps the smallest,
+// ties in the order Tagged, Fixed, Raw-rANS (the reference's chooser order,
+// cpp/chain/lcps_cpp.cpp:1140-1174, with Zstd replaced):
+//   TAGR  per-record widths rANS-coded + MSB-first point-major values
+//         (the Tagged payload of :1005-1061)
+//   FIXP  per-dimension minimum and width + MSB-first payloads (:856-933)
+//   RANS  point-major values rANS-coded when every value is < 4096 (:935-995)
+// rANS sections (RVC1) use the RVN1 table of :560-684 and split the symbols
+// into chunks of lane-interleaved states, one wavefront per chunk.
+// Everything stays on the device: histograms, tables, encoding, sizes, the
+// layout of the output and the final bytes.
+#include "chain_common.hpp"
+#include "chain_rans.hpp"
+
+#pragma clang fp contract(off)
+
+namespace chain {
+
+enum : uint32_t { KIND_TAGR = 0, KIND_FIXP = 1, KIND_RANS = 2 };
+
+struct CompInst {
+    uint64_t base, n;          // first record in the component stream, records
+    uint64_t dmin[CH_MAXD], dmax[CH_MAXD];
+    uint64_t tag_bits;         // sum over records of dims * width
+    uint32_t max_tag, overflow;
+    uint32_t kind, dims;
+    uint32_t width[CH_MAXD];
+    uint64_t fix_bytes[CH_MAXD];
+    uint64_t size;             // component bytes
+    uint64_t off;              // absolute output offset of the component magic
+    uint64_t payload_off;      // FIXP payload / TAGR packed values
+};
+
+struct Section {               // RVC1 section of one component: s = 0 values, 1 widths
+    uint64_t count;
+    uint32_t eligible, pb, alphabet, lanes;
+    uint32_t table_len, nchunks, chunk_base, chunk_symbols;
+    uint64_t payload;          // bytes of all chunks (states + streams)
+    uint64_t size;             // bytes of the whole section
+    uint64_t off;              // absolute output offset of "RVC1"
+};
+
+#define RVC1_HDR_FIXED 30u     // "RVC1" u8 u64 u64 u8 u32 u32
+#define TABLE_BYTES_MAX 12304u // varuint(4096) + 4096 x 3 bytes
+#define CHUNK_STATE_BYTES 256u // 64 lanes x u32
+
+__device__ inline void put_u8(uint8_t* p, uint8_t v) { *p = v; }
+__device__ inline void put_u32(uint8_t* p, uint32_t v) {
+    for (int i = 0; i < 4; ++i) p[i] = (uint8_t)(v >> (8 * i));
+}
+__device__ inline void put_u64(uint8_t* p, uint64_t v) {
+    for (int i = 0; i < 8; ++i) p[i] = (uint8_t)(v >> (8 * i));
+}
+
+__device__ inline uint32_t comp_dims(int c, uint32_t D) { return (c == 0 || c == 3) ? 1u : D; }
+
+// segmented decoupled look-back: tiles [seg_start, tile) form the segment
+__device__ inline uint64_t lookback_seg(uint64_t* status, uint32_t tile, uint32_t seg_start, uint64_t aggregate,
+                                        uint32_t* err) {
+    const uint32_t lane = lane_id();
+    if (tile == seg_start) {
+        if (lane == 0) lb_store(&status[tile], LB_INC | aggregate);
+        return 0;
+    }
+    if (lane == 0) lb_store(&status[tile], LB_AGG | aggregate);
+    uint64_t excl = 0;
+    int64_t t = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (t >= (int64_t)seg_start) {
+        const int64_t mine = t - (int64_t)lane;
+        uint64_t s = mine >= (int64_t)seg_start ? lb_load(&status[mine]) : LB_INC;
+        const uint64_t not_ready = __ballot((s >> 62) == 0);
+        const uint64_t inc_mask = __ballot((s >> 62) == 2);
+        const uint64_t first_inc = inc_mask ? (inc_mask & (~inc_mask + 1)) : 0;
+        const uint64_t first_nr = not_ready ? (not_ready & (~not_ready + 1)) : 0;
+        if (first_nr && (!first_inc || first_nr < first_inc)) {
+            const uint64_t take = first_nr - 1;
+            uint64_t v = ((take >> lane) & 1ull) ? (s & LB_MASK) : 0ull;
+            for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, 64);
+            excl += v;
+            t -= __popcll(take);
+            if (++spins > LB_SPIN_LIMIT) { if (lane == 0) atomicOr(err, 1u); break; }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const uint64_t take = first_inc ? (first_inc | (first_inc - 1)) : ~0ull;
+        uint64_t v = ((take >> lane) & 1ull) ? (s & LB_MASK) : 0ull;
+        for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k, 64);
+        excl += v;
+        if (first_inc) break;
+        t -= 64;
+    }
+    if (lane == 0) lb_store(&status[tile], LB_INC | (excl + aggregate));
+    return excl;
+}
+
+// ------------------------------------------------------------------------
+// component instances
+
+__global__ void k_comp_init(const FrameDev* __restrict__ fr, const FrameOut* __restrict__ fo, uint32_t F, uint32_t D,
+                            CompInst* __restrict__ ci) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 5 * F) return;
+    const int c = i / F;
+    const uint32_t f = i % F;
+    CompInst x;
+    memset(&x, 0, sizeof x);
+    switch (c) {
+        case 0: x.base = fr[f].pt_off; x.n = fr[f].n; break;
+        case 1: x.base = fo[f].diff_base; x.n = fo[f].ndiff; break;
+        case 2: x.base = fo[f].keep_base; x.n = fo[f].nkeep; break;
+        case 3: x.base = fo[f].rec_base; x.n = fo[f].nrec; break;
+        default: x.base = fr[f].pt_off; x.n = fr[f].n; break;
+    }
+    x.dims = comp_dims(c, D);
+    for (int d = 0; d < CH_MAXD; ++d) { x.dmin[d] = ~0ull; x.dmax[d] = 0; }
+    ci[i] = x;
+}
+
+// per-tile statistics of one component type over all frames: per-dimension
+// range, value histogram (< 4096), width histogram and total Tagged bits.
+// Wavefront peer masks merge equal symbols before each LDS atomic.
+__device__ inline void hist_add(uint32_t* h, uint32_t sym, bool valid, int bits) {
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+        const uint64_t bb = __ballot((sym >> b) & 1u);
+        peers &= ((sym >> b) & 1u) ? bb : ~bb;
+    }
+    if (valid && __popcll(peers & lanemask_lt()) == 0) atomicAdd(&h[sym], (uint32_t)__popcll(peers));
+}
+
+template <typename ValT, int DIMS>
+__global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ vals, int c, uint32_t F,
+                                                      const uint32_t* __restrict__ tile_begin,
+                                                      CompInst* __restrict__ ci, uint32_t* __restrict__ vhist,
+                                                      uint32_t* __restrict__ thist) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t f = upper_find(tile_begin, F, tile);
+    CompInst& C = ci[c * F + f];
+    const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+    if (r0 >= C.n) return;
+    __shared__ uint32_t s_vh[RANS_VALUE_LIMIT], s_th[RANS_TAG_LIMIT];
+    __shared__ uint64_t s_red[4][2 * CH_MAXD + 2];
+    for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG) s_vh[k] = 0;
+    if (threadIdx.x < RANS_TAG_LIMIT) s_th[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t mn[DIMS], mx[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) { mn[d] = ~0ull; mx[d] = 0; }
+    uint64_t bits = 0;
+    uint32_t maxtag = 0, ovf = 0;
+    const ValT* src = vals + C.base * DIMS;
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint64_t r = r0 + (uint64_t)j * CH_WG + threadIdx.x;
+        const bool valid = r < C.n;
+        uint32_t tag = 1;
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) {
+            const uint64_t v = valid ? (uint64_t)src[r * DIMS + d] : 0;
+            if (valid) {
+                mn[d] = v < mn[d] ? v : mn[d];
+                mx[d] = v > mx[d] ? v : mx[d];
+                const uint32_t b = (uint32_t)bitlen64(v);
+                tag = b > tag ? b : tag;
+            }
+            const bool small = valid && v < RANS_VALUE_LIMIT;
+            ovf |= (valid && !small) ? 1u : 0u;
+            hist_add(s_vh, small ? (uint32_t)v : 0u, small, 12);
+        }
+        hist_add(s_th, valid ? tag : 0u, valid, 6);
+        if (valid) {
+            bits += (uint64_t)DIMS * tag;
+            maxtag = tag > maxtag ? tag : maxtag;
+        }
+    }
+    // reduce range / bits / maxtag / overflow over the workgroup
+    const uint32_t w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < DIMS; ++d) {
+        for (int k = 32; k >= 1; k >>= 1) {
+            uint64_t a = __shfl_xor(mn[d], k, 64), b = __shfl_xor(mx[d], k, 64);
+            mn[d] = a < mn[d] ? a : mn[d];
+            mx[d] = b > mx[d] ? b : mx[d];
+        }
+    }
+    for (int k = 32; k >= 1; k >>= 1) {
+        bits += __shfl_xor(bits, k, 64);
+        uint32_t a = __shfl_xor(maxtag, k, 64);
+        maxtag = a > maxtag ? a : maxtag;
+        ovf |= __shfl_xor(ovf, k, 64);
+    }
+    if (lane_id() == 0) {
+        for (int d = 0; d < DIMS; ++d) { s_red[w][2 * d] = mn[d]; s_red[w][2 * d + 1] = mx[d]; }
+        s_red[w][2 * CH_MAXD] = bits;
+        s_red[w][2 * CH_MAXD + 1] = ((uint64_t)maxtag << 32) | ovf;
+    }
+    __syncthreads();
+    const uint32_t ci_idx = c * F + f;
+    for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG)
+        if (s_vh[k]) atomicAdd(&vhist[(uint64_t)ci_idx * RANS_VALUE_LIMIT + k], s_vh[k]);
+    if (threadIdx.x < RANS_TAG_LIMIT && s_th[threadIdx.x])
+        atomicAdd(&thist[(uint64_t)ci_idx * RANS_TAG_LIMIT + threadIdx.x], s_th[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        uint64_t tb = 0;
+        uint32_t mt = 0, of = 0;
+        for (int k = 0; k < 4; ++k) {
+            tb += s_red[k][2 * CH_MAXD];
+            const uint32_t a = (uint32_t)(s_red[k][2 * CH_MAXD + 1] >> 32);
+            mt = a > mt ? a : mt;
+            of |= (uint32_t)s_red[k][2 * CH_MAXD + 1];
+        }
+        for (int d = 0; d < DIMS; ++d) {
+            uint64_t a = ~0ull, b = 0;
+            for (int k = 0; k < 4; ++k) {
+                a = s_red[k][2 * d] < a ? s_red[k][2 * d] : a;
+                b = s_red[k][2 * d + 1] > b ? s_red[k][2 * d + 1] : b;
+            }
+            atomicMin((unsigned long long*)&C.dmin[d], a);
+            atomicMax((unsigned long long*)&C.dmax[d], b);
+        }
+        atomicAdd((unsigned long long*)&C.tag_bits, tb);
+        atomicMax(&C.max_tag, mt);
+        if (of) atomicOr(&C.overflow, 1u);
+    }
+}
+
+// ------------------------------------------------------------------------
+// RVN1 table of every section, built on the device (cpp/chain/lcps_cpp.cpp:560-729)
+
+__global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict__ ci, Section* __restrict__ sec,
+                                                      const uint32_t* __restrict__ vhist,
+                                                      const uint32_t* __restrict__ thist, uint32_t F,
+                                                      uint32_t* __restrict__ prob_out, uint32_t* __restrict__ cum_out,
+                                                      uint8_t* __restrict__ tbytes, uint32_t* err) {
+    const uint32_t si = blockIdx.x;
+    const uint32_t cidx = si >> 1, s = si & 1;
+    const CompInst& C = ci[cidx];
+    const uint64_t count = s == 0 ? C.n * C.dims : C.n;
+    const bool eligible = s == 1 || C.overflow == 0;
+    __shared__ uint32_t s_prob[RANS_VALUE_LIMIT];
+    __shared__ uint64_t s_key[RANS_VALUE_LIMIT];
+    __shared__ uint32_t s_tmp[4];
+    __shared__ uint64_t s_red[4];
+    __shared__ uint32_t s_A, s_unique;
+    Section& S = sec[si];
+    if (!eligible || count == 0) {
+        if (threadIdx.x == 0) {
+            S.count = count; S.eligible = eligible; S.pb = 0; S.alphabet = 0; S.table_len = 0;
+        }
+        return;
+    }
+    const uint32_t HB = s == 0 ? RANS_VALUE_LIMIT : RANS_TAG_LIMIT;
+    const uint32_t* hist = s == 0 ? vhist + (uint64_t)cidx * RANS_VALUE_LIMIT : thist + (uint64_t)cidx * RANS_TAG_LIMIT;
+    if (threadIdx.x == 0) { s_A = 0; s_unique = 0; }
+    __syncthreads();
+    uint32_t amax = 0, uq = 0;
+    for (uint32_t b = threadIdx.x; b < HB; b += CH_WG)
+        if (hist[b]) { amax = b + 1 > amax ? b + 1 : amax; ++uq; }
+    atomicMax(&s_A, amax);
+    atomicAdd(&s_unique, uq);
+    __syncthreads();
+    const uint32_t A = s_A;
+    const uint32_t pb = (uint32_t)rvn1_precision_bits(s_unique);
+    const uint64_t M = 1ull << pb;
+    uint64_t sum = 0;
+    for (uint32_t b = threadIdx.x; b < A; b += CH_WG) {
+        const uint64_t f = hist[b];
+        uint32_t pr = rvn1_first_prob(f, pb, count);
+        if (pr == 0 && f != 0) pr = 1;
+        s_prob[b] = pr;
+        sum += pr;
+    }
+    for (int k = 32; k >= 1; k >>= 1) sum += __shfl_xor(sum, k, 64);
+    if (lane_id() == 0) s_red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    sum = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    if (sum != M) {
+        // std::stable_sort by probability == sort of (probability << 12 | index)
+        uint32_t n2 = 1;
+        while (n2 < A) n2 <<= 1;
+        for (uint32_t b = threadIdx.x; b < n2; b += CH_WG) s_key[b] = b < A ? (((uint64_t)s_prob[b] << 12) | b) : ~0ull;
+        __syncthreads();
+        for (uint32_t k = 2; k <= n2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < n2; i += CH_WG) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint64_t a = s_key[i], b = s_key[l];
+                        const bool up = (i & k) == 0;
+                        if ((a > b) == up) { s_key[i] = b; s_key[l] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        if (threadIdx.x == 0) {
+            if (sum < M) {
+                s_prob[s_key[A - 1] & 0xfffu] += (uint32_t)(M - sum);
+            } else {
+                int64_t error = (int64_t)(sum - M);
+                uint64_t cur = sum;
+                while (error > 0) {
+                    const uint64_t sum_at_start = cur;
+                    bool changed = false;
+                    for (uint32_t j = A; j-- > 1 && error > 0;) {
+                        uint32_t& pr = s_prob[s_key[j] & 0xfffu];
+                        if (pr <= 1) {
+                            if (j == A - 1) { atomicOr(err, 4u); error = 0; }
+                            break;
+                        }
+                        const uint64_t np = rvn1_reduced_prob((int)pb, sum_at_start, pr);
+                        int64_t red = (int64_t)pr - (int64_t)np;
+                        if (red == 0) red = 1;
+                        if (red >= (int64_t)pr) red = (int64_t)pr - 1;
+                        if (red > error) red = error;
+                        pr -= (uint32_t)red;
+                        cur -= (uint64_t)red;
+                        error -= red;
+                        changed = true;
+                    }
+                    if (!changed && error > 0) { atomicOr(err, 4u); break; }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // cumulative frequencies (exclusive scan, 16 entries per thread)
+    {
+        const uint32_t b0 = threadIdx.x * 16;
+        uint32_t local = 0;
+        for (uint32_t k = 0; k < 16; ++k) local += (b0 + k < A) ? s_prob[b0 + k] : 0u;
+        uint32_t tot;
+        uint32_t run = block_excl_scan(local, s_tmp, &tot);
+        for (uint32_t k = 0; k < 16 && b0 + k < A; ++k) {
+            prob_out[(uint64_t)si * RANS_VALUE_LIMIT + b0 + k] = s_prob[b0 + k];
+            cum_out[(uint64_t)si * RANS_VALUE_LIMIT + b0 + k] = run;
+            run += s_prob[b0 + k];
+        }
+        if (threadIdx.x == 0 && tot != M) atomicOr(err, 4u);
+    }
+    if (threadIdx.x == 0) {
+        // compact table bytes, cpp/chain/lcps_cpp.cpp:686-729
+        uint8_t* o = tbytes + (uint64_t)si * TABLE_BYTES_MAX;
+        uint32_t len = 0;
+        uint64_t v = A;
+        while (v >= 0x80u) { o[len++] = (uint8_t)((uint8_t)v | 0x80u); v >>= 7; }
+        o[len++] = (uint8_t)v;
+        for (uint32_t i = 0; i < A; ++i) {
+            const uint32_t pr = s_prob[i];
+            if (pr == 0) {
+                uint32_t extra = 0;
+                while (extra < 63 && i + extra + 1 < A && s_prob[i + extra + 1] == 0) ++extra;
+                o[len++] = (uint8_t)((extra << 2) | 3u);
+                i += extra;
+                continue;
+            }
+            uint32_t eb = 0;
+            if (pr >= (1u << 6)) { ++eb; if (pr >= (1u << 14)) ++eb; }
+            o[len++] = (uint8_t)((pr << 2) | eb);
+            for (uint32_t k = 0; k < eb; ++k) o[len++] = (uint8_t)(pr >> (8 * (k + 1) - 2));
+        }
+        S.count = count; S.eligible = 1; S.pb = pb; S.alphabet = A; S.table_len = len;
+    }
+}
+
+// chunk plan of every section (one thread): lanes, chunk count, chunk bases
+__global__ void k_rans_plan(Section* sec, uint32_t nsec, uint32_t F, uint32_t lanes_max, uint32_t chunk_symbols,
+                            uint32_t* comp_chunk /* [6]: begin of component c, [5] = total */) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t base = 0;
+    for (uint32_t si = 0; si < nsec; ++si) {
+        if (si % (2 * F) == 0) comp_chunk[si / (2 * F)] = base;
+        Section& S = sec[si];
+        S.chunk_symbols = chunk_symbols;
+        const bool used = S.eligible && S.count > 0;
+        S.lanes = used ? rvc1_lanes_for(S.count, lanes_max) : 0;
+        S.nchunks = used ? (uint32_t)((S.count + chunk_symbols - 1) / chunk_symbols) : 0;
+        S.chunk_base = base;
+        base += S.nchunks;
+    }
+    comp_chunk[5] = base;
+}
+
+__device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_t hi, uint32_t chunk) {
+    // largest si in [lo, hi) with chunk_base <= chunk
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sec[mid].chunk_base <= chunk) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// one wavefront per chunk; lanes beyond the section's lane count idle
+template <typename ValT, int DIMS>
+__global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ vals, int c, uint32_t F,
+                                                    const CompInst* __restrict__ ci, const Section* __restrict__ sec,
+                                                    const uint32_t* __restrict__ comp_chunk,
+                                                    const uint32_t* __restrict__ prob_tab,
+                                                    const uint32_t* __restrict__ cum_tab, uint8_t* __restrict__ scratch,
+                                                    uint64_t stride, uint32_t* __restrict__ chunk_len,
+                                                    uint32_t* __restrict__ chunk_pos, uint32_t* err) {
+    const uint32_t chunk = comp_chunk[c] + blockIdx.x;
+    if (chunk >= comp_chunk[c + 1]) return;
+    const uint32_t si = find_section(sec, 2 * F * c, 2 * F * (c + 1), chunk);
+    const Section& S = sec[si];
+    const CompInst& C = ci[si >> 1];
+    const bool tags = (si & 1) != 0;
+    __shared__ uint32_t s_prob[RANS_VALUE_LIMIT], s_cum[RANS_VALUE_LIMIT];
+    for (uint32_t b = threadIdx.x; b < S.alphabet; b += 64) {
+        s_prob[b] = prob_tab[(uint64_t)si * RANS_VALUE_LIMIT + b];
+        s_cum[b] = cum_tab[(uint64_t)si * RANS_VALUE_LIMIT + b];
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
+    const uint32_t M = 1u << pb, lower = 4u * M;
+    const uint64_t first = (uint64_t)(chunk - S.chunk_base) * S.chunk_symbols;
+    const uint64_t Sc = S.count - first < S.chunk_symbols ? S.count - first : S.chunk_symbols;
+    const uint64_t T = (Sc + L - 1) / L;
+    uint8_t* out = scratch + (uint64_t)chunk * stride;
+    uint8_t* stream = out + CHUNK_STATE_BYTES;
+    const uint64_t end = stride - CHUNK_STATE_BYTES;
+    uint64_t pos = end;
+    uint32_t x = lower;
+    const ValT* src = vals + C.base * DIMS;
+    const uint64_t lt = lanemask_lt();
+    for (uint64_t t = T; t-- > 0;) {
+        const uint64_t k = t * L + lane;
+        const bool active = lane < L && k < Sc;
+        uint32_t sym = 0;
+        if (active) {
+            const uint64_t q = first + k;
+            if (!tags) {
+                sym = (uint32_t)src[q];
+            } else {
+                uint32_t tg = 1;
+#pragma unroll
+                for (int d = 0; d < DIMS; ++d) {
+                    const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
+                    tg = b > tg ? b : tg;
+                }
+                sym = tg;
+            }
+        }
+        const uint32_t f = active ? s_prob[sym] : 1u;
+        const uint32_t cu = active ? s_cum[sym] : 0u;
+        const uint32_t xmax = 1024u * f;     // (lower >> pb) * 256 * f
+        uint32_t kb = 0, xx = x;
+        if (active)
+            while (xx >= xmax) { ++kb; xx >>= 8; }
+        const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
+        const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+        pos -= c0 + c1 + c2;
+        if (kb > 0) stream[pos + __popcll(m0 & lt)] = (uint8_t)(x >> (8 * (kb - 1)));
+        if (kb > 1) stream[pos + c0 + __popcll(m1 & lt)] = (uint8_t)(x >> (8 * (kb - 2)));
+        if (kb > 2) stream[pos + c0 + c1 + __popcll(m2 & lt)] = (uint8_t)(x >> (8 * (kb - 3)));
+        if (active) x = (xx / f) * M + (xx % f) + cu;
+    }
+    if (lane < L) put_u32(out + 4 * lane, x);
+    if (lane == 0) {
+        chunk_len[chunk] = (uint32_t)(end - pos);
+        chunk_pos[chunk] = (uint32_t)pos;
+    }
+}
+
+// ------------------------------------------------------------------------
+// candidate sizes and the choice, one thread per component instance
+
+__device__ inline uint64_t section_size(const Section& S, const uint32_t* chunk_len, uint32_t* chunk_off) {
+    uint64_t payload = 0;
+    for (uint32_t k = 0; k < S.nchunks; ++k) {
+        if (chunk_off) chunk_off[S.chunk_base + k] = (uint32_t)payload;
+        payload += 4ull * S.lanes + chunk_len[S.chunk_base + k];
+    }
+    return RVC1_HDR_FIXED + S.table_len + 4ull * S.nchunks + payload;
+}
+
+__global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, uint32_t ninst,
+                         const uint32_t* __restrict__ chunk_len, uint32_t* __restrict__ chunk_off) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    CompInst& C = ci[i];
+    Section& sv = sec[2 * i];
+    Section& st = sec[2 * i + 1];
+    st.size = section_size(st, chunk_len, chunk_off);
+    st.payload = st.size - RVC1_HDR_FIXED - st.table_len - 4ull * st.nchunks;
+    const uint64_t tag_size = 4 + 4 + 8 + 1 + 8 + st.size + 8 + (C.tag_bits + 7) / 8;
+    uint64_t fix_size = 4 + 4 + 8 + (uint64_t)C.dims * 17;
+    for (uint32_t d = 0; d < C.dims; ++d) {
+        uint32_t w = 1;
+        if (C.n && C.dmin[d] != C.dmax[d]) w = (uint32_t)bitlen64(C.dmax[d] - C.dmin[d]);
+        C.width[d] = w;
+        C.fix_bytes[d] = (C.n * w + 7) / 8;
+        fix_size += C.fix_bytes[d];
+        if (C.n == 0) C.dmin[d] = 0;
+    }
+    uint32_t kind = KIND_TAGR;
+    uint64_t best = tag_size;
+    if (fix_size < best) { best = fix_size; kind = KIND_FIXP; }
+    if (sv.eligible) {
+        sv.size = section_size(sv, chunk_len, chunk_off);
+        sv.payload = sv.size - RVC1_HDR_FIXED - sv.table_len - 4ull * sv.nchunks;
+        const uint64_t rans_size = 4 + 4 + 8 + 8 + sv.size;
+        if (rans_size < best) { best = rans_size; kind = KIND_RANS; }
+    }
+    C.kind = kind;
+    C.size = best;
+}
+
+// output layout: one thread walks the frames in order
+__global__ void k_layout(CompInst* __restrict__ ci, Section* __restrict__ sec, const FrameOut* __restrict__ fo,
+                         uint32_t F, uint32_t D, uint64_t* __restrict__ out_off, uint64_t* __restrict__ out_size,
+                         uint64_t* __restrict__ total) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint64_t off = 0;
+    for (uint32_t f = 0; f < F; ++f) {
+        out_off[f] = off;
+        uint64_t p = off + 48 + 8ull * D;
+        for (int c = 0; c < 5; ++c) {
+            CompInst& C = ci[c * F + f];
+            p += 8;
+            C.off = p;
+            const uint32_t i = c * F + f;
+            if (C.kind == KIND_TAGR) {
+                sec[2 * i + 1].off = p + 4 + 4 + 8 + 1 + 8;
+                C.payload_off = sec[2 * i + 1].off + sec[2 * i + 1].size + 8;
+            } else if (C.kind == KIND_FIXP) {
+                C.payload_off = p + 16 + 17ull * C.dims;
+            } else {
+                sec[2 * i].off = p + 16 + 8;
+            }
+            p += C.size;
+        }
+        p += 16 + 16 * fo[f].ncorr;
+        out_size[f] = p - off;
+        off = p;
+    }
+    *total = off;
+}
+
+// ------------------------------------------------------------------------
+// container bytes
+
+__device__ inline void write_section_header(uint8_t* out, const Section& S, const uint8_t* tbytes, uint32_t si,
+                                            const uint32_t* chunk_len, uint32_t tid, uint32_t nthreads) {
+    uint8_t* p = out + S.off;
+    const bool used = S.eligible && S.count > 0;
+    if (tid == 0) {
+        p[0] = 'R'; p[1] = 'V'; p[2] = 'C'; p[3] = '1';
+        put_u8(p + 4, (uint8_t)(used ? S.pb : 0));
+        put_u64(p + 5, S.count);
+        put_u64(p + 13, S.table_len);
+    }
+    const uint8_t* tb = tbytes + (uint64_t)si * TABLE_BYTES_MAX;
+    for (uint32_t k = tid; k < S.table_len; k += nthreads) p[21 + k] = tb[k];
+    uint8_t* q = p + 21 + S.table_len;
+    if (tid == 0) {
+        put_u8(q, (uint8_t)S.lanes);
+        put_u32(q + 1, used ? S.chunk_symbols : 0u);
+        put_u32(q + 5, S.nchunks);
+    }
+    for (uint32_t k = tid; k < S.nchunks; k += nthreads)
+        put_u32(q + 9 + 4 * k, 4u * S.lanes + chunk_len[S.chunk_base + k]);
+}
+
+__global__ __launch_bounds__(CH_WG) void k_write_components(const CompInst* __restrict__ ci,
+                                                            const Section* __restrict__ sec, uint32_t F,
+                                                            const uint8_t* __restrict__ tbytes,
+                                                            const uint32_t* __restrict__ chunk_len,
+                                                            uint8_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x;
+    const CompInst& C = ci[i];
+    const uint32_t tid = threadIdx.x;
+    uint8_t* p = out + C.off;
+    if (tid == 0) {
+        put_u64(p - 8, C.size);
+        const char* mg = C.kind == KIND_TAGR ? "TAGR" : (C.kind == KIND_FIXP ? "FIXP" : "RANS");
+        for (int k = 0; k < 4; ++k) p[k] = (uint8_t)mg[k];
+        put_u32(p + 4, C.dims);
+        put_u64(p + 8, C.n);
+        if (C.kind == KIND_TAGR) {
+            put_u8(p + 16, (uint8_t)C.max_tag);
+            put_u64(p + 17, sec[2 * i + 1].size);
+            put_u64(out + sec[2 * i + 1].off + sec[2 * i + 1].size, (C.tag_bits + 7) / 8);
+        } else if (C.kind == KIND_FIXP) {
+            for (uint32_t d = 0; d < C.dims; ++d) {
+                put_u64(p + 16 + 17 * d, C.dmin[d]);
+                put_u8(p + 16 + 17 * d + 8, (uint8_t)C.width[d]);
+                put_u64(p + 16 + 17 * d + 9, C.fix_bytes[d]);
+            }
+        } else {
+            put_u64(p + 16, sec[2 * i].size);
+        }
+    }
+    if (C.kind == KIND_TAGR) write_section_header(out, sec[2 * i + 1], tbytes, 2 * i + 1, chunk_len, tid, CH_WG);
+    if (C.kind == KIND_RANS) write_section_header(out, sec[2 * i], tbytes, 2 * i, chunk_len, tid, CH_WG);
+}
+
+// frame header and exact corrections (cpp/chain/lcps_cpp.cpp:2306-2329 layout, magic LCH1)
+__global__ __launch_bounds__(CH_WG) void k_write_frames(const FrameDev* __restrict__ fr,
+                                                        const FrameOut* __restrict__ fo, uint32_t D,
+                                                        const uint64_t* __restrict__ out_off,
+                                                        const uint64_t* __restrict__ out_size,
+                                                        const uint64_t* __restrict__ corr_idx,
+                                                        const double* __restrict__ corr_val,
+                                                        uint8_t* __restrict__ out) {
+    const uint32_t f = blockIdx.x;
+    const FrameDev& F = fr[f];
+    const FrameOut& O = fo[f];
+    uint8_t* p = out + out_off[f];
+    if (threadIdx.x == 0) {
+        p[0] = 'L'; p[1] = 'C'; p[2] = 'H'; p[3] = '1';
+        put_u32(p + 4, D);
+        put_u64(p + 8, F.n);
+        put_u64(p + 16, F.p);
+        put_u64(p + 24, (uint64_t)__double_as_longlong(F.eb));
+        put_u64(p + 32, (uint64_t)O.wrap_min);
+        put_u64(p + 40, (uint64_t)O.wrap_max);
+        for (uint32_t d = 0; d < D; ++d) put_u64(p + 48 + 8 * d, (uint64_t)__double_as_longlong(F.mins[d]));
+    }
+    uint8_t* q = p + out_size[f] - 16 - 16 * O.ncorr;
+    if (threadIdx.x == 0) {
+        put_u64(q, O.ncorr);
+        put_u64(q + 8, 16 * O.ncorr);
+    }
+    for (uint64_t k = threadIdx.x; k < O.ncorr; k += CH_WG) {
+        put_u64(q + 16 + 16 * k, corr_idx[O.corr_base + k]);
+        put_u64(q + 24 + 16 * k, (uint64_t)__double_as_longlong(corr_val[O.corr_base + k]));
+    }
+}
+
+// copy every chunk of a chosen section into place
+__global__ __launch_bounds__(CH_WG) void k_copy_chunks(const CompInst* __restrict__ ci, const Section* __restrict__ sec,
+                                                       uint32_t nsec, const uint32_t* __restrict__ comp_chunk,
+                                                       const uint8_t* __restrict__ scratch, uint64_t stride,
+                                                       const uint32_t* __restrict__ chunk_len,
+                                                       const uint32_t* __restrict__ chunk_pos,
+                                                       const uint32_t* __restrict__ chunk_off,
+                                                       uint8_t* __restrict__ out) {
+    const uint32_t chunk = blockIdx.x;
+    if (chunk >= comp_chunk[5]) return;
+    const uint32_t si = find_section(sec, 0, nsec, chunk);
+    const Section& S = sec[si];
+    const CompInst& C = ci[si >> 1];
+    const bool tags = (si & 1) != 0;
+    const bool used = tags ? C.kind == KIND_TAGR : C.kind == KIND_RANS;
+    if (!used) return;
+    uint8_t* dst = out + S.off + RVC1_HDR_FIXED + S.table_len + 4ull * S.nchunks + chunk_off[chunk];
+    const uint8_t* src = scratch + (uint64_t)chunk * stride;
+    const uint32_t nstate = 4u * S.lanes;
+    for (uint32_t k = threadIdx.x; k < nstate; k += CH_WG) dst[k] = src[k];
+    const uint8_t* st = src + CHUNK_STATE_BYTES + chunk_pos[chunk];
+    const uint32_t len = chunk_len[chunk];
+    for (uint32_t k = threadIdx.x; k < len; k += CH_WG) dst[nstate + k] = st[k];
+}
+
+// Fixed payloads: one thread per output byte, MSB-first, byte-padded per dimension
+template <typename ValT, int DIMS>
+__global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ vals, int c, uint32_t F,
+                                                      const uint32_t* __restrict__ tile_begin,
+                                                      const CompInst* __restrict__ ci, uint8_t* __restrict__ out) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t f = upper_find(tile_begin, F, tile);
+    const CompInst& C = ci[c * F + f];
+    if (C.kind != KIND_FIXP) return;
+    const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+    if (r0 >= C.n) return;
+    const uint64_t r1 = r0 + CH_TILE < C.n ? r0 + CH_TILE : C.n;
+    const ValT* src = vals + C.base * DIMS;
+    uint64_t doff = C.payload_off;
+    for (int d = 0; d < DIMS; ++d) {
+        const uint32_t w = C.width[d];
+        const uint64_t mn = C.dmin[d];
+        const uint64_t kb0 = (r0 * w + 7) / 8, kb1 = (r1 * w + 7) / 8;
+        for (uint64_t k = kb0 + threadIdx.x; k < kb1; k += CH_WG) {
+            uint64_t r = (8 * k) / w;
+            uint32_t off = (uint32_t)(8 * k - r * w);
+            uint64_t v = r < C.n ? (uint64_t)src[r * DIMS + d] - mn : 0;
+            uint32_t byte = 0;
+            for (int i = 0; i < 8; ++i) {
+                if (off == w) { ++r; off = 0; v = r < C.n ? (uint64_t)src[r * DIMS + d] - mn : 0; }
+                const uint32_t bit = r < C.n ? (uint32_t)((v >> (w - 1 - off)) & 1u) : 0u;
+                byte = (byte << 1) | bit;
+                ++off;
+            }
+            out[doff + k] = (uint8_t)byte;
+        }
+        doff += C.fix_bytes[d];
+    }
+}
+
+// Tagged payloads: 1024 records per tile, bit offsets by a segmented look-back
+// over each frame's tiles, bits assembled in LDS; the partial bytes at the
+// two ends of a tile go to side arrays and are merged by k_tag_seams.
+#define TAG_TILE 1024
+template <typename ValT, int DIMS>
+__global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ vals, int c, uint32_t F,
+                                                       const uint32_t* __restrict__ tile_begin,
+                                                       const CompInst* __restrict__ ci, uint64_t* status,
+                                                       uint32_t* ticket, uint32_t* err,
+                                                       uint8_t* __restrict__ head, uint8_t* __restrict__ tail,
+                                                       uint64_t* __restrict__ head_pos, uint8_t* __restrict__ out) {
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_words[(TAG_TILE * 4 * 63) / 32 + 4];
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t f = upper_find(tile_begin, F, tile);
+    const CompInst& C = ci[c * F + f];
+    if (threadIdx.x == 0) head_pos[tile] = ~0ull;
+    if (C.kind != KIND_TAGR) return;
+    const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * TAG_TILE;
+    if (r0 >= C.n) return;
+    const ValT* src = vals + C.base * DIMS;
+    const uint64_t rt = r0 + threadIdx.x * 4;
+    uint32_t tg[4];
+    uint32_t bits = 0;
+    for (int k = 0; k < 4; ++k) {
+        tg[k] = 0;
+        if (rt + k < C.n) {
+            uint32_t t = 1;
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) {
+                const uint32_t b = (uint32_t)bitlen64((uint64_t)src[(rt + k) * DIMS + d]);
+                t = b > t ? b : t;
+            }
+            tg[k] = t;
+            bits += DIMS * t;
+        }
+    }
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(bits, s_tmp, &tot);
+    if (threadIdx.x < 64) {
+        const uint64_t e = lookback_seg(status, tile, tile_begin[f], tot, err);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    const uint32_t nwords = (tot + 8 + 31) / 32 + 1;
+    for (uint32_t k = threadIdx.x; k < nwords; k += CH_WG) s_words[k] = 0;
+    __syncthreads();
+    const uint64_t b0 = s_excl, b1 = b0 + tot;
+    const uint32_t lb0 = (uint32_t)(b0 & 7);
+    uint32_t o = lb0 + pre;
+    for (int k = 0; k < 4; ++k) {
+        if (!tg[k]) continue;
+        const uint32_t w = tg[k];
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) {
+            const uint64_t v = (uint64_t)src[(rt + k) * DIMS + d];
+            const uint32_t fb = o >> 3, lb = (o + w - 1) >> 3;
+            for (uint32_t B = fb; B <= lb; ++B) {
+                const uint32_t sb = o > 8 * B ? o : 8 * B;
+                const uint32_t se = (o + w) < (8 * B + 8) ? (o + w) : (8 * B + 8);
+                const uint32_t chunkv = (uint32_t)((v >> (o + w - se)) & ((1u << (se - sb)) - 1u));
+                const uint32_t byte = chunkv << (8 * B + 8 - se);
+                atomicOr(&s_words[B >> 2], byte << (8 * (B & 3)));
+            }
+            o += w;
+        }
+    }
+    __syncthreads();
+    const uint64_t gb0 = b0 >> 3;
+    const uint32_t nbytes = (uint32_t)(((b1 + 7) >> 3) - gb0);
+    const bool last = r0 + TAG_TILE >= C.n;
+    for (uint32_t Lb = threadIdx.x; Lb < nbytes; Lb += CH_WG) {
+        const uint8_t byte = (uint8_t)(s_words[Lb >> 2] >> (8 * (Lb & 3)));
+        if (Lb == 0 && lb0 != 0) {
+            head[tile] = byte;
+            head_pos[tile] = C.payload_off + gb0;
+        } else if (Lb == nbytes - 1 && (b1 & 7) != 0 && !last) {
+            tail[tile] = byte;
+        } else {
+            out[C.payload_off + gb0 + Lb] = byte;
+        }
+    }
+}
+
+__global__ void k_tag_seams(const uint8_t* __restrict__ head, const uint8_t* __restrict__ tail,
+                            const uint64_t* __restrict__ head_pos, uint32_t ntiles, uint8_t* __restrict__ out) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles || head_pos[t] == ~0ull) return;
+    out[head_pos[t]] = head[t] | tail[t - 1];
+}
+
+}  // namespace chain

@@ -293,7 +293,8 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
                                                     const uint32_t* __restrict__ tile_begin, uint32_t nframes,
                                                     KeyT* __restrict__ key_out, RelT* __restrict__ rel_out,
                                                     uint64_t* __restrict__ corr_idx, double* __restrict__ corr_val,
-                                                    uint32_t* __restrict__ tile_ncorr) {
+                                                    uint32_t* __restrict__ tile_ncorr, uint32_t nwords,
+                                                    uint64_t total_pts) {
     const uint32_t tile = blockIdx.x;
     const uint32_t f = upper_find(tile_begin, nframes, tile);
     const FrameDev& F = fr[f];
@@ -303,9 +304,9 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
     const uint64_t p = F.p, magic = F.p_magic;
     const uint32_t l = F.p_l;
     double mins[D];
-    uint32_t shift[D];
+    uint32_t shift[D], word[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) { mins[d] = F.mins[d]; shift[d] = F.key_shift[d]; }
+    for (int d = 0; d < D; ++d) { mins[d] = F.mins[d]; shift[d] = F.key_shift[d]; word[d] = F.key_word[d]; }
     __shared__ uint32_t s_tmp[4];
     uint32_t run = 0;
     const uint64_t cbase = (uint64_t)tile * CH_TILE * D;
@@ -316,6 +317,9 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
         double xs[D];
         if (valid) {
             KeyT key = 0;
+            KeyT kw[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) kw[d] = 0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const double x = load_coord<T, D, SOA>(pts, F, i, d);
@@ -326,9 +330,16 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
                 if (!within_eb(restored, x, eb)) cflags |= 1u << d;
                 const uint64_t b = div_p(q, p, magic, l);
                 rel_out[(F.pt_off + i) * D + d] = (RelT)(q - b * p);
-                key |= (KeyT)b << shift[d];
+                if (nwords == 1) key |= (KeyT)b << shift[d];
+                else kw[word[d]] |= (KeyT)b << shift[d];
             }
-            key_out[F.pt_off + i] = key;
+            if (nwords == 1) {
+                key_out[F.pt_off + i] = key;
+            } else {
+#pragma unroll
+                for (int w = 0; w < D; ++w)
+                    if ((uint32_t)w < nwords) key_out[(uint64_t)w * total_pts + F.pt_off + i] = kw[w];
+            }
         }
         const uint32_t cnt = __popc(cflags);
         if (__syncthreads_or(cnt != 0)) {
@@ -503,6 +514,62 @@ __global__ __launch_bounds__(CH_WG) void k_link(const KeyT* __restrict__ skey, c
     }
 }
 
+// keys wider than 64 bits: the sort runs word by word (least significant word
+// first); before the passes over word w its values are gathered in the order
+// the previous words left behind
+__global__ void k_gather_word(const uint64_t* __restrict__ word, const uint32_t* __restrict__ order, uint64_t T,
+                              uint64_t* __restrict__ out) {
+    for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < T; s += (uint64_t)gridDim.x * blockDim.x)
+        out[s] = word[order[s]];
+}
+
+// successor + decision for multi-word keys: equal tuples need every word equal
+template <int D>
+__global__ __launch_bounds__(CH_WG) void k_link_wide(const uint64_t* __restrict__ words,
+                                                     const uint32_t* __restrict__ sval, uint64_t T, uint32_t nwords,
+                                                     const FrameDev* __restrict__ fr,
+                                                     const uint32_t* __restrict__ frame_pt_begin, uint32_t nframes,
+                                                     uint32_t* __restrict__ idrel) {
+    __shared__ uint32_t s_off[CH_MAXF_LDS + 1];
+    const uint32_t* offs = frame_pt_begin;
+    if (nframes <= CH_MAXF_LDS) {
+        for (uint32_t k = threadIdx.x; k <= nframes; k += CH_WG) s_off[k] = frame_pt_begin[k];
+        __syncthreads();
+        offs = s_off;
+    }
+    for (uint64_t s = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; s < T; s += (uint64_t)gridDim.x * CH_WG) {
+        const uint32_t i = sval[s];
+        uint32_t out = 0;
+        if (s + 1 < T) {
+            const uint32_t j = sval[s + 1];
+            bool same = true;
+            for (uint32_t w = 0; w < nwords && same; ++w) same = words[(uint64_t)w * T + i] == words[(uint64_t)w * T + j];
+            const uint32_t f = upper_find(offs, nframes, i);
+            const FrameDev& F = fr[f];
+            if (same && (F.flags & FL_L1) && j < offs[f + 1]) {
+                const uint64_t dist = (uint64_t)(j - i);
+                uint64_t mb = 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const uint32_t wd = F.key_width[d];
+                    const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
+                    const uint64_t b = (words[(uint64_t)F.key_word[d] * T + i] >> F.key_shift[d]) & mask;
+                    mb = b > mb ? b : mb;
+                }
+                const int64_t local_bits = bitlen64(mb);
+                const int64_t tag_bits = bitlen64((uint64_t)local_bits);
+                const int64_t a = local_bits * D + tag_bits;
+                const int64_t c = (int64_t)D * (int64_t)F.max_bid_bits;
+                const int64_t smaller = a < c ? a : c;
+                const int64_t bl = bitlen64(dist);
+                const int64_t lhs = bl + bitlen64((uint64_t)bl);
+                if (lhs < smaller - F.threshold) out = (uint32_t)dist;
+            }
+        }
+        idrel[i] = out;
+    }
+}
+
 // ------------------------------------------------------------------------
 // K5: ordered compaction of the explicit points' block ids (index order),
 // per-frame point tiles, one decoupled look-back over all tiles.
@@ -513,7 +580,8 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ tile_begin, uint32_t nframes,
                                                     uint32_t ntiles, uint64_t* status, uint32_t* ticket,
                                                     uint32_t* err, uint64_t* __restrict__ blocks,
-                                                    FrameOut* __restrict__ fo, uint64_t* __restrict__ rec_total) {
+                                                    FrameOut* __restrict__ fo, uint64_t* __restrict__ rec_total,
+                                                    uint32_t nwords, uint64_t T) {
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
@@ -543,11 +611,11 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
         if (flags & (1u << k)) {
-            const uint64_t key = (uint64_t)keys[F.pt_off + start + k];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const uint32_t wd = F.key_width[d];
                 const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
+                const uint64_t key = (uint64_t)keys[(uint64_t)(nwords == 1 ? 0 : F.key_word[d]) * T + F.pt_off + start + k];
                 blocks[rec * D + d] = (key >> F.key_shift[d]) & mask;
             }
             ++rec;

@@ -49,8 +49,9 @@ class _Stats(C.Structure):
 
 
 EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "chain_engine_stream",
-           "chain_buf_free", "chain_compress", "chain_compress_batch", "chain_decompress",
-           "chain_decompress_batch", "chain_last_stage_times"]
+           "chain_buf_free", "chain_compress", "chain_compress_batch", "chain_compress_batch_device",
+           "chain_engine_set_rans", "chain_decompress", "chain_decompress_batch", "chain_decompress_batch_device",
+           "chain_device_malloc", "chain_device_free", "chain_copy", "chain_last_stage_times"]
 
 _LIB = None
 
@@ -75,10 +76,19 @@ def load_library():
                                  C.c_int64, C.c_int64, C.c_uint32, C.c_int, P(_Buf), P(_Stats)]
     L.chain_compress_batch.argtypes = [C.c_void_p, P(_Frame), C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_int,
                                        C.c_int, P(_Buf), P(_Stats)]
+    L.chain_compress_batch_device.argtypes = [C.c_void_p, P(_Frame), C.c_uint32, C.c_int, C.c_int, C.c_uint32,
+                                              P(C.c_void_p), P(C.c_uint64), P(C.c_uint64), P(_Stats)]
+    L.chain_engine_set_rans.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
     L.chain_decompress.argtypes = [C.c_void_p, P(C.c_uint8), C.c_size_t, C.c_int, C.c_int, P(_Buf),
                                    P(C.c_uint64), P(C.c_uint32)]
     L.chain_decompress_batch.argtypes = [C.c_void_p, P(P(C.c_uint8)), P(C.c_size_t), C.c_uint32, C.c_int, C.c_int,
                                          P(_Buf), P(C.c_uint64), P(C.c_uint32)]
+    L.chain_decompress_batch_device.argtypes = [C.c_void_p, P(P(C.c_uint8)), P(C.c_size_t), C.c_void_p, C.c_uint32,
+                                                C.c_int, C.c_int, P(C.c_void_p), P(C.c_uint64), P(C.c_uint64),
+                                                P(C.c_uint32)]
+    L.chain_device_malloc.argtypes = [C.c_void_p, C.c_uint64, P(C.c_void_p)]
+    L.chain_device_free.argtypes = [C.c_void_p, C.c_void_p]
+    L.chain_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     L.chain_last_stage_times.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_char_p, C.c_size_t]
     _LIB = L
     return L
@@ -131,6 +141,30 @@ class Engine:
     @property
     def stream(self) -> int:
         return int(self._L.chain_engine_stream(self._h) or 0)
+
+    def set_rans(self, lanes_max=64, chunk_symbols=65536):
+        """RVC1 lane count and chunk size of the containers this engine writes."""
+        self._check(self._L.chain_engine_set_rans(self._h, lanes_max, chunk_symbols))
+
+    def compress_batch_device(self, ptrs, ns, dtype, dims, eb, p, threshold, flags=0, layout="aos"):
+        """Device-resident LCH1 batch: returns (device pointer, offsets, sizes, stats)."""
+        F = len(ptrs)
+        ebs = eb if isinstance(eb, (list, tuple, np.ndarray)) else [eb] * F
+        ps = p if isinstance(p, (list, tuple, np.ndarray)) else [p] * F
+        ts = threshold if isinstance(threshold, (list, tuple, np.ndarray)) else [threshold] * F
+        fl = flags if isinstance(flags, (list, tuple, np.ndarray)) else [flags] * F
+        fr = (_Frame * F)()
+        for k in range(F):
+            fr[k] = _Frame(ptrs[k], int(ns[k]), float(ebs[k]), int(ps[k]), int(ts[k]), int(fl[k]))
+        dptr = C.c_void_p()
+        offs = (C.c_uint64 * F)()
+        sizes = (C.c_uint64 * F)()
+        sts = (_Stats * F)()
+        rc = self._L.chain_compress_batch_device(self._h, fr, F, _dtype_code(dtype),
+                                                 CHAIN_AOS if layout == "aos" else CHAIN_SOA, dims, C.byref(dptr),
+                                                 offs, sizes, sts)
+        self._check(rc)
+        return int(dptr.value or 0), list(offs), list(sizes), [sts[k] for k in range(F)]
 
     def compress(self, points, eb, p, threshold, flags=0, fmt="lch1", layout="aos"):
         """Compress one frame (host array, (N, D) for AoS or (D, N) for SoA)."""
@@ -213,9 +247,44 @@ class Engine:
             res.append(arr.reshape((ns[j], ds[j]) if layout == "aos" else (ds[j], ns[j])))
         return res
 
+    def decompress_batch_device(self, streams, d_streams=None, dtype=np.float64, layout="aos"):
+        """Decode streams (host copies plan the work; d_streams is a device
+        pointer to the same bytes concatenated). Returns (device pointer,
+        element offsets, ns, dims); the output stays on the device."""
+        k = len(streams)
+        bufs = [(C.c_uint8 * max(1, len(s))).from_buffer_copy(s if s else b"\0") for s in streams]
+        ptrs = (C.POINTER(C.c_uint8) * k)(*[C.cast(b, C.POINTER(C.c_uint8)) for b in bufs])
+        lens = (C.c_size_t * k)(*[len(s) for s in streams])
+        dout = C.c_void_p()
+        offs = (C.c_uint64 * k)()
+        ns = (C.c_uint64 * k)()
+        ds = (C.c_uint32 * k)()
+        rc = self._L.chain_decompress_batch_device(self._h, ptrs, lens, C.c_void_p(d_streams or 0), k,
+                                                   _dtype_code(dtype), CHAIN_AOS if layout == "aos" else CHAIN_SOA,
+                                                   C.byref(dout), offs, ns, ds)
+        self._check(rc)
+        return int(dout.value or 0), list(offs), list(ns), int(ds[0]) if k else 0
+
+    def device_malloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(self._L.chain_device_malloc(self._h, nbytes, C.byref(p)))
+        return int(p.value)
+
+    def device_free(self, ptr: int):
+        self._check(self._L.chain_device_free(self._h, C.c_void_p(ptr)))
+
+    def to_device(self, ptr: int, data) -> None:
+        buf = np.ascontiguousarray(data)
+        self._check(self._L.chain_copy(self._h, C.c_void_p(ptr), buf.ctypes.data_as(C.c_void_p), buf.nbytes, 1))
+
+    def to_host(self, ptr: int, nbytes: int) -> bytes:
+        out = (C.c_uint8 * max(1, nbytes))()
+        self._check(self._L.chain_copy(self._h, out, C.c_void_p(ptr), nbytes, 0))
+        return bytes(out)[:nbytes]
+
     def stage_times(self):
         ms = (C.c_float * 64)()
         names = C.create_string_buffer(4096)
         k = self._L.chain_last_stage_times(self._h, ms, 64, names, 4096)
         labels = names.value.decode().split(";") if k else []
-        return {labels[i]: float(ms[i]) for i in range(k)}
+        return [(labels[i], float(ms[i])) for i in range(k)]

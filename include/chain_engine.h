@@ -89,6 +89,18 @@ int chain_compress(chain_engine* e, const void* points, int dtype, int layout, u
 int chain_compress_batch(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
                          uint32_t dims, int points_on_device, int format, chain_buf* outs, chain_stats* stats);
 
+/* Device-resident batch compression (LCH1): the frames' points are device
+ * pointers (contiguous frames are read in place), the containers are written
+ * into a device buffer owned by the engine and valid until its next call;
+ * offsets/sizes give each frame's container inside *d_out. */
+int chain_compress_batch_device(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
+                                uint32_t dims, void** d_out, uint64_t* offsets, uint64_t* sizes, chain_stats* stats);
+
+/* RVC1 parameters of the containers this engine writes: the largest lane
+ * count per chunk (power of two <= 64) and the symbols per chunk. Both are
+ * recorded in every section, so any decoder reads any setting. */
+int chain_engine_set_rans(chain_engine* e, uint32_t lanes_max, uint32_t chunk_symbols);
+
 /* Decompress one LCH1 or LCP3 stream to float64 (or float32) coordinates.
  * Replaces lcps_decompress_gpu (cu/decompress/lcps_gpu_decompress_ans.cu:2701-2921)
  * and, for the reference's formats, lcps_decompress_serial
@@ -99,6 +111,21 @@ int chain_decompress(chain_engine* e, const uint8_t* bytes, size_t len, int out_
 /* Batch decompression: one launch per stage over all streams. */
 int chain_decompress_batch(chain_engine* e, const uint8_t* const* bytes, const size_t* lens, uint32_t nstreams,
                            int out_dtype, int out_layout, chain_buf* outs, uint64_t* ns, uint32_t* dims);
+
+/* Device-resident batch decompression: bytes/lens are host copies of the
+ * streams (their headers plan the launch) and d_streams holds the same
+ * streams concatenated in device memory (NULL: upload the host copies). The
+ * coordinates stay in an engine-owned device buffer, valid until the next
+ * decompression call; out_offsets are element offsets of every frame. */
+int chain_decompress_batch_device(chain_engine* e, const uint8_t* const* bytes, const size_t* lens,
+                                  const void* d_streams, uint32_t nstreams, int out_dtype, int out_layout,
+                                  void** d_out, uint64_t* out_offsets, uint64_t* ns, uint32_t* dims);
+
+/* Device buffers for callers that keep frames in HBM without another
+ * runtime (synchronous copies on the engine's stream). */
+int chain_device_malloc(chain_engine* e, uint64_t bytes, void** out);
+int chain_device_free(chain_engine* e, void* p);
+int chain_copy(chain_engine* e, void* dst, const void* src, uint64_t bytes, int to_device);
 
 /* Per-stage device times (milliseconds, from HIP events on the engine's
  * stream) of the last call. names is a ';'-separated list. Returns the count. */

@@ -713,6 +713,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     hipLaunchKernelGGL(k_frame_split_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[1],
                        &B.totals[0]);
     tm.end();
+
     // corrections: per-tile counts -> offsets -> ordered gather
     tm.begin("corrections");
     scan_u32(st, B.tile_ncorr, B.tile_corr_off, ntiles, B.status, B.ticket, A.err, &B.totals[2]);

@@ -674,15 +674,19 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
         const uint64_t mn = C.dmin[d];
         const uint64_t kb0 = (r0 * w + 7) / 8, kb1 = (r1 * w + 7) / 8;
         for (uint64_t k = kb0 + threadIdx.x; k < kb1; k += CH_WG) {
+            // bits [8k, 8k+8): whole runs of the values they cover, MSB first
             uint64_t r = (8 * k) / w;
-            uint32_t off = (uint32_t)(8 * k - r * w);
-            uint64_t v = r < C.n ? (uint64_t)src[r * DIMS + d] - mn : 0;
-            uint32_t byte = 0;
-            for (int i = 0; i < 8; ++i) {
-                if (off == w) { ++r; off = 0; v = r < C.n ? (uint64_t)src[r * DIMS + d] - mn : 0; }
-                const uint32_t bit = r < C.n ? (uint32_t)((v >> (w - 1 - off)) & 1u) : 0u;
-                byte = (byte << 1) | bit;
-                ++off;
+            uint32_t off = (uint32_t)(8 * k - r * w);   // bits of value r before this byte
+            uint32_t byte = 0, have = 0;
+            while (have < 8) {
+                if (r >= C.n) { byte <<= (8 - have); break; }
+                const uint64_t v = (uint64_t)src[r * DIMS + d] - mn;
+                const uint32_t avail = w - off;
+                const uint32_t take = avail < 8 - have ? avail : 8 - have;
+                byte = (byte << take) | (uint32_t)((v >> (avail - take)) & ((1u << take) - 1u));
+                have += take;
+                off = 0;
+                ++r;
             }
             out[doff + k] = (uint8_t)byte;
         }

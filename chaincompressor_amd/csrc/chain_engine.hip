@@ -250,7 +250,7 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D
     });
     tm.end();
     tm.begin("rans_table");
-    hipLaunchKernelGGL(k_rans_table, dim3(nsec), dim3(CH_WG), 0, st, x.ci, x.sec, x.vhist, x.thist, F, x.prob, x.cum,
+    hipLaunchKernelGGL(k_rans_table, dim3(nsec), dim3(CH_WG), 0, st, x.ci, x.sec, x.vhist, x.thist, F, x.pc,
                        x.tbytes, err);
     hipLaunchKernelGGL(k_rans_plan, dim3(1), dim3(1), 0, st, x.sec, nsec, F, rc.lanes_max, rc.chunk_symbols,
                        x.comp_chunk);
@@ -261,14 +261,15 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D
         constexpr int DC = decltype(dd)::value;
         if (comp_chunk_bound[c])
             hipLaunchKernelGGL((k_rans_encode<V, DC>), dim3((unsigned)comp_chunk_bound[c]), dim3(64), 0, st,
-                               (const V*)v, c, F, x.ci, x.sec, x.comp_chunk, x.prob, x.cum, x.scratch, stride,
+                               (const V*)v, c, F, x.ci, x.sec, x.comp_chunk, x.pc, x.scratch, stride,
                                x.chunk_len, x.chunk_pos, err);
     });
     tm.end();
     tm.begin("assemble");
-    hipLaunchKernelGGL(k_choose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst, x.chunk_len,
-                       x.chunk_off);
-    hipLaunchKernelGGL(k_layout, dim3(1), dim3(1), 0, st, x.ci, x.sec, fo, F, D, x.out_off, x.out_size, x.total);
+    hipLaunchKernelGGL(k_section_sizes, dim3(nsec), dim3(CH_WG), 0, st, x.sec, x.chunk_len, x.chunk_off);
+    hipLaunchKernelGGL(k_choose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst);
+    hipLaunchKernelGGL(k_layout, dim3(1), dim3(1), 0, st, x.ci, x.sec, fo, F, D, x.out_off, x.out_size, x.total,
+                       x.tag_tiles);
     hipLaunchKernelGGL(k_write_frames, dim3(F), dim3(CH_WG), 0, st, fdev, fo, D, x.out_off, x.out_size, corr_idx,
                        corr_val, x.out);
     hipLaunchKernelGGL(k_write_components, dim3(ninst), dim3(CH_WG), 0, st, x.ci, x.sec, F, x.tbytes, x.chunk_len,
@@ -287,10 +288,10 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D
                            x.out);
         fill64(st, x.status1k, ntiles1k, 0);
         fill32(st, ticket, 1, 0);
-        hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(ntiles1k), dim3(CH_WG), 0, st, (const V*)v, c, F, x.tb1k,
-                           x.ci, x.status1k, ticket, err, x.head, x.tail, x.head_pos, x.out);
+        hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(ntiles1k), dim3(CH_WG), 0, st, (const V*)v, c, F,
+                           x.tag_tiles, x.ci, x.status1k, ticket, err, x.head, x.tail, x.head_pos, x.out);
         hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG)), dim3(CH_WG), 0, st, x.head, x.tail, x.head_pos,
-                           ntiles1k, x.out);
+                           x.tag_tiles + c * (F + 1) + F, x.out);
     });
     tm.end();
 }
@@ -445,8 +446,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             Section* sec;
             uint32_t* vhist;
             uint32_t* thist;
-            uint32_t* prob;
-            uint32_t* cum;
+            uint64_t* pc;
             uint8_t* tbytes;
             uint32_t* comp_chunk;
             uint8_t* scratch;
@@ -461,14 +461,14 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             uint8_t* tail;
             uint64_t* head_pos;
             uint64_t* status1k;
+            uint32_t* tag_tiles;
             uint8_t* out;
         } x;
         x.ci = c.take<CompInst>(5 * F);
         x.sec = c.take<Section>(10 * F);
         x.vhist = c.take<uint32_t>(5ull * F * RANS_VALUE_LIMIT);
         x.thist = c.take<uint32_t>(5ull * F * RANS_TAG_LIMIT);
-        x.prob = c.take<uint32_t>(10ull * F * RANS_VALUE_LIMIT);
-        x.cum = c.take<uint32_t>(10ull * F * RANS_VALUE_LIMIT);
+        x.pc = c.take<uint64_t>(10ull * F * RANS_VALUE_LIMIT);
         x.tbytes = c.take<uint8_t>(10ull * F * TABLE_BYTES_MAX);
         x.comp_chunk = c.take<uint32_t>(8);
         x.scratch = c.take<uint8_t>(chunk_bound * stride);
@@ -483,6 +483,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.tail = c.take<uint8_t>(ntiles1k);
         x.head_pos = c.take<uint64_t>(ntiles1k);
         x.status1k = c.take<uint64_t>(ntiles1k + 1);
+        x.tag_tiles = c.take<uint32_t>(5ull * (F + 1));
         x.out = c.take<uint8_t>(out_bound);
         return x;
     };
@@ -834,8 +835,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
 
 // little-endian reader over one stream inside the pooled input
 struct HostReader {
-    const uint8_t* base;   // pool
-    uint64_t pos, end;     // absolute offsets in the pool
+    const uint8_t* base;   // the stream's host bytes
+    uint64_t pos, end;     // offsets inside the stream
+    uint64_t dev;          // offset of the stream in the device input pool
     const uint8_t* bytes(uint64_t k) {
         require(k <= end - pos, CHAIN_ERUNTIME, "truncated LCPS bitstream");
         const uint8_t* q = base + pos;
@@ -859,7 +861,7 @@ struct HostReader {
     }
     HostReader sub(uint64_t len) {
         require(len <= end - pos, CHAIN_ERUNTIME, "truncated LCPS bitstream");
-        HostReader r{base, pos, pos + len};
+        HostReader r{base, pos, pos + len, dev};
         pos += len;
         return r;
     }
@@ -873,7 +875,7 @@ struct DecPlan {
     std::vector<DecFixed> fixed;
     std::vector<DecRaw> raws;
     std::vector<DecTagged> tagged;
-    std::vector<uint32_t> prob, cum;
+    std::vector<uint64_t> pc;   // (cum << 32) | freq
     std::vector<uint16_t> slots;
     uint64_t T = 0, R = 0, ND = 0, NK = 0, tags = 0, tag_tiles = 0, fixed_total = 0, raw_total = 0;
     uint32_t D = 0;
@@ -923,14 +925,13 @@ static void parse_rvc1(HostReader r, uint64_t expected, int comp, uint64_t dst, 
     sec.alphabet = (uint32_t)A;
     sec.chunk_symbols = S;
     sec.count = expected;
-    sec.table_off = P.prob.size();
+    sec.table_off = P.pc.size();
     sec.comp = comp;
     sec.dst = dst;
     uint32_t sum = 0;
     for (uint64_t k = 0; k < A; ++k) {
         require(prob[k] <= M - sum, CHAIN_ERUNTIME, "invalid rANS probability sum");
-        P.prob.push_back(prob[k]);
-        P.cum.push_back(sum);
+        P.pc.push_back(((uint64_t)sum << 32) | prob[k]);
         sum += prob[k];
     }
     require(sum == M, CHAIN_ERUNTIME, "incomplete rANS probability table");
@@ -951,7 +952,7 @@ static void parse_rvc1(HostReader r, uint64_t expected, int comp, uint64_t dst, 
         DecChunk ch;
         ch.sec = si;
         ch.bytes = cb;
-        ch.byte_off = r.pos;
+        ch.byte_off = r.dev + r.pos;
         ch.first = (uint64_t)k * S;
         (void)r.bytes(cb);
         P.chunks.push_back(ch);
@@ -972,7 +973,7 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
         if (n) {
             DecRaw j;
             std::memset(&j, 0, sizeof j);
-            j.byte_off = r.pos;
+            j.byte_off = r.dev + r.pos;
             j.count = n * dims;
             j.dst = dst;
             j.comp = comp;
@@ -1003,7 +1004,7 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
         }
         require(r.end - r.pos == total, CHAIN_ERUNTIME, "fixed raw size mismatch");
         for (uint32_t d = 0; d < dims; ++d) {
-            js[d].byte_off += r.pos;
+            js[d].byte_off += r.dev + r.pos;
             if (n) {
                 js[d].first_thread = P.fixed_total;
                 P.fixed_total += n;
@@ -1024,7 +1025,7 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
     parse_rvc1(r.sub(r.u64()), n, -1, tag_off, P);
     P.tags += n;
     const uint64_t bytes = r.u64();
-    const uint64_t at = r.pos;
+    const uint64_t at = r.dev + r.pos;
     (void)r.bytes(bytes);
     r.finish();
     if (n) {
@@ -1043,8 +1044,8 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
     }
 }
 
-static void parse_stream(const uint8_t* pool, uint64_t off, uint64_t len, DecPlan& P) {
-    HostReader r{pool, off, off + len};
+static void parse_stream(const uint8_t* bytes, uint64_t dev_off, uint64_t len, DecPlan& P) {
+    HostReader r{bytes, 0, len, dev_off};
     const uint8_t* magic = r.bytes(4);
     const bool lch1 = std::memcmp(magic, "LCH1", 4) == 0, lcp3 = std::memcmp(magic, "LCP3", 4) == 0;
     require(lch1 || lcp3 || std::memcmp(magic, "LCP2", 4) != 0, CHAIN_ERUNTIME,
@@ -1102,7 +1103,7 @@ static void parse_stream(const uint8_t* pool, uint64_t off, uint64_t len, DecPla
     require(F.ncorr <= F.n * D, CHAIN_ERUNTIME, "invalid LCP2 correction count");
     HostReader er = r.sub(r.u64());
     require(er.end - er.pos == 16 * F.ncorr, CHAIN_ERUNTIME, "raw exact correction size mismatch");
-    F.corr_off = er.pos;
+    F.corr_off = er.dev + er.pos;
     r.finish();
     P.T += F.n;
     P.frames.push_back(F);
@@ -1125,14 +1126,12 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     hipStream_t st = e->stream;
     StageTimer& tm = e->timer;
     tm.reset();
-    // pool the inputs
+    // device pool = the streams concatenated; parsed in place on the host
     std::vector<uint64_t> in_off(nstreams + 1, 0);
     for (uint32_t k = 0; k < nstreams; ++k) in_off[k + 1] = in_off[k] + lens[k];
-    std::vector<uint8_t> pool(in_off[nstreams]);
-    for (uint32_t k = 0; k < nstreams; ++k)
-        if (lens[k]) std::memcpy(pool.data() + in_off[k], bytes[k], lens[k]);
+    const uint64_t pool_size = in_off[nstreams];
     DecPlan P;
-    for (uint32_t k = 0; k < nstreams; ++k) parse_stream(pool.data(), in_off[k], lens[k], P);
+    for (uint32_t k = 0; k < nstreams; ++k) parse_stream(bytes[k], in_off[k], lens[k], P);
     const uint32_t F = nstreams, D = P.D;
     require(P.T < (1ull << 31), CHAIN_EINVAL, "a GPU decode batch holds fewer than 2^31 particles");
     // tiles: points (chain resolution) and records (L2 inverse), per frame
@@ -1163,8 +1162,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             DecFixed* fixed;
             DecRaw* raws;
             DecTagged* tagged;
-            uint32_t* prob;
-            uint32_t* cum;
+            uint64_t* pc;
             uint16_t* slots;
             uint32_t *idrel, *rel;
             uint64_t *diff, *keep;
@@ -1183,15 +1181,14 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             uint64_t* totals;
             uint8_t* out;
         } x;
-        x.in = c.take<uint8_t>(pool.size());
+        x.in = c.take<uint8_t>(res.d_pool ? 0 : pool_size);
         x.fr = c.take<DecFrame>(F);
         x.secs = c.take<DecSection>(P.secs.size());
         x.chunks = c.take<DecChunk>(P.chunks.size());
         x.fixed = c.take<DecFixed>(P.fixed.size());
         x.raws = c.take<DecRaw>(P.raws.size());
         x.tagged = c.take<DecTagged>(P.tagged.size());
-        x.prob = c.take<uint32_t>(P.prob.size());
-        x.cum = c.take<uint32_t>(P.cum.size());
+        x.pc = c.take<uint64_t>(P.pc.size());
         x.slots = c.take<uint16_t>(P.slots.size());
         x.idrel = c.take<uint32_t>(T);
         x.rel = c.take<uint32_t>(T * D);
@@ -1233,15 +1230,15 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         if (n) HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
     };
     tm.begin("h2d");
-    if (!res.d_pool) h2d(X.in, pool.data(), pool.size());
+    if (!res.d_pool)
+        for (uint32_t k = 0; k < nstreams; ++k) h2d(X.in + in_off[k], bytes[k], lens[k]);
     h2d(X.fr, P.frames.data(), F * sizeof(DecFrame));
     h2d(X.secs, P.secs.data(), P.secs.size() * sizeof(DecSection));
     h2d(X.chunks, P.chunks.data(), P.chunks.size() * sizeof(DecChunk));
     h2d(X.fixed, P.fixed.data(), P.fixed.size() * sizeof(DecFixed));
     h2d(X.raws, P.raws.data(), P.raws.size() * sizeof(DecRaw));
     h2d(X.tagged, P.tagged.data(), P.tagged.size() * sizeof(DecTagged));
-    h2d(X.prob, P.prob.data(), P.prob.size() * 4);
-    h2d(X.cum, P.cum.data(), P.cum.size() * 4);
+    h2d(X.pc, P.pc.data(), P.pc.size() * 8);
     h2d(X.slots, P.slots.data(), P.slots.size() * 2);
     h2d(X.ptb, ptb.data(), (F + 1) * 4);
     h2d(X.rtb, rtb.data(), (F + 1) * 4);
@@ -1256,7 +1253,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     tm.begin("rans_decode");
     if (!P.chunks.empty())
         hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.chunks.size()), dim3(64), 0, st, X.in, X.secs, X.chunks,
-                           X.prob, X.cum, X.slots, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+                           X.pc, X.slots, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
     tm.end();
     tm.begin("unpack");
     if (P.fixed_total)
@@ -1293,13 +1290,19 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     scan_u32(st, X.ec, X.eoff, npt, X.status, X.ticket, X.err, &X.totals[1]);
     hipLaunchKernelGGL(k_dec_check_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.eoff,
                        &X.totals[1], X.err);
-    fill32(st, X.done, npt + 1, 0);
-    fill32(st, X.ticket, 1, 0);
     with_dims(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
-        hipLaunchKernelGGL((k_dec_chain<DD>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F, X.ptb, npt, X.idrel, X.eoff,
-                           X.ticket, X.done, X.root, X.err);
+        hipLaunchKernelGGL((k_dec_chain_local<DD>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.idrel, X.eoff,
+                           X.root, X.err);
     });
+    {
+        uint32_t maxt = 1;
+        for (uint32_t f = 0; f < F; ++f) maxt = std::max(maxt, ptb[f + 1] - ptb[f]);
+        uint32_t rounds = 1;
+        while ((1u << (rounds - 1)) < maxt) ++rounds;   // ceil(log2(tiles)) + 1
+        for (uint32_t r = 0; r < rounds; ++r)
+            hipLaunchKernelGGL(k_dec_chain_jump, dim3(grid_for(T)), dim3(CH_WG), 0, st, X.root, T);
+    }
     tm.end();
     tm.begin("dequantize");
     with_dims(D, [&](auto dd) {

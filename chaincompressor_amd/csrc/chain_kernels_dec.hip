@@ -70,26 +70,20 @@ struct DecChunk {
     uint64_t first;             // first symbol of the chunk within its section
 };
 
+// one wavefront per chunk; (cum << 32 | freq) entries and the slot -> symbol
+// table (precision <= 14 bits) are read from global memory behind the caches
 __global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in, const DecSection* __restrict__ secs,
                                                  const DecChunk* __restrict__ chunks,
-                                                 const uint32_t* __restrict__ prob_pool,
-                                                 const uint32_t* __restrict__ cum_pool,
+                                                 const uint64_t* __restrict__ pc_pool,
                                                  const uint16_t* __restrict__ slot_pool, uint8_t* __restrict__ tags,
                                                  uint32_t* idrel, uint64_t* diff, uint64_t* keep, uint8_t* choice,
                                                  uint32_t* rel, uint32_t* err) {
     const DecChunk ch = chunks[blockIdx.x];
     const DecSection S = secs[ch.sec];
-    __shared__ uint32_t s_prob[RANS_VALUE_LIMIT], s_cum[RANS_VALUE_LIMIT];
-    __shared__ uint16_t s_slot[1u << 14];
     const uint32_t lane = threadIdx.x, pb = S.pb, M = 1u << pb, lower = 4u * M;
-    for (uint32_t b = lane; b < S.alphabet; b += 64) {
-        s_prob[b] = prob_pool[S.table_off + b];
-        s_cum[b] = cum_pool[S.table_off + b];
-    }
-    const bool lds_slots = S.slot_off != ~0ull;
-    if (lds_slots)
-        for (uint32_t b = lane; b < M; b += 64) s_slot[b] = slot_pool[S.slot_off + b];
-    __syncthreads();
+    const uint64_t* pc = pc_pool + S.table_off;
+    const uint16_t* slots = slot_pool + (S.slot_off == ~0ull ? 0 : S.slot_off);
+    const bool direct = S.slot_off != ~0ull;
     const uint32_t L = S.lanes;
     const uint64_t Sc = S.count - ch.first < S.chunk_symbols ? S.count - ch.first : S.chunk_symbols;
     const uint64_t T = (Sc + L - 1) / L;
@@ -111,17 +105,18 @@ __global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in,
         if (active) {
             const uint32_t slot = x & (M - 1);
             uint32_t s;
-            if (lds_slots) {
-                s = s_slot[slot];
+            if (direct) {
+                s = slots[slot];
             } else {   // largest symbol with cum <= slot
                 uint32_t lo = 0, hi = S.alphabet;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (s_cum[mid] <= slot) lo = mid; else hi = mid;
+                    if ((uint32_t)(pc[mid] >> 32) <= slot) lo = mid; else hi = mid;
                 }
                 s = lo;
             }
-            x = s_prob[s] * (x >> pb) + slot - s_cum[s];
+            const uint64_t e = pc[s];
+            x = (uint32_t)e * (x >> pb) + slot - (uint32_t)(e >> 32);
             const uint64_t q = ch.first + k;
             if (S.comp < 0) tags[S.dst + q] = (uint8_t)s;
             else store_value(S.comp, S.dst + q, s, idrel, diff, keep, choice, rel, err);
@@ -478,24 +473,25 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 }
 
 // ------------------------------------------------------------------------
-// Chain resolution: the explicit record of every point. Tiles are taken from
-// the end of the batch; a point whose successor lies in a later tile reads
-// that tile's result once its done flag is set (the later tile holds an
-// earlier ticket, so it is running or finished: the wait ends).
+// Chain resolution: the explicit record of every point. A tile first
+// resolves the links that stay inside it (pointer jumping in LDS); a point
+// whose chain leaves the tile keeps CH_EXT | first point outside. Rounds of
+// global pointer jumping then replace every CH_EXT entry by the entry it
+// names. Each such hop goes to a later tile, so ceil(log2(tiles)) + 1 rounds
+// resolve every chain; no tile ever waits for another.
 
 #define CH_LOCAL 0x80000000u
+#define CH_EXT 0x80000000u
 template <int D>
-__global__ __launch_bounds__(CH_WG) void k_dec_chain(const DecFrame* __restrict__ fr, uint32_t F,
-                                                     const uint32_t* __restrict__ tile_begin, uint32_t ntiles,
-                                                     const uint32_t* __restrict__ idrel,
-                                                     const uint32_t* __restrict__ tile_expl_off, uint32_t* ticket,
-                                                     uint32_t* done, uint32_t* __restrict__ root, uint32_t* err) {
-    __shared__ uint32_t s_tile, s_tmp[4];
+__global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __restrict__ fr, uint32_t F,
+                                                           const uint32_t* __restrict__ tile_begin,
+                                                           const uint32_t* __restrict__ idrel,
+                                                           const uint32_t* __restrict__ tile_expl_off,
+                                                           uint32_t* __restrict__ res, uint32_t* err) {
+    __shared__ uint32_t s_tmp[4];
     __shared__ uint32_t s_val[CH_TILE];
     __shared__ uint32_t s_ext[CH_TILE];
-    if (threadIdx.x == 0) s_tile = ntiles - 1 - atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const uint32_t f = upper_find(tile_begin, F, tile);
     const DecFrame& Fr = fr[f];
     const uint64_t p0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
@@ -526,10 +522,11 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain(const DecFrame* __restrict_
         }
     }
     __syncthreads();
-    // pointer jumping inside the tile (12 synchronous rounds cover 4096 links);
-    // an entry pointing to itself with s_ext set is an exit of the tile
+    // 12 synchronous rounds cover the 4096 links a tile can hold; an entry
+    // pointing to itself with s_ext set is an exit of the tile
     for (int round = 0; round < 12; ++round) {
         uint32_t nv[CH_ITEMS], ne[CH_ITEMS];
+        int open = 0;
         for (int k = 0; k < CH_ITEMS; ++k) {
             const uint32_t j = threadIdx.x * CH_ITEMS + k;
             nv[k] = 0; ne[k] = 0xffffffffu;
@@ -540,6 +537,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain(const DecFrame* __restrict_
                 const uint32_t tj = nv[k] & ~CH_LOCAL;
                 nv[k] = s_val[tj];
                 ne[k] = s_ext[tj];
+                open |= (nv[k] & CH_LOCAL) && ne[k] == 0xffffffffu;
             }
         }
         __syncthreads();
@@ -547,31 +545,26 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain(const DecFrame* __restrict_
             const uint32_t j = threadIdx.x * CH_ITEMS + k;
             if (j < cnt) { s_val[j] = nv[k]; s_ext[j] = ne[k]; }
         }
-        __syncthreads();
+        if (!__syncthreads_or(open)) break;   // every in-tile pointer resolved
     }
-    // resolve exits from the later tiles
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint32_t j = threadIdx.x * CH_ITEMS + k;
         if (j >= cnt) break;
-        uint32_t v = s_val[j];
-        const uint32_t g2 = s_ext[j];
-        if (g2 != 0xffffffffu) {
-            const uint64_t local2 = g2 - Fr.pt_off;
-            const uint32_t t2 = tile_begin[f] + (uint32_t)(local2 / CH_TILE);
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&done[t2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-                if (++spins > LB_SPIN_LIMIT) { atomicOr(err, DERR_WAIT); break; }
-                __builtin_amdgcn_s_sleep(2);
-            }
-            v = __hip_atomic_load(&root[g2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else if (v & CH_LOCAL) {
-            atomicOr(err, DERR_CHAIN);
-            v = 0;
-        }
-        root[Fr.pt_off + p0 + j] = v;
+        const uint32_t v = s_val[j], g2 = s_ext[j];
+        uint32_t out;
+        if (g2 != 0xffffffffu) out = CH_EXT | g2;
+        else if (v & CH_LOCAL) { atomicOr(err, DERR_CHAIN); out = 0; }
+        else out = v;
+        res[Fr.pt_off + p0 + j] = out;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&done[tile], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// one round of global pointer jumping over the tile exits
+__global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__ res, uint64_t T) {
+    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < T; g += (uint64_t)gridDim.x * CH_WG) {
+        const uint32_t v = res[g];
+        if (v & CH_EXT) res[g] = __hip_atomic_load(&res[v & ~CH_EXT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // every frame: explicit points == records
@@ -597,8 +590,9 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
         const uint32_t f = upper_find(frame_pt_begin, F, (uint32_t)g);
         const DecFrame& Fr = fr[f];
         const uint64_t i = g - Fr.pt_off;
-        const uint64_t rec = root[g];
-        if (rec - Fr.rec_base >= Fr.nrec) { atomicOr(err, DERR_CHAIN); continue; }
+        const uint32_t rv = root[g];
+        const uint64_t rec = rv;
+        if ((rv & CH_EXT) || rec - Fr.rec_base >= Fr.nrec) { atomicOr(err, DERR_CHAIN); continue; }
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const uint64_t off = rel[g * D + d];

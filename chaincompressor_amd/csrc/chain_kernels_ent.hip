@@ -233,7 +233,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
 __global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict__ ci, Section* __restrict__ sec,
                                                       const uint32_t* __restrict__ vhist,
                                                       const uint32_t* __restrict__ thist, uint32_t F,
-                                                      uint32_t* __restrict__ prob_out, uint32_t* __restrict__ cum_out,
+                                                      uint64_t* __restrict__ pc_out,
                                                       uint8_t* __restrict__ tbytes, uint32_t* err) {
     const uint32_t si = blockIdx.x;
     const uint32_t cidx = si >> 1, s = si & 1;
@@ -334,8 +334,7 @@ __global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict
         uint32_t tot;
         uint32_t run = block_excl_scan(local, s_tmp, &tot);
         for (uint32_t k = 0; k < 16 && b0 + k < A; ++k) {
-            prob_out[(uint64_t)si * RANS_VALUE_LIMIT + b0 + k] = s_prob[b0 + k];
-            cum_out[(uint64_t)si * RANS_VALUE_LIMIT + b0 + k] = run;
+            pc_out[(uint64_t)si * RANS_VALUE_LIMIT + b0 + k] = ((uint64_t)run << 32) | s_prob[b0 + k];
             run += s_prob[b0 + k];
         }
         if (threadIdx.x == 0 && tot != M) atomicOr(err, 4u);
@@ -392,13 +391,15 @@ __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_
     return lo;
 }
 
-// one wavefront per chunk; lanes beyond the section's lane count idle
+// one wavefront per chunk; lanes beyond the section's lane count idle. The
+// (cum << 32 | freq) table stays in global memory behind the caches (no LDS,
+// so many chunks stay resident) and symbols are fetched 8 steps ahead.
+#define ENC_GROUP 8
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ vals, int c, uint32_t F,
                                                     const CompInst* __restrict__ ci, const Section* __restrict__ sec,
                                                     const uint32_t* __restrict__ comp_chunk,
-                                                    const uint32_t* __restrict__ prob_tab,
-                                                    const uint32_t* __restrict__ cum_tab, uint8_t* __restrict__ scratch,
+                                                    const uint64_t* __restrict__ pc_tab, uint8_t* __restrict__ scratch,
                                                     uint64_t stride, uint32_t* __restrict__ chunk_len,
                                                     uint32_t* __restrict__ chunk_pos, uint32_t* err) {
     const uint32_t chunk = comp_chunk[c] + blockIdx.x;
@@ -407,12 +408,7 @@ __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ val
     const Section& S = sec[si];
     const CompInst& C = ci[si >> 1];
     const bool tags = (si & 1) != 0;
-    __shared__ uint32_t s_prob[RANS_VALUE_LIMIT], s_cum[RANS_VALUE_LIMIT];
-    for (uint32_t b = threadIdx.x; b < S.alphabet; b += 64) {
-        s_prob[b] = prob_tab[(uint64_t)si * RANS_VALUE_LIMIT + b];
-        s_cum[b] = cum_tab[(uint64_t)si * RANS_VALUE_LIMIT + b];
-    }
-    __syncthreads();
+    const uint64_t* pc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
     const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
     const uint32_t M = 1u << pb, lower = 4u * M;
     const uint64_t first = (uint64_t)(chunk - S.chunk_base) * S.chunk_symbols;
@@ -425,37 +421,49 @@ __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ val
     uint32_t x = lower;
     const ValT* src = vals + C.base * DIMS;
     const uint64_t lt = lanemask_lt();
-    for (uint64_t t = T; t-- > 0;) {
-        const uint64_t k = t * L + lane;
-        const bool active = lane < L && k < Sc;
-        uint32_t sym = 0;
-        if (active) {
-            const uint64_t q = first + k;
-            if (!tags) {
-                sym = (uint32_t)src[q];
-            } else {
-                uint32_t tg = 1;
+    for (uint64_t t = T; t > 0;) {
+        const uint32_t g = t < ENC_GROUP ? (uint32_t)t : ENC_GROUP;
+        uint32_t sym[ENC_GROUP];
 #pragma unroll
-                for (int d = 0; d < DIMS; ++d) {
-                    const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
-                    tg = b > tg ? b : tg;
+        for (int j = 0; j < ENC_GROUP; ++j) {
+            sym[j] = 0xffffffffu;
+            if ((uint32_t)j < g) {
+                const uint64_t k = (t - 1 - j) * L + lane;
+                if (lane < L && k < Sc) {
+                    const uint64_t q = first + k;
+                    if (!tags) {
+                        sym[j] = (uint32_t)src[q];
+                    } else {
+                        uint32_t tg = 1;
+#pragma unroll
+                        for (int d = 0; d < DIMS; ++d) {
+                            const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
+                            tg = b > tg ? b : tg;
+                        }
+                        sym[j] = tg;
+                    }
                 }
-                sym = tg;
             }
         }
-        const uint32_t f = active ? s_prob[sym] : 1u;
-        const uint32_t cu = active ? s_cum[sym] : 0u;
-        const uint32_t xmax = 1024u * f;     // (lower >> pb) * 256 * f
-        uint32_t kb = 0, xx = x;
-        if (active)
-            while (xx >= xmax) { ++kb; xx >>= 8; }
-        const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
-        const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
-        pos -= c0 + c1 + c2;
-        if (kb > 0) stream[pos + __popcll(m0 & lt)] = (uint8_t)(x >> (8 * (kb - 1)));
-        if (kb > 1) stream[pos + c0 + __popcll(m1 & lt)] = (uint8_t)(x >> (8 * (kb - 2)));
-        if (kb > 2) stream[pos + c0 + c1 + __popcll(m2 & lt)] = (uint8_t)(x >> (8 * (kb - 3)));
-        if (active) x = (xx / f) * M + (xx % f) + cu;
+#pragma unroll
+        for (int j = 0; j < ENC_GROUP; ++j) {
+            if ((uint32_t)j >= g) break;
+            const bool active = sym[j] != 0xffffffffu;
+            const uint64_t e = active ? pc[sym[j]] : 1ull;
+            const uint32_t f = (uint32_t)e, cu = (uint32_t)(e >> 32);
+            const uint32_t xmax = 1024u * f;     // (lower >> pb) * 256 * f
+            uint32_t kb = 0, xx = x;
+            if (active)
+                while (xx >= xmax) { ++kb; xx >>= 8; }
+            const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
+            const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
+            pos -= c0 + c1 + c2;
+            if (kb > 0) stream[pos + __popcll(m0 & lt)] = (uint8_t)(x >> (8 * (kb - 1)));
+            if (kb > 1) stream[pos + c0 + __popcll(m1 & lt)] = (uint8_t)(x >> (8 * (kb - 2)));
+            if (kb > 2) stream[pos + c0 + c1 + __popcll(m2 & lt)] = (uint8_t)(x >> (8 * (kb - 3)));
+            if (active) x = (xx / f) * M + (xx % f) + cu;
+        }
+        t -= g;
     }
     if (lane < L) put_u32(out + 4 * lane, x);
     if (lane == 0) {
@@ -467,24 +475,35 @@ __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ val
 // ------------------------------------------------------------------------
 // candidate sizes and the choice, one thread per component instance
 
-__device__ inline uint64_t section_size(const Section& S, const uint32_t* chunk_len, uint32_t* chunk_off) {
-    uint64_t payload = 0;
-    for (uint32_t k = 0; k < S.nchunks; ++k) {
-        if (chunk_off) chunk_off[S.chunk_base + k] = (uint32_t)payload;
-        payload += 4ull * S.lanes + chunk_len[S.chunk_base + k];
+// bytes of every section and each chunk's offset inside it: one workgroup
+// per section, block scan over its chunks
+__global__ __launch_bounds__(CH_WG) void k_section_sizes(Section* __restrict__ sec,
+                                                         const uint32_t* __restrict__ chunk_len,
+                                                         uint32_t* __restrict__ chunk_off) {
+    __shared__ uint32_t s_tmp[4];
+    Section& S = sec[blockIdx.x];
+    const uint32_t nch = S.nchunks, base = S.chunk_base, st = 4u * S.lanes;
+    uint64_t run = 0;
+    for (uint32_t k0 = 0; k0 < nch; k0 += CH_WG) {
+        const uint32_t k = k0 + threadIdx.x;
+        const uint32_t v = k < nch ? st + chunk_len[base + k] : 0u;
+        uint32_t tot;
+        const uint32_t pre = block_excl_scan(v, s_tmp, &tot);
+        if (k < nch) chunk_off[base + k] = (uint32_t)(run + pre);
+        run += tot;
     }
-    return RVC1_HDR_FIXED + S.table_len + 4ull * S.nchunks + payload;
+    if (threadIdx.x == 0) {
+        S.payload = run;
+        S.size = RVC1_HDR_FIXED + S.table_len + 4ull * nch + run;
+    }
 }
 
-__global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, uint32_t ninst,
-                         const uint32_t* __restrict__ chunk_len, uint32_t* __restrict__ chunk_off) {
+__global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, uint32_t ninst) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     CompInst& C = ci[i];
     Section& sv = sec[2 * i];
     Section& st = sec[2 * i + 1];
-    st.size = section_size(st, chunk_len, chunk_off);
-    st.payload = st.size - RVC1_HDR_FIXED - st.table_len - 4ull * st.nchunks;
     const uint64_t tag_size = 4 + 4 + 8 + 1 + 8 + st.size + 8 + (C.tag_bits + 7) / 8;
     uint64_t fix_size = 4 + 4 + 8 + (uint64_t)C.dims * 17;
     for (uint32_t d = 0; d < C.dims; ++d) {
@@ -499,8 +518,6 @@ __global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, u
     uint64_t best = tag_size;
     if (fix_size < best) { best = fix_size; kind = KIND_FIXP; }
     if (sv.eligible) {
-        sv.size = section_size(sv, chunk_len, chunk_off);
-        sv.payload = sv.size - RVC1_HDR_FIXED - sv.table_len - 4ull * sv.nchunks;
         const uint64_t rans_size = 4 + 4 + 8 + 8 + sv.size;
         if (rans_size < best) { best = rans_size; kind = KIND_RANS; }
     }
@@ -509,10 +526,20 @@ __global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, u
 }
 
 // output layout: one thread walks the frames in order
+#define TAG_TILE 1024
 __global__ void k_layout(CompInst* __restrict__ ci, Section* __restrict__ sec, const FrameOut* __restrict__ fo,
                          uint32_t F, uint32_t D, uint64_t* __restrict__ out_off, uint64_t* __restrict__ out_size,
-                         uint64_t* __restrict__ total) {
+                         uint64_t* __restrict__ total, uint32_t* __restrict__ tag_tiles /* [5][F+1] */) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int c = 0; c < 5; ++c) {
+        uint32_t run = 0;
+        for (uint32_t f = 0; f < F; ++f) {
+            tag_tiles[c * (F + 1) + f] = run;
+            const CompInst& C = ci[c * F + f];
+            if (C.kind == KIND_TAGR) run += (uint32_t)((C.n + TAG_TILE - 1) / TAG_TILE);
+        }
+        tag_tiles[c * (F + 1) + F] = run;
+    }
     uint64_t off = 0;
     for (uint32_t f = 0; f < F; ++f) {
         out_off[f] = off;
@@ -697,10 +724,11 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
 // Tagged payloads: 1024 records per tile, bit offsets by a segmented look-back
 // over each frame's tiles, bits assembled in LDS; the partial bytes at the
 // two ends of a tile go to side arrays and are merged by k_tag_seams.
-#define TAG_TILE 1024
+// tag_tiles[f] (F+1 entries) counts the tiles of the frames whose component
+// is Tagged; only that many workgroups take a ticket.
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ vals, int c, uint32_t F,
-                                                       const uint32_t* __restrict__ tile_begin,
+                                                       const uint32_t* __restrict__ tag_tiles,
                                                        const CompInst* __restrict__ ci, uint64_t* status,
                                                        uint32_t* ticket, uint32_t* err,
                                                        uint8_t* __restrict__ head, uint8_t* __restrict__ tail,
@@ -708,15 +736,15 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_words[(TAG_TILE * 4 * 63) / 32 + 4];
+    const uint32_t* tile_begin = tag_tiles + c * (F + 1);
+    if (blockIdx.x >= tile_begin[F]) return;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t f = upper_find(tile_begin, F, tile);
     const CompInst& C = ci[c * F + f];
     if (threadIdx.x == 0) head_pos[tile] = ~0ull;
-    if (C.kind != KIND_TAGR) return;
     const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * TAG_TILE;
-    if (r0 >= C.n) return;
     const ValT* src = vals + C.base * DIMS;
     const uint64_t rt = r0 + threadIdx.x * 4;
     uint32_t tg[4];
@@ -781,9 +809,10 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
 }
 
 __global__ void k_tag_seams(const uint8_t* __restrict__ head, const uint8_t* __restrict__ tail,
-                            const uint64_t* __restrict__ head_pos, uint32_t ntiles, uint8_t* __restrict__ out) {
+                            const uint64_t* __restrict__ head_pos, const uint32_t* __restrict__ ntiles_dev,
+                            uint8_t* __restrict__ out) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles || head_pos[t] == ~0ull) return;
+    if (t >= *ntiles_dev || head_pos[t] == ~0ull) return;
     out[head_pos[t]] = head[t] | tail[t - 1];
 }
 

@@ -218,7 +218,8 @@ class Engine:
         return [_take(outs[k]) for k in range(F)], [sts[k] for k in range(F)]
 
     def decompress(self, data: bytes, dtype=np.float64, layout="aos"):
-        buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data if data else b"\0")
+        data = bytes(data)
+        buf = C.cast(C.c_char_p(data), C.POINTER(C.c_uint8))   # no copy; data stays alive below
         out = _Buf()
         n = C.c_uint64()
         d = C.c_uint32()
@@ -232,8 +233,8 @@ class Engine:
 
     def decompress_batch(self, streams, dtype=np.float64, layout="aos"):
         k = len(streams)
-        bufs = [(C.c_uint8 * max(1, len(s))).from_buffer_copy(s if s else b"\0") for s in streams]
-        ptrs = (C.POINTER(C.c_uint8) * k)(*[C.cast(b, C.POINTER(C.c_uint8)) for b in bufs])
+        bufs = [bytes(s) for s in streams]   # kept alive for the call; pointers need no copy
+        ptrs = (C.POINTER(C.c_uint8) * k)(*[C.cast(C.c_char_p(b), C.POINTER(C.c_uint8)) for b in bufs])
         lens = (C.c_size_t * k)(*[len(s) for s in streams])
         outs = (_Buf * k)()
         ns = (C.c_uint64 * k)()
@@ -252,8 +253,8 @@ class Engine:
         pointer to the same bytes concatenated). Returns (device pointer,
         element offsets, ns, dims); the output stays on the device."""
         k = len(streams)
-        bufs = [(C.c_uint8 * max(1, len(s))).from_buffer_copy(s if s else b"\0") for s in streams]
-        ptrs = (C.POINTER(C.c_uint8) * k)(*[C.cast(b, C.POINTER(C.c_uint8)) for b in bufs])
+        bufs = [bytes(s) for s in streams]   # kept alive for the call; pointers need no copy
+        ptrs = (C.POINTER(C.c_uint8) * k)(*[C.cast(C.c_char_p(b), C.POINTER(C.c_uint8)) for b in bufs])
         lens = (C.c_size_t * k)(*[len(s) for s in streams])
         dout = C.c_void_p()
         offs = (C.c_uint64 * k)()

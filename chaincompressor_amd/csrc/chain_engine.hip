@@ -984,32 +984,29 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
         return;
     }
     if (std::memcmp(magic, "FIXP", 4) == 0) {
-        std::vector<DecFixed> js(dims);
+        DecFixed j;
+        std::memset(&j, 0, sizeof j);
         uint64_t total = 0;
         for (uint32_t d = 0; d < dims; ++d) {
-            DecFixed& j = js[d];
-            std::memset(&j, 0, sizeof j);
-            j.minimum = (uint64_t)r.i64();
-            j.width = r.u8();
+            j.minimum[d] = (uint64_t)r.i64();
+            j.width[d] = r.u8();
             const uint64_t bytes = r.u64();
-            require(j.width >= 1 && j.width <= 64, CHAIN_ERUNTIME, "invalid fixed width");
-            require(bytes == (n * j.width + 7) / 8, CHAIN_ERUNTIME, "invalid fixed payload size");
-            require((int64_t)j.minimum >= 0, CHAIN_ERUNTIME, "fixed value overflow");
-            j.n = n;
-            j.dims = dims;
-            j.dst = dst + d;
-            j.comp = comp;
-            j.byte_off = total;
+            require(j.width[d] >= 1 && j.width[d] <= 64, CHAIN_ERUNTIME, "invalid fixed width");
+            require(bytes == (n * j.width[d] + 7) / 8, CHAIN_ERUNTIME, "invalid fixed payload size");
+            require((int64_t)j.minimum[d] >= 0, CHAIN_ERUNTIME, "fixed value overflow");
+            j.byte_off[d] = total;
             total += bytes;
         }
         require(r.end - r.pos == total, CHAIN_ERUNTIME, "fixed raw size mismatch");
-        for (uint32_t d = 0; d < dims; ++d) {
-            js[d].byte_off += r.dev + r.pos;
-            if (n) {
-                js[d].first_thread = P.fixed_total;
-                P.fixed_total += n;
-                P.fixed.push_back(js[d]);
-            }
+        for (uint32_t d = 0; d < dims; ++d) j.byte_off[d] += r.dev + r.pos;
+        j.n = n;
+        j.dims = dims;
+        j.dst = dst;
+        j.comp = comp;
+        if (n) {
+            j.first_thread = P.fixed_total;
+            P.fixed_total += n;
+            P.fixed.push_back(j);
         }
         return;
     }

@@ -81,9 +81,18 @@ __global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in,
     const DecChunk ch = chunks[blockIdx.x];
     const DecSection S = secs[ch.sec];
     const uint32_t lane = threadIdx.x, pb = S.pb, M = 1u << pb, lower = 4u * M;
-    const uint64_t* pc = pc_pool + S.table_off;
-    const uint16_t* slots = slot_pool + (S.slot_off == ~0ull ? 0 : S.slot_off);
+    // small tables (the common case: widths, in-block offsets, choices) live in LDS
+    __shared__ uint64_t s_pc[256];
+    __shared__ uint16_t s_slot[4096];
     const bool direct = S.slot_off != ~0ull;
+    const bool small = direct && M <= 4096 && S.alphabet <= 256;
+    if (small) {
+        for (uint32_t b = lane; b < S.alphabet; b += 64) s_pc[b] = pc_pool[S.table_off + b];
+        for (uint32_t b = lane; b < M; b += 64) s_slot[b] = slot_pool[S.slot_off + b];
+    }
+    __syncthreads();
+    const uint64_t* pc = small ? s_pc : pc_pool + S.table_off;
+    const uint16_t* slots = small ? s_slot : slot_pool + (direct ? S.slot_off : 0);
     const uint32_t L = S.lanes;
     const uint64_t Sc = S.count - ch.first < S.chunk_symbols ? S.count - ch.first : S.chunk_symbols;
     const uint64_t T = (Sc + L - 1) / L;
@@ -141,18 +150,19 @@ __global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in,
     if (lane == 0 && pos != stream_len) atomicOr(err, DERR_RANS);
 }
 
-// Fixed payload job: one component dimension of one frame
+// Fixed payload job: one component of one frame (all its dimensions)
 struct DecFixed {
-    uint64_t byte_off;          // payload of this dimension in the input pool
-    uint64_t n;                 // values
-    uint64_t dst;               // first value index (record r -> dst + r*dims)
-    uint64_t minimum;
-    uint32_t width, dims;
+    uint64_t byte_off[CH_MAXD]; // payload of each dimension in the input pool
+    uint64_t minimum[CH_MAXD];
+    uint32_t width[CH_MAXD];
+    uint64_t n;                 // records
+    uint64_t dst;               // first value index (record r -> dst + r*dims + d)
+    uint32_t dims;
     int32_t comp;
-    uint32_t pad;
     uint64_t first_thread;      // prefix of n over jobs
 };
 
+// one thread per record: every dimension's bits, written together
 __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__ in, const DecFixed* __restrict__ jobs,
                                                      uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
                                                      uint64_t* keep, uint8_t* choice, uint32_t* rel, uint32_t* err) {
@@ -162,18 +172,25 @@ __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__
             const uint32_t mid = (lo + hi) >> 1;
             if (jobs[mid].first_thread <= g) lo = mid; else hi = mid;
         }
-        const DecFixed J = jobs[lo];
+        const DecFixed& J = jobs[lo];
         const uint64_t r = g - J.first_thread;
-        const uint64_t bit = r * J.width;
-        const uint8_t* p = in + J.byte_off + (bit >> 3);
-        const uint32_t sh = (uint32_t)(bit & 7);
-        const uint32_t nb = (sh + J.width + 7) >> 3;
-        unsigned __int128 acc = 0;
-        for (uint32_t k = 0; k < nb; ++k) acc = (acc << 8) | p[k];
-        const uint64_t v = (uint64_t)(acc >> (8 * nb - sh - J.width)) &
-                           (J.width >= 64 ? ~0ull : ((1ull << J.width) - 1ull));
-        if (v > (uint64_t)INT64_MAX - J.minimum) atomicOr(err, DERR_VALUE);
-        store_value(J.comp, J.dst + r * J.dims, J.minimum + v, idrel, diff, keep, choice, rel, err);
+        for (uint32_t d = 0; d < J.dims; ++d) {
+            const uint32_t w = J.width[d];
+            const uint64_t bit = r * w;
+            const uint8_t* p = in + J.byte_off[d] + (bit >> 3);
+            const uint32_t sh = (uint32_t)(bit & 7);
+            const uint32_t nb = (sh + w + 7) >> 3;
+            uint64_t acc_hi = 0, acc = 0;
+            for (uint32_t q = 0; q < nb; ++q) {
+                acc_hi = (acc_hi << 8) | (acc >> 56);
+                acc = (acc << 8) | p[q];
+            }
+            const uint32_t drop = 8 * nb - sh - w;
+            uint64_t v = drop ? ((acc >> drop) | (acc_hi << (64 - drop))) : acc;
+            v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
+            if (v > (uint64_t)INT64_MAX - J.minimum[d]) atomicOr(err, DERR_VALUE);
+            store_value(J.comp, J.dst + r * J.dims + d, J.minimum[d] + v, idrel, diff, keep, choice, rel, err);
+        }
     }
 }
 
@@ -350,41 +367,48 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     Agg a;
     a.reset = 0;
     for (int d = 0; d < D; ++d) a.v[d] = 0;
-    uint64_t vals[CH_ITEMS][D];
-    uint32_t resets = 0;
+    // element of record r: (reset, value) per dimension; computed twice (for
+    // the aggregate and for the output) instead of kept in a private array
+    const uint64_t kidx0 = kidx;
+    auto element = [&](uint64_t r, uint64_t kix, bool check, uint64_t* v) -> bool {
+        const uint64_t grec = Fr.rec_base + r;
+        if (choice[grec] == 0) {
+            const bool inb = kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep;
+            if (check && !inb) atomicOr(err, DERR_L2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint64_t kv = inb ? keep[kix * D + d] : (uint64_t)Fr.wrap_min;
+                if (check && ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max)) atomicOr(err, DERR_L2);
+                v[d] = kv - (uint64_t)Fr.wrap_min;
+            }
+            return true;
+        }
+        if (check && r == 0) atomicOr(err, DERR_L2);
+        const uint64_t didx = grec - kix;   // global diff index
+        const bool inb = didx >= Fr.diff_base && didx < Fr.diff_base + Fr.ndiff;
+        if (check && !inb) atomicOr(err, DERR_L2);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const uint64_t sym = inb ? diff[didx * D + d] : 0;
+            const int64_t corr = (sym & 1u) ? -(int64_t)(sym >> 1) - 1 : (int64_t)(sym >> 1);
+            if (check && ((int64_t)sym < 0 || corr < Fr.min_corr || corr > Fr.max_corr)) atomicOr(err, DERR_L2);
+            const uint64_t delta = corr > 0 ? R - (uint64_t)corr : (uint64_t)(-corr);
+            v[d] = delta < R ? delta : delta % R;
+        }
+        return false;
+    };
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint64_t r = r0 + k;
         if (r >= Fr.nrec) break;
-        const uint64_t grec = Fr.rec_base + r;
-        const uint8_t c = choice[grec];
-        if (c == 0) {
-            resets |= 1u << k;
+        uint64_t v[D];
+        if (element(r, kidx, true, v)) {
             a.reset = 1;
-            const bool inb = kidx >= Fr.keep_base && kidx < Fr.keep_base + Fr.nkeep;
-            if (!inb) atomicOr(err, DERR_L2);
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const uint64_t kv = inb ? keep[kidx * D + d] : (uint64_t)Fr.wrap_min;
-                if ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max) atomicOr(err, DERR_L2);
-                vals[k][d] = kv - (uint64_t)Fr.wrap_min;
-                a.v[d] = vals[k][d];
-            }
+            for (int d = 0; d < D; ++d) a.v[d] = v[d];
             ++kidx;
         } else {
-            if (r == 0) atomicOr(err, DERR_L2);
-            const uint64_t didx = grec - kidx;   // global diff index
-            const bool inb = didx >= Fr.diff_base && didx < Fr.diff_base + Fr.ndiff;
-            if (!inb) atomicOr(err, DERR_L2);
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const uint64_t sym = inb ? diff[didx * D + d] : 0;
-                if ((int64_t)sym < 0) atomicOr(err, DERR_L2);
-                const int64_t corr = (sym & 1u) ? -(int64_t)(sym >> 1) - 1 : (int64_t)(sym >> 1);
-                if (corr < Fr.min_corr || corr > Fr.max_corr) atomicOr(err, DERR_L2);
-                const uint64_t delta = corr > 0 ? R - (uint64_t)corr : (uint64_t)(-corr);
-                vals[k][d] = delta % R;
-                a.v[d] = addmod(a.v[d], vals[k][d], R);
-            }
+            for (int d = 0; d < D; ++d) a.v[d] = addmod(a.v[d], v[d], R);
         }
     }
     // inclusive scan of thread aggregates over the workgroup (Hillis-Steele in LDS)
@@ -460,13 +484,16 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
         for (int d = 0; d < D; ++d) before.v[d] = s_v[threadIdx.x - 1][d];
         agg_combine<D>(run, before, R);
     }
+    kidx = kidx0;
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint64_t r = r0 + k;
         if (r >= Fr.nrec) break;
-        const bool rs = (resets >> k) & 1u;
+        uint64_t v[D];
+        const bool rs = element(r, kidx, false, v);
+        kidx += rs ? 1 : 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            run.v[d] = rs ? vals[k][d] : addmod(run.v[d], vals[k][d], R);
+            run.v[d] = rs ? v[d] : addmod(run.v[d], v[d], R);
             compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + run.v[d];
         }
     }

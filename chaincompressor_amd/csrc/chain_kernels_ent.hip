@@ -408,7 +408,13 @@ __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ val
     const Section& S = sec[si];
     const CompInst& C = ci[si >> 1];
     const bool tags = (si & 1) != 0;
-    const uint64_t* pc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
+    const uint64_t* gpc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
+    __shared__ uint64_t s_pc[256];
+    const bool small = S.alphabet <= 256;
+    if (small)
+        for (uint32_t b = threadIdx.x; b < S.alphabet; b += 64) s_pc[b] = gpc[b];
+    __syncthreads();
+    const uint64_t* pc = small ? s_pc : gpc;
     const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
     const uint32_t M = 1u << pb, lower = 4u * M;
     const uint64_t first = (uint64_t)(chunk - S.chunk_base) * S.chunk_symbols;
@@ -445,11 +451,14 @@ __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ val
                 }
             }
         }
+        uint64_t ent[ENC_GROUP];   // table entries of the group, fetched together
+#pragma unroll
+        for (int j = 0; j < ENC_GROUP; ++j) ent[j] = sym[j] != 0xffffffffu ? pc[sym[j]] : 1ull;
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
             if ((uint32_t)j >= g) break;
             const bool active = sym[j] != 0xffffffffu;
-            const uint64_t e = active ? pc[sym[j]] : 1ull;
+            const uint64_t e = ent[j];
             const uint32_t f = (uint32_t)e, cu = (uint32_t)(e >> 32);
             const uint32_t xmax = 1024u * f;     // (lower >> pb) * 256 * f
             uint32_t kb = 0, xx = x;

@@ -308,14 +308,10 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
 #pragma unroll
     for (int d = 0; d < D; ++d) { mins[d] = F.mins[d]; shift[d] = F.key_shift[d]; word[d] = F.key_word[d]; }
     __shared__ uint32_t s_tmp[4];
-    uint32_t run = 0;
-    const uint64_t cbase = (uint64_t)tile * CH_TILE * D;
+    uint64_t cmask = 0;   // correction flags, bit j*D + d
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
-        const bool valid = i < end;
-        uint32_t cflags = 0;
-        double xs[D];
-        if (valid) {
+        if (i < end) {
             KeyT key = 0;
             KeyT kw[D];
 #pragma unroll
@@ -323,11 +319,10 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const double x = load_coord<T, D, SOA>(pts, F, i, d);
-                xs[d] = x;
                 const double qv = floor((x - mins[d]) / denom);
                 const uint64_t q = (uint64_t)qv;
                 const double restored = fma((double)q + 0.5, denom, mins[d]);
-                if (!within_eb(restored, x, eb)) cflags |= 1u << d;
+                if (!within_eb(restored, x, eb)) cmask |= 1ull << (j * D + d);
                 const uint64_t b = div_p(q, p, magic, l);
                 rel_out[(F.pt_off + i) * D + d] = (RelT)(q - b * p);
                 if (nwords == 1) key |= (KeyT)b << shift[d];
@@ -341,19 +336,24 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
                     if ((uint32_t)w < nwords) key_out[(uint64_t)w * total_pts + F.pt_off + i] = kw[w];
             }
         }
-        const uint32_t cnt = __popc(cflags);
-        if (__syncthreads_or(cnt != 0)) {
+    }
+    // exact corrections, in element order (j, thread, d); rare, so the
+    // ordered write path runs only when the tile has any
+    uint32_t run = 0;
+    const uint64_t cbase = (uint64_t)tile * CH_TILE * D;
+    if (__syncthreads_or(cmask != 0)) {
+        for (int j = 0; j < CH_ITEMS; ++j) {
+            const uint32_t bits = (uint32_t)(cmask >> (j * D)) & ((1u << D) - 1u);
             uint32_t tot;
-            uint32_t pos = run + block_excl_scan(cnt, s_tmp, &tot);
-            if (valid) {
+            uint32_t pos = run + block_excl_scan(__popc(bits), s_tmp, &tot);
+            const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
 #pragma unroll
-                for (int d = 0; d < D; ++d)
-                    if (cflags & (1u << d)) {
-                        corr_idx[cbase + pos] = i * D + d;
-                        corr_val[cbase + pos] = xs[d];
-                        ++pos;
-                    }
-            }
+            for (int d = 0; d < D; ++d)
+                if (bits & (1u << d)) {
+                    corr_idx[cbase + pos] = i * D + d;
+                    corr_val[cbase + pos] = load_coord<T, D, SOA>(pts, F, i, d);
+                    ++pos;
+                }
             run += tot;
         }
     }

@@ -1,16 +1,31 @@
 #!/usr/bin/env python3
 """Chain compressor benchmark on one MI355X.
 
-Default workload: one frame of 50,000,000 float32 xyz particles (64 Gaussian
-clusters, sigma log-uniform in [0.005, 0.05], plus 10 % uniform background,
-cluster-major order), eb = 1e-3, p = 16, threshold = 2. A step is one
-compression of the frame (points resident in HBM -> LCH1 container in HBM)
-followed by one decompression of that container (-> float64 coordinates in
-HBM). Throughput = uncompressed input bytes N*3*4 per wall-clock second of
-the timed steps; the per-kernel device times come from HIP events the engine
-records on its own stream.
+Workloads (synthetic, seeded; chaincompressor_amd/workloads.py):
 
-    python bench.py [--steps K] [--warmup W] [--workload plain|batched|serving]
+* plain    -- one frame of 50,000,000 float32 xyz particles (64 Gaussian
+  clusters, sigma log-uniform in [0.005, 0.05], plus 10 % uniform background,
+  cluster-major order), eb = 1e-3, p = 16, threshold = 2. A step is one
+  compression of the frame (points resident in HBM -> LCH1 container in HBM)
+  followed by one decompression of that container (-> float64 coordinates in
+  HBM). ``value`` is the compression throughput: uncompressed input bytes
+  N*3*4 per wall-clock second of the timed compressions. The same frame is
+  also run at eb = 1e-2 and 1e-4.
+* batched  -- 256 float32 frames with log-uniform counts in [1e4, 4e6], each
+  half jittered lattice and half Gaussian mix, eb = 1e-3: all frames through
+  one call (one launch per stage) against one call per frame.
+* md       -- 128 float64 MD-like frames (fcc + thermal noise, periodic box),
+  counts uniform in [1e5, 2e6], eb = 1e-4: batch decompression against one
+  call per frame.
+* serving  -- Poisson arrivals (200 req/s), 70 % compress / 30 % decompress,
+  sizes log-uniform in [5e3, 2e7], float32 80 % / float64 20 %, eb in
+  {1e-2, 1e-3, 1e-4}, admitted under a 64 GB device budget; p50/p99 latency.
+
+The roofline object names the timed region with the most device time per
+step (HIP events on the engine's stream) and divides its algorithmic bytes
+(stated by the engine for every region) by its time.
+
+    python bench.py [--steps K] [--warmup W] [--workload default|plain|batched|md|serving|all]
 
 Prints one JSON line.
 """
@@ -29,6 +44,35 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# timed region -> the HIP kernels it launches
+REGION_KERNELS = {
+    "bbox": "chain::k_bbox + k_bbox_finish",
+    "quantize": "chain::k_quantize",
+    "sort_hist": "chain::k_sort_hist",
+    "sort_scan": "chain::k_scan_u32",
+    "sort_scatter": "chain::k_sort_scatter",
+    "link": "chain::k_link",
+    "explicit": "chain::k_explicit + k_frame_counts",
+    "l2_stats": "chain::k_l2_stats + k_l2_wrap",
+    "l2_split": "chain::k_l2_split + k_frame_split_counts",
+    "corrections": "chain::k_scan_u32 + k_corr_gather + k_frame_corr",
+    "comp_stats": "chain::k_comp_stats (one launch per component)",
+    "rans_table": "chain::k_rans_table + k_rans_plan",
+    "rans_encode": "chain::k_rans_encode (one launch per component)",
+    "assemble": "chain::k_section_sizes + k_choose + k_layout + k_write_frames + k_write_components + k_copy_chunks",
+    "pack": "chain::k_pack_fixed + k_pack_tagged + k_tag_seams (per component)",
+    "rans_decode": "chain::k_dec_rans",
+    "unpack_fixed": "chain::k_dec_fixed",
+    "unpack_raw": "chain::k_dec_raw",
+    "unpack_tagged": "chain::k_dec_tagged",
+    "l2_counts": "chain::k_dec_tile_zero_counts + k_scan_u32",
+    "l2_inverse": "chain::k_dec_l2",
+    "chain_counts": "chain::k_dec_tile_zero_counts + k_scan_u32 + k_dec_check_counts",
+    "chain_local": "chain::k_dec_chain_local",
+    "chain_jump": "chain::k_dec_chain_jump (ceil(log2 tiles)+1 launches)",
+    "dequantize": "chain::k_dec_dequant + k_dec_corr",
+}
 
 
 def log(*a):
@@ -50,160 +94,339 @@ def quiet_stdout():
             os.close(fd)
 
 
-def agg_stages(stages):
+def agg_regions(steps):
+    """steps: list of [(name, ms, bytes)] -> {name: [ms, launches, bytes]} summed."""
     out = {}
-    for name, ms in stages:
-        t, c = out.get(name, (0.0, 0))
-        out[name] = (t + ms, c + 1)
+    for s in steps:
+        for name, ms, b in s:
+            a = out.setdefault(name, [0.0, 0, 0])
+            a[0] += ms
+            a[1] += 1
+            a[2] += b
     return out
 
 
-def cpu_baseline(pts_sample, eb, p, t):
+def roofline(c_steps, d_steps, K):
+    regions = agg_regions(c_steps)
+    for k, v in agg_regions(d_steps).items():
+        a = regions.setdefault(k, [0.0, 0, 0])
+        a[0] += v[0]
+        a[1] += v[1]
+        a[2] += v[2]
+    regions.pop("h2d", None)
+    name, (ms, cnt, b) = max(regions.items(), key=lambda kv: kv[1][0])
+    achieved = b / (ms / 1e3) / 1e9 if b and ms > 0 else None
+    return {
+        "bound": "hbm",
+        "kernel": name,
+        "hip_kernels": REGION_KERNELS.get(name, name),
+        "launches_per_step": cnt / K,
+        "ms_per_step": round(ms / K, 4),
+        "alg_bytes_per_step": int(b / K),
+        "achieved": round(achieved, 2) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
+        "traffic": traffic_from_profiles(name),
+    }
+
+
+def region_table(steps, K):
+    return {k: {"ms": round(v[0] / K, 4), "GBps": round(v[2] / (v[0] / 1e3) / 1e9, 1) if v[2] and v[0] else None}
+            for k, v in agg_regions(steps).items()}
+
+
+def pipeline_fraction(steps, K):
+    """Stated algorithmic bytes of all regions over their summed device time."""
+    a = agg_regions(steps)
+    a.pop("h2d", None)
+    ms = sum(v[0] for v in a.values())
+    b = sum(v[2] for v in a.values())
+    return round(b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if ms else None
+
+
+def traffic_from_profiles(region):
+    """HBM bytes per step of a region measured with rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel), if recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic_50M_f32_eb1e-3.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        v = t.get("regions", {}).get(region)
+        return v
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(pts_sample, eb, p, t, runs=5):
     """The reference CPU compressor (oracle/_ref, compiled from the reference's
-    cpp/chain/lcps_cpp.cpp) on a bounded sample, one thread; falls back to the
-    oracle's C restatement when the compiled reference is absent."""
+    cpp/chain/lcps_cpp.cpp) on a bounded sample, one thread, median of `runs`
+    after one warm-up; falls back to the oracle's C restatement when the
+    compiled reference is absent."""
     nbytes = pts_sample.nbytes
     ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    sys.path.insert(0, ref_dir)
     try:
-        sys.path.insert(0, ref_dir)
         import lcps_cpp  # noqa: F401
+        import lcps_cpu_decompress  # noqa: F401
         kind = "reference"
     except ImportError:
         kind = "port"
-    times = []
-    for _ in range(2):
-        t0 = time.perf_counter()
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+    def comp():
         if kind == "reference":
             with quiet_stdout():
-                lcps_cpp.lcps_compress2(pts_sample, eb, p, t, 0)
-        else:
-            sys.path.insert(0, os.path.join(ROOT, "oracle"))
-            import oracle as O
-            O.compress_lcp2(pts_sample, eb, p, t)
-        times.append(time.perf_counter() - t0)
-    best = min(times)
-    return {"value": round(nbytes / best / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": kind,
-            "sample": f"{pts_sample.shape[0]} points (every {{stride}}th point of the frame, order kept), "
-                      f"lcps_compress2 ec=True (Zstd-22 chooser), best of 2 runs, {best:.2f} s"}
+                return lcps_cpp.lcps_compress2(pts_sample, eb, p, t, 0)[0]
+        return O.compress_lcp2(pts_sample, eb, p, t)[0]
 
+    def dec(b):
+        if kind == "reference":
+            with quiet_stdout():
+                return lcps_cpu_decompress.lcps_decompress_serial(b)
+        return O.decode(b)
+
+    blob = comp()
+    tc, td = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        comp()
+        tc.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        dec(blob)
+        td.append(time.perf_counter() - t0)
+    mc, md = float(np.median(tc)), float(np.median(td))
+    return {"value": round(nbytes / mc / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": kind,
+            "decompress_value": round(nbytes / md / 1e9, 6),
+            "sample": f"{pts_sample.shape[0]} points (every {{stride}}th point of the frame, order kept), "
+                      f"lcps_compress2 ec=True (Zstd-22 chooser) and lcps_decompress_serial, median of {runs} "
+                      f"after one warm-up: {mc:.2f} s / {md:.2f} s; host nproc={os.cpu_count()}"}
+
+
+def upload(eng, arr):
+    ptr = eng.device_malloc(max(1, arr.nbytes))
+    eng.to_device(ptr, arr)
+    return ptr
+
+
+# ----------------------------------------------------------------------------
+# plain: one 50M-particle frame
 
 def run_plain(args, eng):
     from chaincompressor_amd import workloads
     n = args.n
     t0 = time.perf_counter()
     pts = workloads.gaussian_clusters(n, seed=workloads.BASE_SEED + 2)
-    log(f"generated {n} points in {time.perf_counter() - t0:.1f} s")
+    log(f"plain: generated {n} points in {time.perf_counter() - t0:.1f} s")
     nbytes = pts.nbytes
-    dptr = eng.device_malloc(nbytes)
-    eng.to_device(dptr, pts)
-    eb, p, thr = args.eb, args.p, args.threshold
+    dptr = upload(eng, pts)
+    p, thr = args.p, args.threshold
 
-    def compress():
-        return eng.compress_batch_device([dptr], [n], np.float32, 3, eb, p, thr)
+    def one_eb(eb, steps, warmup, timed_regions):
+        def compress():
+            return eng.compress_batch_device([dptr], [n], np.float32, 3, eb, p, thr)
 
-    # one compression to size the decode input; keep a device copy of the container
-    d_out, offs, sizes, st = compress()
-    blob = eng.to_host(d_out + offs[0], sizes[0])
-    dblob = eng.device_malloc(len(blob))
-    eng.to_device(dblob, np.frombuffer(blob, dtype=np.uint8))
+        d_out, offs, sizes, st = compress()
+        blob = eng.to_host(d_out + offs[0], sizes[0])
+        dblob = upload(eng, np.frombuffer(blob, dtype=np.uint8))
 
-    def decompress():
-        return eng.decompress_batch_device([blob], dblob)
+        def decompress():
+            return eng.decompress_batch_device([blob], dblob)
 
-    for _ in range(args.warmup):
-        compress()
-        decompress()
-    tc = td = 0.0
-    c_stages, d_stages = [], []
-    for _ in range(args.steps):
+        for _ in range(warmup):
+            compress()
+            decompress()
+        tc = td = 0.0
+        c_steps, d_steps = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            compress()
+            t1 = time.perf_counter()
+            if timed_regions:
+                c_steps.append(eng.stage_times(with_bytes=True))
+            decompress()
+            t2 = time.perf_counter()
+            if timed_regions:
+                d_steps.append(eng.stage_times(with_bytes=True))
+            tc += t1 - t0
+            td += t2 - t1
+        dec_ptr, _, _, _ = decompress()
+        dec = np.frombuffer(eng.to_host(dec_ptr, n * 3 * 8), dtype=np.float64).reshape(n, 3)
+        maxerr = float(np.max(np.abs(dec - pts)))
+        assert maxerr <= eb, maxerr
+        eng.device_free(dblob)
+        return {"compress_gbps": nbytes * steps / tc / 1e9, "decompress_gbps": nbytes * steps / td / 1e9,
+                "ms_per_step": (tc + td) / steps * 1e3, "bytes": int(sizes[0]), "st": st[0], "maxerr": maxerr,
+                "c_steps": c_steps, "d_steps": d_steps}
+
+    main = one_eb(args.eb, args.steps, args.warmup, True)
+    sweep = {}
+    for eb in (1e-2, 1e-4):
+        r = one_eb(eb, 2, 1, False)
+        sweep[f"{eb:g}"] = {"compress_gbps": round(r["compress_gbps"], 2),
+                            "decompress_gbps": round(r["decompress_gbps"], 2),
+                            "bits_per_particle": round(r["bytes"] * 8 / n, 3), "max_abs_error": r["maxerr"]}
+    eng.device_free(dptr)
+    return pts, nbytes, main, sweep
+
+
+# ----------------------------------------------------------------------------
+# batched: 256 uneven frames, one call vs one call per frame
+
+def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batched"):
+    from chaincompressor_amd import workloads
+    t0 = time.perf_counter()
+    if frames is None:
+        frames = workloads.uneven_batch(frames=args.batch_frames)
+    ns = [int(f.shape[0]) for f in frames]
+    total = sum(ns)
+    es = np.dtype(dtype).itemsize
+    nbytes = total * 3 * es
+    big = np.concatenate(frames).astype(dtype, copy=False)
+    log(f"{label}: {len(frames)} frames, {total} points ({nbytes / 1e9:.2f} GB) ready in "
+        f"{time.perf_counter() - t0:.1f} s")
+    dptr = upload(eng, big)
+    offs = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64)
+    ptrs = [dptr + int(o) * 3 * es for o in offs]
+    p, thr = args.p, args.threshold
+
+    def comp_batched():
+        return eng.compress_batch_device(ptrs, ns, dtype, 3, eb, p, thr)
+
+    # containers of the batched call, kept in a device buffer for decoding
+    d_out, coffs, csizes, _ = comp_batched()
+    streams = [eng.to_host(d_out + o, s) for o, s in zip(coffs, csizes)]
+    blob = b"".join(streams)
+    dblob = upload(eng, np.frombuffer(blob, dtype=np.uint8))
+    soffs = np.concatenate([[0], np.cumsum(csizes)[:-1]]).astype(np.int64).tolist()
+
+    # the per-frame path writes the same containers
+    identical = True
+    for k in range(len(ns)):
+        o, oo, ss, _ = eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
+        if ss[0] != csizes[k] or eng.to_host(o + oo[0], ss[0]) != streams[k]:
+            identical = False
+            break
+
+    def comp_frames():
+        for k in range(len(ns)):
+            eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
+
+    def dec_batched():
+        return eng.decompress_batch_device(streams, dblob)
+
+    def dec_frames():
+        for k in range(len(ns)):
+            eng.decompress_batch_device([streams[k]], dblob + soffs[k])
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
         t0 = time.perf_counter()
-        compress()
-        t1 = time.perf_counter()
-        c_stages.append(eng.stage_times())
-        decompress()
-        t2 = time.perf_counter()
-        d_stages.append(eng.stage_times())
-        tc += t1 - t0
-        td += t2 - t1
-    # correctness of the timed configuration (bit-exact round trip is covered by the tests)
-    dec_ptr, doffs, dns, ddims = decompress()
-    dec = np.frombuffer(eng.to_host(dec_ptr, n * 3 * 8), dtype=np.float64).reshape(n, 3)
-    maxerr = float(np.max(np.abs(dec - pts)))
+        for _ in range(steps):
+            fn()
+        return (time.perf_counter() - t0) / steps
+
+    steps = args.batch_steps
+    t_cb = timed(comp_batched, steps, 1)
+    t_cf = timed(comp_frames, steps, 1)
+    t_db = timed(dec_batched, steps, 1)
+    t_df = timed(dec_frames, steps, 1)
+    # correctness of the batched decode: every coordinate within eb
+    dptr_out, doffs, dns, _ = dec_batched()
+    dec = np.frombuffer(eng.to_host(dptr_out, total * 3 * 8), dtype=np.float64).reshape(total, 3)
+    maxerr = float(np.max(np.abs(dec - big.astype(np.float64))))
     assert maxerr <= eb, maxerr
     eng.device_free(dptr)
     eng.device_free(dblob)
-    return pts, nbytes, tc, td, c_stages, d_stages, st[0], sizes[0], maxerr
+    r = {
+        "frames": len(ns), "points": total, "input_bytes": nbytes, "eb": eb, "dtype": np.dtype(dtype).name,
+        "compress_gbps_batched": round(nbytes / t_cb / 1e9, 2),
+        "compress_gbps_per_frame": round(nbytes / t_cf / 1e9, 2),
+        "compress_speedup": round(t_cf / t_cb, 2),
+        "decompress_gbps_batched": round(nbytes / t_db / 1e9, 2),
+        "decompress_gbps_per_frame": round(nbytes / t_df / 1e9, 2),
+        "decompress_speedup": round(t_df / t_db, 2),
+        "bits_per_particle": round(sum(csizes) * 8 / total, 3),
+        "containers_identical_to_per_frame": identical,
+        "max_abs_error": maxerr, "steps": steps,
+    }
+    return r
 
+
+# ----------------------------------------------------------------------------
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
+    ap.add_argument("--workload", default="default", choices=["default", "plain", "batched", "md", "all"])
     ap.add_argument("--n", type=int, default=50_000_000)
     ap.add_argument("--eb", type=float, default=1e-3)
     ap.add_argument("--p", type=int, default=16)
     ap.add_argument("--threshold", type=int, default=2)
+    ap.add_argument("--batch-frames", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not supported"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not supported "
+                                   "(frames and requests shard across GPUs without exchange; see the design notes)"}))
         sys.exit(2)
-    from chaincompressor_amd import engine
+    from chaincompressor_amd import engine, workloads
     eng = engine.Engine(0)
-    pts, nbytes, tc, td, c_stages, d_stages, st, csize, maxerr = run_plain(args, eng)
+    wl = args.workload
     K = args.steps
-    comp_gbs = nbytes * K / tc / 1e9
-    dec_gbs = nbytes * K / td / 1e9
-    cs = agg_stages([x for s in c_stages for x in s])
-    ds = agg_stages([x for s in d_stages for x in s])
-    comp_dev_ms = sum(v[0] for v in cs.values()) / K
-    dec_dev_ms = sum(v[0] for k, v in ds.items() if k != "h2d") / K
-    # dominant kernel stage of compression and its algorithmic bytes per launch
-    n = args.n
-    dom, (dom_ms, dom_cnt) = max(cs.items(), key=lambda kv: kv[1][0])
-    per_launch_ms = dom_ms / dom_cnt
-    key_bytes = 4  # block keys of this workload fit 32 bits (see "key_bits" below)
-    alg = {
-        "sort_scatter": n * 2 * (key_bytes + 4),          # read key+index, write key+index
-        "sort_hist": n * key_bytes,
-        "quantize": n * 3 * 4 + n * key_bytes + n * 3 * 1,  # points in, keys + u8 offsets out
-        "link": n * (2 * key_bytes + 2 * 4) + n * 4,
-        "explicit": n * (4 + key_bytes),
-    }
-    achieved = alg.get(dom, 0) / (per_launch_ms / 1e3) / 1e9 if dom in alg else None
     line = {
-        "metric": "compress GB/s (one 50M-particle float32 frame, eb=1e-3; decompress GB/s alongside)",
-        "value": round(comp_gbs, 3),
+        "metric": "compress/decompress GB/s, batched vs plain at fixed eb; serving p50/p99 ms",
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": round((tc + td) / K * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Gaussian-cluster frame)",
-        "config": {"workload": f"plain single frame: {n} float32 xyz Gaussian clusters + 10% background",
-                   "eb": args.eb, "p": args.p, "threshold": args.threshold, "container": "LCH1"},
-        "compress_gbps": round(comp_gbs, 3),
-        "decompress_gbps": round(dec_gbs, 3),
-        "compress_device_ms": round(comp_dev_ms, 3),
-        "decompress_device_ms": round(dec_dev_ms, 3),
-        "compressed_bytes": int(csize),
-        "bits_per_particle": round(csize * 8 / n, 3),
-        "max_abs_error": maxerr,
-        "stages_compress_ms": {k: round(v[0] / K, 3) for k, v in cs.items()},
-        "stages_decompress_ms": {k: round(v[0] / K, 3) for k, v in ds.items()},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1) if achieved else None,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": None},
+        "data": "synthetic (seeded generators in chaincompressor_amd/workloads.py)",
     }
-    if not args.no_cpu_baseline:
-        stride = max(1, n // args.cpu_sample)
+    pts = None
+    if wl in ("default", "plain", "all"):
+        pts, nbytes, m, sweep = run_plain(args, eng)
+        n = args.n
+        line.update({
+            "value": round(m["compress_gbps"], 3),
+            "ms_per_step": round(m["ms_per_step"], 3),
+            "config": {"workload": f"plain single frame: {n} float32 xyz Gaussian clusters + 10% background",
+                       "eb": args.eb, "p": args.p, "threshold": args.threshold, "container": "LCH1",
+                       "step": "compress (HBM -> HBM) then decompress (HBM -> HBM, float64)"},
+            "compress_gbps": round(m["compress_gbps"], 3),
+            "decompress_gbps": round(m["decompress_gbps"], 3),
+            "compressed_bytes": m["bytes"],
+            "bits_per_particle": round(m["bytes"] * 8 / n, 3),
+            "max_abs_error": m["maxerr"],
+            "eb_sweep": sweep,
+            "regions_compress": region_table(m["c_steps"], K),
+            "regions_decompress": region_table(m["d_steps"], K),
+            "pipeline_alg_frac": {"compress": pipeline_fraction(m["c_steps"], K),
+                                  "decompress": pipeline_fraction(m["d_steps"], K)},
+            "roofline": roofline(m["c_steps"], m["d_steps"], K),
+        })
+    if wl in ("default", "batched", "all"):
+        line["batched"] = run_batched(args, eng)
+        if "value" not in line:
+            line["value"] = line["batched"]["compress_gbps_batched"]
+            line["config"] = {"workload": "batched: 256 uneven float32 frames", "eb": 1e-3}
+    if wl in ("md", "all"):
+        frames = workloads.md_fcc_batch()
+        line["md"] = run_batched(args, eng, frames=frames, eb=1e-4, dtype=np.float64, label="md")
+        if "value" not in line:
+            line["value"] = line["md"]["decompress_gbps_batched"]
+            line["config"] = {"workload": "md: 128 float64 fcc frames", "eb": 1e-4}
+    if pts is not None and not args.no_cpu_baseline:
+        stride = max(1, args.n // args.cpu_sample)
         sample = np.ascontiguousarray(pts[::stride][:args.cpu_sample])
         cb = cpu_baseline(sample, args.eb, args.p, args.threshold)
         cb["sample"] = cb["sample"].replace("{stride}", str(stride))

@@ -82,10 +82,12 @@ struct StageTimer {
     bool on = false;
     hipStream_t s = nullptr;
     std::vector<std::string> names;
+    std::vector<uint64_t> bytes;  // algorithmic HBM bytes of each timed region (0: not stated)
     std::vector<hipEvent_t> ev;   // 2 per stage
     size_t used = 0;
-    void begin(const char* name) {
-        if (!on) return;
+    // returns the index of the timed region (or -1 when timing is off)
+    int begin(const char* name, uint64_t alg_bytes = 0) {
+        if (!on) return -1;
         if (used + 2 > ev.size()) {
             for (int k = 0; k < 2; ++k) {
                 hipEvent_t e;
@@ -93,14 +95,23 @@ struct StageTimer {
                 ev.push_back(e);
             }
         }
-        if (names.size() < used / 2 + 1) names.resize(used / 2 + 1);
-        names[used / 2] = name;
+        const size_t i = used / 2;
+        if (names.size() < i + 1) {
+            names.resize(i + 1);
+            bytes.resize(i + 1);
+        }
+        names[i] = name;
+        bytes[i] = alg_bytes;
         HIPCHK(hipEventRecord(ev[used], s));
+        return (int)i;
     }
     void end() {
         if (!on) return;
         HIPCHK(hipEventRecord(ev[used + 1], s));
         used += 2;
+    }
+    void set_bytes(int i, uint64_t b) {
+        if (i >= 0 && (size_t)i < bytes.size()) bytes[i] = b;
     }
     void reset() { used = 0; }
     ~StageTimer() {
@@ -217,19 +228,23 @@ static void scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t
     hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)tiles), dim3(CH_WG), 0, st, in, out, n, status, ticket, err, total);
 }
 
+// timed regions of the entropy stage whose algorithmic bytes are known only
+// once the stage has run (they depend on the chosen encodings)
+struct EntropyTimes {
+    int stats[5], encode[5], assemble, pack[5];
+};
+
 template <class X>
-static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D, bool rel8, const FrameDev* fdev,
+static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint32_t F, uint32_t D, bool rel8, const FrameDev* fdev,
                           FrameOut* fo, const uint32_t* idrel, const uint64_t* diff, const uint64_t* keep,
                           const uint8_t* choice2, const void* rel, const uint64_t* corr_idx, const double* corr_val,
-                          const uint32_t* tb, uint32_t ntiles, const std::vector<uint32_t>& tile1k,
+                          const uint32_t* tb, uint32_t ntiles,
                           uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const RansConfig& rc, uint64_t stride,
                           uint32_t* err, uint32_t* ticket, X& x) {
-    tm.begin("comp_stats");
     const uint32_t ninst = 5 * F, nsec = 10 * F;
     hipLaunchKernelGGL(k_comp_init, dim3(cdiv(ninst, CH_WG)), dim3(CH_WG), 0, st, fdev, fo, F, D, x.ci);
     fill32(st, x.vhist, 5ull * F * RANS_VALUE_LIMIT, 0);
     fill32(st, x.thist, 5ull * F * RANS_TAG_LIMIT, 0);
-    HIPCHK(hipMemcpyAsync(x.tb1k, tile1k.data(), (F + 1) * 4, hipMemcpyHostToDevice, st));
     // every component: a typed view and its launches
     auto each_comp = [&](auto&& fn) {
         with_dims(D, [&](auto dd) {
@@ -245,27 +260,30 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D
     each_comp([&](int c, auto vt, auto dd, const void* v) {
         using V = typename decltype(vt)::type;
         constexpr int DC = decltype(dd)::value;
+        et.stats[c] = tm.begin("comp_stats");
         hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
                            x.vhist, x.thist);
+        tm.end();
     });
-    tm.end();
-    tm.begin("rans_table");
+    tm.begin("rans_table", 5ull * F * (RANS_VALUE_LIMIT + RANS_TAG_LIMIT) * 4 + 10ull * F * RANS_VALUE_LIMIT * 8);
     hipLaunchKernelGGL(k_rans_table, dim3(nsec), dim3(CH_WG), 0, st, x.ci, x.sec, x.vhist, x.thist, F, x.pc,
                        x.tbytes, err);
     hipLaunchKernelGGL(k_rans_plan, dim3(1), dim3(1), 0, st, x.sec, nsec, F, rc.lanes_max, rc.chunk_symbols,
                        x.comp_chunk);
     tm.end();
-    tm.begin("rans_encode");
     each_comp([&](int c, auto vt, auto dd, const void* v) {
         using V = typename decltype(vt)::type;
         constexpr int DC = decltype(dd)::value;
-        if (comp_chunk_bound[c])
+        et.encode[c] = -1;
+        if (comp_chunk_bound[c]) {
+            et.encode[c] = tm.begin("rans_encode");
             hipLaunchKernelGGL((k_rans_encode<V, DC>), dim3((unsigned)comp_chunk_bound[c]), dim3(64), 0, st,
                                (const V*)v, c, F, x.ci, x.sec, x.comp_chunk, x.pc, x.scratch, stride,
                                x.chunk_len, x.chunk_pos, err);
+            tm.end();
+        }
     });
-    tm.end();
-    tm.begin("assemble");
+    et.assemble = tm.begin("assemble");
     hipLaunchKernelGGL(k_section_sizes, dim3(nsec), dim3(CH_WG), 0, st, x.sec, x.chunk_len, x.chunk_off);
     hipLaunchKernelGGL(k_choose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst);
     hipLaunchKernelGGL(k_layout, dim3(1), dim3(1), 0, st, x.ci, x.sec, fo, F, D, x.out_off, x.out_size, x.total,
@@ -280,10 +298,10 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D
         hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunk_bound), dim3(CH_WG), 0, st, x.ci, x.sec, nsec,
                            x.comp_chunk, x.scratch, stride, x.chunk_len, x.chunk_pos, x.chunk_off, x.out);
     tm.end();
-    tm.begin("pack");
     each_comp([&](int c, auto vt, auto dd, const void* v) {
         using V = typename decltype(vt)::type;
         constexpr int DC = decltype(dd)::value;
+        et.pack[c] = tm.begin("pack");
         hipLaunchKernelGGL((k_pack_fixed<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
                            x.out);
         fill64(st, x.status1k, ntiles1k, 0);
@@ -292,8 +310,8 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, uint32_t F, uint32_t D
                            x.tag_tiles, x.ci, x.status1k, ticket, err, x.head, x.tail, x.head_pos, x.out);
         hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG)), dim3(CH_WG), 0, st, x.head, x.tail, x.head_pos,
                            x.tag_tiles + c * (F + 1) + F, x.out);
+        tm.end();
     });
-    tm.end();
 }
 
 static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
@@ -456,7 +474,6 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             uint64_t* out_off;
             uint64_t* out_size;
             uint64_t* total;
-            uint32_t* tb1k;
             uint8_t* head;
             uint8_t* tail;
             uint64_t* head_pos;
@@ -478,7 +495,6 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.out_off = c.take<uint64_t>(F);
         x.out_size = c.take<uint64_t>(F);
         x.total = c.take<uint64_t>(1);
-        x.tb1k = c.take<uint32_t>(F + 1);
         x.head = c.take<uint8_t>(ntiles1k);
         x.tail = c.take<uint8_t>(ntiles1k);
         x.head_pos = c.take<uint64_t>(ntiles1k);
@@ -520,7 +536,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     HIPCHK(hipMemcpyAsync(A.ptb, ptb32.data(), (F + 1) * 4, hipMemcpyHostToDevice, st));
     fill32(st, A.bad, F, 0);
     fill32(st, A.err, 4, 0);
-    tm.begin("bbox");
+    tm.begin("bbox", T * D * es);
     with_dtype(dtype, [&](auto tt) {
         using T_ = typename decltype(tt)::type;
         with_dims(D, [&](auto dd) {
@@ -602,11 +618,12 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     auto B = carve_b(cB, kbytes, rbytes, nwords);
     HIPCHK(hipMemcpyAsync(A.fdev, fdev.data(), F * sizeof(FrameDev), hipMemcpyHostToDevice, st));
 
+    int t_explicit = -1, t_l2p = -1;
     auto run_typed = [&](auto key_tag, auto rel_tag) {
         using KeyT = typename decltype(key_tag)::type;
         using RelT = typename decltype(rel_tag)::type;
         // K2 quantise
-        tm.begin("quantize");
+        tm.begin("quantize", T * D * es + T * sizeof(KeyT) * nwords + T * D * sizeof(RelT));
         with_dtype(dtype, [&](auto tt) {
             using T_ = typename decltype(tt)::type;
             with_dims(D, [&](auto dd) {
@@ -641,14 +658,15 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             }
             const uint32_t shift = 8 * ps;
             const uint32_t ps_all = done_passes++;
-            tm.begin("sort_hist");
+            tm.begin("sort_hist", T * sizeof(KeyT));
             hipLaunchKernelGGL((k_sort_hist<KeyT>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin, T, shift,
                                (uint32_t)sort_tiles, B.counts);
             tm.end();
-            tm.begin("sort_scan");
+            tm.begin("sort_scan", 2ull * 256 * sort_tiles * 4);
             scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status, B.ticket, A.err, nullptr);
             tm.end();
-            tm.begin("sort_scatter");
+            tm.begin("sort_scatter", ps_all == 0 ? T * sizeof(KeyT) + T * (sizeof(KeyT) + 4)
+                                                 : 2 * T * (sizeof(KeyT) + 4));
             if (ps_all == 0)
                 hipLaunchKernelGGL((k_sort_scatter<KeyT, true>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
                                    vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
@@ -662,7 +680,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             vout = (vout == B.va) ? B.vb : B.va;
         }
         // K4 link + decision
-        tm.begin("link");
+        tm.begin("link", T * (sizeof(KeyT) * nwords + 8));
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
             if (nwords == 1)
@@ -674,7 +692,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         });
         tm.end();
         // K5 explicit compaction
-        tm.begin("explicit");
+        t_explicit = tm.begin("explicit");
         fill64(st, B.status, ntiles, 0);
         fill32(st, B.ticket, 1, 0);
         with_dims(D, [&](auto dd) {
@@ -694,7 +712,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     else with_rel(TypeTag<uint32_t>{});
 
     // K6/K7 L2
-    tm.begin("l2_stats");
+    const int t_l2s = tm.begin("l2_stats");
     // frame record bases as a flat array for the record-space kernels
     hipLaunchKernelGGL(k_copy_rec_base, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, B.rec_base);
     const uint64_t rec_tiles_max = cdiv(T, CH_TILE);
@@ -704,7 +722,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                            &B.totals[0], F, B.rec_base, B.fo);
         hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
         tm.end();
-        tm.begin("l2_split");
+        t_l2p = tm.begin("l2_split");
         fill64(st, B.status, rec_tiles_max, 0);
         fill32(st, B.ticket, 1, 0);
         hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)rec_tiles_max), dim3(CH_WG), 0, st, B.blocks,
@@ -716,7 +734,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     tm.end();
 
     // corrections: per-tile counts -> offsets -> ordered gather
-    tm.begin("corrections");
+    const int t_corr = tm.begin("corrections");
     scan_u32(st, B.tile_ncorr, B.tile_corr_off, ntiles, B.status, B.ticket, A.err, &B.totals[2]);
     with_dims(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
@@ -727,12 +745,21 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                        &B.totals[2]);
     tm.end();
 
+    const size_t kbytes_all = kbytes * nwords;
+    auto set_pre_bytes = [&](uint64_t R, uint64_t NC) {
+        tm.set_bytes(t_explicit, T * (4 + kbytes_all) + 8ull * D * R);
+        tm.set_bytes(t_l2s, 8ull * D * R);
+        tm.set_bytes(t_l2p, 16ull * D * R + R);
+        tm.set_bytes(t_corr, 32ull * NC + 8ull * ntiles);
+    };
+
     if (format == CHAIN_FMT_LCH1) {
         Carver cC;
         cC.base = e->arena.base + size_a + size_b;
         auto X = carve_c(cC);
-        entropy_stage(st, tm, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
-                      B.corr_val, A.tb, ntiles, tile1k, ntiles1k, comp_chunk_bound, rc, stride, A.err, B.ticket, X);
+        EntropyTimes et;
+        entropy_stage(st, tm, et, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
+                      B.corr_val, A.tb, ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err, B.ticket, X);
         std::vector<uint64_t> offs(F), sizes(F);
         uint64_t total = 0;
         uint32_t errw[4];
@@ -742,9 +769,48 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         HIPCHK(hipMemcpyAsync(errw, A.err, sizeof errw, hipMemcpyDeviceToHost, st));
         std::vector<CompInst> ci(5 * F);
         std::vector<FrameOut> fo(F);
+        std::vector<Section> secs(tm.on ? 10 * F : 0);
         HIPCHK(hipMemcpyAsync(ci.data(), X.ci, 5 * F * sizeof(CompInst), hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(fo.data(), B.fo, F * sizeof(FrameOut), hipMemcpyDeviceToHost, st));
+        if (tm.on) HIPCHK(hipMemcpyAsync(secs.data(), X.sec, 10 * F * sizeof(Section), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        if (tm.on) {
+            // algorithmic bytes of the regions whose work depends on the data
+            uint64_t R = 0, NC = 0;
+            for (uint32_t f = 0; f < F; ++f) {
+                R += fo[f].nrec;
+                NC += fo[f].ncorr;
+            }
+            set_pre_bytes(R, NC);
+            const uint64_t esz[5] = {4, 8, 8, 1, rbytes};
+            uint64_t copied = 0;
+            for (int c = 0; c < 5; ++c) {
+                uint64_t in_b = 0, enc_b = 0, pack_b = 0;
+                for (uint32_t f = 0; f < F; ++f) {
+                    const CompInst& I = ci[c * F + f];
+                    const uint64_t vb = I.n * I.dims * esz[c];
+                    in_b += vb;
+                    for (int sx = 0; sx < 2; ++sx) {
+                        const Section& S = secs[2 * (c * F + f) + sx];
+                        if (S.nchunks) {
+                            enc_b += vb + S.payload;
+                            copied += S.payload;
+                        }
+                    }
+                    if (I.kind == KIND_FIXP) {
+                        uint64_t w = 0;
+                        for (uint32_t d = 0; d < I.dims; ++d) w += I.fix_bytes[d];
+                        pack_b += vb + w;
+                    } else if (I.kind == KIND_TAGR) {
+                        pack_b += vb + (I.tag_bits + 7) / 8;
+                    }
+                }
+                tm.set_bytes(et.stats[c], in_b);
+                tm.set_bytes(et.encode[c], enc_b);
+                tm.set_bytes(et.pack[c], pack_b);
+            }
+            tm.set_bytes(et.assemble, 2 * copied);
+        }
         require(errw[0] == 0, CHAIN_EDEVICE, "device error word set in the entropy stage");
         require(total <= out_bound, CHAIN_EDEVICE, "container exceeds its bound");
         res.stats.resize(F);
@@ -783,6 +849,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     require(errw[0] == 0, CHAIN_EDEVICE, "device error word set (look-back wait limit or invalid block range)");
 
     const uint64_t R = totals[0], ND = totals[1], NC = totals[2];
+    set_pre_bytes(R, NC);
     std::vector<uint32_t> h_idrel(T);
     std::vector<uint8_t> h_choice(R);
     std::vector<uint64_t> h_diff(ND * D), h_keep((R - ND) * D);
@@ -1226,7 +1293,19 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     auto h2d = [&](void* dst, const void* src, size_t n) {
         if (n) HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
     };
-    tm.begin("h2d");
+    // algorithmic bytes of every timed region, from the plan
+    const uint64_t comp_es[5] = {4, 8, 8, 1, 4};
+    auto es_of = [&](int32_t comp) -> uint64_t { return comp < 0 ? 1 : comp_es[comp]; };
+    uint64_t b_rans = 0, b_fixed = 0, b_raw = 0, b_tagged = 0;
+    for (const DecChunk& ch : P.chunks) b_rans += ch.bytes;
+    for (const DecSection& sc : P.secs) b_rans += sc.count * es_of(sc.comp);
+    for (const DecFixed& j : P.fixed) {
+        for (uint32_t d = 0; d < j.dims; ++d) b_fixed += (j.n * j.width[d] + 7) / 8;
+        b_fixed += j.n * j.dims * es_of(j.comp);
+    }
+    for (const DecRaw& j : P.raws) b_raw += j.count * (8 + es_of(j.comp));
+    for (const DecTagged& j : P.tagged) b_tagged += j.bytes + j.n + j.n * j.dims * es_of(j.comp);
+    tm.begin("h2d", res.d_pool ? 0 : pool_size);
     if (!res.d_pool)
         for (uint32_t k = 0; k < nstreams; ++k) h2d(X.in + in_off[k], bytes[k], lens[k]);
     h2d(X.fr, P.frames.data(), F * sizeof(DecFrame));
@@ -1247,51 +1326,66 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     fill32(st, X.err, 4, 0);
     tm.end();
 
-    tm.begin("rans_decode");
+    tm.begin("rans_decode", b_rans);
     if (!P.chunks.empty())
         hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.chunks.size()), dim3(64), 0, st, X.in, X.secs, X.chunks,
                            X.pc, X.slots, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
     tm.end();
-    tm.begin("unpack");
-    if (P.fixed_total)
+    if (P.fixed_total) {
+        tm.begin("unpack_fixed", b_fixed);
         hipLaunchKernelGGL(k_dec_fixed, dim3(grid_for(P.fixed_total)), dim3(CH_WG), 0, st, X.in, X.fixed,
                            (uint32_t)P.fixed.size(), P.fixed_total, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
-    if (P.raw_total)
+        tm.end();
+    }
+    if (P.raw_total) {
+        tm.begin("unpack_raw", b_raw);
         hipLaunchKernelGGL(k_dec_raw, dim3(grid_for(P.raw_total)), dim3(CH_WG), 0, st, X.in, X.raws,
                            (uint32_t)P.raws.size(), P.raw_total, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+        tm.end();
+    }
     if (P.tag_tiles) {
+        tm.begin("unpack_tagged", b_tagged);
         fill64(st, X.status, P.tag_tiles, 0);
         fill32(st, X.ticket, 1, 0);
         hipLaunchKernelGGL(k_dec_tagged, dim3((unsigned)P.tag_tiles), dim3(CH_WG), 0, st, X.in, X.tagged,
                            (uint32_t)P.tagged.size(), X.tags, X.status, X.ticket, X.idrel, X.diff, X.keep, X.choice,
                            X.rel, X.err);
+        tm.end();
     }
-    tm.end();
-    tm.begin("l2_inverse");
     if (nrt) {
+        tm.begin("l2_counts", P.R + 8ull * nrt);
         hipLaunchKernelGGL(k_dec_tile_zero_counts, dim3(nrt), dim3(CH_WG), 0, st, X.choice, (const uint32_t*)nullptr,
                            X.rtb, X.sbr, X.snr, F, X.zc);
         scan_u32(st, X.zc, X.zoff, nrt, X.status, X.ticket, X.err, &X.totals[0]);
         fill32(st, X.flags, nrt + 1, 0);
         fill32(st, X.ticket, 1, 0);
+        tm.end();
+        tm.begin("l2_inverse", P.R + 8ull * D * (P.ND + P.NK) + 8ull * D * P.R);
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
             hipLaunchKernelGGL((k_dec_l2<DD>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice, X.diff,
                                X.keep, X.zoff, X.flags, X.aggv, X.ticket, X.compact, X.err);
         });
+        tm.end();
     }
-    tm.end();
-    tm.begin("chain");
+    tm.begin("chain_counts", 4 * T + 8ull * npt);
     hipLaunchKernelGGL(k_dec_tile_zero_counts, dim3(npt), dim3(CH_WG), 0, st, (const uint8_t*)nullptr, X.idrel, X.ptb,
                        X.sbp, X.snp, F, X.ec);
     scan_u32(st, X.ec, X.eoff, npt, X.status, X.ticket, X.err, &X.totals[1]);
     hipLaunchKernelGGL(k_dec_check_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.eoff,
                        &X.totals[1], X.err);
+    tm.end();
+    // the O(N) algorithmic traffic of chain resolution (read successors,
+    // write roots) is stated once, on the local pass; the jump rounds over
+    // tile seams are extra work of this formulation
+    tm.begin("chain_local", 8 * T);
     with_dims(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
         hipLaunchKernelGGL((k_dec_chain_local<DD>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.idrel, X.eoff,
                            X.root, X.err);
     });
+    tm.end();
+    tm.begin("chain_jump");
     {
         uint32_t maxt = 1;
         for (uint32_t f = 0; f < F; ++f) maxt = std::max(maxt, ptb[f + 1] - ptb[f]);
@@ -1301,7 +1395,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             hipLaunchKernelGGL(k_dec_chain_jump, dim3(grid_for(T)), dim3(CH_WG), 0, st, X.root, T);
     }
     tm.end();
-    tm.begin("dequantize");
+    tm.begin("dequantize", 4 * T + 4 * T * D + 8ull * D * P.R + T * D * es);
     with_dims(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
         auto go = [&](auto ot) {
@@ -1573,6 +1667,15 @@ int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, siz
         std::strncpy(names, all.c_str(), names_len - 1);
         names[names_len - 1] = 0;
     }
+    return k;
+}
+
+int chain_last_stage_bytes(chain_engine* e, uint64_t* bytes, int max) {
+    if (!e) return 0;
+    std::lock_guard<std::mutex> lock(e->mu);
+    const int k = (int)(e->timer.used / 2);
+    for (int i = 0; i < k && i < max; ++i)
+        if (bytes) bytes[i] = e->timer.bytes[i];
     return k;
 }
 

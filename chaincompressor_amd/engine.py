@@ -51,7 +51,8 @@ class _Stats(C.Structure):
 EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "chain_engine_stream",
            "chain_buf_free", "chain_compress", "chain_compress_batch", "chain_compress_batch_device",
            "chain_engine_set_rans", "chain_decompress", "chain_decompress_batch", "chain_decompress_batch_device",
-           "chain_device_malloc", "chain_device_free", "chain_copy", "chain_last_stage_times"]
+           "chain_device_malloc", "chain_device_free", "chain_copy", "chain_last_stage_times",
+           "chain_last_stage_bytes"]
 
 _LIB = None
 
@@ -90,6 +91,7 @@ def load_library():
     L.chain_device_free.argtypes = [C.c_void_p, C.c_void_p]
     L.chain_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     L.chain_last_stage_times.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_char_p, C.c_size_t]
+    L.chain_last_stage_bytes.argtypes = [C.c_void_p, P(C.c_uint64), C.c_int]
     _LIB = L
     return L
 
@@ -283,9 +285,14 @@ class Engine:
         self._check(self._L.chain_copy(self._h, out, C.c_void_p(ptr), nbytes, 0))
         return bytes(out)[:nbytes]
 
-    def stage_times(self):
-        ms = (C.c_float * 64)()
-        names = C.create_string_buffer(4096)
-        k = self._L.chain_last_stage_times(self._h, ms, 64, names, 4096)
+    def stage_times(self, with_bytes=False):
+        """[(stage, ms)] of the last call, or [(stage, ms, algorithmic bytes)]."""
+        ms = (C.c_float * 128)()
+        names = C.create_string_buffer(8192)
+        k = self._L.chain_last_stage_times(self._h, ms, 128, names, 8192)
         labels = names.value.decode().split(";") if k else []
-        return [(labels[i], float(ms[i])) for i in range(k)]
+        if not with_bytes:
+            return [(labels[i], float(ms[i])) for i in range(k)]
+        b = (C.c_uint64 * 128)()
+        self._L.chain_last_stage_bytes(self._h, b, 128)
+        return [(labels[i], float(ms[i]), int(b[i])) for i in range(k)]

@@ -131,6 +131,12 @@ int chain_copy(chain_engine* e, void* dst, const void* src, uint64_t bytes, int 
  * stream) of the last call. names is a ';'-separated list. Returns the count. */
 int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, size_t names_len);
 
+/* Algorithmic HBM bytes of the same timed regions (0 where none is stated):
+ * every input a region must read and every output it must write, counted
+ * once, so bytes / time is the region's achieved bandwidth against HBM peak.
+ * Returns the count. */
+int chain_last_stage_bytes(chain_engine* e, uint64_t* bytes, int max);
+
 #ifdef __cplusplus
 }
 #endif

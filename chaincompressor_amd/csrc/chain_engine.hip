@@ -119,6 +119,65 @@ struct StageTimer {
     }
 };
 
+// Pinned host staging for the small copies of a call (plans, tables,
+// results): one bump allocation per copy, so H2D copies are true async DMA
+// and results need one synchronisation. Blocks outgrown during a call are
+// released at the next reset, after the stream has drained.
+struct Staging {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+    std::vector<char*> retired;
+    struct Pending {
+        void* dst;
+        const char* src;
+        size_t n;
+    };
+    std::vector<Pending> pend;
+    void reset(hipStream_t st) {
+        (void)hipStreamSynchronize(st);
+        for (char* r : retired) (void)hipHostFree(r);
+        retired.clear();
+        pend.clear();
+        off = 0;
+    }
+    char* take(size_t n) {
+        n = (n + 63) & ~size_t(63);
+        if (off + n > cap) {
+            if (base) retired.push_back(base);
+            base = nullptr;
+            const size_t want = std::max<size_t>(n, std::max<size_t>(2 * cap, size_t(1) << 20));
+            HIPCHK(hipHostMalloc((void**)&base, want, hipHostMallocDefault));
+            cap = want;
+            off = 0;
+        }
+        char* q = base + off;
+        off += n;
+        return q;
+    }
+    void h2d(hipStream_t st, void* dst, const void* src, size_t n) {
+        if (!n) return;
+        char* q = take(n);
+        std::memcpy(q, src, n);
+        HIPCHK(hipMemcpyAsync(dst, q, n, hipMemcpyHostToDevice, st));
+    }
+    // lands in dst at the next finish()
+    void d2h(hipStream_t st, void* dst, const void* src, size_t n) {
+        if (!n) return;
+        char* q = take(n);
+        HIPCHK(hipMemcpyAsync(q, src, n, hipMemcpyDeviceToHost, st));
+        pend.push_back({dst, q, n});
+    }
+    void finish(hipStream_t st) {
+        HIPCHK(hipStreamSynchronize(st));
+        for (const Pending& p : pend) std::memcpy(p.dst, p.src, p.n);
+        pend.clear();
+    }
+    ~Staging() {
+        for (char* r : retired) (void)hipHostFree(r);
+        if (base) (void)hipHostFree(base);
+    }
+};
+
 // RVC1 parameters of the containers an engine writes (recorded in every section)
 struct RansConfig {
     uint32_t lanes_max = 64;
@@ -136,6 +195,7 @@ struct chain_engine {
     std::mutex mu;
     std::string err;
     chain::StageTimer timer;
+    chain::Staging stage;
     chain::RansConfig rans;
 };
 
@@ -231,7 +291,7 @@ static void scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t
 // timed regions of the entropy stage whose algorithmic bytes are known only
 // once the stage has run (they depend on the chosen encodings)
 struct EntropyTimes {
-    int stats[5], encode[5], assemble, pack[5];
+    int stats[5], encode, assemble, pack[5];
 };
 
 template <class X>
@@ -271,18 +331,24 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
     hipLaunchKernelGGL(k_rans_plan, dim3(1), dim3(1), 0, st, x.sec, nsec, F, rc.lanes_max, rc.chunk_symbols,
                        x.comp_chunk);
     tm.end();
-    each_comp([&](int c, auto vt, auto dd, const void* v) {
-        using V = typename decltype(vt)::type;
-        constexpr int DC = decltype(dd)::value;
-        et.encode[c] = -1;
-        if (comp_chunk_bound[c]) {
-            et.encode[c] = tm.begin("rans_encode");
-            hipLaunchKernelGGL((k_rans_encode<V, DC>), dim3((unsigned)comp_chunk_bound[c]), dim3(64), 0, st,
-                               (const V*)v, c, F, x.ci, x.sec, x.comp_chunk, x.pc, x.scratch, stride,
-                               x.chunk_len, x.chunk_pos, err);
-            tm.end();
-        }
-    });
+    const uint64_t chunk_bound = comp_chunk_bound[0] + comp_chunk_bound[1] + comp_chunk_bound[2] +
+                                 comp_chunk_bound[3] + comp_chunk_bound[4];
+    et.encode = -1;
+    if (chunk_bound) {
+        et.encode = tm.begin("rans_encode");
+        with_dims(D, [&](auto dd) {
+            constexpr int DD = decltype(dd)::value;
+            if (rel8)
+                hipLaunchKernelGGL((k_rans_encode<DD, uint8_t>), dim3((unsigned)chunk_bound), dim3(64), 0, st, idrel,
+                                   diff, keep, choice2, (const uint8_t*)rel, F, x.ci, x.sec, x.comp_chunk, x.pc,
+                                   x.scratch, stride, x.chunk_len, x.chunk_pos);
+            else
+                hipLaunchKernelGGL((k_rans_encode<DD, uint32_t>), dim3((unsigned)chunk_bound), dim3(64), 0, st,
+                                   idrel, diff, keep, choice2, (const uint32_t*)rel, F, x.ci, x.sec, x.comp_chunk,
+                                   x.pc, x.scratch, stride, x.chunk_len, x.chunk_pos);
+        });
+        tm.end();
+    }
     et.assemble = tm.begin("assemble");
     hipLaunchKernelGGL(k_section_sizes, dim3(nsec), dim3(CH_WG), 0, st, x.sec, x.chunk_len, x.chunk_off);
     hipLaunchKernelGGL(k_choose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst);
@@ -292,8 +358,6 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                        corr_val, x.out);
     hipLaunchKernelGGL(k_write_components, dim3(ninst), dim3(CH_WG), 0, st, x.ci, x.sec, F, x.tbytes, x.chunk_len,
                        x.out);
-    const uint64_t chunk_bound = comp_chunk_bound[0] + comp_chunk_bound[1] + comp_chunk_bound[2] +
-                                 comp_chunk_bound[3] + comp_chunk_bound[4];
     if (chunk_bound)
         hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunk_bound), dim3(CH_WG), 0, st, x.ci, x.sec, nsec,
                            x.comp_chunk, x.scratch, stride, x.chunk_len, x.chunk_pos, x.chunk_off, x.out);
@@ -325,6 +389,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     hipStream_t st = e->stream;
     StageTimer& tm = e->timer;
     tm.reset();
+    e->stage.reset(st);
 
     std::vector<FramePlan> plan(F);
     std::vector<uint64_t> pt_off(F + 1, 0);
@@ -531,9 +596,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     }
     std::vector<uint32_t> ptb32(F + 1);
     for (uint32_t f = 0; f <= F; ++f) ptb32[f] = (uint32_t)pt_off[f];
-    HIPCHK(hipMemcpyAsync(A.fdev, fdev.data(), F * sizeof(FrameDev), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(A.tb, tile_begin.data(), (F + 1) * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(A.ptb, ptb32.data(), (F + 1) * 4, hipMemcpyHostToDevice, st));
+    Staging& sg = e->stage;
+    sg.h2d(st, A.fdev, fdev.data(), F * sizeof(FrameDev));
+    sg.h2d(st, A.tb, tile_begin.data(), (F + 1) * 4);
+    sg.h2d(st, A.ptb, ptb32.data(), (F + 1) * 4);
     fill32(st, A.bad, F, 0);
     fill32(st, A.err, 4, 0);
     tm.begin("bbox", T * D * es);
@@ -548,9 +614,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     tm.end();
     std::vector<uint64_t> box((size_t)F * D * 4);
     std::vector<uint32_t> bad(F);
-    HIPCHK(hipMemcpyAsync(box.data(), A.box, box.size() * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(bad.data(), A.bad, F * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    sg.d2h(st, box.data(), A.box, box.size() * 8);
+    sg.d2h(st, bad.data(), A.bad, F * 4);
+    sg.finish(st);
 
     // ---- plan: minima, quantised range, key layout (lcps_cpp.cpp:2072-2110)
     uint32_t key_bits_max = 0, nwords = 1;
@@ -616,7 +682,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     Carver cB;
     cB.base = e->arena.base + size_a;
     auto B = carve_b(cB, kbytes, rbytes, nwords);
-    HIPCHK(hipMemcpyAsync(A.fdev, fdev.data(), F * sizeof(FrameDev), hipMemcpyHostToDevice, st));
+    sg.h2d(st, A.fdev, fdev.data(), F * sizeof(FrameDev));
 
     int t_explicit = -1, t_l2p = -1;
     auto run_typed = [&](auto key_tag, auto rel_tag) {
@@ -763,17 +829,17 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         std::vector<uint64_t> offs(F), sizes(F);
         uint64_t total = 0;
         uint32_t errw[4];
-        HIPCHK(hipMemcpyAsync(offs.data(), X.out_off, F * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(sizes.data(), X.out_size, F * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&total, X.total, 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(errw, A.err, sizeof errw, hipMemcpyDeviceToHost, st));
+        sg.d2h(st, offs.data(), X.out_off, F * 8);
+        sg.d2h(st, sizes.data(), X.out_size, F * 8);
+        sg.d2h(st, &total, X.total, 8);
+        sg.d2h(st, errw, A.err, sizeof errw);
         std::vector<CompInst> ci(5 * F);
         std::vector<FrameOut> fo(F);
         std::vector<Section> secs(tm.on ? 10 * F : 0);
-        HIPCHK(hipMemcpyAsync(ci.data(), X.ci, 5 * F * sizeof(CompInst), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(fo.data(), B.fo, F * sizeof(FrameOut), hipMemcpyDeviceToHost, st));
-        if (tm.on) HIPCHK(hipMemcpyAsync(secs.data(), X.sec, 10 * F * sizeof(Section), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        sg.d2h(st, ci.data(), X.ci, 5 * F * sizeof(CompInst));
+        sg.d2h(st, fo.data(), B.fo, F * sizeof(FrameOut));
+        if (tm.on) sg.d2h(st, secs.data(), X.sec, 10 * F * sizeof(Section));
+        sg.finish(st);
         if (tm.on) {
             // algorithmic bytes of the regions whose work depends on the data
             uint64_t R = 0, NC = 0;
@@ -783,7 +849,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             }
             set_pre_bytes(R, NC);
             const uint64_t esz[5] = {4, 8, 8, 1, rbytes};
-            uint64_t copied = 0;
+            uint64_t copied = 0, enc_all = 0;
             for (int c = 0; c < 5; ++c) {
                 uint64_t in_b = 0, enc_b = 0, pack_b = 0;
                 for (uint32_t f = 0; f < F; ++f) {
@@ -806,9 +872,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                     }
                 }
                 tm.set_bytes(et.stats[c], in_b);
-                tm.set_bytes(et.encode[c], enc_b);
+                enc_all += enc_b;
                 tm.set_bytes(et.pack[c], pack_b);
             }
+            tm.set_bytes(et.encode, enc_all);
             tm.set_bytes(et.assemble, 2 * copied);
         }
         require(errw[0] == 0, CHAIN_EDEVICE, "device error word set in the entropy stage");
@@ -842,10 +909,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     std::vector<FrameOut> fo(F);
     uint64_t totals[4];
     uint32_t errw[4];
-    HIPCHK(hipMemcpyAsync(fo.data(), B.fo, F * sizeof(FrameOut), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(totals, B.totals, sizeof totals, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(errw, A.err, sizeof errw, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    sg.d2h(st, fo.data(), B.fo, F * sizeof(FrameOut));
+    sg.d2h(st, totals, B.totals, sizeof totals);
+    sg.d2h(st, errw, A.err, sizeof errw);
+    sg.finish(st);
     require(errw[0] == 0, CHAIN_EDEVICE, "device error word set (look-back wait limit or invalid block range)");
 
     const uint64_t R = totals[0], ND = totals[1], NC = totals[2];
@@ -942,8 +1009,8 @@ struct DecPlan {
     std::vector<DecFixed> fixed;
     std::vector<DecRaw> raws;
     std::vector<DecTagged> tagged;
-    std::vector<uint64_t> pc;   // (cum << 32) | freq
-    std::vector<uint16_t> slots;
+    std::vector<DecJob> jobs;
+    std::vector<uint32_t> cum;  // per section: alphabet + 1 cumulative frequencies
     uint64_t T = 0, R = 0, ND = 0, NK = 0, tags = 0, tag_tiles = 0, fixed_total = 0, raw_total = 0;
     uint32_t D = 0;
 };
@@ -992,26 +1059,28 @@ static void parse_rvc1(HostReader r, uint64_t expected, int comp, uint64_t dst, 
     sec.alphabet = (uint32_t)A;
     sec.chunk_symbols = S;
     sec.count = expected;
-    sec.table_off = P.pc.size();
+    sec.cum_off = P.cum.size();
     sec.comp = comp;
     sec.dst = dst;
     uint32_t sum = 0;
     for (uint64_t k = 0; k < A; ++k) {
         require(prob[k] <= M - sum, CHAIN_ERUNTIME, "invalid rANS probability sum");
-        P.pc.push_back(((uint64_t)sum << 32) | prob[k]);
+        P.cum.push_back(sum);
         sum += prob[k];
     }
+    P.cum.push_back(sum);
     require(sum == M, CHAIN_ERUNTIME, "incomplete rANS probability table");
-    if (M <= (1u << 14)) {
-        sec.slot_off = P.slots.size();
-        for (uint64_t k = 0; k < A; ++k)
-            for (uint32_t j = 0; j < prob[k]; ++j) P.slots.push_back((uint16_t)k);
-    } else {
-        sec.slot_off = ~0ull;
-    }
     const uint32_t si = (uint32_t)P.secs.size();
     P.secs.push_back(sec);
     const uint8_t* sizes = r.bytes(4ull * nchunks);
+    for (uint32_t k = 0; k < nchunks; k += DEC_WAVES) {
+        DecJob j;
+        j.sec = si;
+        j.chunk0 = (uint32_t)P.chunks.size() + k;
+        j.nchunks = std::min<uint32_t>(DEC_WAVES, nchunks - k);
+        j.pad = 0;
+        P.jobs.push_back(j);
+    }
     for (uint32_t k = 0; k < nchunks; ++k) {
         uint32_t cb = 0;
         for (int i = 0; i < 4; ++i) cb |= (uint32_t)sizes[4 * k + i] << (8 * i);
@@ -1190,6 +1259,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     hipStream_t st = e->stream;
     StageTimer& tm = e->timer;
     tm.reset();
+    e->stage.reset(st);
     // device pool = the streams concatenated; parsed in place on the host
     std::vector<uint64_t> in_off(nstreams + 1, 0);
     for (uint32_t k = 0; k < nstreams; ++k) in_off[k + 1] = in_off[k] + lens[k];
@@ -1223,11 +1293,11 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             DecFrame* fr;
             DecSection* secs;
             DecChunk* chunks;
+            DecJob* jobs;
             DecFixed* fixed;
             DecRaw* raws;
             DecTagged* tagged;
-            uint64_t* pc;
-            uint16_t* slots;
+            uint32_t* cum;
             uint32_t *idrel, *rel;
             uint64_t *diff, *keep;
             uint8_t *choice, *tags;
@@ -1249,11 +1319,11 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.fr = c.take<DecFrame>(F);
         x.secs = c.take<DecSection>(P.secs.size());
         x.chunks = c.take<DecChunk>(P.chunks.size());
+        x.jobs = c.take<DecJob>(P.jobs.size());
         x.fixed = c.take<DecFixed>(P.fixed.size());
         x.raws = c.take<DecRaw>(P.raws.size());
         x.tagged = c.take<DecTagged>(P.tagged.size());
-        x.pc = c.take<uint64_t>(P.pc.size());
-        x.slots = c.take<uint16_t>(P.slots.size());
+        x.cum = c.take<uint32_t>(P.cum.size());
         x.idrel = c.take<uint32_t>(T);
         x.rel = c.take<uint32_t>(T * D);
         x.diff = c.take<uint64_t>(P.ND * D);
@@ -1290,9 +1360,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     c1.base = e->arena_d.base;
     auto X = carve(c1);
     if (res.d_pool) X.in = const_cast<uint8_t*>(res.d_pool);
-    auto h2d = [&](void* dst, const void* src, size_t n) {
-        if (n) HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
-    };
+    Staging& sg = e->stage;
+    auto h2d = [&](void* dst, const void* src, size_t n) { sg.h2d(st, dst, src, n); };
     // algorithmic bytes of every timed region, from the plan
     const uint64_t comp_es[5] = {4, 8, 8, 1, 4};
     auto es_of = [&](int32_t comp) -> uint64_t { return comp < 0 ? 1 : comp_es[comp]; };
@@ -1307,15 +1376,16 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     for (const DecTagged& j : P.tagged) b_tagged += j.bytes + j.n + j.n * j.dims * es_of(j.comp);
     tm.begin("h2d", res.d_pool ? 0 : pool_size);
     if (!res.d_pool)
-        for (uint32_t k = 0; k < nstreams; ++k) h2d(X.in + in_off[k], bytes[k], lens[k]);
+        for (uint32_t k = 0; k < nstreams; ++k)
+            if (lens[k]) HIPCHK(hipMemcpyAsync(X.in + in_off[k], bytes[k], lens[k], hipMemcpyHostToDevice, st));
     h2d(X.fr, P.frames.data(), F * sizeof(DecFrame));
     h2d(X.secs, P.secs.data(), P.secs.size() * sizeof(DecSection));
     h2d(X.chunks, P.chunks.data(), P.chunks.size() * sizeof(DecChunk));
     h2d(X.fixed, P.fixed.data(), P.fixed.size() * sizeof(DecFixed));
     h2d(X.raws, P.raws.data(), P.raws.size() * sizeof(DecRaw));
     h2d(X.tagged, P.tagged.data(), P.tagged.size() * sizeof(DecTagged));
-    h2d(X.pc, P.pc.data(), P.pc.size() * 8);
-    h2d(X.slots, P.slots.data(), P.slots.size() * 2);
+    h2d(X.jobs, P.jobs.data(), P.jobs.size() * sizeof(DecJob));
+    h2d(X.cum, P.cum.data(), P.cum.size() * 4);
     h2d(X.ptb, ptb.data(), (F + 1) * 4);
     h2d(X.rtb, rtb.data(), (F + 1) * 4);
     h2d(X.ptb32, ptb32.data(), (F + 1) * 4);
@@ -1328,8 +1398,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
 
     tm.begin("rans_decode", b_rans);
     if (!P.chunks.empty())
-        hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.chunks.size()), dim3(64), 0, st, X.in, X.secs, X.chunks,
-                           X.pc, X.slots, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+        hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.jobs.size()), dim3(64 * DEC_WAVES), 0, st, X.in, X.secs,
+                           X.chunks, X.jobs, X.cum, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
     tm.end();
     if (P.fixed_total) {
         tm.begin("unpack_fixed", b_fixed);
@@ -1417,8 +1487,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     });
     tm.end();
     uint32_t errw[4];
-    HIPCHK(hipMemcpyAsync(errw, X.err, sizeof errw, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    sg.d2h(st, errw, X.err, sizeof errw);
+    sg.finish(st);
     if (errw[0]) {
         std::string m = "invalid LCPS bitstream (device checks:";
         if (errw[0] & DERR_RANS) m += " rANS";
@@ -1668,6 +1738,10 @@ int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, siz
         names[names_len - 1] = 0;
     }
     return k;
+}
+
+int chain_engine_set_timing(chain_engine* e, int on) {
+    return guarded(e, [&] { e->timer.on = on != 0; });
 }
 
 int chain_last_stage_bytes(chain_engine* e, uint64_t* bytes, int max) {

@@ -56,8 +56,7 @@ __device__ inline void store_value(int c, uint64_t gidx, uint64_t v, uint32_t* i
 struct DecSection {
     uint32_t pb, lanes, alphabet, chunk_symbols;
     uint64_t count;
-    uint64_t table_off;         // entry offset into the prob/cum pools
-    uint64_t slot_off;          // entry offset into the slot pool (~0: binary search)
+    uint64_t cum_off;           // alphabet + 1 cumulative frequencies in the u32 pool
     int32_t comp;               // 0..4 values of that component, -1 widths
     uint32_t pad;
     uint64_t dst;               // first global value index (values) or first width index (tags)
@@ -70,29 +69,57 @@ struct DecChunk {
     uint64_t first;             // first symbol of the chunk within its section
 };
 
-// one wavefront per chunk; (cum << 32 | freq) entries and the slot -> symbol
-// table (precision <= 14 bits) are read from global memory behind the caches
-__global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in, const DecSection* __restrict__ secs,
-                                                 const DecChunk* __restrict__ chunks,
-                                                 const uint64_t* __restrict__ pc_pool,
-                                                 const uint16_t* __restrict__ slot_pool, uint8_t* __restrict__ tags,
-                                                 uint32_t* idrel, uint64_t* diff, uint64_t* keep, uint8_t* choice,
-                                                 uint32_t* rel, uint32_t* err) {
-    const DecChunk ch = chunks[blockIdx.x];
-    const DecSection S = secs[ch.sec];
-    const uint32_t lane = threadIdx.x, pb = S.pb, M = 1u << pb, lower = 4u * M;
-    // small tables (the common case: widths, in-block offsets, choices) live in LDS
-    __shared__ uint64_t s_pc[256];
-    __shared__ uint16_t s_slot[4096];
-    const bool direct = S.slot_off != ~0ull;
-    const bool small = direct && M <= 4096 && S.alphabet <= 256;
-    if (small) {
-        for (uint32_t b = lane; b < S.alphabet; b += 64) s_pc[b] = pc_pool[S.table_off + b];
-        for (uint32_t b = lane; b < M; b += 64) s_slot[b] = slot_pool[S.slot_off + b];
+// up to DEC_WAVES consecutive chunks of one section, decoded by one workgroup
+#define DEC_WAVES 4
+struct DecJob {
+    uint32_t sec, chunk0, nchunks, pad;
+};
+
+// rANS decoding. A workgroup of DEC_WAVES wavefronts takes up to DEC_WAVES
+// chunks of one section, one chunk per wavefront. The section's cumulative
+// frequencies and a coarse index of 4096 buckets (the symbol at the start of
+// every bucket of 2^(pb-12) slots) are built in LDS, so every precision up
+// to 20 bits decodes without global table reads: the symbol of a slot is a
+// bucket lookup, refined by binary search over the symbols inside the bucket
+// when pb > 12. Each wavefront stages its chunk's byte stream through its own
+// LDS ring of DEC_RING bytes, refilled in halves of 64 lanes x 32 bytes with
+// the next half already in registers.
+#define DEC_RING 4096u
+#define DEC_HALF 2048u
+#define DEC_BUCKETS 4096u
+__global__ __launch_bounds__(64 * DEC_WAVES) void k_dec_rans(const uint8_t* __restrict__ in,
+                                                             const DecSection* __restrict__ secs,
+                                                             const DecChunk* __restrict__ chunks,
+                                                             const DecJob* __restrict__ jobs,
+                                                             const uint32_t* __restrict__ cum_pool,
+                                                             uint8_t* __restrict__ tags, uint32_t* idrel,
+                                                             uint64_t* diff, uint64_t* keep, uint8_t* choice,
+                                                             uint32_t* rel, uint32_t* err) {
+    __shared__ uint32_t s_cum[RANS_VALUE_LIMIT + 1];
+    __shared__ uint16_t s_bucket[DEC_BUCKETS + 1];
+    __shared__ uint8_t s_ring[DEC_WAVES][DEC_RING];
+    const DecJob J = jobs[blockIdx.x];
+    const DecSection S = secs[J.sec];
+    const uint32_t A = S.alphabet, pb = S.pb, M = 1u << pb, lower = 4u * M;
+    const uint32_t sh = pb > 12 ? pb - 12 : 0u;
+    const uint32_t nb = M >> sh;   // buckets (<= 4096)
+    for (uint32_t i = threadIdx.x; i <= A; i += blockDim.x) s_cum[i] = cum_pool[S.cum_off + i];
+    __syncthreads();
+    // bucket b: the largest symbol s < A with cum[s] <= b << sh
+    for (uint32_t b = threadIdx.x; b <= nb; b += blockDim.x) {
+        const uint32_t slot0 = b << sh;
+        uint32_t lo = 0, hi = A - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_cum[mid] <= slot0) lo = mid; else hi = mid - 1;
+        }
+        s_bucket[b] = (uint16_t)lo;
     }
     __syncthreads();
-    const uint64_t* pc = small ? s_pc : pc_pool + S.table_off;
-    const uint16_t* slots = small ? s_slot : slot_pool + (direct ? S.slot_off : 0);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (wave >= J.nchunks) return;
+    const DecChunk ch = chunks[J.chunk0 + wave];
+    uint8_t* ring = s_ring[wave];
     const uint32_t L = S.lanes;
     const uint64_t Sc = S.count - ch.first < S.chunk_symbols ? S.count - ch.first : S.chunk_symbols;
     const uint64_t T = (Sc + L - 1) / L;
@@ -106,26 +133,58 @@ __global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in,
     }
     const uint8_t* stream = p + 4 * L;
     const uint64_t stream_len = ch.bytes - 4ull * L;
+    // ring staging: byte b of the stream lives at ring[b % DEC_RING]; this
+    // lane moves bytes from + 64*j + lane, j < 32, of every half
+    uint32_t pre[8];
+    auto fetch = [&](uint64_t from) {
+#pragma unroll
+        for (int w = 0; w < 8; ++w) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t b = from + (uint64_t)(4 * w + k) * 64 + lane;
+                v |= (uint32_t)(b < stream_len ? stream[b] : 0u) << (8 * k);
+            }
+            pre[w] = v;
+        }
+    };
+    auto commit = [&](uint64_t from) {
+#pragma unroll
+        for (int w = 0; w < 8; ++w)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                ring[(uint32_t)(from + (uint64_t)(4 * w + k) * 64 + lane) & (DEC_RING - 1)] = (uint8_t)(pre[w] >> (8 * k));
+        __builtin_amdgcn_wave_barrier();
+    };
+    uint64_t loaded = 0;
+    fetch(0);
+    commit(0);
+    loaded = DEC_HALF;
+    if (loaded < stream_len) fetch(loaded);
     uint64_t pos = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (uint64_t t = 0; t < T; ++t) {
+        // keep at least DEC_HALF staged bytes ahead of pos (a step takes <= 256)
+        if (loaded - pos < DEC_HALF && loaded < stream_len) {
+            commit(loaded);
+            loaded += DEC_HALF;
+            if (loaded < stream_len) fetch(loaded);
+        }
         const uint64_t k = t * L + lane;
         const bool active = lane < L && k < Sc;
         if (active) {
             const uint32_t slot = x & (M - 1);
-            uint32_t s;
-            if (direct) {
-                s = slots[slot];
-            } else {   // largest symbol with cum <= slot
-                uint32_t lo = 0, hi = S.alphabet;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if ((uint32_t)(pc[mid] >> 32) <= slot) lo = mid; else hi = mid;
+            const uint32_t b = slot >> sh;
+            uint32_t s = s_bucket[b];
+            if (sh) {
+                uint32_t hi = s_bucket[b + 1];
+                while (s < hi) {
+                    const uint32_t mid = (s + hi + 1) >> 1;
+                    if (s_cum[mid] <= slot) s = mid; else hi = mid - 1;
                 }
-                s = lo;
             }
-            const uint64_t e = pc[s];
-            x = (uint32_t)e * (x >> pb) + slot - (uint32_t)(e >> 32);
+            const uint32_t c0 = s_cum[s];
+            x = (s_cum[s + 1] - c0) * (x >> pb) + slot - c0;
             const uint64_t q = ch.first + k;
             if (S.comp < 0) tags[S.dst + q] = (uint8_t)s;
             else store_value(S.comp, S.dst + q, s, idrel, diff, keep, choice, rel, err);
@@ -137,7 +196,7 @@ __global__ __launch_bounds__(64) void k_dec_rans(const uint8_t* __restrict__ in,
             if (!m) break;
             if (need) {
                 const uint64_t at = pos + __popcll(m & lt);
-                x = (x << 8) | (at < stream_len ? stream[at] : 0u);
+                x = (x << 8) | (at < stream_len ? (uint32_t)ring[(uint32_t)at & (DEC_RING - 1)] : 0u);
             }
             pos += __popcll(m);
         }

@@ -391,31 +391,68 @@ __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_
     return lo;
 }
 
-// one wavefront per chunk; lanes beyond the section's lane count idle. The
-// (cum << 32 | freq) table stays in global memory behind the caches (no LDS,
-// so many chunks stay resident) and symbols are fetched 8 steps ahead.
+// rANS encoding, one wavefront per chunk; lanes beyond the section's lane
+// count idle. Symbols are loaded one group of ENC_GROUP steps ahead of the
+// group being coded. For alphabets <= 256 the table lives in LDS together
+// with an exact reciprocal of every frequency (rcp31_* in chain_rans.hpp),
+// so the state update needs no integer division.
 #define ENC_GROUP 8
+
+struct EncLds {
+    uint64_t pc[256];   // (cum << 32) | freq
+    uint64_t m[256];    // reciprocal multiplier
+    uint8_t l[256];     // reciprocal shift - 31
+};
+
 template <typename ValT, int DIMS>
-__global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ vals, int c, uint32_t F,
-                                                    const CompInst* __restrict__ ci, const Section* __restrict__ sec,
-                                                    const uint32_t* __restrict__ comp_chunk,
-                                                    const uint64_t* __restrict__ pc_tab, uint8_t* __restrict__ scratch,
-                                                    uint64_t stride, uint32_t* __restrict__ chunk_len,
-                                                    uint32_t* __restrict__ chunk_pos, uint32_t* err) {
-    const uint32_t chunk = comp_chunk[c] + blockIdx.x;
-    if (chunk >= comp_chunk[c + 1]) return;
-    const uint32_t si = find_section(sec, 2 * F * c, 2 * F * (c + 1), chunk);
-    const Section& S = sec[si];
-    const CompInst& C = ci[si >> 1];
+__device__ inline void enc_group_load(const ValT* __restrict__ src, bool tags, uint64_t first, uint64_t Sc,
+                                      uint32_t L, uint32_t lane, uint64_t t, uint32_t g, uint32_t* sym) {
+#pragma unroll
+    for (int j = 0; j < ENC_GROUP; ++j) {
+        sym[j] = 0xffffffffu;
+        if ((uint32_t)j < g) {
+            const uint64_t k = (t - 1 - j) * L + lane;
+            if (lane < L && k < Sc) {
+                const uint64_t q = first + k;
+                if (!tags) {
+                    sym[j] = (uint32_t)src[q];
+                } else {
+                    uint32_t tg = 1;
+#pragma unroll
+                    for (int d = 0; d < DIMS; ++d) {
+                        const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
+                        tg = b > tg ? b : tg;
+                    }
+                    sym[j] = tg;
+                }
+            }
+        }
+    }
+}
+
+template <typename ValT, int DIMS>
+__device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, uint32_t si, const CompInst& C,
+                                 const Section& S, const uint64_t* __restrict__ gpc, EncLds& lds,
+                                 uint8_t* __restrict__ scratch, uint64_t stride, uint32_t* __restrict__ chunk_len,
+                                 uint32_t* __restrict__ chunk_pos) {
     const bool tags = (si & 1) != 0;
-    const uint64_t* gpc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
-    __shared__ uint64_t s_pc[256];
+    const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
     const bool small = S.alphabet <= 256;
     if (small)
-        for (uint32_t b = threadIdx.x; b < S.alphabet; b += 64) s_pc[b] = gpc[b];
+        for (uint32_t b = lane; b < S.alphabet; b += 64) {
+            const uint64_t e = gpc[b];
+            const uint32_t f = (uint32_t)e;
+            lds.pc[b] = e;
+            if (f) {
+                const uint32_t l = rcp31_shift(f);
+                lds.l[b] = (uint8_t)l;
+                lds.m[b] = rcp31_mult(f, l);
+            } else {
+                lds.l[b] = 0;
+                lds.m[b] = 0;
+            }
+        }
     __syncthreads();
-    const uint64_t* pc = small ? s_pc : gpc;
-    const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
     const uint32_t M = 1u << pb, lower = 4u * M;
     const uint64_t first = (uint64_t)(chunk - S.chunk_base) * S.chunk_symbols;
     const uint64_t Sc = S.count - first < S.chunk_symbols ? S.count - first : S.chunk_symbols;
@@ -427,37 +464,27 @@ __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ val
     uint32_t x = lower;
     const ValT* src = vals + C.base * DIMS;
     const uint64_t lt = lanemask_lt();
-    for (uint64_t t = T; t > 0;) {
-        const uint32_t g = t < ENC_GROUP ? (uint32_t)t : ENC_GROUP;
-        uint32_t sym[ENC_GROUP];
+    uint32_t cur[ENC_GROUP], nxt[ENC_GROUP];
+    uint64_t t = T;
+    uint32_t g = t < ENC_GROUP ? (uint32_t)t : ENC_GROUP;
+    enc_group_load<ValT, DIMS>(src, tags, first, Sc, L, lane, t, g, cur);
+    while (t > 0) {
+        const uint64_t tn = t - g;
+        const uint32_t gn = tn < ENC_GROUP ? (uint32_t)tn : ENC_GROUP;
+        if (tn > 0) enc_group_load<ValT, DIMS>(src, tags, first, Sc, L, lane, tn, gn, nxt);
+        uint64_t ent[ENC_GROUP], mm[ENC_GROUP];
+        uint32_t ll[ENC_GROUP];
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
-            sym[j] = 0xffffffffu;
-            if ((uint32_t)j < g) {
-                const uint64_t k = (t - 1 - j) * L + lane;
-                if (lane < L && k < Sc) {
-                    const uint64_t q = first + k;
-                    if (!tags) {
-                        sym[j] = (uint32_t)src[q];
-                    } else {
-                        uint32_t tg = 1;
-#pragma unroll
-                        for (int d = 0; d < DIMS; ++d) {
-                            const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
-                            tg = b > tg ? b : tg;
-                        }
-                        sym[j] = tg;
-                    }
-                }
-            }
+            const bool a = cur[j] != 0xffffffffu;
+            ent[j] = a ? (small ? lds.pc[cur[j]] : gpc[cur[j]]) : 1ull;
+            mm[j] = a && small ? lds.m[cur[j]] : 0ull;
+            ll[j] = a && small ? lds.l[cur[j]] : 0u;
         }
-        uint64_t ent[ENC_GROUP];   // table entries of the group, fetched together
-#pragma unroll
-        for (int j = 0; j < ENC_GROUP; ++j) ent[j] = sym[j] != 0xffffffffu ? pc[sym[j]] : 1ull;
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
             if ((uint32_t)j >= g) break;
-            const bool active = sym[j] != 0xffffffffu;
+            const bool active = cur[j] != 0xffffffffu;
             const uint64_t e = ent[j];
             const uint32_t f = (uint32_t)e, cu = (uint32_t)(e >> 32);
             const uint32_t xmax = 1024u * f;     // (lower >> pb) * 256 * f
@@ -470,14 +497,51 @@ __global__ __launch_bounds__(64) void k_rans_encode(const ValT* __restrict__ val
             if (kb > 0) stream[pos + __popcll(m0 & lt)] = (uint8_t)(x >> (8 * (kb - 1)));
             if (kb > 1) stream[pos + c0 + __popcll(m1 & lt)] = (uint8_t)(x >> (8 * (kb - 2)));
             if (kb > 2) stream[pos + c0 + c1 + __popcll(m2 & lt)] = (uint8_t)(x >> (8 * (kb - 3)));
-            if (active) x = (xx / f) * M + (xx % f) + cu;
+            if (active) {
+                const uint32_t q = small ? rcp31_div(xx, mm[j], ll[j]) : xx / f;
+                x = q * M + (xx - q * f) + cu;
+            }
         }
-        t -= g;
+#pragma unroll
+        for (int j = 0; j < ENC_GROUP; ++j) cur[j] = nxt[j];
+        t = tn;
+        g = gn;
     }
     if (lane < L) put_u32(out + 4 * lane, x);
     if (lane == 0) {
         chunk_len[chunk] = (uint32_t)(end - pos);
         chunk_pos[chunk] = (uint32_t)pos;
+    }
+}
+
+// every component's chunks in one launch: block b codes chunk b of the
+// batch (the chunks of component c are [comp_chunk[c], comp_chunk[c+1]))
+template <int D, typename RelT>
+__global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__ idrel,
+                                                    const uint64_t* __restrict__ diff,
+                                                    const uint64_t* __restrict__ keep,
+                                                    const uint8_t* __restrict__ choice2,
+                                                    const RelT* __restrict__ rel, uint32_t F,
+                                                    const CompInst* __restrict__ ci, const Section* __restrict__ sec,
+                                                    const uint32_t* __restrict__ comp_chunk,
+                                                    const uint64_t* __restrict__ pc_tab, uint8_t* __restrict__ scratch,
+                                                    uint64_t stride, uint32_t* __restrict__ chunk_len,
+                                                    uint32_t* __restrict__ chunk_pos) {
+    const uint32_t chunk = blockIdx.x;
+    if (chunk >= comp_chunk[5]) return;
+    int c = 0;
+    while (c < 4 && chunk >= comp_chunk[c + 1]) ++c;
+    const uint32_t si = find_section(sec, 2 * F * c, 2 * F * (c + 1), chunk);
+    const Section& S = sec[si];
+    const CompInst& C = ci[si >> 1];
+    const uint64_t* gpc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
+    __shared__ EncLds lds;
+    switch (c) {
+    case 0: enc_chunk<uint32_t, 1>(idrel, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
+    case 1: enc_chunk<uint64_t, D>(diff, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
+    case 2: enc_chunk<uint64_t, D>(keep, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
+    case 3: enc_chunk<uint8_t, 1>(choice2, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
+    default: enc_chunk<RelT, D>(rel, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
     }
 }
 

@@ -16,6 +16,24 @@ namespace chain {
 #define RANS_VALUE_LIMIT 4096u   // RVC1 value sections: symbols < 4096
 #define RANS_TAG_LIMIT 64u       // Tagged widths are 1..63
 
+// Exact floor(x / f) for 0 <= x < 2^31 by one multiplication
+// (Granlund & Montgomery 1994, Thm 4.2 with N = 31): l = ceil(log2 f),
+// m = ceil(2^(31+l) / f) < 2^33, floor(x / f) = (x * m) >> (31 + l).
+// The rANS encoder divides its state (< 1024 f <= 2^30) by frequencies f <= 2^20.
+__host__ __device__ inline uint32_t rcp31_shift(uint32_t f) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return f > 1 ? 32u - (uint32_t)__clz(f - 1) : 0u;
+#else
+    return f > 1 ? 32u - (uint32_t)__builtin_clz(f - 1) : 0u;
+#endif
+}
+__host__ __device__ inline uint64_t rcp31_mult(uint32_t f, uint32_t l) {
+    return ((1ull << (31 + l)) + f - 1) / f;
+}
+__host__ __device__ inline uint32_t rcp31_div(uint32_t x, uint64_t m, uint32_t l) {
+    return (uint32_t)(((uint64_t)x * m) >> (31 + l));
+}
+
 // lcps_cpp.cpp:541-552
 __host__ __device__ inline int rvn1_precision_bits(uint64_t unique) {
     int ub = bitlen64(unique) + 2;

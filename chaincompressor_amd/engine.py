@@ -52,7 +52,7 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_buf_free", "chain_compress", "chain_compress_batch", "chain_compress_batch_device",
            "chain_engine_set_rans", "chain_decompress", "chain_decompress_batch", "chain_decompress_batch_device",
            "chain_device_malloc", "chain_device_free", "chain_copy", "chain_last_stage_times",
-           "chain_last_stage_bytes"]
+           "chain_last_stage_bytes", "chain_engine_set_timing"]
 
 _LIB = None
 
@@ -92,6 +92,7 @@ def load_library():
     L.chain_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
     L.chain_last_stage_times.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_char_p, C.c_size_t]
     L.chain_last_stage_bytes.argtypes = [C.c_void_p, P(C.c_uint64), C.c_int]
+    L.chain_engine_set_timing.argtypes = [C.c_void_p, C.c_int]
     _LIB = L
     return L
 
@@ -147,6 +148,10 @@ class Engine:
     def set_rans(self, lanes_max=64, chunk_symbols=65536):
         """RVC1 lane count and chunk size of the containers this engine writes."""
         self._check(self._L.chain_engine_set_rans(self._h, lanes_max, chunk_symbols))
+
+    def set_timing(self, on: bool):
+        """Per-stage HIP event timing (on by default)."""
+        self._check(self._L.chain_engine_set_timing(self._h, 1 if on else 0))
 
     def compress_batch_device(self, ptrs, ns, dtype, dims, eb, p, threshold, flags=0, layout="aos"):
         """Device-resident LCH1 batch: returns (device pointer, offsets, sizes, stats)."""

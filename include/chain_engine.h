@@ -127,6 +127,10 @@ int chain_device_malloc(chain_engine* e, uint64_t bytes, void** out);
 int chain_device_free(chain_engine* e, void* p);
 int chain_copy(chain_engine* e, void* dst, const void* src, uint64_t bytes, int to_device);
 
+/* Per-stage HIP event timing of every call (on by default; off saves one
+ * event pair per stage, for latency-sensitive callers such as a server). */
+int chain_engine_set_timing(chain_engine* e, int on);
+
 /* Per-stage device times (milliseconds, from HIP events on the engine's
  * stream) of the last call. names is a ';'-separated list. Returns the count. */
 int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, size_t names_len);

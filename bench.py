@@ -356,19 +356,109 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
 
 
 # ----------------------------------------------------------------------------
+# serving: Poisson arrivals through the request server
+
+def run_serving(args, eng, seconds):
+    import queue
+    import threading
+    from chaincompressor_amd import engine, workloads
+    t0 = time.perf_counter()
+    pool_max = 20_000_000
+    pools = {np.float32: workloads.gaussian_clusters(pool_max, seed=workloads.BASE_SEED + 5),
+             np.float64: workloads.gaussian_clusters(pool_max, seed=workloads.BASE_SEED + 6, dtype=np.float64)}
+    trace = workloads.serving_trace(rate=args.serve_rate, seconds=seconds)
+    # decompress requests replay streams compressed beforehand: for every
+    # dtype and eb, 12 log-spaced sizes; a request takes the nearest size
+    lib_sizes = np.unique(np.round(np.exp(np.linspace(np.log(5e3), np.log(2e7), 12))).astype(np.int64))
+    library = {}
+    for dt, pool in pools.items():
+        for eb in (1e-2, 1e-3, 1e-4):
+            for n in lib_sizes:
+                blob, _ = eng.compress(pool[:int(n)], eb, args.p, args.threshold)
+                library[(dt, eb, int(n))] = blob
+    rng = np.random.default_rng(workloads.BASE_SEED + 7)
+    log(f"serving: {len(trace)} requests over {seconds} s; pools and {len(library)} streams ready in "
+        f"{time.perf_counter() - t0:.1f} s")
+    srv = engine.Server(0, hbm_budget_bytes=args.serve_budget_gb << 30, workers=args.serve_workers,
+                        max_batch=args.serve_max_batch)
+    pending = queue.Queue()
+    starts = rng.integers(0, pool_max, size=len(trace))
+
+    def submitter():
+        base = time.perf_counter()
+        for k, (t_arr, kind, n, dt, eb) in enumerate(trace):
+            delay = base + t_arr - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            if kind == "compress":
+                a = int(min(starts[k], pool_max - n))
+                tk = srv.submit_compress(pools[dt][a:a + n], eb, args.p, args.threshold)
+                pending.put((tk, kind, n * 3 * np.dtype(dt).itemsize))
+            else:
+                m = int(lib_sizes[np.argmin(np.abs(np.log(lib_sizes) - np.log(n)))])
+                tk = srv.submit_decompress(library[(dt, eb, m)], dtype=dt)
+                pending.put((tk, kind, m * 3 * np.dtype(dt).itemsize))
+        pending.put(None)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = {"compress": [], "decompress": []}
+    nbytes = 0
+    first, last, failed, batches = None, 0.0, 0, []
+    while True:
+        item = pending.get()
+        if item is None:
+            break
+        tk, kind, b = item
+        r = srv.poll(tk)
+        if r.status != 0:
+            failed += 1
+            continue
+        lat[kind].append((r.t_end - r.t_submit) * 1e3)
+        nbytes += b
+        first = r.t_submit if first is None else min(first, r.t_submit)
+        last = max(last, r.t_end)
+        batches.append(r.batch)
+    th.join()
+    srv.close()
+    allv = np.array(lat["compress"] + lat["decompress"])
+
+    def pct(v, q):
+        return round(float(np.percentile(v, q)), 3) if len(v) else None
+
+    wall = last - first if first is not None else 0.0
+    return {
+        "requests": len(trace), "failed": failed, "seconds": seconds, "rate_per_s": args.serve_rate,
+        "workers": args.serve_workers, "hbm_budget_gb": args.serve_budget_gb,
+        "p50_ms": pct(allv, 50), "p99_ms": pct(allv, 99),
+        "compress_p50_ms": pct(lat["compress"], 50), "compress_p99_ms": pct(lat["compress"], 99),
+        "decompress_p50_ms": pct(lat["decompress"], 50), "decompress_p99_ms": pct(lat["decompress"], 99),
+        "aggregate_gbps": round(nbytes / wall / 1e9, 3) if wall else None,
+        "offered_gbps": round(nbytes / seconds / 1e9, 3),
+        "mean_batch": round(float(np.mean(batches)), 2) if batches else None,
+        "max_batch": int(max(batches)) if batches else None,
+        "latency": "submit to result on the host (host input -> host output, PCIe included)",
+    }
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="default", choices=["default", "plain", "batched", "md", "all"])
+    ap.add_argument("--workload", default="default", choices=["default", "plain", "batched", "md", "serving", "all"])
     ap.add_argument("--n", type=int, default=50_000_000)
     ap.add_argument("--eb", type=float, default=1e-3)
     ap.add_argument("--p", type=int, default=16)
     ap.add_argument("--threshold", type=int, default=2)
     ap.add_argument("--batch-frames", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=None,
+                    help="length of the arrival trace (default 20 s, 60 s with --workload serving/all)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-max-batch", type=int, default=32)
+    ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -425,6 +515,12 @@ def main():
         if "value" not in line:
             line["value"] = line["md"]["decompress_gbps_batched"]
             line["config"] = {"workload": "md: 128 float64 fcc frames", "eb": 1e-4}
+    if wl in ("default", "serving", "all"):
+        secs = args.serve_seconds or (20.0 if wl == "default" else 60.0)
+        line["serving"] = run_serving(args, eng, secs)
+        if "value" not in line:
+            line["value"] = line["serving"]["aggregate_gbps"]
+            line["config"] = {"workload": "serving: Poisson 200 req/s, mixed requests", "budget_gb": 64}
     if pts is not None and not args.no_cpu_baseline:
         stride = max(1, args.n // args.cpu_sample)
         sample = np.ascontiguousarray(pts[::stride][:args.cpu_sample])

@@ -17,8 +17,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <stdexcept>
+#include <unordered_map>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -49,21 +51,22 @@ static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 struct Arena {
     char* base = nullptr;
     size_t cap = 0;
+    void release() {
+        if (base) (void)hipFree(base);
+        base = nullptr;
+        cap = 0;
+    }
     void ensure(size_t bytes, size_t budget) {
         if (bytes <= cap) return;
         size_t want = bytes + bytes / 8 + (32u << 20);
         if (budget && want > budget) want = bytes;
         if (budget && want > budget)
             throw Error(CHAIN_ENOMEM, "device memory budget exceeded: need " + std::to_string(bytes) + " bytes");
-        if (base) HIPCHK(hipFree(base));
-        base = nullptr;
-        cap = 0;
+        release();
         HIPCHK(hipMalloc(&base, want));
         cap = want;
     }
-    ~Arena() {
-        if (base) (void)hipFree(base);
-    }
+    ~Arena() { release(); }
 };
 
 // two-pass carving: first pass sums sizes, second hands out pointers
@@ -192,6 +195,10 @@ struct chain_engine {
     hipStream_t stream = nullptr;
     chain::Arena arena;     // compression
     chain::Arena arena_d;   // decompression (so either direction's device output survives the other)
+    // both arenas share the budget: growing one releases the other when the
+    // two would not fit together (engines whose outputs never stay on the
+    // device, such as the server's workers)
+    bool shared_budget = false;
     std::mutex mu;
     std::string err;
     chain::StageTimer timer;
@@ -200,6 +207,12 @@ struct chain_engine {
 };
 
 namespace chain {
+
+static void ensure_arena(chain_engine* e, Arena& a, Arena& other, size_t bytes) {
+    if (e->shared_budget && e->budget && bytes > a.cap && other.cap && bytes + other.cap > e->budget)
+        other.release();
+    a.ensure(bytes, e->budget && e->shared_budget ? e->budget - other.cap : e->budget);
+}
 
 // ------------------------------------------------------------------------
 // template dispatch helpers
@@ -255,10 +268,105 @@ struct FramePlan {
     uint64_t relmax;
 };
 
+// Pinned host buffers for large outputs, recycled: device-to-host copies run
+// at DMA speed into pages that are already mapped, and chain_buf_free hands
+// a buffer back to the pool instead of unmapping it. Sizes are rounded up to
+// 8 classes per power of two (<= 12.5 % slack); at most `cap` bytes stay
+// cached. The pool lives until the process ends.
+struct PinnedPool {
+    std::mutex mu;
+    std::unordered_map<void*, size_t> live;
+    std::multimap<size_t, void*> idle;
+    size_t cached = 0, cap = size_t(8) << 30;
+    static constexpr size_t kMin = size_t(1) << 20;   // smaller outputs use malloc
+    static size_t klass(size_t n) {
+        size_t p2 = kMin;
+        while (p2 < n) p2 <<= 1;
+        if (p2 == kMin) return kMin;
+        const size_t step = p2 / 16;   // classes between p2/2 and p2
+        return (n + step - 1) / step * step;
+    }
+    void* acquire(size_t n) {
+        if (n < kMin) return nullptr;
+        const size_t c = klass(n);
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            auto it = idle.find(c);
+            if (it != idle.end()) {
+                void* p = it->second;
+                idle.erase(it);
+                cached -= c;
+                live[p] = c;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipHostMalloc(&p, c, hipHostMallocDefault) != hipSuccess) return nullptr;
+        std::lock_guard<std::mutex> lock(mu);
+        live[p] = c;
+        return p;
+    }
+    bool release(void* p) {
+        size_t c = 0;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            auto it = live.find(p);
+            if (it == live.end()) return false;
+            c = it->second;
+            live.erase(it);
+            if (cached + c <= cap) {
+                idle.emplace(c, p);
+                cached += c;
+                return true;
+            }
+        }
+        (void)hipHostFree(p);
+        return true;
+    }
+};
+
+static PinnedPool& pinned_pool() {
+    static PinnedPool* pool = new PinnedPool();   // never destroyed: outlives every buffer
+    return *pool;
+}
+
+static uint8_t* host_alloc(size_t n) {
+    if (void* p = pinned_pool().acquire(n)) return static_cast<uint8_t*>(p);
+    return static_cast<uint8_t*>(std::malloc(n ? n : 1));
+}
+
+// Fill caller-owned chain_bufs straight from device memory: one host buffer
+// per output (pinned and recycled when large) and one copy, no intermediate
+// host vectors. On failure every buffer allocated here is released again.
+static void device_to_bufs(hipStream_t st, const uint8_t* dsrc, const uint64_t* offs, const uint64_t* sizes,
+                           uint32_t count, chain_buf* out) {
+    uint32_t done = 0;
+    try {
+        for (; done < count; ++done) {
+            out[done].data = host_alloc(sizes[done]);
+            out[done].size = sizes[done];
+            if (!out[done].data) throw std::bad_alloc();
+            if (sizes[done])
+                HIPCHK(hipMemcpyAsync(out[done].data, dsrc + offs[done], sizes[done], hipMemcpyDeviceToHost, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        for (uint32_t k = 0; k <= done && k < count; ++k) {
+            if (out[k].data && !pinned_pool().release(out[k].data)) std::free(out[k].data);
+            out[k].data = nullptr;
+            out[k].size = 0;
+        }
+        throw;
+    }
+}
+
 struct CompressResult {
-    // host copies of the containers, or (keep_on_device) offsets into the
+    // host copies of the containers (host_out: caller's bufs filled directly;
+    // otherwise containers), or (keep_on_device) offsets into the
     // engine-owned device buffer d_out
     bool keep_on_device = false;
+    chain_buf* host_out = nullptr;
     std::vector<std::vector<uint8_t>> containers;
     std::vector<chain_stats> stats;
     uint8_t* d_out = nullptr;
@@ -574,7 +682,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         (void)carve_c(cc0);
         size_c = cc0.off;
     }
-    e->arena.ensure(size_a + size_b + size_c, e->budget);
+    ensure_arena(e, e->arena, e->arena_d, size_a + size_b + size_c);
 
     Carver cA;
     cA.base = e->arena.base;
@@ -894,6 +1002,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         res.sizes = sizes;
         if (res.keep_on_device) {
             res.d_out = X.out;
+            return;
+        }
+        if (res.host_out) {
+            device_to_bufs(st, X.out, offs.data(), sizes.data(), F, res.host_out);
             return;
         }
         std::vector<uint8_t> all(total);
@@ -1244,8 +1356,8 @@ static void parse_stream(const uint8_t* bytes, uint64_t dev_off, uint64_t len, D
 
 struct DecodeResult {
     bool keep_on_device = false;
+    chain_buf* host_out = nullptr;     // caller's bufs, filled directly (one per stream)
     const uint8_t* d_pool = nullptr;   // device copy of the concatenated streams (skips the upload)
-    std::vector<std::vector<uint8_t>> outs;
     std::vector<uint64_t> ns, out_off;
     uint32_t dims = 0;
     void* d_out = nullptr;
@@ -1355,7 +1467,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     };
     Carver c0;
     (void)carve(c0);
-    e->arena_d.ensure(c0.off, e->budget);
+    ensure_arena(e, e->arena_d, e->arena, c0.off);
     Carver c1;
     c1.base = e->arena_d.base;
     auto X = carve(c1);
@@ -1512,14 +1624,16 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         res.d_out = X.out;
         return;
     }
-    res.outs.resize(F);
-    std::vector<uint8_t> all(T * D * es);
-    HIPCHK(hipMemcpyAsync(all.data(), X.out, all.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (uint32_t f = 0; f < F; ++f) {
-        const uint64_t a = P.frames[f].out_off * es, b = a + P.frames[f].n * D * es;
-        res.outs[f].assign(all.begin() + a, all.begin() + b);
+    if (res.host_out) {
+        std::vector<uint64_t> boffs(F), bsizes(F);
+        for (uint32_t f = 0; f < F; ++f) {
+            boffs[f] = P.frames[f].out_off * es;
+            bsizes[f] = P.frames[f].n * D * es;
+        }
+        device_to_bufs(st, X.out, boffs.data(), bsizes.data(), F, res.host_out);
+        return;
     }
+    throw Error(CHAIN_EINVAL, "decoded coordinates need a host or device destination");
 }
 }  // namespace chain
 
@@ -1594,7 +1708,7 @@ void* chain_engine_stream(chain_engine* e) { return e ? (void*)e->stream : nullp
 
 void chain_buf_free(chain_buf* b) {
     if (!b) return;
-    std::free(b->data);
+    if (b->data && !chain::pinned_pool().release(b->data)) std::free(b->data);
     b->data = nullptr;
     b->size = 0;
 }
@@ -1612,8 +1726,9 @@ int chain_compress(chain_engine* e, const void* points, int dtype, int layout, u
         if (!out) throw Error(CHAIN_EINVAL, "out must not be NULL");
         chain_frame fr{points, n, eb, p, threshold, flags};
         chain::CompressResult res;
+        res.host_out = out;
         chain::compress_batch_impl(e, &fr, 1, dtype, layout, dims, false, format, res);
-        to_buf(res.containers[0], out);
+        if (!res.containers.empty()) to_buf(res.containers[0], out);
         if (stats) *stats = res.stats[0];
     });
 }
@@ -1623,9 +1738,10 @@ int chain_compress_batch(chain_engine* e, const chain_frame* frames, uint32_t nf
     return guarded(e, [&] {
         if (!frames || !outs) throw Error(CHAIN_EINVAL, "frames and outs must not be NULL");
         chain::CompressResult res;
+        res.host_out = outs;
         chain::compress_batch_impl(e, frames, nframes, dtype, layout, dims, points_on_device != 0, format, res);
         for (uint32_t f = 0; f < nframes; ++f) {
-            to_buf(res.containers[f], &outs[f]);
+            if (!res.containers.empty()) to_buf(res.containers[f], &outs[f]);
             if (stats) stats[f] = res.stats[f];
         }
     });
@@ -1663,8 +1779,8 @@ int chain_decompress(chain_engine* e, const uint8_t* bytes, size_t len, int out_
     return guarded(e, [&] {
         if (!bytes || !out) throw Error(CHAIN_EINVAL, "NULL argument");
         chain::DecodeResult res;
+        res.host_out = out;
         chain::decompress_batch_impl(e, &bytes, &len, 1, out_dtype, out_layout, res);
-        to_buf(res.outs[0], out);
         if (n) *n = res.ns[0];
         if (dims) *dims = res.dims;
     });
@@ -1675,9 +1791,9 @@ int chain_decompress_batch(chain_engine* e, const uint8_t* const* bytes, const s
     return guarded(e, [&] {
         if (!bytes || !lens || !outs) throw Error(CHAIN_EINVAL, "NULL argument");
         chain::DecodeResult res;
+        res.host_out = outs;
         chain::decompress_batch_impl(e, bytes, lens, nstreams, out_dtype, out_layout, res);
         for (uint32_t k = 0; k < nstreams; ++k) {
-            to_buf(res.outs[k], &outs[k]);
             if (ns) ns[k] = res.ns[k];
             if (dims) dims[k] = res.dims;
         }
@@ -1754,3 +1870,5 @@ int chain_last_stage_bytes(chain_engine* e, uint64_t* bytes, int max) {
 }
 
 }  // extern "C"
+
+#include "chain_server.hpp"

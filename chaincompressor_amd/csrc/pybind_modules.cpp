@@ -147,10 +147,15 @@ static py::array lcps_decompress_gpu(py::bytes compressed, bool output_soa, cons
     check(e, rc);
     std::vector<py::ssize_t> shape = output_soa ? std::vector<py::ssize_t>{(py::ssize_t)dims, (py::ssize_t)n}
                                                 : std::vector<py::ssize_t>{(py::ssize_t)n, (py::ssize_t)dims};
-    py::array result = odt == CHAIN_F32 ? py::array(py::dtype::of<float>(), shape) : py::array(py::dtype::of<double>(), shape);
-    std::memcpy(result.mutable_data(), out.data, out.size);
-    chain_buf_free(&out);
-    return result;
+    // the array takes ownership of the library's buffer (no copy), as the
+    // reference decoder hands its storage over with a capsule
+    uint8_t* data = out.data;
+    py::capsule owner(data, [](void* ptr) {
+        chain_buf b{static_cast<uint8_t*>(ptr), 0};
+        chain_buf_free(&b);
+    });
+    return odt == CHAIN_F32 ? py::array(py::dtype::of<float>(), shape, data, owner)
+                            : py::array(py::dtype::of<double>(), shape, data, owner);
 }
 
 PYBIND11_MODULE(lcps_gpu_decompress_ans, m) {

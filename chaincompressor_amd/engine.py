@@ -52,7 +52,24 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_buf_free", "chain_compress", "chain_compress_batch", "chain_compress_batch_device",
            "chain_engine_set_rans", "chain_decompress", "chain_decompress_batch", "chain_decompress_batch_device",
            "chain_device_malloc", "chain_device_free", "chain_copy", "chain_last_stage_times",
-           "chain_last_stage_bytes", "chain_engine_set_timing"]
+           "chain_last_stage_bytes", "chain_engine_set_timing", "chain_server_create", "chain_server_destroy",
+           "chain_server_last_error", "chain_serve_submit", "chain_serve_poll", "chain_server_clock"]
+
+CHAIN_PENDING = 5
+REQ_COMPRESS, REQ_DECOMPRESS = 0, 1
+
+class _Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("points", C.c_void_p), ("dtype", C.c_int), ("layout", C.c_int),
+                ("dims", C.c_uint32), ("n", C.c_uint64), ("eb", C.c_double), ("p", C.c_int64),
+                ("threshold", C.c_int64), ("flags", C.c_uint32), ("format", C.c_int),
+                ("bytes", C.c_void_p), ("len", C.c_size_t), ("out_dtype", C.c_int), ("out_layout", C.c_int)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("out", _Buf), ("n", C.c_uint64), ("dims", C.c_uint32), ("batch", C.c_uint32),
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
+                ("error", C.c_char * 256)]
+
 
 _LIB = None
 
@@ -93,6 +110,14 @@ def load_library():
     L.chain_last_stage_times.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_char_p, C.c_size_t]
     L.chain_last_stage_bytes.argtypes = [C.c_void_p, P(C.c_uint64), C.c_int]
     L.chain_engine_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.chain_server_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_void_p)]
+    L.chain_server_destroy.argtypes = [C.c_void_p]
+    L.chain_server_last_error.argtypes = [C.c_void_p]
+    L.chain_server_last_error.restype = C.c_char_p
+    L.chain_serve_submit.argtypes = [C.c_void_p, P(_Request), P(C.c_uint64)]
+    L.chain_serve_poll.argtypes = [C.c_void_p, C.c_uint64, C.c_int, P(_Result)]
+    L.chain_server_clock.argtypes = [C.c_void_p]
+    L.chain_server_clock.restype = C.c_double
     _LIB = L
     return L
 
@@ -102,6 +127,29 @@ def _take(buf: _Buf) -> bytes:
         return C.string_at(buf.data, buf.size) if buf.size else b""
     finally:
         load_library().chain_buf_free(C.byref(buf))
+
+
+class _Owned:
+    """Releases a library-allocated buffer when the last array viewing it dies."""
+
+    def __init__(self, buf: _Buf):
+        self.buf = buf
+
+    def __del__(self):
+        load_library().chain_buf_free(C.byref(self.buf))
+
+
+def _take_array(buf: _Buf, dtype, shape) -> np.ndarray:
+    """A numpy view of a library-allocated coordinate buffer (no copy); the
+    buffer goes back to the library when the array is garbage-collected."""
+    dt = np.dtype(dtype)
+    if buf.size == 0:
+        load_library().chain_buf_free(C.byref(buf))
+        return np.empty(shape, dtype=dt)
+    owner = _Owned(buf)
+    raw = (C.c_uint8 * buf.size).from_address(C.addressof(buf.data.contents))
+    raw._owner = owner
+    return np.frombuffer(raw, dtype=dt).reshape(shape)
 
 
 def _dtype_code(dt) -> int:
@@ -234,9 +282,7 @@ class Engine:
                                       CHAIN_AOS if layout == "aos" else CHAIN_SOA, C.byref(out), C.byref(n),
                                       C.byref(d))
         self._check(rc)
-        raw = _take(out)
-        arr = np.frombuffer(raw, dtype=dtype).copy()
-        return arr.reshape((n.value, d.value) if layout == "aos" else (d.value, n.value))
+        return _take_array(out, dtype, (n.value, d.value) if layout == "aos" else (d.value, n.value))
 
     def decompress_batch(self, streams, dtype=np.float64, layout="aos"):
         k = len(streams)
@@ -249,11 +295,8 @@ class Engine:
         rc = self._L.chain_decompress_batch(self._h, ptrs, lens, k, _dtype_code(dtype),
                                             CHAIN_AOS if layout == "aos" else CHAIN_SOA, outs, ns, ds)
         self._check(rc)
-        res = []
-        for j in range(k):
-            arr = np.frombuffer(_take(outs[j]), dtype=dtype).copy()
-            res.append(arr.reshape((ns[j], ds[j]) if layout == "aos" else (ds[j], ns[j])))
-        return res
+        return [_take_array(outs[j], dtype, (ns[j], ds[j]) if layout == "aos" else (ds[j], ns[j]))
+                for j in range(k)]
 
     def decompress_batch_device(self, streams, d_streams=None, dtype=np.float64, layout="aos"):
         """Decode streams (host copies plan the work; d_streams is a device
@@ -301,3 +344,90 @@ class Engine:
         b = (C.c_uint64 * 128)()
         self._L.chain_last_stage_bytes(self._h, b, 128)
         return [(labels[i], float(ms[i]), int(b[i])) for i in range(k)]
+
+
+class Served:
+    """Result of one served request."""
+    __slots__ = ("kind", "status", "error", "data", "n", "dims", "batch", "t_submit", "t_start", "t_end")
+
+    @property
+    def latency(self) -> float:
+        return self.t_end - self.t_submit
+
+
+class Server:
+    """Request server over one GPU (continuous batching, HBM-budget admission).
+
+    Input arrays / byte strings are kept referenced until their result is
+    polled. Compress results are container bytes; decompress results are
+    coordinate arrays."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 64 << 30, workers: int = 4, max_batch: int = 64):
+        L = load_library()
+        h = C.c_void_p()
+        rc = L.chain_server_create(device, hbm_budget_bytes, workers, max_batch, C.byref(h))
+        if rc != 0:
+            raise ChainError(L.chain_server_last_error(None).decode())
+        self._h, self._L = h, L
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.chain_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.close()
+
+    def clock(self) -> float:
+        return float(self._L.chain_server_clock(self._h))
+
+    def _submit(self, req, keep):
+        t = C.c_uint64()
+        rc = self._L.chain_serve_submit(self._h, C.byref(req), C.byref(t))
+        if rc != 0:
+            msg = self._L.chain_server_last_error(self._h).decode()
+            raise (ChainValueError if rc == 1 else ChainError)(msg)
+        self._keep[t.value] = keep
+        return t.value
+
+    def submit_compress(self, points, eb, p=16, threshold=2, flags=0, fmt="lch1", layout="aos") -> int:
+        pts = np.ascontiguousarray(points)
+        if pts.ndim != 2:
+            raise ChainValueError("points must be a 2-D array")
+        n, d = pts.shape if layout == "aos" else pts.shape[::-1]
+        req = _Request(kind=REQ_COMPRESS, points=pts.ctypes.data, dtype=_dtype_code(pts.dtype),
+                       layout=CHAIN_AOS if layout == "aos" else CHAIN_SOA, dims=d, n=n, eb=float(eb), p=int(p),
+                       threshold=int(threshold), flags=int(flags), format=_FORMATS[fmt])
+        return self._submit(req, ("c", pts))
+
+    def submit_decompress(self, data: bytes, dtype=np.float64, layout="aos") -> int:
+        b = bytes(data)
+        req = _Request(kind=REQ_DECOMPRESS, bytes=C.cast(C.c_char_p(b), C.c_void_p).value, len=len(b),
+                       out_dtype=_dtype_code(dtype), out_layout=CHAIN_AOS if layout == "aos" else CHAIN_SOA)
+        return self._submit(req, ("d", b, np.dtype(dtype), layout))
+
+    def poll(self, ticket: int, wait_ms: int = -1):
+        """The request's Served result, or None if it has not finished."""
+        r = _Result()
+        rc = self._L.chain_serve_poll(self._h, ticket, int(wait_ms), C.byref(r))
+        if rc == CHAIN_PENDING:
+            return None
+        if rc != 0:
+            raise ChainError("invalid poll")
+        keep = self._keep.pop(ticket, None)
+        s = Served()
+        s.kind = "compress" if keep and keep[0] == "c" else "decompress"
+        s.status, s.error = r.status, r.error.decode(errors="replace")
+        s.n, s.dims, s.batch = int(r.n), int(r.dims), int(r.batch)
+        s.t_submit, s.t_start, s.t_end = r.t_submit, r.t_start, r.t_end
+        if r.status != 0:
+            s.data = None
+            load_library().chain_buf_free(C.byref(r.out))
+        elif s.kind == "compress":
+            s.data = _take(r.out)
+        else:
+            dt, layout = keep[2], keep[3]
+            s.data = _take_array(r.out, dt, (s.n, s.dims) if layout == "aos" else (s.dims, s.n))
+        return s

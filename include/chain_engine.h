@@ -32,6 +32,7 @@ enum chain_status {
     CHAIN_ERUNTIME = 2,    /* data or format error (the reference raises RuntimeError) */
     CHAIN_EDEVICE = 3,     /* HIP runtime error or device error word                  */
     CHAIN_ENOMEM = 4,      /* device memory budget exceeded                           */
+    CHAIN_PENDING = 5,     /* chain_serve_poll: the request has not finished yet      */
 };
 
 enum chain_dtype { CHAIN_F32 = 0, CHAIN_F64 = 1 };
@@ -140,6 +141,59 @@ int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, siz
  * once, so bytes / time is the region's achieved bandwidth against HBM peak.
  * Returns the count. */
 int chain_last_stage_bytes(chain_engine* e, uint64_t* bytes, int max);
+
+/* ------------------------------------------------------------------------
+ * Request server: continuous batching on one GPU. No counterpart in the
+ * reference, whose modules run one synchronous frame per call; this is the
+ * serving form built on the same engine and bitstreams.
+ *
+ * W worker threads each own an engine (stream + arenas), so batches overlap
+ * on the device. A free worker takes the oldest waiting request and every
+ * later waiting request of the same kind and shape, up to max_batch
+ * requests and its share (hbm_budget / W) of the device-memory budget, and
+ * runs them through one launch per stage. Results are identical to running
+ * each request alone. Input buffers are borrowed until the request's result
+ * has been polled. */
+typedef struct chain_server chain_server;
+
+enum chain_request_kind { CHAIN_REQ_COMPRESS = 0, CHAIN_REQ_DECOMPRESS = 1 };
+
+typedef struct {
+    int kind;                  /* chain_request_kind */
+    /* compress: host points, (N, D) AoS or (D, N) SoA */
+    const void* points;
+    int dtype, layout;
+    uint32_t dims;
+    uint64_t n;
+    double eb;
+    int64_t p, threshold;
+    uint32_t flags;
+    int format;                /* chain_format */
+    /* decompress: one LCH1 / LCP3 stream in host memory */
+    const uint8_t* bytes;
+    size_t len;
+    int out_dtype, out_layout;
+} chain_request;
+
+typedef struct {
+    int status;                /* chain_status of the request */
+    chain_buf out;             /* container or coordinates, release with chain_buf_free */
+    uint64_t n;                /* particles */
+    uint32_t dims;
+    uint32_t batch;            /* requests in the batch it ran with */
+    double t_submit, t_start, t_end;   /* seconds on chain_server_clock */
+    char error[256];
+} chain_result;
+
+int chain_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers, uint32_t max_batch,
+                        chain_server** out);
+void chain_server_destroy(chain_server* s);   /* finishes queued requests first */
+const char* chain_server_last_error(const chain_server* s);
+int chain_serve_submit(chain_server* s, const chain_request* r, uint64_t* ticket);
+/* wait_ms < 0 waits until done, 0 polls; CHAIN_PENDING if not finished.
+ * On CHAIN_OK the result is handed over (and forgotten by the server). */
+int chain_serve_poll(chain_server* s, uint64_t ticket, int wait_ms, chain_result* out);
+double chain_server_clock(chain_server* s);
 
 #ifdef __cplusplus
 }

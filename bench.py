@@ -260,9 +260,17 @@ def run_plain(args, eng):
                 "ms_per_step": (tc + td) / steps * 1e3, "bytes": int(sizes[0]), "st": st[0], "maxerr": maxerr,
                 "c_steps": c_steps, "d_steps": d_steps}
 
+    if args.calibrate_copy:
+        # a plain device-to-device copy of known size, for scaling HBM
+        # counters (FETCH_SIZE / WRITE_SIZE) read by a profiler
+        cb = 1 << 30
+        a, b = eng.device_malloc(cb), eng.device_malloc(cb)
+        eng.copy_device(b, a, cb)
+        eng.device_free(a)
+        eng.device_free(b)
     main = one_eb(args.eb, args.steps, args.warmup, True)
     sweep = {}
-    for eb in (1e-2, 1e-4):
+    for eb in (() if args.no_sweep else (1e-2, 1e-4)):
         r = one_eb(eb, 2, 1, False)
         sweep[f"{eb:g}"] = {"compress_gbps": round(r["compress_gbps"], 2),
                             "decompress_gbps": round(r["decompress_gbps"], 2),
@@ -291,8 +299,18 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
     ptrs = [dptr + int(o) * 3 * es for o in offs]
     p, thr = args.p, args.threshold
 
+    # the batched path: entropy chunking tuned on this batch, launches
+    # replayed from a captured HIP graph
+    (lanes, chunk), trials = eng.autotune(ptrs, ns, dtype, 3, eb, p, thr)
+    tuned = {"lanes_max": lanes, "chunk_symbols": chunk,
+             "trials": [{"chunk_symbols": c, "ms": round(ms, 3), "bytes": b} for c, ms, b in trials]}
+
     def comp_batched():
-        return eng.compress_batch_device(ptrs, ns, dtype, 3, eb, p, thr)
+        eng.set_graphs(True)
+        try:
+            return eng.compress_batch_device(ptrs, ns, dtype, 3, eb, p, thr)
+        finally:
+            eng.set_graphs(False)
 
     # containers of the batched call, kept in a device buffer for decoding
     d_out, coffs, csizes, _ = comp_batched()
@@ -301,7 +319,7 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
     dblob = upload(eng, np.frombuffer(blob, dtype=np.uint8))
     soffs = np.concatenate([[0], np.cumsum(csizes)[:-1]]).astype(np.int64).tolist()
 
-    # the per-frame path writes the same containers
+    # the per-frame path (same rANS setting) writes the same containers
     identical = True
     for k in range(len(ns)):
         o, oo, ss, _ = eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
@@ -310,8 +328,13 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
             break
 
     def comp_frames():
-        for k in range(len(ns)):
-            eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
+        # the plain path: one call per frame with the default rANS setting
+        eng.set_rans(64, 65536)
+        try:
+            for k in range(len(ns)):
+                eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
+        finally:
+            eng.set_rans(lanes, chunk)
 
     def dec_batched():
         return eng.decompress_batch_device(streams, dblob)
@@ -340,6 +363,7 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
     assert maxerr <= eb, maxerr
     eng.device_free(dptr)
     eng.device_free(dblob)
+    eng.set_rans(64, 65536)
     r = {
         "frames": len(ns), "points": total, "input_bytes": nbytes, "eb": eb, "dtype": np.dtype(dtype).name,
         "compress_gbps_batched": round(nbytes / t_cb / 1e9, 2),
@@ -351,6 +375,9 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
         "bits_per_particle": round(sum(csizes) * 8 / total, 3),
         "containers_identical_to_per_frame": identical,
         "max_abs_error": maxerr, "steps": steps,
+        "autotune": tuned, "graph_captures_replays": list(eng.graph_stats()),
+        "per_frame_path": "one call per frame, default rANS setting (64 lanes, 65536 symbols per chunk), no graph; "
+                          "per-frame decompression reads the batched path's containers",
     }
     return r
 
@@ -461,6 +488,9 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the eb = 1e-2 / 1e-4 runs of the plain frame")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="one 1 GiB device-to-device copy before the plain frame (profiler counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus above 1 is not supported "

@@ -14,6 +14,7 @@
 #include "chain_format.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -181,6 +182,18 @@ struct Staging {
     }
 };
 
+// executable graph of the last captured launch sequence
+struct GraphCache {
+    std::string key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t captures = 0, replays = 0;
+    ~GraphCache() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
 // RVC1 parameters of the containers an engine writes (recorded in every section)
 struct RansConfig {
     uint32_t lanes_max = 64;
@@ -204,6 +217,10 @@ struct chain_engine {
     chain::StageTimer timer;
     chain::Staging stage;
     chain::RansConfig rans;
+    bool use_graphs = false;
+    chain::GraphCache graph;
+    bool autotune = false;
+    std::map<uint32_t, chain::RansConfig> tuned;   // size bucket -> setting
 };
 
 namespace chain {
@@ -484,6 +501,66 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                            x.tag_tiles + c * (F + 1) + F, x.out);
         tm.end();
     });
+}
+
+// bytes of every value that shapes the launch sequence of a call
+struct GraphKey {
+    std::string bytes;
+    template <class V>
+    GraphKey& add(const V& v) {
+        bytes.append(reinterpret_cast<const char*>(&v), sizeof v);
+        return *this;
+    }
+};
+
+// Replay the cached graph when the launch shape matches, otherwise capture
+// the launches once (no host synchronisation, memcpy or memset inside) and
+// update or re-instantiate the executable graph.
+template <class Fn>
+static void run_graph(chain_engine* e, const std::string& key, Fn&& issue) {
+    hipStream_t st = e->stream;
+    GraphCache& G = e->graph;
+    if (G.exec && G.key == key) {
+        HIPCHK(hipGraphLaunch(G.exec, st));
+        ++G.replays;
+        return;
+    }
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        issue();
+    } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(st, &g));
+    bool updated = false;
+    if (G.exec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult why;
+        updated = hipGraphExecUpdate(G.exec, g, &err_node, &why) == hipSuccess && why == hipGraphExecUpdateSuccess;
+        (void)hipGetLastError();
+        if (!updated) {
+            (void)hipGraphExecDestroy(G.exec);
+            G.exec = nullptr;
+        }
+    }
+    if (!updated) {
+        const hipError_t rc = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+        if (rc != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            G.exec = nullptr;
+            G.key.clear();
+            throw Error(CHAIN_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(rc));
+        }
+    }
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    G.graph = g;
+    G.key = key;
+    ++G.captures;
+    HIPCHK(hipGraphLaunch(G.exec, st));
 }
 
 static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
@@ -792,132 +869,164 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     auto B = carve_b(cB, kbytes, rbytes, nwords);
     sg.h2d(st, A.fdev, fdev.data(), F * sizeof(FrameDev));
 
-    int t_explicit = -1, t_l2p = -1;
-    auto run_typed = [&](auto key_tag, auto rel_tag) {
-        using KeyT = typename decltype(key_tag)::type;
-        using RelT = typename decltype(rel_tag)::type;
-        // K2 quantise
-        tm.begin("quantize", T * D * es + T * sizeof(KeyT) * nwords + T * D * sizeof(RelT));
-        with_dtype(dtype, [&](auto tt) {
-            using T_ = typename decltype(tt)::type;
+    int t_explicit = -1, t_l2p = -1, t_l2s = -1, t_corr = -1;
+    EntropyTimes et{};
+    Carver cC;
+    cC.base = e->arena.base + size_a + size_b;
+    auto X = carve_c(cC);   // entropy-stage buffers (LCH1 only)
+    // every launch after the planning sync: issued directly, or captured
+    // once into a HIP graph and replayed for batches of the same launch shape
+    auto issue = [&]() {
+        auto run_typed = [&](auto key_tag, auto rel_tag) {
+            using KeyT = typename decltype(key_tag)::type;
+            using RelT = typename decltype(rel_tag)::type;
+            // K2 quantise
+            tm.begin("quantize", T * D * es + T * sizeof(KeyT) * nwords + T * D * sizeof(RelT));
+            with_dtype(dtype, [&](auto tt) {
+                using T_ = typename decltype(tt)::type;
+                with_dims(D, [&](auto dd) {
+                    constexpr int DD = decltype(dd)::value;
+                    if (layout == CHAIN_SOA)
+                        hipLaunchKernelGGL((k_quantize<T_, DD, true, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
+                                           (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
+                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
+                    else
+                        hipLaunchKernelGGL((k_quantize<T_, DD, false, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
+                                           (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
+                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
+                });
+            });
+            tm.end();
+            // K3 sort
+            const KeyT* kin = (const KeyT*)B.keys_in;
+            const uint32_t* vin = nullptr;
+            KeyT* kout = (KeyT*)B.ka;
+            uint32_t* vout = B.va;
+            uint32_t done_passes = 0;
+            for (uint32_t w = 0; w < nwords; ++w)
+            for (uint32_t ps = 0; ps < word_passes[w]; ++ps) {
+                if (w > 0 && ps == 0) {
+                    // values of word w in the order left by the words before it
+                    KeyT* g = (KeyT*)((kin == (const KeyT*)B.ka) ? B.kb : B.ka);
+                    if (kin == (const KeyT*)B.keys_in) g = (KeyT*)B.ka;
+                    hipLaunchKernelGGL(k_gather_word, dim3(grid_for(T)), dim3(CH_WG), 0, st,
+                                       (const uint64_t*)B.keys_in + (uint64_t)w * T, vin, T, (uint64_t*)g);
+                    kin = g;
+                    kout = (g == (KeyT*)B.ka) ? (KeyT*)B.kb : (KeyT*)B.ka;
+                }
+                const uint32_t shift = 8 * ps;
+                const uint32_t ps_all = done_passes++;
+                tm.begin("sort_hist", T * sizeof(KeyT));
+                hipLaunchKernelGGL((k_sort_hist<KeyT>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin, T, shift,
+                                   (uint32_t)sort_tiles, B.counts);
+                tm.end();
+                tm.begin("sort_scan", 2ull * 256 * sort_tiles * 4);
+                scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status, B.ticket, A.err, nullptr);
+                tm.end();
+                tm.begin("sort_scatter", ps_all == 0 ? T * sizeof(KeyT) + T * (sizeof(KeyT) + 4)
+                                                     : 2 * T * (sizeof(KeyT) + 4));
+                if (ps_all == 0)
+                    hipLaunchKernelGGL((k_sort_scatter<KeyT, true>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
+                                       vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
+                else
+                    hipLaunchKernelGGL((k_sort_scatter<KeyT, false>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st,
+                                       kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
+                tm.end();
+                kin = kout;
+                vin = vout;
+                kout = (kout == (KeyT*)B.ka) ? (KeyT*)B.kb : (KeyT*)B.ka;
+                vout = (vout == B.va) ? B.vb : B.va;
+            }
+            // K4 link + decision
+            tm.begin("link", T * (sizeof(KeyT) * nwords + 8));
             with_dims(D, [&](auto dd) {
                 constexpr int DD = decltype(dd)::value;
-                if (layout == CHAIN_SOA)
-                    hipLaunchKernelGGL((k_quantize<T_, DD, true, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
-                                       (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                       B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
+                if (nwords == 1)
+                    hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st, kin, vin, T, A.fdev,
+                                       A.ptb, F, B.idrel);
                 else
-                    hipLaunchKernelGGL((k_quantize<T_, DD, false, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
-                                       (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                       B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
+                    hipLaunchKernelGGL((k_link_wide<DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st,
+                                       (const uint64_t*)B.keys_in, vin, T, nwords, A.fdev, A.ptb, F, B.idrel);
             });
+            tm.end();
+            // K5 explicit compaction
+            t_explicit = tm.begin("explicit");
+            fill64(st, B.status, ntiles, 0);
+            fill32(st, B.ticket, 1, 0);
+            with_dims(D, [&](auto dd) {
+                constexpr int DD = decltype(dd)::value;
+                hipLaunchKernelGGL((k_explicit<KeyT, DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.idrel,
+                                   (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, B.status, B.ticket, A.err, B.blocks,
+                                   B.fo, &B.totals[0], nwords, T);
+            });
+            hipLaunchKernelGGL(k_frame_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[0]);
+            tm.end();
+        };
+        auto with_rel = [&](auto key_tag) {
+            if (rel8) run_typed(key_tag, TypeTag<uint8_t>{});
+            else run_typed(key_tag, TypeTag<uint32_t>{});
+        };
+        if (key64) with_rel(TypeTag<uint64_t>{});
+        else with_rel(TypeTag<uint32_t>{});
+
+        // K6/K7 L2
+        t_l2s = tm.begin("l2_stats");
+        // frame record bases as a flat array for the record-space kernels
+        hipLaunchKernelGGL(k_copy_rec_base, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, B.rec_base);
+        const uint64_t rec_tiles_max = cdiv(T, CH_TILE);
+        with_dims(D, [&](auto dd) {
+            constexpr int DD = decltype(dd)::value;
+            hipLaunchKernelGGL((k_l2_stats<DD>), dim3(grid_for(T, CH_WG * CH_ITEMS)), dim3(CH_WG), 0, st, B.blocks,
+                               &B.totals[0], F, B.rec_base, B.fo);
+            hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
+            tm.end();
+            t_l2p = tm.begin("l2_split");
+            fill64(st, B.status, rec_tiles_max, 0);
+            fill32(st, B.ticket, 1, 0);
+            hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)rec_tiles_max), dim3(CH_WG), 0, st, B.blocks,
+                               &B.totals[0], F, B.rec_base, A.fdev, B.fo, B.status, B.ticket, A.err, B.choice2, B.diff,
+                               B.keep, &B.totals[1]);
         });
+        hipLaunchKernelGGL(k_frame_split_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[1],
+                           &B.totals[0]);
         tm.end();
-        // K3 sort
-        const KeyT* kin = (const KeyT*)B.keys_in;
-        const uint32_t* vin = nullptr;
-        KeyT* kout = (KeyT*)B.ka;
-        uint32_t* vout = B.va;
-        uint32_t done_passes = 0;
-        for (uint32_t w = 0; w < nwords; ++w)
-        for (uint32_t ps = 0; ps < word_passes[w]; ++ps) {
-            if (w > 0 && ps == 0) {
-                // values of word w in the order left by the words before it
-                KeyT* g = (KeyT*)((kin == (const KeyT*)B.ka) ? B.kb : B.ka);
-                if (kin == (const KeyT*)B.keys_in) g = (KeyT*)B.ka;
-                hipLaunchKernelGGL(k_gather_word, dim3(grid_for(T)), dim3(CH_WG), 0, st,
-                                   (const uint64_t*)B.keys_in + (uint64_t)w * T, vin, T, (uint64_t*)g);
-                kin = g;
-                kout = (g == (KeyT*)B.ka) ? (KeyT*)B.kb : (KeyT*)B.ka;
-            }
-            const uint32_t shift = 8 * ps;
-            const uint32_t ps_all = done_passes++;
-            tm.begin("sort_hist", T * sizeof(KeyT));
-            hipLaunchKernelGGL((k_sort_hist<KeyT>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin, T, shift,
-                               (uint32_t)sort_tiles, B.counts);
-            tm.end();
-            tm.begin("sort_scan", 2ull * 256 * sort_tiles * 4);
-            scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status, B.ticket, A.err, nullptr);
-            tm.end();
-            tm.begin("sort_scatter", ps_all == 0 ? T * sizeof(KeyT) + T * (sizeof(KeyT) + 4)
-                                                 : 2 * T * (sizeof(KeyT) + 4));
-            if (ps_all == 0)
-                hipLaunchKernelGGL((k_sort_scatter<KeyT, true>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
-                                   vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
-            else
-                hipLaunchKernelGGL((k_sort_scatter<KeyT, false>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st,
-                                   kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
-            tm.end();
-            kin = kout;
-            vin = vout;
-            kout = (kout == (KeyT*)B.ka) ? (KeyT*)B.kb : (KeyT*)B.ka;
-            vout = (vout == B.va) ? B.vb : B.va;
+
+        // corrections: per-tile counts -> offsets -> ordered gather
+        t_corr = tm.begin("corrections");
+        scan_u32(st, B.tile_ncorr, B.tile_corr_off, ntiles, B.status, B.ticket, A.err, &B.totals[2]);
+        with_dims(D, [&](auto dd) {
+            constexpr int DD = decltype(dd)::value;
+            hipLaunchKernelGGL((k_corr_gather<DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.corr_tmp_idx, B.corr_tmp_val,
+                               B.tile_ncorr, B.tile_corr_off, B.corr_idx, B.corr_val);
+        });
+        hipLaunchKernelGGL(k_frame_corr, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.tb, B.tile_corr_off,
+                           &B.totals[2]);
+        tm.end();
+        if (format == CHAIN_FMT_LCH1)
+            entropy_stage(st, tm, et, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
+                          B.corr_val, A.tb, ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err, B.ticket, X);
+    };
+    if (e->use_graphs) {
+        GraphKey key;
+        key.add(format).add(dtype).add(layout).add(D).add(F).add(T).add(nwords).add(key64).add(rel8);
+        key.add(dpts).add(e->arena.base).add(ntiles).add(ntiles1k).add(rc.lanes_max).add(rc.chunk_symbols);
+        for (uint32_t w = 0; w < CH_MAXD; ++w) key.add(word_passes[w]);
+        for (int c = 0; c < 5; ++c) key.add(comp_chunk_bound[c]);
+        for (uint32_t f = 0; f <= F; ++f) key.add(pt_off[f]);
+        const bool was = tm.on;
+        tm.begin("graph");
+        tm.on = false;
+        try {
+            run_graph(e, key.bytes, issue);
+        } catch (...) {
+            tm.on = was;
+            throw;
         }
-        // K4 link + decision
-        tm.begin("link", T * (sizeof(KeyT) * nwords + 8));
-        with_dims(D, [&](auto dd) {
-            constexpr int DD = decltype(dd)::value;
-            if (nwords == 1)
-                hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st, kin, vin, T, A.fdev,
-                                   A.ptb, F, B.idrel);
-            else
-                hipLaunchKernelGGL((k_link_wide<DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st,
-                                   (const uint64_t*)B.keys_in, vin, T, nwords, A.fdev, A.ptb, F, B.idrel);
-        });
+        tm.on = was;
         tm.end();
-        // K5 explicit compaction
-        t_explicit = tm.begin("explicit");
-        fill64(st, B.status, ntiles, 0);
-        fill32(st, B.ticket, 1, 0);
-        with_dims(D, [&](auto dd) {
-            constexpr int DD = decltype(dd)::value;
-            hipLaunchKernelGGL((k_explicit<KeyT, DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.idrel,
-                               (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, B.status, B.ticket, A.err, B.blocks,
-                               B.fo, &B.totals[0], nwords, T);
-        });
-        hipLaunchKernelGGL(k_frame_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[0]);
-        tm.end();
-    };
-    auto with_rel = [&](auto key_tag) {
-        if (rel8) run_typed(key_tag, TypeTag<uint8_t>{});
-        else run_typed(key_tag, TypeTag<uint32_t>{});
-    };
-    if (key64) with_rel(TypeTag<uint64_t>{});
-    else with_rel(TypeTag<uint32_t>{});
-
-    // K6/K7 L2
-    const int t_l2s = tm.begin("l2_stats");
-    // frame record bases as a flat array for the record-space kernels
-    hipLaunchKernelGGL(k_copy_rec_base, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, B.rec_base);
-    const uint64_t rec_tiles_max = cdiv(T, CH_TILE);
-    with_dims(D, [&](auto dd) {
-        constexpr int DD = decltype(dd)::value;
-        hipLaunchKernelGGL((k_l2_stats<DD>), dim3(grid_for(T, CH_WG * CH_ITEMS)), dim3(CH_WG), 0, st, B.blocks,
-                           &B.totals[0], F, B.rec_base, B.fo);
-        hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
-        tm.end();
-        t_l2p = tm.begin("l2_split");
-        fill64(st, B.status, rec_tiles_max, 0);
-        fill32(st, B.ticket, 1, 0);
-        hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)rec_tiles_max), dim3(CH_WG), 0, st, B.blocks,
-                           &B.totals[0], F, B.rec_base, A.fdev, B.fo, B.status, B.ticket, A.err, B.choice2, B.diff,
-                           B.keep, &B.totals[1]);
-    });
-    hipLaunchKernelGGL(k_frame_split_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[1],
-                       &B.totals[0]);
-    tm.end();
-
-    // corrections: per-tile counts -> offsets -> ordered gather
-    const int t_corr = tm.begin("corrections");
-    scan_u32(st, B.tile_ncorr, B.tile_corr_off, ntiles, B.status, B.ticket, A.err, &B.totals[2]);
-    with_dims(D, [&](auto dd) {
-        constexpr int DD = decltype(dd)::value;
-        hipLaunchKernelGGL((k_corr_gather<DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.corr_tmp_idx, B.corr_tmp_val,
-                           B.tile_ncorr, B.tile_corr_off, B.corr_idx, B.corr_val);
-    });
-    hipLaunchKernelGGL(k_frame_corr, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.tb, B.tile_corr_off,
-                       &B.totals[2]);
-    tm.end();
+    } else {
+        issue();
+    }
 
     const size_t kbytes_all = kbytes * nwords;
     auto set_pre_bytes = [&](uint64_t R, uint64_t NC) {
@@ -928,12 +1037,6 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     };
 
     if (format == CHAIN_FMT_LCH1) {
-        Carver cC;
-        cC.base = e->arena.base + size_a + size_b;
-        auto X = carve_c(cC);
-        EntropyTimes et;
-        entropy_stage(st, tm, et, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
-                      B.corr_val, A.tb, ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err, B.ticket, X);
         std::vector<uint64_t> offs(F), sizes(F);
         uint64_t total = 0;
         uint32_t errw[4];
@@ -1635,6 +1738,109 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     }
     throw Error(CHAIN_EINVAL, "decoded coordinates need a host or device destination");
 }
+// ------------------------------------------------------------------------
+// autotuning of the entropy stage's chunking
+
+struct TuneTrial {
+    uint32_t lanes_max, chunk_symbols;
+    double compress_ms, decompress_ms;
+    uint64_t bytes;
+};
+
+// Time compression + decompression of (a sample of) the batch for every
+// candidate chunk size and keep the fastest whose output is within 2 % of
+// the smallest. Frames are taken in order up to 2^24 points (at least one).
+// The containers are valid under every setting, so this only trades speed
+// against the few bytes of lane states per chunk.
+static RansConfig autotune_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
+                                uint32_t D, bool on_device, std::vector<TuneTrial>& trials) {
+    uint32_t Fs = 0;
+    uint64_t pts = 0;
+    while (Fs < F && (Fs == 0 || pts + frames[Fs].n <= (1ull << 24))) pts += frames[Fs++].n;
+    const RansConfig saved = e->rans;
+    const bool timer_was = e->timer.on;
+    const bool graphs_was = e->use_graphs;
+    e->timer.on = false;
+    e->use_graphs = false;
+    const uint32_t cands[4] = {8192, 16384, 32768, 65536};
+    try {
+        for (uint32_t cs : cands) {
+            e->rans = RansConfig{64, cs};
+            TuneTrial tr{64, cs, 0, 0, 0};
+            for (int rep = 0; rep < 2; ++rep) {   // the first run sizes the arenas
+                CompressResult cr;
+                cr.keep_on_device = true;
+                auto t0 = std::chrono::steady_clock::now();
+                compress_batch_impl(e, frames, Fs, dtype, layout, D, on_device, CHAIN_FMT_LCH1, cr);
+                auto t1 = std::chrono::steady_clock::now();
+                std::vector<std::vector<uint8_t>> host(Fs);
+                std::vector<const uint8_t*> ptrs(Fs);
+                std::vector<size_t> lens(Fs);
+                bool contiguous = true;
+                uint64_t total = 0;
+                for (uint32_t f = 0; f < Fs; ++f) {
+                    host[f].resize(cr.sizes[f]);
+                    if (cr.sizes[f])
+                        HIPCHK(hipMemcpyAsync(host[f].data(), cr.d_out + cr.offsets[f], cr.sizes[f],
+                                              hipMemcpyDeviceToHost, e->stream));
+                    contiguous = contiguous && cr.offsets[f] == cr.offsets[0] + total;
+                    total += cr.sizes[f];
+                }
+                HIPCHK(hipStreamSynchronize(e->stream));
+                for (uint32_t f = 0; f < Fs; ++f) {
+                    ptrs[f] = host[f].data();
+                    lens[f] = host[f].size();
+                }
+                DecodeResult dr;
+                dr.keep_on_device = true;
+                dr.d_pool = contiguous ? cr.d_out + cr.offsets[0] : nullptr;
+                auto t2 = std::chrono::steady_clock::now();
+                decompress_batch_impl(e, ptrs.data(), lens.data(), Fs, CHAIN_F64, CHAIN_AOS, dr);
+                auto t3 = std::chrono::steady_clock::now();
+                tr.compress_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+                tr.decompress_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+                tr.bytes = total;
+            }
+            trials.push_back(tr);
+        }
+    } catch (...) {
+        e->rans = saved;
+        e->timer.on = timer_was;
+        e->use_graphs = graphs_was;
+        throw;
+    }
+    e->timer.on = timer_was;
+    e->use_graphs = graphs_was;
+    uint64_t best_bytes = ~0ull;
+    for (const TuneTrial& t : trials) best_bytes = std::min(best_bytes, t.bytes);
+    const TuneTrial* best = nullptr;
+    for (const TuneTrial& t : trials)
+        if (t.bytes <= best_bytes + best_bytes / 50 &&
+            (!best || t.compress_ms + t.decompress_ms < best->compress_ms + best->decompress_ms))
+            best = &t;
+    return RansConfig{best->lanes_max, best->chunk_symbols};
+}
+
+// automatic mode: one tuned setting per (log2 mean frame size, log2 frames)
+static uint32_t tune_bucket(const chain_frame* frames, uint32_t F) {
+    uint64_t T = 0;
+    for (uint32_t f = 0; f < F; ++f) T += frames[f].n;
+    const uint32_t a = (uint32_t)bitlen64(T / F), b = (uint32_t)bitlen64(F);
+    return (a << 8) | b;
+}
+
+static void apply_autotune(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
+                           uint32_t D, bool on_device, int format) {
+    if (!e->autotune || format != CHAIN_FMT_LCH1 || F == 0) return;
+    const uint32_t key = tune_bucket(frames, F);
+    auto it = e->tuned.find(key);
+    if (it == e->tuned.end()) {
+        std::vector<TuneTrial> trials;
+        it = e->tuned.emplace(key, autotune_impl(e, frames, F, dtype, layout, D, on_device, trials)).first;
+    }
+    e->rans = it->second;
+}
+
 }  // namespace chain
 
 // ------------------------------------------------------------------------
@@ -1725,6 +1931,7 @@ int chain_compress(chain_engine* e, const void* points, int dtype, int layout, u
     return guarded(e, [&] {
         if (!out) throw Error(CHAIN_EINVAL, "out must not be NULL");
         chain_frame fr{points, n, eb, p, threshold, flags};
+        chain::apply_autotune(e, &fr, 1, dtype, layout, dims, false, format);
         chain::CompressResult res;
         res.host_out = out;
         chain::compress_batch_impl(e, &fr, 1, dtype, layout, dims, false, format, res);
@@ -1737,6 +1944,7 @@ int chain_compress_batch(chain_engine* e, const chain_frame* frames, uint32_t nf
                          uint32_t dims, int points_on_device, int format, chain_buf* outs, chain_stats* stats) {
     return guarded(e, [&] {
         if (!frames || !outs) throw Error(CHAIN_EINVAL, "frames and outs must not be NULL");
+        chain::apply_autotune(e, frames, nframes, dtype, layout, dims, points_on_device != 0, format);
         chain::CompressResult res;
         res.host_out = outs;
         chain::compress_batch_impl(e, frames, nframes, dtype, layout, dims, points_on_device != 0, format, res);
@@ -1751,6 +1959,7 @@ int chain_compress_batch_device(chain_engine* e, const chain_frame* frames, uint
                                 uint32_t dims, void** d_out, uint64_t* offsets, uint64_t* sizes, chain_stats* stats) {
     return guarded(e, [&] {
         if (!frames || !d_out || !offsets || !sizes) throw Error(CHAIN_EINVAL, "NULL argument");
+        chain::apply_autotune(e, frames, nframes, dtype, layout, dims, true, CHAIN_FMT_LCH1);
         chain::CompressResult res;
         res.keep_on_device = true;
         chain::compress_batch_impl(e, frames, nframes, dtype, layout, dims, true, CHAIN_FMT_LCH1, res);
@@ -1832,7 +2041,9 @@ int chain_device_free(chain_engine* e, void* p) {
 
 int chain_copy(chain_engine* e, void* dst, const void* src, uint64_t bytes, int to_device) {
     return guarded(e, [&] {
-        HIPCHK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, e->stream));
+        const hipMemcpyKind kind = to_device == 2 ? hipMemcpyDeviceToDevice
+                                   : to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, kind, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
     });
 }
@@ -1854,6 +2065,40 @@ int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, siz
         names[names_len - 1] = 0;
     }
     return k;
+}
+
+int chain_engine_set_autotune(chain_engine* e, int on) {
+    return guarded(e, [&] { e->autotune = on != 0; });
+}
+
+int chain_engine_autotune(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
+                          uint32_t dims, int points_on_device, uint32_t* lanes_max, uint32_t* chunk_symbols,
+                          float* trial_ms, uint64_t* trial_bytes) {
+    return guarded(e, [&] {
+        if (!frames || nframes == 0) throw Error(CHAIN_EINVAL, "empty batch");
+        std::vector<chain::TuneTrial> trials;
+        const chain::RansConfig best =
+            chain::autotune_impl(e, frames, nframes, dtype, layout, dims, points_on_device != 0, trials);
+        e->rans = best;
+        e->tuned[chain::tune_bucket(frames, nframes)] = best;
+        if (lanes_max) *lanes_max = best.lanes_max;
+        if (chunk_symbols) *chunk_symbols = best.chunk_symbols;
+        for (size_t k = 0; k < trials.size(); ++k) {
+            if (trial_ms) trial_ms[k] = (float)(trials[k].compress_ms + trials[k].decompress_ms);
+            if (trial_bytes) trial_bytes[k] = trials[k].bytes;
+        }
+    });
+}
+
+int chain_engine_set_graphs(chain_engine* e, int on) {
+    return guarded(e, [&] { e->use_graphs = on != 0; });
+}
+
+int chain_engine_graph_stats(chain_engine* e, uint64_t* captures, uint64_t* replays) {
+    return guarded(e, [&] {
+        if (captures) *captures = e->graph.captures;
+        if (replays) *replays = e->graph.replays;
+    });
 }
 
 int chain_engine_set_timing(chain_engine* e, int on) {

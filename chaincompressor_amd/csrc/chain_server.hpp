@@ -9,11 +9,13 @@
 // launch per stage. Requests that arrive while batches are in flight wait
 // only until a worker frees up; nothing waits for a batch to fill.
 //
-// HBM admission: the budget is divided evenly between the workers. A batch
-// is formed only while the estimated device bytes of its requests (arena
-// sizes of the compress / decompress paths) fit in the worker's share, and
-// the worker's engine enforces the share on its arenas. A request larger
-// than one share is refused with CHAIN_ENOMEM at submission.
+// HBM admission: the budget is divided evenly between the workers, and each
+// worker's share is allocated once, at creation, as its two arenas: three
+// quarters for compression, one quarter for decompression (device memory
+// stays resident; nothing is allocated or freed while serving). A batch is
+// formed only while the estimated device bytes of its requests fit in the
+// arena of its kind; a request larger than that is refused with
+// CHAIN_ENOMEM at submission.
 //
 // The integer and bitstream arithmetic is that of the engine: a request's
 // result does not depend on the batch it ran in. If a batch fails, each of
@@ -63,6 +65,7 @@ inline uint64_t serve_estimate_decompress(uint64_t n, uint32_t D, uint32_t es_ou
 struct Server {
     int device = 0;
     uint64_t budget = 0, share = 0;
+    uint64_t cap_c = 0, cap_d = 0;   // per-worker arena capacity per kind
     uint32_t max_batch = 64;
     uint64_t max_batch_points = 1ull << 27;
     std::vector<chain_engine*> engines;
@@ -100,7 +103,8 @@ struct Server {
                 ++it;
                 continue;
             }
-            if (!first && (batch.size() >= max_batch || bytes + q.est > share || points + q.req.n > max_batch_points))
+            const uint64_t cap = q.req.kind == CHAIN_REQ_COMPRESS ? cap_c : cap_d;
+            if (!first && (batch.size() >= max_batch || bytes + q.est > cap || points + q.req.n > max_batch_points))
                 break;
             bytes += q.est;
             points += q.req.n;
@@ -240,17 +244,26 @@ int chain_server_create(int device, uint64_t hbm_budget_bytes, uint32_t workers,
     s->device = device;
     s->budget = hbm_budget_bytes;
     s->share = hbm_budget_bytes / workers;
+    s->cap_d = s->share / 4;
+    s->cap_c = s->share - s->cap_d;
     s->max_batch = max_batch ? max_batch : 64;
     for (uint32_t w = 0; w < workers; ++w) {
         chain_engine* e = nullptr;
-        const int rc = chain_engine_create(device, s->share, &e);
+        int rc = chain_engine_create(device, s->share, &e);
+        if (rc == CHAIN_OK) {
+            s->engines.push_back(e);
+            e->shared_budget = true;
+            e->timer.on = false;
+            rc = guarded(e, [&] {
+                e->arena.ensure(s->cap_c, s->cap_c);
+                e->arena_d.ensure(s->cap_d, s->cap_d);
+            });
+            if (rc != CHAIN_OK) g_create_err = e->err;
+        }
         if (rc != CHAIN_OK) {
             delete s;
             return rc;
         }
-        e->shared_budget = true;
-        e->timer.on = false;
-        s->engines.push_back(e);
     }
     for (chain_engine* e : s->engines) s->threads.emplace_back([s, e] { s->worker(e); });
     *out = s;
@@ -296,7 +309,8 @@ int chain_serve_submit(chain_server* s, const chain_request* r, uint64_t* ticket
     } else {
         return fail(CHAIN_EINVAL, "unknown request kind");
     }
-    if (it.est > s->share) return fail(CHAIN_ENOMEM, "request exceeds one worker's share of the HBM budget");
+    if (it.est > (r->kind == CHAIN_REQ_COMPRESS ? s->cap_c : s->cap_d))
+        return fail(CHAIN_ENOMEM, "request exceeds one worker's share of the HBM budget");
     it.ticket = s->next_ticket++;
     *ticket = it.ticket;
     s->queue.push_back(it);

@@ -53,7 +53,9 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_engine_set_rans", "chain_decompress", "chain_decompress_batch", "chain_decompress_batch_device",
            "chain_device_malloc", "chain_device_free", "chain_copy", "chain_last_stage_times",
            "chain_last_stage_bytes", "chain_engine_set_timing", "chain_server_create", "chain_server_destroy",
-           "chain_server_last_error", "chain_serve_submit", "chain_serve_poll", "chain_server_clock"]
+           "chain_server_last_error", "chain_serve_submit", "chain_serve_poll", "chain_server_clock",
+           "chain_engine_set_graphs", "chain_engine_graph_stats", "chain_engine_autotune",
+           "chain_engine_set_autotune"]
 
 CHAIN_PENDING = 5
 REQ_COMPRESS, REQ_DECOMPRESS = 0, 1
@@ -110,6 +112,11 @@ def load_library():
     L.chain_last_stage_times.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_char_p, C.c_size_t]
     L.chain_last_stage_bytes.argtypes = [C.c_void_p, P(C.c_uint64), C.c_int]
     L.chain_engine_set_timing.argtypes = [C.c_void_p, C.c_int]
+    L.chain_engine_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.chain_engine_set_autotune.argtypes = [C.c_void_p, C.c_int]
+    L.chain_engine_autotune.argtypes = [C.c_void_p, P(_Frame), C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_int,
+                                        P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_uint64)]
+    L.chain_engine_graph_stats.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]
     L.chain_server_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_void_p)]
     L.chain_server_destroy.argtypes = [C.c_void_p]
     L.chain_server_last_error.argtypes = [C.c_void_p]
@@ -200,6 +207,36 @@ class Engine:
     def set_timing(self, on: bool):
         """Per-stage HIP event timing (on by default)."""
         self._check(self._L.chain_engine_set_timing(self._h, 1 if on else 0))
+
+    def set_autotune(self, on: bool):
+        """Tune the entropy stage's chunking once per batch size bucket."""
+        self._check(self._L.chain_engine_set_autotune(self._h, 1 if on else 0))
+
+    def autotune(self, ptrs, ns, dtype, dims, eb, p, threshold, on_device=True, layout="aos"):
+        """Tune on a batch (device pointers, or host arrays with on_device=False);
+        returns ((lanes_max, chunk_symbols), [(chunk_symbols, ms, bytes)])."""
+        F = len(ptrs)
+        fr = (_Frame * F)()
+        for k in range(F):
+            fr[k] = _Frame(ptrs[k], int(ns[k]), float(eb), int(p), int(threshold), 0)
+        lm, cs = C.c_uint32(), C.c_uint32()
+        ms = (C.c_float * 4)()
+        by = (C.c_uint64 * 4)()
+        self._check(self._L.chain_engine_autotune(self._h, fr, F, _dtype_code(dtype),
+                                                  CHAIN_AOS if layout == "aos" else CHAIN_SOA, dims,
+                                                  1 if on_device else 0, C.byref(lm), C.byref(cs), ms, by))
+        trials = [(c, float(ms[k]), int(by[k])) for k, c in enumerate((8192, 16384, 32768, 65536))]
+        return (int(lm.value), int(cs.value)), trials
+
+    def set_graphs(self, on: bool):
+        """Capture compression launches in a HIP graph and replay it for
+        batches of the same launch shape."""
+        self._check(self._L.chain_engine_set_graphs(self._h, 1 if on else 0))
+
+    def graph_stats(self):
+        c, r = C.c_uint64(), C.c_uint64()
+        self._check(self._L.chain_engine_graph_stats(self._h, C.byref(c), C.byref(r)))
+        return int(c.value), int(r.value)
 
     def compress_batch_device(self, ptrs, ns, dtype, dims, eb, p, threshold, flags=0, layout="aos"):
         """Device-resident LCH1 batch: returns (device pointer, offsets, sizes, stats)."""
@@ -327,6 +364,9 @@ class Engine:
     def to_device(self, ptr: int, data) -> None:
         buf = np.ascontiguousarray(data)
         self._check(self._L.chain_copy(self._h, C.c_void_p(ptr), buf.ctypes.data_as(C.c_void_p), buf.nbytes, 1))
+
+    def copy_device(self, dst: int, src: int, nbytes: int) -> None:
+        self._check(self._L.chain_copy(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes, 2))
 
     def to_host(self, ptr: int, nbytes: int) -> bytes:
         out = (C.c_uint8 * max(1, nbytes))()

@@ -123,10 +123,32 @@ int chain_decompress_batch_device(chain_engine* e, const uint8_t* const* bytes, 
                                   void** d_out, uint64_t* out_offsets, uint64_t* ns, uint32_t* dims);
 
 /* Device buffers for callers that keep frames in HBM without another
- * runtime (synchronous copies on the engine's stream). */
+ * runtime (synchronous copies on the engine's stream; to_device: 1 host to
+ * device, 0 device to host, 2 device to device). */
 int chain_device_malloc(chain_engine* e, uint64_t bytes, void** out);
 int chain_device_free(chain_engine* e, void* p);
 int chain_copy(chain_engine* e, void* dst, const void* src, uint64_t bytes, int to_device);
+
+/* Autotuning of the entropy stage: compress + decompress (a sample of up to
+ * 2^24 points of) the given batch with 64 lanes and 8192, 16384, 32768 and
+ * 65536 symbols per chunk, keep the fastest setting whose output is within
+ * 2 % of the smallest, apply it to the engine and remember it for the
+ * batch's size bucket (log2 of the mean frame size and of the frame count).
+ * trial_ms / trial_bytes (4 entries each, may be NULL) report every trial.
+ * With chain_engine_set_autotune(e, 1), every compression whose bucket has
+ * no setting yet runs this first. Containers stay valid under any setting. */
+int chain_engine_autotune(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
+                          uint32_t dims, int points_on_device, uint32_t* lanes_max, uint32_t* chunk_symbols,
+                          float* trial_ms, uint64_t* trial_bytes);
+int chain_engine_set_autotune(chain_engine* e, int on);
+
+/* Compression through a HIP graph: every launch after the planning
+ * synchronisation is captured once and the executable graph is replayed
+ * while batches keep the same launch shape (frame sizes, dims, dtype, input
+ * and arena addresses); a new shape is captured again and the executable
+ * graph updated in place. Results are identical either way. Off by default. */
+int chain_engine_set_graphs(chain_engine* e, int on);
+int chain_engine_graph_stats(chain_engine* e, uint64_t* captures, uint64_t* replays);
 
 /* Per-stage HIP event timing of every call (on by default; off saves one
  * event pair per stage, for latency-sensitive callers such as a server). */

@@ -1483,6 +1483,9 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     for (uint32_t k = 0; k < nstreams; ++k) parse_stream(bytes[k], in_off[k], lens[k], P);
     const uint32_t F = nstreams, D = P.D;
     require(P.T < (1ull << 31), CHAIN_EINVAL, "a GPU decode batch holds fewer than 2^31 particles");
+    // in-block offsets fit a byte when every frame's p is at most 256
+    uint32_t rel8 = 1;
+    for (const DecFrame& fd : P.frames) rel8 &= fd.p <= 256 ? 1u : 0u;
     // tiles: points (chain resolution) and records (L2 inverse), per frame
     std::vector<uint32_t> ptb(F + 1, 0), rtb(F + 1, 0), ptb32(F + 1, 0);
     for (uint32_t f = 0; f < F; ++f) {
@@ -1513,7 +1516,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             DecRaw* raws;
             DecTagged* tagged;
             uint32_t* cum;
-            uint32_t *idrel, *rel;
+            uint32_t* idrel;
+            void* rel;
             uint64_t *diff, *keep;
             uint8_t *choice, *tags;
             uint64_t* compact;
@@ -1525,6 +1529,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             uint32_t* flags;
             uint64_t* aggv;
             uint32_t* done;
+            uint32_t* jump_open;   // per-round open-list counts
+            uint32_t *list_a, *list_b;
             uint32_t* ticket;
             uint32_t* err;
             uint64_t* totals;
@@ -1540,7 +1546,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.tagged = c.take<DecTagged>(P.tagged.size());
         x.cum = c.take<uint32_t>(P.cum.size());
         x.idrel = c.take<uint32_t>(T);
-        x.rel = c.take<uint32_t>(T * D);
+        x.rel = c.take<uint8_t>(T * D * (rel8 ? 1 : 4));
         x.diff = c.take<uint64_t>(P.ND * D);
         x.keep = c.take<uint64_t>(P.NK * D);
         x.choice = c.take<uint8_t>(P.R);
@@ -1562,6 +1568,9 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.flags = c.take<uint32_t>(nrt + 1);
         x.aggv = c.take<uint64_t>((uint64_t)(nrt + 1) * 2 * CH_MAXD);
         x.done = c.take<uint32_t>(npt + 1);
+        x.jump_open = c.take<uint32_t>(64);
+        x.list_a = c.take<uint32_t>(T);
+        x.list_b = c.take<uint32_t>(T);
         x.ticket = c.take<uint32_t>(4);
         x.err = c.take<uint32_t>(4);
         x.totals = c.take<uint64_t>(4);
@@ -1578,7 +1587,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     Staging& sg = e->stage;
     auto h2d = [&](void* dst, const void* src, size_t n) { sg.h2d(st, dst, src, n); };
     // algorithmic bytes of every timed region, from the plan
-    const uint64_t comp_es[5] = {4, 8, 8, 1, 4};
+    const uint64_t comp_es[5] = {4, 8, 8, 1, rel8 ? 1u : 4u};
     auto es_of = [&](int32_t comp) -> uint64_t { return comp < 0 ? 1 : comp_es[comp]; };
     uint64_t b_rans = 0, b_fixed = 0, b_raw = 0, b_tagged = 0;
     for (const DecChunk& ch : P.chunks) b_rans += ch.bytes;
@@ -1614,18 +1623,18 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     tm.begin("rans_decode", b_rans);
     if (!P.chunks.empty())
         hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.jobs.size()), dim3(64 * DEC_WAVES), 0, st, X.in, X.secs,
-                           X.chunks, X.jobs, X.cum, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+                           X.chunks, X.jobs, X.cum, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
     tm.end();
     if (P.fixed_total) {
         tm.begin("unpack_fixed", b_fixed);
         hipLaunchKernelGGL(k_dec_fixed, dim3(grid_for(P.fixed_total)), dim3(CH_WG), 0, st, X.in, X.fixed,
-                           (uint32_t)P.fixed.size(), P.fixed_total, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+                           (uint32_t)P.fixed.size(), P.fixed_total, X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
         tm.end();
     }
     if (P.raw_total) {
         tm.begin("unpack_raw", b_raw);
         hipLaunchKernelGGL(k_dec_raw, dim3(grid_for(P.raw_total)), dim3(CH_WG), 0, st, X.in, X.raws,
-                           (uint32_t)P.raws.size(), P.raw_total, X.idrel, X.diff, X.keep, X.choice, X.rel, X.err);
+                           (uint32_t)P.raws.size(), P.raw_total, X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
         tm.end();
     }
     if (P.tag_tiles) {
@@ -1634,7 +1643,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         fill32(st, X.ticket, 1, 0);
         hipLaunchKernelGGL(k_dec_tagged, dim3((unsigned)P.tag_tiles), dim3(CH_WG), 0, st, X.in, X.tagged,
                            (uint32_t)P.tagged.size(), X.tags, X.status, X.ticket, X.idrel, X.diff, X.keep, X.choice,
-                           X.rel, X.err);
+                           X.rel, rel8, X.err);
         tm.end();
     }
     if (nrt) {
@@ -1664,41 +1673,52 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     // write roots) is stated once, on the local pass; the jump rounds over
     // tile seams are extra work of this formulation
     tm.begin("chain_local", 8 * T);
-    with_dims(D, [&](auto dd) {
-        constexpr int DD = decltype(dd)::value;
-        hipLaunchKernelGGL((k_dec_chain_local<DD>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.idrel, X.eoff,
-                           X.root, X.err);
-    });
-    tm.end();
-    tm.begin("chain_jump");
+    uint32_t rounds = 1;
     {
         uint32_t maxt = 1;
         for (uint32_t f = 0; f < F; ++f) maxt = std::max(maxt, ptb[f + 1] - ptb[f]);
-        uint32_t rounds = 1;
         while ((1u << (rounds - 1)) < maxt) ++rounds;   // ceil(log2(tiles)) + 1
-        for (uint32_t r = 0; r < rounds; ++r)
-            hipLaunchKernelGGL(k_dec_chain_jump, dim3(grid_for(T)), dim3(CH_WG), 0, st, X.root, T);
     }
-    tm.end();
-    tm.begin("dequantize", 4 * T + 4 * T * D + 8ull * D * P.R + T * D * es);
+    fill32(st, X.jump_open, rounds + 1, 0);
     with_dims(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
-        auto go = [&](auto ot) {
+        hipLaunchKernelGGL((k_dec_chain_local<DD>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.idrel, X.eoff,
+                           X.root, X.list_a, &X.jump_open[0], X.err);
+    });
+    tm.end();
+    tm.begin("chain_jump");
+    for (uint32_t r = 0; r < rounds; ++r) {
+        uint32_t* lin = (r & 1) ? X.list_b : X.list_a;
+        uint32_t* lout = (r & 1) ? X.list_a : X.list_b;
+        hipLaunchKernelGGL(k_dec_chain_jump, dim3((unsigned)std::min<uint64_t>(cdiv(T, CH_TILE), 2048)), dim3(CH_WG), 0,
+                           st, X.root,
+                           lin, &X.jump_open[r], lout, &X.jump_open[r + 1]);
+    }
+    tm.end();
+    tm.begin("dequantize", 4 * T + comp_es[4] * T * D + 8ull * D * P.R + T * D * es);
+    with_dims(D, [&](auto dd) {
+        constexpr int DD = decltype(dd)::value;
+        auto go = [&](auto ot, auto rt) {
             using O = typename decltype(ot)::type;
+            using RT = typename decltype(rt)::type;
             if (out_layout == CHAIN_SOA) {
-                hipLaunchKernelGGL((k_dec_dequant<DD, O, true>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
-                                   X.ptb32, T, X.root, X.compact, X.rel, (O*)X.out, X.err);
+                hipLaunchKernelGGL((k_dec_dequant<DD, O, true, RT>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
+                                   X.ptb32, T, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
                 hipLaunchKernelGGL((k_dec_corr<O, true>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
                                    (O*)X.out, X.err);
             } else {
-                hipLaunchKernelGGL((k_dec_dequant<DD, O, false>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
-                                   X.ptb32, T, X.root, X.compact, X.rel, (O*)X.out, X.err);
+                hipLaunchKernelGGL((k_dec_dequant<DD, O, false, RT>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
+                                   X.ptb32, T, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
                 hipLaunchKernelGGL((k_dec_corr<O, false>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
                                    (O*)X.out, X.err);
             }
         };
-        if (out_dtype == CHAIN_F32) go(TypeTag<float>{});
-        else go(TypeTag<double>{});
+        auto go_rel = [&](auto ot) {
+            if (rel8) go(ot, TypeTag<uint8_t>{});
+            else go(ot, TypeTag<uint32_t>{});
+        };
+        if (out_dtype == CHAIN_F32) go_rel(TypeTag<float>{});
+        else go_rel(TypeTag<double>{});
     });
     tm.end();
     uint32_t errw[4];

@@ -42,14 +42,24 @@ struct DecFrame {
 };
 
 // destination of a decoded value: component c of frame f, value index k
+// in-block offsets are stored as u8 when every frame of the batch has
+// p <= 256 (rel8), else as u32; larger values are format errors either way
 __device__ inline void store_value(int c, uint64_t gidx, uint64_t v, uint32_t* idrel, uint64_t* diff, uint64_t* keep,
-                                   uint8_t* choice, uint32_t* rel, uint32_t* err) {
+                                   uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
     switch (c) {
         case 0: if (v >> 32) atomicOr(err, DERR_VALUE); idrel[gidx] = (uint32_t)v; break;
         case 1: diff[gidx] = v; break;
         case 2: keep[gidx] = v; break;
         case 3: if (v > 1) atomicOr(err, DERR_VALUE); choice[gidx] = (uint8_t)v; break;
-        default: if (v >> 32) atomicOr(err, DERR_VALUE); rel[gidx] = (uint32_t)v; break;
+        default:
+            if (rel8) {
+                if (v > 0xffu) atomicOr(err, DERR_QUANT);
+                static_cast<uint8_t*>(rel)[gidx] = (uint8_t)v;
+            } else {
+                if (v >> 32) atomicOr(err, DERR_VALUE);
+                static_cast<uint32_t*>(rel)[gidx] = (uint32_t)v;
+            }
+            break;
     }
 }
 
@@ -94,7 +104,7 @@ __global__ __launch_bounds__(64 * DEC_WAVES) void k_dec_rans(const uint8_t* __re
                                                              const uint32_t* __restrict__ cum_pool,
                                                              uint8_t* __restrict__ tags, uint32_t* idrel,
                                                              uint64_t* diff, uint64_t* keep, uint8_t* choice,
-                                                             uint32_t* rel, uint32_t* err) {
+                                                             void* rel, uint32_t rel8, uint32_t* err) {
     __shared__ uint32_t s_cum[RANS_VALUE_LIMIT + 1];
     __shared__ uint16_t s_bucket[DEC_BUCKETS + 1];
     __shared__ uint8_t s_ring[DEC_WAVES][DEC_RING];
@@ -187,7 +197,7 @@ __global__ __launch_bounds__(64 * DEC_WAVES) void k_dec_rans(const uint8_t* __re
             x = (s_cum[s + 1] - c0) * (x >> pb) + slot - c0;
             const uint64_t q = ch.first + k;
             if (S.comp < 0) tags[S.dst + q] = (uint8_t)s;
-            else store_value(S.comp, S.dst + q, s, idrel, diff, keep, choice, rel, err);
+            else store_value(S.comp, S.dst + q, s, idrel, diff, keep, choice, rel, rel8, err);
         }
         // a valid stream needs at most ceil(pb / 8) <= 3 bytes per step
         for (int round = 0; round < 4; ++round) {
@@ -224,7 +234,7 @@ struct DecFixed {
 // one thread per record: every dimension's bits, written together
 __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__ in, const DecFixed* __restrict__ jobs,
                                                      uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
-                                                     uint64_t* keep, uint8_t* choice, uint32_t* rel, uint32_t* err) {
+                                                     uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
     for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < total; g += (uint64_t)gridDim.x * CH_WG) {
         uint32_t lo = 0, hi = njobs;
         while (hi - lo > 1) {
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__
             uint64_t v = drop ? ((acc >> drop) | (acc_hi << (64 - drop))) : acc;
             v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
             if (v > (uint64_t)INT64_MAX - J.minimum[d]) atomicOr(err, DERR_VALUE);
-            store_value(J.comp, J.dst + r * J.dims + d, J.minimum[d] + v, idrel, diff, keep, choice, rel, err);
+            store_value(J.comp, J.dst + r * J.dims + d, J.minimum[d] + v, idrel, diff, keep, choice, rel, rel8, err);
         }
     }
 }
@@ -263,7 +273,7 @@ struct DecRaw {
 
 __global__ __launch_bounds__(CH_WG) void k_dec_raw(const uint8_t* __restrict__ in, const DecRaw* __restrict__ jobs,
                                                    uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
-                                                   uint64_t* keep, uint8_t* choice, uint32_t* rel, uint32_t* err) {
+                                                   uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
     for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < total; g += (uint64_t)gridDim.x * CH_WG) {
         uint32_t lo = 0, hi = njobs;
         while (hi - lo > 1) {
@@ -276,7 +286,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_raw(const uint8_t* __restrict__ i
         uint64_t v = 0;
         for (int b = 0; b < 8; ++b) v |= (uint64_t)p[b] << (8 * b);
         if ((int64_t)v < 0) atomicOr(err, DERR_VALUE);
-        store_value(J.comp, J.dst + k, v, idrel, diff, keep, choice, rel, err);
+        store_value(J.comp, J.dst + k, v, idrel, diff, keep, choice, rel, rel8, err);
     }
 }
 
@@ -296,7 +306,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
                                                       const DecTagged* __restrict__ jobs, uint32_t njobs,
                                                       const uint8_t* __restrict__ tags, uint64_t* status,
                                                       uint32_t* ticket, uint32_t* idrel, uint64_t* diff,
-                                                      uint64_t* keep, uint8_t* choice, uint32_t* rel,
+                                                      uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8,
                                                       uint32_t* err) {
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
@@ -342,7 +352,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
             uint64_t v = drop ? ((acc >> drop) | (drop < 64 ? (acc_hi << (64 - drop)) : 0)) : acc;
             v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
             if (B + nb > J.bytes) atomicOr(err, DERR_VALUE);
-            store_value(J.comp, J.dst + (r0 + k) * J.dims + d, v, idrel, diff, keep, choice, rel, err);
+            store_value(J.comp, J.dst + (r0 + k) * J.dims + d, v, idrel, diff, keep, choice, rel, rel8, err);
             bit += w;
         }
     }
@@ -573,38 +583,61 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
                                                            const uint32_t* __restrict__ tile_begin,
                                                            const uint32_t* __restrict__ idrel,
                                                            const uint32_t* __restrict__ tile_expl_off,
-                                                           uint32_t* __restrict__ res, uint32_t* err) {
-    __shared__ uint32_t s_tmp[4];
+                                                           uint32_t* __restrict__ res, uint32_t* __restrict__ open_list,
+                                                           uint32_t* __restrict__ open_count, uint32_t* err) {
+    constexpr uint32_t W = CH_WG / 64;
     __shared__ uint32_t s_val[CH_TILE];
     __shared__ uint32_t s_ext[CH_TILE];
+    __shared__ uint32_t s_off[CH_ITEMS * W];
     const uint32_t tile = blockIdx.x;
     const uint32_t f = upper_find(tile_begin, F, tile);
     const DecFrame& Fr = fr[f];
     const uint64_t p0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
     const uint64_t p1 = p0 + CH_TILE < Fr.n ? p0 + CH_TILE : Fr.n;
     const uint32_t cnt = (uint32_t)(p1 - p0);
-    // explicit ranks (blocked: thread t owns points t*16 .. t*16+15)
-    uint32_t z = 0;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    const uint32_t* src = idrel + Fr.pt_off + p0;
+    // striped: item k of thread t is point k*CH_WG + t of the tile, so rows of
+    // 256 points are read, ranked and written together. Explicit ranks in
+    // index order: ballot prefix inside the wave + scanned (row, wave) counts.
+    uint32_t dv[CH_ITEMS], pre[CH_ITEMS];
+#pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint32_t j = threadIdx.x * CH_ITEMS + k;
-        z += (j < cnt && idrel[Fr.pt_off + p0 + j] == 0) ? 1u : 0u;
+        const uint32_t j = k * CH_WG + threadIdx.x;
+        dv[k] = j < cnt ? src[j] : 1u;
+        const uint64_t m = __ballot(j < cnt && dv[k] == 0);
+        pre[k] = __popcll(m & lt);
+        if (lane == 0) s_off[k * W + wave] = __popcll(m);
     }
-    uint32_t ztot;
-    uint32_t rank = tile_expl_off[tile] + block_excl_scan(z, s_tmp, &ztot);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t v = threadIdx.x < CH_ITEMS * W ? s_off[threadIdx.x] : 0u;
+        const uint32_t inc = wave_incl_scan(v);
+        if (threadIdx.x < CH_ITEMS * W) s_off[threadIdx.x] = inc - v;
+    }
+    __syncthreads();
+    const uint32_t rank0 = tile_expl_off[tile];
     // s_val: LOCAL | j (pointer inside the tile) or a final explicit index;
     // s_ext: global successor index for pointers leaving the tile (else ~0)
+#pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint32_t j = threadIdx.x * CH_ITEMS + k;
-        if (j >= cnt) break;
-        const uint32_t d = idrel[Fr.pt_off + p0 + j];
+        const uint32_t j = k * CH_WG + threadIdx.x;
+        if (j >= cnt) continue;
         s_ext[j] = 0xffffffffu;
-        if (d == 0) {
-            s_val[j] = rank++;
+        if (dv[k] == 0) {
+            s_val[j] = rank0 + s_off[k * W + wave] + pre[k];
         } else {
-            const uint64_t t = p0 + j + d;
-            if (t >= Fr.n) { atomicOr(err, DERR_CHAIN); s_val[j] = 0; continue; }
-            if (t < p1) s_val[j] = CH_LOCAL | (uint32_t)(t - p0);
-            else { s_val[j] = CH_LOCAL | j; s_ext[j] = (uint32_t)(Fr.pt_off + t); }
+            const uint64_t t = p0 + j + dv[k];
+            if (t >= Fr.n) {
+                atomicOr(err, DERR_CHAIN);
+                s_val[j] = 0;
+            } else if (t < p1) {
+                s_val[j] = CH_LOCAL | (uint32_t)(t - p0);
+            } else {
+                s_val[j] = CH_LOCAL | j;
+                s_ext[j] = (uint32_t)(Fr.pt_off + t);
+            }
         }
     }
     __syncthreads();
@@ -613,9 +646,11 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
     for (int round = 0; round < 12; ++round) {
         uint32_t nv[CH_ITEMS], ne[CH_ITEMS];
         int open = 0;
+#pragma unroll
         for (int k = 0; k < CH_ITEMS; ++k) {
-            const uint32_t j = threadIdx.x * CH_ITEMS + k;
-            nv[k] = 0; ne[k] = 0xffffffffu;
+            const uint32_t j = k * CH_WG + threadIdx.x;
+            nv[k] = 0;
+            ne[k] = 0xffffffffu;
             if (j >= cnt) continue;
             nv[k] = s_val[j];
             ne[k] = s_ext[j];
@@ -627,29 +662,117 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
             }
         }
         __syncthreads();
+#pragma unroll
         for (int k = 0; k < CH_ITEMS; ++k) {
-            const uint32_t j = threadIdx.x * CH_ITEMS + k;
-            if (j < cnt) { s_val[j] = nv[k]; s_ext[j] = ne[k]; }
+            const uint32_t j = k * CH_WG + threadIdx.x;
+            if (j < cnt) {
+                s_val[j] = nv[k];
+                s_ext[j] = ne[k];
+            }
         }
         if (!__syncthreads_or(open)) break;   // every in-tile pointer resolved
     }
+    // results; points whose chain leaves the tile go to the open list, in
+    // tile order, with one reservation per tile
+    uint32_t* dst = res + Fr.pt_off + p0;
+    uint32_t ext_bits = 0;
+#pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint32_t j = threadIdx.x * CH_ITEMS + k;
-        if (j >= cnt) break;
-        const uint32_t v = s_val[j], g2 = s_ext[j];
-        uint32_t out;
-        if (g2 != 0xffffffffu) out = CH_EXT | g2;
-        else if (v & CH_LOCAL) { atomicOr(err, DERR_CHAIN); out = 0; }
-        else out = v;
-        res[Fr.pt_off + p0 + j] = out;
+        const uint32_t j = k * CH_WG + threadIdx.x;
+        bool ext = false;
+        if (j < cnt) {
+            const uint32_t v = s_val[j], g2 = s_ext[j];
+            uint32_t out;
+            if (g2 != 0xffffffffu) {
+                out = CH_EXT | g2;
+                ext = true;
+            } else if (v & CH_LOCAL) {
+                atomicOr(err, DERR_CHAIN);
+                out = 0;
+            } else {
+                out = v;
+            }
+            dst[j] = out;
+        }
+        const uint64_t m = __ballot(ext);
+        pre[k] = __popcll(m & lt);
+        ext_bits |= (ext ? 1u : 0u) << k;
+        // s_off (the explicit-rank offsets) was last read before the rounds
+        if (lane == 0) s_off[k * W + wave] = __popcll(m);
+    }
+    __syncthreads();
+    __shared__ uint32_t s_base;
+    if (threadIdx.x < 64) {
+        const uint32_t v = threadIdx.x < CH_ITEMS * W ? s_off[threadIdx.x] : 0u;
+        const uint32_t inc = wave_incl_scan(v);
+        if (threadIdx.x < CH_ITEMS * W) s_off[threadIdx.x] = inc - v;
+        const uint32_t total = __shfl(inc, 63, 64);
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(open_count, total) : 0u;
+    }
+    __syncthreads();
+    if (ext_bits) {
+        const uint32_t base = s_base;
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k)
+            if (ext_bits & (1u << k))
+                open_list[base + s_off[k * W + wave] + pre[k]] =
+                    (uint32_t)(Fr.pt_off + p0 + k * CH_WG + threadIdx.x);
     }
 }
 
-// one round of global pointer jumping over the tile exits
-__global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__ res, uint64_t T) {
-    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < T; g += (uint64_t)gridDim.x * CH_WG) {
-        const uint32_t v = res[g];
-        if (v & CH_EXT) res[g] = __hip_atomic_load(&res[v & ~CH_EXT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// one round of global pointer jumping over the open points: every point of
+// list_in takes its target's value; those still pointing out of their tile
+// go to list_out (one reservation per 4096 list entries). Rounds after
+// convergence find an empty list.
+__global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__ res,
+                                                          const uint32_t* __restrict__ list_in,
+                                                          const uint32_t* __restrict__ count_in,
+                                                          uint32_t* __restrict__ list_out,
+                                                          uint32_t* __restrict__ count_out) {
+    constexpr uint32_t W = CH_WG / 64;
+    __shared__ uint32_t s_off[CH_ITEMS * W];
+    __shared__ uint32_t s_base;
+    const uint32_t n = *count_in;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    for (uint32_t t0 = blockIdx.x * CH_TILE; t0 < n; t0 += gridDim.x * CH_TILE) {
+        uint32_t idx[CH_ITEMS], pre[CH_ITEMS];
+        uint32_t open_bits = 0;
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const uint32_t kk = t0 + k * CH_WG + threadIdx.x;
+            bool open = false;
+            idx[k] = 0;
+            if (kk < n) {
+                const uint32_t i = list_in[kk];
+                idx[k] = i;
+                const uint32_t v = res[i];
+                const uint32_t nv =
+                    __hip_atomic_load(&res[v & ~CH_EXT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                res[i] = nv;
+                open = (nv & CH_EXT) != 0;
+            }
+            const uint64_t m = __ballot(open);
+            pre[k] = __popcll(m & lt);
+            open_bits |= (open ? 1u : 0u) << k;
+            if (lane == 0) s_off[k * W + wave] = __popcll(m);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const uint32_t v = threadIdx.x < CH_ITEMS * W ? s_off[threadIdx.x] : 0u;
+            const uint32_t inc = wave_incl_scan(v);
+            if (threadIdx.x < CH_ITEMS * W) s_off[threadIdx.x] = inc - v;
+            const uint32_t total = __shfl(inc, 63, 64);
+            if (threadIdx.x == 0) s_base = total ? atomicAdd(count_out, total) : 0u;
+        }
+        __syncthreads();
+        if (open_bits) {
+            const uint32_t base = s_base;
+#pragma unroll
+            for (int k = 0; k < CH_ITEMS; ++k)
+                if (open_bits & (1u << k)) list_out[base + s_off[k * W + wave] + pre[k]] = idx[k];
+        }
+        __syncthreads();   // s_off / s_base are reused by the next tile
     }
 }
 
@@ -665,12 +788,12 @@ __global__ void k_dec_check_counts(const DecFrame* __restrict__ fr, uint32_t F, 
 }
 
 // q = block * p + offset; x = fma(q + 0.5, 2 eb, min) (serial.cpp:1294-1311)
-template <int D, typename OutT, bool SOA>
+template <int D, typename OutT, bool SOA, typename RelT>
 __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restrict__ fr, uint32_t F,
                                                        const uint32_t* __restrict__ frame_pt_begin, uint64_t T,
                                                        const uint32_t* __restrict__ root,
                                                        const uint64_t* __restrict__ compact,
-                                                       const uint32_t* __restrict__ rel, OutT* __restrict__ out,
+                                                       const RelT* __restrict__ rel, OutT* __restrict__ out,
                                                        uint32_t* err) {
     for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < T; g += (uint64_t)gridDim.x * CH_WG) {
         const uint32_t f = upper_find(frame_pt_begin, F, (uint32_t)g);

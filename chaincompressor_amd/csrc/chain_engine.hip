@@ -976,7 +976,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         const uint64_t rec_tiles_max = cdiv(T, CH_TILE);
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
-            hipLaunchKernelGGL((k_l2_stats<DD>), dim3(grid_for(T, CH_WG * CH_ITEMS)), dim3(CH_WG), 0, st, B.blocks,
+            hipLaunchKernelGGL((k_l2_stats<DD>), dim3(std::min<unsigned>(grid_for(T, CH_WG * CH_ITEMS), 1024)),
+                               dim3(CH_WG), 0, st, B.blocks,
                                &B.totals[0], F, B.rec_base, B.fo);
             hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
             tm.end();

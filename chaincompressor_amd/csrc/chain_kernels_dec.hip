@@ -407,6 +407,39 @@ __device__ inline void agg_combine(Agg& left, const Agg& right, uint64_t R) {   
     }
 }
 
+// Striped layout: item k of thread t is record k*CH_WG + t of the tile, so
+// every row of 256 records is loaded with coalesced accesses. Pass 1 reduces
+// each (row, wave) segment of 64 records with an ordered shuffle tree; wave 0
+// combines the 64 segment aggregates into the tile aggregate and does the
+// cross-tile look-back; pass 2 recomputes the elements, scans each segment
+// with shuffles and writes prefix (+) element.
+template <int D>
+__device__ inline void agg_shfl_up(uint32_t& r, uint64_t* v, uint32_t off) {
+    r = __shfl_up(r, off, 64);
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[d] = __shfl_up(v[d], off, 64);
+}
+
+template <int D>
+__device__ inline void agg_shfl_down(uint32_t& r, uint64_t* v, uint32_t off) {
+    r = __shfl_down(r, off, 64);
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[d] = __shfl_down(v[d], off, 64);
+}
+
+// left = left (+) right
+template <int D>
+__device__ inline void agg_comb(uint32_t& lr, uint64_t* lv, uint32_t rr, const uint64_t* rv, uint64_t R) {
+    if (rr) {
+        lr = 1;
+#pragma unroll
+        for (int d = 0; d < D; ++d) lv[d] = rv[d];
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) lv[d] = addmod(lv[d], rv[d], R);
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ fr, uint32_t F,
                                                   const uint32_t* __restrict__ rtile_begin,
@@ -415,33 +448,51 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
                                                   const uint32_t* __restrict__ tile_zero_off,
                                                   uint32_t* flags, uint64_t* aggv, uint32_t* ticket,
                                                   uint64_t* __restrict__ compact, uint32_t* err) {
-    __shared__ uint32_t s_tile, s_tmp[4];
-    __shared__ uint64_t s_v[CH_WG][D];
-    __shared__ uint32_t s_r[CH_WG];
-    __shared__ Agg s_in;
+    constexpr uint32_t W = CH_WG / 64;
+    constexpr uint32_t NS = CH_ITEMS * W;   // (row, wave) segments per tile = 64
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_zoff[NS];         // keep records before each segment
+    __shared__ uint32_t s_sr[NS];           // segment aggregates, then exclusive prefixes
+    __shared__ uint64_t s_sv[NS][D];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t f = upper_find(rtile_begin, F, tile);
     const DecFrame& Fr = fr[f];
     const uint64_t R = Fr.range;
-    const uint64_t r0 = (uint64_t)(tile - rtile_begin[f]) * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
-    // zeros before each record: tile prefix (global over all frames) + local
-    uint32_t z = 0;
-    for (int k = 0; k < CH_ITEMS; ++k) z += (r0 + k < Fr.nrec) ? (choice[Fr.rec_base + r0 + k] == 0) : 0u;
-    uint32_t ztot;
-    const uint32_t zpre = block_excl_scan(z, s_tmp, &ztot);
-    uint64_t kidx = (uint64_t)tile_zero_off[tile] + zpre;   // global keep index
-    // thread-local sequential aggregate
-    Agg a;
-    a.reset = 0;
-    for (int d = 0; d < D; ++d) a.v[d] = 0;
-    // element of record r: (reset, value) per dimension; computed twice (for
-    // the aggregate and for the output) instead of kept in a private array
-    const uint64_t kidx0 = kidx;
-    auto element = [&](uint64_t r, uint64_t kix, bool check, uint64_t* v) -> bool {
+    const uint64_t t0 = (uint64_t)(tile - rtile_begin[f]) * CH_TILE;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    // keep records (choice 0) before each record of the tile
+    uint32_t zpre[CH_ITEMS];
+    uint32_t zbits = 0;
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+        const bool z = r < Fr.nrec && choice[Fr.rec_base + r] == 0;
+        const uint64_t m = __ballot(z);
+        zpre[k] = __popcll(m & lt);
+        zbits |= (z ? 1u : 0u) << k;
+        if (lane == 0) s_zoff[k * W + wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t v = s_zoff[threadIdx.x];
+        s_zoff[threadIdx.x] = wave_incl_scan(v) - v;
+    }
+    __syncthreads();
+    const uint64_t kbase = tile_zero_off[tile];
+    // element of item k: keep -> (reset, value - wrap_min), diff -> (0, delta mod R)
+    auto element = [&](int k, bool check, uint64_t* v) -> uint32_t {
+        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+        if (r >= Fr.nrec) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) v[d] = 0;
+            return 0;   // identity
+        }
         const uint64_t grec = Fr.rec_base + r;
-        if (choice[grec] == 0) {
+        const uint64_t kix = kbase + s_zoff[k * W + wave] + zpre[k];   // global keep index (or keeps before)
+        if ((zbits >> k) & 1u) {
             const bool inb = kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep;
             if (check && !inb) atomicOr(err, DERR_L2);
 #pragma unroll
@@ -450,7 +501,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
                 if (check && ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max)) atomicOr(err, DERR_L2);
                 v[d] = kv - (uint64_t)Fr.wrap_min;
             }
-            return true;
+            return 1;
         }
         if (check && r == 0) atomicOr(err, DERR_L2);
         const uint64_t didx = grec - kix;   // global diff index
@@ -464,106 +515,141 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             const uint64_t delta = corr > 0 ? R - (uint64_t)corr : (uint64_t)(-corr);
             v[d] = delta < R ? delta : delta % R;
         }
-        return false;
+        return 0;
     };
+    // pass 1: ordered reduction of every (row, wave) segment
+#pragma unroll 1
     for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t r = r0 + k;
-        if (r >= Fr.nrec) break;
         uint64_t v[D];
-        if (element(r, kidx, true, v)) {
-            a.reset = 1;
+        uint32_t rs = element(k, true, v);
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            uint32_t orr = rs;
+            uint64_t ov[D];
 #pragma unroll
-            for (int d = 0; d < D; ++d) a.v[d] = v[d];
-            ++kidx;
-        } else {
+            for (int d = 0; d < D; ++d) ov[d] = v[d];
+            agg_shfl_down<D>(orr, ov, off);
+            if (lane + off < 64 && (lane & (2 * off - 1)) == 0) agg_comb<D>(rs, v, orr, ov, R);
+        }
+        if (lane == 0) {
+            s_sr[k * W + wave] = rs;
 #pragma unroll
-            for (int d = 0; d < D; ++d) a.v[d] = addmod(a.v[d], v[d], R);
+            for (int d = 0; d < D; ++d) s_sv[k * W + wave][d] = v[d];
         }
     }
-    // inclusive scan of thread aggregates over the workgroup (Hillis-Steele in LDS)
-    s_r[threadIdx.x] = a.reset;
-    for (int d = 0; d < D; ++d) s_v[threadIdx.x][d] = a.v[d];
     __syncthreads();
-    for (uint32_t off = 1; off < CH_WG; off <<= 1) {
-        Agg left;
-        const bool has = threadIdx.x >= off;
-        if (has) {
-            left.reset = s_r[threadIdx.x - off];
-            for (int d = 0; d < D; ++d) left.v[d] = s_v[threadIdx.x - off][d];
+    // wave 0: exclusive scan of the 64 segment aggregates, tile aggregate,
+    // look-back with multi-word aggregates (flag 1 = tile aggregate, 2 = inclusive)
+    if (threadIdx.x < 64) {
+        uint32_t rs = s_sr[lane];
+        uint64_t v[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) v[d] = s_sv[lane][d];
+        const uint32_t r_self = rs;
+        uint64_t v_self[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) v_self[d] = v[d];
+        for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan
+            uint32_t orr = rs;
+            uint64_t ov[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) ov[d] = v[d];
+            agg_shfl_up<D>(orr, ov, off);
+            if (lane >= off) {
+                agg_comb<D>(orr, ov, rs, v, R);   // earlier (+) mine
+                rs = orr;
+#pragma unroll
+                for (int d = 0; d < D; ++d) v[d] = ov[d];
+            }
         }
-        __syncthreads();
-        if (has) {
-            Agg cur;
-            cur.reset = s_r[threadIdx.x];
-            for (int d = 0; d < D; ++d) cur.v[d] = s_v[threadIdx.x][d];
-            agg_combine<D>(left, cur, R);
-            s_r[threadIdx.x] = left.reset;
-            for (int d = 0; d < D; ++d) s_v[threadIdx.x][d] = left.v[d];
-        }
-        __syncthreads();
-    }
-    // look-back with multi-word aggregates: flag 1 = tile aggregate, 2 = inclusive
-    if (threadIdx.x == 0) {
+        // tile aggregate = inclusive value of lane 63
         Agg tot;
-        tot.reset = s_r[CH_WG - 1];
-        for (int d = 0; d < D; ++d) tot.v[d] = s_v[CH_WG - 1][d];
+        tot.reset = __shfl(rs, 63, 64);
+#pragma unroll
+        for (int d = 0; d < D; ++d) tot.v[d] = __shfl(v[d], 63, 64);
         Agg incoming;
         incoming.reset = 0;
+#pragma unroll
         for (int d = 0; d < D; ++d) incoming.v[d] = 0;
-        const uint32_t seg = rtile_begin[f];
-        if (tile != seg) {
-            for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + d] = tot.v[d];
-            aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + CH_MAXD - 1] |= 0;   // keep layout
-            __hip_atomic_store(&flags[tile], 1u | (tot.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            Agg acc;
-            acc.reset = 0;
-            for (int d = 0; d < D; ++d) acc.v[d] = 0;
-            int64_t t = (int64_t)tile - 1;
-            uint32_t spins = 0;
-            while (t >= (int64_t)seg) {
-                const uint32_t fl = __hip_atomic_load(&flags[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                if ((fl & 0xff) == 0) {
-                    if (++spins > LB_SPIN_LIMIT) { atomicOr(err, DERR_WAIT); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+        if (lane == 0) {
+            const uint32_t seg = rtile_begin[f];
+            if (tile != seg) {
+                for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + d] = tot.v[d];
+                __hip_atomic_store(&flags[tile], 1u | (tot.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                Agg acc;
+                acc.reset = 0;
+                for (int d = 0; d < D; ++d) acc.v[d] = 0;
+                int64_t t = (int64_t)tile - 1;
+                uint32_t spins = 0;
+                while (t >= (int64_t)seg) {
+                    const uint32_t fl = __hip_atomic_load(&flags[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((fl & 0xff) == 0) {
+                        if (++spins > LB_SPIN_LIMIT) { atomicOr(err, DERR_WAIT); break; }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    Agg x;
+                    x.reset = fl >> 8;
+                    const int slot = (fl & 0xff) == 2 ? 1 : 0;
+                    for (int d = 0; d < D; ++d) x.v[d] = aggv[((uint64_t)t * 2 + slot) * CH_MAXD + d];
+                    agg_combine<D>(x, acc, R);   // x (+) acc
+                    acc = x;
+                    if ((fl & 0xff) == 2 || x.reset) break;
+                    --t;
                 }
-                Agg x;
-                x.reset = fl >> 8;
-                const int slot = (fl & 0xff) == 2 ? 1 : 0;
-                for (int d = 0; d < D; ++d) x.v[d] = aggv[((uint64_t)t * 2 + slot) * CH_MAXD + d];
-                agg_combine<D>(x, acc, R);   // x (+) acc
-                acc = x;
-                if ((fl & 0xff) == 2 || x.reset) break;
-                --t;
+                incoming = acc;
             }
-            incoming = acc;
+            Agg inc = incoming;
+            agg_combine<D>(inc, tot, R);
+            for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 1) * CH_MAXD + d] = inc.v[d];
+            __hip_atomic_store(&flags[tile], 2u | (inc.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        Agg inc = incoming;
-        agg_combine<D>(inc, tot, R);
-        for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 1) * CH_MAXD + d] = inc.v[d];
-        __hip_atomic_store(&flags[tile], 2u | (inc.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        s_in = incoming;
+        incoming.reset = __shfl(incoming.reset, 0, 64);
+#pragma unroll
+        for (int d = 0; d < D; ++d) incoming.v[d] = __shfl(incoming.v[d], 0, 64);
+        // exclusive prefix of segment `lane` = incoming (+) inclusive of segment lane-1
+        uint32_t pr = incoming.reset;
+        uint64_t pv[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) pv[d] = incoming.v[d];
+        uint32_t br = __shfl_up(rs, 1, 64);
+        uint64_t bv[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) bv[d] = __shfl_up(v[d], 1, 64);
+        if (lane > 0) agg_comb<D>(pr, pv, br, bv, R);
+        (void)r_self;
+        (void)v_self;
+        s_sr[lane] = pr;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s_sv[lane][d] = pv[d];
     }
     __syncthreads();
-    // exclusive prefix of this thread = incoming (+) inclusive of threads before
-    Agg run = s_in;
-    if (threadIdx.x > 0) {
-        Agg before;
-        before.reset = s_r[threadIdx.x - 1];
-        for (int d = 0; d < D; ++d) before.v[d] = s_v[threadIdx.x - 1][d];
-        agg_combine<D>(run, before, R);
-    }
-    kidx = kidx0;
+    // pass 2: segment scan + prefix, output
+#pragma unroll 1
     for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t r = r0 + k;
-        if (r >= Fr.nrec) break;
         uint64_t v[D];
-        const bool rs = element(r, kidx, false, v);
-        kidx += rs ? 1 : 0;
+        uint32_t rs = element(k, false, v);
+        for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan in the wave
+            uint32_t orr = rs;
+            uint64_t ov[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            run.v[d] = rs ? v[d] : addmod(run.v[d], v[d], R);
-            compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + run.v[d];
+            for (int d = 0; d < D; ++d) ov[d] = v[d];
+            agg_shfl_up<D>(orr, ov, off);
+            if (lane >= off) {
+                agg_comb<D>(orr, ov, rs, v, R);
+                rs = orr;
+#pragma unroll
+                for (int d = 0; d < D; ++d) v[d] = ov[d];
+            }
+        }
+        uint32_t pr = s_sr[k * W + wave];
+        uint64_t pv[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) pv[d] = s_sv[k * W + wave][d];
+        agg_comb<D>(pr, pv, rs, v, R);
+        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+        if (r < Fr.nrec) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + pv[d];
         }
     }
 }

@@ -396,12 +396,11 @@ __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_
 // group being coded. For alphabets <= 256 the table lives in LDS together
 // with an exact reciprocal of every frequency (rcp31_* in chain_rans.hpp),
 // so the state update needs no integer division.
-#define ENC_GROUP 8
+#define ENC_GROUP 4
 
 struct EncLds {
     uint64_t pc[256];   // (cum << 32) | freq
-    uint64_t m[256];    // reciprocal multiplier
-    uint8_t l[256];     // reciprocal shift - 31
+    uint64_t ml[256];   // reciprocal multiplier (< 2^34) | shift - 31 in bits 56..63
 };
 
 template <typename ValT, int DIMS>
@@ -443,14 +442,8 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             const uint64_t e = gpc[b];
             const uint32_t f = (uint32_t)e;
             lds.pc[b] = e;
-            if (f) {
-                const uint32_t l = rcp31_shift(f);
-                lds.l[b] = (uint8_t)l;
-                lds.m[b] = rcp31_mult(f, l);
-            } else {
-                lds.l[b] = 0;
-                lds.m[b] = 0;
-            }
+            const uint32_t l = f ? rcp31_shift(f) : 0u;
+            lds.ml[b] = f ? (rcp31_mult(f, l) | ((uint64_t)l << 56)) : 0ull;
         }
     __syncthreads();
     const uint32_t M = 1u << pb, lower = 4u * M;
@@ -472,14 +465,12 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
         const uint64_t tn = t - g;
         const uint32_t gn = tn < ENC_GROUP ? (uint32_t)tn : ENC_GROUP;
         if (tn > 0) enc_group_load<ValT, DIMS>(src, tags, first, Sc, L, lane, tn, gn, nxt);
-        uint64_t ent[ENC_GROUP], mm[ENC_GROUP];
-        uint32_t ll[ENC_GROUP];
+        uint64_t ent[ENC_GROUP], ml[ENC_GROUP];
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
             const bool a = cur[j] != 0xffffffffu;
             ent[j] = a ? (small ? lds.pc[cur[j]] : gpc[cur[j]]) : 1ull;
-            mm[j] = a && small ? lds.m[cur[j]] : 0ull;
-            ll[j] = a && small ? lds.l[cur[j]] : 0u;
+            ml[j] = a && small ? lds.ml[cur[j]] : 0ull;
         }
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
@@ -498,7 +489,8 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             if (kb > 1) stream[pos + c0 + __popcll(m1 & lt)] = (uint8_t)(x >> (8 * (kb - 2)));
             if (kb > 2) stream[pos + c0 + c1 + __popcll(m2 & lt)] = (uint8_t)(x >> (8 * (kb - 3)));
             if (active) {
-                const uint32_t q = small ? rcp31_div(xx, mm[j], ll[j]) : xx / f;
+                const uint32_t q =
+                    small ? rcp31_div(xx, ml[j] & ((1ull << 56) - 1), (uint32_t)(ml[j] >> 56)) : xx / f;
                 x = q * M + (xx - q * f) + cu;
             }
         }

@@ -649,67 +649,93 @@ __device__ inline uint32_t frame_of_rec(const uint64_t* s_base, uint32_t nframes
     return lo;
 }
 
+// Grid-stride over records, 256 consecutive records per block step (coalesced
+// D x u64 loads); per-thread min / max / max |difference| per frame, flushed
+// when the frame changes and reduced per block at the end, so a frame costs
+// a few atomics per block, not per record.
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_l2_stats(const uint64_t* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
                                                     const uint64_t* __restrict__ frame_rec_base,
                                                     FrameOut* __restrict__ fo) {
+    constexpr uint32_t W = CH_WG / 64;
     __shared__ uint64_t s_base[CH_MAXF_LDS];
+    __shared__ uint32_t s_f[W];
+    __shared__ uint64_t s_mn[W], s_mx[W], s_md[W];
     const uint64_t R = *rec_total;
     const bool lds = nframes <= CH_MAXF_LDS;
-    if (lds) {
+    if (lds)
         for (uint32_t k = threadIdx.x; k < nframes; k += CH_WG) s_base[k] = frame_rec_base[k];
-        __syncthreads();
-    }
+    __syncthreads();
     const uint64_t* bases = lds ? s_base : frame_rec_base;
-    const uint64_t gw = (blockIdx.x * (uint64_t)CH_WG + threadIdx.x) >> 6;
-    const uint64_t nw = ((uint64_t)gridDim.x * CH_WG) >> 6;
-    for (uint64_t wb = gw * 64 * CH_ITEMS; wb < R; wb += nw * 64 * CH_ITEMS) {   // wave-uniform trip count
-        const uint64_t r0 = wb + (uint64_t)lane_id() * CH_ITEMS;
-        uint32_t f = r0 < R ? frame_of_rec(bases, nframes, r0) : 0xffffffffu;
-        uint64_t mn = ~0ull, mx = 0, md = 0;
-        uint32_t cur = f;
-        for (int k = 0; k < CH_ITEMS && r0 + k < R; ++k) {
-            const uint64_t r = r0 + k;
-            while (cur + 1 < nframes && bases[cur + 1] <= r) {
-                if (mn != ~0ull) {
-                    atomicMin((unsigned long long*)&fo[cur].blk_min, mn);
-                    atomicMax((unsigned long long*)&fo[cur].blk_max, mx);
-                    atomicMax((unsigned long long*)&fo[cur].blk_maxdiff, md);
-                }
-                mn = ~0ull; mx = 0; md = 0;
-                ++cur;
-            }
-            const bool has_prev = r > bases[cur];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const uint64_t v = blocks[r * D + d];
-                mn = v < mn ? v : mn;
-                mx = v > mx ? v : mx;
-                if (has_prev) {
-                    const uint64_t u = blocks[(r - 1) * D + d];
-                    const uint64_t ad = u > v ? u - v : v - u;
-                    md = ad > md ? ad : md;
-                }
-            }
-        }
-        // wave-uniform fast path
-        const uint32_t c0 = __shfl(cur, 0, 64);
-        (void)f;
-        if (__ballot(!(cur == c0 || mn == ~0ull)) == 0) {
-            for (int k = 32; k >= 1; k >>= 1) {
-                uint64_t a = __shfl_xor(mn, k, 64), b = __shfl_xor(mx, k, 64), c = __shfl_xor(md, k, 64);
-                mn = a < mn ? a : mn; mx = b > mx ? b : mx; md = c > md ? c : md;
-            }
-            if (lane_id() == 0 && mn != ~0ull && c0 != 0xffffffffu) {
-                atomicMin((unsigned long long*)&fo[c0].blk_min, mn);
-                atomicMax((unsigned long long*)&fo[c0].blk_max, mx);
-                atomicMax((unsigned long long*)&fo[c0].blk_maxdiff, md);
-            }
-        } else if (mn != ~0ull) {
+    uint32_t cur = 0xffffffffu;
+    uint64_t mn = ~0ull, mx = 0, md = 0;
+    auto flush = [&]() {
+        if (mn != ~0ull) {
             atomicMin((unsigned long long*)&fo[cur].blk_min, mn);
             atomicMax((unsigned long long*)&fo[cur].blk_max, mx);
             atomicMax((unsigned long long*)&fo[cur].blk_maxdiff, md);
+        }
+        mn = ~0ull;
+        mx = 0;
+        md = 0;
+    };
+    for (uint64_t r = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; r < R; r += (uint64_t)gridDim.x * CH_WG) {
+        const uint32_t f = frame_of_rec(bases, nframes, r);
+        if (f != cur) {
+            flush();
+            cur = f;
+        }
+        const bool has_prev = r > bases[f];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const uint64_t v = blocks[r * D + d];
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+            if (has_prev) {
+                const uint64_t u = blocks[(r - 1) * D + d];
+                const uint64_t ad = u > v ? u - v : v - u;
+                md = ad > md ? ad : md;
+            }
+        }
+    }
+    // waves whose lanes all hold one frame reduce by shuffles; others flush per lane
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t has = __ballot(mn != ~0ull);
+    uint32_t c0 = 0xffffffffu;
+    if (has) c0 = __shfl(cur, __ffsll((unsigned long long)has) - 1, 64);
+    const bool uniform = has && __ballot(mn != ~0ull && cur != c0) == 0;
+    if (uniform) {
+        for (int k = 32; k >= 1; k >>= 1) {
+            const uint64_t a = __shfl_xor(mn, k, 64), b = __shfl_xor(mx, k, 64), c = __shfl_xor(md, k, 64);
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+            md = c > md ? c : md;
+        }
+    } else {
+        flush();
+    }
+    if (lane == 0) {
+        s_f[wave] = uniform ? c0 : 0xffffffffu;
+        s_mn[wave] = mn;
+        s_mx[wave] = mx;
+        s_md[wave] = md;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 0; w < W; ++w) {
+            if (s_f[w] == 0xffffffffu) continue;
+            uint64_t a = s_mn[w], b = s_mx[w], c = s_md[w];
+            for (uint32_t w2 = w + 1; w2 < W; ++w2)
+                if (s_f[w2] == s_f[w]) {
+                    a = s_mn[w2] < a ? s_mn[w2] : a;
+                    b = s_mx[w2] > b ? s_mx[w2] : b;
+                    c = s_md[w2] > c ? s_md[w2] : c;
+                    s_f[w2] = 0xffffffffu;
+                }
+            atomicMin((unsigned long long*)&fo[s_f[w]].blk_min, a);
+            atomicMax((unsigned long long*)&fo[s_f[w]].blk_max, b);
+            atomicMax((unsigned long long*)&fo[s_f[w]].blk_maxdiff, c);
         }
     }
 }
@@ -736,6 +762,9 @@ __global__ void k_l2_wrap(FrameOut* fo, uint32_t nframes, uint32_t* err) {
 // K7: L2 choice per record and ordered split into diff / keep streams
 // (cpp/chain/lcps_cpp.cpp:1360-1454; keep-all when L2 is off, :1456-1478).
 
+// Striped tiles of 4096 records (item k of thread t = record k*256 + t):
+// choices by ballot, ordered positions from (row, wave) counts and a
+// decoupled look-back over tiles; coalesced loads and stores.
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
@@ -744,8 +773,9 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
                                                     uint64_t* status, uint32_t* ticket, uint32_t* err,
                                                     uint8_t* __restrict__ choice2, uint64_t* __restrict__ diff,
                                                     uint64_t* __restrict__ keep, uint64_t* __restrict__ diff_total) {
+    constexpr uint32_t W = CH_WG / 64;
     __shared__ uint64_t s_base[CH_MAXF_LDS];
-    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint32_t s_tile, s_off[CH_ITEMS * W];
     __shared__ uint64_t s_excl;
     const uint64_t R = *rec_total;
     const bool lds = nframes <= CH_MAXF_LDS;
@@ -758,45 +788,61 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
     // the grid is sized for the largest possible record count; tiles past the
     // end hold no records and no later tile waits on them
     if ((uint64_t)tile * CH_TILE >= R) return;
-    const uint64_t r0 = (uint64_t)tile * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
-    uint32_t chosen = 0, cnt = 0;
-    uint32_t f = r0 < R ? frame_of_rec(bases, nframes, r0) : 0;
-    uint32_t fstart = f;
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t r = r0 + k;
-        if (r >= R) break;
-        while (f + 1 < nframes && bases[f + 1] <= r) ++f;
-        if (r == bases[f] || !(fr[f].flags & FL_L2)) continue;
-        const FrameOut& o = fo[f];
-        uint64_t mv = 0, mdf = 0;
+    const uint64_t t0 = (uint64_t)tile * CH_TILE;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    uint32_t chosen = 0, pre[CH_ITEMS], fk[CH_ITEMS];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const uint64_t v = blocks[r * D + d], u = blocks[(r - 1) * D + d];
-            mv = v > mv ? v : mv;
-            const uint64_t ad = u > v ? u - v : v - u;
-            mdf = ad > mdf ? ad : mdf;
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+        bool c = false;
+        fk[k] = 0;
+        if (r < R) {
+            const uint32_t f = frame_of_rec(bases, nframes, r);
+            fk[k] = f;
+            if (r != bases[f] && (fr[f].flags & FL_L2)) {
+                const FrameOut& o = fo[f];
+                uint64_t mv = 0, mdf = 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const uint64_t v = blocks[r * D + d], u = blocks[(r - 1) * D + d];
+                    mv = v > mv ? v : mv;
+                    const uint64_t ad = u > v ? u - v : v - u;
+                    mdf = ad > mdf ? ad : mdf;
+                }
+                const uint64_t tvw = bitlen64(mv), tdw = bitlen64(mdf);
+                const uint64_t tv = (uint64_t)D * tvw + bitlen64(tvw);
+                const uint64_t td = (uint64_t)D * tdw + bitlen64(tdw);
+                const uint64_t value_bits = tv < o.fixed_value_bits ? tv : o.fixed_value_bits;
+                const uint64_t diff_bits = td > o.fixed_diff_bits ? td : o.fixed_diff_bits;
+                c = value_bits >= diff_bits;
+            }
         }
-        const uint64_t tvw = bitlen64(mv), tdw = bitlen64(mdf);
-        const uint64_t tv = (uint64_t)D * tvw + bitlen64(tvw);
-        const uint64_t td = (uint64_t)D * tdw + bitlen64(tdw);
-        const uint64_t value_bits = tv < o.fixed_value_bits ? tv : o.fixed_value_bits;
-        const uint64_t diff_bits = td > o.fixed_diff_bits ? td : o.fixed_diff_bits;
-        if (value_bits >= diff_bits) { chosen |= 1u << k; ++cnt; }
+        const uint64_t m = __ballot(c);
+        pre[k] = __popcll(m & lt);
+        chosen |= (c ? 1u : 0u) << k;
+        if (lane == 0) s_off[k * W + wave] = __popcll(m);
     }
-    uint32_t tot;
-    const uint32_t pre = block_excl_scan(cnt, s_tmp, &tot);
+    __syncthreads();
     if (threadIdx.x < 64) {
+        const uint32_t v = s_off[threadIdx.x];
+        const uint32_t inc = wave_incl_scan(v);
+        s_off[threadIdx.x] = inc - v;
+        const uint32_t tot = __shfl(inc, 63, 64);
         const uint64_t e = lookback(status, tile, tot, err);
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
-    uint64_t dpos = s_excl + pre;   // diffs before record r0
-    f = fstart;
+#pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t r = r0 + k;
-        if (r >= R) break;
-        while (f + 1 < nframes && bases[f + 1] <= r) ++f;
-        if (r == bases[f]) { fo[f].diff_base = dpos; fo[f].keep_base = r - dpos; }
+        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+        if (r >= R) continue;
+        const uint32_t f = fk[k];
+        const uint64_t dpos = s_excl + s_off[k * W + wave] + pre[k];   // diffs before record r
+        if (r == bases[f]) {
+            fo[f].diff_base = dpos;
+            fo[f].keep_base = r - dpos;
+        }
         const bool c = (chosen >> k) & 1u;
         choice2[r] = (uint8_t)c;
         if (!c) {
@@ -813,10 +859,9 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
                 else if (corr > o.wrap_max_corr) corr -= o.wrap_range;
                 diff[dpos * D + d] = corr >= 0 ? ((uint64_t)corr << 1) : (((uint64_t)(-(corr + 1)) << 1) | 1ull);
             }
-            ++dpos;
         }
+        if (r == R - 1) *diff_total = dpos + (c ? 1u : 0u);
     }
-    if (r0 < R && r0 + CH_ITEMS >= R) *diff_total = dpos;
 }
 
 __global__ void k_copy_rec_base(const FrameOut* fo, uint32_t nframes, uint64_t* rec_base) {

@@ -446,8 +446,8 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
         using V = typename decltype(vt)::type;
         constexpr int DC = decltype(dd)::value;
         et.stats[c] = tm.begin("comp_stats");
-        hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
-                           x.vhist, x.thist);
+        hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(std::min<uint32_t>(ntiles, 2048)), dim3(CH_WG), 0, st,
+                           (const V*)v, c, F, tb, ntiles, x.ci, x.vhist, x.thist);
         tm.end();
     });
     tm.begin("rans_table", 5ull * F * (RANS_VALUE_LIMIT + RANS_TAG_LIMIT) * 4 + 10ull * F * RANS_VALUE_LIMIT * 8);

@@ -122,9 +122,17 @@ __global__ void k_comp_init(const FrameDev* __restrict__ fr, const FrameOut* __r
 
 // per-tile statistics of one component type over all frames: per-dimension
 // range, value histogram (< 4096), width histogram and total Tagged bits.
-// Wavefront peer masks merge equal symbols before each LDS atomic.
+// Wavefront peer masks merge equal symbols before each LDS atomic; a wave
+// whose valid lanes all hold one symbol needs a single atomic.
 __device__ inline void hist_add(uint32_t* h, uint32_t sym, bool valid, int bits) {
-    uint64_t peers = __ballot(valid);
+    const uint64_t act = __ballot(valid);
+    if (!act) return;
+    const uint32_t s0 = __shfl(sym, __ffsll((unsigned long long)act) - 1, 64);
+    if (__ballot(valid && sym != s0) == 0) {
+        if (lane_id() == 0) atomicAdd(&h[s0], (uint32_t)__popcll(act));
+        return;
+    }
+    uint64_t peers = act;
     for (int b = 0; b < bits; ++b) {
         const uint64_t bb = __ballot((sym >> b) & 1u);
         peers &= ((sym >> b) & 1u) ? bb : ~bb;
@@ -132,99 +140,132 @@ __device__ inline void hist_add(uint32_t* h, uint32_t sym, bool valid, int bits)
     if (valid && __popcll(peers & lanemask_lt()) == 0) atomicAdd(&h[sym], (uint32_t)__popcll(peers));
 }
 
+// One workgroup walks several tiles (grid-stride) and keeps its histograms
+// and range statistics in LDS / registers until the frame changes, so the
+// global atomics are per workgroup and frame, not per tile.
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ vals, int c, uint32_t F,
-                                                      const uint32_t* __restrict__ tile_begin,
+                                                      const uint32_t* __restrict__ tile_begin, uint32_t ntiles,
                                                       CompInst* __restrict__ ci, uint32_t* __restrict__ vhist,
                                                       uint32_t* __restrict__ thist) {
-    const uint32_t tile = blockIdx.x;
-    const uint32_t f = upper_find(tile_begin, F, tile);
-    CompInst& C = ci[c * F + f];
-    const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
-    if (r0 >= C.n) return;
+    constexpr int VBITS = sizeof(ValT) == 1 ? 8 : 12;   // bits that can differ among values < 4096
     __shared__ uint32_t s_vh[RANS_VALUE_LIMIT], s_th[RANS_TAG_LIMIT];
     __shared__ uint64_t s_red[4][2 * CH_MAXD + 2];
     for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG) s_vh[k] = 0;
     if (threadIdx.x < RANS_TAG_LIMIT) s_th[threadIdx.x] = 0;
     __syncthreads();
     uint64_t mn[DIMS], mx[DIMS];
-#pragma unroll
-    for (int d = 0; d < DIMS; ++d) { mn[d] = ~0ull; mx[d] = 0; }
     uint64_t bits = 0;
     uint32_t maxtag = 0, ovf = 0;
-    const ValT* src = vals + C.base * DIMS;
-    for (int j = 0; j < CH_ITEMS; ++j) {
-        const uint64_t r = r0 + (uint64_t)j * CH_WG + threadIdx.x;
-        const bool valid = r < C.n;
-        uint32_t tag = 1;
+    auto reset = [&]() {
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
-            const uint64_t v = valid ? (uint64_t)src[r * DIMS + d] : 0;
-            if (valid) {
-                mn[d] = v < mn[d] ? v : mn[d];
-                mx[d] = v > mx[d] ? v : mx[d];
-                const uint32_t b = (uint32_t)bitlen64(v);
-                tag = b > tag ? b : tag;
-            }
-            const bool small = valid && v < RANS_VALUE_LIMIT;
-            ovf |= (valid && !small) ? 1u : 0u;
-            hist_add(s_vh, small ? (uint32_t)v : 0u, small, 12);
+            mn[d] = ~0ull;
+            mx[d] = 0;
         }
-        hist_add(s_th, valid ? tag : 0u, valid, 6);
-        if (valid) {
-            bits += (uint64_t)DIMS * tag;
-            maxtag = tag > maxtag ? tag : maxtag;
-        }
-    }
-    // reduce range / bits / maxtag / overflow over the workgroup
+        bits = 0;
+        maxtag = 0;
+        ovf = 0;
+    };
+    reset();
     const uint32_t w = threadIdx.x >> 6;
+    // publish the statistics of instance f (called by the whole workgroup)
+    auto flush = [&](uint32_t f) {
+        CompInst& C = ci[c * F + f];
 #pragma unroll
-    for (int d = 0; d < DIMS; ++d) {
-        for (int k = 32; k >= 1; k >>= 1) {
-            uint64_t a = __shfl_xor(mn[d], k, 64), b = __shfl_xor(mx[d], k, 64);
-            mn[d] = a < mn[d] ? a : mn[d];
-            mx[d] = b > mx[d] ? b : mx[d];
-        }
-    }
-    for (int k = 32; k >= 1; k >>= 1) {
-        bits += __shfl_xor(bits, k, 64);
-        uint32_t a = __shfl_xor(maxtag, k, 64);
-        maxtag = a > maxtag ? a : maxtag;
-        ovf |= __shfl_xor(ovf, k, 64);
-    }
-    if (lane_id() == 0) {
-        for (int d = 0; d < DIMS; ++d) { s_red[w][2 * d] = mn[d]; s_red[w][2 * d + 1] = mx[d]; }
-        s_red[w][2 * CH_MAXD] = bits;
-        s_red[w][2 * CH_MAXD + 1] = ((uint64_t)maxtag << 32) | ovf;
-    }
-    __syncthreads();
-    const uint32_t ci_idx = c * F + f;
-    for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG)
-        if (s_vh[k]) atomicAdd(&vhist[(uint64_t)ci_idx * RANS_VALUE_LIMIT + k], s_vh[k]);
-    if (threadIdx.x < RANS_TAG_LIMIT && s_th[threadIdx.x])
-        atomicAdd(&thist[(uint64_t)ci_idx * RANS_TAG_LIMIT + threadIdx.x], s_th[threadIdx.x]);
-    if (threadIdx.x == 0) {
-        uint64_t tb = 0;
-        uint32_t mt = 0, of = 0;
-        for (int k = 0; k < 4; ++k) {
-            tb += s_red[k][2 * CH_MAXD];
-            const uint32_t a = (uint32_t)(s_red[k][2 * CH_MAXD + 1] >> 32);
-            mt = a > mt ? a : mt;
-            of |= (uint32_t)s_red[k][2 * CH_MAXD + 1];
-        }
         for (int d = 0; d < DIMS; ++d) {
-            uint64_t a = ~0ull, b = 0;
-            for (int k = 0; k < 4; ++k) {
-                a = s_red[k][2 * d] < a ? s_red[k][2 * d] : a;
-                b = s_red[k][2 * d + 1] > b ? s_red[k][2 * d + 1] : b;
+            for (int k = 32; k >= 1; k >>= 1) {
+                const uint64_t a = __shfl_xor(mn[d], k, 64), b = __shfl_xor(mx[d], k, 64);
+                mn[d] = a < mn[d] ? a : mn[d];
+                mx[d] = b > mx[d] ? b : mx[d];
             }
-            atomicMin((unsigned long long*)&C.dmin[d], a);
-            atomicMax((unsigned long long*)&C.dmax[d], b);
         }
-        atomicAdd((unsigned long long*)&C.tag_bits, tb);
-        atomicMax(&C.max_tag, mt);
-        if (of) atomicOr(&C.overflow, 1u);
+        for (int k = 32; k >= 1; k >>= 1) {
+            bits += __shfl_xor(bits, k, 64);
+            const uint32_t a = __shfl_xor(maxtag, k, 64);
+            maxtag = a > maxtag ? a : maxtag;
+            ovf |= __shfl_xor(ovf, k, 64);
+        }
+        if (lane_id() == 0) {
+            for (int d = 0; d < DIMS; ++d) {
+                s_red[w][2 * d] = mn[d];
+                s_red[w][2 * d + 1] = mx[d];
+            }
+            s_red[w][2 * CH_MAXD] = bits;
+            s_red[w][2 * CH_MAXD + 1] = ((uint64_t)maxtag << 32) | ovf;
+        }
+        __syncthreads();
+        const uint32_t ci_idx = c * F + f;
+        for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG)
+            if (s_vh[k]) {
+                atomicAdd(&vhist[(uint64_t)ci_idx * RANS_VALUE_LIMIT + k], s_vh[k]);
+                s_vh[k] = 0;
+            }
+        if (threadIdx.x < RANS_TAG_LIMIT && s_th[threadIdx.x]) {
+            atomicAdd(&thist[(uint64_t)ci_idx * RANS_TAG_LIMIT + threadIdx.x], s_th[threadIdx.x]);
+            s_th[threadIdx.x] = 0;
+        }
+        if (threadIdx.x == 0) {
+            uint64_t tb = 0;
+            uint32_t mt = 0, of = 0;
+            for (int k = 0; k < 4; ++k) {
+                tb += s_red[k][2 * CH_MAXD];
+                const uint32_t a = (uint32_t)(s_red[k][2 * CH_MAXD + 1] >> 32);
+                mt = a > mt ? a : mt;
+                of |= (uint32_t)s_red[k][2 * CH_MAXD + 1];
+            }
+            for (int d = 0; d < DIMS; ++d) {
+                uint64_t a = ~0ull, b = 0;
+                for (int k = 0; k < 4; ++k) {
+                    a = s_red[k][2 * d] < a ? s_red[k][2 * d] : a;
+                    b = s_red[k][2 * d + 1] > b ? s_red[k][2 * d + 1] : b;
+                }
+                atomicMin((unsigned long long*)&C.dmin[d], a);
+                atomicMax((unsigned long long*)&C.dmax[d], b);
+            }
+            atomicAdd((unsigned long long*)&C.tag_bits, tb);
+            atomicMax(&C.max_tag, mt);
+            if (of) atomicOr(&C.overflow, 1u);
+        }
+        __syncthreads();
+        reset();
+    };
+    uint32_t cur = 0xffffffffu;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint32_t f = upper_find(tile_begin, F, tile);
+        const CompInst& C = ci[c * F + f];
+        const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+        if (r0 >= C.n) continue;
+        if (f != cur) {
+            if (cur != 0xffffffffu) flush(cur);
+            cur = f;
+        }
+        const ValT* src = vals + C.base * DIMS;
+        for (int j = 0; j < CH_ITEMS; ++j) {
+            const uint64_t r = r0 + (uint64_t)j * CH_WG + threadIdx.x;
+            const bool valid = r < C.n;
+            uint32_t tag = 1;
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) {
+                const uint64_t v = valid ? (uint64_t)src[r * DIMS + d] : 0;
+                if (valid) {
+                    mn[d] = v < mn[d] ? v : mn[d];
+                    mx[d] = v > mx[d] ? v : mx[d];
+                    const uint32_t b = (uint32_t)bitlen64(v);
+                    tag = b > tag ? b : tag;
+                }
+                const bool small = valid && v < RANS_VALUE_LIMIT;
+                ovf |= (valid && !small) ? 1u : 0u;
+                hist_add(s_vh, small ? (uint32_t)v : 0u, small, VBITS);
+            }
+            hist_add(s_th, valid ? tag : 0u, valid, 6);
+            if (valid) {
+                bits += (uint64_t)DIMS * tag;
+                maxtag = tag > maxtag ? tag : maxtag;
+            }
+        }
     }
+    if (cur != 0xffffffffu) flush(cur);
 }
 
 // ------------------------------------------------------------------------

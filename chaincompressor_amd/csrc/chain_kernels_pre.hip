@@ -582,23 +582,36 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
                                                     uint32_t* err, uint64_t* __restrict__ blocks,
                                                     FrameOut* __restrict__ fo, uint64_t* __restrict__ rec_total,
                                                     uint32_t nwords, uint64_t T) {
-    __shared__ uint32_t s_tile, s_tmp[4];
+    // striped: item k of thread t is point k*256 + t of the tile (coalesced);
+    // record positions from ballots, scanned (row, wave) counts and a
+    // decoupled look-back over tiles
+    constexpr uint32_t W = CH_WG / 64;
+    __shared__ uint32_t s_tile, s_off[CH_ITEMS * W];
     __shared__ uint64_t s_excl;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t f = upper_find(tile_begin, nframes, tile);
     const FrameDev& F = fr[f];
-    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
-    uint32_t flags = 0, cnt = 0;
+    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    uint32_t flags = 0, pre[CH_ITEMS];
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t i = start + k;
-        if (i < F.n && idrel[F.pt_off + i] == 0) { flags |= 1u << k; ++cnt; }
+        const uint64_t i = start + k * CH_WG + threadIdx.x;
+        const bool ex = i < F.n && idrel[F.pt_off + i] == 0;
+        const uint64_t m = __ballot(ex);
+        pre[k] = __popcll(m & lt);
+        flags |= (ex ? 1u : 0u) << k;
+        if (lane == 0) s_off[k * W + wave] = __popcll(m);
     }
-    uint32_t tot;
-    const uint32_t pre = block_excl_scan(cnt, s_tmp, &tot);
+    __syncthreads();
     if (threadIdx.x < 64) {
+        const uint32_t v = s_off[threadIdx.x];
+        const uint32_t inc = wave_incl_scan(v);
+        s_off[threadIdx.x] = inc - v;
+        const uint32_t tot = __shfl(inc, 63, 64);
         const uint64_t e = lookback(status, tile, tot, err);
         if (threadIdx.x == 0) {
             s_excl = e;
@@ -607,18 +620,20 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
         }
     }
     __syncthreads();
-    uint64_t rec = s_excl + pre;
+    if (flags) {
+        const uint64_t base = s_excl;
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        if (flags & (1u << k)) {
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            if (!(flags & (1u << k))) continue;
+            const uint64_t i = start + k * CH_WG + threadIdx.x;
+            const uint64_t rec = base + s_off[k * W + wave] + pre[k];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const uint32_t wd = F.key_width[d];
                 const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
-                const uint64_t key = (uint64_t)keys[(uint64_t)(nwords == 1 ? 0 : F.key_word[d]) * T + F.pt_off + start + k];
+                const uint64_t key = (uint64_t)keys[(uint64_t)(nwords == 1 ? 0 : F.key_word[d]) * T + F.pt_off + i];
                 blocks[rec * D + d] = (key >> F.key_shift[d]) & mask;
             }
-            ++rec;
         }
     }
 }

@@ -1679,6 +1679,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         uint32_t maxt = 1;
         for (uint32_t f = 0; f < F; ++f) maxt = std::max(maxt, ptb[f + 1] - ptb[f]);
         while ((1u << (rounds - 1)) < maxt) ++rounds;   // ceil(log2(tiles)) + 1
+        ++rounds;                                        // hops through followers
     }
     fill32(st, X.jump_open, rounds + 1, 0);
     with_dims(D, [&](auto dd) {
@@ -1695,6 +1696,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
                            st, X.root,
                            lin, &X.jump_open[r], lout, &X.jump_open[r + 1]);
     }
+    hipLaunchKernelGGL(k_dec_chain_follow, dim3(grid_for(T)), dim3(CH_WG), 0, st, X.root, T);
     tm.end();
     tm.begin("dequantize", 4 * T + comp_es[4] * T * D + 8ull * D * P.R + T * D * es);
     with_dims(D, [&](auto dd) {

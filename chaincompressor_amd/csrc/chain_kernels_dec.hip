@@ -232,31 +232,53 @@ struct DecFixed {
 };
 
 // one thread per record: every dimension's bits, written together
+// One thread per record (all its dimensions). Job starts live in LDS. A
+// value of width <= 33 bits is cut from two aligned 32-bit words (MSB-first:
+// byte-swapped), unless those words would reach past the end of its payload;
+// other values are assembled byte by byte.
+#define DEC_FIX_LDS 1024u
 __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__ in, const DecFixed* __restrict__ jobs,
                                                      uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
                                                      uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
+    __shared__ uint64_t s_first[DEC_FIX_LDS];
+    const bool lds = njobs <= DEC_FIX_LDS;
+    if (lds)
+        for (uint32_t k = threadIdx.x; k < njobs; k += CH_WG) s_first[k] = jobs[k].first_thread;
+    __syncthreads();
     for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < total; g += (uint64_t)gridDim.x * CH_WG) {
         uint32_t lo = 0, hi = njobs;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (jobs[mid].first_thread <= g) lo = mid; else hi = mid;
+            const uint64_t ft = lds ? s_first[mid] : jobs[mid].first_thread;
+            if (ft <= g) lo = mid; else hi = mid;
         }
         const DecFixed& J = jobs[lo];
         const uint64_t r = g - J.first_thread;
         for (uint32_t d = 0; d < J.dims; ++d) {
             const uint32_t w = J.width[d];
             const uint64_t bit = r * w;
-            const uint8_t* p = in + J.byte_off[d] + (bit >> 3);
+            const uint64_t a = J.byte_off[d] + (bit >> 3);
             const uint32_t sh = (uint32_t)(bit & 7);
-            const uint32_t nb = (sh + w + 7) >> 3;
-            uint64_t acc_hi = 0, acc = 0;
-            for (uint32_t q = 0; q < nb; ++q) {
-                acc_hi = (acc_hi << 8) | (acc >> 56);
-                acc = (acc << 8) | p[q];
+            const uint64_t a4 = a & ~3ull;
+            const uint64_t end = J.byte_off[d] + (J.n * w + 7) / 8;
+            uint64_t v;
+            if (w <= 33 && a4 + 8 <= end) {
+                const uint32_t* q = reinterpret_cast<const uint32_t*>(in + a4);
+                const uint64_t x = ((uint64_t)__builtin_bswap32(q[0]) << 32) | __builtin_bswap32(q[1]);
+                const uint32_t s = (uint32_t)(a - a4) * 8 + sh;   // <= 31
+                v = (x << s) >> (64 - w);
+            } else {
+                const uint8_t* p = in + a;
+                const uint32_t nb = (sh + w + 7) >> 3;
+                uint64_t acc_hi = 0, acc = 0;
+                for (uint32_t qq = 0; qq < nb; ++qq) {
+                    acc_hi = (acc_hi << 8) | (acc >> 56);
+                    acc = (acc << 8) | p[qq];
+                }
+                const uint32_t drop = 8 * nb - sh - w;
+                v = drop ? ((acc >> drop) | (acc_hi << (64 - drop))) : acc;
+                v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
             }
-            const uint32_t drop = 8 * nb - sh - w;
-            uint64_t v = drop ? ((acc >> drop) | (acc_hi << (64 - drop))) : acc;
-            v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
             if (v > (uint64_t)INT64_MAX - J.minimum[d]) atomicOr(err, DERR_VALUE);
             store_value(J.comp, J.dst + r * J.dims + d, J.minimum[d] + v, idrel, diff, keep, choice, rel, rel8, err);
         }
@@ -656,11 +678,14 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 
 // ------------------------------------------------------------------------
 // Chain resolution: the explicit record of every point. A tile first
-// resolves the links that stay inside it (pointer jumping in LDS); a point
-// whose chain leaves the tile keeps CH_EXT | first point outside. Rounds of
-// global pointer jumping then replace every CH_EXT entry by the entry it
-// names. Each such hop goes to a later tile, so ceil(log2(tiles)) + 1 rounds
-// resolve every chain; no tile ever waits for another.
+// resolves the links that stay inside it (pointer jumping in LDS). The last
+// in-tile point of a chain segment that continues in a later tile (its exit)
+// keeps CH_EXT | its successor and joins the open list; the other points of
+// the segment keep CH_EXT | their exit. Rounds of global pointer jumping over
+// the open list resolve the exits (every hop goes to a later tile, through
+// at most one follower per tile, so ceil(log2(tiles)) + 2 rounds suffice);
+// a last pass lets every follower take its exit's value. No tile ever waits
+// for another.
 
 #define CH_LOCAL 0x80000000u
 #define CH_EXT 0x80000000u
@@ -770,8 +795,11 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
             const uint32_t v = s_val[j], g2 = s_ext[j];
             uint32_t out;
             if (g2 != 0xffffffffu) {
-                out = CH_EXT | g2;
-                ext = true;
+                // the exit point of a chain segment (still pointing at itself)
+                // names its successor in a later tile and joins the open list;
+                // the points that reach it name it and follow it afterwards
+                ext = v == (CH_LOCAL | j);
+                out = CH_EXT | (ext ? g2 : (uint32_t)(Fr.pt_off + p0 + (v & ~CH_LOCAL)));
             } else if (v & CH_LOCAL) {
                 atomicOr(err, DERR_CHAIN);
                 out = 0;
@@ -859,6 +887,14 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__
                 if (open_bits & (1u << k)) list_out[base + s_off[k * W + wave] + pre[k]] = idx[k];
         }
         __syncthreads();   // s_off / s_base are reused by the next tile
+    }
+}
+
+// followers of tile exits take their exit's (resolved) value
+__global__ __launch_bounds__(CH_WG) void k_dec_chain_follow(uint32_t* __restrict__ res, uint64_t T) {
+    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < T; g += (uint64_t)gridDim.x * CH_WG) {
+        const uint32_t v = res[g];
+        if (v & CH_EXT) res[g] = res[v & ~CH_EXT];
     }
 }
 

@@ -140,7 +140,7 @@ __device__ inline void hist_add(uint32_t* h, uint32_t sym, bool valid, int bits)
     if (valid && __popcll(peers & lanemask_lt()) == 0) atomicAdd(&h[sym], (uint32_t)__popcll(peers));
 }
 
-// One workgroup walks several tiles (grid-stride) and keeps its histograms
+// One workgroup walks a range of tiles and keeps its histograms
 // and range statistics in LDS / registers until the frame changes, so the
 // global atomics are per workgroup and frame, not per tile.
 template <typename ValT, int DIMS>
@@ -231,7 +231,11 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         reset();
     };
     uint32_t cur = 0xffffffffu;
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // contiguous tile ranges per workgroup: the instance changes only at
+    // frame boundaries
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t tb0 = blockIdx.x * per, tb1 = tb0 + per < ntiles ? tb0 + per : ntiles;
+    for (uint32_t tile = tb0; tile < tb1; ++tile) {
         const uint32_t f = upper_find(tile_begin, F, tile);
         const CompInst& C = ci[c * F + f];
         const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;

@@ -664,10 +664,10 @@ __device__ inline uint32_t frame_of_rec(const uint64_t* s_base, uint32_t nframes
     return lo;
 }
 
-// Grid-stride over records, 256 consecutive records per block step (coalesced
-// D x u64 loads); per-thread min / max / max |difference| per frame, flushed
-// when the frame changes and reduced per block at the end, so a frame costs
-// a few atomics per block, not per record.
+// Contiguous record ranges per workgroup, 256 consecutive records per step
+// (coalesced D x u64 loads); per-thread min / max / max |difference| per
+// frame, flushed when the frame changes and reduced per block at the end, so
+// a frame costs a few atomics per block, not per record.
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_l2_stats(const uint64_t* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
@@ -695,7 +695,11 @@ __global__ __launch_bounds__(CH_WG) void k_l2_stats(const uint64_t* __restrict__
         mx = 0;
         md = 0;
     };
-    for (uint64_t r = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; r < R; r += (uint64_t)gridDim.x * CH_WG) {
+    // every workgroup walks one contiguous range of records, 256 at a time,
+    // so a thread's frame changes only at frame boundaries
+    const uint64_t per = ((R + gridDim.x - 1) / gridDim.x + CH_WG - 1) / CH_WG * CH_WG;
+    const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = b0 + per < R ? b0 + per : R;
+    for (uint64_t r = b0 + threadIdx.x; r < b1; r += CH_WG) {
         const uint32_t f = frame_of_rec(bases, nframes, r);
         if (f != cur) {
             flush();

@@ -121,25 +121,8 @@ __global__ void k_comp_init(const FrameDev* __restrict__ fr, const FrameOut* __r
 }
 
 // per-tile statistics of one component type over all frames: per-dimension
-// range, value histogram (< 4096), width histogram and total Tagged bits.
-// Wavefront peer masks merge equal symbols before each LDS atomic; a wave
-// whose valid lanes all hold one symbol needs a single atomic.
-__device__ inline void hist_add(uint32_t* h, uint32_t sym, bool valid, int bits) {
-    const uint64_t act = __ballot(valid);
-    if (!act) return;
-    const uint32_t s0 = __shfl(sym, __ffsll((unsigned long long)act) - 1, 64);
-    if (__ballot(valid && sym != s0) == 0) {
-        if (lane_id() == 0) atomicAdd(&h[s0], (uint32_t)__popcll(act));
-        return;
-    }
-    uint64_t peers = act;
-    for (int b = 0; b < bits; ++b) {
-        const uint64_t bb = __ballot((sym >> b) & 1u);
-        peers &= ((sym >> b) & 1u) ? bb : ~bb;
-    }
-    if (valid && __popcll(peers & lanemask_lt()) == 0) atomicAdd(&h[sym], (uint32_t)__popcll(peers));
-}
-
+// Component statistics: range, value histogram (< 4096), width histogram and
+// total Tagged bits.
 // One workgroup walks a range of tiles and keeps its histograms
 // and range statistics in LDS / registers until the frame changes, so the
 // global atomics are per workgroup and frame, not per tile.
@@ -148,7 +131,6 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
                                                       const uint32_t* __restrict__ tile_begin, uint32_t ntiles,
                                                       CompInst* __restrict__ ci, uint32_t* __restrict__ vhist,
                                                       uint32_t* __restrict__ thist) {
-    constexpr int VBITS = sizeof(ValT) == 1 ? 8 : 12;   // bits that can differ among values < 4096
     __shared__ uint32_t s_vh[RANS_VALUE_LIMIT], s_th[RANS_TAG_LIMIT];
     __shared__ uint64_t s_red[4][2 * CH_MAXD + 2];
     for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG) s_vh[k] = 0;
@@ -260,9 +242,10 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
                 }
                 const bool small = valid && v < RANS_VALUE_LIMIT;
                 ovf |= (valid && !small) ? 1u : 0u;
-                hist_add(s_vh, small ? (uint32_t)v : 0u, small, VBITS);
+                // direct LDS atomics: cheaper here than merging equal lanes first
+                if (small) atomicAdd(&s_vh[(uint32_t)v], 1u);
             }
-            hist_add(s_th, valid ? tag : 0u, valid, 6);
+            if (valid) atomicAdd(&s_th[tag], 1u);
             if (valid) {
                 bits += (uint64_t)DIMS * tag;
                 maxtag = tag > maxtag ? tag : maxtag;

@@ -13,13 +13,25 @@ the MI355X engine:
   returns the reference's LCP3 bytes exactly, ec=True this engine's LCH1
 * chain_decompress(bytes) -> float64 (N, D) for LCH1 and LCP3 streams
 
+chain(..., ec=True, container="lcp2") writes the reference's own LCP2 instead
+of LCH1: the GPU's LCP3 streams through the host-side Zstd-22 / RVN1 chooser
+(chaincompressor_amd.lcp2), byte-identical to lcps_compress2(..., ec=True)
+with the reference's libzstd.
+
 The LCP-S and Draco wrappers of the reference (lcps_compress,
 lcps_decompress, dracoIds) are competitor baselines outside this engine.
 """
 from chaincompressor_amd import lcps_gpu_cpp, lcps_gpu_decompress_ans
 
 
-def chain(coords, eb, p, threshold, diff, bl=True, l1=True, l2=True, ec=True):
+def chain(coords, eb, p, threshold, diff, bl=True, l1=True, l2=True, ec=True, *, container="lch1"):
+    if ec and container == "lcp2":
+        from chaincompressor_amd import lcp2
+        raw = lcps_gpu_cpp.lcps_compress2(coords, eb, p, threshold, diff, bl, l1, l2, False)
+        out, stats = lcp2.to_lcp2(raw[0])
+        return [out] + stats
+    if container not in ("lch1", "lcp2"):
+        raise ValueError("container must be 'lch1' or 'lcp2'")
     return lcps_gpu_cpp.lcps_compress2(coords, eb, p, threshold, diff, bl, l1, l2, ec)
 
 

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "chain_int.hpp"
+
 #define CH_WG 256
 #define CH_ITEMS 16
 #define CH_TILE (CH_WG * CH_ITEMS)
@@ -51,24 +53,5 @@ struct FrameOut {
     int64_t wrap_min, wrap_max, wrap_range, wrap_min_corr, wrap_max_corr;
     uint64_t fixed_value_bits, fixed_diff_bits;
 };
-
-// q / p for q < 2^63 (Granlund & Montgomery 1994, Thm 4.2 with N = 63):
-// m = ceil(2^(63+l)/p) fits 64 bits and floor(q/p) = mulhi(q, m) >> (l-1).
-__host__ __device__ inline uint64_t div_p(uint64_t q, uint64_t p, uint64_t magic, uint32_t l) {
-    if (l == 0) return q;
-#ifdef __HIP_DEVICE_COMPILE__
-    return __umul64hi(q, magic) >> (l - 1);
-#else
-    return (uint64_t)(((unsigned __int128)q * magic) >> 64) >> (l - 1);
-#endif
-}
-
-__host__ __device__ inline int bitlen64(uint64_t v) {
-#ifdef __HIP_DEVICE_COMPILE__
-    return v ? 64 - __clzll((long long)v) : 0;
-#else
-    return v ? 64 - __builtin_clzll(v) : 0;
-#endif
-}
 
 }  // namespace chain

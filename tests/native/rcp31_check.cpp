@@ -1,5 +1,5 @@
 // CPU check of the rANS encoder's division by multiplication
-// (rcp31_shift / rcp31_mult / rcp31_div in chaincompressor_amd/csrc/chain_rans.hpp).
+// (rcp31_shift / rcp31_mult / rcp31_div in chaincompressor_amd/csrc/chain_int.hpp).
 // (x * m) >> s is nondecreasing in x, so it equals floor(x / f) on all of
 // [0, 1024 f) exactly when it does at every quotient boundary k f - 1 and k f;
 // those are checked for every frequency 1 <= f <= 2^20.
@@ -7,7 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 
-#include "../../chaincompressor_amd/csrc/chain_rans.hpp"
+#include "../../chaincompressor_amd/csrc/chain_int.hpp"
 
 int main(int argc, char** argv) {
     const uint32_t fmax = argc > 1 ? (uint32_t)std::strtoul(argv[1], nullptr, 10) : (1u << 20);

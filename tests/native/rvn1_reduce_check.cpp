@@ -1,12 +1,12 @@
 // CPU check of the engine's integer emulation of the reference's x87
-// long-double rounding (chaincompressor_amd/csrc/chain_rans.hpp) against
+// long-double rounding (chaincompressor_amd/csrc/chain_int.hpp) against
 // real long double arithmetic, as cpp/chain/lcps_cpp.cpp:596-641 computes it.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <random>
 
-#include "../../chaincompressor_amd/csrc/chain_rans.hpp"
+#include "../../chaincompressor_amd/csrc/chain_int.hpp"
 
 int main(int argc, char** argv) {
     const uint64_t iters = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 2000000;

@@ -97,3 +97,24 @@ def test_batch_matches_single(eng):
         single, _ = eng.compress(f, 1e-3, 16, 2, fmt="lcp3")
         assert o == single
         assert o == O.compress_streams(f, 1e-3, 16, 2).lcp3()
+
+
+def test_gpu_lcp3_exports_to_reference_lcp2(eng, golden_small):
+    """GPU LCP3 -> host LCP2 export == lcps_compress2(..., ec=True) bytes
+    (with the reference's libzstd 1.4.9; otherwise the decoded coordinates)."""
+    lcp2 = pytest.importorskip("chaincompressor_amd.lcp2")
+    try:
+        same = lcp2.zstd_version() == "1.4.9"
+    except lcp2.Lcp2Error as e:
+        pytest.skip(str(e))
+    g = golden_small
+    for name in [str(x) for x in g["names"]]:
+        eb, p, t, bl, l1, l2 = g[f"{name}__params"]
+        flags = (0 if bl else 1) | (0 if l1 else 2) | (0 if l2 else 4)
+        print("case", name, flush=True)
+        b3, _ = eng.compress(g[f"{name}__points"], float(eb), int(p), int(t), flags=flags, fmt="lcp3")
+        out, st = lcp2.to_lcp2(b3)
+        if same:
+            assert out == g[f"{name}__lcp2"].tobytes(), name
+            assert st == [int(x) for x in g[f"{name}__stats"]], name
+        assert sha(O.decode(out).tobytes()) == str(g[f"{name}__decoded_sha"]), name

@@ -150,6 +150,13 @@ def test_pybind_modules_drop_in(eng):
     assert dec.shape == pts.shape and dec.dtype == np.float64
     assert np.all(np.abs(dec - pts) <= 1e-3)
     ret = chain_caller.chain(pts, 1e-3, 16, 2, 0, ec=False)
-    ref3 = O.compress_streams(pts, 1e-3, 16, 2).lcp3()
-    assert ret[0] == ref3
+    streams = O.compress_streams(pts, 1e-3, 16, 2)
+    assert ret[0] == streams.lcp3()
     assert np.array_equal(chain_caller.chain_decompress(ret[0]), dec)
+    # ec=True with the reference's own container: LCP2 bytes and stats of lcps_compress2
+    try:
+        ret2 = chain_caller.chain(pts, 1e-3, 16, 2, 0, container="lcp2")
+    except ImportError:
+        return
+    b2, st2 = streams.lcp2()
+    assert ret2[0] == b2 and ret2[1:] == st2

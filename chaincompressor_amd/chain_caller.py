@@ -11,7 +11,8 @@ the MI355X engine:
 * chain(coords (N, D), eb, p, threshold, diff, bl, l1, l2, ec) -> [bytes, stats...]
   mirrors lcps_cpp.lcps_compress2 (cpp/chain/lcps_cpp.cpp:2340): ec=False
   returns the reference's LCP3 bytes exactly, ec=True this engine's LCH1
-* chain_decompress(bytes) -> float64 (N, D) for LCH1 and LCP3 streams
+* chain_decompress(bytes) -> float64 (N, D) for LCH1, LCP3 and LCP2 streams
+  (LCP2 is converted to LCP3 on the host first)
 
 chain(..., ec=True, container="lcp2") writes the reference's own LCP2 instead
 of LCH1: the GPU's LCP3 streams through the host-side Zstd-22 / RVN1 chooser
@@ -35,12 +36,22 @@ def chain(coords, eb, p, threshold, diff, bl=True, l1=True, l2=True, ec=True, *,
     return lcps_gpu_cpp.lcps_compress2(coords, eb, p, threshold, diff, bl, l1, l2, ec)
 
 
+def _device_stream(data):
+    """LCP2 (the reference CPU compressor's container) becomes the LCP3 of the
+    same streams on the host; LCH1 and LCP3 go to the GPU as they are."""
+    b = bytes(data)
+    if b[:4] == b"LCP2":
+        from chaincompressor_amd import lcp2
+        return lcp2.to_lcp3(b)
+    return b
+
+
 def chain_decompress(compressed_data, timer=False):
-    return lcps_gpu_decompress_ans.lcps_decompress_gpu(compressed_data)
+    return lcps_gpu_decompress_ans.lcps_decompress_gpu(_device_stream(compressed_data))
 
 
 def gpu_decompress(compressed_data, timer=False):
-    return lcps_gpu_decompress_ans.lcps_decompress_gpu(compressed_data)
+    return lcps_gpu_decompress_ans.lcps_decompress_gpu(_device_stream(compressed_data))
 
 
 def chain_gpu(coords, eb, p, threshold, diff, timer=False):

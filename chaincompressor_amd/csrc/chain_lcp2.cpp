@@ -336,9 +336,211 @@ Mat read_raw1(In r, uint32_t expect_dims) {
     return m;
 }
 
+// ---------------------------------------------------------------------------
+// LCP2 -> LCP3 (the decoder side, cpp/decompress/lcps_cpu_decompress_serial.cpp:981-1174)
+
+std::vector<uint8_t> unzstd(const uint8_t* src, size_t n, uint64_t size) {
+    need(size <= (1ull << 40), "implausible Zstd output size");
+    std::vector<uint8_t> out(size);
+    const size_t k = ZSTD_decompress(out.data(), out.size(), src, n);
+    need(!ZSTD_isError(k), "ZSTD_decompress failed", CHAIN_LCP2_EZSTD);
+    need(k == size, "Zstd output size mismatch");
+    return out;
+}
+
+In sub_blob(In& r) {
+    const uint64_t ln = r.u64();
+    need(ln <= r.end - r.pos, "truncated LCP2 section");
+    In s{r.p, r.pos, r.pos + ln};
+    r.pos += ln;
+    return s;
+}
+
+uint64_t read_varuint(In& r) {
+    uint64_t v = 0;
+    for (int shift = 0; shift <= 63; shift += 7) {
+        const uint8_t b = *r.take(1);
+        need(shift != 63 || (b & 0xfeu) == 0, "LCPS varuint overflow");
+        v |= (uint64_t)(b & 0x7fu) << shift;
+        if ((b & 0x80u) == 0) return v;
+    }
+    throw Fail(CHAIN_LCP2_EFORMAT, "invalid LCPS varuint");
+}
+
+// RVN1 decoder: table, then one state run forward over the bytes
+std::vector<uint64_t> unrvn1(In r, uint64_t expected) {
+    need(std::memcmp(r.take(4), "RVN1", 4) == 0, "invalid RVN1 header");
+    const uint32_t pb = *r.take(1);
+    const uint64_t n = r.u64();
+    need(n == expected, "RVN1 symbol count mismatch");
+    uint64_t x = r.u32();
+    In tr = sub_blob(r);
+    const uint64_t nb = r.u64();
+    need(nb == r.end - r.pos, "RVN1 byte count mismatch");
+    const uint8_t* bytes = r.take(nb);
+    std::vector<uint64_t> out(n);
+    if (n == 0) return out;
+    need(pb >= 12 && pb <= 20, "invalid RVN1 precision");
+    const uint64_t A = read_varuint(tr);
+    need(A > 0 && A <= kAlphabetLimit, "invalid rANS alphabet");
+    std::vector<uint32_t> prob(A, 0);
+    for (uint64_t i = 0; i < A;) {
+        const uint8_t first = *tr.take(1);
+        const uint32_t token = first & 3u;
+        if (token == 3) {
+            const uint64_t zeros = (first >> 2) + 1u;
+            need(zeros <= A - i, "invalid rANS zero run");
+            i += zeros;
+        } else {
+            uint32_t pr = first >> 2;
+            for (uint32_t b = 0; b < token; ++b) pr |= (uint32_t)*tr.take(1) << (6 + 8 * b);
+            need(pr > 0, "invalid rANS probability");
+            prob[i++] = pr;
+        }
+    }
+    need(tr.pos == tr.end, "unexpected trailing RVN1 table bytes");
+    const uint64_t M = 1ull << pb, lower = 4 * M;
+    std::vector<uint32_t> cum(A + 1, 0), slot(M);
+    for (uint64_t s = 0; s < A; ++s) {
+        need(prob[s] <= M - cum[s], "invalid rANS probability sum");
+        cum[s + 1] = cum[s] + prob[s];
+        for (uint32_t k = cum[s]; k < cum[s + 1]; ++k) slot[k] = (uint32_t)s;
+    }
+    need(cum[A] == M, "incomplete rANS probability table");
+    need(x >= lower && x < 256 * lower, "invalid RVN1 state");
+    uint64_t pos = 0;
+    for (uint64_t k = 0; k < n; ++k) {
+        const uint32_t sl = (uint32_t)(x & (M - 1));
+        const uint32_t s = slot[sl];
+        out[k] = s;
+        x = (uint64_t)prob[s] * (x >> pb) + sl - cum[s];
+        while (x < lower) {
+            need(pos < nb, "RVN1 stream exhausted");
+            x = (x << 8) | bytes[pos++];
+        }
+    }
+    need(x == lower && pos == nb, "RVN1 stream does not end in its initial state");
+    return out;
+}
+
+uint64_t read_msb(const std::vector<uint8_t>& b, uint64_t bit, uint32_t w) {
+    uint64_t v = 0;
+    for (uint32_t k = 0; k < w; ++k, ++bit) v = (v << 1) | ((b[bit >> 3] >> (7 - (bit & 7))) & 1u);
+    return v;
+}
+
+Mat read_component(In r, uint32_t expect_dims) {
+    const uint8_t* magic = r.take(4);
+    Mat m;
+    m.dims = r.u32();
+    m.n = r.u64();
+    need(m.dims == expect_dims, "component dimension mismatch");
+    need(m.n <= (1ull << 40), "implausible component size");
+    const uint64_t nv = m.n * m.dims;
+    m.v.assign(nv, 0);
+    if (std::memcmp(magic, "RAW1", 4) == 0) {
+        need(r.end - r.pos == 8 * nv, "RAW1 matrix size mismatch");
+        for (auto& x : m.v) x = (int64_t)r.u64();
+    } else if (std::memcmp(magic, "RVMX", 4) == 0) {
+        const std::vector<uint64_t> v = unrvn1(sub_blob(r), nv);
+        for (uint64_t i = 0; i < nv; ++i) m.v[i] = (int64_t)v[i];
+    } else if (std::memcmp(magic, "ZFMX", 4) == 0) {
+        std::vector<int64_t> mn(m.dims);
+        std::vector<uint32_t> w(m.dims);
+        std::vector<uint64_t> sz(m.dims);
+        uint64_t total = 0;
+        for (uint32_t d = 0; d < m.dims; ++d) {
+            mn[d] = (int64_t)r.u64();
+            w[d] = *r.take(1);
+            sz[d] = r.u64();
+            need(w[d] >= 1 && w[d] <= 64 && sz[d] == (m.n * w[d] + 7) / 8, "invalid fixed-width header");
+            total += sz[d];
+        }
+        need(r.u64() == total, "fixed payload size mismatch");
+        In z = sub_blob(r);
+        const std::vector<uint8_t> payload = unzstd(z.p + z.pos, z.end - z.pos, total);
+        uint64_t off = 0;
+        for (uint32_t d = 0; d < m.dims; ++d) {
+            const std::vector<uint8_t> part(payload.begin() + off, payload.begin() + off + sz[d]);
+            for (uint64_t i = 0; i < m.n; ++i)
+                m.v[i * m.dims + d] = (int64_t)((uint64_t)mn[d] + read_msb(part, i * w[d], w[d]));
+            off += sz[d];
+        }
+    } else if (std::memcmp(magic, "ZTG4", 4) == 0) {
+        const uint32_t mw = *r.take(1);
+        const std::vector<uint64_t> tags = unrvn1(sub_blob(r), m.n);
+        const uint64_t psize = r.u64();
+        In z = sub_blob(r);
+        const std::vector<uint8_t> packed = unzstd(z.p + z.pos, z.end - z.pos, psize);
+        uint64_t bit = 0;
+        for (uint64_t i = 0; i < m.n; ++i) {
+            need(tags[i] >= 1 && tags[i] <= mw && tags[i] <= 64, "invalid Tagged width");
+            need(bit + m.dims * tags[i] <= 8 * psize, "Tagged payload exhausted");
+            for (uint32_t d = 0; d < m.dims; ++d, bit += tags[i])
+                m.v[i * m.dims + d] = (int64_t)read_msb(packed, bit, (uint32_t)tags[i]);
+        }
+        need((bit + 7) / 8 == psize, "Tagged payload size mismatch");
+    } else {
+        throw Fail(CHAIN_LCP2_EFORMAT, "unknown LCP2 component");
+    }
+    need(r.pos == r.end, "unexpected trailing component bytes");
+    return m;
+}
+
 }  // namespace
 
 extern "C" {
+
+int chain_lcp3_from_lcp2(const uint8_t* lcp2, size_t len, uint8_t** out, size_t* out_len, char* err,
+                         size_t err_len) {
+    try {
+        need(lcp2 && out && out_len, "NULL argument", CHAIN_LCP2_EINVAL);
+        In r{lcp2, 0, len};
+        need(std::memcmp(r.take(4), "LCP2", 4) == 0, "not an LCP2 stream");
+        const uint32_t D = r.u32();
+        const uint64_t n = r.u64();
+        need(D >= 1 && D <= 64, "invalid LCP2 dimensions");
+        Out o;
+        o.raw("LCP3", 4);
+        o.u32(D);
+        o.u64(n);
+        o.raw(r.take(32 + 8ull * D), 32 + 8ull * D);
+        const uint32_t cdims[5] = {1, D, D, 1, D};
+        for (int c = 0; c < 5; ++c) {
+            const Mat m = read_component(sub_blob(r), cdims[c]);
+            Out comp;
+            comp.raw("RAW1", 4);
+            comp.u32(m.dims);
+            comp.u64(m.n);
+            for (int64_t x : m.v) comp.u64((uint64_t)x);
+            o.blob(comp.b);
+        }
+        const uint64_t ncorr = r.u64();
+        need(ncorr <= (1ull << 40), "implausible correction count");
+        In z = sub_blob(r);
+        need(r.pos == r.end, "unexpected trailing LCP2 bytes");
+        o.u64(ncorr);
+        o.blob(ncorr ? unzstd(z.p + z.pos, z.end - z.pos, 16 * ncorr) : std::vector<uint8_t>());
+        *out = (uint8_t*)std::malloc(o.b.size() ? o.b.size() : 1);
+        need(*out != nullptr, "out of host memory", CHAIN_LCP2_ENOMEM);
+        std::memcpy(*out, o.b.data(), o.b.size());
+        *out_len = o.b.size();
+        return CHAIN_LCP2_OK;
+    } catch (const Fail& f) {
+        if (err && err_len) {
+            std::strncpy(err, f.what(), err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        return f.code;
+    } catch (const std::bad_alloc&) {
+        if (err && err_len) {
+            std::strncpy(err, "out of host memory", err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        return CHAIN_LCP2_ENOMEM;
+    }
+}
+
 
 int chain_lcp2_from_lcp3(const uint8_t* lcp3, size_t len, int zstd_level, uint8_t** out, size_t* out_len,
                          uint64_t stats[4], char* err, size_t err_len) {

@@ -5,7 +5,8 @@
 reference's entropy-coded LCP2 (lcps_compress2(..., ec=True)): the Zstd-22 /
 RVN1 chooser over every component (cpp/chain/lcps_cpp.cpp:1140-1174). The
 result is readable by the reference's CPU decoder; with the reference's
-libzstd the bytes are identical to its own.
+libzstd the bytes are identical to its own. ``to_lcp3(lcp2)`` goes the other
+way, so LCP2 files written by the reference's CPU compressor decode on the GPU.
 """
 from __future__ import annotations
 
@@ -29,6 +30,8 @@ def _lib():
         L = C.CDLL(LIB_PATH)
         L.chain_lcp2_from_lcp3.argtypes = [C.c_char_p, C.c_size_t, C.c_int, C.POINTER(C.POINTER(C.c_uint8)),
                                            C.POINTER(C.c_size_t), C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]
+        L.chain_lcp3_from_lcp2.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.POINTER(C.c_uint8)),
+                                           C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]
         L.chain_lcp2_free.argtypes = [C.c_void_p]
         L.chain_lcp2_zstd_version.restype = C.c_char_p
         _LIB = L
@@ -52,5 +55,22 @@ def to_lcp2(lcp3: bytes, zstd_level: int = 22):
         raise Lcp2Error(err.value.decode())
     try:
         return C.string_at(out, n.value), [int(x) for x in st]
+    finally:
+        L.chain_lcp2_free(out)
+
+
+def to_lcp3(lcp2: bytes) -> bytes:
+    """LCP2 bytes (as lcps_compress2(..., ec=True) writes them) -> the LCP3 of
+    the same integer streams, which the GPU engine decodes."""
+    L = _lib()
+    data = bytes(lcp2)
+    out = C.POINTER(C.c_uint8)()
+    n = C.c_size_t()
+    err = C.create_string_buffer(256)
+    rc = L.chain_lcp3_from_lcp2(data, len(data), C.byref(out), C.byref(n), err, 256)
+    if rc != 0:
+        raise Lcp2Error(err.value.decode())
+    try:
+        return C.string_at(out, n.value)
     finally:
         L.chain_lcp2_free(out)

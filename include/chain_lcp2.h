@@ -37,6 +37,11 @@ enum chain_lcp2_status {
  * NULL) receives the message of a failure. */
 int chain_lcp2_from_lcp3(const uint8_t* lcp3, size_t len, int zstd_level, uint8_t** out, size_t* out_len,
                          uint64_t stats[4], char* err, size_t err_len);
+/* The reverse: an LCP2 stream (any of its component encodings RAW1, RVMX,
+ * ZFMX, ZTG4; cpp/decompress/lcps_cpu_decompress_serial.cpp:981-1174) to the
+ * LCP3 of the same integer streams, which the GPU engine decodes. */
+int chain_lcp3_from_lcp2(const uint8_t* lcp2, size_t len, uint8_t** out, size_t* out_len, char* err,
+                         size_t err_len);
 void chain_lcp2_free(uint8_t* p);
 /* version of the libzstd the library runs with */
 const char* chain_lcp2_zstd_version(void);

@@ -63,6 +63,18 @@ def test_golden_reference_lcp3_streams(eng, golden_small):
     assert n >= 8
 
 
+def test_golden_reference_lcp2_streams(golden_small, golden_large):
+    """The reference CPU compressor's LCP2 bytes through chain_caller's decoder
+    (host LCP2 -> LCP3, then the GPU): bit-identical to lcps_decompress_serial."""
+    pytest.importorskip("chaincompressor_amd.lcp2")
+    from chaincompressor_amd import chain_caller
+    for g in (golden_small, golden_large):
+        for name in [str(x) for x in g["names"]]:
+            print("case", name, flush=True)
+            dec = chain_caller.chain_decompress(g[f"{name}__lcp2"].tobytes())
+            assert sha(np.ascontiguousarray(dec).tobytes()) == str(g[f"{name}__decoded_sha"]), name
+
+
 def test_golden_large_roundtrip(eng, golden_large):
     g = golden_large
     db = g["dambreak_qb11__points"]

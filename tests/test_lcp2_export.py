@@ -71,3 +71,24 @@ def test_rejects_non_lcp3():
         lcp2.to_lcp2(b"LCH1" + bytes(60))
     with pytest.raises(lcp2.Lcp2Error):
         lcp2.to_lcp2(b"LCP3")
+
+
+def test_reference_lcp2_imports_to_lcp3(golden_small, golden_large):
+    """The reference's own LCP2 bytes -> LCP3 of the same streams (compared
+    with the reference's LCP3 bytes / digests) -> the reference's decoded output."""
+    g = golden_small
+    for name in [str(x) for x in g["names"]]:
+        lcp3 = lcp2.to_lcp3(g[f"{name}__lcp2"].tobytes())
+        assert sha(lcp3) == str(g[f"{name}__lcp3_sha"]), name
+        assert sha(O.decode(lcp3).tobytes()) == str(g[f"{name}__decoded_sha"]), name
+    gl = golden_large
+    for name in [str(x) for x in gl["names"]]:
+        lcp3 = lcp2.to_lcp3(gl[f"{name}__lcp2"].tobytes())
+        assert sha(lcp3) == str(gl[f"{name}__lcp3_sha"]), name
+
+
+def test_import_rejects_damaged_lcp2(golden_small):
+    b = golden_small[f"{str(golden_small['names'][0])}__lcp2"].tobytes()
+    for bad in (b[:-1], b + b"\0", b[:20], b"LCP3" + b[4:]):
+        with pytest.raises(lcp2.Lcp2Error):
+            lcp2.to_lcp3(bad)

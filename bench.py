@@ -282,6 +282,12 @@ def run_plain(args, eng):
         eng.device_free(a)
         eng.device_free(b)
     main = one_eb(args.eb, args.steps, args.warmup, True)
+    # parameter search on the device: p x threshold pairs of the same frame
+    t0 = time.perf_counter()
+    rows, best = eng.search(None, args.eb, [4, 8, 16, 32, 64], [0, 2, 4], device=(dptr, n, 3, np.float32))
+    search = {"pairs": len(rows), "seconds": round(time.perf_counter() - t0, 3),
+              "best": {"p": best[0], "threshold": best[1], "bits_per_particle": round(best[2] * 8 / n, 3)},
+              "p_values": [4, 8, 16, 32, 64], "thresholds": [0, 2, 4]}
     sweep = {}
     for eb in (() if args.no_sweep else (1e-2, 1e-4)):
         r = one_eb(eb, 2, 1, False)
@@ -290,6 +296,7 @@ def run_plain(args, eng):
                             "bits_per_particle": round(r["bytes"] * 8 / n, 3), "max_abs_error": r["maxerr"],
                             "chunk_symbols": r["rans"]["chunk_symbols"]}
     eng.device_free(dptr)
+    main["search"] = search
     return pts, nbytes, main, sweep
 
 
@@ -544,6 +551,7 @@ def main():
             "bits_per_particle": round(m["bytes"] * 8 / n, 3),
             "max_abs_error": m["maxerr"],
             "eb_sweep": sweep,
+            "parameter_search": m["search"],
             "regions_compress": region_table(m["c_steps"], K),
             "regions_decompress": region_table(m["d_steps"], K),
             "pipeline_alg_frac": {"compress": pipeline_fraction(m["c_steps"], K),

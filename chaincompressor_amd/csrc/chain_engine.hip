@@ -2090,6 +2090,51 @@ int chain_last_stage_times(chain_engine* e, float* ms, int max, char* names, siz
     return k;
 }
 
+int chain_compress_search(chain_engine* e, const void* points, int dtype, int layout, uint32_t dims, uint64_t n,
+                          double eb, const int64_t* ps, uint32_t nps, const int64_t* ts, uint32_t nts, uint32_t flags,
+                          int points_on_device, chain_stats* stats) {
+    return guarded(e, [&] {
+        if (!points || !ps || !ts || !stats || nps == 0 || nts == 0) throw Error(CHAIN_EINVAL, "NULL or empty argument");
+        if (n == 0) throw Error(CHAIN_ERUNTIME, "zero-size array to reduction operation minimum which has no identity");
+        const size_t es = dtype == CHAIN_F32 ? 4 : 8;
+        const uint64_t bytes = n * dims * es;
+        // the frame once in device memory; every configuration reads it
+        void* dpts = const_cast<void*>(points);
+        void* owned = nullptr;
+        if (!points_on_device) {
+            HIPCHK(hipMalloc(&owned, bytes));
+            HIPCHK(hipMemcpyAsync(owned, points, bytes, hipMemcpyHostToDevice, e->stream));
+            dpts = owned;
+        }
+        try {
+            const uint64_t total = (uint64_t)nps * nts;
+            const uint64_t per_call = std::max<uint64_t>(1, (1ull << 27) / n);   // bounded device memory
+            std::vector<chain_frame> fr;
+            for (uint64_t k0 = 0; k0 < total; k0 += per_call) {
+                const uint64_t k1 = std::min(total, k0 + per_call);
+                fr.clear();
+                for (uint64_t k = k0; k < k1; ++k)
+                    fr.push_back(chain_frame{dpts, n, eb, ps[k / nts], ts[k % nts], flags});
+                chain::CompressResult res;
+                res.keep_on_device = true;
+                chain::compress_batch_impl(e, fr.data(), (uint32_t)fr.size(), dtype, layout, dims, true,
+                                           CHAIN_FMT_LCH1, res);
+                for (uint64_t k = k0; k < k1; ++k) stats[k] = res.stats[k - k0];
+            }
+        } catch (...) {
+            if (owned) {
+                (void)hipStreamSynchronize(e->stream);
+                (void)hipFree(owned);
+            }
+            throw;
+        }
+        if (owned) {
+            HIPCHK(hipStreamSynchronize(e->stream));
+            HIPCHK(hipFree(owned));
+        }
+    });
+}
+
 int chain_engine_set_autotune(chain_engine* e, int on) {
     return guarded(e, [&] { e->autotune = on != 0; });
 }

@@ -55,7 +55,7 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_last_stage_bytes", "chain_engine_set_timing", "chain_server_create", "chain_server_destroy",
            "chain_server_last_error", "chain_serve_submit", "chain_serve_poll", "chain_server_clock",
            "chain_engine_set_graphs", "chain_engine_graph_stats", "chain_engine_autotune",
-           "chain_engine_set_autotune"]
+           "chain_engine_set_autotune", "chain_compress_search"]
 
 CHAIN_PENDING = 5
 REQ_COMPRESS, REQ_DECOMPRESS = 0, 1
@@ -114,6 +114,9 @@ def load_library():
     L.chain_engine_set_timing.argtypes = [C.c_void_p, C.c_int]
     L.chain_engine_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.chain_engine_set_autotune.argtypes = [C.c_void_p, C.c_int]
+    L.chain_compress_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_uint64, C.c_double,
+                                        P(C.c_int64), C.c_uint32, P(C.c_int64), C.c_uint32, C.c_uint32, C.c_int,
+                                        P(_Stats)]
     L.chain_engine_autotune.argtypes = [C.c_void_p, P(_Frame), C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_int,
                                         P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_uint64)]
     L.chain_engine_graph_stats.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]
@@ -207,6 +210,29 @@ class Engine:
     def set_timing(self, on: bool):
         """Per-stage HIP event timing (on by default)."""
         self._check(self._L.chain_engine_set_timing(self._h, 1 if on else 0))
+
+    def search(self, points, eb, ps, thresholds, flags=0, layout="aos", device=None):
+        """Container size for every (p, threshold) pair of one frame, on the
+        device. points: host array, or device=(ptr, n, dims, dtype).
+        Returns [(p, threshold, bytes, num_chained)] and the smallest entry."""
+        ps = [int(x) for x in ps]
+        ts = [int(x) for x in thresholds]
+        pa = (C.c_int64 * len(ps))(*ps)
+        ta = (C.c_int64 * len(ts))(*ts)
+        st = (_Stats * (len(ps) * len(ts)))()
+        if device is None:
+            pts = np.ascontiguousarray(points)
+            n, d = pts.shape if layout == "aos" else pts.shape[::-1]
+            ptr, dt, on_dev = pts.ctypes.data, pts.dtype, 0
+        else:
+            ptr, n, d, dt = device
+            on_dev = 1
+        self._check(self._L.chain_compress_search(self._h, C.c_void_p(ptr), _dtype_code(dt),
+                                                  CHAIN_AOS if layout == "aos" else CHAIN_SOA, d, n, float(eb), pa,
+                                                  len(ps), ta, len(ts), int(flags), on_dev, st))
+        rows = [(ps[k // len(ts)], ts[k % len(ts)], int(st[k].bytes), int(st[k].num_chained))
+                for k in range(len(ps) * len(ts))]
+        return rows, min(rows, key=lambda r: r[2])
 
     def set_autotune(self, on: bool):
         """Tune the entropy stage's chunking once per batch size bucket."""

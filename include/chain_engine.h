@@ -97,6 +97,16 @@ int chain_compress_batch(chain_engine* e, const chain_frame* frames, uint32_t nf
 int chain_compress_batch_device(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
                                 uint32_t dims, void** d_out, uint64_t* offsets, uint64_t* sizes, chain_stats* stats);
 
+/* (p, threshold) search on the device: compresses one frame (LCH1) under
+ * every pair ps[i] x ts[j] and reports each configuration's statistics in
+ * stats[i * nts + j] (bytes = container size; num_chained etc. as
+ * lcps_compress2 returns them), as the reference harness's parameter sweeps
+ * do with one CPU call per pair (reader.py:80-103, 227-248). The pairs run as
+ * the frames of batches, so a sweep costs a few launches per stage. */
+int chain_compress_search(chain_engine* e, const void* points, int dtype, int layout, uint32_t dims, uint64_t n,
+                          double eb, const int64_t* ps, uint32_t nps, const int64_t* ts, uint32_t nts, uint32_t flags,
+                          int points_on_device, chain_stats* stats);
+
 /* RVC1 parameters of the containers this engine writes: the largest lane
  * count per chunk (power of two <= 64) and the symbols per chunk. Both are
  * recorded in every section, so any decoder reads any setting. */

@@ -1227,7 +1227,7 @@ struct DecPlan {
     std::vector<DecTagged> tagged;
     std::vector<DecJob> jobs;
     std::vector<uint32_t> cum;  // per section: alphabet + 1 cumulative frequencies
-    uint64_t T = 0, R = 0, ND = 0, NK = 0, tags = 0, tag_tiles = 0, fixed_total = 0, raw_total = 0;
+    uint64_t T = 0, R = 0, ND = 0, NK = 0, tags = 0, tag_tiles = 0, fixed_tiles = 0, raw_total = 0;
     uint32_t D = 0;
 };
 
@@ -1350,14 +1350,17 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
             total += bytes;
         }
         require(r.end - r.pos == total, CHAIN_ERUNTIME, "fixed raw size mismatch");
-        for (uint32_t d = 0; d < dims; ++d) j.byte_off[d] += r.dev + r.pos;
+        for (uint32_t d = 0; d < dims; ++d) {
+            j.byte_off[d] += r.dev + r.pos;
+            j.end[d] = j.byte_off[d] + (n * j.width[d] + 7) / 8;
+        }
         j.n = n;
         j.dims = dims;
         j.dst = dst;
         j.comp = comp;
         if (n) {
-            j.first_thread = P.fixed_total;
-            P.fixed_total += n;
+            j.tile0 = (uint32_t)P.fixed_tiles;
+            P.fixed_tiles += cdiv(n, DEC_FIX_RECS);
             P.fixed.push_back(j);
         }
         return;
@@ -1626,10 +1629,10 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.jobs.size()), dim3(64 * DEC_WAVES), 0, st, X.in, X.secs,
                            X.chunks, X.jobs, X.cum, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
     tm.end();
-    if (P.fixed_total) {
+    if (P.fixed_tiles) {
         tm.begin("unpack_fixed", b_fixed);
-        hipLaunchKernelGGL(k_dec_fixed, dim3(grid_for(P.fixed_total)), dim3(CH_WG), 0, st, X.in, X.fixed,
-                           (uint32_t)P.fixed.size(), P.fixed_total, X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
+        hipLaunchKernelGGL(k_dec_fixed, dim3((unsigned)P.fixed_tiles), dim3(CH_WG), 0, st, X.in, X.fixed,
+                           (uint32_t)P.fixed.size(), X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
         tm.end();
     }
     if (P.raw_total) {
@@ -1705,13 +1708,13 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             using O = typename decltype(ot)::type;
             using RT = typename decltype(rt)::type;
             if (out_layout == CHAIN_SOA) {
-                hipLaunchKernelGGL((k_dec_dequant<DD, O, true, RT>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
-                                   X.ptb32, T, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
+                hipLaunchKernelGGL((k_dec_dequant<DD, O, true, RT>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F,
+                                   X.ptb, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
                 hipLaunchKernelGGL((k_dec_corr<O, true>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
                                    (O*)X.out, X.err);
             } else {
-                hipLaunchKernelGGL((k_dec_dequant<DD, O, false, RT>), dim3(grid_for(T)), dim3(CH_WG), 0, st, X.fr, F,
-                                   X.ptb32, T, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
+                hipLaunchKernelGGL((k_dec_dequant<DD, O, false, RT>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F,
+                                   X.ptb, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
                 hipLaunchKernelGGL((k_dec_corr<O, false>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
                                    (O*)X.out, X.err);
             }

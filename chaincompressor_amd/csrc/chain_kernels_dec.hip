@@ -223,65 +223,137 @@ __global__ __launch_bounds__(64 * DEC_WAVES) void k_dec_rans(const uint8_t* __re
 struct DecFixed {
     uint64_t byte_off[CH_MAXD]; // payload of each dimension in the input pool
     uint64_t minimum[CH_MAXD];
+    uint64_t end[CH_MAXD];      // end of each dimension's payload in the input pool
     uint32_t width[CH_MAXD];
     uint64_t n;                 // records
     uint64_t dst;               // first value index (record r -> dst + r*dims + d)
     uint32_t dims;
     int32_t comp;
-    uint64_t first_thread;      // prefix of n over jobs
+    uint32_t tile0;             // first tile of DEC_FIX_RECS records (prefix over jobs)
+    uint32_t pad;
 };
 
-// one thread per record: every dimension's bits, written together
-// One thread per record (all its dimensions). Job starts live in LDS. A
-// value of width <= 33 bits is cut from two aligned 32-bit words (MSB-first:
-// byte-swapped), unless those words would reach past the end of its payload;
-// other values are assembled byte by byte.
+// One workgroup per tile of DEC_FIX_RECS records of one job: every dimension
+// of those records, dimension by dimension, so that width, payload and
+// minimum are uniform (scalar) values. A value of width <= 33 bits is cut from
+// two aligned 32-bit words (MSB-first: byte-swapped); every such pair of the
+// tile is loaded before the first is used. Values whose pair would reach past
+// the end of their payload, or wider ones, are assembled byte by byte
+// afterwards.
 #define DEC_FIX_LDS 1024u
+#define DEC_FIX_ITEMS 4
+#define DEC_FIX_RECS (CH_WG * DEC_FIX_ITEMS)
+
+// J is a register copy of the job (uniform, indexed only by unrolled
+// constants); Jg is the job in memory, read by the rare byte-by-byte path
+template <typename OutT>
+__device__ __forceinline__ void fixed_tile(const uint8_t* __restrict__ in, const DecFixed J,
+                                          const DecFixed* __restrict__ Jg, uint32_t tile_in_job,
+                                          OutT* __restrict__ out, uint64_t vmax, uint32_t verr, uint32_t* err) {
+    const uint32_t dims = J.dims;
+    const uint64_t rbase = (uint64_t)tile_in_job * DEC_FIX_RECS;
+    uint32_t q0[CH_MAXD][DEC_FIX_ITEMS], q1[CH_MAXD][DEC_FIX_ITEMS];
+    uint32_t slow = 0;   // bit d * DEC_FIX_ITEMS + j
+#pragma unroll
+    for (int d = 0; d < CH_MAXD; ++d) {
+        if ((uint32_t)d >= dims) break;
+        const uint32_t w = J.width[d];
+        const bool narrow = w <= 33;
+#pragma unroll
+        for (int j = 0; j < DEC_FIX_ITEMS; ++j) {
+            const uint64_t r = rbase + (uint64_t)j * CH_WG + threadIdx.x;
+            const uint64_t a4 = (J.byte_off[d] + ((r * w) >> 3)) & ~3ull;
+            const bool live = r < J.n;
+            const bool fast = live && narrow && a4 + 8 <= J.end[d];
+            q0[d][j] = q1[d][j] = 0;
+            if (fast) {
+                const uint32_t* q = reinterpret_cast<const uint32_t*>(in + a4);
+                q0[d][j] = q[0];
+                q1[d][j] = q[1];
+            }
+            if (live && !fast) slow |= 1u << (d * DEC_FIX_ITEMS + j);
+        }
+    }
+    auto put = [&](uint64_t r, uint32_t d, uint64_t mn, uint64_t v) {
+        if (v > (uint64_t)INT64_MAX - mn) atomicOr(err, DERR_VALUE);
+        const uint64_t y = mn + v;
+        if (y > vmax) atomicOr(err, verr);
+        out[J.dst + r * dims + d] = (OutT)y;
+    };
+#pragma unroll
+    for (int d = 0; d < CH_MAXD; ++d) {
+        if ((uint32_t)d >= dims) break;
+        const uint32_t w = J.width[d];
+#pragma unroll
+        for (int j = 0; j < DEC_FIX_ITEMS; ++j) {
+            const uint64_t r = rbase + (uint64_t)j * CH_WG + threadIdx.x;
+            if (r >= J.n || ((slow >> (d * DEC_FIX_ITEMS + j)) & 1u)) continue;
+            const uint64_t bit = r * w;
+            const uint32_t s = (uint32_t)((J.byte_off[d] + (bit >> 3)) & 3ull) * 8 + (uint32_t)(bit & 7);
+            const uint64_t x = ((uint64_t)__builtin_bswap32(q0[d][j]) << 32) | __builtin_bswap32(q1[d][j]);
+            put(r, d, J.minimum[d], (x << s) >> (64 - w));
+        }
+    }
+    while (slow) {
+        const uint32_t b = (uint32_t)__builtin_ctz(slow);
+        slow &= slow - 1;
+        const uint32_t d = b / DEC_FIX_ITEMS, j = b % DEC_FIX_ITEMS;
+        const uint64_t r = rbase + (uint64_t)j * CH_WG + threadIdx.x;
+        const uint32_t w = Jg->width[d];
+        const uint64_t bit = r * w;
+        const uint8_t* p = in + Jg->byte_off[d] + (bit >> 3);
+        const uint32_t sh = (uint32_t)(bit & 7);
+        const uint32_t nb = (sh + w + 7) >> 3;
+        uint64_t acc_hi = 0, acc = 0;
+        for (uint32_t qq = 0; qq < nb; ++qq) {
+            acc_hi = (acc_hi << 8) | (acc >> 56);
+            acc = (acc << 8) | p[qq];
+        }
+        const uint32_t drop = 8 * nb - sh - w;
+        uint64_t v = drop ? ((acc >> drop) | (acc_hi << (64 - drop))) : acc;
+        v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
+        put(r, d, Jg->minimum[d], v);
+    }
+}
+
 __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__ in, const DecFixed* __restrict__ jobs,
-                                                     uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
+                                                     uint32_t njobs, uint32_t* idrel, uint64_t* diff,
                                                      uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
-    __shared__ uint64_t s_first[DEC_FIX_LDS];
-    const bool lds = njobs <= DEC_FIX_LDS;
-    if (lds)
-        for (uint32_t k = threadIdx.x; k < njobs; k += CH_WG) s_first[k] = jobs[k].first_thread;
-    __syncthreads();
-    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < total; g += (uint64_t)gridDim.x * CH_WG) {
+    __shared__ uint32_t s_tile0[DEC_FIX_LDS];
+    __shared__ uint32_t s_job;
+    const uint32_t tile = blockIdx.x;
+    if (njobs <= DEC_FIX_LDS) {
+        for (uint32_t k = threadIdx.x; k < njobs; k += CH_WG) s_tile0[k] = jobs[k].tile0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t lo = 0, hi = njobs;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_tile0[mid] <= tile) lo = mid; else hi = mid;
+            }
+            s_job = lo;
+        }
+    } else if (threadIdx.x == 0) {
         uint32_t lo = 0, hi = njobs;
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
-            const uint64_t ft = lds ? s_first[mid] : jobs[mid].first_thread;
-            if (ft <= g) lo = mid; else hi = mid;
+            if (jobs[mid].tile0 <= tile) lo = mid; else hi = mid;
         }
-        const DecFixed& J = jobs[lo];
-        const uint64_t r = g - J.first_thread;
-        for (uint32_t d = 0; d < J.dims; ++d) {
-            const uint32_t w = J.width[d];
-            const uint64_t bit = r * w;
-            const uint64_t a = J.byte_off[d] + (bit >> 3);
-            const uint32_t sh = (uint32_t)(bit & 7);
-            const uint64_t a4 = a & ~3ull;
-            const uint64_t end = J.byte_off[d] + (J.n * w + 7) / 8;
-            uint64_t v;
-            if (w <= 33 && a4 + 8 <= end) {
-                const uint32_t* q = reinterpret_cast<const uint32_t*>(in + a4);
-                const uint64_t x = ((uint64_t)__builtin_bswap32(q[0]) << 32) | __builtin_bswap32(q[1]);
-                const uint32_t s = (uint32_t)(a - a4) * 8 + sh;   // <= 31
-                v = (x << s) >> (64 - w);
-            } else {
-                const uint8_t* p = in + a;
-                const uint32_t nb = (sh + w + 7) >> 3;
-                uint64_t acc_hi = 0, acc = 0;
-                for (uint32_t qq = 0; qq < nb; ++qq) {
-                    acc_hi = (acc_hi << 8) | (acc >> 56);
-                    acc = (acc << 8) | p[qq];
-                }
-                const uint32_t drop = 8 * nb - sh - w;
-                v = drop ? ((acc >> drop) | (acc_hi << (64 - drop))) : acc;
-                v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
-            }
-            if (v > (uint64_t)INT64_MAX - J.minimum[d]) atomicOr(err, DERR_VALUE);
-            store_value(J.comp, J.dst + r * J.dims + d, J.minimum[d] + v, idrel, diff, keep, choice, rel, rel8, err);
-        }
+        s_job = lo;
+    }
+    __syncthreads();
+    const DecFixed* Jg = jobs + __builtin_amdgcn_readfirstlane(s_job);
+    const DecFixed J = *Jg;
+    const uint32_t t = tile - J.tile0;
+    switch (J.comp) {
+        case 0: fixed_tile<uint32_t>(in, J, Jg, t, idrel, 0xffffffffull, DERR_VALUE, err); break;
+        case 1: fixed_tile<uint64_t>(in, J, Jg, t, diff, ~0ull, 0, err); break;
+        case 2: fixed_tile<uint64_t>(in, J, Jg, t, keep, ~0ull, 0, err); break;
+        case 3: fixed_tile<uint8_t>(in, J, Jg, t, choice, 1, DERR_VALUE, err); break;
+        default:
+            if (rel8) fixed_tile<uint8_t>(in, J, Jg, t, static_cast<uint8_t*>(rel), 0xffull, DERR_QUANT, err);
+            else fixed_tile<uint32_t>(in, J, Jg, t, static_cast<uint32_t*>(rel), 0xffffffffull, DERR_VALUE, err);
+            break;
     }
 }
 
@@ -910,31 +982,69 @@ __global__ void k_dec_check_counts(const DecFrame* __restrict__ fr, uint32_t F, 
 }
 
 // q = block * p + offset; x = fma(q + 0.5, 2 eb, min) (serial.cpp:1294-1311)
+// One workgroup per point tile of a frame (the frame's fields are uniform).
+// Each pass over DQ_ITEMS points per thread loads every root first, then
+// every block and offset those roots name, then computes and stores.
+#define DQ_ITEMS 8
 template <int D, typename OutT, bool SOA, typename RelT>
 __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restrict__ fr, uint32_t F,
-                                                       const uint32_t* __restrict__ frame_pt_begin, uint64_t T,
+                                                       const uint32_t* __restrict__ tile_begin,
                                                        const uint32_t* __restrict__ root,
                                                        const uint64_t* __restrict__ compact,
                                                        const RelT* __restrict__ rel, OutT* __restrict__ out,
                                                        uint32_t* err) {
-    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < T; g += (uint64_t)gridDim.x * CH_WG) {
-        const uint32_t f = upper_find(frame_pt_begin, F, (uint32_t)g);
-        const DecFrame& Fr = fr[f];
-        const uint64_t i = g - Fr.pt_off;
-        const uint32_t rv = root[g];
-        const uint64_t rec = rv;
-        if ((rv & CH_EXT) || rec - Fr.rec_base >= Fr.nrec) { atomicOr(err, DERR_CHAIN); continue; }
+    const uint32_t tile = blockIdx.x;
+    const uint32_t f = upper_find(tile_begin, F, tile);
+    const DecFrame& Fr = fr[f];
+    const uint64_t n = Fr.n, pt_off = Fr.pt_off, rec_base = Fr.rec_base, nrec = Fr.nrec, p = Fr.p;
+    const uint64_t out_off = Fr.out_off;
+    const double step = 2.0 * Fr.eb;
+    double mins[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const uint64_t off = rel[g * D + d];
-            const uint64_t block = compact[rec * D + d];
-            if (off >= Fr.p || block > ((uint64_t)INT64_MAX - off) / Fr.p) atomicOr(err, DERR_QUANT);
-            const int64_t q = (int64_t)(block * Fr.p + off);
-            const double x = fma((double)q + 0.5, 2.0 * Fr.eb, Fr.mins[d]);
-            if (SOA) out[Fr.out_off + (uint64_t)d * Fr.n + i] = (OutT)x;
-            else out[Fr.out_off + i * D + d] = (OutT)x;
+    for (int d = 0; d < D; ++d) mins[d] = Fr.mins[d];
+    const uint64_t i0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+    uint32_t bad = 0;
+    for (int h = 0; h < CH_ITEMS; h += DQ_ITEMS) {
+        uint32_t rv[DQ_ITEMS];
+#pragma unroll
+        for (int j = 0; j < DQ_ITEMS; ++j) {
+            const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
+            rv[j] = i < n ? root[pt_off + i] : 0u;
+        }
+        uint64_t blk[DQ_ITEMS][D];
+        RelT off[DQ_ITEMS][D];
+        uint32_t ok = 0;
+#pragma unroll
+        for (int j = 0; j < DQ_ITEMS; ++j) {
+            const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
+            const uint64_t rec = rv[j];
+            const bool live = i < n;
+            const bool good = live && !(rv[j] & CH_EXT) && rec - rec_base < nrec;
+            if (live && !good) bad |= DERR_CHAIN;
+            ok |= good ? 1u << j : 0u;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                blk[j][d] = good ? compact[rec * D + d] : 0ull;
+                off[j][d] = good ? rel[(pt_off + i) * D + d] : RelT(0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < DQ_ITEMS; ++j) {
+            if (!((ok >> j) & 1u)) continue;
+            const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint64_t o = off[j][d], b = blk[j][d];
+                // b * p + o must fit in int64
+                if (o >= p || __umul64hi(b, p) != 0 || b * p > (uint64_t)INT64_MAX - o) bad |= DERR_QUANT;
+                const int64_t q = (int64_t)(b * p + o);
+                const double x = fma((double)q + 0.5, step, mins[d]);
+                if (SOA) out[out_off + (uint64_t)d * n + i] = (OutT)x;
+                else out[out_off + i * D + d] = (OutT)x;
+            }
         }
     }
+    if (bad) atomicOr(err, bad);
 }
 
 // exact corrections: (u64 index, f64 value) records, ascending, finite

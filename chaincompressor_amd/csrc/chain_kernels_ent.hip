@@ -120,21 +120,37 @@ __global__ void k_comp_init(const FrameDev* __restrict__ fr, const FrameOut* __r
     ci[i] = x;
 }
 
-// per-tile statistics of one component type over all frames: per-dimension
 // Component statistics: range, value histogram (< 4096), width histogram and
 // total Tagged bits.
 // One workgroup walks a range of tiles and keeps its histograms
 // and range statistics in LDS / registers until the frame changes, so the
 // global atomics are per workgroup and frame, not per tile.
+// Small values and widths (< CS_PRIV_BINS, the common case: in-block offsets,
+// choice flags, short links) are counted in per-thread LDS columns of 16-bit
+// pairs: bin b of thread t is half (b & 1) of word [b >> 1][t], so the 64
+// lanes of a wavefront add to 64 distinct words and never serialise on a
+// shared bin. Larger values use the shared histogram's LDS atomics. A fold
+// moves the columns into the shared histograms before a 16-bit half can
+// overflow (at most CH_ITEMS * DIMS additions per tile).
+#define CS_PRIV_BINS 32u
+#define CS_PRIV_ROWS (CS_PRIV_BINS / 2)
+#define CS_FOLD_TILES 1000u
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ vals, int c, uint32_t F,
                                                       const uint32_t* __restrict__ tile_begin, uint32_t ntiles,
                                                       CompInst* __restrict__ ci, uint32_t* __restrict__ vhist,
                                                       uint32_t* __restrict__ thist) {
+    static_assert(CH_ITEMS * 4 * CS_FOLD_TILES < 65536u, "16-bit private counters overflow");
     __shared__ uint32_t s_vh[RANS_VALUE_LIMIT], s_th[RANS_TAG_LIMIT];
+    __shared__ uint32_t s_pv[CS_PRIV_ROWS][CH_WG], s_pt[CS_PRIV_ROWS][CH_WG];
     __shared__ uint64_t s_red[4][2 * CH_MAXD + 2];
-    for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG) s_vh[k] = 0;
-    if (threadIdx.x < RANS_TAG_LIMIT) s_th[threadIdx.x] = 0;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t k = tid; k < RANS_VALUE_LIMIT; k += CH_WG) s_vh[k] = 0;
+    if (tid < RANS_TAG_LIMIT) s_th[tid] = 0;
+    for (uint32_t k = 0; k < CS_PRIV_ROWS; ++k) {
+        s_pv[k][tid] = 0;
+        s_pt[k][tid] = 0;
+    }
     __syncthreads();
     uint64_t mn[DIMS], mx[DIMS];
     uint64_t bits = 0;
@@ -150,9 +166,42 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         ovf = 0;
     };
     reset();
-    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t w = tid >> 6;
+    // private columns -> shared histograms (whole workgroup; ends synchronised).
+    // Thread (row, g) sums 16 words of its row; 16 lanes reduce the row.
+    auto fold = [&]() {
+        __syncthreads();
+        const uint32_t row = tid >> 4, g = tid & 15u;
+        uint32_t vl = 0, vh = 0, tl = 0, th = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 16; ++i) {
+            const uint32_t col = g + 16u * i;
+            const uint32_t a = s_pv[row][col], b = s_pt[row][col];
+            s_pv[row][col] = 0;
+            s_pt[row][col] = 0;
+            vl += a & 0xffffu;
+            vh += a >> 16;
+            tl += b & 0xffffu;
+            th += b >> 16;
+        }
+        for (int k = 8; k >= 1; k >>= 1) {
+            vl += __shfl_xor(vl, k, 64);
+            vh += __shfl_xor(vh, k, 64);
+            tl += __shfl_xor(tl, k, 64);
+            th += __shfl_xor(th, k, 64);
+        }
+        if (g == 0) {
+            if (vl) s_vh[2 * row] += vl;
+            if (vh) s_vh[2 * row + 1] += vh;
+            // width t (1..CS_PRIV_BINS) lives in private bin t - 1
+            if (tl) s_th[2 * row + 1] += tl;
+            if (th) s_th[2 * row + 2] += th;
+        }
+        __syncthreads();
+    };
     // publish the statistics of instance f (called by the whole workgroup)
     auto flush = [&](uint32_t f) {
+        fold();
         CompInst& C = ci[c * F + f];
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
@@ -178,16 +227,16 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         }
         __syncthreads();
         const uint32_t ci_idx = c * F + f;
-        for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG)
+        for (uint32_t k = tid; k < RANS_VALUE_LIMIT; k += CH_WG)
             if (s_vh[k]) {
                 atomicAdd(&vhist[(uint64_t)ci_idx * RANS_VALUE_LIMIT + k], s_vh[k]);
                 s_vh[k] = 0;
             }
-        if (threadIdx.x < RANS_TAG_LIMIT && s_th[threadIdx.x]) {
-            atomicAdd(&thist[(uint64_t)ci_idx * RANS_TAG_LIMIT + threadIdx.x], s_th[threadIdx.x]);
-            s_th[threadIdx.x] = 0;
+        if (tid < RANS_TAG_LIMIT && s_th[tid]) {
+            atomicAdd(&thist[(uint64_t)ci_idx * RANS_TAG_LIMIT + tid], s_th[tid]);
+            s_th[tid] = 0;
         }
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             uint64_t tb = 0;
             uint32_t mt = 0, of = 0;
             for (int k = 0; k < 4; ++k) {
@@ -212,7 +261,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         __syncthreads();
         reset();
     };
-    uint32_t cur = 0xffffffffu;
+    uint32_t cur = 0xffffffffu, since_fold = 0;
     // contiguous tile ranges per workgroup: the instance changes only at
     // frame boundaries
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
@@ -225,31 +274,45 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         if (f != cur) {
             if (cur != 0xffffffffu) flush(cur);
             cur = f;
+            since_fold = 0;
+        } else if (++since_fold == CS_FOLD_TILES) {
+            fold();
+            since_fold = 0;
         }
         const ValT* src = vals + C.base * DIMS;
+        // every load of the tile is issued before the first is used
+        ValT x[CH_ITEMS][DIMS];
+#pragma unroll
         for (int j = 0; j < CH_ITEMS; ++j) {
-            const uint64_t r = r0 + (uint64_t)j * CH_WG + threadIdx.x;
-            const bool valid = r < C.n;
+            const uint64_t r = r0 + (uint64_t)j * CH_WG + tid;
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) x[j][d] = r < C.n ? src[r * DIMS + d] : ValT(0);
+        }
+#pragma unroll
+        for (int j = 0; j < CH_ITEMS; ++j) {
+            const uint64_t r = r0 + (uint64_t)j * CH_WG + tid;
+            if (r >= C.n) continue;
             uint32_t tag = 1;
 #pragma unroll
             for (int d = 0; d < DIMS; ++d) {
-                const uint64_t v = valid ? (uint64_t)src[r * DIMS + d] : 0;
-                if (valid) {
-                    mn[d] = v < mn[d] ? v : mn[d];
-                    mx[d] = v > mx[d] ? v : mx[d];
-                    const uint32_t b = (uint32_t)bitlen64(v);
-                    tag = b > tag ? b : tag;
-                }
-                const bool small = valid && v < RANS_VALUE_LIMIT;
-                ovf |= (valid && !small) ? 1u : 0u;
-                // direct LDS atomics: cheaper here than merging equal lanes first
-                if (small) atomicAdd(&s_vh[(uint32_t)v], 1u);
+                const uint64_t v = (uint64_t)x[j][d];
+                mn[d] = v < mn[d] ? v : mn[d];
+                mx[d] = v > mx[d] ? v : mx[d];
+                const uint32_t b = (uint32_t)bitlen64(v);
+                tag = b > tag ? b : tag;
+                if (v < CS_PRIV_BINS)
+                    atomicAdd(&s_pv[(uint32_t)v >> 1][tid], 1u << (((uint32_t)v & 1u) << 4));
+                else if (v < RANS_VALUE_LIMIT)
+                    atomicAdd(&s_vh[(uint32_t)v], 1u);
+                else
+                    ovf = 1u;
             }
-            if (valid) atomicAdd(&s_th[tag], 1u);
-            if (valid) {
-                bits += (uint64_t)DIMS * tag;
-                maxtag = tag > maxtag ? tag : maxtag;
-            }
+            if (tag <= CS_PRIV_BINS)
+                atomicAdd(&s_pt[(tag - 1) >> 1][tid], 1u << (((tag - 1) & 1u) << 4));
+            else
+                atomicAdd(&s_th[tag], 1u);
+            bits += (uint64_t)DIMS * tag;
+            maxtag = tag > maxtag ? tag : maxtag;
         }
     }
     if (cur != 0xffffffffu) flush(cur);

@@ -423,7 +423,7 @@ template <class X>
 static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint32_t F, uint32_t D, bool rel8, const FrameDev* fdev,
                           FrameOut* fo, const uint32_t* idrel, const uint64_t* diff, const uint64_t* keep,
                           const uint8_t* choice2, const void* rel, const uint64_t* corr_idx, const double* corr_val,
-                          const uint32_t* tb, uint32_t ntiles,
+                          const uint32_t* tb, const uint32_t* tb_host, uint32_t ntiles,
                           uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const RansConfig& rc, uint64_t stride,
                           uint32_t* err, uint32_t* ticket, X& x) {
     const uint32_t ninst = 5 * F, nsec = 10 * F;
@@ -442,12 +442,29 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
             else fn(4, TypeTag<uint32_t>{}, IC<DD>{}, rel);
         });
     };
+    // statistics slots: the most any frame has decides the reducer's grid
+    const uint32_t stats_grid = std::min<uint32_t>(ntiles, CS_GRID);
+    uint32_t reduce_y = 1;
+    if (ntiles) {
+        const uint32_t per = (uint32_t)cdiv(ntiles, stats_grid);
+        for (uint32_t f = 0; f < F; ++f) {
+            const uint32_t t0 = tb_host[f], t1 = tb_host[f + 1];
+            if (t0 == t1) continue;
+            const uint32_t lo = (uint32_t)cdiv(t0, per), hi = std::min((uint32_t)cdiv(t1, per), stats_grid);
+            const uint32_t cnt = (hi > lo ? hi - lo : 0) + (t0 % per ? 1u : 0u);
+            reduce_y = std::max(reduce_y, (uint32_t)cdiv(cnt, CS_REDUCE_SLOTS));
+        }
+    }
     each_comp([&](int c, auto vt, auto dd, const void* v) {
         using V = typename decltype(vt)::type;
         constexpr int DC = decltype(dd)::value;
         et.stats[c] = tm.begin("comp_stats");
-        hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(std::min<uint32_t>(ntiles, 2048)), dim3(CH_WG), 0, st,
-                           (const V*)v, c, F, tb, ntiles, x.ci, x.vhist, x.thist);
+        if (ntiles) {
+            hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(stats_grid), dim3(CH_WG), 0, st, (const V*)v, c, F, tb,
+                               ntiles, x.ci, x.slots, x.vhist, x.thist);
+            hipLaunchKernelGGL(k_comp_reduce, dim3(F, reduce_y), dim3(CH_WG), 0, st, x.slots, stats_grid, ntiles, tb,
+                               F, c, x.ci, x.vhist, x.thist);
+        }
         tm.end();
     });
     tm.begin("rans_table", 5ull * F * (RANS_VALUE_LIMIT + RANS_TAG_LIMIT) * 4 + 10ull * F * RANS_VALUE_LIMIT * 8);
@@ -714,6 +731,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             Section* sec;
             uint32_t* vhist;
             uint32_t* thist;
+            CompSlot* slots;
             uint64_t* pc;
             uint8_t* tbytes;
             uint32_t* comp_chunk;
@@ -735,6 +753,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.sec = c.take<Section>(10 * F);
         x.vhist = c.take<uint32_t>(5ull * F * RANS_VALUE_LIMIT);
         x.thist = c.take<uint32_t>(5ull * F * RANS_TAG_LIMIT);
+        x.slots = c.take<CompSlot>(CS_GRID + F);
         x.pc = c.take<uint64_t>(10ull * F * RANS_VALUE_LIMIT);
         x.tbytes = c.take<uint8_t>(10ull * F * TABLE_BYTES_MAX);
         x.comp_chunk = c.take<uint32_t>(8);
@@ -1005,7 +1024,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         tm.end();
         if (format == CHAIN_FMT_LCH1)
             entropy_stage(st, tm, et, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
-                          B.corr_val, A.tb, ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err, B.ticket, X);
+                          B.corr_val, A.tb, tile_begin.data(), ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err,
+                          B.ticket, X);
     };
     if (e->use_graphs) {
         GraphKey key;

@@ -122,43 +122,73 @@ __global__ void k_comp_init(const FrameDev* __restrict__ fr, const FrameOut* __r
 
 // Component statistics: range, value histogram (< 4096), width histogram and
 // total Tagged bits.
-// One workgroup walks a range of tiles and keeps its histograms
-// and range statistics in LDS / registers until the frame changes, so the
-// global atomics are per workgroup and frame, not per tile.
+// One workgroup walks a contiguous range of tiles and keeps its histograms
+// and range statistics in LDS / registers until the frame changes.
 // Small values and widths (< CS_PRIV_BINS, the common case: in-block offsets,
 // choice flags, short links) are counted in per-thread LDS columns of 16-bit
 // pairs: bin b of thread t is half (b & 1) of word [b >> 1][t], so the 64
 // lanes of a wavefront add to 64 distinct words and never serialise on a
-// shared bin. Larger values use the shared histogram's LDS atomics. A fold
-// moves the columns into the shared histograms before a 16-bit half can
-// overflow (at most CH_ITEMS * DIMS additions per tile).
+// shared bin. Larger values use a shared LDS histogram. A fold moves the
+// columns into 32-bit totals before a 16-bit half can overflow (at most
+// CH_ITEMS * DIMS additions per tile).
+// Publishing: the small bins and the range statistics of a (workgroup, frame)
+// pair go to a slot of their own with plain stores (k_comp_reduce combines
+// them), because thousands of workgroups adding into the same few words
+// serialise on the device's atomic unit; larger bins are spread over many
+// words and are added with global atomics directly.
 #define CS_PRIV_BINS 32u
 #define CS_PRIV_ROWS (CS_PRIV_BINS / 2)
 #define CS_FOLD_TILES 1000u
+#define CS_GRID 2048u
+#define CS_REDUCE_SLOTS 64u
+
+struct CompSlot {
+    uint32_t vh[CS_PRIV_BINS];     // values 0 .. CS_PRIV_BINS-1
+    uint32_t th[CS_PRIV_BINS];     // widths 1 .. CS_PRIV_BINS
+    uint64_t dmin[CH_MAXD], dmax[CH_MAXD];
+    uint64_t bits;
+    uint32_t max_tag, overflow;
+    uint32_t pad[12];
+};
+static_assert(sizeof(CompSlot) == 384, "slot layout");
+
+// slot of the (workgroup b, frame f) pair: b for the frame of b's first tile,
+// G + f for a frame that starts inside b's range (at most one workgroup)
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ vals, int c, uint32_t F,
                                                       const uint32_t* __restrict__ tile_begin, uint32_t ntiles,
-                                                      CompInst* __restrict__ ci, uint32_t* __restrict__ vhist,
-                                                      uint32_t* __restrict__ thist) {
+                                                      const CompInst* __restrict__ ci, CompSlot* __restrict__ slots,
+                                                      uint32_t* __restrict__ vhist, uint32_t* __restrict__ thist) {
     static_assert(CH_ITEMS * 4 * CS_FOLD_TILES < 65536u, "16-bit private counters overflow");
+    // values of at most 32 bits are compared and measured in 32-bit registers
+    using AccT = typename std::conditional<(sizeof(ValT) <= 4), uint32_t, uint64_t>::type;
     __shared__ uint32_t s_vh[RANS_VALUE_LIMIT], s_th[RANS_TAG_LIMIT];
+    __shared__ uint32_t s_sv[CS_PRIV_BINS], s_st[CS_PRIV_BINS];
     __shared__ uint32_t s_pv[CS_PRIV_ROWS][CH_WG], s_pt[CS_PRIV_ROWS][CH_WG];
     __shared__ uint64_t s_red[4][2 * CH_MAXD + 2];
     const uint32_t tid = threadIdx.x;
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t tb0 = blockIdx.x * per, tb1 = tb0 + per < ntiles ? tb0 + per : ntiles;
+    if (tb0 >= tb1) return;
     for (uint32_t k = tid; k < RANS_VALUE_LIMIT; k += CH_WG) s_vh[k] = 0;
     if (tid < RANS_TAG_LIMIT) s_th[tid] = 0;
+    if (tid < CS_PRIV_BINS) {
+        s_sv[tid] = 0;
+        s_st[tid] = 0;
+    }
     for (uint32_t k = 0; k < CS_PRIV_ROWS; ++k) {
         s_pv[k][tid] = 0;
         s_pt[k][tid] = 0;
     }
     __syncthreads();
-    uint64_t mn[DIMS], mx[DIMS];
+    const uint32_t f_first = upper_find(tile_begin, F, tb0);
+    AccT mn[DIMS], mx[DIMS];
     uint64_t bits = 0;
     uint32_t maxtag = 0, ovf = 0;
     auto reset = [&]() {
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
-            mn[d] = ~0ull;
+            mn[d] = (AccT)~0ull;
             mx[d] = 0;
         }
         bits = 0;
@@ -167,7 +197,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
     };
     reset();
     const uint32_t w = tid >> 6;
-    // private columns -> shared histograms (whole workgroup; ends synchronised).
+    // private columns -> 32-bit totals (whole workgroup; ends synchronised).
     // Thread (row, g) sums 16 words of its row; 16 lanes reduce the row.
     auto fold = [&]() {
         __syncthreads();
@@ -191,22 +221,20 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
             th += __shfl_xor(th, k, 64);
         }
         if (g == 0) {
-            if (vl) s_vh[2 * row] += vl;
-            if (vh) s_vh[2 * row + 1] += vh;
-            // width t (1..CS_PRIV_BINS) lives in private bin t - 1
-            if (tl) s_th[2 * row + 1] += tl;
-            if (th) s_th[2 * row + 2] += th;
+            s_sv[2 * row] += vl;
+            s_sv[2 * row + 1] += vh;
+            s_st[2 * row] += tl;
+            s_st[2 * row + 1] += th;
         }
         __syncthreads();
     };
-    // publish the statistics of instance f (called by the whole workgroup)
+    // publish the statistics of frame f (called by the whole workgroup)
     auto flush = [&](uint32_t f) {
         fold();
-        CompInst& C = ci[c * F + f];
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
             for (int k = 32; k >= 1; k >>= 1) {
-                const uint64_t a = __shfl_xor(mn[d], k, 64), b = __shfl_xor(mx[d], k, 64);
+                const AccT a = __shfl_xor(mn[d], k, 64), b = __shfl_xor(mx[d], k, 64);
                 mn[d] = a < mn[d] ? a : mn[d];
                 mx[d] = b > mx[d] ? b : mx[d];
             }
@@ -219,24 +247,22 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         }
         if (lane_id() == 0) {
             for (int d = 0; d < DIMS; ++d) {
-                s_red[w][2 * d] = mn[d];
+                // a wavefront without records keeps the 64-bit identity (~0, 0)
+                s_red[w][2 * d] = bits ? (uint64_t)mn[d] : ~0ull;
                 s_red[w][2 * d + 1] = mx[d];
             }
             s_red[w][2 * CH_MAXD] = bits;
             s_red[w][2 * CH_MAXD + 1] = ((uint64_t)maxtag << 32) | ovf;
         }
         __syncthreads();
-        const uint32_t ci_idx = c * F + f;
-        for (uint32_t k = tid; k < RANS_VALUE_LIMIT; k += CH_WG)
-            if (s_vh[k]) {
-                atomicAdd(&vhist[(uint64_t)ci_idx * RANS_VALUE_LIMIT + k], s_vh[k]);
-                s_vh[k] = 0;
-            }
-        if (tid < RANS_TAG_LIMIT && s_th[tid]) {
-            atomicAdd(&thist[(uint64_t)ci_idx * RANS_TAG_LIMIT + tid], s_th[tid]);
-            s_th[tid] = 0;
+        CompSlot& S = slots[f == f_first ? blockIdx.x : gridDim.x + f];
+        if (tid < CS_PRIV_BINS) {
+            S.vh[tid] = s_sv[tid];
+            S.th[tid] = s_st[tid];
+            s_sv[tid] = 0;
+            s_st[tid] = 0;
         }
-        if (tid == 0) {
+        if (tid == CS_PRIV_BINS) {
             uint64_t tb = 0;
             uint32_t mt = 0, of = 0;
             for (int k = 0; k < 4; ++k) {
@@ -245,32 +271,37 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
                 mt = a > mt ? a : mt;
                 of |= (uint32_t)s_red[k][2 * CH_MAXD + 1];
             }
-            for (int d = 0; d < DIMS; ++d) {
+            for (int d = 0; d < CH_MAXD; ++d) {
                 uint64_t a = ~0ull, b = 0;
-                for (int k = 0; k < 4; ++k) {
-                    a = s_red[k][2 * d] < a ? s_red[k][2 * d] : a;
-                    b = s_red[k][2 * d + 1] > b ? s_red[k][2 * d + 1] : b;
-                }
-                atomicMin((unsigned long long*)&C.dmin[d], a);
-                atomicMax((unsigned long long*)&C.dmax[d], b);
+                if (d < DIMS)
+                    for (int k = 0; k < 4; ++k) {
+                        a = s_red[k][2 * d] < a ? s_red[k][2 * d] : a;
+                        b = s_red[k][2 * d + 1] > b ? s_red[k][2 * d + 1] : b;
+                    }
+                S.dmin[d] = a;
+                S.dmax[d] = b;
             }
-            atomicAdd((unsigned long long*)&C.tag_bits, tb);
-            atomicMax(&C.max_tag, mt);
-            if (of) atomicOr(&C.overflow, 1u);
+            S.bits = tb;
+            S.max_tag = mt;
+            S.overflow = of;
+        }
+        const uint32_t ci_idx = c * F + f;
+        for (uint32_t k = CS_PRIV_BINS + tid; k < RANS_VALUE_LIMIT; k += CH_WG)
+            if (s_vh[k]) {
+                atomicAdd(&vhist[(uint64_t)ci_idx * RANS_VALUE_LIMIT + k], s_vh[k]);
+                s_vh[k] = 0;
+            }
+        if (tid > CS_PRIV_BINS && tid < RANS_TAG_LIMIT && s_th[tid]) {
+            atomicAdd(&thist[(uint64_t)ci_idx * RANS_TAG_LIMIT + tid], s_th[tid]);
+            s_th[tid] = 0;
         }
         __syncthreads();
         reset();
     };
     uint32_t cur = 0xffffffffu, since_fold = 0;
-    // contiguous tile ranges per workgroup: the instance changes only at
-    // frame boundaries
-    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-    const uint32_t tb0 = blockIdx.x * per, tb1 = tb0 + per < ntiles ? tb0 + per : ntiles;
     for (uint32_t tile = tb0; tile < tb1; ++tile) {
         const uint32_t f = upper_find(tile_begin, F, tile);
-        const CompInst& C = ci[c * F + f];
-        const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
-        if (r0 >= C.n) continue;
+        // every frame the range touches is published, even with no records
         if (f != cur) {
             if (cur != 0xffffffffu) flush(cur);
             cur = f;
@@ -279,6 +310,10 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
             fold();
             since_fold = 0;
         }
+        const CompInst& C = ci[c * F + f];
+        const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+        const uint64_t n = C.n;
+        if (r0 >= n) continue;
         const ValT* src = vals + C.base * DIMS;
         // every load of the tile is issued before the first is used
         ValT x[CH_ITEMS][DIMS];
@@ -286,19 +321,19 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         for (int j = 0; j < CH_ITEMS; ++j) {
             const uint64_t r = r0 + (uint64_t)j * CH_WG + tid;
 #pragma unroll
-            for (int d = 0; d < DIMS; ++d) x[j][d] = r < C.n ? src[r * DIMS + d] : ValT(0);
+            for (int d = 0; d < DIMS; ++d) x[j][d] = r < n ? src[r * DIMS + d] : ValT(0);
         }
 #pragma unroll
         for (int j = 0; j < CH_ITEMS; ++j) {
             const uint64_t r = r0 + (uint64_t)j * CH_WG + tid;
-            if (r >= C.n) continue;
+            if (r >= n) continue;
             uint32_t tag = 1;
 #pragma unroll
             for (int d = 0; d < DIMS; ++d) {
-                const uint64_t v = (uint64_t)x[j][d];
+                const AccT v = (AccT)x[j][d];
                 mn[d] = v < mn[d] ? v : mn[d];
                 mx[d] = v > mx[d] ? v : mx[d];
-                const uint32_t b = (uint32_t)bitlen64(v);
+                const uint32_t b = sizeof(AccT) == 4 ? 32u - (uint32_t)__clz((uint32_t)v) : (uint32_t)bitlen64(v);
                 tag = b > tag ? b : tag;
                 if (v < CS_PRIV_BINS)
                     atomicAdd(&s_pv[(uint32_t)v >> 1][tid], 1u << (((uint32_t)v & 1u) << 4));
@@ -316,6 +351,95 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         }
     }
     if (cur != 0xffffffffu) flush(cur);
+}
+
+// Combines the slots of every frame of one component: block (f, y) takes up
+// to CS_REDUCE_SLOTS slots of frame f and adds them into the frame's
+// statistics. Frame f's slots are G + f when it starts inside a workgroup's
+// range, then the workgroups whose first tile lies in f.
+__global__ __launch_bounds__(CH_WG) void k_comp_reduce(const CompSlot* __restrict__ slots, uint32_t G, uint32_t ntiles,
+                                                       const uint32_t* __restrict__ tile_begin, uint32_t F, int c,
+                                                       CompInst* __restrict__ ci, uint32_t* __restrict__ vhist,
+                                                       uint32_t* __restrict__ thist) {
+    __shared__ uint32_t s_sum[4][2 * CS_PRIV_BINS];
+    const uint32_t f = blockIdx.x;
+    const uint32_t t0 = tile_begin[f], t1 = tile_begin[f + 1];
+    if (t0 == t1) return;
+    const uint32_t per = (ntiles + G - 1) / G;
+    const uint32_t b_lo = (t0 + per - 1) / per;
+    uint32_t b_hi = (t1 + per - 1) / per;
+    b_hi = b_hi < G ? b_hi : G;
+    const uint32_t mid = t0 % per != 0 ? 1u : 0u;
+    const uint32_t cnt = (b_hi > b_lo ? b_hi - b_lo : 0u) + mid;
+    const uint32_t i0 = blockIdx.y * CS_REDUCE_SLOTS;
+    if (i0 >= cnt) return;
+    const uint32_t i1 = i0 + CS_REDUCE_SLOTS < cnt ? i0 + CS_REDUCE_SLOTS : cnt;
+    auto slot_of = [&](uint32_t i) -> const CompSlot& {
+        return slots[mid ? (i == 0 ? G + f : b_lo + i - 1) : b_lo + i];
+    };
+    const uint32_t tid = threadIdx.x, word = tid & 63u, grp = tid >> 6;
+    // bins: thread (grp, word) sums word `word` of slots i0 + grp, + 4, ...
+    uint32_t acc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < CS_REDUCE_SLOTS / 4; ++k) {
+        const uint32_t i = i0 + grp + 4 * k;
+        if (i < i1) {
+            const CompSlot& S = slot_of(i);
+            acc += word < CS_PRIV_BINS ? S.vh[word] : S.th[word - CS_PRIV_BINS];
+        }
+    }
+    s_sum[grp][word] = acc;
+    // range statistics: lane l of wavefront 0 takes slot i0 + l
+    if (grp == 0) {
+        const uint32_t i = i0 + word;
+        const bool live = i < i1;
+        uint64_t mn[CH_MAXD], mx[CH_MAXD], bits = 0;
+        uint32_t mt = 0, of = 0;
+        for (int d = 0; d < CH_MAXD; ++d) {
+            mn[d] = ~0ull;
+            mx[d] = 0;
+        }
+        if (live) {
+            const CompSlot& S = slot_of(i);
+            for (int d = 0; d < CH_MAXD; ++d) {
+                mn[d] = S.dmin[d];
+                mx[d] = S.dmax[d];
+            }
+            bits = S.bits;
+            mt = S.max_tag;
+            of = S.overflow;
+        }
+        for (int k = 32; k >= 1; k >>= 1) {
+            for (int d = 0; d < CH_MAXD; ++d) {
+                const uint64_t a = __shfl_xor(mn[d], k, 64), b = __shfl_xor(mx[d], k, 64);
+                mn[d] = a < mn[d] ? a : mn[d];
+                mx[d] = b > mx[d] ? b : mx[d];
+            }
+            bits += __shfl_xor(bits, k, 64);
+            const uint32_t a = __shfl_xor(mt, k, 64);
+            mt = a > mt ? a : mt;
+            of |= __shfl_xor(of, k, 64);
+        }
+        if (word == 0) {
+            CompInst& C = ci[c * F + f];
+            for (uint32_t d = 0; d < C.dims; ++d) {
+                if (mn[d] != ~0ull) atomicMin((unsigned long long*)&C.dmin[d], mn[d]);
+                if (mx[d]) atomicMax((unsigned long long*)&C.dmax[d], mx[d]);
+            }
+            if (bits) atomicAdd((unsigned long long*)&C.tag_bits, bits);
+            if (mt) atomicMax(&C.max_tag, mt);
+            if (of) atomicOr(&C.overflow, 1u);
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * CS_PRIV_BINS) {
+        const uint32_t v = s_sum[0][tid] + s_sum[1][tid] + s_sum[2][tid] + s_sum[3][tid];
+        const uint64_t ci_idx = (uint64_t)c * F + f;
+        if (v) {
+            if (tid < CS_PRIV_BINS) atomicAdd(&vhist[ci_idx * RANS_VALUE_LIMIT + tid], v);
+            else atomicAdd(&thist[ci_idx * RANS_TAG_LIMIT + (tid - CS_PRIV_BINS) + 1], v);
+        }
+    }
 }
 
 // ------------------------------------------------------------------------

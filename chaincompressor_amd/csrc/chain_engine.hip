@@ -708,7 +708,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
 
     // LCH1 entropy stage buffers, sized from the particle counts alone
     const RansConfig rc = e->rans;
-    const uint64_t stride = CHUNK_STATE_BYTES + 3ull * rc.chunk_symbols;
+    // per-chunk scratch: states, then at most 3 bytes per symbol, rounded so
+    // that the encoder's aligned block stores stay aligned
+    const uint64_t stride = CHUNK_STATE_BYTES + align256(3ull * rc.chunk_symbols);
     uint64_t comp_chunk_bound[5] = {0, 0, 0, 0, 0};
     uint64_t out_bound = 0;
     std::vector<uint32_t> tile1k(F + 1, 0);

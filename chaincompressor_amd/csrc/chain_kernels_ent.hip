@@ -609,56 +609,72 @@ __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_
 // rANS encoding, one wavefront per chunk; lanes beyond the section's lane
 // count idle. Symbols are loaded one group of ENC_GROUP steps ahead of the
 // group being coded. For alphabets <= 256 the table lives in LDS together
-// with an exact reciprocal of every frequency (rcp31_* in chain_rans.hpp),
-// so the state update needs no integer division.
+// with an exact reciprocal of every frequency, so the state update needs no
+// integer division: with l = ceil(log2 f) and m = ceil(2^(31+l) / f) < 2^32
+// (rcp31_* in chain_int.hpp), floor(x / f) = mulhi(x, m) >> (l - 1) for
+// l >= 1, and x for f = 1.
 #define ENC_GROUP 4
 
 struct EncLds {
     uint64_t pc[256];   // (cum << 32) | freq
-    uint64_t ml[256];   // reciprocal multiplier (< 2^34) | shift - 31 in bits 56..63
+    uint64_t ml[256];   // reciprocal multiplier m (< 2^32) | l << 32
 };
 
+// symbols t-1, ..., t-g of this lane (steps are coded backwards); a missing
+// symbol is 0xffffffff. The loads are unconditional (clamped to the chunk's
+// first symbol) so that they stay in flight across the group being coded.
 template <typename ValT, int DIMS>
 __device__ inline void enc_group_load(const ValT* __restrict__ src, bool tags, uint64_t first, uint64_t Sc,
                                       uint32_t L, uint32_t lane, uint64_t t, uint32_t g, uint32_t* sym) {
 #pragma unroll
     for (int j = 0; j < ENC_GROUP; ++j) {
-        sym[j] = 0xffffffffu;
-        if ((uint32_t)j < g) {
-            const uint64_t k = (t - 1 - j) * L + lane;
-            if (lane < L && k < Sc) {
-                const uint64_t q = first + k;
-                if (!tags) {
-                    sym[j] = (uint32_t)src[q];
-                } else {
-                    uint32_t tg = 1;
+        const uint64_t k = (t - 1 - j) * L + lane;
+        const bool valid = (uint32_t)j < g && lane < L && k < Sc;
+        const uint64_t q = first + (valid ? k : 0ull);
+        uint32_t v;
+        if (!tags) {
+            v = (uint32_t)src[q];
+        } else {
+            uint32_t tg = 1;
 #pragma unroll
-                    for (int d = 0; d < DIMS; ++d) {
-                        const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
-                        tg = b > tg ? b : tg;
-                    }
-                    sym[j] = tg;
-                }
+            for (int d = 0; d < DIMS; ++d) {
+                const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
+                tg = b > tg ? b : tg;
             }
+            v = tg;
         }
+        sym[j] = valid ? v : 0xffffffffu;
     }
 }
 
-template <typename ValT, int DIMS>
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// One wavefront codes one chunk, lane l holding the state of lane l. The
+// renormalisation bytes of a step are placed plane by plane (every emitting
+// lane's first byte in lane order, then the second bytes, then the third) at
+// decreasing offsets. They are staged in an LDS ring of ENC_RING bytes and
+// written out in aligned blocks of ENC_FLUSH bytes (16 per lane), so the
+// global stores are wide and few; what is left at the end is copied byte by
+// byte. A step has no vector branches: lanes without a byte to emit write to
+// a byte of their own past the ring.
+#define ENC_RING 2048u
+#define ENC_FLUSH 1024u
+template <typename ValT, int DIMS, bool SMALL>
 __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, uint32_t si, const CompInst& C,
                                  const Section& S, const uint64_t* __restrict__ gpc, EncLds& lds,
-                                 uint8_t* __restrict__ scratch, uint64_t stride, uint32_t* __restrict__ chunk_len,
-                                 uint32_t* __restrict__ chunk_pos) {
+                                 uint8_t* __restrict__ ring, uint8_t* __restrict__ scratch, uint64_t stride,
+                                 uint32_t* __restrict__ chunk_len, uint32_t* __restrict__ chunk_pos) {
     const bool tags = (si & 1) != 0;
     const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
-    const bool small = S.alphabet <= 256;
-    if (small)
+    if (SMALL)
         for (uint32_t b = lane; b < S.alphabet; b += 64) {
             const uint64_t e = gpc[b];
             const uint32_t f = (uint32_t)e;
             lds.pc[b] = e;
             const uint32_t l = f ? rcp31_shift(f) : 0u;
-            lds.ml[b] = f ? (rcp31_mult(f, l) | ((uint64_t)l << 56)) : 0ull;
+            lds.ml[b] = f ? (rcp31_mult(f, l) | ((uint64_t)l << 32)) : 0ull;
         }
     __syncthreads();
     const uint32_t M = 1u << pb, lower = 4u * M;
@@ -667,11 +683,12 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
     const uint64_t T = (Sc + L - 1) / L;
     uint8_t* out = scratch + (uint64_t)chunk * stride;
     uint8_t* stream = out + CHUNK_STATE_BYTES;
-    const uint64_t end = stride - CHUNK_STATE_BYTES;
-    uint64_t pos = end;
+    // stream offsets: [pos, end) written so far, [flushed, end) already global
+    const uint32_t end = (uint32_t)(stride - CHUNK_STATE_BYTES);
+    uint32_t pos = end, flushed = end;
     uint32_t x = lower;
     const ValT* src = vals + C.base * DIMS;
-    const uint64_t lt = lanemask_lt();
+    const uint32_t dummy = ENC_RING + lane;
     uint32_t cur[ENC_GROUP], nxt[ENC_GROUP];
     uint64_t t = T;
     uint32_t g = t < ENC_GROUP ? (uint32_t)t : ENC_GROUP;
@@ -684,8 +701,10 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
             const bool a = cur[j] != 0xffffffffu;
-            ent[j] = a ? (small ? lds.pc[cur[j]] : gpc[cur[j]]) : 1ull;
-            ml[j] = a && small ? lds.ml[cur[j]] : 0ull;
+            const uint32_t sj = a ? cur[j] : 0u;
+            const uint64_t e = SMALL ? lds.pc[sj] : gpc[sj];
+            ent[j] = a ? e : 1ull;
+            ml[j] = SMALL ? lds.ml[sj] : 0ull;
         }
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
@@ -693,32 +712,64 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             const bool active = cur[j] != 0xffffffffu;
             const uint64_t e = ent[j];
             const uint32_t f = (uint32_t)e, cu = (uint32_t)(e >> 32);
+            // bytes to emit: x < 2^30 <= 256^3 * 1024 * f, so at most three
             const uint32_t xmax = 1024u * f;     // (lower >> pb) * 256 * f
-            uint32_t kb = 0, xx = x;
-            if (active)
-                while (xx >= xmax) { ++kb; xx >>= 8; }
+            const uint32_t kb = active ? (uint32_t)(x >= xmax) + (uint32_t)((x >> 8) >= xmax) +
+                                             (uint32_t)((x >> 16) >= xmax)
+                                       : 0u;
+            const uint32_t xx = x >> (8 * kb);
             const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
             const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
             pos -= c0 + c1 + c2;
-            if (kb > 0) stream[pos + __popcll(m0 & lt)] = (uint8_t)(x >> (8 * (kb - 1)));
-            if (kb > 1) stream[pos + c0 + __popcll(m1 & lt)] = (uint8_t)(x >> (8 * (kb - 2)));
-            if (kb > 2) stream[pos + c0 + c1 + __popcll(m2 & lt)] = (uint8_t)(x >> (8 * (kb - 3)));
-            if (active) {
-                const uint32_t q =
-                    small ? rcp31_div(xx, ml[j] & ((1ull << 56) - 1), (uint32_t)(ml[j] >> 56)) : xx / f;
-                x = q * M + (xx - q * f) + cu;
+            const uint32_t a0 = kb > 0 ? (pos + mbcnt64(m0)) & (ENC_RING - 1) : dummy;
+            const uint32_t a1 = kb > 1 ? (pos + c0 + mbcnt64(m1)) & (ENC_RING - 1) : dummy;
+            const uint32_t a2 = kb > 2 ? (pos + c0 + c1 + mbcnt64(m2)) & (ENC_RING - 1) : dummy;
+            ring[a0] = (uint8_t)(x >> (8 * (kb > 0 ? kb - 1 : 0)));
+            ring[a1] = (uint8_t)(x >> (8 * (kb > 1 ? kb - 2 : 0)));
+            ring[a2] = (uint8_t)x;
+            uint32_t q;
+            if (SMALL) {
+                const uint32_t m = (uint32_t)ml[j], l = (uint32_t)(ml[j] >> 32);
+                q = l ? __umulhi(xx, m) >> (l - 1) : xx;
+            } else {
+                q = xx / f;
             }
+            const uint32_t xn = q * M + (xx - q * f) + cu;
+            x = active ? xn : x;
+        }
+        // a group adds at most ENC_GROUP * 192 bytes: one flush keeps the
+        // unflushed bytes below ENC_FLUSH + 768 < ENC_RING
+        if (flushed - pos >= ENC_FLUSH) {
+            __syncthreads();
+            const uint32_t o = flushed - ENC_FLUSH + 16u * lane;
+            const uint4 v = *reinterpret_cast<const uint4*>(ring + (o & (ENC_RING - 1)));
+            *reinterpret_cast<uint4*>(stream + o) = v;
+            flushed -= ENC_FLUSH;
+            __syncthreads();
         }
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) cur[j] = nxt[j];
         t = tn;
         g = gn;
     }
+    __syncthreads();
+    for (uint32_t o = pos + lane; o < flushed; o += 64) stream[o] = ring[o & (ENC_RING - 1)];
     if (lane < L) put_u32(out + 4 * lane, x);
     if (lane == 0) {
-        chunk_len[chunk] = (uint32_t)(end - pos);
-        chunk_pos[chunk] = (uint32_t)pos;
+        chunk_len[chunk] = end - pos;
+        chunk_pos[chunk] = pos;
     }
+}
+
+template <typename ValT, int DIMS>
+__device__ inline void enc_chunk_any(const ValT* __restrict__ vals, uint32_t chunk, uint32_t si, const CompInst& C,
+                                     const Section& S, const uint64_t* __restrict__ gpc, EncLds& lds,
+                                     uint8_t* __restrict__ ring, uint8_t* __restrict__ scratch, uint64_t stride,
+                                     uint32_t* __restrict__ chunk_len, uint32_t* __restrict__ chunk_pos) {
+    if (S.alphabet <= 256)
+        enc_chunk<ValT, DIMS, true>(vals, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
+    else
+        enc_chunk<ValT, DIMS, false>(vals, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
 }
 
 // every component's chunks in one launch: block b codes chunk b of the
@@ -743,12 +794,14 @@ __global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__
     const CompInst& C = ci[si >> 1];
     const uint64_t* gpc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
     __shared__ EncLds lds;
+    // the ring, then one byte per lane for lanes without a byte to emit
+    __shared__ __attribute__((aligned(16))) uint8_t ring[ENC_RING + 64];
     switch (c) {
-    case 0: enc_chunk<uint32_t, 1>(idrel, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
-    case 1: enc_chunk<uint64_t, D>(diff, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
-    case 2: enc_chunk<uint64_t, D>(keep, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
-    case 3: enc_chunk<uint8_t, 1>(choice2, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
-    default: enc_chunk<RelT, D>(rel, chunk, si, C, S, gpc, lds, scratch, stride, chunk_len, chunk_pos); break;
+    case 0: enc_chunk_any<uint32_t, 1>(idrel, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
+    case 1: enc_chunk_any<uint64_t, D>(diff, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
+    case 2: enc_chunk_any<uint64_t, D>(keep, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
+    case 3: enc_chunk_any<uint8_t, 1>(choice2, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
+    default: enc_chunk_any<RelT, D>(rel, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     }
 }
 

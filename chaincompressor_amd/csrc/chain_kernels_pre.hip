@@ -379,21 +379,43 @@ __global__ __launch_bounds__(CH_WG) void k_corr_gather(const uint64_t* __restric
 
 // ------------------------------------------------------------------------
 // K3: stable LSD radix sort of (block key, point index), 8-bit digits.
-// Pass = tile histogram -> digit-major exclusive scan -> scatter. The scatter
-// ranks keys with wavefront ballots (8 ballots give each lane the mask of
-// lanes holding its digit), stages the tile in LDS in digit order and writes
-// contiguous runs per digit.
+// Pass = tile histogram -> digit-major exclusive scan -> scatter. Wavefront w
+// of a tile owns its positions [w * 1024, (w + 1) * 1024), item r of lane l
+// at w * 1024 + r * 64 + l, so ranking the wavefronts one after another, each
+// in item order, is the tile's order (the sort stays stable). The scatter
+// loads every item first, ranks them within the wavefront with ballots (8
+// ballots give each lane the mask of lanes holding its digit) and a per-
+// wavefront digit counter in LDS, adds the counts of the earlier wavefronts,
+// stages the tile in LDS in digit order and writes contiguous runs per digit.
 
+#define SORT_WAVE_ITEMS (CH_TILE / CH_WG)   // items per lane
 template <typename KeyT>
 __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ keys, uint64_t T, uint32_t shift,
                                                      uint32_t ntiles, uint32_t* __restrict__ counts) {
     __shared__ uint32_t s_h[256];
     s_h[threadIdx.x] = 0;
-    __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
+    uint32_t dg[CH_ITEMS];
+#pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * CH_WG + threadIdx.x;
-        if (i < T) atomicAdd(&s_h[(uint32_t)(keys[i] >> shift) & 0xffu], 1u);
+        dg[j] = i < T ? (uint32_t)(keys[i] >> shift) & 0xffu : 256u;
+    }
+    __syncthreads();
+    // one LDS add per distinct digit of a wavefront item: 8 ballots give each
+    // lane the mask of lanes holding its digit, the lowest of them adds
+#pragma unroll
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint32_t digit = dg[j];
+        uint64_t peers = __ballot(digit < 256u);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bb = __ballot((digit >> b) & 1u);
+            peers &= ((digit >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+        if (digit < 256u && below == 0) atomicAdd(&s_h[digit], (uint32_t)__popcll(peers));
     }
     __syncthreads();
     counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
@@ -405,57 +427,75 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
                                                         uint64_t T, uint32_t shift, uint32_t ntiles,
                                                         const uint32_t* __restrict__ counts,
                                                         const uint32_t* __restrict__ offsets) {
-    __shared__ uint32_t s_lstart[256], s_goff[256], s_run[256], s_tmp[4];
+    __shared__ uint32_t s_lstart[256], s_goff[256], s_tmp[4];
     __shared__ uint32_t s_wcnt[4][256];
     __shared__ KeyT s_key[CH_TILE];
     __shared__ uint32_t s_val[CH_TILE];
-    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
     const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
     const uint32_t cnt_tile = (uint32_t)(T - base < CH_TILE ? T - base : CH_TILE);
+    KeyT key[SORT_WAVE_ITEMS];
+    uint32_t val[SORT_WAVE_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SORT_WAVE_ITEMS; ++r) {
+        const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
+        const bool valid = p < cnt_tile;
+        key[r] = valid ? kin[base + p] : KeyT(0);
+        val[r] = FIRST ? (uint32_t)(base + p) : (valid ? vin[base + p] : 0u);
+    }
     {
         const uint32_t c = counts[(uint64_t)tid * ntiles + blockIdx.x];
         s_goff[tid] = offsets[(uint64_t)tid * ntiles + blockIdx.x];
         uint32_t tot;
         s_lstart[tid] = block_excl_scan(c, s_tmp, &tot);
-        s_run[tid] = 0;
         for (int k = 0; k < 4; ++k) s_wcnt[k][tid] = 0;
     }
     __syncthreads();
-    const uint64_t lt = lanemask_lt();
-    for (int r = 0; r < CH_ITEMS; ++r) {
-        const uint32_t pos = r * CH_WG + tid;
-        const bool valid = pos < cnt_tile;
-        KeyT key = 0;
-        uint32_t val = 0, digit = 0;
-        if (valid) {
-            key = kin[base + pos];
-            val = FIRST ? (uint32_t)(base + pos) : vin[base + pos];
-            digit = (uint32_t)(key >> shift) & 0xffu;
-        }
+    // rank of every item among its wavefront's items of the same digit
+    uint32_t wr[SORT_WAVE_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SORT_WAVE_ITEMS; ++r) {
+        const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
+        const bool valid = p < cnt_tile;
+        const uint32_t digit = (uint32_t)(key[r] >> shift) & 0xffu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             const uint64_t bb = __ballot((digit >> b) & 1u);
             peers &= ((digit >> b) & 1u) ? bb : ~bb;
         }
-        const uint32_t prank = __popcll(peers & lt);
-        if (valid && prank == 0) s_wcnt[w][digit] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t rank = s_run[digit] + prank;
-            for (uint32_t k = 0; k < w; ++k) rank += s_wcnt[k][digit];
-            const uint32_t lp = s_lstart[digit] + rank;
-            s_key[lp] = key;
-            s_val[lp] = val;
-        }
-        __syncthreads();
-        {
-            uint32_t a = 0;
-            for (int k = 0; k < 4; ++k) { a += s_wcnt[k][tid]; s_wcnt[k][tid] = 0; }
-            s_run[tid] += a;
-        }
-        __syncthreads();
+        const uint32_t prank = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+        uint32_t before = 0;
+        if (valid) before = s_wcnt[w][digit];
+        wr[r] = before + prank;
+        // lanes of one digit read the count before its leader updates it
+        __builtin_amdgcn_wave_barrier();
+        if (valid && prank == 0) s_wcnt[w][digit] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
     }
+    __syncthreads();
+    {
+        // digit tid: start of every wavefront's run in the tile
+        const uint32_t a0 = s_wcnt[0][tid], a1 = s_wcnt[1][tid], a2 = s_wcnt[2][tid];
+        const uint32_t st = s_lstart[tid];
+        s_wcnt[0][tid] = st;
+        s_wcnt[1][tid] = st + a0;
+        s_wcnt[2][tid] = st + a0 + a1;
+        s_wcnt[3][tid] = st + a0 + a1 + a2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_WAVE_ITEMS; ++r) {
+        const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
+        if (p < cnt_tile) {
+            const uint32_t digit = (uint32_t)(key[r] >> shift) & 0xffu;
+            const uint32_t lp = s_wcnt[w][digit] + wr[r];
+            s_key[lp] = key[r];
+            s_val[lp] = val[r];
+        }
+    }
+    __syncthreads();
     for (uint32_t p = tid; p < cnt_tile; p += CH_WG) {
         const KeyT k = s_key[p];
         const uint32_t d = (uint32_t)(k >> shift) & 0xffu;

@@ -1015,7 +1015,12 @@ __global__ __launch_bounds__(CH_WG) void k_copy_chunks(const CompInst* __restric
     for (uint32_t k = threadIdx.x; k < len; k += CH_WG) dst[nstate + k] = st[k];
 }
 
-// Fixed payloads: one thread per output byte, MSB-first, byte-padded per dimension
+// Fixed payloads, MSB-first, byte-padded per dimension. Eight consecutive
+// values of width w fill exactly w bytes, so thread g of a tile packs values
+// 8g .. 8g+7 of each dimension into bytes [g w, g w + w) of the tile's part
+// (a tile starts at a multiple of 4096 values, hence on a byte). Values past
+// the end are zero, which is the padding of the last byte.
+#define PACK_GROUPS (CH_TILE / 8)
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ vals, int c, uint32_t F,
                                                       const uint32_t* __restrict__ tile_begin,
@@ -1025,30 +1030,56 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
     const CompInst& C = ci[c * F + f];
     if (C.kind != KIND_FIXP) return;
     const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
-    if (r0 >= C.n) return;
-    const uint64_t r1 = r0 + CH_TILE < C.n ? r0 + CH_TILE : C.n;
+    const uint64_t n = C.n;
+    if (r0 >= n) return;
+    const uint64_t r1 = r0 + CH_TILE < n ? r0 + CH_TILE : n;
     const ValT* src = vals + C.base * DIMS;
     uint64_t doff = C.payload_off;
+#pragma unroll
     for (int d = 0; d < DIMS; ++d) {
         const uint32_t w = C.width[d];
         const uint64_t mn = C.dmin[d];
-        const uint64_t kb0 = (r0 * w + 7) / 8, kb1 = (r1 * w + 7) / 8;
-        for (uint64_t k = kb0 + threadIdx.x; k < kb1; k += CH_WG) {
-            // bits [8k, 8k+8): whole runs of the values they cover, MSB first
-            uint64_t r = (8 * k) / w;
-            uint32_t off = (uint32_t)(8 * k - r * w);   // bits of value r before this byte
-            uint32_t byte = 0, have = 0;
-            while (have < 8) {
-                if (r >= C.n) { byte <<= (8 - have); break; }
-                const uint64_t v = (uint64_t)src[r * DIMS + d] - mn;
-                const uint32_t avail = w - off;
-                const uint32_t take = avail < 8 - have ? avail : 8 - have;
-                byte = (byte << take) | (uint32_t)((v >> (avail - take)) & ((1u << take) - 1u));
-                have += take;
-                off = 0;
-                ++r;
+        ValT x[PACK_GROUPS / CH_WG][8];
+#pragma unroll
+        for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {
+            const uint64_t ra = r0 + 8ull * (h * CH_WG + threadIdx.x);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x[h][k] = ra + k < r1 ? src[(ra + k) * DIMS + d] : ValT(0);
+        }
+#pragma unroll
+        for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {
+            const uint64_t ra = r0 + 8ull * (h * CH_WG + threadIdx.x);
+            if (ra >= r1) continue;
+            const uint32_t nvals = r1 - ra < 8 ? (uint32_t)(r1 - ra) : 8u;
+            const uint32_t nbytes = (nvals * w + 7) / 8;
+            uint8_t* o = out + doff + (ra * w) / 8;
+            uint64_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = (uint32_t)k < nvals ? (uint64_t)x[h][k] - mn : 0ull;
+            if (w <= 8) {
+                uint64_t acc = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) acc = (acc << w) | v[k];
+                for (uint32_t b = 0; b < nbytes; ++b) o[b] = (uint8_t)(acc >> (8 * (w - 1 - b)));
+            } else {
+                // a 64-bit window, values pushed in pieces of at most 32 bits
+                uint64_t acc = 0;
+                uint32_t nb = 0, pos = 0;
+                for (int k = 0; k < 8; ++k) {
+                    uint32_t rem = w;
+                    while (rem) {
+                        const uint32_t take = rem < 32u ? rem : 32u;
+                        acc = (acc << take) | ((v[k] >> (rem - take)) & ((1ull << take) - 1ull));
+                        nb += take;
+                        rem -= take;
+                        while (nb >= 8) {
+                            nb -= 8;
+                            if (pos < nbytes) o[pos] = (uint8_t)(acc >> nb);
+                            ++pos;
+                        }
+                    }
+                }
             }
-            out[doff + k] = (uint8_t)byte;
         }
         doff += C.fix_bytes[d];
     }

@@ -194,10 +194,23 @@ __global__ __launch_bounds__(CH_WG) void k_bbox(const T* __restrict__ pts, const
 #pragma unroll
     for (int d = 0; d < D; ++d) { mn[d] = ~0ull; mx[d] = 0; zp[d] = ~0ull; zn[d] = ~0ull; }
     bool nonfinite = false;
-    for (uint64_t i = start + threadIdx.x; i < end; i += CH_WG) {
+    // every coordinate of the tile is loaded before the first is used
+    T xs[CH_ITEMS][D];
+#pragma unroll
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
+#pragma unroll
+        for (int d = 0; d < D; ++d) xs[j][d] = i < end ? pts[SOA ? F.pt_off * D + (uint64_t)d * F.n + i
+                                                                 : (F.pt_off + i) * D + d]
+                                                       : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
+        if (i >= end) continue;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const double x = load_coord<T, D, SOA>(pts, F, i, d);
+            const double x = (double)xs[j][d];
             nonfinite |= !isfinite(x);
             const uint64_t k = okey(x);
             mn[d] = k < mn[d] ? k : mn[d];
@@ -395,27 +408,10 @@ __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ ke
     __shared__ uint32_t s_h[256];
     s_h[threadIdx.x] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
-    uint32_t dg[CH_ITEMS];
-#pragma unroll
+    __syncthreads();
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * CH_WG + threadIdx.x;
-        dg[j] = i < T ? (uint32_t)(keys[i] >> shift) & 0xffu : 256u;
-    }
-    __syncthreads();
-    // one LDS add per distinct digit of a wavefront item: 8 ballots give each
-    // lane the mask of lanes holding its digit, the lowest of them adds
-#pragma unroll
-    for (int j = 0; j < CH_ITEMS; ++j) {
-        const uint32_t digit = dg[j];
-        uint64_t peers = __ballot(digit < 256u);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const uint64_t bb = __ballot((digit >> b) & 1u);
-            peers &= ((digit >> b) & 1u) ? bb : ~bb;
-        }
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
-        if (digit < 256u && below == 0) atomicAdd(&s_h[digit], (uint32_t)__popcll(peers));
+        if (i < T) atomicAdd(&s_h[(uint32_t)(keys[i] >> shift) & 0xffu], 1u);
     }
     __syncthreads();
     counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
@@ -637,10 +633,19 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
     uint32_t flags = 0, pre[CH_ITEMS];
+    // one-word keys of the tile are loaded before the look-back waits
+    KeyT kpre[CH_ITEMS];
+    uint32_t idr[CH_ITEMS];
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint64_t i = start + k * CH_WG + threadIdx.x;
-        const bool ex = i < F.n && idrel[F.pt_off + i] == 0;
+        const bool live = i < F.n;
+        idr[k] = live ? idrel[F.pt_off + i] : 1u;
+        kpre[k] = live && nwords == 1 ? keys[F.pt_off + i] : KeyT(0);
+    }
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const bool ex = idr[k] == 0;
         const uint64_t m = __ballot(ex);
         pre[k] = __popcll(m & lt);
         flags |= (ex ? 1u : 0u) << k;
@@ -671,7 +676,8 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
             for (int d = 0; d < D; ++d) {
                 const uint32_t wd = F.key_width[d];
                 const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
-                const uint64_t key = (uint64_t)keys[(uint64_t)(nwords == 1 ? 0 : F.key_word[d]) * T + F.pt_off + i];
+                const uint64_t key = nwords == 1 ? (uint64_t)kpre[k]
+                                                 : (uint64_t)keys[(uint64_t)F.key_word[d] * T + F.pt_off + i];
                 blocks[rec * D + d] = (key >> F.key_shift[d]) & mask;
             }
         }

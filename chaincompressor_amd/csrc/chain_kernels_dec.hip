@@ -986,6 +986,7 @@ __global__ void k_dec_check_counts(const DecFrame* __restrict__ fr, uint32_t F, 
 // Each pass over DQ_ITEMS points per thread loads every root first, then
 // every block and offset those roots name, then computes and stores.
 #define DQ_ITEMS 8
+#define DQ_STAGE 4
 template <int D, typename OutT, bool SOA, typename RelT>
 __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restrict__ fr, uint32_t F,
                                                        const uint32_t* __restrict__ tile_begin,
@@ -1003,6 +1004,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
 #pragma unroll
     for (int d = 0; d < D; ++d) mins[d] = Fr.mins[d];
     const uint64_t i0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+    __shared__ OutT s_out[SOA ? 1 : DQ_STAGE * CH_WG * D];
     uint32_t bad = 0;
     for (int h = 0; h < CH_ITEMS; h += DQ_ITEMS) {
         uint32_t rv[DQ_ITEMS];
@@ -1028,19 +1030,40 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
                 off[j][d] = good ? rel[(pt_off + i) * D + d] : RelT(0);
             }
         }
+        // interleaved output is staged in LDS, DQ_STAGE items (contiguous
+        // points) at a time, and written back element by element, so every
+        // store instruction covers one contiguous span
 #pragma unroll
-        for (int j = 0; j < DQ_ITEMS; ++j) {
-            if (!((ok >> j) & 1u)) continue;
-            const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
+        for (int hs = 0; hs < DQ_ITEMS; hs += DQ_STAGE) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const uint64_t o = off[j][d], b = blk[j][d];
-                // b * p + o must fit in int64
-                if (o >= p || __umul64hi(b, p) != 0 || b * p > (uint64_t)INT64_MAX - o) bad |= DERR_QUANT;
-                const int64_t q = (int64_t)(b * p + o);
-                const double x = fma((double)q + 0.5, step, mins[d]);
-                if (SOA) out[out_off + (uint64_t)d * n + i] = (OutT)x;
-                else out[out_off + i * D + d] = (OutT)x;
+            for (int jj = 0; jj < DQ_STAGE; ++jj) {
+                const int j = hs + jj;
+                const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
+                const bool good = (ok >> j) & 1u;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const uint64_t o = off[j][d], b = blk[j][d];
+                    // b * p + o must fit in int64
+                    if (good && (o >= p || __umul64hi(b, p) != 0 || b * p > (uint64_t)INT64_MAX - o))
+                        bad |= DERR_QUANT;
+                    const int64_t q = (int64_t)(b * p + o);
+                    const double x = fma((double)q + 0.5, step, mins[d]);
+                    if (SOA) {
+                        if (good) out[out_off + (uint64_t)d * n + i] = (OutT)x;
+                    } else {
+                        s_out[(jj * CH_WG + threadIdx.x) * D + d] = (OutT)x;
+                    }
+                }
+            }
+            if (!SOA) {
+                const uint64_t p0 = i0 + (uint64_t)(h + hs) * CH_WG;
+                __syncthreads();
+                if (p0 < n) {
+                    const uint32_t np = n - p0 < DQ_STAGE * CH_WG ? (uint32_t)(n - p0) : DQ_STAGE * CH_WG;
+                    OutT* dst = out + out_off + p0 * D;
+                    for (uint32_t e = threadIdx.x; e < np * D; e += CH_WG) dst[e] = s_out[e];
+                }
+                __syncthreads();
             }
         }
     }

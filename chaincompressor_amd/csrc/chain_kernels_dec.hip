@@ -664,34 +664,78 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
         incoming.reset = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) incoming.v[d] = 0;
-        if (lane == 0) {
-            const uint32_t seg = rtile_begin[f];
-            if (tile != seg) {
+        const uint32_t seg = rtile_begin[f];
+        if (tile != seg) {
+            if (lane == 0) {
                 for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + d] = tot.v[d];
                 __hip_atomic_store(&flags[tile], 1u | (tot.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                Agg acc;
-                acc.reset = 0;
-                for (int d = 0; d < D; ++d) acc.v[d] = 0;
-                int64_t t = (int64_t)tile - 1;
-                uint32_t spins = 0;
-                while (t >= (int64_t)seg) {
-                    const uint32_t fl = __hip_atomic_load(&flags[t], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    if ((fl & 0xff) == 0) {
-                        if (++spins > LB_SPIN_LIMIT) { atomicOr(err, DERR_WAIT); break; }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    Agg x;
-                    x.reset = fl >> 8;
-                    const int slot = (fl & 0xff) == 2 ? 1 : 0;
-                    for (int d = 0; d < D; ++d) x.v[d] = aggv[((uint64_t)t * 2 + slot) * CH_MAXD + d];
-                    agg_combine<D>(x, acc, R);   // x (+) acc
-                    acc = x;
-                    if ((fl & 0xff) == 2 || x.reset) break;
-                    --t;
-                }
-                incoming = acc;
             }
+            // look-back over 64 predecessors at a time: lane l reads tile
+            // t - l; the window up to the nearest inclusive (or resetting)
+            // aggregate is combined in order, farther tiles on the left
+            uint32_t ar = 0;
+            uint64_t av[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) av[d] = 0;
+            int64_t t = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            while (t >= (int64_t)seg) {
+                const int64_t mine = t - (int64_t)lane;
+                const bool beyond = mine < (int64_t)seg;
+                const uint32_t fl =
+                    beyond ? 2u : __hip_atomic_load(&flags[mine], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t st = fl & 0xffu;
+                const uint64_t not_ready = __ballot(!beyond && st == 0);
+                const uint64_t stop = __ballot(st == 2 || (fl >> 8) != 0);
+                const uint64_t first_stop = stop ? (stop & (~stop + 1)) : 0ull;
+                const uint64_t need = first_stop ? (first_stop | (first_stop - 1)) : ~0ull;
+                if (not_ready & need) {
+                    if (++spins > LB_SPIN_LIMIT) {
+                        if (lane == 0) atomicOr(err, DERR_WAIT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                uint32_t yr = 0;
+                uint64_t yv[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) yv[d] = 0;
+                if (((need >> lane) & 1ull) && !beyond) {
+                    yr = fl >> 8;
+                    const uint64_t slot = st == 2 ? 1 : 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) yv[d] = aggv[((uint64_t)mine * 2 + slot) * CH_MAXD + d];
+                }
+                for (uint32_t off = 1; off < 64; off <<= 1) {
+                    uint32_t orr = yr;
+                    uint64_t ov[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) ov[d] = yv[d];
+                    agg_shfl_down<D>(orr, ov, off);
+                    if (lane + off < 64 && (lane & (2 * off - 1)) == 0) {
+                        agg_comb<D>(orr, ov, yr, yv, R);   // farther (+) nearer
+                        yr = orr;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) yv[d] = ov[d];
+                    }
+                }
+                uint32_t wr = __shfl(yr, 0, 64);
+                uint64_t wv[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) wv[d] = __shfl(yv[d], 0, 64);
+                agg_comb<D>(wr, wv, ar, av, R);   // window (+) what was nearer
+                ar = wr;
+#pragma unroll
+                for (int d = 0; d < D; ++d) av[d] = wv[d];
+                if (first_stop) break;
+                t -= 64;
+            }
+            incoming.reset = ar;
+#pragma unroll
+            for (int d = 0; d < D; ++d) incoming.v[d] = av[d];
+        }
+        if (lane == 0) {
             Agg inc = incoming;
             agg_combine<D>(inc, tot, R);
             for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 1) * CH_MAXD + d] = inc.v[d];

@@ -557,15 +557,16 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     const uint64_t t0 = (uint64_t)(tile - rtile_begin[f]) * CH_TILE;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    // keep records (choice 0) before each record of the tile
-    uint32_t zpre[CH_ITEMS];
+    // keep records (choice 0) before each record of the tile, within its
+    // (row, wave) segment (in LDS: the passes index items by a loop variable)
+    __shared__ uint16_t s_zpre[CH_ITEMS][CH_WG];
     uint32_t zbits = 0;
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint64_t r = t0 + k * CH_WG + threadIdx.x;
         const bool z = r < Fr.nrec && choice[Fr.rec_base + r] == 0;
         const uint64_t m = __ballot(z);
-        zpre[k] = __popcll(m & lt);
+        s_zpre[k][threadIdx.x] = (uint16_t)__popcll(m & lt);
         zbits |= (z ? 1u : 0u) << k;
         if (lane == 0) s_zoff[k * W + wave] = __popcll(m);
     }
@@ -576,8 +577,21 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     }
     __syncthreads();
     const uint64_t kbase = tile_zero_off[tile];
+    // raw values of item k: the keep record or the diff symbol it reads
+    // (loaded for a group of items before any is used)
+    auto load_raw = [&](int k, uint64_t* raw) {
+        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+        const uint64_t kix = kbase + s_zoff[k * W + wave] + s_zpre[k][threadIdx.x];   // global keep index (or keeps before)
+        const bool z = (zbits >> k) & 1u;
+        const uint64_t idx = z ? kix : Fr.rec_base + r - kix;
+        const bool inb = r < Fr.nrec && (z ? kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep
+                                           : idx >= Fr.diff_base && idx < Fr.diff_base + Fr.ndiff);
+        const uint64_t* src = z ? keep : diff;
+#pragma unroll
+        for (int d = 0; d < D; ++d) raw[d] = inb ? src[idx * D + d] : (z ? (uint64_t)Fr.wrap_min : 0ull);
+    };
     // element of item k: keep -> (reset, value - wrap_min), diff -> (0, delta mod R)
-    auto element = [&](int k, bool check, uint64_t* v) -> uint32_t {
+    auto element = [&](int k, bool check, const uint64_t* raw, uint64_t* v) -> uint32_t {
         const uint64_t r = t0 + k * CH_WG + threadIdx.x;
         if (r >= Fr.nrec) {
 #pragma unroll
@@ -585,13 +599,13 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             return 0;   // identity
         }
         const uint64_t grec = Fr.rec_base + r;
-        const uint64_t kix = kbase + s_zoff[k * W + wave] + zpre[k];   // global keep index (or keeps before)
+        const uint64_t kix = kbase + s_zoff[k * W + wave] + s_zpre[k][threadIdx.x];
         if ((zbits >> k) & 1u) {
             const bool inb = kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep;
             if (check && !inb) atomicOr(err, DERR_L2);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const uint64_t kv = inb ? keep[kix * D + d] : (uint64_t)Fr.wrap_min;
+                const uint64_t kv = raw[d];
                 if (check && ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max)) atomicOr(err, DERR_L2);
                 v[d] = kv - (uint64_t)Fr.wrap_min;
             }
@@ -603,7 +617,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
         if (check && !inb) atomicOr(err, DERR_L2);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const uint64_t sym = inb ? diff[didx * D + d] : 0;
+            const uint64_t sym = raw[d];
             const int64_t corr = (sym & 1u) ? -(int64_t)(sym >> 1) - 1 : (int64_t)(sym >> 1);
             if (check && ((int64_t)sym < 0 || corr < Fr.min_corr || corr > Fr.max_corr)) atomicOr(err, DERR_L2);
             const uint64_t delta = corr > 0 ? R - (uint64_t)corr : (uint64_t)(-corr);
@@ -611,11 +625,18 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
         }
         return 0;
     };
+#define L2_GROUP 4
     // pass 1: ordered reduction of every (row, wave) segment
 #pragma unroll 1
-    for (int k = 0; k < CH_ITEMS; ++k) {
+    for (int kg = 0; kg < CH_ITEMS; kg += L2_GROUP) {
+      uint64_t raw[L2_GROUP][D];
+#pragma unroll
+      for (int j = 0; j < L2_GROUP; ++j) load_raw(kg + j, raw[j]);
+#pragma unroll
+      for (int j = 0; j < L2_GROUP; ++j) {
+        const int k = kg + j;
         uint64_t v[D];
-        uint32_t rs = element(k, true, v);
+        uint32_t rs = element(k, true, raw[j], v);
         for (uint32_t off = 1; off < 64; off <<= 1) {
             uint32_t orr = rs;
             uint64_t ov[D];
@@ -629,6 +650,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 #pragma unroll
             for (int d = 0; d < D; ++d) s_sv[k * W + wave][d] = v[d];
         }
+      }
     }
     __syncthreads();
     // wave 0: exclusive scan of the 64 segment aggregates, tile aggregate,
@@ -763,9 +785,15 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     __syncthreads();
     // pass 2: segment scan + prefix, output
 #pragma unroll 1
-    for (int k = 0; k < CH_ITEMS; ++k) {
+    for (int kg = 0; kg < CH_ITEMS; kg += L2_GROUP) {
+      uint64_t raw[L2_GROUP][D];
+#pragma unroll
+      for (int j = 0; j < L2_GROUP; ++j) load_raw(kg + j, raw[j]);
+#pragma unroll
+      for (int j = 0; j < L2_GROUP; ++j) {
+        const int k = kg + j;
         uint64_t v[D];
-        uint32_t rs = element(k, false, v);
+        uint32_t rs = element(k, false, raw[j], v);
         for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan in the wave
             uint32_t orr = rs;
             uint64_t ov[D];
@@ -789,6 +817,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 #pragma unroll
             for (int d = 0; d < D; ++d) compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + pv[d];
         }
+      }
     }
 }
 

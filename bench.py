@@ -57,9 +57,9 @@ REGION_KERNELS = {
     "l2_stats": "chain::k_l2_stats + k_l2_wrap",
     "l2_split": "chain::k_l2_split + k_frame_split_counts",
     "corrections": "chain::k_scan_u32 + k_corr_gather + k_frame_corr",
-    "comp_stats": "chain::k_comp_stats (one launch per component)",
+    "comp_stats": "chain::k_comp_stats + k_comp_reduce (one pair per component)",
     "rans_table": "chain::k_rans_table + k_rans_plan",
-    "rans_encode": "chain::k_rans_encode (one launch per component)",
+    "rans_encode": "chain::k_rans_encode (one launch, every component's chunks)",
     "assemble": "chain::k_section_sizes + k_choose + k_layout + k_write_frames + k_write_components + k_copy_chunks",
     "pack": "chain::k_pack_fixed + k_pack_tagged + k_tag_seams (per component)",
     "rans_decode": "chain::k_dec_rans",
@@ -70,7 +70,7 @@ REGION_KERNELS = {
     "l2_inverse": "chain::k_dec_l2",
     "chain_counts": "chain::k_dec_tile_zero_counts + k_scan_u32 + k_dec_check_counts",
     "chain_local": "chain::k_dec_chain_local",
-    "chain_jump": "chain::k_dec_chain_jump (ceil(log2 tiles)+1 launches)",
+    "chain_jump": "chain::k_dec_chain_jump (ceil(log2 tiles)+2 launches) + k_dec_chain_follow",
     "dequantize": "chain::k_dec_dequant + k_dec_corr",
 }
 
@@ -226,7 +226,12 @@ def run_plain(args, eng):
     def one_eb(eb, steps, warmup, timed_regions):
         # the engine's own tuner picks the entropy chunking for this frame;
         # repeated compressions of the same shape replay a captured HIP graph
-        (lanes, chunk), _ = eng.autotune([dptr], [n], np.float32, 3, eb, p, thr)
+        if args.rans_chunk:
+            lanes, chunk = 64, args.rans_chunk
+            eng.set_autotune(False)
+            eng.set_rans(lanes, chunk)
+        else:
+            (lanes, chunk), _ = eng.autotune([dptr], [n], np.float32, 3, eb, p, thr)
 
         def compress(graph=True):
             eng.set_graphs(graph)
@@ -283,11 +288,13 @@ def run_plain(args, eng):
         eng.device_free(b)
     main = one_eb(args.eb, args.steps, args.warmup, True)
     # parameter search on the device: p x threshold pairs of the same frame
-    t0 = time.perf_counter()
-    rows, best = eng.search(None, args.eb, [4, 8, 16, 32, 64], [0, 2, 4], device=(dptr, n, 3, np.float32))
-    search = {"pairs": len(rows), "seconds": round(time.perf_counter() - t0, 3),
-              "best": {"p": best[0], "threshold": best[1], "bits_per_particle": round(best[2] * 8 / n, 3)},
-              "p_values": [4, 8, 16, 32, 64], "thresholds": [0, 2, 4]}
+    search = None
+    if not args.no_search:
+        t0 = time.perf_counter()
+        rows, best = eng.search(None, args.eb, [4, 8, 16, 32, 64], [0, 2, 4], device=(dptr, n, 3, np.float32))
+        search = {"pairs": len(rows), "seconds": round(time.perf_counter() - t0, 3),
+                  "best": {"p": best[0], "threshold": best[1], "bits_per_particle": round(best[2] * 8 / n, 3)},
+                  "p_values": [4, 8, 16, 32, 64], "thresholds": [0, 2, 4]}
     sweep = {}
     for eb in (() if args.no_sweep else (1e-2, 1e-4)):
         r = one_eb(eb, 2, 1, False)
@@ -510,6 +517,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true", help="skip the eb = 1e-2 / 1e-4 runs of the plain frame")
+    ap.add_argument("--no-search", action="store_true", help="skip the (p, threshold) search of the plain frame")
+    ap.add_argument("--rans-chunk", type=int, default=0,
+                    help="symbols per rANS chunk for the plain frame instead of the tuner's choice (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 1 GiB device-to-device copy before the plain frame (profiler counter scale)")
     args = ap.parse_args()

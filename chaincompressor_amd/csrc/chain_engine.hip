@@ -963,8 +963,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             with_dims(D, [&](auto dd) {
                 constexpr int DD = decltype(dd)::value;
                 if (nwords == 1)
-                    hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st, kin, vin, T, A.fdev,
-                                       A.ptb, F, B.idrel);
+                    hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(LINK_XCDS * (unsigned)cdiv(cdiv(T, CH_TILE), LINK_XCDS)),
+                                       dim3(CH_WG), 0, st, kin, vin, T, A.fdev, A.ptb, F, B.idrel);
                 else
                     hipLaunchKernelGGL((k_link_wide<DD>), dim3(grid_for(T)), dim3(CH_WG), 0, st,
                                        (const uint64_t*)B.keys_in, vin, T, nwords, A.fdev, A.ptb, F, B.idrel);

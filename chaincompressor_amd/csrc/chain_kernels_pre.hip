@@ -506,6 +506,12 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
 // (cpp/chain/lcps_cpp.cpp:2155-2200), computed in sorted order and scattered
 // to index order.
 
+// Tiles of 4096 sorted positions, all loads of a tile issued first. The
+// workgroups of a launch are dispatched to the 8 XCDs in turn (workgroup b on
+// XCD b % 8), so workgroup b takes tile (b % 8) * per + b / 8: every XCD walks
+// a contiguous run of the sorted order, whose scattered link writes fall in a
+// smaller set of index ranges and merge better in that XCD's L2.
+#define LINK_XCDS 8u
 template <typename KeyT, int D>
 __global__ __launch_bounds__(CH_WG) void k_link(const KeyT* __restrict__ skey, const uint32_t* __restrict__ sval,
                                                 uint64_t T, const FrameDev* __restrict__ fr,
@@ -518,12 +524,31 @@ __global__ __launch_bounds__(CH_WG) void k_link(const KeyT* __restrict__ skey, c
         __syncthreads();
         offs = s_off;
     }
-    for (uint64_t s = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; s < T; s += (uint64_t)gridDim.x * CH_WG) {
-        const uint32_t i = sval[s];
-        const KeyT key = skey[s];
+    const uint32_t ntiles = (uint32_t)((T + CH_TILE - 1) / CH_TILE);
+    const uint32_t per = gridDim.x / LINK_XCDS;
+    const uint32_t tile = (blockIdx.x % LINK_XCDS) * per + blockIdx.x / LINK_XCDS;
+    if (tile >= ntiles) return;
+    const uint64_t s0 = (uint64_t)tile * CH_TILE;
+    uint32_t iv[CH_ITEMS], jv[CH_ITEMS];
+    KeyT kv[CH_ITEMS], kn[CH_ITEMS];
+#pragma unroll
+    for (int q = 0; q < CH_ITEMS; ++q) {
+        const uint64_t s = s0 + (uint64_t)q * CH_WG + threadIdx.x;
+        const bool live = s < T, next = s + 1 < T;
+        iv[q] = live ? sval[s] : 0u;
+        kv[q] = live ? skey[s] : KeyT(0);
+        jv[q] = next ? sval[s + 1] : 0u;
+        kn[q] = next ? skey[s + 1] : KeyT(0);
+    }
+#pragma unroll
+    for (int q = 0; q < CH_ITEMS; ++q) {
+        const uint64_t s = s0 + (uint64_t)q * CH_WG + threadIdx.x;
+        if (s >= T) continue;
+        const uint32_t i = iv[q];
+        const KeyT key = kv[q];
         uint32_t out = 0;
-        if (s + 1 < T && skey[s + 1] == key) {
-            const uint32_t j = sval[s + 1];
+        if (s + 1 < T && kn[q] == key) {
+            const uint32_t j = jv[q];
             const uint32_t f = upper_find(offs, nframes, i);
             const FrameDev& F = fr[f];
             if ((F.flags & FL_L1) && j < offs[f + 1]) {

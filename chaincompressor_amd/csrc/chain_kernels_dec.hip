@@ -92,12 +92,13 @@ struct DecJob {
 // to 20 bits decodes without global table reads: the symbol of a slot is a
 // bucket lookup, refined by binary search over the symbols inside the bucket
 // when pb > 12. Each wavefront stages its chunk's byte stream through its own
-// LDS ring of DEC_RING bytes, refilled in halves of 64 lanes x 32 bytes with
-// the next half already in registers.
-#define DEC_RING 4096u
-#define DEC_HALF 2048u
+// LDS ring of DEC_RING bytes, refilled in halves of 64 lanes x 16 bytes with
+// the next half already in registers. Ring and tables fit four workgroups in
+// a CU's LDS, and the register budget is held to four waves per SIMD.
+#define DEC_RING 2048u
+#define DEC_HALF 1024u
 #define DEC_BUCKETS 4096u
-__global__ __launch_bounds__(64 * DEC_WAVES) void k_dec_rans(const uint8_t* __restrict__ in,
+__global__ __launch_bounds__(64 * DEC_WAVES, 4) void k_dec_rans(const uint8_t* __restrict__ in,
                                                              const DecSection* __restrict__ secs,
                                                              const DecChunk* __restrict__ chunks,
                                                              const DecJob* __restrict__ jobs,
@@ -144,23 +145,27 @@ __global__ __launch_bounds__(64 * DEC_WAVES) void k_dec_rans(const uint8_t* __re
     const uint8_t* stream = p + 4 * L;
     const uint64_t stream_len = ch.bytes - 4ull * L;
     // ring staging: byte b of the stream lives at ring[b % DEC_RING]; this
-    // lane moves bytes from + 64*j + lane, j < 32, of every half
-    uint32_t pre[8];
+    // lane moves bytes from + 64*j + lane, j < DEC_HALF / 64, of every half
+    constexpr int PW = DEC_HALF / 256;   // staged words per lane
+    uint32_t pre[PW];
     auto fetch = [&](uint64_t from) {
 #pragma unroll
-        for (int w = 0; w < 8; ++w) {
+        for (int w = 0; w < PW; ++w) {
             uint32_t v = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint64_t b = from + (uint64_t)(4 * w + k) * 64 + lane;
-                v |= (uint32_t)(b < stream_len ? stream[b] : 0u) << (8 * k);
+                // unconditional load (past the stream: the chunk's first state
+                // byte) so that the half's loads are all in flight together
+                const uint32_t by = *(b < stream_len ? stream + b : p);
+                v |= (b < stream_len ? by : 0u) << (8 * k);
             }
             pre[w] = v;
         }
     };
     auto commit = [&](uint64_t from) {
 #pragma unroll
-        for (int w = 0; w < 8; ++w)
+        for (int w = 0; w < PW; ++w)
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 ring[(uint32_t)(from + (uint64_t)(4 * w + k) * 64 + lane) & (DEC_RING - 1)] = (uint8_t)(pre[w] >> (8 * k));
@@ -265,12 +270,12 @@ __device__ __forceinline__ void fixed_tile(const uint8_t* __restrict__ in, const
             const uint64_t a4 = (J.byte_off[d] + ((r * w) >> 3)) & ~3ull;
             const bool live = r < J.n;
             const bool fast = live && narrow && a4 + 8 <= J.end[d];
-            q0[d][j] = q1[d][j] = 0;
-            if (fast) {
-                const uint32_t* q = reinterpret_cast<const uint32_t*>(in + a4);
-                q0[d][j] = q[0];
-                q1[d][j] = q[1];
-            }
+            // unconditional loads (a lane that does not use them reads the
+            // payload's last aligned word pair, which a component header
+            // always precedes) so that every load of the tile is in flight
+            const uint32_t* q = reinterpret_cast<const uint32_t*>(in + (fast ? a4 : ((J.end[d] - 8) & ~3ull)));
+            q0[d][j] = q[0];
+            q1[d][j] = q[1];
             if (live && !fast) slow |= 1u << (d * DEC_FIX_ITEMS + j);
         }
     }
@@ -561,10 +566,16 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     // (row, wave) segment (in LDS: the passes index items by a loop variable)
     __shared__ uint16_t s_zpre[CH_ITEMS][CH_WG];
     uint32_t zbits = 0;
+    uint8_t chv[CH_ITEMS];
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {   // loads first (a tile holds at least one record)
+        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+        chv[k] = choice[Fr.rec_base + (r < Fr.nrec ? r : t0)];
+    }
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        const bool z = r < Fr.nrec && choice[Fr.rec_base + r] == 0;
+        const bool z = r < Fr.nrec && chv[k] == 0;
         const uint64_t m = __ballot(z);
         s_zpre[k][threadIdx.x] = (uint16_t)__popcll(m & lt);
         zbits |= (z ? 1u : 0u) << k;
@@ -858,10 +869,17 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
     // 256 points are read, ranked and written together. Explicit ranks in
     // index order: ballot prefix inside the wave + scanned (row, wave) counts.
     uint32_t dv[CH_ITEMS], pre[CH_ITEMS];
+    // every load first (unconditional: a tile holds at least one point), then
+    // the ballots
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint32_t j = k * CH_WG + threadIdx.x;
-        dv[k] = j < cnt ? src[j] : 1u;
+        dv[k] = src[j < cnt ? j : 0u];
+    }
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint32_t j = k * CH_WG + threadIdx.x;
+        dv[k] = j < cnt ? dv[k] : 1u;
         const uint64_t m = __ballot(j < cnt && dv[k] == 0);
         pre[k] = __popcll(m & lt);
         if (lane == 0) s_off[k * W + wave] = __popcll(m);

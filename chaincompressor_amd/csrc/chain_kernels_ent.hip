@@ -321,7 +321,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         for (int j = 0; j < CH_ITEMS; ++j) {
             const uint64_t r = r0 + (uint64_t)j * CH_WG + tid;
 #pragma unroll
-            for (int d = 0; d < DIMS; ++d) x[j][d] = r < n ? src[r * DIMS + d] : ValT(0);
+            for (int d = 0; d < DIMS; ++d) x[j][d] = src[(r < n ? r : r0) * DIMS + d];   // unconditional
         }
 #pragma unroll
         for (int j = 0; j < CH_ITEMS; ++j) {
@@ -1044,7 +1044,7 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
         for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {
             const uint64_t ra = r0 + 8ull * (h * CH_WG + threadIdx.x);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) x[h][k] = ra + k < r1 ? src[(ra + k) * DIMS + d] : ValT(0);
+            for (int k = 0; k < 8; ++k) x[h][k] = src[(ra + k < r1 ? ra + k : r0) * DIMS + d];   // unconditional
         }
 #pragma unroll
         for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {

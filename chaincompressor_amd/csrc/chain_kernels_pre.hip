@@ -200,9 +200,11 @@ __global__ __launch_bounds__(CH_WG) void k_bbox(const T* __restrict__ pts, const
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
 #pragma unroll
-        for (int d = 0; d < D; ++d) xs[j][d] = i < end ? pts[SOA ? F.pt_off * D + (uint64_t)d * F.n + i
-                                                                 : (F.pt_off + i) * D + d]
-                                                       : T(0);
+        for (int d = 0; d < D; ++d) {
+            // unconditional load at a valid point (the tile's first)
+            const uint64_t ic = i < end ? i : start;
+            xs[j][d] = pts[SOA ? F.pt_off * D + (uint64_t)d * F.n + ic : (F.pt_off + ic) * D + d];
+        }
     }
 #pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
@@ -665,8 +667,12 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
     for (int k = 0; k < CH_ITEMS; ++k) {
         const uint64_t i = start + k * CH_WG + threadIdx.x;
         const bool live = i < F.n;
-        idr[k] = live ? idrel[F.pt_off + i] : 1u;
-        kpre[k] = live && nwords == 1 ? keys[F.pt_off + i] : KeyT(0);
+        // unconditional loads at a valid index (the tile's first point)
+        const uint64_t ic = F.pt_off + (live ? i : start);
+        const uint32_t v = idrel[ic];
+        const KeyT kk = keys[ic];
+        idr[k] = live ? v : 1u;
+        kpre[k] = live && nwords == 1 ? kk : KeyT(0);
     }
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {

@@ -410,10 +410,17 @@ __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ ke
     __shared__ uint32_t s_h[256];
     s_h[threadIdx.x] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
+    KeyT kx[CH_ITEMS];
+#pragma unroll
+    for (int j = 0; j < CH_ITEMS; ++j) {   // unconditional loads (the tile's first key past the end)
+        const uint64_t i = base + (uint64_t)j * CH_WG + threadIdx.x;
+        kx[j] = keys[i < T ? i : base];
+    }
     __syncthreads();
+#pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * CH_WG + threadIdx.x;
-        if (i < T) atomicAdd(&s_h[(uint32_t)(keys[i] >> shift) & 0xffu], 1u);
+        if (i < T) atomicAdd(&s_h[(uint32_t)(kx[j] >> shift) & 0xffu], 1u);
     }
     __syncthreads();
     counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
@@ -536,11 +543,12 @@ __global__ __launch_bounds__(CH_WG) void k_link(const KeyT* __restrict__ skey, c
 #pragma unroll
     for (int q = 0; q < CH_ITEMS; ++q) {
         const uint64_t s = s0 + (uint64_t)q * CH_WG + threadIdx.x;
-        const bool live = s < T, next = s + 1 < T;
-        iv[q] = live ? sval[s] : 0u;
-        kv[q] = live ? skey[s] : KeyT(0);
-        jv[q] = next ? sval[s + 1] : 0u;
-        kn[q] = next ? skey[s + 1] : KeyT(0);
+        // unconditional loads at valid positions (the tile's first one)
+        const uint64_t sc = s < T ? s : s0, nc = s + 1 < T ? s + 1 : s0;
+        iv[q] = sval[sc];
+        kv[q] = skey[sc];
+        jv[q] = sval[nc];
+        kn[q] = skey[nc];
     }
 #pragma unroll
     for (int q = 0; q < CH_ITEMS; ++q) {
@@ -861,6 +869,8 @@ __global__ void k_l2_wrap(FrameOut* fo, uint32_t nframes, uint32_t* err) {
 // Striped tiles of 4096 records (item k of thread t = record k*256 + t):
 // choices by ballot, ordered positions from (row, wave) counts and a
 // decoupled look-back over tiles; coalesced loads and stores.
+#define L2S_GROUP 4
+#define L2S_MAXF_LDS 1024u   // frame bases kept in LDS up to this many frames
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
@@ -870,11 +880,11 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
                                                     uint8_t* __restrict__ choice2, uint64_t* __restrict__ diff,
                                                     uint64_t* __restrict__ keep, uint64_t* __restrict__ diff_total) {
     constexpr uint32_t W = CH_WG / 64;
-    __shared__ uint64_t s_base[CH_MAXF_LDS];
+    __shared__ uint64_t s_base[L2S_MAXF_LDS];
     __shared__ uint32_t s_tile, s_off[CH_ITEMS * W];
     __shared__ uint64_t s_excl;
     const uint64_t R = *rec_total;
-    const bool lds = nframes <= CH_MAXF_LDS;
+    const bool lds = nframes <= L2S_MAXF_LDS;
     if (lds)
         for (uint32_t k = threadIdx.x; k < nframes; k += CH_WG) s_base[k] = frame_rec_base[k];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
@@ -887,37 +897,59 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
     const uint64_t t0 = (uint64_t)tile * CH_TILE;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    uint32_t chosen = 0, pre[CH_ITEMS], fk[CH_ITEMS];
+    // Items are taken L2S_GROUP at a time: the blocks of the group's records
+    // and of their predecessors are loaded unconditionally (clamped to the
+    // tile's first record) before any is used.
+    // per-item ballot prefix and frame live in LDS: the groups are a loop
+    __shared__ uint16_t s_pre[CH_ITEMS][CH_WG];
+    __shared__ uint32_t s_fk[CH_ITEMS][CH_WG];
+    uint32_t chosen = 0;
+#pragma unroll 1
+    for (int kg = 0; kg < CH_ITEMS; kg += L2S_GROUP) {
+        uint64_t cur[L2S_GROUP][D], prv[L2S_GROUP][D];
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        bool c = false;
-        fk[k] = 0;
-        if (r < R) {
-            const uint32_t f = frame_of_rec(bases, nframes, r);
-            fk[k] = f;
-            if (r != bases[f] && (fr[f].flags & FL_L2)) {
-                const FrameOut& o = fo[f];
-                uint64_t mv = 0, mdf = 0;
+        for (int j = 0; j < L2S_GROUP; ++j) {
+            const uint64_t r = t0 + (kg + j) * CH_WG + threadIdx.x;
+            const uint64_t rc = r < R ? r : t0, rp = rc ? rc - 1 : rc;
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const uint64_t v = blocks[r * D + d], u = blocks[(r - 1) * D + d];
-                    mv = v > mv ? v : mv;
-                    const uint64_t ad = u > v ? u - v : v - u;
-                    mdf = ad > mdf ? ad : mdf;
-                }
-                const uint64_t tvw = bitlen64(mv), tdw = bitlen64(mdf);
-                const uint64_t tv = (uint64_t)D * tvw + bitlen64(tvw);
-                const uint64_t td = (uint64_t)D * tdw + bitlen64(tdw);
-                const uint64_t value_bits = tv < o.fixed_value_bits ? tv : o.fixed_value_bits;
-                const uint64_t diff_bits = td > o.fixed_diff_bits ? td : o.fixed_diff_bits;
-                c = value_bits >= diff_bits;
+            for (int d = 0; d < D; ++d) {
+                cur[j][d] = blocks[rc * D + d];
+                prv[j][d] = blocks[rp * D + d];
             }
         }
-        const uint64_t m = __ballot(c);
-        pre[k] = __popcll(m & lt);
-        chosen |= (c ? 1u : 0u) << k;
-        if (lane == 0) s_off[k * W + wave] = __popcll(m);
+#pragma unroll
+        for (int j = 0; j < L2S_GROUP; ++j) {
+            const int k = kg + j;
+            const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+            bool c = false;
+            uint32_t fkk = 0;
+            if (r < R) {
+                const uint32_t f = frame_of_rec(bases, nframes, r);
+                fkk = f;
+                if (r != bases[f] && (fr[f].flags & FL_L2)) {
+                    const FrameOut& o = fo[f];
+                    uint64_t mv = 0, mdf = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const uint64_t v = cur[j][d], u = prv[j][d];
+                        mv = v > mv ? v : mv;
+                        const uint64_t ad = u > v ? u - v : v - u;
+                        mdf = ad > mdf ? ad : mdf;
+                    }
+                    const uint64_t tvw = bitlen64(mv), tdw = bitlen64(mdf);
+                    const uint64_t tv = (uint64_t)D * tvw + bitlen64(tvw);
+                    const uint64_t td = (uint64_t)D * tdw + bitlen64(tdw);
+                    const uint64_t value_bits = tv < o.fixed_value_bits ? tv : o.fixed_value_bits;
+                    const uint64_t diff_bits = td > o.fixed_diff_bits ? td : o.fixed_diff_bits;
+                    c = value_bits >= diff_bits;
+                }
+            }
+            const uint64_t m = __ballot(c);
+            s_pre[k][threadIdx.x] = (uint16_t)__popcll(m & lt);
+            s_fk[k][threadIdx.x] = fkk;
+            chosen |= (c ? 1u : 0u) << k;
+            if (lane == 0) s_off[k * W + wave] = __popcll(m);
+        }
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -929,34 +961,49 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
+#pragma unroll 1
+    for (int kg = 0; kg < CH_ITEMS; kg += L2S_GROUP) {
+        uint64_t cur[L2S_GROUP][D], prv[L2S_GROUP][D];
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        if (r >= R) continue;
-        const uint32_t f = fk[k];
-        const uint64_t dpos = s_excl + s_off[k * W + wave] + pre[k];   // diffs before record r
-        if (r == bases[f]) {
-            fo[f].diff_base = dpos;
-            fo[f].keep_base = r - dpos;
-        }
-        const bool c = (chosen >> k) & 1u;
-        choice2[r] = (uint8_t)c;
-        if (!c) {
-            const uint64_t kpos = r - dpos;
-#pragma unroll
-            for (int d = 0; d < D; ++d) keep[kpos * D + d] = blocks[r * D + d];
-        } else {
-            const FrameOut& o = fo[f];
+        for (int j = 0; j < L2S_GROUP; ++j) {
+            const uint64_t r = t0 + (kg + j) * CH_WG + threadIdx.x;
+            const uint64_t rc = r < R ? r : t0, rp = rc ? rc - 1 : rc;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                // prediction = previous record clamped to [wrap_min, wrap_max] (an identity here)
-                int64_t corr = (int64_t)blocks[(r - 1) * D + d] - (int64_t)blocks[r * D + d];
-                if (corr < o.wrap_min_corr) corr += o.wrap_range;
-                else if (corr > o.wrap_max_corr) corr -= o.wrap_range;
-                diff[dpos * D + d] = corr >= 0 ? ((uint64_t)corr << 1) : (((uint64_t)(-(corr + 1)) << 1) | 1ull);
+                cur[j][d] = blocks[rc * D + d];
+                prv[j][d] = blocks[rp * D + d];
             }
         }
-        if (r == R - 1) *diff_total = dpos + (c ? 1u : 0u);
+#pragma unroll
+        for (int j = 0; j < L2S_GROUP; ++j) {
+            const int k = kg + j;
+            const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+            if (r >= R) continue;
+            const uint32_t f = s_fk[k][threadIdx.x];
+            const uint64_t dpos = s_excl + s_off[k * W + wave] + s_pre[k][threadIdx.x];   // diffs before record r
+            if (r == bases[f]) {
+                fo[f].diff_base = dpos;
+                fo[f].keep_base = r - dpos;
+            }
+            const bool c = (chosen >> k) & 1u;
+            choice2[r] = (uint8_t)c;
+            if (!c) {
+                const uint64_t kpos = r - dpos;
+#pragma unroll
+                for (int d = 0; d < D; ++d) keep[kpos * D + d] = cur[j][d];
+            } else {
+                const FrameOut& o = fo[f];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    // prediction = previous record clamped to [wrap_min, wrap_max] (an identity here)
+                    int64_t corr = (int64_t)prv[j][d] - (int64_t)cur[j][d];
+                    if (corr < o.wrap_min_corr) corr += o.wrap_range;
+                    else if (corr > o.wrap_max_corr) corr -= o.wrap_range;
+                    diff[dpos * D + d] = corr >= 0 ? ((uint64_t)corr << 1) : (((uint64_t)(-(corr + 1)) << 1) | 1ull);
+                }
+            }
+            if (r == R - 1) *diff_total = dpos + (c ? 1u : 0u);
+        }
     }
 }
 

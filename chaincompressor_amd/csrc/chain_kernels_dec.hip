@@ -1013,21 +1013,27 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
     for (uint32_t t0 = blockIdx.x * CH_TILE; t0 < n; t0 += gridDim.x * CH_TILE) {
-        uint32_t idx[CH_ITEMS], pre[CH_ITEMS];
+        uint32_t idx[CH_ITEMS], pre[CH_ITEMS], nvs[CH_ITEMS];
         uint32_t open_bits = 0;
+        // the three dependent loads of every item, each level for all items
+        // at once (past the list's end: entry t0, which exists)
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const uint32_t kk = t0 + k * CH_WG + threadIdx.x;
+            idx[k] = list_in[kk < n ? kk : t0];
+        }
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) nvs[k] = res[idx[k]];
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k)
+            nvs[k] = __hip_atomic_load(&res[nvs[k] & ~CH_EXT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int k = 0; k < CH_ITEMS; ++k) {
             const uint32_t kk = t0 + k * CH_WG + threadIdx.x;
             bool open = false;
-            idx[k] = 0;
             if (kk < n) {
-                const uint32_t i = list_in[kk];
-                idx[k] = i;
-                const uint32_t v = res[i];
-                const uint32_t nv =
-                    __hip_atomic_load(&res[v & ~CH_EXT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                res[i] = nv;
-                open = (nv & CH_EXT) != 0;
+                res[idx[k]] = nvs[k];
+                open = (nvs[k] & CH_EXT) != 0;
             }
             const uint64_t m = __ballot(open);
             pre[k] = __popcll(m & lt);
@@ -1054,10 +1060,21 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__
 }
 
 // followers of tile exits take their exit's (resolved) value
+// (CH_TILE points per workgroup, both load levels issued for all items)
 __global__ __launch_bounds__(CH_WG) void k_dec_chain_follow(uint32_t* __restrict__ res, uint64_t T) {
-    for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < T; g += (uint64_t)gridDim.x * CH_WG) {
-        const uint32_t v = res[g];
-        if (v & CH_EXT) res[g] = res[v & ~CH_EXT];
+    const uint64_t g0 = (uint64_t)blockIdx.x * CH_TILE;
+    uint32_t v[CH_ITEMS], t[CH_ITEMS];
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t g = g0 + (uint64_t)k * CH_WG + threadIdx.x;
+        v[k] = res[g < T ? g : g0];
+    }
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) t[k] = (v[k] & CH_EXT) ? res[v[k] & ~CH_EXT] : 0u;
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) {
+        const uint64_t g = g0 + (uint64_t)k * CH_WG + threadIdx.x;
+        if (g < T && (v[k] & CH_EXT)) res[g] = t[k];
     }
 }
 

@@ -1577,7 +1577,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.keep = c.take<uint64_t>(P.NK * D);
         x.choice = c.take<uint8_t>(P.R);
         x.tags = c.take<uint8_t>(P.tags);
-        x.compact = c.take<uint64_t>(P.R * D);
+        x.compact = c.take<uint64_t>((P.R + 1) * D);   // one spare record: unconditional gathers
         x.root = c.take<uint32_t>(T);
         x.zc = c.take<uint32_t>(nrt + 1);
         x.zoff = c.take<uint32_t>(nrt + 1);

@@ -1119,7 +1119,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
 #pragma unroll
         for (int j = 0; j < DQ_ITEMS; ++j) {
             const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
-            rv[j] = i < n ? root[pt_off + i] : 0u;
+            rv[j] = root[pt_off + (i < n ? i : i0)];   // unconditional (the tile's first point)
         }
         uint64_t blk[DQ_ITEMS][D];
         RelT off[DQ_ITEMS][D];
@@ -1132,10 +1132,13 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
             const bool good = live && !(rv[j] & CH_EXT) && rec - rec_base < nrec;
             if (live && !good) bad |= DERR_CHAIN;
             ok |= good ? 1u << j : 0u;
+            // unconditional loads: the frame's first record (the record
+            // array holds one spare) and the tile's first point otherwise
+            const uint64_t rc = good ? rec : rec_base, ic = live ? i : i0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                blk[j][d] = good ? compact[rec * D + d] : 0ull;
-                off[j][d] = good ? rel[(pt_off + i) * D + d] : RelT(0);
+                blk[j][d] = compact[rc * D + d];
+                off[j][d] = rel[(pt_off + ic) * D + d];
             }
         }
         // interleaved output is staged in LDS, DQ_STAGE items (contiguous

@@ -1573,8 +1573,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.cum = c.take<uint32_t>(P.cum.size());
         x.idrel = c.take<uint32_t>(T);
         x.rel = c.take<uint8_t>(T * D * (rel8 ? 1 : 4));
-        x.diff = c.take<uint64_t>(P.ND * D);
-        x.keep = c.take<uint64_t>(P.NK * D);
+        x.diff = c.take<uint64_t>((P.ND + 1) * D);   // one spare record each: unconditional loads
+        x.keep = c.take<uint64_t>((P.NK + 1) * D);
         x.choice = c.take<uint8_t>(P.R);
         x.tags = c.take<uint8_t>(P.tags);
         x.compact = c.take<uint64_t>((P.R + 1) * D);   // one spare record: unconditional gathers

@@ -470,9 +470,18 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tile_zero_counts(const uint8_t* _
     const uint32_t f = upper_find(tile_begin, F, tile);
     const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
     uint32_t c = 0;
+    const uint64_t nf = seg_n[f], bf = seg_base[f];
+    uint32_t xv[CH_ITEMS];
+#pragma unroll
+    for (int j = 0; j < CH_ITEMS; ++j) {   // unconditional loads (the tile's first entry past the end)
+        const uint64_t r = r0 + (uint64_t)j * CH_WG + threadIdx.x;
+        const uint64_t rc = bf + (r < nf ? r : r0);
+        xv[j] = v8 ? (uint32_t)v8[rc] : v32[rc];
+    }
+#pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t r = r0 + (uint64_t)j * CH_WG + threadIdx.x;
-        if (r < seg_n[f]) c += v8 ? (v8[seg_base[f] + r] == 0) : (v32[seg_base[f] + r] == 0);
+        c += r < nf && xv[j] == 0;
     }
     __shared__ uint32_t s_tmp[4];
     uint32_t tot;
@@ -598,8 +607,13 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
         const bool inb = r < Fr.nrec && (z ? kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep
                                            : idx >= Fr.diff_base && idx < Fr.diff_base + Fr.ndiff);
         const uint64_t* src = z ? keep : diff;
+        // unconditional load (record 0 of the array, which always exists)
+        const uint64_t ic = inb ? idx : 0ull;
 #pragma unroll
-        for (int d = 0; d < D; ++d) raw[d] = inb ? src[idx * D + d] : (z ? (uint64_t)Fr.wrap_min : 0ull);
+        for (int d = 0; d < D; ++d) {
+            const uint64_t v = src[ic * D + d];
+            raw[d] = inb ? v : (z ? (uint64_t)Fr.wrap_min : 0ull);
+        }
     };
     // element of item k: keep -> (reset, value - wrap_min), diff -> (0, delta mod R)
     auto element = [&](int k, bool check, const uint64_t* raw, uint64_t* v) -> uint32_t {

@@ -1111,15 +1111,23 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * TAG_TILE;
     const ValT* src = vals + C.base * DIMS;
     const uint64_t rt = r0 + threadIdx.x * 4;
+    // the thread's 4 records, loaded once and unconditionally (the tile's
+    // first record past the end)
+    uint64_t xv[4][DIMS];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) xv[k][d] = (uint64_t)src[(rt + k < C.n ? rt + k : r0) * DIMS + d];
     uint32_t tg[4];
     uint32_t bits = 0;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
         tg[k] = 0;
         if (rt + k < C.n) {
             uint32_t t = 1;
 #pragma unroll
             for (int d = 0; d < DIMS; ++d) {
-                const uint32_t b = (uint32_t)bitlen64((uint64_t)src[(rt + k) * DIMS + d]);
+                const uint32_t b = (uint32_t)bitlen64(xv[k][d]);
                 t = b > t ? b : t;
             }
             tg[k] = t;
@@ -1138,12 +1146,13 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     const uint64_t b0 = s_excl, b1 = b0 + tot;
     const uint32_t lb0 = (uint32_t)(b0 & 7);
     uint32_t o = lb0 + pre;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (!tg[k]) continue;
         const uint32_t w = tg[k];
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
-            const uint64_t v = (uint64_t)src[(rt + k) * DIMS + d];
+            const uint64_t v = xv[k][d];
             const uint32_t fb = o >> 3, lb = (o + w - 1) >> 3;
             for (uint32_t B = fb; B <= lb; ++B) {
                 const uint32_t sb = o > 8 * B ? o : 8 * B;

@@ -444,9 +444,9 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
 #pragma unroll
     for (int r = 0; r < SORT_WAVE_ITEMS; ++r) {
         const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
-        const bool valid = p < cnt_tile;
-        key[r] = valid ? kin[base + p] : KeyT(0);
-        val[r] = FIRST ? (uint32_t)(base + p) : (valid ? vin[base + p] : 0u);
+        const uint64_t pc = base + (p < cnt_tile ? p : 0u);   // unconditional (the tile's first key)
+        key[r] = kin[pc];
+        val[r] = FIRST ? (uint32_t)(base + p) : vin[pc];
     }
     {
         const uint32_t c = counts[(uint64_t)tid * ntiles + blockIdx.x];

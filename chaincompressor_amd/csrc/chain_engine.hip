@@ -1413,7 +1413,7 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
         j.dims = dims;
         j.tile0 = (uint32_t)P.tag_tiles;
         j.comp = comp;
-        P.tag_tiles += cdiv(n, CH_TILE);
+        P.tag_tiles += cdiv(n, DEC_TAG_TILE);
         P.tagged.push_back(j);
     }
 }

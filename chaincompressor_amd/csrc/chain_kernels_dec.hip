@@ -389,7 +389,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_raw(const uint8_t* __restrict__ i
     }
 }
 
-// Tagged payload job (one component of one frame): tiles of 4096 records
+// Tagged payload job (one component of one frame), decoded in tiles of DEC_TAG_TILE records
 struct DecTagged {
     uint64_t byte_off;          // packed values in the input pool
     uint64_t bytes;             // packed bytes
@@ -401,6 +401,13 @@ struct DecTagged {
     uint32_t pad;
 };
 
+// Tiles of DEC_TAG_TILE records, DEC_TAG_ITEMS consecutive records per
+// thread (their bit offsets are one block scan). After the look-back the
+// tile's payload bytes are staged in LDS with batched loads, and every value
+// is cut from LDS.
+#define DEC_TAG_TILE 1024u
+#define DEC_TAG_ITEMS (DEC_TAG_TILE / CH_WG)
+#define DEC_TAG_LDS (DEC_TAG_TILE * CH_MAXD * 63u / 8u + 16u)   // widest tile's bytes
 __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict__ in,
                                                       const DecTagged* __restrict__ jobs, uint32_t njobs,
                                                       const uint8_t* __restrict__ tags, uint64_t* status,
@@ -409,6 +416,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
                                                       uint32_t* err) {
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
+    __shared__ uint8_t s_pay[DEC_TAG_LDS];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -418,12 +426,22 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
         if (jobs[mid].tile0 <= tile) lo = mid; else hi = mid;
     }
     const DecTagged J = jobs[lo];
-    const uint64_t r0 = (uint64_t)(tile - J.tile0) * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
-    uint32_t tg[CH_ITEMS];
+    const uint64_t rfirst = (uint64_t)(tile - J.tile0) * DEC_TAG_TILE;   // < J.n
+    const uint64_t r0 = rfirst + (uint64_t)threadIdx.x * DEC_TAG_ITEMS;
+    uint32_t tg[DEC_TAG_ITEMS];
+#pragma unroll
+    for (int k = 0; k < DEC_TAG_ITEMS; ++k) tg[k] = tags[J.tag_off + (r0 + k < J.n ? r0 + k : rfirst)];
     uint32_t bits = 0;
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        tg[k] = r0 + k < J.n ? tags[J.tag_off + r0 + k] : 0u;
-        if (r0 + k < J.n && (tg[k] < 1 || tg[k] > 63)) atomicOr(err, DERR_VALUE);
+#pragma unroll
+    for (int k = 0; k < DEC_TAG_ITEMS; ++k) {
+        if (r0 + k >= J.n) {
+            tg[k] = 0;
+            continue;
+        }
+        if (tg[k] < 1 || tg[k] > 63) {
+            atomicOr(err, DERR_VALUE);
+            tg[k] = 1;   // keep the work bounded; the stream is rejected
+        }
         bits += tg[k] * J.dims;
     }
     uint32_t tot;
@@ -433,24 +451,46 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
         if (threadIdx.x == 0) s_excl = e;
     }
     __syncthreads();
-    uint64_t bit = s_excl + pre;
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        if (r0 + k >= J.n) break;
+    const uint64_t bit0 = s_excl;
+    const uint64_t B0 = bit0 >> 3;
+    const uint64_t want = ((bit0 + tot + 7) >> 3) - B0;   // <= DEC_TAG_LDS
+    const uint64_t have = J.bytes > B0 ? J.bytes - B0 : 0;
+    const uint32_t nst = (uint32_t)(want < have ? want : have);
+    if (want > have) {
+        if (threadIdx.x == 0) atomicOr(err, DERR_VALUE);
+    }
+    // stage [B0, B0 + nst) of the payload; 16 loads in flight per thread
+    for (uint32_t q0 = 0; q0 < nst; q0 += CH_WG * 16) {
+        uint8_t v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t q = q0 + j * CH_WG + threadIdx.x;
+            v[j] = in[J.byte_off + B0 + (q < nst ? q : 0u)];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t q = q0 + j * CH_WG + threadIdx.x;
+            if (q < nst) s_pay[q] = v[j];
+        }
+    }
+    __syncthreads();
+    uint32_t bit = (uint32_t)(bit0 & 7) + pre;   // bit offset inside s_pay
+#pragma unroll
+    for (int k = 0; k < DEC_TAG_ITEMS; ++k) {
         const uint32_t w = tg[k];
+        if (!w) continue;
         for (uint32_t d = 0; d < J.dims; ++d) {
-            const uint64_t B = bit >> 3;
-            const uint32_t sh = (uint32_t)(bit & 7);
+            const uint32_t B = bit >> 3, sh = bit & 7;
             const uint32_t nb = (sh + w + 7) >> 3;
             uint64_t acc_hi = 0, acc = 0;
             for (uint32_t q = 0; q < nb; ++q) {
-                const uint8_t by = B + q < J.bytes ? in[J.byte_off + B + q] : 0;
+                const uint8_t by = B + q < nst ? s_pay[B + q] : 0;
                 acc_hi = (acc_hi << 8) | (acc >> 56);
                 acc = (acc << 8) | by;
             }
             const uint32_t drop = 8 * nb - sh - w;
             uint64_t v = drop ? ((acc >> drop) | (drop < 64 ? (acc_hi << (64 - drop)) : 0)) : acc;
             v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
-            if (B + nb > J.bytes) atomicOr(err, DERR_VALUE);
             store_value(J.comp, J.dst + (r0 + k) * J.dims + d, v, idrel, diff, keep, choice, rel, rel8, err);
             bit += w;
         }

@@ -461,12 +461,17 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
         et.stats[c] = tm.begin("comp_stats");
         if (ntiles) {
             hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(stats_grid), dim3(CH_WG), 0, st, (const V*)v, c, F, tb,
-                               ntiles, x.ci, x.slots, x.vhist, x.thist);
-            hipLaunchKernelGGL(k_comp_reduce, dim3(F, reduce_y), dim3(CH_WG), 0, st, x.slots, stats_grid, ntiles, tb,
-                               F, c, x.ci, x.vhist, x.thist);
+                               ntiles, x.ci, x.slots + (uint64_t)c * (stats_grid + F), x.vhist, x.thist);
         }
         tm.end();
     });
+    if (ntiles) {
+        // one launch combines every component's slots
+        tm.begin("comp_stats");
+        hipLaunchKernelGGL(k_comp_reduce, dim3(F, reduce_y, 5), dim3(CH_WG), 0, st, x.slots, stats_grid, ntiles, tb,
+                           F, x.ci, x.vhist, x.thist);
+        tm.end();
+    }
     tm.begin("rans_table", 5ull * F * (RANS_VALUE_LIMIT + RANS_TAG_LIMIT) * 4 + 10ull * F * RANS_VALUE_LIMIT * 8);
     hipLaunchKernelGGL(k_rans_table, dim3(nsec), dim3(CH_WG), 0, st, x.ci, x.sec, x.vhist, x.thist, F, x.pc,
                        x.tbytes, err);
@@ -504,16 +509,18 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
         hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunk_bound), dim3(CH_WG), 0, st, x.ci, x.sec, nsec,
                            x.comp_chunk, x.scratch, stride, x.chunk_len, x.chunk_pos, x.chunk_off, x.out);
     tm.end();
+    // every component's look-back words and ticket, cleared by two launches
+    fill64(st, x.status1k, 5ull * (ntiles1k + 1), 0);
+    fill32(st, x.tickets, 5, 0);
     each_comp([&](int c, auto vt, auto dd, const void* v) {
         using V = typename decltype(vt)::type;
         constexpr int DC = decltype(dd)::value;
         et.pack[c] = tm.begin("pack");
         hipLaunchKernelGGL((k_pack_fixed<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
                            x.out);
-        fill64(st, x.status1k, ntiles1k, 0);
-        fill32(st, ticket, 1, 0);
         hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(ntiles1k), dim3(CH_WG), 0, st, (const V*)v, c, F,
-                           x.tag_tiles, x.ci, x.status1k, ticket, err, x.head, x.tail, x.head_pos, x.out);
+                           x.tag_tiles, x.ci, x.status1k + (uint64_t)c * (ntiles1k + 1), x.tickets + c, err, x.head,
+                           x.tail, x.head_pos, x.out);
         hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG)), dim3(CH_WG), 0, st, x.head, x.tail, x.head_pos,
                            x.tag_tiles + c * (F + 1) + F, x.out);
         tm.end();
@@ -748,6 +755,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             uint8_t* tail;
             uint64_t* head_pos;
             uint64_t* status1k;
+            uint32_t* tickets;
             uint32_t* tag_tiles;
             uint8_t* out;
         } x;
@@ -755,7 +763,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.sec = c.take<Section>(10 * F);
         x.vhist = c.take<uint32_t>(5ull * F * RANS_VALUE_LIMIT);
         x.thist = c.take<uint32_t>(5ull * F * RANS_TAG_LIMIT);
-        x.slots = c.take<CompSlot>(CS_GRID + F);
+        x.slots = c.take<CompSlot>(5ull * (CS_GRID + F));
         x.pc = c.take<uint64_t>(10ull * F * RANS_VALUE_LIMIT);
         x.tbytes = c.take<uint8_t>(10ull * F * TABLE_BYTES_MAX);
         x.comp_chunk = c.take<uint32_t>(8);
@@ -769,7 +777,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.head = c.take<uint8_t>(ntiles1k);
         x.tail = c.take<uint8_t>(ntiles1k);
         x.head_pos = c.take<uint64_t>(ntiles1k);
-        x.status1k = c.take<uint64_t>(ntiles1k + 1);
+        x.status1k = c.take<uint64_t>(5ull * (ntiles1k + 1));
+        x.tickets = c.take<uint32_t>(5);
         x.tag_tiles = c.take<uint32_t>(5ull * (F + 1));
         x.out = c.take<uint8_t>(out_bound);
         return x;

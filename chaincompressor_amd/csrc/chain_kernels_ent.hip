@@ -353,15 +353,17 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
     if (cur != 0xffffffffu) flush(cur);
 }
 
-// Combines the slots of every frame of one component: block (f, y) takes up
-// to CS_REDUCE_SLOTS slots of frame f and adds them into the frame's
-// statistics. Frame f's slots are G + f when it starts inside a workgroup's
+// Combines the slots of every frame of every component: block (f, y, c) takes
+// up to CS_REDUCE_SLOTS slots of frame f of component c and adds them into
+// that instance's statistics. Frame f's slots are G + f when it starts inside a workgroup's
 // range, then the workgroups whose first tile lies in f.
-__global__ __launch_bounds__(CH_WG) void k_comp_reduce(const CompSlot* __restrict__ slots, uint32_t G, uint32_t ntiles,
-                                                       const uint32_t* __restrict__ tile_begin, uint32_t F, int c,
-                                                       CompInst* __restrict__ ci, uint32_t* __restrict__ vhist,
-                                                       uint32_t* __restrict__ thist) {
+__global__ __launch_bounds__(CH_WG) void k_comp_reduce(const CompSlot* __restrict__ all_slots, uint32_t G,
+                                                       uint32_t ntiles, const uint32_t* __restrict__ tile_begin,
+                                                       uint32_t F, CompInst* __restrict__ ci,
+                                                       uint32_t* __restrict__ vhist, uint32_t* __restrict__ thist) {
     __shared__ uint32_t s_sum[4][2 * CS_PRIV_BINS];
+    const int c = (int)blockIdx.z;   // component; its slots follow the previous components'
+    const CompSlot* __restrict__ slots = all_slots + (uint64_t)c * (G + F);
     const uint32_t f = blockIdx.x;
     const uint32_t t0 = tile_begin[f], t1 = tile_begin[f + 1];
     if (t0 == t1) return;

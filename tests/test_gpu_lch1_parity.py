@@ -83,3 +83,33 @@ def test_batch_lch1_matches_single(eng):
         single, _ = eng.compress(f, 1e-3, 16, 2, fmt="lch1")
         assert o == single
         assert o == O.compress_streams(f, 1e-3, 16, 2).lch1()[0]
+
+
+@pytest.mark.parametrize("seed,d,chunk,mixed", [(70, 4, 65536, False), (71, 3, 8192, False), (72, 1, 16384, False),
+                                                 (73, 2, 4096, False), (70, 4, 8192, True), (71, 3, 16384, True),
+                                                 (72, 1, 65536, True), (73, 2, 8192, True)])
+def test_wide_values_lch1(eng, seed, d, chunk, mixed):
+    """Quantised values of about 50 bits (p = 1, eb ~ 1e-9 over a range of 2e6).
+    Uniform points make the Fixed payloads 45-50 bits wide; with most points
+    packed next to the minimum (mixed) the kept blocks go Tagged with widths
+    up to 50, for D up to 4 (the widest Tagged tiles)."""
+    rng = np.random.default_rng(seed)
+    n = 30_000
+    pts = rng.uniform(-1e6, 1e6, size=(n, d))
+    pts[: n // 3] = np.round(pts[: n // 3], 3)       # a share of short differences
+    if mixed:
+        far = rng.random(n) < 0.1
+        pts[~far] = -1e6 + rng.uniform(0, 1e-3, size=(int((~far).sum()), d))
+    eb = 1e-12 * 1e6 / 2 ** 10
+    print("case", seed, d, chunk, mixed, flush=True)
+    eng.set_rans(64, chunk)
+    try:
+        b, st = eng.compress(pts, eb, 1, 2, fmt="lch1")
+    finally:
+        eng.set_rans(64, 65536)
+    ref, kinds = O.compress_streams(pts, eb, 1, 2).lch1(64, chunk)
+    assert b == ref, kinds
+    want = O.decode(b)
+    got = eng.decompress(b)
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    assert np.all(np.abs(got - pts) <= eb)

@@ -405,11 +405,15 @@ static void fill32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(CH_WG), 0, st, p, n, v);
 }
 
+// status (cdiv(n, CH_TILE) words) and ticket start at zero: cleared here
+// unless the caller cleared them already (clear = false)
 static void scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* status,
-                     uint32_t* ticket, uint32_t* err, uint64_t* total) {
+                     uint32_t* ticket, uint32_t* err, uint64_t* total, bool clear = true) {
     const uint64_t tiles = cdiv(n, CH_TILE);
-    fill64(st, status, tiles, 0);
-    fill32(st, ticket, 1, 0);
+    if (clear) {
+        fill64(st, status, tiles, 0);
+        fill32(st, ticket, 1, 0);
+    }
     hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)tiles), dim3(CH_WG), 0, st, in, out, n, status, ticket, err, total);
 }
 
@@ -425,11 +429,9 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                           const uint8_t* choice2, const void* rel, const uint64_t* corr_idx, const double* corr_val,
                           const uint32_t* tb, const uint32_t* tb_host, uint32_t ntiles,
                           uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const RansConfig& rc, uint64_t stride,
-                          uint32_t* err, uint32_t* ticket, X& x) {
+                          uint32_t* err, X& x) {
     const uint32_t ninst = 5 * F, nsec = 10 * F;
-    hipLaunchKernelGGL(k_comp_init, dim3(cdiv(ninst, CH_WG)), dim3(CH_WG), 0, st, fdev, fo, F, D, x.ci);
-    fill32(st, x.vhist, 5ull * F * RANS_VALUE_LIMIT, 0);
-    fill32(st, x.thist, 5ull * F * RANS_TAG_LIMIT, 0);
+    hipLaunchKernelGGL(k_comp_init, dim3(ninst), dim3(CH_WG), 0, st, fdev, fo, F, D, x.ci, x.vhist, x.thist);
     // every component: a typed view and its launches
     auto each_comp = [&](auto&& fn) {
         with_dims(D, [&](auto dd) {
@@ -656,6 +658,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     // phase B sizes depend on the plan; compute an upper bound now so the
     // arena is allocated once (keys as u64, rel as u32 worst case)
     const uint64_t sort_tiles = cdiv(T, CH_TILE);
+    // look-back words: one region per scan of the pipeline (each radix pass,
+    // explicit compaction, L2 split, corrections), all cleared by one launch
+    const uint64_t lb_sort = cdiv(256 * sort_tiles, CH_TILE), lb_corr = cdiv(ntiles, CH_TILE);
     auto carve_b = [&](Carver& c, size_t key_bytes, size_t rel_bytes, uint32_t key_words) {
         struct {
             void* keys_in;
@@ -673,8 +678,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             double* corr_val;
             uint32_t* counts;
             uint32_t* offsets;
-            uint64_t* status;
-            uint32_t* ticket;
+            uint64_t* status;   // lb_words: sort passes, explicit, L2 split, corrections
+            uint32_t* ticket;   // lb_tickets, in the same order
+            uint64_t lb_words;
+            uint32_t lb_tickets;
             uint64_t* blocks;
             uint8_t* choice2;
             uint64_t* diff;
@@ -698,8 +705,12 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         b.corr_val = c.take<double>(T * D);
         b.counts = c.take<uint32_t>(256 * sort_tiles);
         b.offsets = c.take<uint32_t>(256 * sort_tiles);
-        b.status = c.take<uint64_t>(std::max<uint64_t>(cdiv(256 * sort_tiles, CH_TILE), std::max<uint64_t>(ntiles, sort_tiles)) + 1);
-        b.ticket = c.take<uint32_t>(4);
+        // at most key_bytes radix passes per key word
+        const uint64_t passes = key_bytes * key_words;
+        b.lb_words = passes * lb_sort + ntiles + sort_tiles + lb_corr;
+        b.lb_tickets = (uint32_t)passes + 3;
+        b.status = c.take<uint64_t>(b.lb_words);
+        b.ticket = c.take<uint32_t>(b.lb_tickets);
         b.blocks = c.take<uint64_t>(T * D);
         b.choice2 = c.take<uint8_t>(T);
         b.diff = c.take<uint64_t>(T * D);
@@ -906,7 +917,13 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     auto X = carve_c(cC);   // entropy-stage buffers (LCH1 only)
     // every launch after the planning sync: issued directly, or captured
     // once into a HIP graph and replayed for batches of the same launch shape
+    uint64_t* const lb_expl = B.status + (uint64_t)kbytes * nwords * lb_sort;
+    uint64_t* const lb_l2 = lb_expl + ntiles;
+    uint64_t* const lb_cor = lb_l2 + sort_tiles;
+    uint32_t* const tk_expl = B.ticket + kbytes * nwords;
     auto issue = [&]() {
+        fill64(st, B.status, B.lb_words, 0);
+        fill32(st, B.ticket, B.lb_tickets, 0);
         auto run_typed = [&](auto key_tag, auto rel_tag) {
             using KeyT = typename decltype(key_tag)::type;
             using RelT = typename decltype(rel_tag)::type;
@@ -951,7 +968,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                                    (uint32_t)sort_tiles, B.counts);
                 tm.end();
                 tm.begin("sort_scan", 2ull * 256 * sort_tiles * 4);
-                scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status, B.ticket, A.err, nullptr);
+                scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status + (uint64_t)ps_all * lb_sort,
+                         B.ticket + ps_all, A.err, nullptr, false);
                 tm.end();
                 tm.begin("sort_scatter", ps_all == 0 ? T * sizeof(KeyT) + T * (sizeof(KeyT) + 4)
                                                      : 2 * T * (sizeof(KeyT) + 4));
@@ -981,12 +999,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             tm.end();
             // K5 explicit compaction
             t_explicit = tm.begin("explicit");
-            fill64(st, B.status, ntiles, 0);
-            fill32(st, B.ticket, 1, 0);
             with_dims(D, [&](auto dd) {
                 constexpr int DD = decltype(dd)::value;
                 hipLaunchKernelGGL((k_explicit<KeyT, DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.idrel,
-                                   (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, B.status, B.ticket, A.err, B.blocks,
+                                   (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, lb_expl, tk_expl, A.err, B.blocks,
                                    B.fo, &B.totals[0], nwords, T);
             });
             hipLaunchKernelGGL(k_frame_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[0]);
@@ -1012,10 +1028,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
             tm.end();
             t_l2p = tm.begin("l2_split");
-            fill64(st, B.status, rec_tiles_max, 0);
-            fill32(st, B.ticket, 1, 0);
             hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)rec_tiles_max), dim3(CH_WG), 0, st, B.blocks,
-                               &B.totals[0], F, B.rec_base, A.fdev, B.fo, B.status, B.ticket, A.err, B.choice2, B.diff,
+                               &B.totals[0], F, B.rec_base, A.fdev, B.fo, lb_l2, tk_expl + 1, A.err, B.choice2, B.diff,
                                B.keep, &B.totals[1]);
         });
         hipLaunchKernelGGL(k_frame_split_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[1],
@@ -1024,7 +1038,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
 
         // corrections: per-tile counts -> offsets -> ordered gather
         t_corr = tm.begin("corrections");
-        scan_u32(st, B.tile_ncorr, B.tile_corr_off, ntiles, B.status, B.ticket, A.err, &B.totals[2]);
+        scan_u32(st, B.tile_ncorr, B.tile_corr_off, ntiles, lb_cor, tk_expl + 2, A.err, &B.totals[2], false);
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
             hipLaunchKernelGGL((k_corr_gather<DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.corr_tmp_idx, B.corr_tmp_val,
@@ -1036,7 +1050,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         if (format == CHAIN_FMT_LCH1)
             entropy_stage(st, tm, et, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
                           B.corr_val, A.tb, tile_begin.data(), ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err,
-                          B.ticket, X);
+                          X);
     };
     if (e->use_graphs) {
         GraphKey key;

@@ -100,10 +100,15 @@ __device__ inline uint64_t lookback_seg(uint64_t* status, uint32_t tile, uint32_
 // ------------------------------------------------------------------------
 // component instances
 
-__global__ void k_comp_init(const FrameDev* __restrict__ fr, const FrameOut* __restrict__ fo, uint32_t F, uint32_t D,
-                            CompInst* __restrict__ ci) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 5 * F) return;
+// one workgroup per component instance: its statistics record and its two
+// histograms start at zero
+__global__ __launch_bounds__(CH_WG) void k_comp_init(const FrameDev* __restrict__ fr, const FrameOut* __restrict__ fo,
+                                                     uint32_t F, uint32_t D, CompInst* __restrict__ ci,
+                                                     uint32_t* __restrict__ vhist, uint32_t* __restrict__ thist) {
+    const uint32_t i = blockIdx.x;
+    for (uint32_t k = threadIdx.x; k < RANS_VALUE_LIMIT; k += CH_WG) vhist[(uint64_t)i * RANS_VALUE_LIMIT + k] = 0;
+    if (threadIdx.x < RANS_TAG_LIMIT) thist[(uint64_t)i * RANS_TAG_LIMIT + threadIdx.x] = 0;
+    if (threadIdx.x != 0) return;
     const int c = i / F;
     const uint32_t f = i % F;
     CompInst x;

@@ -332,14 +332,13 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         for (int j = 0; j < CH_ITEMS; ++j) {
             const uint64_t r = r0 + (uint64_t)j * CH_WG + tid;
             if (r >= n) continue;
-            uint32_t tag = 1;
+            AccT any = 0;   // bit length of the OR = widest bit length
 #pragma unroll
             for (int d = 0; d < DIMS; ++d) {
                 const AccT v = (AccT)x[j][d];
                 mn[d] = v < mn[d] ? v : mn[d];
                 mx[d] = v > mx[d] ? v : mx[d];
-                const uint32_t b = sizeof(AccT) == 4 ? 32u - (uint32_t)__clz((uint32_t)v) : (uint32_t)bitlen64(v);
-                tag = b > tag ? b : tag;
+                any |= v;
                 if (v < CS_PRIV_BINS)
                     atomicAdd(&s_pv[(uint32_t)v >> 1][tid], 1u << (((uint32_t)v & 1u) << 4));
                 else if (v < RANS_VALUE_LIMIT)
@@ -347,6 +346,8 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
                 else
                     ovf = 1u;
             }
+            const uint32_t wb = sizeof(AccT) == 4 ? 32u - (uint32_t)__clz((uint32_t)any) : (uint32_t)bitlen64(any);
+            const uint32_t tag = wb > 1u ? wb : 1u;
             if (tag <= CS_PRIV_BINS)
                 atomicAdd(&s_pt[(tag - 1) >> 1][tid], 1u << (((tag - 1) & 1u) << 4));
             else
@@ -627,30 +628,45 @@ struct EncLds {
     uint64_t ml[256];   // reciprocal multiplier m (< 2^32) | l << 32
 };
 
-// symbols t-1, ..., t-g of this lane (steps are coded backwards); a missing
-// symbol is 0xffffffff. The loads are unconditional (clamped to the chunk's
-// first symbol) so that they stay in flight across the group being coded.
-template <typename ValT, int DIMS>
-__device__ inline void enc_group_load(const ValT* __restrict__ src, bool tags, uint64_t first, uint64_t Sc,
-                                      uint32_t L, uint32_t lane, uint64_t t, uint32_t g, uint32_t* sym) {
+// Raw input of steps t-1, ..., t-g of this lane (steps are coded
+// backwards): the value, or for a tags section the DIMS values of the record.
+// Loads are unconditional (a missing step reads the chunk's first symbol) and
+// their results are not touched here, so they stay in flight while the group
+// before is coded; bit j of *valid marks step t-1-j as present.
+template <typename ValT, int RD>
+__device__ inline void enc_group_load(const ValT* __restrict__ base, uint32_t Sc, uint32_t L, uint32_t lane,
+                                      uint32_t t, uint32_t g, ValT (&raw)[ENC_GROUP][RD], uint32_t& valid) {
+    valid = 0;
 #pragma unroll
     for (int j = 0; j < ENC_GROUP; ++j) {
-        const uint64_t k = (t - 1 - j) * L + lane;
-        const bool valid = (uint32_t)j < g && lane < L && k < Sc;
-        const uint64_t q = first + (valid ? k : 0ull);
-        uint32_t v;
-        if (!tags) {
-            v = (uint32_t)src[q];
-        } else {
-            uint32_t tg = 1;
+        const uint32_t k = (t - 1 - j) * L + lane;
+        const bool v = (uint32_t)j < g && lane < L && k < Sc;
+        valid |= v ? 1u << j : 0u;
+        const uint32_t q = v ? k : 0u;
 #pragma unroll
-            for (int d = 0; d < DIMS; ++d) {
-                const uint32_t b = (uint32_t)bitlen64((uint64_t)src[q * DIMS + d]);
-                tg = b > tg ? b : tg;
-            }
-            v = tg;
+        for (int d = 0; d < RD; ++d) raw[j][d] = base[q * RD + d];
+    }
+}
+
+// symbols of a loaded group; a missing step is 0xffffffff. A record's tag is
+// max(1, bit length of its widest value) = max(1, bit length of the OR of its
+// values).
+template <typename ValT, int RD, bool TAGS>
+__device__ inline void enc_group_sym(const ValT (&raw)[ENC_GROUP][RD], uint32_t valid, uint32_t* sym) {
+#pragma unroll
+    for (int j = 0; j < ENC_GROUP; ++j) {
+        uint32_t v;
+        if (!TAGS) {
+            v = (uint32_t)raw[j][0];
+        } else {
+            ValT o = raw[j][0];
+#pragma unroll
+            for (int d = 1; d < RD; ++d) o |= raw[j][d];
+            const uint32_t b = sizeof(ValT) == 8 ? (uint32_t)bitlen64((uint64_t)o)
+                                                 : 32u - (uint32_t)__clz((uint32_t)o);
+            v = b > 1u ? b : 1u;
         }
-        sym[j] = valid ? v : 0xffffffffu;
+        sym[j] = (valid >> j) & 1u ? v : 0xffffffffu;
     }
 }
 
@@ -668,12 +684,12 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
 // a byte of their own past the ring.
 #define ENC_RING 2048u
 #define ENC_FLUSH 1024u
-template <typename ValT, int DIMS, bool SMALL>
-__device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, uint32_t si, const CompInst& C,
+template <typename ValT, int DIMS, bool SMALL, bool TAGS>
+__device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, const CompInst& C,
                                  const Section& S, const uint64_t* __restrict__ gpc, EncLds& lds,
                                  uint8_t* __restrict__ ring, uint8_t* __restrict__ scratch, uint64_t stride,
                                  uint32_t* __restrict__ chunk_len, uint32_t* __restrict__ chunk_pos) {
-    const bool tags = (si & 1) != 0;
+    constexpr int RD = TAGS ? DIMS : 1;   // values per symbol
     const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
     if (SMALL)
         for (uint32_t b = lane; b < S.alphabet; b += 64) {
@@ -686,24 +702,27 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
     __syncthreads();
     const uint32_t M = 1u << pb, lower = 4u * M;
     const uint64_t first = (uint64_t)(chunk - S.chunk_base) * S.chunk_symbols;
-    const uint64_t Sc = S.count - first < S.chunk_symbols ? S.count - first : S.chunk_symbols;
-    const uint64_t T = (Sc + L - 1) / L;
+    const uint32_t Sc = (uint32_t)(S.count - first < S.chunk_symbols ? S.count - first : S.chunk_symbols);
+    const uint32_t T = (Sc + L - 1) / L;
     uint8_t* out = scratch + (uint64_t)chunk * stride;
     uint8_t* stream = out + CHUNK_STATE_BYTES;
     // stream offsets: [pos, end) written so far, [flushed, end) already global
     const uint32_t end = (uint32_t)(stride - CHUNK_STATE_BYTES);
     uint32_t pos = end, flushed = end;
     uint32_t x = lower;
-    const ValT* src = vals + C.base * DIMS;
+    // the chunk's symbols: values from `first`, or records from `first` for a tags section
+    const ValT* base = vals + C.base * DIMS + first * RD;
     const uint32_t dummy = ENC_RING + lane;
-    uint32_t cur[ENC_GROUP], nxt[ENC_GROUP];
-    uint64_t t = T;
-    uint32_t g = t < ENC_GROUP ? (uint32_t)t : ENC_GROUP;
-    enc_group_load<ValT, DIMS>(src, tags, first, Sc, L, lane, t, g, cur);
+    uint32_t cur[ENC_GROUP];
+    ValT raw[ENC_GROUP][RD];
+    uint32_t valid;
+    uint32_t t = T;
+    uint32_t g = t < ENC_GROUP ? t : ENC_GROUP;
+    enc_group_load<ValT, RD>(base, Sc, L, lane, t, g, raw, valid);
+    enc_group_sym<ValT, RD, TAGS>(raw, valid, cur);
     while (t > 0) {
-        const uint64_t tn = t - g;
-        const uint32_t gn = tn < ENC_GROUP ? (uint32_t)tn : ENC_GROUP;
-        if (tn > 0) enc_group_load<ValT, DIMS>(src, tags, first, Sc, L, lane, tn, gn, nxt);
+        // table entries of this group first: a wait for them is then no
+        // wait for the next group's loads, issued after them
         uint64_t ent[ENC_GROUP], ml[ENC_GROUP];
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
@@ -713,6 +732,9 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             ent[j] = a ? e : 1ull;
             ml[j] = SMALL ? lds.ml[sj] : 0ull;
         }
+        const uint32_t tn = t - g;
+        const uint32_t gn = tn < ENC_GROUP ? tn : ENC_GROUP;
+        if (tn > 0) enc_group_load<ValT, RD>(base, Sc, L, lane, tn, gn, raw, valid);
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
             if ((uint32_t)j >= g) break;
@@ -754,8 +776,7 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             flushed -= ENC_FLUSH;
             __syncthreads();
         }
-#pragma unroll
-        for (int j = 0; j < ENC_GROUP; ++j) cur[j] = nxt[j];
+        if (tn > 0) enc_group_sym<ValT, RD, TAGS>(raw, valid, cur);
         t = tn;
         g = gn;
     }
@@ -773,10 +794,19 @@ __device__ inline void enc_chunk_any(const ValT* __restrict__ vals, uint32_t chu
                                      const Section& S, const uint64_t* __restrict__ gpc, EncLds& lds,
                                      uint8_t* __restrict__ ring, uint8_t* __restrict__ scratch, uint64_t stride,
                                      uint32_t* __restrict__ chunk_len, uint32_t* __restrict__ chunk_pos) {
-    if (S.alphabet <= 256)
-        enc_chunk<ValT, DIMS, true>(vals, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
-    else
-        enc_chunk<ValT, DIMS, false>(vals, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
+    // section 2i holds the values of instance i, section 2i + 1 its record tags
+    const bool small = S.alphabet <= 256;
+    if (si & 1) {
+        if (small)
+            enc_chunk<ValT, DIMS, true, true>(vals, chunk, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
+        else
+            enc_chunk<ValT, DIMS, false, true>(vals, chunk, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
+    } else {
+        if (small)
+            enc_chunk<ValT, DIMS, true, false>(vals, chunk, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
+        else
+            enc_chunk<ValT, DIMS, false, false>(vals, chunk, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos);
+    }
 }
 
 // every component's chunks in one launch: block b codes chunk b of the
@@ -1131,12 +1161,11 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     for (int k = 0; k < 4; ++k) {
         tg[k] = 0;
         if (rt + k < C.n) {
-            uint32_t t = 1;
+            uint64_t any = xv[k][0];   // bit length of the OR = widest bit length
 #pragma unroll
-            for (int d = 0; d < DIMS; ++d) {
-                const uint32_t b = (uint32_t)bitlen64(xv[k][d]);
-                t = b > t ? b : t;
-            }
+            for (int d = 1; d < DIMS; ++d) any |= xv[k][d];
+            const uint32_t wb = (uint32_t)bitlen64(any);
+            const uint32_t t = wb > 1u ? wb : 1u;
             tg[k] = t;
             bits += DIMS * t;
         }

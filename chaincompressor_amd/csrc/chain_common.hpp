@@ -24,13 +24,15 @@
 
 namespace chain {
 
-enum : uint32_t { FL_L1 = 1u, FL_L2 = 2u };
+// FL_Q32: every quantised coordinate of the frame is below 2^32;
+// FL_P2: the block size p is a power of two
+enum : uint32_t { FL_L1 = 1u, FL_L2 = 2u, FL_Q32 = 4u, FL_P2 = 8u };
 
 struct FrameDev {
     uint64_t pt_off;        // first global point of the frame
     uint64_t n;             // points
     uint32_t tile0;         // first point tile
-    uint32_t flags;         // FL_L1 | FL_L2
+    uint32_t flags;         // FL_L1 | FL_L2 | FL_Q32 | FL_P2
     double eb, denom;       // denom = 2*eb
     double mins[CH_MAXD];
     uint64_t p;             // effective p (p, or 1 without blocking)

@@ -886,7 +886,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         P.key_bits = total_bits;
         key_bits_max = std::max(key_bits_max, total_bits);
         relmax = std::max(relmax, P.relmax);
-        Fd.flags = ((P.flags & CHAIN_NO_L1) ? 0u : FL_L1) | ((P.flags & CHAIN_NO_L2) ? 0u : FL_L2);
+        uint64_t qtop = 0;
+        for (uint32_t d = 0; d < D; ++d) qtop = std::max(qtop, P.qmax[d]);
+        Fd.flags = ((P.flags & CHAIN_NO_L1) ? 0u : FL_L1) | ((P.flags & CHAIN_NO_L2) ? 0u : FL_L2) |
+                   (qtop <= 0xffffffffull && P.pe <= 0x80000000ull ? FL_Q32 : 0u) | ((P.pe & (P.pe - 1)) == 0 ? FL_P2 : 0u);
         Fd.eb = P.eb;
         Fd.denom = denom;
         Fd.p = P.pe;

@@ -303,6 +303,7 @@ __device__ inline bool within_eb(double restored, double original, double eb) {
     return difference >= 0.0 ? remainder <= 0.0 : remainder >= 0.0;
 }
 
+#define QZ_GROUP 4   // points per thread whose loads are in flight together
 template <typename T, int D, bool SOA, typename KeyT, typename RelT>
 __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, const FrameDev* __restrict__ fr,
                                                     const uint32_t* __restrict__ tile_begin, uint32_t nframes,
@@ -322,9 +323,27 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
     uint32_t shift[D], word[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) { mins[d] = F.mins[d]; shift[d] = F.key_shift[d]; word[d] = F.key_word[d]; }
+    // frame-uniform fast paths: q < 2^32 converts with one instruction, and
+    // a power-of-two p splits q with a shift and a mask
+    const bool q32 = (F.flags & FL_Q32) != 0, p2 = (F.flags & FL_P2) != 0;
     __shared__ uint32_t s_tmp[4];
     uint64_t cmask = 0;   // correction flags, bit j*D + d
-    for (int j = 0; j < CH_ITEMS; ++j) {
+#pragma unroll 1
+    for (int j0 = 0; j0 < CH_ITEMS; j0 += QZ_GROUP) {
+        // every coordinate of the group is loaded before the first is used;
+        // points past the end load the tile's first point
+        T xs[QZ_GROUP][D];
+#pragma unroll
+        for (int jj = 0; jj < QZ_GROUP; ++jj) {
+            const uint64_t i = start + (uint64_t)(j0 + jj) * CH_WG + threadIdx.x;
+            const uint64_t ic = i < end ? i : start;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                xs[jj][d] = pts[SOA ? F.pt_off * D + (uint64_t)d * F.n + ic : (F.pt_off + ic) * D + d];
+        }
+#pragma unroll
+        for (int jj = 0; jj < QZ_GROUP; ++jj) {
+        const int j = j0 + jj;
         const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
         if (i < end) {
             KeyT key = 0;
@@ -333,13 +352,23 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
             for (int d = 0; d < D; ++d) kw[d] = 0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const double x = load_coord<T, D, SOA>(pts, F, i, d);
+                const double x = (double)xs[jj][d];
+                // 0 <= qv < 2^63 is an integer, so (double)(uint64_t)qv == qv
                 const double qv = floor((x - mins[d]) / denom);
-                const uint64_t q = (uint64_t)qv;
-                const double restored = fma((double)q + 0.5, denom, mins[d]);
+                const double restored = fma(qv + 0.5, denom, mins[d]);
                 if (!within_eb(restored, x, eb)) cmask |= 1ull << (j * D + d);
-                const uint64_t b = div_p(q, p, magic, l);
-                rel_out[(F.pt_off + i) * D + d] = (RelT)(q - b * p);
+                uint64_t b, r;
+                if (q32) {
+                    const uint32_t q = (uint32_t)qv;
+                    const uint32_t bq = p2 ? q >> l : (uint32_t)div_p(q, p, magic, l);
+                    b = bq;
+                    r = p2 ? (q & (uint32_t)(p - 1)) : q - bq * (uint32_t)p;
+                } else {
+                    const uint64_t q = (uint64_t)qv;
+                    b = p2 ? q >> l : div_p(q, p, magic, l);
+                    r = q - b * p;
+                }
+                rel_out[(F.pt_off + i) * D + d] = (RelT)r;
                 if (nwords == 1) key |= (KeyT)b << shift[d];
                 else kw[word[d]] |= (KeyT)b << shift[d];
             }
@@ -350,6 +379,7 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
                 for (int w = 0; w < D; ++w)
                     if ((uint32_t)w < nwords) key_out[(uint64_t)w * total_pts + F.pt_off + i] = kw[w];
             }
+        }
         }
     }
     // exact corrections, in element order (j, thread, d); rare, so the

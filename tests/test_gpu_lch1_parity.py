@@ -59,6 +59,10 @@ def test_rans_parameters(eng, lanes, chunk):
     (3, 100_000, 2, np.float32, 1e-3, 1, 5),
     (4, 60_000, 4, np.float32, 2e-3, 256, -2),
     (5, 90_000, 1, np.float64, 1e-5, 64, 3),
+    # p not a power of two: quantised values below 2^32, then above it
+    (6, 120_000, 3, np.float32, 1e-3, 10, 2),
+    (7, 80_000, 2, np.float64, 1e-9, 12, 1),
+    (8, 50_000, 3, np.float64, 1e-11, 7, 0),
 ])
 def test_random_lch1(eng, seed, n, d, dtype, eb, p, t):
     from chaincompressor_amd import workloads

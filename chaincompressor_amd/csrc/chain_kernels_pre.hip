@@ -735,19 +735,36 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
     }
     __syncthreads();
     if (flags) {
+        // the key layout in registers: the record stores then need no wait
+        uint32_t sh[D], kw[D];
+        uint64_t mk[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const uint32_t wd = F.key_width[d];
+            mk[d] = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
+            sh[d] = F.key_shift[d];
+            kw[d] = F.key_word[d];
+        }
         const uint64_t base = s_excl;
+        if (nwords == 1) {
 #pragma unroll
-        for (int k = 0; k < CH_ITEMS; ++k) {
-            if (!(flags & (1u << k))) continue;
-            const uint64_t i = start + k * CH_WG + threadIdx.x;
-            const uint64_t rec = base + s_off[k * W + wave] + pre[k];
+            for (int k = 0; k < CH_ITEMS; ++k) {
+                if (!(flags & (1u << k))) continue;
+                const uint64_t rec = base + s_off[k * W + wave] + pre[k];
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const uint32_t wd = F.key_width[d];
-                const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
-                const uint64_t key = nwords == 1 ? (uint64_t)kpre[k]
-                                                 : (uint64_t)keys[(uint64_t)F.key_word[d] * T + F.pt_off + i];
-                blocks[rec * D + d] = (key >> F.key_shift[d]) & mask;
+                for (int d = 0; d < D; ++d) blocks[rec * D + d] = ((uint64_t)kpre[k] >> sh[d]) & mk[d];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < CH_ITEMS; ++k) {
+                if (!(flags & (1u << k))) continue;
+                const uint64_t i = start + k * CH_WG + threadIdx.x;
+                const uint64_t rec = base + s_off[k * W + wave] + pre[k];
+                uint64_t kd[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) kd[d] = (uint64_t)keys[(uint64_t)kw[d] * T + F.pt_off + i];
+#pragma unroll
+                for (int d = 0; d < D; ++d) blocks[rec * D + d] = (kd[d] >> sh[d]) & mk[d];
             }
         }
     }
@@ -927,6 +944,15 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
     const uint64_t t0 = (uint64_t)tile * CH_TILE;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
+    // A tile inside one frame (the usual case) reads that frame's parameters
+    // once, into registers; a tile across frames looks them up per record.
+    const uint64_t tl = (t0 + CH_TILE < R ? t0 + CH_TILE : R) - 1;
+    const uint32_t fa = frame_of_rec(bases, nframes, t0);
+    const bool one = fa == frame_of_rec(bases, nframes, tl);
+    const uint64_t base_a = bases[fa];
+    const bool l2_a = (fr[fa].flags & FL_L2) != 0;
+    const uint64_t fvb_a = fo[fa].fixed_value_bits, fdb_a = fo[fa].fixed_diff_bits;
+    const int64_t wminc_a = fo[fa].wrap_min_corr, wmaxc_a = fo[fa].wrap_max_corr, wr_a = fo[fa].wrap_range;
     // Items are taken L2S_GROUP at a time: the blocks of the group's records
     // and of their predecessors are loaded unconditionally (clamped to the
     // tile's first record) before any is used.
@@ -954,10 +980,18 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
             bool c = false;
             uint32_t fkk = 0;
             if (r < R) {
-                const uint32_t f = frame_of_rec(bases, nframes, r);
+                uint32_t f = fa;
+                uint64_t fbase = base_a, fvb = fvb_a, fdb = fdb_a;
+                bool l2 = l2_a;
+                if (!one) {
+                    f = frame_of_rec(bases, nframes, r);
+                    fbase = bases[f];
+                    l2 = (fr[f].flags & FL_L2) != 0;
+                    fvb = fo[f].fixed_value_bits;
+                    fdb = fo[f].fixed_diff_bits;
+                }
                 fkk = f;
-                if (r != bases[f] && (fr[f].flags & FL_L2)) {
-                    const FrameOut& o = fo[f];
+                if (r != fbase && l2) {
                     uint64_t mv = 0, mdf = 0;
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
@@ -969,8 +1003,8 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
                     const uint64_t tvw = bitlen64(mv), tdw = bitlen64(mdf);
                     const uint64_t tv = (uint64_t)D * tvw + bitlen64(tvw);
                     const uint64_t td = (uint64_t)D * tdw + bitlen64(tdw);
-                    const uint64_t value_bits = tv < o.fixed_value_bits ? tv : o.fixed_value_bits;
-                    const uint64_t diff_bits = td > o.fixed_diff_bits ? td : o.fixed_diff_bits;
+                    const uint64_t value_bits = tv < fvb ? tv : fvb;
+                    const uint64_t diff_bits = td > fdb ? td : fdb;
                     c = value_bits >= diff_bits;
                 }
             }
@@ -1009,9 +1043,9 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
             const int k = kg + j;
             const uint64_t r = t0 + k * CH_WG + threadIdx.x;
             if (r >= R) continue;
-            const uint32_t f = s_fk[k][threadIdx.x];
+            const uint32_t f = one ? fa : s_fk[k][threadIdx.x];
             const uint64_t dpos = s_excl + s_off[k * W + wave] + s_pre[k][threadIdx.x];   // diffs before record r
-            if (r == bases[f]) {
+            if (r == (one ? base_a : bases[f])) {
                 fo[f].diff_base = dpos;
                 fo[f].keep_base = r - dpos;
             }
@@ -1022,13 +1056,18 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
 #pragma unroll
                 for (int d = 0; d < D; ++d) keep[kpos * D + d] = cur[j][d];
             } else {
-                const FrameOut& o = fo[f];
+                int64_t wminc = wminc_a, wmaxc = wmaxc_a, wr = wr_a;
+                if (!one) {
+                    wminc = fo[f].wrap_min_corr;
+                    wmaxc = fo[f].wrap_max_corr;
+                    wr = fo[f].wrap_range;
+                }
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     // prediction = previous record clamped to [wrap_min, wrap_max] (an identity here)
                     int64_t corr = (int64_t)prv[j][d] - (int64_t)cur[j][d];
-                    if (corr < o.wrap_min_corr) corr += o.wrap_range;
-                    else if (corr > o.wrap_max_corr) corr -= o.wrap_range;
+                    if (corr < wminc) corr += wr;
+                    else if (corr > wmaxc) corr -= wr;
                     diff[dpos * D + d] = corr >= 0 ? ((uint64_t)corr << 1) : (((uint64_t)(-(corr + 1)) << 1) | 1ull);
                 }
             }

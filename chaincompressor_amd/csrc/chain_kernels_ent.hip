@@ -620,12 +620,16 @@ __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_
 // with an exact reciprocal of every frequency, so the state update needs no
 // integer division: with l = ceil(log2 f) and m = ceil(2^(31+l) / f) < 2^32
 // (rcp31_* in chain_int.hpp), floor(x / f) = mulhi(x, m) >> (l - 1) for
-// l >= 1, and x for f = 1.
+// l >= 1, and x for f = 1. The update x' = (x / f) * M + x % f + cum is
+// taken as x + (x / f) * (M - f) + cum, where x / f < 1024 and M - f < 2^20
+// make the product a 24-bit multiply.
 #define ENC_GROUP 4
+#define ENC_XMAX_IDLE (1u << 30)   // above every state: an idle lane emits nothing
 
 struct EncLds {
-    uint64_t pc[256];   // (cum << 32) | freq
-    uint64_t ml[256];   // reciprocal multiplier m (< 2^32) | l << 32
+    // per symbol: {1024 * f (bytes are emitted while x >= this), cum, m,
+    //              (M - f) | (l - 1) << 24 | (f == 1) << 29}
+    uint4 tab[256];
 };
 
 // Raw input of steps t-1, ..., t-g of this lane (steps are coded
@@ -680,8 +684,7 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
 // decreasing offsets. They are staged in an LDS ring of ENC_RING bytes and
 // written out in aligned blocks of ENC_FLUSH bytes (16 per lane), so the
 // global stores are wide and few; what is left at the end is copied byte by
-// byte. A step has no vector branches: lanes without a byte to emit write to
-// a byte of their own past the ring.
+// byte. Only the lanes with a byte for a plane write it.
 #define ENC_RING 2048u
 #define ENC_FLUSH 1024u
 template <typename ValT, int DIMS, bool SMALL, bool TAGS>
@@ -691,16 +694,17 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
                                  uint32_t* __restrict__ chunk_len, uint32_t* __restrict__ chunk_pos) {
     constexpr int RD = TAGS ? DIMS : 1;   // values per symbol
     const uint32_t lane = threadIdx.x, L = S.lanes, pb = S.pb;
+    const uint32_t M = 1u << pb, lower = 4u * M;
     if (SMALL)
         for (uint32_t b = lane; b < S.alphabet; b += 64) {
             const uint64_t e = gpc[b];
             const uint32_t f = (uint32_t)e;
-            lds.pc[b] = e;
             const uint32_t l = f ? rcp31_shift(f) : 0u;
-            lds.ml[b] = f ? (rcp31_mult(f, l) | ((uint64_t)l << 32)) : 0ull;
+            const uint32_t m = f ? (uint32_t)rcp31_mult(f, l) : 0u;
+            lds.tab[b] = make_uint4(1024u * f, (uint32_t)(e >> 32), m,
+                                    (M - f) | ((l ? l - 1 : 0u) << 24) | ((f == 1 ? 1u : 0u) << 29));
         }
     __syncthreads();
-    const uint32_t M = 1u << pb, lower = 4u * M;
     const uint64_t first = (uint64_t)(chunk - S.chunk_base) * S.chunk_symbols;
     const uint32_t Sc = (uint32_t)(S.count - first < S.chunk_symbols ? S.count - first : S.chunk_symbols);
     const uint32_t T = (Sc + L - 1) / L;
@@ -712,7 +716,6 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
     uint32_t x = lower;
     // the chunk's symbols: values from `first`, or records from `first` for a tags section
     const ValT* base = vals + C.base * DIMS + first * RD;
-    const uint32_t dummy = ENC_RING + lane;
     uint32_t cur[ENC_GROUP];
     ValT raw[ENC_GROUP][RD];
     uint32_t valid;
@@ -723,14 +726,18 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
     while (t > 0) {
         // table entries of this group first: a wait for them is then no
         // wait for the next group's loads, issued after them
-        uint64_t ent[ENC_GROUP], ml[ENC_GROUP];
+        uint4 te[ENC_GROUP];       // alphabet <= 256: LDS entries
+        uint64_t ent[ENC_GROUP];   // larger alphabets: (cum << 32) | f
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
             const bool a = cur[j] != 0xffffffffu;
             const uint32_t sj = a ? cur[j] : 0u;
-            const uint64_t e = SMALL ? lds.pc[sj] : gpc[sj];
-            ent[j] = a ? e : 1ull;
-            ml[j] = SMALL ? lds.ml[sj] : 0ull;
+            if (SMALL) {
+                te[j] = lds.tab[sj];
+                te[j].x = a ? te[j].x : ENC_XMAX_IDLE;
+            } else {
+                ent[j] = gpc[sj];
+            }
         }
         const uint32_t tn = t - g;
         const uint32_t gn = tn < ENC_GROUP ? tn : ENC_GROUP;
@@ -739,31 +746,28 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
         for (int j = 0; j < ENC_GROUP; ++j) {
             if ((uint32_t)j >= g) break;
             const bool active = cur[j] != 0xffffffffu;
-            const uint64_t e = ent[j];
-            const uint32_t f = (uint32_t)e, cu = (uint32_t)(e >> 32);
+            const uint32_t f = SMALL ? 0u : (uint32_t)ent[j];
             // bytes to emit: x < 2^30 <= 256^3 * 1024 * f, so at most three
-            const uint32_t xmax = 1024u * f;     // (lower >> pb) * 256 * f
-            const uint32_t kb = active ? (uint32_t)(x >= xmax) + (uint32_t)((x >> 8) >= xmax) +
-                                             (uint32_t)((x >> 16) >= xmax)
-                                       : 0u;
+            const uint32_t xmax = SMALL ? te[j].x : (active ? 1024u * f : ENC_XMAX_IDLE);
+            const uint32_t kb = (uint32_t)(x >= xmax) + (uint32_t)((x >> 8) >= xmax) + (uint32_t)((x >> 16) >= xmax);
             const uint32_t xx = x >> (8 * kb);
             const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
             const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
             pos -= c0 + c1 + c2;
-            const uint32_t a0 = kb > 0 ? (pos + mbcnt64(m0)) & (ENC_RING - 1) : dummy;
-            const uint32_t a1 = kb > 1 ? (pos + c0 + mbcnt64(m1)) & (ENC_RING - 1) : dummy;
-            const uint32_t a2 = kb > 2 ? (pos + c0 + c1 + mbcnt64(m2)) & (ENC_RING - 1) : dummy;
-            ring[a0] = (uint8_t)(x >> (8 * (kb > 0 ? kb - 1 : 0)));
-            ring[a1] = (uint8_t)(x >> (8 * (kb > 1 ? kb - 2 : 0)));
-            ring[a2] = (uint8_t)x;
-            uint32_t q;
+            // planes 1 and 2 are rare: their lanes (and whole blocks) are skipped
+            if (kb > 0) ring[(pos + mbcnt64(m0)) & (ENC_RING - 1)] = (uint8_t)(x >> (8 * (kb - 1)));
+            if (kb > 1) ring[(pos + c0 + mbcnt64(m1)) & (ENC_RING - 1)] = (uint8_t)(x >> (8 * (kb - 2)));
+            if (kb > 2) ring[(pos + c0 + c1 + mbcnt64(m2)) & (ENC_RING - 1)] = (uint8_t)x;
+            uint32_t xn;
             if (SMALL) {
-                const uint32_t m = (uint32_t)ml[j], l = (uint32_t)(ml[j] >> 32);
-                q = l ? __umulhi(xx, m) >> (l - 1) : xx;
+                const uint32_t w = te[j].w;
+                uint32_t q = __umulhi(xx, te[j].z) >> ((w >> 24) & 31u);
+                q = (w >> 29) & 1u ? xx : q;
+                xn = xx + __umul24(q, w & 0xffffffu) + te[j].y;
             } else {
-                q = xx / f;
+                const uint32_t q = xx / (active ? f : 1u);
+                xn = q * M + (xx - q * f) + (uint32_t)(ent[j] >> 32);
             }
-            const uint32_t xn = q * M + (xx - q * f) + cu;
             x = active ? xn : x;
         }
         // a group adds at most ENC_GROUP * 192 bytes: one flush keeps the
@@ -831,8 +835,7 @@ __global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__
     const CompInst& C = ci[si >> 1];
     const uint64_t* gpc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
     __shared__ EncLds lds;
-    // the ring, then one byte per lane for lanes without a byte to emit
-    __shared__ __attribute__((aligned(16))) uint8_t ring[ENC_RING + 64];
+    __shared__ __attribute__((aligned(16))) uint8_t ring[ENC_RING];
     switch (c) {
     case 0: enc_chunk_any<uint32_t, 1>(idrel, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     case 1: enc_chunk_any<uint64_t, D>(diff, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;

@@ -204,17 +204,27 @@ __global__ __launch_bounds__(64 * DEC_WAVES, 4) void k_dec_rans(const uint8_t* _
             if (S.comp < 0) tags[S.dst + q] = (uint8_t)s;
             else store_value(S.comp, S.dst + q, s, idrel, diff, keep, choice, rel, rel8, err);
         }
-        // a valid stream needs at most ceil(pb / 8) <= 3 bytes per step
-        for (int round = 0; round < 4; ++round) {
-            const bool need = active && x < lower;
-            const uint64_t m = __ballot(need);
-            if (!m) break;
-            if (need) {
-                const uint64_t at = pos + __popcll(m & lt);
-                x = (x << 8) | (at < stream_len ? (uint32_t)ring[(uint32_t)at & (DEC_RING - 1)] : 0u);
-            }
-            pos += __popcll(m);
+        // Bytes this step reads. lower = 2^(pb+2), pb >= 12: x << 8 | byte
+        // < lower exactly when x < lower >> 8, whatever the byte, and two
+        // bytes fall short exactly when x < lower >> 16 (pb >= 14; below,
+        // only x = 0 would, which no valid stream reaches: x >= 4 after a
+        // step). So the count is known before any byte is read and the (at
+        // most three) planes are read together; a state still below lower
+        // afterwards is a corrupt stream.
+        const uint32_t kb = active ? (uint32_t)(x < lower) + (uint32_t)(x < (lower >> 8)) + (uint32_t)(x < (lower >> 16))
+                                   : 0u;
+        const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
+        const uint32_t n0 = __popcll(m0), n1 = __popcll(m1);
+        if (kb > 0) {
+            auto byte_at = [&](uint64_t at) -> uint32_t {
+                return at < stream_len ? (uint32_t)ring[(uint32_t)at & (DEC_RING - 1)] : 0u;
+            };
+            uint32_t v = byte_at(pos + __popcll(m0 & lt));
+            if (kb > 1) v = (v << 8) | byte_at(pos + n0 + __popcll(m1 & lt));
+            if (kb > 2) v = (v << 8) | byte_at(pos + n0 + n1 + __popcll(m2 & lt));
+            x = (x << (8 * kb)) | v;
         }
+        pos += n0 + n1 + __popcll(m2);
         if (active && x < lower) {
             atomicOr(err, DERR_RANS);
             x = lower;

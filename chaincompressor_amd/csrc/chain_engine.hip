@@ -907,6 +907,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     uint32_t word_passes[CH_MAXD];
     for (uint32_t w = 0; w < CH_MAXD; ++w) word_passes[w] = (word_bits[w] + 7) / 8;
     if (nwords == 1) word_passes[0] = std::max<uint32_t>(1, (key_bits_max + 7) / 8);
+    // the first radix pass (low byte of key word 0) takes its histogram from quantise
+    const bool hist_q = word_passes[0] >= 1;
 
     Carver cB;
     cB.base = e->arena.base + size_a;
@@ -930,7 +932,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         auto run_typed = [&](auto key_tag, auto rel_tag) {
             using KeyT = typename decltype(key_tag)::type;
             using RelT = typename decltype(rel_tag)::type;
-            // K2 quantise
+            // K2 quantise (with the first radix pass's histogram)
+            if (hist_q) fill32(st, B.counts, 256 * sort_tiles, 0);
             tm.begin("quantize", T * D * es + T * sizeof(KeyT) * nwords + T * D * sizeof(RelT));
             with_dtype(dtype, [&](auto tt) {
                 using T_ = typename decltype(tt)::type;
@@ -939,11 +942,13 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                     if (layout == CHAIN_SOA)
                         hipLaunchKernelGGL((k_quantize<T_, DD, true, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
                                            (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
+                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T,
+                                           hist_q ? B.counts : nullptr, (uint32_t)sort_tiles);
                     else
                         hipLaunchKernelGGL((k_quantize<T_, DD, false, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
                                            (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T);
+                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T,
+                                           hist_q ? B.counts : nullptr, (uint32_t)sort_tiles);
                 });
             });
             tm.end();
@@ -966,10 +971,12 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                 }
                 const uint32_t shift = 8 * ps;
                 const uint32_t ps_all = done_passes++;
-                tm.begin("sort_hist", T * sizeof(KeyT));
-                hipLaunchKernelGGL((k_sort_hist<KeyT>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin, T, shift,
-                                   (uint32_t)sort_tiles, B.counts);
-                tm.end();
+                if (!(hist_q && ps_all == 0)) {   // the first pass's histogram came with quantise
+                    tm.begin("sort_hist", T * sizeof(KeyT));
+                    hipLaunchKernelGGL((k_sort_hist<KeyT>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin, T,
+                                       shift, (uint32_t)sort_tiles, B.counts);
+                    tm.end();
+                }
                 tm.begin("sort_scan", 2ull * 256 * sort_tiles * 4);
                 scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status + (uint64_t)ps_all * lb_sort,
                          B.ticket + ps_all, A.err, nullptr, false);

@@ -310,12 +310,24 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
                                                     KeyT* __restrict__ key_out, RelT* __restrict__ rel_out,
                                                     uint64_t* __restrict__ corr_idx, double* __restrict__ corr_val,
                                                     uint32_t* __restrict__ tile_ncorr, uint32_t nwords,
-                                                    uint64_t total_pts) {
+                                                    uint64_t total_pts, uint32_t* __restrict__ counts,
+                                                    uint32_t sort_ntiles) {
     const uint32_t tile = blockIdx.x;
     const uint32_t f = upper_find(tile_begin, nframes, tile);
     const FrameDev& F = fr[f];
     const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
     const uint64_t end = start + CH_TILE < F.n ? start + CH_TILE : F.n;
+    // With counts set, the first radix pass's digit histogram (low byte of
+    // key word 0) is gathered here: the tile's points fall in at most two
+    // sort tiles of CH_TILE global positions, counted in LDS and added to
+    // counts[digit * sort_ntiles + sort tile] (cleared before the launch).
+    __shared__ uint32_t s_h[2][256];
+    const uint64_t st0 = (F.pt_off + start) / CH_TILE;
+    if (counts) {
+        s_h[0][threadIdx.x] = 0;
+        s_h[1][threadIdx.x] = 0;
+        __syncthreads();
+    }
     const double denom = F.denom, eb = F.eb;
     const uint64_t p = F.p, magic = F.p_magic;
     const uint32_t l = F.p_l;
@@ -379,7 +391,19 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
                 for (int w = 0; w < D; ++w)
                     if ((uint32_t)w < nwords) key_out[(uint64_t)w * total_pts + F.pt_off + i] = kw[w];
             }
+            if (counts) {
+                const uint32_t digit = (uint32_t)(nwords == 1 ? key : kw[0]) & 0xffu;
+                atomicAdd(&s_h[(uint32_t)((F.pt_off + i) / CH_TILE - st0)][digit], 1u);
+            }
         }
+        }
+    }
+    if (counts) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const uint32_t v = s_h[s][threadIdx.x];
+            if (v) atomicAdd(&counts[(uint64_t)threadIdx.x * sort_ntiles + st0 + s], v);
         }
     }
     // exact corrections, in element order (j, thread, d); rare, so the

@@ -190,38 +190,61 @@ __global__ __launch_bounds__(CH_WG) void k_bbox(const T* __restrict__ pts, const
     const FrameDev& F = fr[f];
     const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
     const uint64_t end = start + CH_TILE < F.n ? start + CH_TILE : F.n;
-    uint64_t mn[D], mx[D], zp[D], zn[D];
+    // Order-preserving keys of the input width (float32 -> double is exact
+    // and keeps the order, -0 < +0 included), zero positions relative to the
+    // tile; both widened once per thread before the reduction.
+    using BK = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+    using BU = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type;
+    constexpr BK BK_MAX = ~BK(0);
+    BK kmn[D], kmx[D];
+    uint32_t rzp[D], rzn[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) { mn[d] = ~0ull; mx[d] = 0; zp[d] = ~0ull; zn[d] = ~0ull; }
+    for (int d = 0; d < D; ++d) { kmn[d] = BK_MAX; kmx[d] = 0; rzp[d] = ~0u; rzn[d] = ~0u; }
     bool nonfinite = false;
     // every coordinate of the tile is loaded before the first is used
     T xs[CH_ITEMS][D];
 #pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
+        // unconditional loads at a valid point (the tile's first)
+        const uint64_t ic = i < end ? i : start;
+        const T* p = SOA ? pts + F.pt_off * D + ic : pts + (F.pt_off + ic) * D;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            // unconditional load at a valid point (the tile's first)
-            const uint64_t ic = i < end ? i : start;
-            xs[j][d] = pts[SOA ? F.pt_off * D + (uint64_t)d * F.n + ic : (F.pt_off + ic) * D + d];
-        }
+        for (int d = 0; d < D; ++d) xs[j][d] = SOA ? p[(uint64_t)d * F.n] : p[d];
     }
 #pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
-        const uint64_t i = start + (uint64_t)j * CH_WG + threadIdx.x;
-        if (i >= end) continue;
+        const uint32_t rel = (uint32_t)j * CH_WG + threadIdx.x;   // point start + rel
+        if (start + rel >= end) continue;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const double x = (double)xs[j][d];
+            const T x = xs[j][d];
             nonfinite |= !isfinite(x);
-            const uint64_t k = okey(x);
-            mn[d] = k < mn[d] ? k : mn[d];
-            mx[d] = k > mx[d] ? k : mx[d];
-            if (x == 0.0) {
-                if (signbit(x)) zn[d] = i < zn[d] ? i : zn[d];
-                else zp[d] = i < zp[d] ? i : zp[d];
+            BU b;
+            __builtin_memcpy(&b, &x, sizeof(b));
+            const BK k = (b >> (8 * sizeof(BU) - 1)) ? BK(~b) : BK(b | (BU(1) << (8 * sizeof(BU) - 1)));
+            kmn[d] = k < kmn[d] ? k : kmn[d];
+            kmx[d] = k > kmx[d] ? k : kmx[d];
+            if (x == T(0)) {
+                if (signbit(x)) rzn[d] = rel < rzn[d] ? rel : rzn[d];
+                else rzp[d] = rel < rzp[d] ? rel : rzp[d];
             }
         }
+    }
+    // widen: key of the input width -> the double's key
+    auto widen = [](BK k) -> uint64_t {
+        const BU b = (k >> (8 * sizeof(BK) - 1)) ? BU(k & ~(BK(1) << (8 * sizeof(BK) - 1))) : BU(~k);
+        T x;
+        __builtin_memcpy(&x, &b, sizeof(x));
+        return okey((double)x);
+    };
+    uint64_t mn[D], mx[D], zp[D], zn[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        mn[d] = kmn[d] == BK_MAX ? ~0ull : widen(kmn[d]);
+        mx[d] = kmx[d] == 0 ? 0ull : widen(kmx[d]);
+        zp[d] = rzp[d] == ~0u ? ~0ull : start + rzp[d];
+        zn[d] = rzn[d] == ~0u ? ~0ull : start + rzn[d];
     }
     __shared__ uint64_t s_red[4][CH_MAXD][4];
     const uint32_t w = threadIdx.x >> 6;

@@ -520,7 +520,8 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
         et.pack[c] = tm.begin("pack");
         hipLaunchKernelGGL((k_pack_fixed<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
                            x.out);
-        hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(ntiles1k), dim3(CH_WG), 0, st, (const V*)v, c, F,
+        hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(std::min<uint32_t>(ntiles1k, PACK_TAG_WGS)), dim3(CH_WG), 0, st,
+                           (const V*)v, c, F,
                            x.tag_tiles, x.ci, x.status1k + (uint64_t)c * (ntiles1k + 1), x.tickets + c, err, x.head,
                            x.tail, x.head_pos, x.out);
         hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG)), dim3(CH_WG), 0, st, x.head, x.tail, x.head_pos,

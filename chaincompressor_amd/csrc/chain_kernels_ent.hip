@@ -1129,7 +1129,8 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
 // over each frame's tiles, bits assembled in LDS; the partial bytes at the
 // two ends of a tile go to side arrays and are merged by k_tag_seams.
 // tag_tiles[f] (F+1 entries) counts the tiles of the frames whose component
-// is Tagged; only that many workgroups take a ticket.
+// is Tagged. Workgroups take tiles by ticket, in order, until they run out.
+#define PACK_TAG_WGS 2048u
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ vals, int c, uint32_t F,
                                                        const uint32_t* __restrict__ tag_tiles,
@@ -1141,10 +1142,16 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_words[(TAG_TILE * 4 * 63) / 32 + 4];
     const uint32_t* tile_begin = tag_tiles + c * (F + 1);
-    if (blockIdx.x >= tile_begin[F]) return;
+    // The grid is sized for the largest possible tile count, capped at
+    // PACK_TAG_WGS workgroups; each workgroup takes tiles by ticket until
+    // they run out, so an unused component costs one short launch.
+    const uint32_t total = tile_begin[F];
+    if (blockIdx.x >= total) return;
+    for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
+    if (tile >= total) break;
     const uint32_t f = upper_find(tile_begin, F, tile);
     const CompInst& C = ci[c * F + f];
     if (threadIdx.x == 0) head_pos[tile] = ~0ull;
@@ -1217,6 +1224,8 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
         } else {
             out[C.payload_off + gb0 + Lb] = byte;
         }
+    }
+    __syncthreads();   // LDS reused by the next tile
     }
 }
 

@@ -628,9 +628,11 @@ __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_
 
 struct EncLds {
     // per symbol: {1024 * f (bytes are emitted while x >= this), cum, m,
-    //              (M - f) | (l - 1) << 24 | (f == 1) << 29}
-    uint4 tab[256];
+    //              (M - f) | (l - 1) << 24 | (f == 1) << 29};
+    // entry 256 is an idle step: no byte, and the update leaves x as it is
+    uint4 tab[257];
 };
+#define ENC_IDLE_SYM 256u
 
 // Raw input of steps t-1, ..., t-g of this lane (steps are coded
 // backwards): the value, or for a tags section the DIMS values of the record.
@@ -652,10 +654,11 @@ __device__ inline void enc_group_load(const ValT* __restrict__ base, uint32_t Sc
     }
 }
 
-// symbols of a loaded group; a missing step is 0xffffffff. A record's tag is
+// symbols of a loaded group; a missing step is ENC_IDLE_SYM (LDS table) or
+// 0xffffffff (global table). A record's tag is
 // max(1, bit length of its widest value) = max(1, bit length of the OR of its
 // values).
-template <typename ValT, int RD, bool TAGS>
+template <typename ValT, int RD, bool TAGS, bool SMALL>
 __device__ inline void enc_group_sym(const ValT (&raw)[ENC_GROUP][RD], uint32_t valid, uint32_t* sym) {
 #pragma unroll
     for (int j = 0; j < ENC_GROUP; ++j) {
@@ -670,7 +673,7 @@ __device__ inline void enc_group_sym(const ValT (&raw)[ENC_GROUP][RD], uint32_t 
                                                  : 32u - (uint32_t)__clz((uint32_t)o);
             v = b > 1u ? b : 1u;
         }
-        sym[j] = (valid >> j) & 1u ? v : 0xffffffffu;
+        sym[j] = (valid >> j) & 1u ? v : (SMALL ? ENC_IDLE_SYM : 0xffffffffu);
     }
 }
 
@@ -704,6 +707,7 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             lds.tab[b] = make_uint4(1024u * f, (uint32_t)(e >> 32), m,
                                     (M - f) | ((l ? l - 1 : 0u) << 24) | ((f == 1 ? 1u : 0u) << 29));
         }
+    if (SMALL && lane == 0) lds.tab[ENC_IDLE_SYM] = make_uint4(ENC_XMAX_IDLE, 0u, 0u, 0u);
     __syncthreads();
     const uint64_t first = (uint64_t)(chunk - S.chunk_base) * S.chunk_symbols;
     const uint32_t Sc = (uint32_t)(S.count - first < S.chunk_symbols ? S.count - first : S.chunk_symbols);
@@ -722,7 +726,7 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
     uint32_t t = T;
     uint32_t g = t < ENC_GROUP ? t : ENC_GROUP;
     enc_group_load<ValT, RD>(base, Sc, L, lane, t, g, raw, valid);
-    enc_group_sym<ValT, RD, TAGS>(raw, valid, cur);
+    enc_group_sym<ValT, RD, TAGS, SMALL>(raw, valid, cur);
     while (t > 0) {
         // table entries of this group first: a wait for them is then no
         // wait for the next group's loads, issued after them
@@ -730,13 +734,10 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
         uint64_t ent[ENC_GROUP];   // larger alphabets: (cum << 32) | f
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
-            const bool a = cur[j] != 0xffffffffu;
-            const uint32_t sj = a ? cur[j] : 0u;
             if (SMALL) {
-                te[j] = lds.tab[sj];
-                te[j].x = a ? te[j].x : ENC_XMAX_IDLE;
+                te[j] = lds.tab[cur[j]];   // an idle step reads the idle entry
             } else {
-                ent[j] = gpc[sj];
+                ent[j] = gpc[cur[j] != 0xffffffffu ? cur[j] : 0u];
             }
         }
         const uint32_t tn = t - g;
@@ -745,7 +746,7 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
 #pragma unroll
         for (int j = 0; j < ENC_GROUP; ++j) {
             if ((uint32_t)j >= g) break;
-            const bool active = cur[j] != 0xffffffffu;
+            const bool active = SMALL || cur[j] != 0xffffffffu;
             const uint32_t f = SMALL ? 0u : (uint32_t)ent[j];
             // bytes to emit: x < 2^30 <= 256^3 * 1024 * f, so at most three
             const uint32_t xmax = SMALL ? te[j].x : (active ? 1024u * f : ENC_XMAX_IDLE);
@@ -780,7 +781,7 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             flushed -= ENC_FLUSH;
             __syncthreads();
         }
-        if (tn > 0) enc_group_sym<ValT, RD, TAGS>(raw, valid, cur);
+        if (tn > 0) enc_group_sym<ValT, RD, TAGS, SMALL>(raw, valid, cur);
         t = tn;
         g = gn;
     }

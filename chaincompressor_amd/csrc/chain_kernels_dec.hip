@@ -80,7 +80,7 @@ struct DecChunk {
 };
 
 // up to DEC_WAVES consecutive chunks of one section, decoded by one workgroup
-#define DEC_WAVES 4
+#define DEC_WAVES 8
 struct DecJob {
     uint32_t sec, chunk0, nchunks, pad;
 };

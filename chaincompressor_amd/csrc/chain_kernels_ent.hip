@@ -620,15 +620,17 @@ __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_
 // with an exact reciprocal of every frequency, so the state update needs no
 // integer division: with l = ceil(log2 f) and m = ceil(2^(31+l) / f) < 2^32
 // (rcp31_* in chain_int.hpp), floor(x / f) = mulhi(x, m) >> (l - 1) for
-// l >= 1, and x for f = 1. The update x' = (x / f) * M + x % f + cum is
-// taken as x + (x / f) * (M - f) + cum, where x / f < 1024 and M - f < 2^20
-// make the product a 24-bit multiply.
+// l >= 1. The update x' = (x / f) * M + x % f + cum is taken as
+// x + (x / f) * (M - f) + cum, where x / f < 1024 and M - f < 2^20 make the
+// product a 24-bit multiply. For f = 1 the entry holds m = 2^32 - 1 and no
+// shift, so the quotient comes out as x - 1 (x >= 1 after renormalisation),
+// and cum + M - 1 in place of cum: x + (x - 1) * (M - 1) + cum + M - 1 = x * M + cum.
 #define ENC_GROUP 4
 #define ENC_XMAX_IDLE (1u << 30)   // above every state: an idle lane emits nothing
 
 struct EncLds {
     // per symbol: {1024 * f (bytes are emitted while x >= this), cum, m,
-    //              (M - f) | (l - 1) << 24 | (f == 1) << 29};
+    //              (M - f) | (l - 1) << 24}, f = 1 as described above;
     // entry 256 is an idle step: no byte, and the update leaves x as it is
     uint4 tab[257];
 };
@@ -702,10 +704,10 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
         for (uint32_t b = lane; b < S.alphabet; b += 64) {
             const uint64_t e = gpc[b];
             const uint32_t f = (uint32_t)e;
-            const uint32_t l = f ? rcp31_shift(f) : 0u;
-            const uint32_t m = f ? (uint32_t)rcp31_mult(f, l) : 0u;
-            lds.tab[b] = make_uint4(1024u * f, (uint32_t)(e >> 32), m,
-                                    (M - f) | ((l ? l - 1 : 0u) << 24) | ((f == 1 ? 1u : 0u) << 29));
+            const uint32_t l = f > 1 ? rcp31_shift(f) : 1u;
+            const uint32_t m = f > 1 ? (uint32_t)rcp31_mult(f, l) : (f == 1 ? 0xffffffffu : 0u);
+            const uint32_t cum = (uint32_t)(e >> 32) + (f == 1 ? M - 1 : 0u);
+            lds.tab[b] = make_uint4(1024u * f, cum, m, (M - f) | ((l - 1) << 24));
         }
     if (SMALL && lane == 0) lds.tab[ENC_IDLE_SYM] = make_uint4(ENC_XMAX_IDLE, 0u, 0u, 0u);
     __syncthreads();
@@ -750,20 +752,22 @@ __device__ inline void enc_chunk(const ValT* __restrict__ vals, uint32_t chunk, 
             const uint32_t f = SMALL ? 0u : (uint32_t)ent[j];
             // bytes to emit: x < 2^30 <= 256^3 * 1024 * f, so at most three
             const uint32_t xmax = SMALL ? te[j].x : (active ? 1024u * f : ENC_XMAX_IDLE);
-            const uint32_t kb = (uint32_t)(x >= xmax) + (uint32_t)((x >> 8) >= xmax) + (uint32_t)((x >> 16) >= xmax);
+            // byte i is emitted when x >> 8i >= xmax; the three compares are
+            // also the lane masks of the three planes
+            const bool e0 = x >= xmax, e1 = (x >> 8) >= xmax, e2 = (x >> 16) >= xmax;
+            const uint32_t kb = (uint32_t)e0 + (uint32_t)e1 + (uint32_t)e2;
             const uint32_t xx = x >> (8 * kb);
-            const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
+            const uint64_t m0 = __ballot(e0), m1 = __ballot(e1), m2 = __ballot(e2);
             const uint32_t c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2);
             pos -= c0 + c1 + c2;
             // planes 1 and 2 are rare: their lanes (and whole blocks) are skipped
-            if (kb > 0) ring[(pos + mbcnt64(m0)) & (ENC_RING - 1)] = (uint8_t)(x >> (8 * (kb - 1)));
-            if (kb > 1) ring[(pos + c0 + mbcnt64(m1)) & (ENC_RING - 1)] = (uint8_t)(x >> (8 * (kb - 2)));
-            if (kb > 2) ring[(pos + c0 + c1 + mbcnt64(m2)) & (ENC_RING - 1)] = (uint8_t)x;
+            if (e0) ring[(pos + mbcnt64(m0)) & (ENC_RING - 1)] = (uint8_t)(x >> (8 * (kb - 1)));
+            if (e1) ring[(pos + c0 + mbcnt64(m1)) & (ENC_RING - 1)] = (uint8_t)(x >> (8 * (kb - 2)));
+            if (e2) ring[(pos + c0 + c1 + mbcnt64(m2)) & (ENC_RING - 1)] = (uint8_t)x;
             uint32_t xn;
             if (SMALL) {
                 const uint32_t w = te[j].w;
-                uint32_t q = __umulhi(xx, te[j].z) >> ((w >> 24) & 31u);
-                q = (w >> 29) & 1u ? xx : q;
+                const uint32_t q = __umulhi(xx, te[j].z) >> ((w >> 24) & 31u);
                 xn = xx + __umul24(q, w & 0xffffffu) + te[j].y;
             } else {
                 const uint32_t q = xx / (active ? f : 1u);

@@ -432,6 +432,7 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                           uint32_t* err, X& x) {
     const uint32_t ninst = 5 * F, nsec = 10 * F;
     hipLaunchKernelGGL(k_comp_init, dim3(ninst), dim3(CH_WG), 0, st, fdev, fo, F, D, x.ci, x.vhist, x.thist);
+    hipLaunchKernelGGL(k_comp_tiles, dim3(5), dim3(CH_WG), 0, st, x.ci, F, x.ctb);
     // every component: a typed view and its launches
     auto each_comp = [&](auto&& fn) {
         with_dims(D, [&](auto dd) {
@@ -444,34 +445,31 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
             else fn(4, TypeTag<uint32_t>{}, IC<DD>{}, rel);
         });
     };
-    // statistics slots: the most any frame has decides the reducer's grid
+    // statistics slots: a frame has at most min(G, its tiles) + 1 of them (a
+    // component never has more tiles in a frame than the frame has point
+    // tiles); the most any frame can have decides the reducer's grid
     const uint32_t stats_grid = std::min<uint32_t>(ntiles, CS_GRID);
     uint32_t reduce_y = 1;
-    if (ntiles) {
-        const uint32_t per = (uint32_t)cdiv(ntiles, stats_grid);
-        for (uint32_t f = 0; f < F; ++f) {
-            const uint32_t t0 = tb_host[f], t1 = tb_host[f + 1];
-            if (t0 == t1) continue;
-            const uint32_t lo = (uint32_t)cdiv(t0, per), hi = std::min((uint32_t)cdiv(t1, per), stats_grid);
-            const uint32_t cnt = (hi > lo ? hi - lo : 0) + (t0 % per ? 1u : 0u);
-            reduce_y = std::max(reduce_y, (uint32_t)cdiv(cnt, CS_REDUCE_SLOTS));
-        }
+    for (uint32_t f = 0; f < F; ++f) {
+        const uint32_t cnt = std::min<uint32_t>(tb_host[f + 1] - tb_host[f], stats_grid) + 1;
+        reduce_y = std::max(reduce_y, (uint32_t)cdiv(cnt, CS_REDUCE_SLOTS));
     }
     each_comp([&](int c, auto vt, auto dd, const void* v) {
         using V = typename decltype(vt)::type;
         constexpr int DC = decltype(dd)::value;
         et.stats[c] = tm.begin("comp_stats");
         if (ntiles) {
-            hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(stats_grid), dim3(CH_WG), 0, st, (const V*)v, c, F, tb,
-                               ntiles, x.ci, x.slots + (uint64_t)c * (stats_grid + F), x.vhist, x.thist);
+            hipLaunchKernelGGL((k_comp_stats<V, DC>), dim3(stats_grid), dim3(CH_WG), 0, st, (const V*)v, c, F,
+                               x.ctb + (uint64_t)c * (F + 1), x.ci, x.slots + (uint64_t)c * (stats_grid + F),
+                               x.vhist, x.thist);
         }
         tm.end();
     });
     if (ntiles) {
         // one launch combines every component's slots
         tm.begin("comp_stats");
-        hipLaunchKernelGGL(k_comp_reduce, dim3(F, reduce_y, 5), dim3(CH_WG), 0, st, x.slots, stats_grid, ntiles, tb,
-                           F, x.ci, x.vhist, x.thist);
+        hipLaunchKernelGGL(k_comp_reduce, dim3(F, reduce_y, 5), dim3(CH_WG), 0, st, x.slots, stats_grid, x.ctb, F,
+                           x.ci, x.vhist, x.thist);
         tm.end();
     }
     tm.begin("rans_table", 5ull * F * (RANS_VALUE_LIMIT + RANS_TAG_LIMIT) * 4 + 10ull * F * RANS_VALUE_LIMIT * 8);
@@ -753,6 +751,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             uint32_t* vhist;
             uint32_t* thist;
             CompSlot* slots;
+            uint32_t* ctb;   // per component: first statistics tile of each frame, then the total
             uint64_t* pc;
             uint8_t* tbytes;
             uint32_t* comp_chunk;
@@ -776,6 +775,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.vhist = c.take<uint32_t>(5ull * F * RANS_VALUE_LIMIT);
         x.thist = c.take<uint32_t>(5ull * F * RANS_TAG_LIMIT);
         x.slots = c.take<CompSlot>(5ull * (CS_GRID + F));
+        x.ctb = c.take<uint32_t>(5ull * (F + 1));
         x.pc = c.take<uint64_t>(10ull * F * RANS_VALUE_LIMIT);
         x.tbytes = c.take<uint8_t>(10ull * F * TABLE_BYTES_MAX);
         x.comp_chunk = c.take<uint32_t>(8);

@@ -157,11 +157,33 @@ struct CompSlot {
 };
 static_assert(sizeof(CompSlot) == 384, "slot layout");
 
+// Tiles of a component: frame f of component c holds ceil(n / CH_TILE) tiles
+// of its own records, so a component with fewer records than points (diffs,
+// keeps, choices) spreads only its own tiles over the workgroups.
+// ctb[c * (F + 1) + f] = first tile of frame f, [c * (F + 1) + F] = total.
+__global__ __launch_bounds__(CH_WG) void k_comp_tiles(const CompInst* __restrict__ ci, uint32_t F,
+                                                      uint32_t* __restrict__ ctb) {
+    __shared__ uint32_t s_tmp[4];
+    const uint32_t c = blockIdx.x;
+    uint32_t* out = ctb + (uint64_t)c * (F + 1);
+    uint32_t run = 0;
+    for (uint32_t f0 = 0; f0 < F; f0 += CH_WG) {
+        const uint32_t f = f0 + threadIdx.x;
+        const uint32_t v = f < F ? (uint32_t)((ci[(uint64_t)c * F + f].n + CH_TILE - 1) / CH_TILE) : 0u;
+        uint32_t tot;
+        const uint32_t pre = block_excl_scan(v, s_tmp, &tot);
+        if (f < F) out[f] = run + pre;
+        run += tot;
+    }
+    if (threadIdx.x == 0) out[F] = run;
+}
+
 // slot of the (workgroup b, frame f) pair: b for the frame of b's first tile,
-// G + f for a frame that starts inside b's range (at most one workgroup)
+// G + f for a frame that starts inside b's range (at most one workgroup).
+// tile_begin is the component's tile prefix (k_comp_tiles).
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ vals, int c, uint32_t F,
-                                                      const uint32_t* __restrict__ tile_begin, uint32_t ntiles,
+                                                      const uint32_t* __restrict__ tile_begin,
                                                       const CompInst* __restrict__ ci, CompSlot* __restrict__ slots,
                                                       uint32_t* __restrict__ vhist, uint32_t* __restrict__ thist) {
     static_assert(CH_ITEMS * 4 * CS_FOLD_TILES < 65536u, "16-bit private counters overflow");
@@ -172,6 +194,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
     __shared__ uint32_t s_pv[CS_PRIV_ROWS][CH_WG], s_pt[CS_PRIV_ROWS][CH_WG];
     __shared__ uint64_t s_red[4][2 * CH_MAXD + 2];
     const uint32_t tid = threadIdx.x;
+    const uint32_t ntiles = tile_begin[F];
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t tb0 = blockIdx.x * per, tb1 = tb0 + per < ntiles ? tb0 + per : ntiles;
     if (tb0 >= tb1) return;
@@ -364,12 +387,14 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
 // that instance's statistics. Frame f's slots are G + f when it starts inside a workgroup's
 // range, then the workgroups whose first tile lies in f.
 __global__ __launch_bounds__(CH_WG) void k_comp_reduce(const CompSlot* __restrict__ all_slots, uint32_t G,
-                                                       uint32_t ntiles, const uint32_t* __restrict__ tile_begin,
+                                                       const uint32_t* __restrict__ ctb,
                                                        uint32_t F, CompInst* __restrict__ ci,
                                                        uint32_t* __restrict__ vhist, uint32_t* __restrict__ thist) {
     __shared__ uint32_t s_sum[4][2 * CS_PRIV_BINS];
     const int c = (int)blockIdx.z;   // component; its slots follow the previous components'
     const CompSlot* __restrict__ slots = all_slots + (uint64_t)c * (G + F);
+    const uint32_t* __restrict__ tile_begin = ctb + (uint64_t)c * (F + 1);   // the component's tiles
+    const uint32_t ntiles = tile_begin[F];
     const uint32_t f = blockIdx.x;
     const uint32_t t0 = tile_begin[f], t1 = tile_begin[f + 1];
     if (t0 == t1) return;

@@ -164,6 +164,28 @@ struct Staging {
         std::memcpy(q, src, n);
         HIPCHK(hipMemcpyAsync(dst, q, n, hipMemcpyHostToDevice, st));
     }
+    // tables whose device buffers lie in one range: laid out as on the
+    // device in one staging block and sent with one copy (the padding
+    // between buffers carries no data)
+    struct Piece {
+        void* dst;
+        const void* src;
+        size_t n;
+    };
+    void h2d_range(hipStream_t st, const std::vector<Piece>& ps) {
+        char *lo = nullptr, *hi = nullptr;
+        for (const Piece& p : ps) {
+            if (!p.n) continue;
+            char* d = static_cast<char*>(p.dst);
+            lo = (!lo || d < lo) ? d : lo;
+            hi = (!hi || d + p.n > hi) ? d + p.n : hi;
+        }
+        if (!lo) return;
+        char* q = take((size_t)(hi - lo));
+        for (const Piece& p : ps)
+            if (p.n) std::memcpy(q + (static_cast<char*>(p.dst) - lo), p.src, p.n);
+        HIPCHK(hipMemcpyAsync(lo, q, (size_t)(hi - lo), hipMemcpyHostToDevice, st));
+    }
     // lands in dst at the next finish()
     void d2h(hipStream_t st, void* dst, const void* src, size_t n) {
         if (!n) return;
@@ -1605,6 +1627,14 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.raws = c.take<DecRaw>(P.raws.size());
         x.tagged = c.take<DecTagged>(P.tagged.size());
         x.cum = c.take<uint32_t>(P.cum.size());
+        // the tables above and these frame arrays are uploaded as one range
+        x.ptb = c.take<uint32_t>(F + 1);
+        x.rtb = c.take<uint32_t>(F + 1);
+        x.ptb32 = c.take<uint32_t>(F + 1);
+        x.sbr = c.take<uint64_t>(F);
+        x.snr = c.take<uint64_t>(F);
+        x.sbp = c.take<uint64_t>(F);
+        x.snp = c.take<uint64_t>(F);
         x.idrel = c.take<uint32_t>(T);
         x.rel = c.take<uint8_t>(T * D * (rel8 ? 1 : 4));
         x.diff = c.take<uint64_t>((P.ND + 1) * D);   // one spare record each: unconditional loads
@@ -1617,13 +1647,6 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.zoff = c.take<uint32_t>(nrt + 1);
         x.ec = c.take<uint32_t>(npt + 1);
         x.eoff = c.take<uint32_t>(npt + 1);
-        x.ptb = c.take<uint32_t>(F + 1);
-        x.rtb = c.take<uint32_t>(F + 1);
-        x.ptb32 = c.take<uint32_t>(F + 1);
-        x.sbr = c.take<uint64_t>(F);
-        x.snr = c.take<uint64_t>(F);
-        x.sbp = c.take<uint64_t>(F);
-        x.snp = c.take<uint64_t>(F);
         x.status = c.take<uint64_t>(std::max<uint64_t>({P.tag_tiles, cdiv(npt + 1, CH_TILE), cdiv(nrt + 1, CH_TILE)}) + 1);
         x.flags = c.take<uint32_t>(nrt + 1);
         x.aggv = c.take<uint64_t>((uint64_t)(nrt + 1) * 2 * CH_MAXD);
@@ -1645,7 +1668,6 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     auto X = carve(c1);
     if (res.d_pool) X.in = const_cast<uint8_t*>(res.d_pool);
     Staging& sg = e->stage;
-    auto h2d = [&](void* dst, const void* src, size_t n) { sg.h2d(st, dst, src, n); };
     // algorithmic bytes of every timed region, from the plan
     const uint64_t comp_es[5] = {4, 8, 8, 1, rel8 ? 1u : 4u};
     auto es_of = [&](int32_t comp) -> uint64_t { return comp < 0 ? 1 : comp_es[comp]; };
@@ -1662,21 +1684,21 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     if (!res.d_pool)
         for (uint32_t k = 0; k < nstreams; ++k)
             if (lens[k]) HIPCHK(hipMemcpyAsync(X.in + in_off[k], bytes[k], lens[k], hipMemcpyHostToDevice, st));
-    h2d(X.fr, P.frames.data(), F * sizeof(DecFrame));
-    h2d(X.secs, P.secs.data(), P.secs.size() * sizeof(DecSection));
-    h2d(X.chunks, P.chunks.data(), P.chunks.size() * sizeof(DecChunk));
-    h2d(X.fixed, P.fixed.data(), P.fixed.size() * sizeof(DecFixed));
-    h2d(X.raws, P.raws.data(), P.raws.size() * sizeof(DecRaw));
-    h2d(X.tagged, P.tagged.data(), P.tagged.size() * sizeof(DecTagged));
-    h2d(X.jobs, P.jobs.data(), P.jobs.size() * sizeof(DecJob));
-    h2d(X.cum, P.cum.data(), P.cum.size() * 4);
-    h2d(X.ptb, ptb.data(), (F + 1) * 4);
-    h2d(X.rtb, rtb.data(), (F + 1) * 4);
-    h2d(X.ptb32, ptb32.data(), (F + 1) * 4);
-    h2d(X.sbr, seg_base_rec.data(), F * 8);
-    h2d(X.snr, seg_n_rec.data(), F * 8);
-    h2d(X.sbp, seg_base_pt.data(), F * 8);
-    h2d(X.snp, seg_n_pt.data(), F * 8);
+    sg.h2d_range(st, {{X.fr, P.frames.data(), F * sizeof(DecFrame)},
+                      {X.secs, P.secs.data(), P.secs.size() * sizeof(DecSection)},
+                      {X.chunks, P.chunks.data(), P.chunks.size() * sizeof(DecChunk)},
+                      {X.fixed, P.fixed.data(), P.fixed.size() * sizeof(DecFixed)},
+                      {X.raws, P.raws.data(), P.raws.size() * sizeof(DecRaw)},
+                      {X.tagged, P.tagged.data(), P.tagged.size() * sizeof(DecTagged)},
+                      {X.jobs, P.jobs.data(), P.jobs.size() * sizeof(DecJob)},
+                      {X.cum, P.cum.data(), P.cum.size() * 4},
+                      {X.ptb, ptb.data(), (F + 1) * 4},
+                      {X.rtb, rtb.data(), (F + 1) * 4},
+                      {X.ptb32, ptb32.data(), (F + 1) * 4},
+                      {X.sbr, seg_base_rec.data(), F * 8},
+                      {X.snr, seg_n_rec.data(), F * 8},
+                      {X.sbp, seg_base_pt.data(), F * 8},
+                      {X.snp, seg_n_pt.data(), F * 8}});
     fill32(st, X.err, 4, 0);
     tm.end();
 

@@ -186,6 +186,23 @@ struct Staging {
             if (p.n) std::memcpy(q + (static_cast<char*>(p.dst) - lo), p.src, p.n);
         HIPCHK(hipMemcpyAsync(lo, q, (size_t)(hi - lo), hipMemcpyHostToDevice, st));
     }
+    // device buffers in one range, read with one copy; each piece lands in
+    // its host dst at the next finish() (dst and src of a Piece are host
+    // and device here)
+    void d2h_range(hipStream_t st, const std::vector<Piece>& ps) {
+        const char *lo = nullptr, *hi = nullptr;
+        for (const Piece& p : ps) {
+            if (!p.n) continue;
+            const char* s = static_cast<const char*>(p.src);
+            lo = (!lo || s < lo) ? s : lo;
+            hi = (!hi || s + p.n > hi) ? s + p.n : hi;
+        }
+        if (!lo) return;
+        char* q = take((size_t)(hi - lo));
+        HIPCHK(hipMemcpyAsync(q, lo, (size_t)(hi - lo), hipMemcpyDeviceToHost, st));
+        for (const Piece& p : ps)
+            if (p.n) pend.push_back({p.dst, q + (static_cast<const char*>(p.src) - lo), p.n});
+    }
     // lands in dst at the next finish()
     void d2h(hipStream_t st, void* dst, const void* src, size_t n) {
         if (!n) return;
@@ -794,6 +811,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         } x;
         x.ci = c.take<CompInst>(5 * F);
         x.sec = c.take<Section>(10 * F);
+        // read back with the two above as one range
+        x.out_off = c.take<uint64_t>(F);
+        x.out_size = c.take<uint64_t>(F);
+        x.total = c.take<uint64_t>(1);
         x.vhist = c.take<uint32_t>(5ull * F * RANS_VALUE_LIMIT);
         x.thist = c.take<uint32_t>(5ull * F * RANS_TAG_LIMIT);
         x.slots = c.take<CompSlot>(5ull * (CS_GRID + F));
@@ -805,9 +826,6 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.chunk_len = c.take<uint32_t>(chunk_bound);
         x.chunk_pos = c.take<uint32_t>(chunk_bound);
         x.chunk_off = c.take<uint32_t>(chunk_bound);
-        x.out_off = c.take<uint64_t>(F);
-        x.out_size = c.take<uint64_t>(F);
-        x.total = c.take<uint64_t>(1);
         x.head = c.take<uint8_t>(ntiles1k);
         x.tail = c.take<uint8_t>(ntiles1k);
         x.head_pos = c.take<uint64_t>(ntiles1k);
@@ -846,9 +864,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     std::vector<uint32_t> ptb32(F + 1);
     for (uint32_t f = 0; f <= F; ++f) ptb32[f] = (uint32_t)pt_off[f];
     Staging& sg = e->stage;
-    sg.h2d(st, A.fdev, fdev.data(), F * sizeof(FrameDev));
-    sg.h2d(st, A.tb, tile_begin.data(), (F + 1) * 4);
-    sg.h2d(st, A.ptb, ptb32.data(), (F + 1) * 4);
+    sg.h2d_range(st, {{A.fdev, fdev.data(), F * sizeof(FrameDev)},
+                      {A.tb, tile_begin.data(), (F + 1) * 4},
+                      {A.ptb, ptb32.data(), (F + 1) * 4}});
     fill32(st, A.bad, F, 0);
     fill32(st, A.err, 4, 0);
     tm.begin("bbox", T * D * es);
@@ -863,8 +881,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     tm.end();
     std::vector<uint64_t> box((size_t)F * D * 4);
     std::vector<uint32_t> bad(F);
-    sg.d2h(st, box.data(), A.box, box.size() * 8);
-    sg.d2h(st, bad.data(), A.bad, F * 4);
+    sg.d2h_range(st, {{box.data(), A.box, box.size() * 8}, {bad.data(), A.bad, F * 4}});
     sg.finish(st);
 
     // ---- plan: minima, quantised range, key layout (lcps_cpp.cpp:2072-2110)
@@ -1119,16 +1136,16 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         std::vector<uint64_t> offs(F), sizes(F);
         uint64_t total = 0;
         uint32_t errw[4];
-        sg.d2h(st, offs.data(), X.out_off, F * 8);
-        sg.d2h(st, sizes.data(), X.out_size, F * 8);
-        sg.d2h(st, &total, X.total, 8);
-        sg.d2h(st, errw, A.err, sizeof errw);
         std::vector<CompInst> ci(5 * F);
         std::vector<FrameOut> fo(F);
         std::vector<Section> secs(tm.on ? 10 * F : 0);
-        sg.d2h(st, ci.data(), X.ci, 5 * F * sizeof(CompInst));
+        sg.d2h_range(st, {{offs.data(), X.out_off, F * 8},
+                          {sizes.data(), X.out_size, F * 8},
+                          {&total, X.total, 8},
+                          {ci.data(), X.ci, 5 * F * sizeof(CompInst)},
+                          {secs.data(), X.sec, secs.size() * sizeof(Section)}});
+        sg.d2h(st, errw, A.err, sizeof errw);
         sg.d2h(st, fo.data(), B.fo, F * sizeof(FrameOut));
-        if (tm.on) sg.d2h(st, secs.data(), X.sec, 10 * F * sizeof(Section));
         sg.finish(st);
         if (tm.on) {
             // algorithmic bytes of the regions whose work depends on the data

@@ -282,13 +282,35 @@ __global__ __launch_bounds__(CH_WG) void k_bbox_finish(const uint64_t* __restric
     const uint32_t t0 = tile_begin[f], t1 = tile_begin[f + 1];
     __shared__ uint64_t s_red[4][CH_MAXD][4];
     const uint32_t w = threadIdx.x >> 6;
+    // four tiles per thread and round, all their partials loaded first; a
+    // tile past the end reads the round's first tile again, which min, max
+    // and first-zero leave unchanged
+    uint64_t ma[D], mb[D], mc[D], me[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) { ma[d] = ~0ull; mb[d] = 0; mc[d] = ~0ull; me[d] = ~0ull; }
+    for (uint32_t t = t0 + threadIdx.x; t < t1; t += 4 * CH_WG) {
+        uint64_t v[4][D][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t tt = t + u * CH_WG < t1 ? t + u * CH_WG : t;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[u][d][q] = part[((uint64_t)tt * D + d) * 4 + q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                ma[d] = v[u][d][0] < ma[d] ? v[u][d][0] : ma[d];
+                mb[d] = v[u][d][1] > mb[d] ? v[u][d][1] : mb[d];
+                mc[d] = v[u][d][2] < mc[d] ? v[u][d][2] : mc[d];
+                me[d] = v[u][d][3] < me[d] ? v[u][d][3] : me[d];
+            }
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        uint64_t a = ~0ull, b = 0, c = ~0ull, e = ~0ull;
-        for (uint32_t t = t0 + threadIdx.x; t < t1; t += CH_WG) {
-            const uint64_t* o = part + ((uint64_t)t * D + d) * 4;
-            a = o[0] < a ? o[0] : a; b = o[1] > b ? o[1] : b; c = o[2] < c ? o[2] : c; e = o[3] < e ? o[3] : e;
-        }
+        uint64_t a = ma[d], b = mb[d], c = mc[d], e = me[d];
         for (int k = 32; k >= 1; k >>= 1) {
             uint64_t ta = __shfl_xor(a, k, 64), tb = __shfl_xor(b, k, 64);
             uint64_t tc = __shfl_xor(c, k, 64), te = __shfl_xor(e, k, 64);
@@ -878,22 +900,39 @@ __global__ __launch_bounds__(CH_WG) void k_l2_stats(const uint64_t* __restrict__
     // so a thread's frame changes only at frame boundaries
     const uint64_t per = ((R + gridDim.x - 1) / gridDim.x + CH_WG - 1) / CH_WG * CH_WG;
     const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = b0 + per < R ? b0 + per : R;
-    for (uint64_t r = b0 + threadIdx.x; r < b1; r += CH_WG) {
-        const uint32_t f = frame_of_rec(bases, nframes, r);
-        if (f != cur) {
-            flush();
-            cur = f;
-        }
-        const bool has_prev = r > bases[f];
+    // four records per thread and round: the records and their predecessors
+    // are loaded first (unconditionally: past the end, the round's first
+    // record; record 0 stands in for its own predecessor)
+    for (uint64_t rr = b0 + threadIdx.x; rr < b1; rr += 4 * CH_WG) {
+        uint64_t cv[4][D], pv[4][D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const uint64_t v = blocks[r * D + d];
-            mn = v < mn ? v : mn;
-            mx = v > mx ? v : mx;
-            if (has_prev) {
-                const uint64_t u = blocks[(r - 1) * D + d];
-                const uint64_t ad = u > v ? u - v : v - u;
-                md = ad > md ? ad : md;
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t r = rr + (uint64_t)u * CH_WG;
+            const uint64_t rc = r < b1 ? r : rr, rp = rc ? rc - 1 : rc;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                cv[u][d] = blocks[rc * D + d];
+                pv[u][d] = blocks[rp * D + d];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t r = rr + (uint64_t)u * CH_WG;
+            if (r >= b1) continue;
+            const uint32_t f = frame_of_rec(bases, nframes, r);
+            if (f != cur) {
+                flush();
+                cur = f;
+            }
+            const bool has_prev = r > bases[f];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint64_t v = cv[u][d];
+                mn = v < mn ? v : mn;
+                mx = v > mx ? v : mx;
+                const uint64_t w = pv[u][d];
+                const uint64_t ad = w > v ? w - v : v - w;
+                md = has_prev && ad > md ? ad : md;
             }
         }
     }

@@ -1082,7 +1082,21 @@ __global__ __launch_bounds__(CH_WG) void k_copy_chunks(const CompInst* __restric
     for (uint32_t k = threadIdx.x; k < nstate; k += CH_WG) dst[k] = src[k];
     const uint8_t* st = src + CHUNK_STATE_BYTES + chunk_pos[chunk];
     const uint32_t len = chunk_len[chunk];
-    for (uint32_t k = threadIdx.x; k < len; k += CH_WG) dst[nstate + k] = st[k];
+    // eight bytes per thread and round, all loaded before any is stored (a
+    // byte past the end reads the round's first byte and is not stored)
+    for (uint32_t k = threadIdx.x; k < len; k += 8 * CH_WG) {
+        uint8_t b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t i = k + u * CH_WG;
+            b[u] = st[i < len ? i : k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t i = k + u * CH_WG;
+            if (i < len) dst[nstate + i] = b[u];
+        }
+    }
 }
 
 // Fixed payloads, MSB-first, byte-padded per dimension. Eight consecutive

@@ -1754,10 +1754,17 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         fill32(st, X.ticket, 1, 0);
         tm.end();
         tm.begin("l2_inverse", P.R + 8ull * D * (P.ND + P.NK) + 8ull * D * P.R);
+        // 32-bit scan arithmetic when every frame's wrap range is at most 2^31
+        uint64_t max_range = 0;
+        for (const DecFrame& fd : P.frames) max_range = std::max(max_range, fd.range);
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
-            hipLaunchKernelGGL((k_dec_l2<DD>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice, X.diff,
-                               X.keep, X.zoff, X.flags, X.aggv, X.ticket, X.compact, X.err);
+            if (max_range <= (1ull << 31))
+                hipLaunchKernelGGL((k_dec_l2<DD, uint32_t>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice,
+                                   X.diff, X.keep, X.zoff, X.flags, X.aggv, X.ticket, X.compact, X.err);
+            else
+                hipLaunchKernelGGL((k_dec_l2<DD, uint64_t>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice,
+                                   X.diff, X.keep, X.zoff, X.flags, X.aggv, X.ticket, X.compact, X.err);
         });
         tm.end();
     }

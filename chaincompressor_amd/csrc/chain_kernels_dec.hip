@@ -544,19 +544,23 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tile_zero_counts(const uint8_t* _
 // element = (reset, value): keep records reset to keep - wrap_min, diff
 // records add (-correction) mod range. (r1,v1)+(r2,v2) = (r1|r2, r2 ? v2 : v1+v2 mod R)
 
+// V is the value type of the scan: u32 when every range of the launch is at
+// most 2^31 (sums of two values stay below 2^32), else u64 (ranges < 2^63)
+template <typename V>
 struct Agg {
     uint32_t reset;
-    uint64_t v[CH_MAXD];
+    V v[CH_MAXD];
 };
 
-__device__ inline uint64_t addmod(uint64_t a, uint64_t b, uint64_t R) {
-    // a, b < R < 2^63
-    const uint64_t s = a + b;
+template <typename V>
+__device__ inline V addmod(V a, V b, V R) {
+    // a, b < R
+    const V s = a + b;
     return s >= R ? s - R : s;
 }
 
-template <int D>
-__device__ inline void agg_combine(Agg& left, const Agg& right, uint64_t R) {   // left = left (+) right
+template <int D, typename V>
+__device__ inline void agg_combine(Agg<V>& left, const Agg<V>& right, V R) {   // left = left (+) right
     if (right.reset) {
         left = right;
     } else {
@@ -571,23 +575,23 @@ __device__ inline void agg_combine(Agg& left, const Agg& right, uint64_t R) {   
 // combines the 64 segment aggregates into the tile aggregate and does the
 // cross-tile look-back; pass 2 recomputes the elements, scans each segment
 // with shuffles and writes prefix (+) element.
-template <int D>
-__device__ inline void agg_shfl_up(uint32_t& r, uint64_t* v, uint32_t off) {
+template <int D, typename V>
+__device__ inline void agg_shfl_up(uint32_t& r, V* v, uint32_t off) {
     r = __shfl_up(r, off, 64);
 #pragma unroll
     for (int d = 0; d < D; ++d) v[d] = __shfl_up(v[d], off, 64);
 }
 
-template <int D>
-__device__ inline void agg_shfl_down(uint32_t& r, uint64_t* v, uint32_t off) {
+template <int D, typename V>
+__device__ inline void agg_shfl_down(uint32_t& r, V* v, uint32_t off) {
     r = __shfl_down(r, off, 64);
 #pragma unroll
     for (int d = 0; d < D; ++d) v[d] = __shfl_down(v[d], off, 64);
 }
 
 // left = left (+) right
-template <int D>
-__device__ inline void agg_comb(uint32_t& lr, uint64_t* lv, uint32_t rr, const uint64_t* rv, uint64_t R) {
+template <int D, typename V>
+__device__ inline void agg_comb(uint32_t& lr, V* lv, uint32_t rr, const V* rv, V R) {
     if (rr) {
         lr = 1;
 #pragma unroll
@@ -598,7 +602,7 @@ __device__ inline void agg_comb(uint32_t& lr, uint64_t* lv, uint32_t rr, const u
     }
 }
 
-template <int D>
+template <int D, typename V>
 __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ fr, uint32_t F,
                                                   const uint32_t* __restrict__ rtile_begin,
                                                   const uint8_t* __restrict__ choice, const uint64_t* __restrict__ diff,
@@ -611,13 +615,13 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_zoff[NS];         // keep records before each segment
     __shared__ uint32_t s_sr[NS];           // segment aggregates, then exclusive prefixes
-    __shared__ uint64_t s_sv[NS][D];
+    __shared__ V s_sv[NS][D];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t f = upper_find(rtile_begin, F, tile);
     const DecFrame& Fr = fr[f];
-    const uint64_t R = Fr.range;
+    const V R = (V)Fr.range;
     const uint64_t t0 = (uint64_t)(tile - rtile_begin[f]) * CH_TILE;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
@@ -666,7 +670,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
         }
     };
     // element of item k: keep -> (reset, value - wrap_min), diff -> (0, delta mod R)
-    auto element = [&](int k, bool check, const uint64_t* raw, uint64_t* v) -> uint32_t {
+    auto element = [&](int k, bool check, const uint64_t* raw, V* v) -> uint32_t {
         const uint64_t r = t0 + k * CH_WG + threadIdx.x;
         if (r >= Fr.nrec) {
 #pragma unroll
@@ -682,7 +686,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             for (int d = 0; d < D; ++d) {
                 const uint64_t kv = raw[d];
                 if (check && ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max)) atomicOr(err, DERR_L2);
-                v[d] = kv - (uint64_t)Fr.wrap_min;
+                v[d] = (V)(kv - (uint64_t)Fr.wrap_min);
             }
             return 1;
         }
@@ -695,8 +699,8 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             const uint64_t sym = raw[d];
             const int64_t corr = (sym & 1u) ? -(int64_t)(sym >> 1) - 1 : (int64_t)(sym >> 1);
             if (check && ((int64_t)sym < 0 || corr < Fr.min_corr || corr > Fr.max_corr)) atomicOr(err, DERR_L2);
-            const uint64_t delta = corr > 0 ? R - (uint64_t)corr : (uint64_t)(-corr);
-            v[d] = delta < R ? delta : delta % R;
+            const uint64_t delta = corr > 0 ? (uint64_t)R - (uint64_t)corr : (uint64_t)(-corr);
+            v[d] = (V)(delta < (uint64_t)R ? delta : delta % (uint64_t)R);
         }
         return 0;
     };
@@ -710,15 +714,15 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 #pragma unroll
       for (int j = 0; j < L2_GROUP; ++j) {
         const int k = kg + j;
-        uint64_t v[D];
+        V v[D];
         uint32_t rs = element(k, true, raw[j], v);
         for (uint32_t off = 1; off < 64; off <<= 1) {
             uint32_t orr = rs;
-            uint64_t ov[D];
+            V ov[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) ov[d] = v[d];
-            agg_shfl_down<D>(orr, ov, off);
-            if (lane + off < 64 && (lane & (2 * off - 1)) == 0) agg_comb<D>(rs, v, orr, ov, R);
+            agg_shfl_down<D, V>(orr, ov, off);
+            if (lane + off < 64 && (lane & (2 * off - 1)) == 0) agg_comb<D, V>(rs, v, orr, ov, R);
         }
         if (lane == 0) {
             s_sr[k * W + wave] = rs;
@@ -732,32 +736,32 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     // look-back with multi-word aggregates (flag 1 = tile aggregate, 2 = inclusive)
     if (threadIdx.x < 64) {
         uint32_t rs = s_sr[lane];
-        uint64_t v[D];
+        V v[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) v[d] = s_sv[lane][d];
         const uint32_t r_self = rs;
-        uint64_t v_self[D];
+        V v_self[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) v_self[d] = v[d];
         for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan
             uint32_t orr = rs;
-            uint64_t ov[D];
+            V ov[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) ov[d] = v[d];
-            agg_shfl_up<D>(orr, ov, off);
+            agg_shfl_up<D, V>(orr, ov, off);
             if (lane >= off) {
-                agg_comb<D>(orr, ov, rs, v, R);   // earlier (+) mine
+                agg_comb<D, V>(orr, ov, rs, v, R);   // earlier (+) mine
                 rs = orr;
 #pragma unroll
                 for (int d = 0; d < D; ++d) v[d] = ov[d];
             }
         }
         // tile aggregate = inclusive value of lane 63
-        Agg tot;
+        Agg<V> tot;
         tot.reset = __shfl(rs, 63, 64);
 #pragma unroll
         for (int d = 0; d < D; ++d) tot.v[d] = __shfl(v[d], 63, 64);
-        Agg incoming;
+        Agg<V> incoming;
         incoming.reset = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) incoming.v[d] = 0;
@@ -771,7 +775,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             // t - l; the window up to the nearest inclusive (or resetting)
             // aggregate is combined in order, farther tiles on the left
             uint32_t ar = 0;
-            uint64_t av[D];
+            V av[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) av[d] = 0;
             int64_t t = (int64_t)tile - 1;
@@ -795,33 +799,33 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
                     continue;
                 }
                 uint32_t yr = 0;
-                uint64_t yv[D];
+                V yv[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) yv[d] = 0;
                 if (((need >> lane) & 1ull) && !beyond) {
                     yr = fl >> 8;
                     const uint64_t slot = st == 2 ? 1 : 0;
 #pragma unroll
-                    for (int d = 0; d < D; ++d) yv[d] = aggv[((uint64_t)mine * 2 + slot) * CH_MAXD + d];
+                    for (int d = 0; d < D; ++d) yv[d] = (V)aggv[((uint64_t)mine * 2 + slot) * CH_MAXD + d];
                 }
                 for (uint32_t off = 1; off < 64; off <<= 1) {
                     uint32_t orr = yr;
-                    uint64_t ov[D];
+                    V ov[D];
 #pragma unroll
                     for (int d = 0; d < D; ++d) ov[d] = yv[d];
-                    agg_shfl_down<D>(orr, ov, off);
+                    agg_shfl_down<D, V>(orr, ov, off);
                     if (lane + off < 64 && (lane & (2 * off - 1)) == 0) {
-                        agg_comb<D>(orr, ov, yr, yv, R);   // farther (+) nearer
+                        agg_comb<D, V>(orr, ov, yr, yv, R);   // farther (+) nearer
                         yr = orr;
 #pragma unroll
                         for (int d = 0; d < D; ++d) yv[d] = ov[d];
                     }
                 }
                 uint32_t wr = __shfl(yr, 0, 64);
-                uint64_t wv[D];
+                V wv[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) wv[d] = __shfl(yv[d], 0, 64);
-                agg_comb<D>(wr, wv, ar, av, R);   // window (+) what was nearer
+                agg_comb<D, V>(wr, wv, ar, av, R);   // window (+) what was nearer
                 ar = wr;
 #pragma unroll
                 for (int d = 0; d < D; ++d) av[d] = wv[d];
@@ -833,8 +837,8 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             for (int d = 0; d < D; ++d) incoming.v[d] = av[d];
         }
         if (lane == 0) {
-            Agg inc = incoming;
-            agg_combine<D>(inc, tot, R);
+            Agg<V> inc = incoming;
+            agg_combine<D, V>(inc, tot, R);
             for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 1) * CH_MAXD + d] = inc.v[d];
             __hip_atomic_store(&flags[tile], 2u | (inc.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -843,14 +847,14 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
         for (int d = 0; d < D; ++d) incoming.v[d] = __shfl(incoming.v[d], 0, 64);
         // exclusive prefix of segment `lane` = incoming (+) inclusive of segment lane-1
         uint32_t pr = incoming.reset;
-        uint64_t pv[D];
+        V pv[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) pv[d] = incoming.v[d];
         uint32_t br = __shfl_up(rs, 1, 64);
-        uint64_t bv[D];
+        V bv[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) bv[d] = __shfl_up(v[d], 1, 64);
-        if (lane > 0) agg_comb<D>(pr, pv, br, bv, R);
+        if (lane > 0) agg_comb<D, V>(pr, pv, br, bv, R);
         (void)r_self;
         (void)v_self;
         s_sr[lane] = pr;
@@ -867,30 +871,30 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 #pragma unroll
       for (int j = 0; j < L2_GROUP; ++j) {
         const int k = kg + j;
-        uint64_t v[D];
+        V v[D];
         uint32_t rs = element(k, false, raw[j], v);
         for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan in the wave
             uint32_t orr = rs;
-            uint64_t ov[D];
+            V ov[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) ov[d] = v[d];
-            agg_shfl_up<D>(orr, ov, off);
+            agg_shfl_up<D, V>(orr, ov, off);
             if (lane >= off) {
-                agg_comb<D>(orr, ov, rs, v, R);
+                agg_comb<D, V>(orr, ov, rs, v, R);
                 rs = orr;
 #pragma unroll
                 for (int d = 0; d < D; ++d) v[d] = ov[d];
             }
         }
         uint32_t pr = s_sr[k * W + wave];
-        uint64_t pv[D];
+        V pv[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) pv[d] = s_sv[k * W + wave][d];
-        agg_comb<D>(pr, pv, rs, v, R);
+        agg_comb<D, V>(pr, pv, rs, v, R);
         const uint64_t r = t0 + k * CH_WG + threadIdx.x;
         if (r < Fr.nrec) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + pv[d];
+            for (int d = 0; d < D; ++d) compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + (uint64_t)pv[d];
         }
       }
     }

@@ -880,19 +880,33 @@ __global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__
 
 // bytes of every section and each chunk's offset inside it: one workgroup
 // per section, block scan over its chunks
+#define SEC_ITEMS 16
 __global__ __launch_bounds__(CH_WG) void k_section_sizes(Section* __restrict__ sec,
                                                          const uint32_t* __restrict__ chunk_len,
                                                          uint32_t* __restrict__ chunk_off) {
     __shared__ uint32_t s_tmp[4];
     Section& S = sec[blockIdx.x];
     const uint32_t nch = S.nchunks, base = S.chunk_base, st = 4u * S.lanes;
+    // SEC_ITEMS consecutive chunks per thread and round, all loaded first
     uint64_t run = 0;
-    for (uint32_t k0 = 0; k0 < nch; k0 += CH_WG) {
-        const uint32_t k = k0 + threadIdx.x;
-        const uint32_t v = k < nch ? st + chunk_len[base + k] : 0u;
+    for (uint32_t k0 = 0; k0 < nch; k0 += CH_WG * SEC_ITEMS) {
+        const uint32_t kb = k0 + threadIdx.x * SEC_ITEMS;
+        uint32_t v[SEC_ITEMS], sum = 0;
+#pragma unroll
+        for (int i = 0; i < SEC_ITEMS; ++i) {
+            const uint32_t k = kb + i;
+            const uint32_t len = chunk_len[base + (k < nch ? k : 0u)];   // unconditional (chunk 0 exists)
+            v[i] = k < nch ? st + len : 0u;
+            sum += v[i];
+        }
         uint32_t tot;
-        const uint32_t pre = block_excl_scan(v, s_tmp, &tot);
-        if (k < nch) chunk_off[base + k] = (uint32_t)(run + pre);
+        const uint32_t pre = block_excl_scan(sum, s_tmp, &tot);
+        uint64_t at = run + pre;
+#pragma unroll
+        for (int i = 0; i < SEC_ITEMS; ++i) {
+            if (kb + i < nch) chunk_off[base + kb + i] = (uint32_t)at;
+            at += v[i];
+        }
         run += tot;
     }
     if (threadIdx.x == 0) {

@@ -147,13 +147,24 @@ __global__ __launch_bounds__(CH_WG) void k_scan_u32(const uint32_t* __restrict__
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint64_t base = (uint64_t)tile * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
+    // a thread's 16 items in four 16-byte loads and stores when they are all
+    // in range and both arrays are 16-byte aligned (the arena's are)
+    const bool vec = base + CH_ITEMS <= n && ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) == 0;
     uint32_t v[CH_ITEMS];
     uint32_t sum = 0;
+    if (vec) {
+        const uint4* src = reinterpret_cast<const uint4*>(in + base);
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0u;
-        sum += v[k];
+        for (int q = 0; q < CH_ITEMS / 4; ++q) {
+            const uint4 u = src[q];
+            v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) v[k] = base + k < n ? in[base + k] : 0u;
     }
+#pragma unroll
+    for (int k = 0; k < CH_ITEMS; ++k) sum += v[k];
     uint32_t tot;
     uint32_t pre = block_excl_scan(sum, s_tmp, &tot);
     if (threadIdx.x < 64) {
@@ -162,10 +173,23 @@ __global__ __launch_bounds__(CH_WG) void k_scan_u32(const uint32_t* __restrict__
     }
     __syncthreads();
     uint64_t run = s_excl + pre;
+    if (vec) {
+        uint4* dst = reinterpret_cast<uint4*>(out + base);
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        if (base + k < n) out[base + k] = (uint32_t)run;
-        run += v[k];
+        for (int q = 0; q < CH_ITEMS / 4; ++q) {
+            uint4 u;
+            u.x = (uint32_t)run; run += v[4 * q];
+            u.y = (uint32_t)run; run += v[4 * q + 1];
+            u.z = (uint32_t)run; run += v[4 * q + 2];
+            u.w = (uint32_t)run; run += v[4 * q + 3];
+            dst[q] = u;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            if (base + k < n) out[base + k] = (uint32_t)run;
+            run += v[k];
+        }
     }
     if (total_out && base < n && base + CH_ITEMS >= n) *total_out = run;
 }

@@ -1133,16 +1133,56 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
     const uint64_t r1 = r0 + CH_TILE < n ? r0 + CH_TILE : n;
     const ValT* src = vals + C.base * DIMS;
     uint64_t doff = C.payload_off;
+    // byte values: the tile's bytes are staged in LDS with coalesced 4-byte
+    // loads (the arena's arrays are 256-byte aligned and padded, so the words
+    // covering the tile's range are readable), then cut from LDS; the
+    // workgroup branches on the array's alignment as a whole
+    constexpr uint32_t STAGE_WORDS = sizeof(ValT) == 1 ? CH_TILE * DIMS / 4 + 2 : 1;
+    __shared__ uint32_t s_w[STAGE_WORDS];
+    const uint8_t* s_b = nullptr;
+    if constexpr (sizeof(ValT) == 1) {
+        if ((reinterpret_cast<uintptr_t>(vals) & 3u) == 0) {
+            const uint64_t b0 = (C.base + r0) * DIMS, b1 = (C.base + r1) * DIMS;
+            const uint64_t w0 = b0 >> 2, w1 = (b1 + 3) >> 2;   // w1 - w0 <= STAGE_WORDS
+            const uint32_t* vw = reinterpret_cast<const uint32_t*>(vals);
+            constexpr int NW = (STAGE_WORDS + CH_WG - 1) / CH_WG;
+            uint32_t wv[NW];
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const uint64_t wi = w0 + (uint64_t)q * CH_WG + threadIdx.x;
+                wv[q] = vw[wi < w1 ? wi : w0];   // unconditional
+            }
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const uint64_t wi = w0 + (uint64_t)q * CH_WG + threadIdx.x;
+                if (wi < w1) s_w[wi - w0] = wv[q];
+            }
+            __syncthreads();
+            s_b = reinterpret_cast<const uint8_t*>(s_w) + (b0 - 4 * w0);
+        }
+    }
 #pragma unroll
     for (int d = 0; d < DIMS; ++d) {
         const uint32_t w = C.width[d];
         const uint64_t mn = C.dmin[d];
         ValT x[PACK_GROUPS / CH_WG][8];
+        if (s_b) {
 #pragma unroll
-        for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {
-            const uint64_t ra = r0 + 8ull * (h * CH_WG + threadIdx.x);
+            for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {
+                const uint32_t la = 8u * (h * CH_WG + threadIdx.x);   // record ra - r0
 #pragma unroll
-            for (int k = 0; k < 8; ++k) x[h][k] = src[(ra + k < r1 ? ra + k : r0) * DIMS + d];   // unconditional
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t lr = r0 + la + k < r1 ? la + k : 0u;
+                    x[h][k] = (ValT)s_b[lr * DIMS + d];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {
+                const uint64_t ra = r0 + 8ull * (h * CH_WG + threadIdx.x);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) x[h][k] = src[(ra + k < r1 ? ra + k : r0) * DIMS + d];   // unconditional
+            }
         }
 #pragma unroll
         for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {

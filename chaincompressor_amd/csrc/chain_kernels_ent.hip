@@ -475,6 +475,46 @@ __global__ __launch_bounds__(CH_WG) void k_comp_reduce(const CompSlot* __restric
     }
 }
 
+// exclusive max over the lower threads of a 256-thread workgroup (0 for
+// thread 0); s4 holds 4 entries
+__device__ inline uint32_t block_excl_max(uint32_t v, uint32_t* s4) {
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    uint32_t inc = v;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+        const uint32_t t = __shfl_up(inc, k, 64);
+        if (lane >= (uint32_t)k) inc = t > inc ? t : inc;
+    }
+    if (lane == 63) s4[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < w; ++k) before = s4[k] > before ? s4[k] : before;
+    uint32_t ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = 0;
+    __syncthreads();
+    return ex > before ? ex : before;
+}
+
+// exclusive min over the higher threads of a 256-thread workgroup (ident
+// for the last thread); s4 holds 4 entries
+__device__ inline uint32_t block_excl_suffix_min(uint32_t v, uint32_t* s4, uint32_t ident) {
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    uint32_t inc = v;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) {
+        const uint32_t t = __shfl_down(inc, k, 64);
+        if (lane + (uint32_t)k < 64) inc = t < inc ? t : inc;
+    }
+    if (lane == 0) s4[w] = inc;
+    __syncthreads();
+    uint32_t after = ident;
+    for (uint32_t k = w + 1; k < CH_WG / 64; ++k) after = s4[k] < after ? s4[k] : after;
+    uint32_t ex = __shfl_down(inc, 1, 64);
+    if (lane == 63) ex = ident;
+    __syncthreads();
+    return ex < after ? ex : after;
+}
+
 // ------------------------------------------------------------------------
 // RVN1 table of every section, built on the device (cpp/chain/lcps_cpp.cpp:560-729)
 
@@ -587,28 +627,87 @@ __global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict
         }
         if (threadIdx.x == 0 && tot != M) atomicOr(err, 4u);
     }
-    if (threadIdx.x == 0) {
-        // compact table bytes, cpp/chain/lcps_cpp.cpp:686-729
+    // compact table bytes, cpp/chain/lcps_cpp.cpp:686-729: varuint(A), then
+    // per symbol (pr << 2) | e and e more bytes (e = 0, 1, 2 for pr < 2^6,
+    // < 2^14, else), and a zero run as one byte ((n - 1) << 2) | 3 per 64
+    // zeros. Every thread writes its own 16 symbols: a zero starts a byte
+    // when it lies a multiple of 64 after its run's start (exclusive max-scan
+    // of "last nonzero + 1"), whose count is bounded by the run's end
+    // (exclusive suffix min-scan of nonzero positions); the byte offsets are
+    // an exclusive sum over the symbols.
+    {
+        __shared__ uint32_t s_t2[4];
         uint8_t* o = tbytes + (uint64_t)si * TABLE_BYTES_MAX;
-        uint32_t len = 0;
-        uint64_t v = A;
-        while (v >= 0x80u) { o[len++] = (uint8_t)((uint8_t)v | 0x80u); v >>= 7; }
-        o[len++] = (uint8_t)v;
-        for (uint32_t i = 0; i < A; ++i) {
-            const uint32_t pr = s_prob[i];
-            if (pr == 0) {
-                uint32_t extra = 0;
-                while (extra < 63 && i + extra + 1 < A && s_prob[i + extra + 1] == 0) ++extra;
-                o[len++] = (uint8_t)((extra << 2) | 3u);
-                i += extra;
-                continue;
+        uint32_t hl = 0;
+        for (uint64_t v = A; v >= 0x80u; v >>= 7) ++hl;
+        ++hl;   // varuint(A) bytes
+        const uint32_t b0 = threadIdx.x * 16;
+        uint32_t pr[16];
+        uint32_t lmax = 0, lmin = A;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const uint32_t i = b0 + k;
+            pr[k] = i < A ? s_prob[i] : 0u;
+            if (i < A && pr[k]) {
+                lmax = i + 1;                  // entries ascend: the last wins
+                lmin = lmin < i ? lmin : i;
             }
-            uint32_t eb = 0;
-            if (pr >= (1u << 6)) { ++eb; if (pr >= (1u << 14)) ++eb; }
-            o[len++] = (uint8_t)((pr << 2) | eb);
-            for (uint32_t k = 0; k < eb; ++k) o[len++] = (uint8_t)(pr >> (8 * (k + 1) - 2));
         }
-        S.count = count; S.eligible = 1; S.pb = pb; S.alphabet = A; S.table_len = len;
+        const uint32_t rs0 = block_excl_max(lmax, s_t2);        // last nonzero + 1 before b0 (or 0)
+        const uint32_t re0 = block_excl_suffix_min(lmin, s_t2, A);   // first nonzero after b0 + 15 (or A)
+        uint32_t re[16];
+        {
+            uint32_t rn = re0;
+#pragma unroll
+            for (int k = 15; k >= 0; --k) {
+                re[k] = rn;
+                const uint32_t i = b0 + k;
+                if (i < A && pr[k]) rn = i;
+            }
+        }
+        uint32_t nbytes[16], cnt = 0;
+        {
+            uint32_t rm = rs0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t i = b0 + k;
+                uint32_t nb = 0;
+                if (i < A) {
+                    if (pr[k]) {
+                        nb = 1u + (pr[k] >= (1u << 6) ? 1u : 0u) + (pr[k] >= (1u << 14) ? 1u : 0u);
+                        rm = i + 1;
+                    } else {
+                        nb = ((i - rm) & 63u) == 0 ? 1u : 0u;
+                    }
+                }
+                nbytes[k] = nb;
+                cnt += nb;
+            }
+        }
+        uint32_t total;
+        uint32_t at = hl + block_excl_scan(cnt, s_tmp, &total);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (!nbytes[k]) continue;
+            const uint32_t i = b0 + k;
+            if (pr[k]) {
+                const uint32_t eb = nbytes[k] - 1;
+                o[at] = (uint8_t)((pr[k] << 2) | eb);
+                for (uint32_t q = 0; q < eb; ++q) o[at + 1 + q] = (uint8_t)(pr[k] >> (8 * (q + 1) - 2));
+            } else {
+                const uint32_t run = re[k] - i;   // zeros from i to the run's end
+                const uint32_t extra = run - 1 < 63u ? run - 1 : 63u;
+                o[at] = (uint8_t)((extra << 2) | 3u);
+            }
+            at += nbytes[k];
+        }
+        if (threadIdx.x == 0) {
+            uint32_t len = 0;
+            uint64_t v = A;
+            while (v >= 0x80u) { o[len++] = (uint8_t)((uint8_t)v | 0x80u); v >>= 7; }
+            o[len++] = (uint8_t)v;
+            S.count = count; S.eligible = 1; S.pb = pb; S.alphabet = A; S.table_len = hl + total;
+        }
     }
 }
 

@@ -1801,7 +1801,6 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
                            st, X.root,
                            lin, &X.jump_open[r], lout, &X.jump_open[r + 1]);
     }
-    hipLaunchKernelGGL(k_dec_chain_follow, dim3((unsigned)cdiv(T, CH_TILE)), dim3(CH_WG), 0, st, X.root, T);
     tm.end();
     tm.begin("dequantize", 4 * T + comp_es[4] * T * D + 8ull * D * P.R + T * D * es);
     with_dims(D, [&](auto dd) {

@@ -908,8 +908,8 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 // the segment keep CH_EXT | their exit. Rounds of global pointer jumping over
 // the open list resolve the exits (every hop goes to a later tile, through
 // at most one follower per tile, so ceil(log2(tiles)) + 2 rounds suffice);
-// a last pass lets every follower take its exit's value. No tile ever waits
-// for another.
+// dequantisation then reads a follower's value at its exit. No tile ever
+// waits for another.
 
 #define CH_LOCAL 0x80000000u
 #define CH_EXT 0x80000000u
@@ -1127,25 +1127,6 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__
     }
 }
 
-// followers of tile exits take their exit's (resolved) value
-// (CH_TILE points per workgroup, both load levels issued for all items)
-__global__ __launch_bounds__(CH_WG) void k_dec_chain_follow(uint32_t* __restrict__ res, uint64_t T) {
-    const uint64_t g0 = (uint64_t)blockIdx.x * CH_TILE;
-    uint32_t v[CH_ITEMS], t[CH_ITEMS];
-#pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t g = g0 + (uint64_t)k * CH_WG + threadIdx.x;
-        v[k] = res[g < T ? g : g0];
-    }
-#pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) t[k] = (v[k] & CH_EXT) ? res[v[k] & ~CH_EXT] : 0u;
-#pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t g = g0 + (uint64_t)k * CH_WG + threadIdx.x;
-        if (g < T && (v[k] & CH_EXT)) res[g] = t[k];
-    }
-}
-
 // every frame: explicit points == records
 __global__ void k_dec_check_counts(const DecFrame* __restrict__ fr, uint32_t F, const uint32_t* __restrict__ tb,
                                    const uint32_t* __restrict__ tile_expl_off, const uint64_t* __restrict__ expl_total,
@@ -1188,6 +1169,19 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
         for (int j = 0; j < DQ_ITEMS; ++j) {
             const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
             rv[j] = root[pt_off + (i < n ? i : i0)];   // unconditional (the tile's first point)
+        }
+        {
+            // a follower holds CH_EXT | the exit of its chain segment, whose
+            // value is final after the jump rounds: take it (the others load
+            // their own entry again, a cache hit)
+            uint32_t rx[DQ_ITEMS];
+#pragma unroll
+            for (int j = 0; j < DQ_ITEMS; ++j) {
+                const uint64_t i = i0 + (uint64_t)(h + j) * CH_WG + threadIdx.x;
+                rx[j] = root[(rv[j] & CH_EXT) ? (uint64_t)(rv[j] & ~CH_EXT) : pt_off + (i < n ? i : i0)];
+            }
+#pragma unroll
+            for (int j = 0; j < DQ_ITEMS; ++j) rv[j] = (rv[j] & CH_EXT) ? rx[j] : rv[j];
         }
         uint64_t blk[DQ_ITEMS][D];
         RelT off[DQ_ITEMS][D];

@@ -68,9 +68,8 @@ REGION_KERNELS = {
     "unpack_tagged": "chain::k_dec_tagged",
     "l2_counts": "chain::k_dec_tile_zero_counts + k_scan_u32",
     "l2_inverse": "chain::k_dec_l2",
-    "chain_counts": "chain::k_dec_tile_zero_counts + k_scan_u32 + k_dec_check_counts",
-    "chain_local": "chain::k_dec_chain_local",
-    "chain_jump": "chain::k_dec_chain_jump (ceil(log2 tiles)+2 launches) + k_dec_chain_follow",
+    "chain_local": "chain::k_dec_chain_local + k_dec_check_counts",
+    "chain_jump": "chain::k_dec_chain_jump (ceil(log2 tiles)+2 launches)",
     "dequantize": "chain::k_dec_dequant + k_dec_corr",
 }
 

@@ -1621,7 +1621,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
             uint8_t *choice, *tags;
             uint64_t* compact;
             uint32_t* root;
-            uint32_t *zc, *zoff, *ec, *eoff;
+            uint32_t *zc, *zoff, *eoff;
+            uint64_t* cstatus;   // look-back words of the chain tiles
             uint32_t *ptb, *rtb, *ptb32;
             uint64_t *sbr, *snr, *sbp, *snp;
             uint64_t* status;
@@ -1662,7 +1663,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.root = c.take<uint32_t>(T);
         x.zc = c.take<uint32_t>(nrt + 1);
         x.zoff = c.take<uint32_t>(nrt + 1);
-        x.ec = c.take<uint32_t>(npt + 1);
+        x.cstatus = c.take<uint64_t>(npt + 1);
         x.eoff = c.take<uint32_t>(npt + 1);
         x.status = c.take<uint64_t>(std::max<uint64_t>({P.tag_tiles, cdiv(npt + 1, CH_TILE), cdiv(nrt + 1, CH_TILE)}) + 1);
         x.flags = c.take<uint32_t>(nrt + 1);
@@ -1768,13 +1769,6 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         });
         tm.end();
     }
-    tm.begin("chain_counts", 4 * T + 8ull * npt);
-    hipLaunchKernelGGL(k_dec_tile_zero_counts, dim3(npt), dim3(CH_WG), 0, st, (const uint8_t*)nullptr, X.idrel, X.ptb,
-                       X.sbp, X.snp, F, X.ec);
-    scan_u32(st, X.ec, X.eoff, npt, X.status, X.ticket, X.err, &X.totals[1]);
-    hipLaunchKernelGGL(k_dec_check_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.eoff,
-                       &X.totals[1], X.err);
-    tm.end();
     // the O(N) algorithmic traffic of chain resolution (read successors,
     // write roots) is stated once, on the local pass; the jump rounds over
     // tile seams are extra work of this formulation
@@ -1787,11 +1781,17 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         ++rounds;                                        // hops through followers
     }
     fill32(st, X.jump_open, rounds + 1, 0);
+    // the local pass ranks the explicit points itself: tiles by ticket, each
+    // tile's first rank from a look-back over the tiles before it
+    fill64(st, X.cstatus, npt, 0);
+    fill32(st, X.ticket + 1, 1, 0);
     with_dims(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
         hipLaunchKernelGGL((k_dec_chain_local<DD>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.idrel, X.eoff,
-                           X.root, X.list_a, &X.jump_open[0], X.err);
+                           X.cstatus, X.ticket + 1, &X.totals[1], X.root, X.list_a, &X.jump_open[0], X.err);
     });
+    hipLaunchKernelGGL(k_dec_check_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.eoff,
+                       &X.totals[1], X.err);
     tm.end();
     tm.begin("chain_jump");
     for (uint32_t r = 0; r < rounds; ++r) {

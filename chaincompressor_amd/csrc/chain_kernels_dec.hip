@@ -917,14 +917,19 @@ template <int D>
 __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __restrict__ fr, uint32_t F,
                                                            const uint32_t* __restrict__ tile_begin,
                                                            const uint32_t* __restrict__ idrel,
-                                                           const uint32_t* __restrict__ tile_expl_off,
+                                                           uint32_t* __restrict__ tile_expl_off, uint64_t* status,
+                                                           uint32_t* ticket, uint64_t* __restrict__ expl_total,
                                                            uint32_t* __restrict__ res, uint32_t* __restrict__ open_list,
                                                            uint32_t* __restrict__ open_count, uint32_t* err) {
     constexpr uint32_t W = CH_WG / 64;
     __shared__ uint32_t s_val[CH_TILE];
     __shared__ uint32_t s_ext[CH_TILE];
     __shared__ uint32_t s_off[CH_ITEMS * W];
-    const uint32_t tile = blockIdx.x;
+    __shared__ uint32_t s_tile, s_rank0;
+    // tiles are taken by ticket, so every tile a look-back waits on has started
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
     const uint32_t f = upper_find(tile_begin, F, tile);
     const DecFrame& Fr = fr[f];
     const uint64_t p0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
@@ -957,9 +962,17 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
         const uint32_t v = threadIdx.x < CH_ITEMS * W ? s_off[threadIdx.x] : 0u;
         const uint32_t inc = wave_incl_scan(v);
         if (threadIdx.x < CH_ITEMS * W) s_off[threadIdx.x] = inc - v;
+        // explicit points of all earlier tiles (every frame's, in order)
+        const uint32_t tot = __shfl(inc, 63, 64);
+        const uint64_t excl = lookback(status, tile, tot, err);
+        if (threadIdx.x == 0) {
+            s_rank0 = (uint32_t)excl;
+            tile_expl_off[tile] = (uint32_t)excl;
+            if (tile + 1 == gridDim.x) *expl_total = excl + tot;
+        }
     }
     __syncthreads();
-    const uint32_t rank0 = tile_expl_off[tile];
+    const uint32_t rank0 = s_rank0;
     // s_val: LOCAL | j (pointer inside the tile) or a final explicit index;
     // s_ext: global successor index for pointers leaving the tile (else ~0)
 #pragma unroll

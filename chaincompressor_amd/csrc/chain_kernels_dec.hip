@@ -925,7 +925,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
     __shared__ uint32_t s_val[CH_TILE];
     __shared__ uint32_t s_ext[CH_TILE];
     __shared__ uint32_t s_off[CH_ITEMS * W];
-    __shared__ uint32_t s_tile, s_rank0;
+    __shared__ uint32_t s_tile, s_rank0, s_tot;
     // tiles are taken by ticket, so every tile a look-back waits on has started
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
@@ -962,18 +962,14 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
         const uint32_t v = threadIdx.x < CH_ITEMS * W ? s_off[threadIdx.x] : 0u;
         const uint32_t inc = wave_incl_scan(v);
         if (threadIdx.x < CH_ITEMS * W) s_off[threadIdx.x] = inc - v;
-        // explicit points of all earlier tiles (every frame's, in order)
-        const uint32_t tot = __shfl(inc, 63, 64);
-        const uint64_t excl = lookback(status, tile, tot, err);
-        if (threadIdx.x == 0) {
-            s_rank0 = (uint32_t)excl;
-            tile_expl_off[tile] = (uint32_t)excl;
-            if (tile + 1 == gridDim.x) *expl_total = excl + tot;
-        }
+        // this tile's explicit count is published now; the ranks of the
+        // earlier tiles are waited for only when the results are written
+        s_tot = __shfl(inc, 63, 64);
+        if (threadIdx.x == 0) lb_store(&status[tile], (tile == 0 ? LB_INC : LB_AGG) | s_tot);
     }
     __syncthreads();
-    const uint32_t rank0 = s_rank0;
-    // s_val: LOCAL | j (pointer inside the tile) or a final explicit index;
+    // s_val: LOCAL | j (pointer inside the tile) or the explicit point's
+    // ordinal inside the tile (its rank is that plus the tile's first rank);
     // s_ext: global successor index for pointers leaving the tile (else ~0)
 #pragma unroll
     for (int k = 0; k < CH_ITEMS; ++k) {
@@ -981,7 +977,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
         if (j >= cnt) continue;
         s_ext[j] = 0xffffffffu;
         if (dv[k] == 0) {
-            s_val[j] = rank0 + s_off[k * W + wave] + pre[k];
+            s_val[j] = s_off[k * W + wave] + pre[k];
         } else {
             const uint64_t t = p0 + j + dv[k];
             if (t >= Fr.n) {
@@ -1027,6 +1023,18 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
         }
         if (!__syncthreads_or(open)) break;   // every in-tile pointer resolved
     }
+    // the tile's first rank: explicit points of all earlier tiles (every
+    // frame's, in order)
+    if (threadIdx.x < 64) {
+        const uint64_t excl = lookback(status, tile, s_tot, err);
+        if (threadIdx.x == 0) {
+            s_rank0 = (uint32_t)excl;
+            tile_expl_off[tile] = (uint32_t)excl;
+            if (tile + 1 == gridDim.x) *expl_total = excl + s_tot;
+        }
+    }
+    __syncthreads();
+    const uint32_t rank0 = s_rank0;
     // results; points whose chain leaves the tile go to the open list, in
     // tile order, with one reservation per tile
     uint32_t* dst = res + Fr.pt_off + p0;
@@ -1048,7 +1056,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
                 atomicOr(err, DERR_CHAIN);
                 out = 0;
             } else {
-                out = v;
+                out = rank0 + v;
             }
             dst[j] = out;
         }

@@ -530,8 +530,11 @@ __global__ __launch_bounds__(CH_WG) void k_corr_gather(const uint64_t* __restric
 template <typename KeyT>
 __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ keys, uint64_t T, uint32_t shift,
                                                      uint32_t ntiles, uint32_t* __restrict__ counts) {
-    __shared__ uint32_t s_h[256];
-    s_h[threadIdx.x] = 0;
+    // one histogram per wavefront: atomics of different wavefronts never
+    // meet on a word; the four are summed at the end
+    __shared__ uint32_t s_h[CH_WG / 64][256];
+#pragma unroll
+    for (int w = 0; w < CH_WG / 64; ++w) s_h[w][threadIdx.x] = 0;
     const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
     KeyT kx[CH_ITEMS];
 #pragma unroll
@@ -540,13 +543,17 @@ __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ ke
         kx[j] = keys[i < T ? i : base];
     }
     __syncthreads();
+    uint32_t* h = s_h[threadIdx.x >> 6];
 #pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
         const uint64_t i = base + (uint64_t)j * CH_WG + threadIdx.x;
-        if (i < T) atomicAdd(&s_h[(uint32_t)(kx[j] >> shift) & 0xffu], 1u);
+        if (i < T) atomicAdd(&h[(uint32_t)(kx[j] >> shift) & 0xffu], 1u);
     }
     __syncthreads();
-    counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < CH_WG / 64; ++w) c += s_h[w][threadIdx.x];
+    counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = c;
 }
 
 template <typename KeyT, bool FIRST>

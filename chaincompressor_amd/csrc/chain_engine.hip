@@ -431,10 +431,15 @@ struct CompressResult {
 
 
 template <class T, int D, bool SOA>
+// the tile partials of every frame are folded in two launches: BBOX_SPLIT
+// workgroups per frame over parts of its tiles, then one per frame over those
+#define BBOX_SPLIT 16u
 static void launch_bbox(hipStream_t st, const void* pts, const FrameDev* dfr, const uint32_t* dtb, uint32_t F,
-                        uint32_t ntiles, uint64_t* part, uint32_t* bad, uint64_t* box) {
+                        uint32_t ntiles, uint64_t* part, uint64_t* part2, uint32_t* bad, uint64_t* box) {
     hipLaunchKernelGGL((k_bbox<T, D, SOA>), dim3(ntiles), dim3(CH_WG), 0, st, (const T*)pts, dfr, dtb, F, part, bad);
-    hipLaunchKernelGGL((k_bbox_finish<D>), dim3(F), dim3(CH_WG), 0, st, part, dtb, box);
+    hipLaunchKernelGGL((k_bbox_finish<D>), dim3(F * BBOX_SPLIT), dim3(CH_WG), 0, st, part, dtb, 0u, BBOX_SPLIT, part2);
+    hipLaunchKernelGGL((k_bbox_finish<D>), dim3(F), dim3(CH_WG), 0, st, part2, (const uint32_t*)nullptr, BBOX_SPLIT, 1u,
+                       box);
 }
 
 static void fill64(hipStream_t st, uint64_t* p, uint64_t n, uint64_t v) {
@@ -673,6 +678,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         struct {
             void* input;
             uint64_t* part;
+            uint64_t* part2;   // per-frame parts of the tile partials
             uint64_t* box;
             uint32_t* bad;
             FrameDev* fdev;
@@ -683,6 +689,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         a.input = contiguous ? nullptr : c.take<char>(T * D * es);
         a.part = c.take<uint64_t>((uint64_t)ntiles * D * 4);
         a.box = c.take<uint64_t>((uint64_t)F * D * 4);
+        a.part2 = c.take<uint64_t>((uint64_t)F * BBOX_SPLIT * D * 4);
         a.bad = c.take<uint32_t>(F);
         a.fdev = c.take<FrameDev>(F);
         a.tb = c.take<uint32_t>(F + 1);
@@ -874,8 +881,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         using T_ = typename decltype(tt)::type;
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
-            if (layout == CHAIN_SOA) launch_bbox<T_, DD, true>(st, dpts, A.fdev, A.tb, F, ntiles, A.part, A.bad, A.box);
-            else launch_bbox<T_, DD, false>(st, dpts, A.fdev, A.tb, F, ntiles, A.part, A.bad, A.box);
+            if (layout == CHAIN_SOA) launch_bbox<T_, DD, true>(st, dpts, A.fdev, A.tb, F, ntiles, A.part, A.part2, A.bad, A.box);
+            else launch_bbox<T_, DD, false>(st, dpts, A.fdev, A.tb, F, ntiles, A.part, A.part2, A.bad, A.box);
         });
     });
     tm.end();

@@ -300,10 +300,16 @@ __global__ __launch_bounds__(CH_WG) void k_bbox(const T* __restrict__ pts, const
 
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_bbox_finish(const uint64_t* __restrict__ part,
-                                                       const uint32_t* __restrict__ tile_begin,
-                                                       uint64_t* __restrict__ frame_box) {
-    const uint32_t f = blockIdx.x;
-    const uint32_t t0 = tile_begin[f], t1 = tile_begin[f + 1];
+                                                       const uint32_t* __restrict__ tile_begin, uint32_t stride,
+                                                       uint32_t split, uint64_t* __restrict__ frame_box) {
+    // frame f's partials are [tile_begin[f], tile_begin[f + 1]), or
+    // [f * stride, (f + 1) * stride) without a table; workgroup b folds part
+    // b % split of frame b / split into output slot b
+    const uint32_t f = blockIdx.x / split, sp = blockIdx.x % split;
+    const uint32_t f0 = tile_begin ? tile_begin[f] : f * stride, f1 = tile_begin ? tile_begin[f + 1] : (f + 1) * stride;
+    const uint32_t per = (f1 - f0 + split - 1) / split;
+    const uint32_t t0 = f0 + sp * per < f1 ? f0 + sp * per : f1;
+    const uint32_t t1 = t0 + per < f1 ? t0 + per : f1;
     __shared__ uint64_t s_red[4][CH_MAXD][4];
     const uint32_t w = threadIdx.x >> 6;
     // four tiles per thread and round, all their partials loaded first; a
@@ -352,7 +358,7 @@ __global__ __launch_bounds__(CH_WG) void k_bbox_finish(const uint64_t* __restric
             c = s_red[k][d][2] < c ? s_red[k][d][2] : c;
             e = s_red[k][d][3] < e ? s_red[k][d][3] : e;
         }
-        uint64_t* o = frame_box + ((uint64_t)f * D + d) * 4;
+        uint64_t* o = frame_box + ((uint64_t)blockIdx.x * D + d) * 4;
         o[0] = a; o[1] = b; o[2] = c; o[3] = e;
     }
 }

@@ -10,14 +10,17 @@ the MI355X engine:
   (cu/decompress/lcps_gpu_decompress_ans.cu:2898)
 * chain(coords (N, D), eb, p, threshold, diff, bl, l1, l2, ec) -> [bytes, stats...]
   mirrors lcps_cpp.lcps_compress2 (cpp/chain/lcps_cpp.cpp:2340): ec=False
-  returns the reference's LCP3 bytes exactly, ec=True this engine's LCH1
+  returns the reference's LCP3 bytes exactly; ec=True returns the reference's
+  own LCP2 container, so streams stored by a drop-in caller stay readable by
+  the reference's lcps_cpu_decompress
 * chain_decompress(bytes) -> float64 (N, D) for LCH1, LCP3 and LCP2 streams
   (LCP2 is converted to LCP3 on the host first)
 
-chain(..., ec=True, container="lcp2") writes the reference's own LCP2 instead
-of LCH1: the GPU's LCP3 streams through the host-side Zstd-22 / RVN1 chooser
-(chaincompressor_amd.lcp2), byte-identical to lcps_compress2(..., ec=True)
-with the reference's libzstd.
+With ec=True the GPU's LCP3 streams go through the host-side Zstd-22 / RVN1
+chooser (chaincompressor_amd.lcp2), byte-identical to
+lcps_compress2(..., ec=True) with the reference's libzstd. Zstd-22 is a serial
+host codec; chain(..., container="lch1") returns this engine's entropy-coded
+LCH1 instead (entropy coding on the GPU, decoded by chain_decompress).
 
 The LCP-S and Draco wrappers of the reference (lcps_compress,
 lcps_decompress, dracoIds) are competitor baselines outside this engine.
@@ -25,14 +28,14 @@ lcps_decompress, dracoIds) are competitor baselines outside this engine.
 from chaincompressor_amd import lcps_gpu_cpp, lcps_gpu_decompress_ans
 
 
-def chain(coords, eb, p, threshold, diff, bl=True, l1=True, l2=True, ec=True, *, container="lch1"):
+def chain(coords, eb, p, threshold, diff, bl=True, l1=True, l2=True, ec=True, *, container="lcp2"):
+    if container not in ("lch1", "lcp2"):
+        raise ValueError("container must be 'lch1' or 'lcp2'")
     if ec and container == "lcp2":
         from chaincompressor_amd import lcp2
         raw = lcps_gpu_cpp.lcps_compress2(coords, eb, p, threshold, diff, bl, l1, l2, False)
         out, stats = lcp2.to_lcp2(raw[0])
         return [out] + stats
-    if container not in ("lch1", "lcp2"):
-        raise ValueError("container must be 'lch1' or 'lcp2'")
     return lcps_gpu_cpp.lcps_compress2(coords, eb, p, threshold, diff, bl, l1, l2, ec)
 
 

@@ -52,6 +52,7 @@ static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 struct Arena {
     char* base = nullptr;
     size_t cap = 0;
+    uint64_t allocs = 0;   // device allocations made for this arena so far
     void release() {
         if (base) (void)hipFree(base);
         base = nullptr;
@@ -66,6 +67,7 @@ struct Arena {
         release();
         HIPCHK(hipMalloc(&base, want));
         cap = want;
+        ++allocs;
     }
     ~Arena() { release(); }
 };

@@ -7,7 +7,6 @@
 // decoupled look-back for the L2 inverse, and an O(N) chain resolution that
 // walks tiles from the end (every successor lies at a larger index).
 #include "chain_common.hpp"
-#include "chain_rans.hpp"
 
 #pragma clang fp contract(off)
 

@@ -13,7 +13,6 @@
 // Everything stays on the device: histograms, tables, encoding, sizes, the
 // layout of the output and the final bytes.
 #include "chain_common.hpp"
-#include "chain_rans.hpp"
 
 #pragma clang fp contract(off)
 

@@ -58,8 +58,17 @@ inline uint64_t serve_estimate_compress(uint64_t n, uint32_t D, uint32_t es, con
     return n * per_point + per_frame;
 }
 
+// Decode carve per point, worst case: idrel, root, list_a, list_b (4 B each),
+// rel (4 D), choice2 (1), diff + keep (8 D together: one record each per
+// explicit block), compact blocks (8 D), Tagged widths (at most 2 per point:
+// idrel and rel are per point, the three record components together hold at
+// most 2 R <= 2 T widths) and the output (D es). The stream pool takes len;
+// the parsed tables (a 24-byte chunk record per >= 4 stream bytes, the
+// cumulative tables of at most 10 sections of 4097 entries) are bounded by
+// 6 len + 10 * 16.5 KB; per-tile look-back and count words add < 1 B per point;
+// every carved buffer is rounded up to 256 bytes.
 inline uint64_t serve_estimate_decompress(uint64_t n, uint32_t D, uint32_t es_out, uint64_t len) {
-    return n * (14ull + 20ull * D + (uint64_t)D * es_out) + 2 * len + (1u << 20);
+    return n * (16ull + 1 + 2 + 1 + 20ull * D + (uint64_t)D * es_out) + 7 * len + 10 * 16512 + (1u << 20);
 }
 
 struct Server {
@@ -344,5 +353,16 @@ int chain_serve_poll(chain_server* s, uint64_t ticket, int wait_ms, chain_result
 }
 
 double chain_server_clock(chain_server* s) { return s ? s->now() : 0.0; }
+
+int chain_server_stats(chain_server* s, uint64_t* arena_allocations) {
+    if (!s || !arena_allocations) return CHAIN_EINVAL;
+    uint64_t a = 0;
+    for (chain_engine* e : s->engines) {
+        std::lock_guard<std::mutex> lock(e->mu);
+        a += e->arena.allocs + e->arena_d.allocs;
+    }
+    *arena_allocations = a;
+    return CHAIN_OK;
+}
 
 }  // extern "C"

@@ -55,7 +55,7 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_last_stage_bytes", "chain_engine_set_timing", "chain_server_create", "chain_server_destroy",
            "chain_server_last_error", "chain_serve_submit", "chain_serve_poll", "chain_server_clock",
            "chain_engine_set_graphs", "chain_engine_graph_stats", "chain_engine_autotune",
-           "chain_engine_set_autotune", "chain_compress_search"]
+           "chain_engine_set_autotune", "chain_compress_search", "chain_server_stats"]
 
 CHAIN_PENDING = 5
 REQ_COMPRESS, REQ_DECOMPRESS = 0, 1
@@ -128,6 +128,7 @@ def load_library():
     L.chain_serve_poll.argtypes = [C.c_void_p, C.c_uint64, C.c_int, P(_Result)]
     L.chain_server_clock.argtypes = [C.c_void_p]
     L.chain_server_clock.restype = C.c_double
+    L.chain_server_stats.argtypes = [C.c_void_p, P(C.c_uint64)]
     _LIB = L
     return L
 
@@ -448,6 +449,13 @@ class Server:
 
     def clock(self) -> float:
         return float(self._L.chain_server_clock(self._h))
+
+    def arena_allocations(self) -> int:
+        """Device allocations made for the workers' arenas since creation."""
+        a = C.c_uint64()
+        if self._L.chain_server_stats(self._h, C.byref(a)) != 0:
+            raise ChainError("invalid server")
+        return int(a.value)
 
     def _submit(self, req, keep):
         t = C.c_uint64()

@@ -226,6 +226,9 @@ int chain_serve_submit(chain_server* s, const chain_request* r, uint64_t* ticket
  * On CHAIN_OK the result is handed over (and forgotten by the server). */
 int chain_serve_poll(chain_server* s, uint64_t ticket, int wait_ms, chain_result* out);
 double chain_server_clock(chain_server* s);
+/* device allocations made for the workers' arenas since creation (two per
+ * worker at creation; a larger count means a batch outgrew its arena) */
+int chain_server_stats(chain_server* s, uint64_t* arena_allocations);
 
 #ifdef __cplusplus
 }

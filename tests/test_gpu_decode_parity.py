@@ -165,10 +165,12 @@ def test_pybind_modules_drop_in(eng):
     streams = O.compress_streams(pts, 1e-3, 16, 2)
     assert ret[0] == streams.lcp3()
     assert np.array_equal(chain_caller.chain_decompress(ret[0]), dec)
-    # ec=True with the reference's own container: LCP2 bytes and stats of lcps_compress2
-    try:
-        ret2 = chain_caller.chain(pts, 1e-3, 16, 2, 0, container="lcp2")
-    except ImportError:
-        return
+    # ec=True (the default) returns the reference's own container: LCP2 bytes
+    # and stats of lcps_compress2
+    ret2 = chain_caller.chain(pts, 1e-3, 16, 2, 0)
     b2, st2 = streams.lcp2()
     assert ret2[0] == b2 and ret2[1:] == st2
+    # this engine's entropy-coded container on request
+    ret3 = chain_caller.chain(pts, 1e-3, 16, 2, 0, container="lch1")
+    assert ret3[0][:4] == b"LCH1"
+    assert np.array_equal(chain_caller.chain_decompress(ret3[0]), dec)

@@ -63,3 +63,42 @@ def test_graph_replays_match_eager():
     assert _containers(g, ptrs_a, ns_a) == eager_a
     g.device_free(pa)
     g.device_free(pb)
+
+
+def test_graph_replay_on_new_data():
+    """Capture on batch A, then replay on batch B of the same frame sizes at
+    the same device address but with other points, eb, threshold and
+    ablation flags (the launch shape, and so the graph key, is unchanged).
+    Frame parameters are uploaded outside the graph, so B's containers must
+    equal B's eager bytes and the oracle's, without a new capture."""
+    import oracle as O
+    from chaincompressor_amd import engine
+    g = engine.Engine(0)
+    ref = engine.Engine(0)
+    sizes = [70_000, 9, 250_000, 33_333]
+    fa = _batch(sizes, 500)
+    # unit-cube data at eb 1e-3 / 1.2e-3 with p = 16: five key bits per
+    # dimension in both batches, so the radix pass count and types agree
+    fb = [f[np.random.default_rng(k).permutation(f.shape[0])] * np.float32(0.93) + np.float32(0.031)
+          for k, f in enumerate(_batch(sizes, 900))]
+    pa, ptrs, ns = _device_batch(g, fa)
+    g.set_graphs(True)
+    da, oa, sa, _ = g.compress_batch_device(ptrs, ns, np.float32, 3, 1e-3, 16, 2)
+    assert [g.to_host(da + o, s) for o, s in zip(oa, sa)] == [ref.compress(f, 1e-3, 16, 2)[0] for f in fa]
+    cap0, _ = g.graph_stats()
+    g.to_device(pa, np.ascontiguousarray(np.concatenate(fb)))
+    ebs, ts, flags = [1.2e-3] * 4, [0, 5, -3, 0], [0, 4, 0, 2]   # NO_L2 on frame 1, NO_L1 on frame 3
+    for rep in range(3):
+        print("new-data replay", rep, flush=True)
+        db, ob, sb, _ = g.compress_batch_device(ptrs, ns, np.float32, 3, ebs, 16, ts, flags=flags)
+        got = [g.to_host(db + o, s) for o, s in zip(ob, sb)]
+        for k, f in enumerate(fb):
+            eager = ref.compress(f, ebs[k], 16, ts[k], flags=flags[k])[0]
+            assert got[k] == eager, k
+            if rep == 0:
+                s = O.compress_streams(f, ebs[k], 16, ts[k], l1=not (flags[k] & 2), l2=not (flags[k] & 4))
+                assert got[k] == s.lch1()[0], k
+    cap1, rep1 = g.graph_stats()
+    assert cap1 == cap0 and rep1 >= 3
+    g.set_graphs(False)
+    g.device_free(pa)

@@ -15,7 +15,8 @@ def eng():
     return engine.Engine(0)
 
 
-@pytest.mark.parametrize("n,eb,p,t", [(10_000_000, 1e-3, 16, 2), (6_000_000, 1e-4, 4, 0)])
+@pytest.mark.parametrize("n,eb,p,t", [(10_000_000, 1e-2, 16, 2), (10_000_000, 1e-3, 16, 2),
+                                      (6_000_000, 1e-4, 4, 0)])
 def test_large_frame_parity(eng, n, eb, p, t):
     from chaincompressor_amd import workloads
     pts = workloads.gaussian_clusters(n, seed=workloads.BASE_SEED + 2)

@@ -115,3 +115,29 @@ def test_failing_request_is_isolated():
     for k, r in enumerate(res):
         if k != 3:
             assert r.status == 0, r.error
+
+
+def test_large_requests_stay_in_the_arenas(eng):
+    """A decompress request of several million points, sized close to one
+    worker's decompression arena, and a million-point compress request run
+    through the server without any arena being reallocated (nothing is
+    allocated while serving), and give the engine's direct results."""
+    from chaincompressor_amd import engine, workloads
+    n = 4_000_000
+    pts = workloads.gaussian_clusters(n, seed=77)
+    blob, _ = eng.compress(pts, 1e-3, 16, 2)
+    srv = engine.Server(0, hbm_budget_bytes=5 << 29, workers=1)   # 2.5 GB: 640 MB decode arena
+    try:
+        a0 = srv.arena_allocations()
+        assert a0 == 2
+        small = pts[:1_000_000]
+        tickets = [srv.submit_decompress(blob), srv.submit_compress(small, 1e-3, 16, 2), srv.submit_decompress(blob)]
+        res = [srv.poll(t) for t in tickets]
+        assert srv.arena_allocations() == a0
+    finally:
+        srv.close()
+    assert all(r.status == 0 for r in res), [r.error for r in res]
+    direct = eng.decompress(blob)
+    assert np.array_equal(res[0].data.view(np.uint64), direct.view(np.uint64))
+    assert np.array_equal(res[2].data.view(np.uint64), direct.view(np.uint64))
+    assert res[1].data == eng.compress(small, 1e-3, 16, 2)[0]

@@ -132,6 +132,7 @@ struct StageTimer {
 struct Staging {
     char* base = nullptr;
     size_t cap = 0, off = 0;
+    bool fixed = false;   // no-allocation calls: never grow, never free
     std::vector<char*> retired;
     struct Pending {
         void* dst;
@@ -141,14 +142,19 @@ struct Staging {
     std::vector<Pending> pend;
     void reset(hipStream_t st) {
         (void)hipStreamSynchronize(st);
-        for (char* r : retired) (void)hipHostFree(r);
-        retired.clear();
+        if (!fixed) {
+            for (char* r : retired) (void)hipHostFree(r);
+            retired.clear();
+        }
         pend.clear();
         off = 0;
     }
     char* take(size_t n) {
         n = (n + 63) & ~size_t(63);
         if (off + n > cap) {
+            if (fixed)
+                throw Error(CHAIN_ENOMEM, "the call's host tables exceed the engine's " + std::to_string(cap) +
+                                              " bytes of pinned staging (no-allocation entry point)");
             if (base) retired.push_back(base);
             base = nullptr;
             const size_t want = std::max<size_t>(n, std::max<size_t>(2 * cap, size_t(1) << 20));
@@ -262,6 +268,7 @@ struct chain_engine {
     chain::GraphCache graph;
     bool autotune = false;
     std::map<uint32_t, chain::RansConfig> tuned;   // size bucket -> setting
+    void* scratch = nullptr;   // this device's no-allocation scratch (device address)
 };
 
 namespace chain {
@@ -1569,6 +1576,12 @@ struct DecodeResult {
     bool keep_on_device = false;
     chain_buf* host_out = nullptr;     // caller's bufs, filled directly (one per stream)
     const uint8_t* d_pool = nullptr;   // device copy of the concatenated streams (skips the upload)
+    // no-allocation form: intermediates carved from this fixed device region
+    // and the coordinates written to the caller's device buffer
+    uint8_t* ext_arena = nullptr;
+    size_t ext_cap = 0;
+    void* ext_out = nullptr;
+    uint64_t ext_out_cap = 0;
     std::vector<uint64_t> ns, out_off;
     uint32_t dims = 0;
     void* d_out = nullptr;
@@ -1684,15 +1697,24 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         x.ticket = c.take<uint32_t>(4);
         x.err = c.take<uint32_t>(4);
         x.totals = c.take<uint64_t>(4);
-        x.out = c.take<uint8_t>(T * D * es);
+        x.out = c.take<uint8_t>(res.ext_out ? 0 : T * D * es);
         return x;
     };
     Carver c0;
     (void)carve(c0);
-    ensure_arena(e, e->arena_d, e->arena, c0.off);
     Carver c1;
-    c1.base = e->arena_d.base;
+    if (res.ext_arena) {
+        require(c0.off <= res.ext_cap, CHAIN_ENOMEM,
+                ("the batch needs " + std::to_string(c0.off) + " bytes of device scratch; the no-allocation entry "
+                 "point has " + std::to_string(res.ext_cap)).c_str());
+        require(T * D * es <= res.ext_out_cap, CHAIN_EINVAL, "the output buffer is smaller than the decoded coordinates");
+        c1.base = reinterpret_cast<char*>(res.ext_arena);
+    } else {
+        ensure_arena(e, e->arena_d, e->arena, c0.off);
+        c1.base = e->arena_d.base;
+    }
     auto X = carve(c1);
+    if (res.ext_out) X.out = static_cast<uint8_t*>(res.ext_out);
     if (res.d_pool) X.in = const_cast<uint8_t*>(res.d_pool);
     Staging& sg = e->stage;
     // algorithmic bytes of every timed region, from the plan
@@ -1986,6 +2008,40 @@ using chain::Error;
 
 static thread_local std::string g_create_err;
 
+// pinned host staging reserved at engine creation; the no-allocation entry
+// point's host tables must fit in it
+#define CHAIN_NOALLOC_STAGING_BYTES (4ull << 20)
+
+// Device scratch of the no-allocation entry point: a fixed region per GPU,
+// held from module load to process end. In host code the array's name is only
+// a stand-in for the symbol; its device address is looked up once per device
+// (with that device current) when an engine is created, and only that address
+// is handed to kernels.
+#define CHAIN_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) uint8_t noalloc_scratch[CHAIN_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[CHAIN_NOALLOC_MAX_DEVICES];
+
+// with `device` current: the device address of its scratch (looked up on the
+// device's first engine), or nullptr with *why set
+static void* scratch_for_device(int device, std::string* why) {
+    if (device < 0 || device >= CHAIN_NOALLOC_MAX_DEVICES) {
+        *why = "device index beyond the no-allocation scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+        if (rc != hipSuccess || !ptr) {
+            *why = std::string("hipGetSymbolAddress(noalloc_scratch): ") + hipGetErrorString(rc);
+            return nullptr;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
 template <class Fn>
 static int guarded(chain_engine* e, Fn&& fn) {
     if (!e) return CHAIN_EINVAL;
@@ -2032,6 +2088,23 @@ int chain_engine_create(int device, uint64_t hbm_budget_bytes, chain_engine** ou
     }
     e->timer.s = e->stream;
     e->timer.on = true;
+    // the no-allocation entry point's device scratch and pinned staging are
+    // set up here, so that its calls allocate nothing
+    std::string why;
+    e->scratch = scratch_for_device(device, &why);
+    if (!e->scratch) {
+        g_create_err = why;
+        chain_engine_destroy(e);
+        return CHAIN_EDEVICE;
+    }
+    try {
+        (void)e->stage.take(CHAIN_NOALLOC_STAGING_BYTES);
+        e->stage.off = 0;
+    } catch (const std::exception& x) {
+        g_create_err = x.what();
+        chain_engine_destroy(e);
+        return CHAIN_ENOMEM;
+    }
     *out = e;
     return CHAIN_OK;
 }
@@ -2155,6 +2228,41 @@ int chain_decompress_batch_device(chain_engine* e, const uint8_t* const* bytes, 
         res.d_pool = (const uint8_t*)d_streams;
         chain::decompress_batch_impl(e, bytes, lens, nstreams, out_dtype, out_layout, res);
         *d_out = res.d_out;
+        for (uint32_t k = 0; k < nstreams; ++k) {
+            out_offsets[k] = res.out_off[k];
+            if (ns) ns[k] = res.ns[k];
+            if (dims) dims[k] = res.dims;
+        }
+    });
+}
+
+int chain_decompress_batch_noalloc(chain_engine* e, const uint8_t* const* bytes, const size_t* lens,
+                                   const void* d_streams, uint32_t nstreams, int out_dtype, int out_layout,
+                                   void* d_out, uint64_t out_capacity, uint64_t* out_offsets, uint64_t* ns,
+                                   uint32_t* dims) {
+    return guarded(e, [&] {
+        if (!bytes || !lens || !d_streams || !d_out || !out_offsets) throw Error(CHAIN_EINVAL, "NULL argument");
+        if (!e->scratch) throw Error(CHAIN_EDEVICE, "the engine has no no-allocation scratch");
+        chain::DecodeResult res;
+        res.d_pool = (const uint8_t*)d_streams;
+        res.ext_arena = static_cast<uint8_t*>(e->scratch);
+        res.ext_cap = CHAIN_NOALLOC_SCRATCH_BYTES;
+        res.ext_out = d_out;
+        res.ext_out_cap = out_capacity;
+        res.keep_on_device = true;
+        // no event creation, no pinned staging growth or release in this call
+        const bool timer_was = e->timer.on;
+        e->timer.on = false;
+        e->stage.fixed = true;
+        try {
+            chain::decompress_batch_impl(e, bytes, lens, nstreams, out_dtype, out_layout, res);
+        } catch (...) {
+            e->timer.on = timer_was;
+            e->stage.fixed = false;
+            throw;
+        }
+        e->timer.on = timer_was;
+        e->stage.fixed = false;
         for (uint32_t k = 0; k < nstreams; ++k) {
             out_offsets[k] = res.out_off[k];
             if (ns) ns[k] = res.ns[k];

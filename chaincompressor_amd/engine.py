@@ -55,7 +55,8 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_last_stage_bytes", "chain_engine_set_timing", "chain_server_create", "chain_server_destroy",
            "chain_server_last_error", "chain_serve_submit", "chain_serve_poll", "chain_server_clock",
            "chain_engine_set_graphs", "chain_engine_graph_stats", "chain_engine_autotune",
-           "chain_engine_set_autotune", "chain_compress_search", "chain_server_stats"]
+           "chain_engine_set_autotune", "chain_compress_search", "chain_server_stats",
+           "chain_decompress_batch_noalloc"]
 
 CHAIN_PENDING = 5
 REQ_COMPRESS, REQ_DECOMPRESS = 0, 1
@@ -106,6 +107,9 @@ def load_library():
     L.chain_decompress_batch_device.argtypes = [C.c_void_p, P(P(C.c_uint8)), P(C.c_size_t), C.c_void_p, C.c_uint32,
                                                 C.c_int, C.c_int, P(C.c_void_p), P(C.c_uint64), P(C.c_uint64),
                                                 P(C.c_uint32)]
+    L.chain_decompress_batch_noalloc.argtypes = [C.c_void_p, P(P(C.c_uint8)), P(C.c_size_t), C.c_void_p, C.c_uint32,
+                                                 C.c_int, C.c_int, C.c_void_p, C.c_uint64, P(C.c_uint64),
+                                                 P(C.c_uint64), P(C.c_uint32)]
     L.chain_device_malloc.argtypes = [C.c_void_p, C.c_uint64, P(C.c_void_p)]
     L.chain_device_free.argtypes = [C.c_void_p, C.c_void_p]
     L.chain_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
@@ -379,6 +383,23 @@ class Engine:
                                                    C.byref(dout), offs, ns, ds)
         self._check(rc)
         return int(dout.value or 0), list(offs), list(ns), int(ds[0]) if k else 0
+
+    def decompress_batch_noalloc(self, streams, d_streams, d_out, out_capacity, dtype=np.float64, layout="aos"):
+        """Decode into the caller's device buffer d_out without any device
+        allocation (intermediates in the library's fixed scratch). Returns
+        (element offsets, ns, dims)."""
+        k = len(streams)
+        bufs = [bytes(s) for s in streams]
+        ptrs = (C.POINTER(C.c_uint8) * k)(*[C.cast(C.c_char_p(b), C.POINTER(C.c_uint8)) for b in bufs])
+        lens = (C.c_size_t * k)(*[len(s) for s in streams])
+        offs = (C.c_uint64 * k)()
+        ns = (C.c_uint64 * k)()
+        ds = (C.c_uint32 * k)()
+        rc = self._L.chain_decompress_batch_noalloc(self._h, ptrs, lens, C.c_void_p(d_streams), k, _dtype_code(dtype),
+                                                    CHAIN_AOS if layout == "aos" else CHAIN_SOA, C.c_void_p(d_out),
+                                                    int(out_capacity), offs, ns, ds)
+        self._check(rc)
+        return list(offs), list(ns), int(ds[0]) if k else 0
 
     def device_malloc(self, nbytes: int) -> int:
         p = C.c_void_p()

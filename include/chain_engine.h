@@ -132,6 +132,27 @@ int chain_decompress_batch_device(chain_engine* e, const uint8_t* const* bytes, 
                                   const void* d_streams, uint32_t nstreams, int out_dtype, int out_layout,
                                   void** d_out, uint64_t* out_offsets, uint64_t* ns, uint32_t* dims);
 
+/* Decompression that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same inputs as chain_decompress_batch_device (host copies of the streams
+ * plan the launches; d_streams, required, holds the same streams concatenated
+ * in device memory), but the coordinates go to the caller's device buffer
+ * d_out of out_capacity bytes (frames back to back; out_offsets are element
+ * offsets) and every intermediate buffer is carved from the library's fixed
+ * device scratch of CHAIN_NOALLOC_SCRATCH_BYTES per GPU. A batch that needs
+ * more scratch fails with CHAIN_ENOMEM, an output buffer that is too small with
+ * CHAIN_EINVAL. The call makes no device or pinned-host allocation or release
+ * (no hipMalloc, hipFree, hipHostMalloc, hipEventCreate). Every call on one
+ * GPU uses that GPU's one scratch region, which keeps the last call's
+ * contents: calls that overlap on one GPU (from several engines or threads)
+ * race and must be serialised by the caller. */
+#define CHAIN_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int chain_decompress_batch_noalloc(chain_engine* e, const uint8_t* const* bytes, const size_t* lens,
+                                   const void* d_streams, uint32_t nstreams, int out_dtype, int out_layout,
+                                   void* d_out, uint64_t out_capacity, uint64_t* out_offsets, uint64_t* ns,
+                                   uint32_t* dims);
+
 /* Device buffers for callers that keep frames in HBM without another
  * runtime (synchronous copies on the engine's stream; to_device: 1 host to
  * device, 0 device to host, 2 device to device). */

@@ -52,7 +52,8 @@ REGION_KERNELS = {
     "sort_hist": "chain::k_sort_hist",
     "sort_scan": "chain::k_scan_u32",
     "sort_scatter": "chain::k_sort_scatter",
-    "link": "chain::k_link",
+    "tile_link": "chain::k_tile_link",
+    "link": "chain::k_link_items (single-word keys) or k_link / k_link_wide",
     "explicit": "chain::k_explicit + k_frame_counts",
     "l2_stats": "chain::k_l2_stats + k_l2_wrap",
     "l2_split": "chain::k_l2_split + k_frame_split_counts",

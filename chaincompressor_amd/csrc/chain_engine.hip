@@ -743,6 +743,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             FrameOut* fo;
             uint64_t* rec_base;
             uint64_t* totals;   // [0] rec total, [1] diff total, [2] corr total
+            void* item_key;     // one item per (tile, distinct key): key, first and last point
+            uint32_t* item_entry;
+            uint32_t* item_exit;
+            uint32_t* item_total;
         } b;
         b.keys_in = c.take<char>(T * key_bytes * key_words);
         b.ka = c.take<char>(T * key_bytes);
@@ -761,8 +765,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         b.offsets = c.take<uint32_t>(256 * sort_tiles);
         // at most key_bytes radix passes per key word
         const uint64_t passes = key_bytes * key_words;
-        b.lb_words = passes * lb_sort + ntiles + sort_tiles + lb_corr;
-        b.lb_tickets = (uint32_t)passes + 3;
+        b.lb_words = passes * lb_sort + ntiles + sort_tiles + lb_corr + ntiles;
+        b.lb_tickets = (uint32_t)passes + 4;
         b.status = c.take<uint64_t>(b.lb_words);
         b.ticket = c.take<uint32_t>(b.lb_tickets);
         b.blocks = c.take<uint64_t>(T * D);
@@ -772,11 +776,19 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         b.fo = c.take<FrameOut>(F);
         b.rec_base = c.take<uint64_t>(F);
         b.totals = c.take<uint64_t>(4);
+        // single-word keys only (the in-tile link path)
+        const bool items = key_words == 1;
+        b.item_key = c.take<char>(items ? T * key_bytes : 0);
+        b.item_entry = c.take<uint32_t>(items ? T : 0);
+        b.item_exit = c.take<uint32_t>(items ? T : 0);
+        b.item_total = c.take<uint32_t>(1);
         return b;
     };
     Carver cb0;
     (void)carve_b(cb0, 8, 4, D);
-    const size_t size_b = cb0.off;
+    Carver cb1;
+    (void)carve_b(cb1, 8, 4, 1);
+    const size_t size_b = std::max(cb0.off, cb1.off);
 
     // LCH1 entropy stage buffers, sized from the particle counts alone
     const RansConfig rc = e->rans;
@@ -963,8 +975,12 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     uint32_t word_passes[CH_MAXD];
     for (uint32_t w = 0; w < CH_MAXD; ++w) word_passes[w] = (word_bits[w] + 7) / 8;
     if (nwords == 1) word_passes[0] = std::max<uint32_t>(1, (key_bits_max + 7) / 8);
-    // the first radix pass (low byte of key word 0) takes its histogram from quantise
-    const bool hist_q = word_passes[0] >= 1;
+    // Single-word keys: successors resolved inside each tile by the quantise
+    // kernel, and across tiles by sorting one item per (tile, key). Wider
+    // keys: every point sorted; the first radix pass (low byte of key word 0)
+    // takes its histogram from quantise.
+    const bool item_link = nwords == 1;
+    const bool hist_q = !item_link && word_passes[0] >= 1;
 
     Carver cB;
     cB.base = e->arena.base + size_a;
@@ -981,7 +997,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     uint64_t* const lb_expl = B.status + (uint64_t)kbytes * nwords * lb_sort;
     uint64_t* const lb_l2 = lb_expl + ntiles;
     uint64_t* const lb_cor = lb_l2 + sort_tiles;
+    uint64_t* const lb_qlink = lb_cor + lb_corr;
     uint32_t* const tk_expl = B.ticket + kbytes * nwords;
+    uint32_t* const tk_qlink = tk_expl + 3;
     auto issue = [&]() {
         fill64(st, B.status, B.lb_words, 0);
         fill32(st, B.ticket, B.lb_tickets, 0);
@@ -1008,8 +1026,20 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                 });
             });
             tm.end();
-            // K3 sort
-            const KeyT* kin = (const KeyT*)B.keys_in;
+            if (item_link) {
+                // K4a successors inside each tile, one item per (tile, key)
+                tm.begin("tile_link", T * sizeof(KeyT) + 4 * T);
+                with_dims(D, [&](auto dd) {
+                    constexpr int DD = decltype(dd)::value;
+                    hipLaunchKernelGGL((k_tile_link<KeyT, DD>), dim3(ntiles), dim3(CH_WG), 0, st, (const KeyT*)B.keys_in,
+                                       A.fdev, A.tb, F, B.idrel, tk_qlink, lb_qlink, A.err, ntiles,
+                                       (KeyT*)B.item_key, B.item_entry, B.item_exit, B.item_total);
+                });
+                tm.end();
+            }
+            // K3 sort: the items (count on the device) or every point
+            const uint32_t* n_dev = item_link ? B.item_total : nullptr;
+            const KeyT* kin = (const KeyT*)(item_link ? B.item_key : B.keys_in);
             const uint32_t* vin = nullptr;
             KeyT* kout = (KeyT*)B.ka;
             uint32_t* vout = B.va;
@@ -1030,7 +1060,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                 if (!(hist_q && ps_all == 0)) {   // the first pass's histogram came with quantise
                     tm.begin("sort_hist", T * sizeof(KeyT));
                     hipLaunchKernelGGL((k_sort_hist<KeyT>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin, T,
-                                       shift, (uint32_t)sort_tiles, B.counts);
+                                       shift, (uint32_t)sort_tiles, B.counts, n_dev);
                     tm.end();
                 }
                 tm.begin("sort_scan", 2ull * 256 * sort_tiles * 4);
@@ -1041,10 +1071,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                                                      : 2 * T * (sizeof(KeyT) + 4));
                 if (ps_all == 0)
                     hipLaunchKernelGGL((k_sort_scatter<KeyT, true>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
-                                       vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
+                                       vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets, n_dev);
                 else
                     hipLaunchKernelGGL((k_sort_scatter<KeyT, false>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st,
-                                       kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets);
+                                       kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets, n_dev);
                 tm.end();
                 kin = kout;
                 vin = vout;
@@ -1052,10 +1082,13 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                 vout = (vout == B.va) ? B.vb : B.va;
             }
             // K4 link + decision
-            tm.begin("link", T * (sizeof(KeyT) * nwords + 8));
+            tm.begin("link", item_link ? 0 : T * (sizeof(KeyT) * nwords + 8));
             with_dims(D, [&](auto dd) {
                 constexpr int DD = decltype(dd)::value;
-                if (nwords == 1)
+                if (item_link)
+                    hipLaunchKernelGGL((k_link_items<KeyT, DD>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
+                                       vin, B.item_total, B.item_entry, B.item_exit, A.fdev, A.ptb, F, B.idrel);
+                else if (nwords == 1)
                     hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(LINK_XCDS * (unsigned)cdiv(cdiv(T, CH_TILE), LINK_XCDS)),
                                        dim3(CH_WG), 0, st, kin, vin, T, A.fdev, A.ptb, F, B.idrel);
                 else
@@ -1141,12 +1174,25 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     }
 
     const size_t kbytes_all = kbytes * nwords;
+    // on the in-tile link path the sort and link regions work on the M items
+    // (key, first point, last point) counted on the device
+    auto set_item_bytes = [&](uint64_t M) {
+        if (!item_link || !tm.on) return;
+        uint32_t pass = 0;
+        for (size_t i = 0; i < tm.used / 2; ++i) {
+            const std::string& nm = tm.names[i];
+            if (nm == "sort_hist") tm.set_bytes((int)i, M * kbytes);
+            else if (nm == "sort_scatter") tm.set_bytes((int)i, pass++ == 0 ? M * kbytes + M * (kbytes + 4) : 2 * M * (kbytes + 4));
+            else if (nm == "link") tm.set_bytes((int)i, M * (2 * kbytes + 8) + 8 * M);
+        }
+    };
     auto set_pre_bytes = [&](uint64_t R, uint64_t NC) {
         tm.set_bytes(t_explicit, T * (4 + kbytes_all) + 8ull * D * R);
         tm.set_bytes(t_l2s, 8ull * D * R);
         tm.set_bytes(t_l2p, 16ull * D * R + R);
         tm.set_bytes(t_corr, 32ull * NC + 8ull * ntiles);
     };
+    uint32_t n_items = 0;
 
     if (format == CHAIN_FMT_LCH1) {
         std::vector<uint64_t> offs(F), sizes(F);
@@ -1162,7 +1208,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                           {secs.data(), X.sec, secs.size() * sizeof(Section)}});
         sg.d2h(st, errw, A.err, sizeof errw);
         sg.d2h(st, fo.data(), B.fo, F * sizeof(FrameOut));
+        sg.d2h(st, &n_items, B.item_total, 4);
         sg.finish(st);
+        set_item_bytes(n_items);
         if (tm.on) {
             // algorithmic bytes of the regions whose work depends on the data
             uint64_t R = 0, NC = 0;
@@ -1239,7 +1287,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     sg.d2h(st, fo.data(), B.fo, F * sizeof(FrameOut));
     sg.d2h(st, totals, B.totals, sizeof totals);
     sg.d2h(st, errw, A.err, sizeof errw);
+    sg.d2h(st, &n_items, B.item_total, 4);
     sg.finish(st);
+    set_item_bytes(n_items);
     require(errw[0] == 0, CHAIN_EDEVICE, "device error word set (look-back wait limit or invalid block range)");
 
     const uint64_t R = totals[0], ND = totals[1], NC = totals[2];

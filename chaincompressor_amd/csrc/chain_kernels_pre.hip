@@ -379,30 +379,17 @@ __device__ inline bool within_eb(double restored, double original, double eb) {
 }
 
 #define QZ_GROUP 4   // points per thread whose loads are in flight together
-template <typename T, int D, bool SOA, typename KeyT, typename RelT>
-__global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, const FrameDev* __restrict__ fr,
-                                                    const uint32_t* __restrict__ tile_begin, uint32_t nframes,
-                                                    KeyT* __restrict__ key_out, RelT* __restrict__ rel_out,
-                                                    uint64_t* __restrict__ corr_idx, double* __restrict__ corr_val,
-                                                    uint32_t* __restrict__ tile_ncorr, uint32_t nwords,
-                                                    uint64_t total_pts, uint32_t* __restrict__ counts,
-                                                    uint32_t sort_ntiles) {
-    const uint32_t tile = blockIdx.x;
-    const uint32_t f = upper_find(tile_begin, nframes, tile);
-    const FrameDev& F = fr[f];
-    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
-    const uint64_t end = start + CH_TILE < F.n ? start + CH_TILE : F.n;
-    // With counts set, the first radix pass's digit histogram (low byte of
-    // key word 0) is gathered here: the tile's points fall in at most two
-    // sort tiles of CH_TILE global positions, counted in LDS and added to
-    // counts[digit * sort_ntiles + sort tile] (cleared before the launch).
-    __shared__ uint32_t s_h[2][256];
-    const uint64_t st0 = (F.pt_off + start) / CH_TILE;
-    if (counts) {
-        s_h[0][threadIdx.x] = 0;
-        s_h[1][threadIdx.x] = 0;
-        __syncthreads();
-    }
+
+// Quantise the tile [start, end) of frame F: rel offsets and block keys to
+// global memory (and, with S_KEY, every key also to s_key[point - start]);
+// with s_h set, the low-byte histogram of key word 0 for the (at most two)
+// sort tiles the points fall in. Returns the thread's correction flags (bit
+// j*D + d for item j, dimension d).
+template <typename T, int D, bool SOA, typename KeyT, typename RelT, bool S_KEY>
+__device__ inline uint64_t quantize_items(const T* __restrict__ pts, const FrameDev& F, uint64_t start,
+                                          uint64_t end, KeyT* __restrict__ key_out, RelT* __restrict__ rel_out,
+                                          uint32_t nwords, uint64_t total_pts, uint32_t (*s_h)[256], uint64_t st0,
+                                          KeyT* s_key) {
     const double denom = F.denom, eb = F.eb;
     const uint64_t p = F.p, magic = F.p_magic;
     const uint32_t l = F.p_l;
@@ -413,7 +400,6 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
     // frame-uniform fast paths: q < 2^32 converts with one instruction, and
     // a power-of-two p splits q with a shift and a mask
     const bool q32 = (F.flags & FL_Q32) != 0, p2 = (F.flags & FL_P2) != 0;
-    __shared__ uint32_t s_tmp[4];
     uint64_t cmask = 0;   // correction flags, bit j*D + d
 #pragma unroll 1
     for (int j0 = 0; j0 < CH_ITEMS; j0 += QZ_GROUP) {
@@ -461,28 +447,30 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
             }
             if (nwords == 1) {
                 key_out[F.pt_off + i] = key;
+                if (S_KEY) s_key[i - start] = key;
             } else {
 #pragma unroll
                 for (int w = 0; w < D; ++w)
                     if ((uint32_t)w < nwords) key_out[(uint64_t)w * total_pts + F.pt_off + i] = kw[w];
             }
-            if (counts) {
+            if (s_h) {
                 const uint32_t digit = (uint32_t)(nwords == 1 ? key : kw[0]) & 0xffu;
                 atomicAdd(&s_h[(uint32_t)((F.pt_off + i) / CH_TILE - st0)][digit], 1u);
             }
         }
         }
     }
-    if (counts) {
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const uint32_t v = s_h[s][threadIdx.x];
-            if (v) atomicAdd(&counts[(uint64_t)threadIdx.x * sort_ntiles + st0 + s], v);
-        }
-    }
-    // exact corrections, in element order (j, thread, d); rare, so the
-    // ordered write path runs only when the tile has any
+    return cmask;
+}
+
+// Exact corrections of a tile, in element order (j, thread, d), into the
+// tile's slot of corr_idx / corr_val; rare, so the ordered write path runs
+// only when the tile has any. Called by the whole workgroup.
+template <typename T, int D, bool SOA>
+__device__ inline void tile_corrections(const T* __restrict__ pts, const FrameDev& F, uint64_t start, uint32_t tile,
+                                        uint64_t cmask, uint64_t* __restrict__ corr_idx,
+                                        double* __restrict__ corr_val, uint32_t* __restrict__ tile_ncorr,
+                                        uint32_t* s_tmp) {
     uint32_t run = 0;
     const uint64_t cbase = (uint64_t)tile * CH_TILE * D;
     if (__syncthreads_or(cmask != 0)) {
@@ -502,6 +490,44 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
         }
     }
     if (threadIdx.x == 0) tile_ncorr[tile] = run;
+}
+
+template <typename T, int D, bool SOA, typename KeyT, typename RelT>
+__global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, const FrameDev* __restrict__ fr,
+                                                    const uint32_t* __restrict__ tile_begin, uint32_t nframes,
+                                                    KeyT* __restrict__ key_out, RelT* __restrict__ rel_out,
+                                                    uint64_t* __restrict__ corr_idx, double* __restrict__ corr_val,
+                                                    uint32_t* __restrict__ tile_ncorr, uint32_t nwords,
+                                                    uint64_t total_pts, uint32_t* __restrict__ counts,
+                                                    uint32_t sort_ntiles) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t f = upper_find(tile_begin, nframes, tile);
+    const FrameDev& F = fr[f];
+    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
+    const uint64_t end = start + CH_TILE < F.n ? start + CH_TILE : F.n;
+    // With counts set, the first radix pass's digit histogram (low byte of
+    // key word 0) is gathered here: the tile's points fall in at most two
+    // sort tiles of CH_TILE global positions, counted in LDS and added to
+    // counts[digit * sort_ntiles + sort tile] (cleared before the launch).
+    __shared__ uint32_t s_h[2][256];
+    const uint64_t st0 = (F.pt_off + start) / CH_TILE;
+    if (counts) {
+        s_h[0][threadIdx.x] = 0;
+        s_h[1][threadIdx.x] = 0;
+        __syncthreads();
+    }
+    __shared__ uint32_t s_tmp[4];
+    const uint64_t cmask = quantize_items<T, D, SOA, KeyT, RelT, false>(
+        pts, F, start, end, key_out, rel_out, nwords, total_pts, counts ? s_h : nullptr, st0, nullptr);
+    if (counts) {
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const uint32_t v = s_h[s][threadIdx.x];
+            if (v) atomicAdd(&counts[(uint64_t)threadIdx.x * sort_ntiles + st0 + s], v);
+        }
+    }
+    tile_corrections<T, D, SOA>(pts, F, start, tile, cmask, corr_idx, corr_val, tile_ncorr, s_tmp);
 }
 
 // gather per-tile correction lists into one ordered list
@@ -535,7 +561,10 @@ __global__ __launch_bounds__(CH_WG) void k_corr_gather(const uint64_t* __restric
 #define SORT_WAVE_ITEMS (CH_TILE / CH_WG)   // items per lane
 template <typename KeyT>
 __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ keys, uint64_t T, uint32_t shift,
-                                                     uint32_t ntiles, uint32_t* __restrict__ counts) {
+                                                     uint32_t ntiles, uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ n_dev) {
+    // n_dev: the item count lives on the device (tiles past it count nothing)
+    if (n_dev) T = *n_dev;
     // one histogram per wavefront: atomics of different wavefronts never
     // meet on a word; the four are summed at the end
     __shared__ uint32_t s_h[CH_WG / 64][256];
@@ -546,7 +575,7 @@ __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ ke
 #pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {   // unconditional loads (the tile's first key past the end)
         const uint64_t i = base + (uint64_t)j * CH_WG + threadIdx.x;
-        kx[j] = keys[i < T ? i : base];
+        kx[j] = keys[i < T ? i : (base < T ? base : 0)];
     }
     __syncthreads();
     uint32_t* h = s_h[threadIdx.x >> 6];
@@ -567,7 +596,10 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
                                                         KeyT* __restrict__ kout, uint32_t* __restrict__ vout,
                                                         uint64_t T, uint32_t shift, uint32_t ntiles,
                                                         const uint32_t* __restrict__ counts,
-                                                        const uint32_t* __restrict__ offsets) {
+                                                        const uint32_t* __restrict__ offsets,
+                                                        const uint32_t* __restrict__ n_dev) {
+    if (n_dev) T = *n_dev;   // the item count lives on the device
+    if ((uint64_t)blockIdx.x * CH_TILE >= T) return;
     __shared__ uint32_t s_lstart[256], s_goff[256], s_tmp[4];
     __shared__ uint32_t s_wcnt[4][256];
     __shared__ KeyT s_key[CH_TILE];
@@ -647,6 +679,56 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
 }
 
 // ------------------------------------------------------------------------
+// The chain decision (cpp/chain/lcps_cpp.cpp:2155-2200): a point whose
+// successor lies `dist` indices later stores the distance instead of its
+// block when bitlen(dist) + bitlen(bitlen(dist)) is below the cheaper explicit
+// cost, min(D lb + bitlen(lb), D bitlen(max block id)), minus the threshold,
+// with lb the bit length of the point's largest block coordinate. `key` is
+// the concatenated block ids of a single-word key.
+template <int D>
+__device__ inline bool chain_relative_mb(const FrameDev& F, uint64_t mb, uint64_t dist) {
+    const int64_t local_bits = bitlen64(mb);
+    const int64_t tag_bits = bitlen64((uint64_t)local_bits);
+    const int64_t a = local_bits * D + tag_bits;
+    const int64_t c = (int64_t)D * (int64_t)F.max_bid_bits;
+    const int64_t smaller = a < c ? a : c;
+    const int64_t bl = bitlen64(dist);
+    const int64_t lhs = bl + bitlen64((uint64_t)bl);
+    return lhs < smaller - F.threshold;
+}
+// mb: the point's largest block coordinate
+template <int D>
+__device__ inline bool chain_relative(const FrameDev& F, uint64_t key, uint64_t dist) {
+    uint64_t mb = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const uint32_t wd = F.key_width[d];
+        const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
+        const uint64_t b = (key >> F.key_shift[d]) & mask;
+        mb = b > mb ? b : mb;
+    }
+    return chain_relative_mb<D>(F, mb, dist);
+}
+
+// The same decision as a distance limit: for a point whose largest block
+// coordinate has bit length lb, the distance is stored iff dist < limit(lb).
+// The left side bl + bitlen(bl), bl = bitlen(dist), never decreases with dist,
+// so the distances that qualify are exactly those below 2^bl* for the largest
+// qualifying bl* (none: limit 1, i.e. never, since dist >= 1).
+// Only bit lengths up to max_bl are examined: the limit is exact for every
+// dist < 2^max_bl (in-tile distances are below 4096 = 2^12).
+template <int D>
+__device__ inline uint64_t chain_limit(const FrameDev& F, int lb, int max_bl) {
+    const int64_t a = (int64_t)lb * D + bitlen64((uint64_t)lb);
+    const int64_t c = (int64_t)D * (int64_t)F.max_bid_bits;
+    const int64_t rhs = (a < c ? a : c) - F.threshold;
+    int best = 0;
+    for (int bl = 1; bl <= max_bl; ++bl)
+        if (bl + bitlen64((uint64_t)bl) < rhs) best = bl;
+    return best ? (1ull << best) : 1ull;   // dist < 2^best, or nothing
+}
+
+// ------------------------------------------------------------------------
 // K4: successor of every point inside its block and the chain decision
 // (cpp/chain/lcps_cpp.cpp:2155-2200), computed in sorted order and scattered
 // to index order.
@@ -699,25 +781,375 @@ __global__ __launch_bounds__(CH_WG) void k_link(const KeyT* __restrict__ skey, c
             const FrameDev& F = fr[f];
             if ((F.flags & FL_L1) && j < offs[f + 1]) {
                 const uint64_t dist = (uint64_t)(j - i);
-                uint64_t mb = 0;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const uint32_t wd = F.key_width[d];
-                    const uint64_t mask = wd >= 64 ? ~0ull : ((1ull << wd) - 1ull);
-                    const uint64_t b = ((uint64_t)key >> F.key_shift[d]) & mask;
-                    mb = b > mb ? b : mb;
-                }
-                const int64_t local_bits = bitlen64(mb);
-                const int64_t tag_bits = bitlen64((uint64_t)local_bits);
-                const int64_t a = local_bits * D + tag_bits;
-                const int64_t c = (int64_t)D * (int64_t)F.max_bid_bits;
-                const int64_t smaller = a < c ? a : c;
-                const int64_t bl = bitlen64(dist);
-                const int64_t lhs = bl + bitlen64((uint64_t)bl);
-                if (lhs < smaller - F.threshold) out = (uint32_t)dist;
+                if (chain_relative<D>(F, (uint64_t)key, dist)) out = (uint32_t)dist;
             }
         }
         idrel[i] = out;
+    }
+}
+
+// ------------------------------------------------------------------------
+// K4 for single-word keys, tile by tile: the successors that stay inside a
+// tile of 4096 points of one frame (the quantise tiles).
+//
+// next[i] (the smallest j > i of the frame with the same block) is found in
+// two steps. Inside the tile, a stable LDS radix sort of (key, position)
+// puts equal keys next to each other in index order, so every point but the
+// last of its key in the tile has its successor right after it; those links
+// and their chain decisions are final here. The last point of each key in
+// the tile (its exit) links to the first point of that key in the next tile
+// of the frame that holds the key: each tile emits one item (key, first
+// point, last point) per distinct key, in tile order (ordered compaction by
+// a look-back over ticketed tiles), the items are sorted by key (stable, so
+// a key's items stay in tile order) and k_link_items links every item's last
+// point to the next item's first point. Exits are written 0 here and
+// overwritten there when they chain. For data with any index locality the
+// items are a small fraction of the points, and the global sort and its
+// scattered link writes shrink with them.
+#define QL_ITEMS (CH_TILE / CH_WG)   // positions per thread (16)
+//
+// The sort runs on a tile key: each dimension's block id minus the tile's
+// smallest, packed into the bits the tile's range of that dimension needs.
+// It is one-to-one on the tile's keys, so it groups exactly like the full key,
+// and a tile of a compact cluster needs a single pass of a few bits where the
+// full key needs two or more passes of eight (the ranking is bound by its
+// ballots, four instructions per key bit and item).
+template <typename KeyT, int D>
+__global__ __launch_bounds__(CH_WG) void k_tile_link(
+    const KeyT* __restrict__ keys, const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_begin,
+    uint32_t nframes, uint32_t* __restrict__ idrel, uint32_t* ticket, uint64_t* status, uint32_t* err,
+    uint32_t ntiles, KeyT* __restrict__ item_key, uint32_t* __restrict__ item_entry,
+    uint32_t* __restrict__ item_exit, uint32_t* __restrict__ item_total) {
+    __shared__ KeyT s_key[CH_TILE];   // tile keys
+    __shared__ __attribute__((aligned(16))) uint16_t s_ia[CH_TILE];
+    __shared__ __attribute__((aligned(16))) uint16_t s_ib[CH_TILE];
+    __shared__ uint16_t s_link[CH_TILE];   // in-tile links by point (< 4096; exits 0)
+    __shared__ uint32_t s_wcnt[CH_WG / 64][256];
+    __shared__ uint32_t s_tmp[4], s_tile;
+    __shared__ KeyT s_rng[CH_WG / 64][2 * CH_MAXD];
+    __shared__ uint64_t s_lim[65];   // distance limit by bit length of the largest block coordinate
+    __shared__ uint64_t s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t f = upper_find(tile_begin, nframes, tile);
+    const FrameDev& F = fr[f];
+    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
+    const uint64_t end = start + CH_TILE < F.n ? start + CH_TILE : F.n;
+    const uint32_t cnt = (uint32_t)(end - start);
+    const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
+    uint32_t* const out = idrel + F.pt_off + start;
+    const bool l1 = (F.flags & FL_L1) != 0;
+    uint32_t nruns = 0;
+    uint16_t* src = s_ia;
+    uint16_t* dst = s_ib;
+    // block layout of the frame's keys, and of the tile keys
+    uint32_t ksh[D];
+    KeyT kmask[D], bmin[D], cmask[D];
+    uint32_t csh[D], cw[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        ksh[d] = F.key_shift[d];
+        const uint32_t wd = F.key_width[d];
+        kmask[d] = wd >= 8 * sizeof(KeyT) ? ~KeyT(0) : (KeyT)((KeyT(1) << wd) - 1);
+    }
+    if (l1) {
+        if (tid < 65) s_lim[tid] = chain_limit<D>(F, (int)tid, 12);
+        {
+            // the tile's keys, coalesced, every load issued first (past the
+            // end: the tile's first key, not counted)
+            const KeyT* kp = keys + F.pt_off + start;
+            KeyT kv[QL_ITEMS];
+#pragma unroll
+            for (int k = 0; k < QL_ITEMS; ++k) {
+                const uint32_t j = k * CH_WG + tid;
+                kv[k] = kp[j < cnt ? j : 0u];
+            }
+            // per-dimension range of the tile's block ids
+            KeyT mn[D], mx[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                mn[d] = (kv[0] >> ksh[d]) & kmask[d];
+                mx[d] = mn[d];
+            }
+            // (a position past the end loaded the tile's first key: it leaves
+            // the range as it is)
+#pragma unroll
+            for (int k = 1; k < QL_ITEMS; ++k) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const KeyT b = (kv[k] >> ksh[d]) & kmask[d];
+                    mn[d] = b < mn[d] ? b : mn[d];
+                    mx[d] = b > mx[d] ? b : mx[d];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                for (int o = 32; o >= 1; o >>= 1) {
+                    const KeyT a = __shfl_xor(mn[d], o, 64), c = __shfl_xor(mx[d], o, 64);
+                    mn[d] = a < mn[d] ? a : mn[d];
+                    mx[d] = c > mx[d] ? c : mx[d];
+                }
+                if (lane == 0) {
+                    s_rng[w][2 * d] = mn[d];
+                    s_rng[w][2 * d + 1] = mx[d];
+                }
+            }
+            __syncthreads();
+            uint32_t acc = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                KeyT a = s_rng[0][2 * d], c = s_rng[0][2 * d + 1];
+#pragma unroll
+                for (int q = 1; q < CH_WG / 64; ++q) {
+                    a = s_rng[q][2 * d] < a ? s_rng[q][2 * d] : a;
+                    c = s_rng[q][2 * d + 1] > c ? s_rng[q][2 * d + 1] : c;
+                }
+                bmin[d] = a;
+                cw[d] = (uint32_t)bitlen64((uint64_t)(c - a));
+                cmask[d] = cw[d] >= 8 * sizeof(KeyT) ? ~KeyT(0) : (KeyT)((KeyT(1) << cw[d]) - 1);
+                csh[d] = acc;
+                acc += cw[d];
+            }
+#pragma unroll
+            for (int k = 0; k < QL_ITEMS; ++k) {
+                const uint32_t j = k * CH_WG + tid;
+                KeyT ck = 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) ck |= (((kv[k] >> ksh[d]) & kmask[d]) - bmin[d]) << csh[d];
+                if (j < cnt) s_key[j] = ck;
+                s_ia[j] = (uint16_t)j;
+            }
+        }
+        // tile keys and the identity order are read by every thread below
+        // (also when no pass runs: every key of the tile equal)
+        __syncthreads();
+        const uint32_t cbits = csh[D - 1] + cw[D - 1];
+        // Stable LSD passes of up to 8 bits. Wavefront w ranks the positions
+        // [w * 1024, (w + 1) * 1024), item r of lane l at w * 1024 + r * 64 + l,
+        // in order: ballots give each lane the lanes of its digit, and the
+        // lowest of them adds the group's size to the wavefront's counter of
+        // that digit with one LDS atomic that returns the count before it. A
+        // wavefront's LDS operations complete in order, so consecutive rounds'
+        // atomics are issued back to back and their results read afterwards.
+        for (uint32_t shift = 0; shift < cbits; shift += 8) {
+            const uint32_t nb = cbits - shift < 8u ? cbits - shift : 8u;
+#pragma unroll
+            for (int k = 0; k < CH_WG / 64; ++k) s_wcnt[k][tid] = 0;
+            __syncthreads();
+            // per item one word: position (12 bits) | digit << 12 | rank in its
+            // group of the round << 20 | the group's leader lane << 26, then
+            // position | digit << 12 | rank in the wavefront's digit << 20
+            uint32_t it[QL_ITEMS];
+#pragma unroll
+            for (int r = 0; r < QL_ITEMS; ++r) {
+                const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
+                it[r] = src[p < cnt ? p : 0u];
+            }
+#pragma unroll
+            for (int r = 0; r < QL_ITEMS; ++r) it[r] |= ((uint32_t)(s_key[it[r]] >> shift) & ((1u << nb) - 1u)) << 12;
+            // four rounds' atomics in flight at a time (register pressure)
+#pragma unroll
+            for (int h = 0; h < QL_ITEMS; h += QL_ITEMS / 4) {
+                uint32_t bf[QL_ITEMS / 4];
+#pragma unroll
+                for (int rr = 0; rr < QL_ITEMS / 4; ++rr) {
+                    const int r = h + rr;
+                    const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
+                    const bool valid = p < cnt;
+                    const uint32_t dg = (it[r] >> 12) & 0xffu;
+                    uint64_t peers = __ballot(valid);
+                    // all 8 bits, unrolled: a bit above the pass's width is 0 in
+                    // every lane's digit, so it leaves the mask as it is
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const uint64_t bb = __ballot((dg >> b) & 1u);
+                        peers &= ((dg >> b) & 1u) ? bb : ~bb;
+                    }
+                    const uint32_t prank = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+                    const uint32_t lead = valid ? (uint32_t)__ffsll((unsigned long long)peers) - 1u : lane;
+                    it[r] |= (prank << 20) | (lead << 26);
+                    bf[rr] = 0;
+                    if (valid && prank == 0) bf[rr] = atomicAdd(&s_wcnt[w][dg], (uint32_t)__popcll(peers));
+                }
+                // every lane: its group's count before this round (from the
+                // leader) + its rank in the group
+#pragma unroll
+                for (int rr = 0; rr < QL_ITEMS / 4; ++rr) {
+                    const int r = h + rr;
+                    const uint32_t wr = __shfl(bf[rr], (int)(it[r] >> 26), 64) + ((it[r] >> 20) & 63u);
+                    it[r] = (it[r] & 0xfffffu) | (wr << 20);
+                }
+            }
+            __syncthreads();
+            {
+                // digit tid: its first position in the tile, then every wavefront's
+                const uint32_t a0 = s_wcnt[0][tid], a1 = s_wcnt[1][tid], a2 = s_wcnt[2][tid], a3 = s_wcnt[3][tid];
+                uint32_t tot;
+                const uint32_t st = block_excl_scan(a0 + a1 + a2 + a3, s_tmp, &tot);
+                s_wcnt[0][tid] = st;
+                s_wcnt[1][tid] = st + a0;
+                s_wcnt[2][tid] = st + a0 + a1;
+                s_wcnt[3][tid] = st + a0 + a1 + a2;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < QL_ITEMS; ++r) {
+                const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
+                if (p < cnt) dst[s_wcnt[w][(it[r] >> 12) & 0xffu] + (it[r] >> 20)] = (uint16_t)(it[r] & 0xfffu);
+            }
+            __syncthreads();
+            uint16_t* t = src;
+            src = dst;
+            dst = t;
+        }
+        // Sorted positions [tid * 16, tid * 16 + 16) of this thread: links to
+        // the next position of the same key (into s_link by point), run
+        // starts counted for the items.
+        const uint32_t s0 = tid * QL_ITEMS;
+        uint32_t first_bits = 0;
+        if (s0 < cnt) {
+            // positions >= cnt of src hold no sorted entry: never used as an index
+            uint32_t ix[QL_ITEMS + 1];
+            const uint4* v = reinterpret_cast<const uint4*>(src + s0);
+            const uint4 v0 = v[0], v1 = v[1];
+            const uint32_t wd[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                ix[2 * q] = wd[q] & 0xffffu;
+                ix[2 * q + 1] = wd[q] >> 16;
+            }
+            ix[QL_ITEMS] = src[s0 + QL_ITEMS < cnt ? s0 + QL_ITEMS : 0u];
+#pragma unroll
+            for (int k = 0; k <= QL_ITEMS; ++k) ix[k] = s0 + k < cnt ? ix[k] : ix[0];   // a valid point
+            KeyT kp = s_key[src[s0 > 0 ? s0 - 1 : 0u]];
+            KeyT kc = s_key[ix[0]];
+#pragma unroll
+            for (int k = 0; k < QL_ITEMS; ++k) {
+                const uint32_t sp = s0 + k;
+                const KeyT kn = s_key[ix[k + 1]];
+                const bool has_next = sp + 1 < cnt && kn == kc;
+                const bool first = sp < cnt && (sp == 0 || kp != kc);
+                first_bits |= (first ? 1u : 0u) << k;
+                uint32_t lb;
+                if (sizeof(KeyT) == 4) {
+                    uint32_t mb = 0;   // largest block coordinate
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const uint32_t b = (uint32_t)(bmin[d] + ((kc >> csh[d]) & cmask[d]));
+                        mb = b > mb ? b : mb;
+                    }
+                    lb = 32u - (uint32_t)__clz(mb);
+                } else {
+                    uint64_t mb = 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const uint64_t b = (uint64_t)(bmin[d] + ((kc >> csh[d]) & cmask[d]));
+                        mb = b > mb ? b : mb;
+                    }
+                    lb = (uint32_t)bitlen64(mb);
+                }
+                const uint32_t dist = ix[k + 1] - ix[k];   // > 0 when has_next: the sort is stable
+                const bool rel = has_next && (uint64_t)dist < s_lim[lb];
+                if (sp < cnt) s_link[ix[k]] = (uint16_t)(rel ? dist : 0u);
+                kp = kc;
+                kc = kn;
+            }
+        }
+        const uint32_t myruns = (uint32_t)__popc(first_bits);
+        const uint32_t run0 = block_excl_scan(myruns, s_tmp, &nruns);
+        // run r starts at dst[r] (the other buffer is free now)
+        uint32_t rr = run0;
+#pragma unroll
+        for (int k = 0; k < QL_ITEMS; ++k)
+            if ((first_bits >> k) & 1u) dst[rr++] = (uint16_t)(s0 + k);
+        // items of this tile, placed after every earlier tile's (tile order)
+        if (tid < 64) {
+            const uint64_t e = lookback(status, tile, nruns, err);
+            if (tid == 0) {
+                s_excl = e;
+                if (tile == ntiles - 1) *item_total = (uint32_t)(e + nruns);
+            }
+        }
+        __syncthreads();
+        {
+            const uint64_t base = s_excl;
+            for (uint32_t r = tid; r < nruns; r += CH_WG) {
+                const uint32_t a = dst[r], b = (r + 1 < nruns ? dst[r + 1] : cnt) - 1u;
+                const uint32_t en = src[a], ex = src[b];
+                const KeyT ck = s_key[en];
+                KeyT key = 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) key |= (bmin[d] + ((ck >> csh[d]) & cmask[d])) << ksh[d];
+                item_key[base + r] = key;
+                item_entry[base + r] = (uint32_t)(F.pt_off + start + en);
+                item_exit[base + r] = (uint32_t)(F.pt_off + start + ex);
+            }
+        }
+        // the links in index order, written out coalesced (exits 0: they are
+        // linked across tiles by k_link_items)
+#pragma unroll
+        for (int k = 0; k < QL_ITEMS; ++k) {
+            const uint32_t j = k * CH_WG + tid;
+            if (j < cnt) out[j] = s_link[j];
+        }
+    } else {
+        // no L1 in this frame: every point is explicit, and the tile has no items
+#pragma unroll
+        for (int k = 0; k < QL_ITEMS; ++k) {
+            const uint32_t j = k * CH_WG + tid;
+            if (j < cnt) out[j] = 0;
+        }
+        if (tid < 64) {
+            const uint64_t e = lookback(status, tile, 0, err);
+            if (tid == 0 && tile == ntiles - 1) *item_total = (uint32_t)e;
+        }
+    }
+}
+
+// The cross-tile links: items sorted by key (stable), so item s + 1 of the
+// same key is the next tile holding it; the last point of item s links to
+// the first point of item s + 1 when both lie in one frame. Grid sized for
+// the largest item count; positions past the actual count exit.
+template <typename KeyT, int D>
+__global__ __launch_bounds__(CH_WG) void k_link_items(const KeyT* __restrict__ skey, const uint32_t* __restrict__ sval,
+                                                      const uint32_t* __restrict__ item_total,
+                                                      const uint32_t* __restrict__ item_entry,
+                                                      const uint32_t* __restrict__ item_exit,
+                                                      const FrameDev* __restrict__ fr,
+                                                      const uint32_t* __restrict__ frame_pt_begin, uint32_t nframes,
+                                                      uint32_t* __restrict__ idrel) {
+    __shared__ uint32_t s_off[CH_MAXF_LDS + 1];
+    const uint32_t* offs = frame_pt_begin;
+    if (nframes <= CH_MAXF_LDS) {
+        for (uint32_t k = threadIdx.x; k <= nframes; k += CH_WG) s_off[k] = frame_pt_begin[k];
+        __syncthreads();
+        offs = s_off;
+    }
+    const uint64_t M = *item_total;
+    const uint64_t s0 = (uint64_t)blockIdx.x * CH_TILE;
+    if (s0 >= M) return;
+    uint32_t av[CH_ITEMS], bv[CH_ITEMS];
+    KeyT kv[CH_ITEMS], kn[CH_ITEMS];
+#pragma unroll
+    for (int q = 0; q < CH_ITEMS; ++q) {
+        const uint64_t s = s0 + (uint64_t)q * CH_WG + threadIdx.x;
+        // unconditional loads at valid positions (the tile's first one)
+        const uint64_t sc = s < M ? s : s0, nc = s + 1 < M ? s + 1 : s0;
+        av[q] = sval[sc];
+        kv[q] = skey[sc];
+        bv[q] = sval[nc];
+        kn[q] = skey[nc];
+    }
+#pragma unroll
+    for (int q = 0; q < CH_ITEMS; ++q) {
+        const uint64_t s = s0 + (uint64_t)q * CH_WG + threadIdx.x;
+        if (s + 1 >= M || kn[q] != kv[q]) continue;
+        const uint32_t ex = item_exit[av[q]], en = item_entry[bv[q]];
+        const uint32_t f = upper_find(offs, nframes, ex);
+        const FrameDev& F = fr[f];
+        if (en < offs[f + 1] && chain_relative<D>(F, (uint64_t)kv[q], (uint64_t)(en - ex))) idrel[ex] = en - ex;
     }
 }
 

@@ -227,11 +227,11 @@ def run_plain(args, eng):
         # the engine's own tuner picks the entropy chunking for this frame;
         # repeated compressions of the same shape replay a captured HIP graph
         if args.rans_chunk:
-            lanes, chunk = 64, args.rans_chunk
             eng.set_autotune(False)
-            eng.set_rans(lanes, chunk)
+            eng.set_tuning(64, args.rans_chunk, 1)
+            tuning = eng.tuning()
         else:
-            (lanes, chunk), _ = eng.autotune([dptr], [n], np.float32, 3, eb, p, thr)
+            tuning, _ = eng.autotune([dptr], [n], np.float32, 3, eb, p, thr)
 
         def compress(graph=True):
             eng.set_graphs(graph)
@@ -273,10 +273,10 @@ def run_plain(args, eng):
         maxerr = float(np.max(np.abs(dec - pts)))
         assert maxerr <= eb, maxerr
         eng.device_free(dblob)
-        eng.set_rans(64, 65536)
+        eng.set_tuning()
         return {"compress_gbps": nbytes * steps / tc / 1e9, "decompress_gbps": nbytes * steps / td / 1e9,
                 "ms_per_step": (tc + td) / steps * 1e3, "bytes": int(sizes[0]), "st": st[0], "maxerr": maxerr,
-                "c_steps": c_steps, "d_steps": d_steps, "rans": {"lanes_max": lanes, "chunk_symbols": chunk}}
+                "c_steps": c_steps, "d_steps": d_steps, "rans": tuning}
 
     if args.calibrate_copy:
         # a plain device-to-device copy of known size, for scaling HBM
@@ -329,9 +329,8 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
 
     # the batched path: entropy chunking tuned on this batch, launches
     # replayed from a captured HIP graph
-    (lanes, chunk), trials = eng.autotune(ptrs, ns, dtype, 3, eb, p, thr)
-    tuned = {"lanes_max": lanes, "chunk_symbols": chunk,
-             "trials": [{"chunk_symbols": c, "ms": round(ms, 3), "bytes": b} for c, ms, b in trials]}
+    best, trials = eng.autotune(ptrs, ns, dtype, 3, eb, p, thr)
+    tuned = dict(best, trials=[dict(t, ms=round(ms, 3), bytes=b) for t, ms, b in trials])
 
     def comp_batched():
         eng.set_graphs(True)
@@ -356,13 +355,9 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
             break
 
     def comp_frames():
-        # the plain path: one call per frame with the default rANS setting
-        eng.set_rans(64, 65536)
-        try:
-            for k in range(len(ns)):
-                eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
-        finally:
-            eng.set_rans(lanes, chunk)
+        # the plain path: one call per frame with the batch's tuned setting
+        for k in range(len(ns)):
+            eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
 
     def dec_batched():
         return eng.decompress_batch_device(streams, dblob)
@@ -391,7 +386,7 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
     assert maxerr <= eb, maxerr
     eng.device_free(dptr)
     eng.device_free(dblob)
-    eng.set_rans(64, 65536)
+    eng.set_tuning()
     r = {
         "frames": len(ns), "points": total, "input_bytes": nbytes, "eb": eb, "dtype": np.dtype(dtype).name,
         "compress_gbps_batched": round(nbytes / t_cb / 1e9, 2),
@@ -404,7 +399,7 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
         "containers_identical_to_per_frame": identical,
         "max_abs_error": maxerr, "steps": steps,
         "autotune": tuned, "graph_captures_replays": list(eng.graph_stats()),
-        "per_frame_path": "one call per frame, default rANS setting (64 lanes, 65536 symbols per chunk), no graph; "
+        "per_frame_path": "one call per frame with the batch's tuned setting, no graph; "
                           "per-frame decompression reads the batched path's containers",
     }
     return r

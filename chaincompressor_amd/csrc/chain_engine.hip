@@ -241,10 +241,15 @@ struct GraphCache {
     }
 };
 
-// RVC1 parameters of the containers an engine writes (recorded in every section)
-struct RansConfig {
+// Tunable settings of an engine: the RVC1 parameters of the containers it
+// writes (recorded in every section, so any decoder reads any setting) and
+// the L1 strategy (successors inside 4096-point tiles plus a sort of one
+// item per (tile, key), or one sort of every point). The output does not
+// depend on any of them except the RVC1 chunking, which the container records.
+struct Tuning {
     uint32_t lanes_max = 64;
     uint32_t chunk_symbols = 65536;
+    uint32_t l1_tiles = 1;
 };
 
 }  // namespace chain
@@ -263,11 +268,11 @@ struct chain_engine {
     std::string err;
     chain::StageTimer timer;
     chain::Staging stage;
-    chain::RansConfig rans;
+    chain::Tuning tuning;
     bool use_graphs = false;
     chain::GraphCache graph;
     bool autotune = false;
-    std::map<uint32_t, chain::RansConfig> tuned;   // size bucket -> setting
+    std::map<uint32_t, chain::Tuning> tuned;   // size bucket -> setting
     void* scratch = nullptr;   // this device's no-allocation scratch (device address)
 };
 
@@ -481,7 +486,7 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                           FrameOut* fo, const uint32_t* idrel, const uint64_t* diff, const uint64_t* keep,
                           const uint8_t* choice2, const void* rel, const uint64_t* corr_idx, const double* corr_val,
                           const uint32_t* tb, const uint32_t* tb_host, uint32_t ntiles,
-                          uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const RansConfig& rc, uint64_t stride,
+                          uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const Tuning& rc, uint64_t stride,
                           uint32_t* err, X& x) {
     const uint32_t ninst = 5 * F, nsec = 10 * F;
     hipLaunchKernelGGL(k_comp_init, dim3(ninst), dim3(CH_WG), 0, st, fdev, fo, F, D, x.ci, x.vhist, x.thist);
@@ -791,7 +796,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     const size_t size_b = std::max(cb0.off, cb1.off);
 
     // LCH1 entropy stage buffers, sized from the particle counts alone
-    const RansConfig rc = e->rans;
+    const Tuning rc = e->tuning;
     // per-chunk scratch: states, then at most 3 bytes per symbol, rounded so
     // that the encoder's aligned block stores stay aligned
     const uint64_t stride = CHUNK_STATE_BYTES + align256(3ull * rc.chunk_symbols);
@@ -979,7 +984,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     // kernel, and across tiles by sorting one item per (tile, key). Wider
     // keys: every point sorted; the first radix pass (low byte of key word 0)
     // takes its histogram from quantise.
-    const bool item_link = nwords == 1;
+    const bool item_link = nwords == 1 && e->tuning.l1_tiles != 0;
     const bool hist_q = !item_link && word_passes[0] >= 1;
 
     Carver cB;
@@ -1155,6 +1160,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         GraphKey key;
         key.add(format).add(dtype).add(layout).add(D).add(F).add(T).add(nwords).add(key64).add(rel8);
         key.add(dpts).add(e->arena.base).add(ntiles).add(ntiles1k).add(rc.lanes_max).add(rc.chunk_symbols);
+        key.add(item_link);
         for (uint32_t w = 0; w < CH_MAXD; ++w) key.add(word_passes[w]);
         for (int c = 0; c < 5; ++c) key.add(comp_chunk_bound[c]);
         for (uint32_t f = 0; f <= F; ++f) key.add(pt_off[f]);
@@ -1950,83 +1956,91 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
 // autotuning of the entropy stage's chunking
 
 struct TuneTrial {
-    uint32_t lanes_max, chunk_symbols;
+    Tuning t;
     double compress_ms, decompress_ms;
     uint64_t bytes;
 };
 
-// Time compression + decompression of (a sample of) the batch for every
-// candidate chunk size and keep the fastest whose output is within 2 % of
-// the smallest. Frames are taken in order up to 2^24 points (at least one).
-// The containers are valid under every setting, so this only trades speed
-// against the few bytes of lane states per chunk.
-static RansConfig autotune_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
-                                uint32_t D, bool on_device, std::vector<TuneTrial>& trials) {
+// Time compression + decompression of (a sample of) the batch under
+// candidate settings and keep the fastest whose output is within 2 % of the
+// smallest. Frames are taken in order up to 2^24 points (at least one). The
+// search is coordinate-wise: 8192 / 16384 / 32768 / 65536 symbols per chunk
+// with 64 lanes and tile-local L1; then 32 lanes, and the global-sort L1, at
+// the best chunk size so far. Containers are valid under every setting.
+static Tuning autotune_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
+                            uint32_t D, bool on_device, std::vector<TuneTrial>& trials) {
     uint32_t Fs = 0;
     uint64_t pts = 0;
     while (Fs < F && (Fs == 0 || pts + frames[Fs].n <= (1ull << 24))) pts += frames[Fs++].n;
-    const RansConfig saved = e->rans;
+    const Tuning saved = e->tuning;
     const bool timer_was = e->timer.on;
     const bool graphs_was = e->use_graphs;
     e->timer.on = false;
     e->use_graphs = false;
-    const uint32_t cands[4] = {8192, 16384, 32768, 65536};
-    try {
-        for (uint32_t cs : cands) {
-            e->rans = RansConfig{64, cs};
-            TuneTrial tr{64, cs, 0, 0, 0};
-            for (int rep = 0; rep < 2; ++rep) {   // the first run sizes the arenas
-                CompressResult cr;
-                cr.keep_on_device = true;
-                auto t0 = std::chrono::steady_clock::now();
-                compress_batch_impl(e, frames, Fs, dtype, layout, D, on_device, CHAIN_FMT_LCH1, cr);
-                auto t1 = std::chrono::steady_clock::now();
-                std::vector<std::vector<uint8_t>> host(Fs);
-                std::vector<const uint8_t*> ptrs(Fs);
-                std::vector<size_t> lens(Fs);
-                bool contiguous = true;
-                uint64_t total = 0;
-                for (uint32_t f = 0; f < Fs; ++f) {
-                    host[f].resize(cr.sizes[f]);
-                    if (cr.sizes[f])
-                        HIPCHK(hipMemcpyAsync(host[f].data(), cr.d_out + cr.offsets[f], cr.sizes[f],
-                                              hipMemcpyDeviceToHost, e->stream));
-                    contiguous = contiguous && cr.offsets[f] == cr.offsets[0] + total;
-                    total += cr.sizes[f];
-                }
-                HIPCHK(hipStreamSynchronize(e->stream));
-                for (uint32_t f = 0; f < Fs; ++f) {
-                    ptrs[f] = host[f].data();
-                    lens[f] = host[f].size();
-                }
-                DecodeResult dr;
-                dr.keep_on_device = true;
-                dr.d_pool = contiguous ? cr.d_out + cr.offsets[0] : nullptr;
-                auto t2 = std::chrono::steady_clock::now();
-                decompress_batch_impl(e, ptrs.data(), lens.data(), Fs, CHAIN_F64, CHAIN_AOS, dr);
-                auto t3 = std::chrono::steady_clock::now();
-                tr.compress_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-                tr.decompress_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
-                tr.bytes = total;
+    auto run = [&](const Tuning& t) {
+        e->tuning = t;
+        TuneTrial tr{t, 0, 0, 0};
+        for (int rep = 0; rep < 2; ++rep) {   // the first run sizes the arenas
+            CompressResult cr;
+            cr.keep_on_device = true;
+            auto t0 = std::chrono::steady_clock::now();
+            compress_batch_impl(e, frames, Fs, dtype, layout, D, on_device, CHAIN_FMT_LCH1, cr);
+            auto t1 = std::chrono::steady_clock::now();
+            std::vector<std::vector<uint8_t>> host(Fs);
+            std::vector<const uint8_t*> ptrs(Fs);
+            std::vector<size_t> lens(Fs);
+            bool contiguous = true;
+            uint64_t total = 0;
+            for (uint32_t f = 0; f < Fs; ++f) {
+                host[f].resize(cr.sizes[f]);
+                if (cr.sizes[f])
+                    HIPCHK(hipMemcpyAsync(host[f].data(), cr.d_out + cr.offsets[f], cr.sizes[f],
+                                          hipMemcpyDeviceToHost, e->stream));
+                contiguous = contiguous && cr.offsets[f] == cr.offsets[0] + total;
+                total += cr.sizes[f];
             }
-            trials.push_back(tr);
+            HIPCHK(hipStreamSynchronize(e->stream));
+            for (uint32_t f = 0; f < Fs; ++f) {
+                ptrs[f] = host[f].data();
+                lens[f] = host[f].size();
+            }
+            DecodeResult dr;
+            dr.keep_on_device = true;
+            dr.d_pool = contiguous ? cr.d_out + cr.offsets[0] : nullptr;
+            auto t2 = std::chrono::steady_clock::now();
+            decompress_batch_impl(e, ptrs.data(), lens.data(), Fs, CHAIN_F64, CHAIN_AOS, dr);
+            auto t3 = std::chrono::steady_clock::now();
+            tr.compress_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            tr.decompress_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+            tr.bytes = total;
         }
+        trials.push_back(tr);
+    };
+    auto best_of = [&]() -> const TuneTrial& {
+        uint64_t best_bytes = ~0ull;
+        for (const TuneTrial& t : trials) best_bytes = std::min(best_bytes, t.bytes);
+        const TuneTrial* best = nullptr;
+        for (const TuneTrial& t : trials)
+            if (t.bytes <= best_bytes + best_bytes / 50 &&
+                (!best || t.compress_ms + t.decompress_ms < best->compress_ms + best->decompress_ms))
+                best = &t;
+        return *best;
+    };
+    try {
+        for (uint32_t cs : {8192u, 16384u, 32768u, 65536u}) run(Tuning{64, cs, 1});
+        Tuning b = best_of().t;
+        run(Tuning{32, b.chunk_symbols, b.l1_tiles});
+        b = best_of().t;
+        run(Tuning{b.lanes_max, b.chunk_symbols, b.l1_tiles ? 0u : 1u});
     } catch (...) {
-        e->rans = saved;
+        e->tuning = saved;
         e->timer.on = timer_was;
         e->use_graphs = graphs_was;
         throw;
     }
     e->timer.on = timer_was;
     e->use_graphs = graphs_was;
-    uint64_t best_bytes = ~0ull;
-    for (const TuneTrial& t : trials) best_bytes = std::min(best_bytes, t.bytes);
-    const TuneTrial* best = nullptr;
-    for (const TuneTrial& t : trials)
-        if (t.bytes <= best_bytes + best_bytes / 50 &&
-            (!best || t.compress_ms + t.decompress_ms < best->compress_ms + best->decompress_ms))
-            best = &t;
-    return RansConfig{best->lanes_max, best->chunk_symbols};
+    return best_of().t;
 }
 
 // automatic mode: one tuned setting per (log2 mean frame size, log2 frames)
@@ -2046,7 +2060,7 @@ static void apply_autotune(chain_engine* e, const chain_frame* frames, uint32_t 
         std::vector<TuneTrial> trials;
         it = e->tuned.emplace(key, autotune_impl(e, frames, F, dtype, layout, D, on_device, trials)).first;
     }
-    e->rans = it->second;
+    e->tuning = it->second;
 }
 
 }  // namespace chain
@@ -2231,14 +2245,33 @@ int chain_compress_batch_device(chain_engine* e, const chain_frame* frames, uint
     });
 }
 
+static void check_tuning(uint32_t lanes_max, uint32_t chunk_symbols) {
+    if (lanes_max < 1 || lanes_max > 64 || (lanes_max & (lanes_max - 1)))
+        throw Error(CHAIN_EINVAL, "lanes_max must be a power of two in [1, 64]");
+    if (chunk_symbols < 64 || chunk_symbols > (1u << 20))
+        throw Error(CHAIN_EINVAL, "chunk_symbols must be in [64, 2^20]");
+}
+
 int chain_engine_set_rans(chain_engine* e, uint32_t lanes_max, uint32_t chunk_symbols) {
     return guarded(e, [&] {
-        if (lanes_max < 1 || lanes_max > 64 || (lanes_max & (lanes_max - 1)))
-            throw Error(CHAIN_EINVAL, "lanes_max must be a power of two in [1, 64]");
-        if (chunk_symbols < 64 || chunk_symbols > (1u << 20))
-            throw Error(CHAIN_EINVAL, "chunk_symbols must be in [64, 2^20]");
-        e->rans.lanes_max = lanes_max;
-        e->rans.chunk_symbols = chunk_symbols;
+        check_tuning(lanes_max, chunk_symbols);
+        e->tuning.lanes_max = lanes_max;
+        e->tuning.chunk_symbols = chunk_symbols;
+    });
+}
+
+int chain_engine_set_tuning(chain_engine* e, const chain_tuning* t) {
+    return guarded(e, [&] {
+        if (!t) throw Error(CHAIN_EINVAL, "NULL argument");
+        check_tuning(t->lanes_max, t->chunk_symbols);
+        e->tuning = chain::Tuning{t->lanes_max, t->chunk_symbols, t->l1_tiles ? 1u : 0u};
+    });
+}
+
+int chain_engine_get_tuning(chain_engine* e, chain_tuning* t) {
+    return guarded(e, [&] {
+        if (!t) throw Error(CHAIN_EINVAL, "NULL argument");
+        *t = chain_tuning{e->tuning.lanes_max, e->tuning.chunk_symbols, e->tuning.l1_tiles, 0};
     });
 }
 
@@ -2411,21 +2444,21 @@ int chain_engine_set_autotune(chain_engine* e, int on) {
 }
 
 int chain_engine_autotune(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
-                          uint32_t dims, int points_on_device, uint32_t* lanes_max, uint32_t* chunk_symbols,
-                          float* trial_ms, uint64_t* trial_bytes) {
+                          uint32_t dims, int points_on_device, chain_tuning* best, chain_tuning* trials,
+                          float* trial_ms, uint64_t* trial_bytes, uint32_t max_trials, uint32_t* ntrials) {
     return guarded(e, [&] {
         if (!frames || nframes == 0) throw Error(CHAIN_EINVAL, "empty batch");
-        std::vector<chain::TuneTrial> trials;
-        const chain::RansConfig best =
-            chain::autotune_impl(e, frames, nframes, dtype, layout, dims, points_on_device != 0, trials);
-        e->rans = best;
-        e->tuned[chain::tune_bucket(frames, nframes)] = best;
-        if (lanes_max) *lanes_max = best.lanes_max;
-        if (chunk_symbols) *chunk_symbols = best.chunk_symbols;
-        for (size_t k = 0; k < trials.size(); ++k) {
-            if (trial_ms) trial_ms[k] = (float)(trials[k].compress_ms + trials[k].decompress_ms);
-            if (trial_bytes) trial_bytes[k] = trials[k].bytes;
+        std::vector<chain::TuneTrial> tr;
+        const chain::Tuning b = chain::autotune_impl(e, frames, nframes, dtype, layout, dims, points_on_device != 0, tr);
+        e->tuning = b;
+        e->tuned[chain::tune_bucket(frames, nframes)] = b;
+        if (best) *best = chain_tuning{b.lanes_max, b.chunk_symbols, b.l1_tiles, 0};
+        for (size_t k = 0; k < tr.size() && k < max_trials; ++k) {
+            if (trials) trials[k] = chain_tuning{tr[k].t.lanes_max, tr[k].t.chunk_symbols, tr[k].t.l1_tiles, 0};
+            if (trial_ms) trial_ms[k] = (float)(tr[k].compress_ms + tr[k].decompress_ms);
+            if (trial_bytes) trial_bytes[k] = tr[k].bytes;
         }
+        if (ntrials) *ntrials = (uint32_t)std::min<size_t>(tr.size(), max_trials);
     });
 }
 

@@ -50,7 +50,7 @@ struct ServeDone {
 
 // Upper bounds of the arena bytes of one frame (chain_engine.hip: carve_a/b/c
 // of compress_batch_impl, carve of decompress_batch_impl).
-inline uint64_t serve_estimate_compress(uint64_t n, uint32_t D, uint32_t es, const RansConfig& rc) {
+inline uint64_t serve_estimate_compress(uint64_t n, uint32_t D, uint32_t es, const Tuning& rc) {
     const uint64_t per_point = (uint64_t)D * es + 76ull * D + 30 + 3ull * (3 * D + 7) + 16 + 40ull * D;
     const uint64_t stride = CHUNK_STATE_BYTES + 3ull * rc.chunk_symbols;
     const uint64_t per_frame = 10 * stride + 5ull * RANS_VALUE_LIMIT * 4 + 10ull * RANS_VALUE_LIMIT * 8 +
@@ -302,7 +302,7 @@ int chain_serve_submit(chain_server* s, const chain_request* r, uint64_t* ticket
         if (!(r->eb > 0.0) || !std::isfinite(2.0 * r->eb)) return fail(CHAIN_EINVAL, "eb must be positive with finite 2*eb");
         if (r->p <= 0) return fail(CHAIN_EINVAL, "p must be positive");
         it.dims = r->dims;
-        it.est = chain::serve_estimate_compress(r->n, r->dims, r->dtype == CHAIN_F32 ? 4 : 8, s->engines[0]->rans);
+        it.est = chain::serve_estimate_compress(r->n, r->dims, r->dtype == CHAIN_F32 ? 4 : 8, s->engines[0]->tuning);
     } else if (r->kind == CHAIN_REQ_DECOMPRESS) {
         if (!r->bytes || r->len < 24) return fail(CHAIN_EINVAL, "decompress request without a stream");
         if (r->out_dtype != CHAIN_F32 && r->out_dtype != CHAIN_F64) return fail(CHAIN_EINVAL, "dtype must be float32 or float64");

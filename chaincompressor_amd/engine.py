@@ -55,11 +55,21 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_last_stage_bytes", "chain_engine_set_timing", "chain_server_create", "chain_server_destroy",
            "chain_server_last_error", "chain_serve_submit", "chain_serve_poll", "chain_server_clock",
            "chain_engine_set_graphs", "chain_engine_graph_stats", "chain_engine_autotune",
-           "chain_engine_set_autotune", "chain_compress_search", "chain_server_stats",
+           "chain_engine_set_autotune", "chain_compress_search", "chain_server_stats", "chain_engine_set_tuning",
+           "chain_engine_get_tuning",
            "chain_decompress_batch_noalloc"]
 
 CHAIN_PENDING = 5
 REQ_COMPRESS, REQ_DECOMPRESS = 0, 1
+
+class _Tuning(C.Structure):
+    _fields_ = [("lanes_max", C.c_uint32), ("chunk_symbols", C.c_uint32), ("l1_tiles", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {"lanes_max": int(self.lanes_max), "chunk_symbols": int(self.chunk_symbols),
+                "l1_tiles": int(self.l1_tiles)}
+
 
 class _Request(C.Structure):
     _fields_ = [("kind", C.c_int), ("points", C.c_void_p), ("dtype", C.c_int), ("layout", C.c_int),
@@ -122,7 +132,10 @@ def load_library():
                                         P(C.c_int64), C.c_uint32, P(C.c_int64), C.c_uint32, C.c_uint32, C.c_int,
                                         P(_Stats)]
     L.chain_engine_autotune.argtypes = [C.c_void_p, P(_Frame), C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.c_int,
-                                        P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_uint64)]
+                                        P(_Tuning), P(_Tuning), P(C.c_float), P(C.c_uint64), C.c_uint32,
+                                        P(C.c_uint32)]
+    L.chain_engine_set_tuning.argtypes = [C.c_void_p, P(_Tuning)]
+    L.chain_engine_get_tuning.argtypes = [C.c_void_p, P(_Tuning)]
     L.chain_engine_graph_stats.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]
     L.chain_server_create.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_void_p)]
     L.chain_server_destroy.argtypes = [C.c_void_p]
@@ -245,19 +258,33 @@ class Engine:
 
     def autotune(self, ptrs, ns, dtype, dims, eb, p, threshold, on_device=True, layout="aos"):
         """Tune on a batch (device pointers, or host arrays with on_device=False);
-        returns ((lanes_max, chunk_symbols), [(chunk_symbols, ms, bytes)])."""
+        returns (best tuning dict, [(tuning dict, ms, bytes)] of every trial)."""
         F = len(ptrs)
         fr = (_Frame * F)()
         for k in range(F):
             fr[k] = _Frame(ptrs[k], int(ns[k]), float(eb), int(p), int(threshold), 0)
-        lm, cs = C.c_uint32(), C.c_uint32()
-        ms = (C.c_float * 4)()
-        by = (C.c_uint64 * 4)()
+        best = _Tuning()
+        cap = 16
+        tr = (_Tuning * cap)()
+        ms = (C.c_float * cap)()
+        by = (C.c_uint64 * cap)()
+        nt = C.c_uint32()
         self._check(self._L.chain_engine_autotune(self._h, fr, F, _dtype_code(dtype),
                                                   CHAIN_AOS if layout == "aos" else CHAIN_SOA, dims,
-                                                  1 if on_device else 0, C.byref(lm), C.byref(cs), ms, by))
-        trials = [(c, float(ms[k]), int(by[k])) for k, c in enumerate((8192, 16384, 32768, 65536))]
-        return (int(lm.value), int(cs.value)), trials
+                                                  1 if on_device else 0, C.byref(best), tr, ms, by, cap,
+                                                  C.byref(nt)))
+        trials = [(tr[k].as_dict(), float(ms[k]), int(by[k])) for k in range(nt.value)]
+        return best.as_dict(), trials
+
+    def set_tuning(self, lanes_max=64, chunk_symbols=65536, l1_tiles=1):
+        """RVC1 layout and L1 strategy of the compressions that follow."""
+        t = _Tuning(lanes_max, chunk_symbols, 1 if l1_tiles else 0, 0)
+        self._check(self._L.chain_engine_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _Tuning()
+        self._check(self._L.chain_engine_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
 
     def set_graphs(self, on: bool):
         """Capture compression launches in a HIP graph and replay it for

@@ -160,17 +160,34 @@ int chain_device_malloc(chain_engine* e, uint64_t bytes, void** out);
 int chain_device_free(chain_engine* e, void* p);
 int chain_copy(chain_engine* e, void* dst, const void* src, uint64_t bytes, int to_device);
 
-/* Autotuning of the entropy stage: compress + decompress (a sample of up to
- * 2^24 points of) the given batch with 64 lanes and 8192, 16384, 32768 and
- * 65536 symbols per chunk, keep the fastest setting whose output is within
- * 2 % of the smallest, apply it to the engine and remember it for the
+/* Tunable settings: the RVC1 layout of the containers an engine writes
+ * (largest lane count per chunk, symbols per chunk; both recorded in every
+ * section, so any decoder reads any setting) and the L1 strategy (l1_tiles=1:
+ * successors inside 4096-point tiles plus a sort of one item per (tile,
+ * key); 0: one radix sort of every point). Only the RVC1 layout shows in the
+ * output; the integer streams are the same under every setting. */
+typedef struct {
+    uint32_t lanes_max;
+    uint32_t chunk_symbols;
+    uint32_t l1_tiles;
+    uint32_t reserved;
+} chain_tuning;
+int chain_engine_set_tuning(chain_engine* e, const chain_tuning* t);
+int chain_engine_get_tuning(chain_engine* e, chain_tuning* t);
+
+/* Autotuning: compress + decompress (a sample of up to 2^24 points of) the
+ * given batch under candidate settings, keep the fastest whose output is
+ * within 2 % of the smallest, apply it to the engine and remember it for the
  * batch's size bucket (log2 of the mean frame size and of the frame count).
- * trial_ms / trial_bytes (4 entries each, may be NULL) report every trial.
- * With chain_engine_set_autotune(e, 1), every compression whose bucket has
- * no setting yet runs this first. Containers stay valid under any setting. */
+ * The search is coordinate-wise: 8192, 16384, 32768 and 65536 symbols per
+ * chunk (64 lanes, tile-local L1), then 32 lanes and then the global-sort L1
+ * at the best setting so far. trials / trial_ms / trial_bytes (up to
+ * max_trials entries each, may be NULL) report every trial, *ntrials their
+ * count. With chain_engine_set_autotune(e, 1), every compression whose bucket
+ * has no setting yet runs this first. Containers stay valid under any setting. */
 int chain_engine_autotune(chain_engine* e, const chain_frame* frames, uint32_t nframes, int dtype, int layout,
-                          uint32_t dims, int points_on_device, uint32_t* lanes_max, uint32_t* chunk_symbols,
-                          float* trial_ms, uint64_t* trial_bytes);
+                          uint32_t dims, int points_on_device, chain_tuning* best, chain_tuning* trials,
+                          float* trial_ms, uint64_t* trial_bytes, uint32_t max_trials, uint32_t* ntrials);
 int chain_engine_set_autotune(chain_engine* e, int on);
 
 /* Compression through a HIP graph: every launch after the planning

@@ -1,6 +1,7 @@
-"""The entropy-stage autotuner: it picks one of its candidate chunk sizes,
-the containers written with the tuned setting equal the oracle's LCH1
-writer with the same setting, and they decode exactly."""
+"""The autotuner: it searches chunk sizes, lane counts and the L1 strategy,
+picks one of its trials, the containers written with the tuned setting equal
+the oracle's LCH1 writer with the same RVC1 setting, and they decode
+exactly. The L1 strategy changes no byte of the output."""
 import numpy as np
 import pytest
 
@@ -19,16 +20,25 @@ def test_autotune_picks_a_candidate_and_stays_exact():
     ns = [f.shape[0] for f in frames]
     offs = np.concatenate([[0], np.cumsum(ns)[:-1]])
     ptrs = [ptr + int(o) * 12 for o in offs]
-    (lanes, chunk), trials = eng.autotune(ptrs, ns, np.float32, 3, 1e-3, 16, 2)
-    assert lanes == 64 and chunk in (8192, 16384, 32768, 65536)
+    best, trials = eng.autotune(ptrs, ns, np.float32, 3, 1e-3, 16, 2)
+    lanes, chunk = best["lanes_max"], best["chunk_symbols"]
+    assert best in [t for t, _, _ in trials]
+    assert {t["chunk_symbols"] for t, _, _ in trials} == {8192, 16384, 32768, 65536}
+    assert {t["lanes_max"] for t, _, _ in trials} == {32, 64} and {t["l1_tiles"] for t, _, _ in trials} == {0, 1}
     assert all(ms > 0 and b > 0 for _, ms, b in trials)
-    best = min(b for _, _, b in trials)
-    assert dict((c, b) for c, _, b in trials)[chunk] <= best * 1.02
+    smallest = min(b for _, _, b in trials)
+    assert [b for t, _, b in trials if t == best][0] <= smallest * 1.02
     outs, _ = eng.compress_batch(frames, 1e-3, 16, 2)
     for f, b in zip(frames, outs):
         ref, _ = O.compress_streams(f, 1e-3, 16, 2).lch1(lanes_max=lanes, chunk_symbols=chunk)
         assert b == ref
         assert np.array_equal(eng.decompress(b).view(np.uint64), O.decode(b).view(np.uint64))
+    # every knob setting writes the oracle's bytes for its RVC1 layout
+    for t, _, _ in trials:
+        eng.set_tuning(**t)
+        outs, _ = eng.compress_batch(frames[:2], 1e-3, 16, 2)
+        for f, b in zip(frames[:2], outs):
+            assert b == O.compress_streams(f, 1e-3, 16, 2).lch1(t["lanes_max"], t["chunk_symbols"])[0], t
     eng.device_free(ptr)
 
 

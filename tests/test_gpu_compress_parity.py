@@ -1,7 +1,9 @@
 """GPU compressor parity: the engine's pre-entropy container (LCP3, the
 reference's ec=False output of cpp/chain/lcps_cpp.cpp:2306-2329) must be
 byte-identical to the golden vectors of the compiled reference and to the
-oracle, called through the C ABI (libchain_engine.so)."""
+oracle, called through the C ABI (libchain_engine.so). Every test runs under
+both L1 strategies (successors inside 4096-point tiles plus an item sort, and
+one radix sort of every point)."""
 import hashlib
 
 import numpy as np
@@ -16,10 +18,12 @@ def sha(b):
     return hashlib.sha256(bytes(b)).hexdigest()
 
 
-@pytest.fixture(scope="module")
-def eng():
+@pytest.fixture(scope="module", params=[1, 0], ids=["l1_tiles", "l1_global"])
+def eng(request):
     from chaincompressor_amd import engine
-    return engine.Engine(0)
+    e = engine.Engine(0)
+    e.set_tuning(64, 65536, request.param)
+    return e
 
 
 def flags_of(bl, l1, l2):

@@ -533,6 +533,8 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
     tm.begin("rans_table", 5ull * F * (RANS_VALUE_LIMIT + RANS_TAG_LIMIT) * 4 + 10ull * F * RANS_VALUE_LIMIT * 8);
     hipLaunchKernelGGL(k_rans_table, dim3(nsec), dim3(CH_WG), 0, st, x.ci, x.sec, x.vhist, x.thist, F, x.pc,
                        x.tbytes, err);
+    hipLaunchKernelGGL(k_prechoose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst, rc.lanes_max,
+                       rc.chunk_symbols);
     hipLaunchKernelGGL(k_rans_plan, dim3(1), dim3(1), 0, st, x.sec, nsec, F, rc.lanes_max, rc.chunk_symbols,
                        x.comp_chunk);
     tm.end();

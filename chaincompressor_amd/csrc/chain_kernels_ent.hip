@@ -1,7 +1,9 @@
 // Entropy stage and LCH1 container assembly of the Chain compressor (gfx950).
 //
 // For each frame and each of the five components (idrel, diff, keep, choice2,
-// rel) the engine builds three candidate encodings and keeps the smallest,
+// rel) the engine sizes three candidate encodings, estimating the rANS-coded
+// ones from their histograms (chain_int.hpp), encodes only the candidates
+// whose estimate is within 1/256 of the best and keeps the smallest of those,
 // ties in the order Tagged, Fixed, Raw-rANS (the reference's chooser order,
 // cpp/chain/lcps_cpp.cpp:1140-1174, with Zstd replaced):
 //   TAGR  per-record widths rANS-coded + MSB-first point-major values
@@ -31,6 +33,7 @@ struct CompInst {
     uint64_t size;             // component bytes
     uint64_t off;              // absolute output offset of the component magic
     uint64_t payload_off;      // FIXP payload / TAGR packed values
+    uint32_t cand, pad;        // contenders (chain_int.hpp lch1_contenders)
 };
 
 struct Section {               // RVC1 section of one component: s = 0 values, 1 widths
@@ -40,6 +43,8 @@ struct Section {               // RVC1 section of one component: s = 0 values, 1
     uint64_t payload;          // bytes of all chunks (states + streams)
     uint64_t size;             // bytes of the whole section
     uint64_t off;              // absolute output offset of "RVC1"
+    uint64_t est_bits;         // estimated stream bits, in 1/65536 (chain_int.hpp)
+    uint32_t encode, pad;      // encoded only when its candidate contends
 };
 
 #define RVC1_HDR_FIXED 30u     // "RVC1" u8 u64 u64 u8 u32 u32
@@ -535,7 +540,7 @@ __global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict
     Section& S = sec[si];
     if (!eligible || count == 0) {
         if (threadIdx.x == 0) {
-            S.count = count; S.eligible = eligible; S.pb = 0; S.alphabet = 0; S.table_len = 0;
+            S.count = count; S.eligible = eligible; S.pb = 0; S.alphabet = 0; S.table_len = 0; S.est_bits = 0;
         }
         return;
     }
@@ -625,6 +630,17 @@ __global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict
             run += s_prob[b0 + k];
         }
         if (threadIdx.x == 0 && tot != M) atomicOr(err, 4u);
+        // estimated stream bits: sum of count * (pb - log2 f) in 1/65536
+        uint64_t eb = 0;
+        for (uint32_t k = 0; k < 16 && b0 + k < A; ++k) {
+            const uint32_t h = hist[b0 + k];
+            if (h) eb += (uint64_t)h * rvc1_symbol_cost_q16(s_prob[b0 + k], pb);
+        }
+        for (int k = 32; k >= 1; k >>= 1) eb += __shfl_xor(eb, k, 64);
+        __syncthreads();
+        if (lane_id() == 0) s_red[threadIdx.x >> 6] = eb;
+        __syncthreads();
+        if (threadIdx.x == 0) S.est_bits = s_red[0] + s_red[1] + s_red[2] + s_red[3];
     }
     // compact table bytes, cpp/chain/lcps_cpp.cpp:686-729: varuint(A), then
     // per symbol (pr << 2) | e and e more bytes (e = 0, 1, 2 for pr < 2^6,
@@ -719,7 +735,7 @@ __global__ void k_rans_plan(Section* sec, uint32_t nsec, uint32_t F, uint32_t la
         if (si % (2 * F) == 0) comp_chunk[si / (2 * F)] = base;
         Section& S = sec[si];
         S.chunk_symbols = chunk_symbols;
-        const bool used = S.eligible && S.count > 0;
+        const bool used = S.eligible && S.count > 0 && S.encode;
         S.lanes = used ? rvc1_lanes_for(S.count, lanes_max) : 0;
         S.nchunks = used ? (uint32_t)((S.count + chunk_symbols - 1) / chunk_symbols) : 0;
         S.chunk_base = base;
@@ -1013,13 +1029,16 @@ __global__ __launch_bounds__(CH_WG) void k_section_sizes(Section* __restrict__ s
     }
 }
 
-__global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, uint32_t ninst) {
+// Estimates of every candidate and the contenders (chain_int.hpp): Fixed is
+// sized exactly here, Tagged and Raw-rANS from their sections' estimates;
+// only the contenders' sections are encoded. One thread per instance.
+__global__ void k_prechoose(CompInst* __restrict__ ci, Section* __restrict__ sec, uint32_t ninst, uint32_t lanes_max,
+                            uint32_t chunk_symbols) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst) return;
     CompInst& C = ci[i];
     Section& sv = sec[2 * i];
     Section& st = sec[2 * i + 1];
-    const uint64_t tag_size = 4 + 4 + 8 + 1 + 8 + st.size + 8 + (C.tag_bits + 7) / 8;
     uint64_t fix_size = 4 + 4 + 8 + (uint64_t)C.dims * 17;
     for (uint32_t d = 0; d < C.dims; ++d) {
         uint32_t w = 1;
@@ -1029,12 +1048,42 @@ __global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, u
         fix_size += C.fix_bytes[d];
         if (C.n == 0) C.dmin[d] = 0;
     }
-    uint32_t kind = KIND_TAGR;
-    uint64_t best = tag_size;
-    if (fix_size < best) { best = fix_size; kind = KIND_FIXP; }
-    if (sv.eligible) {
+    const uint64_t tag_est = 4 + 4 + 8 + 1 + 8 +
+                             rvc1_section_estimate(st.count, st.table_len, rvc1_lanes_for(st.count, lanes_max),
+                                                   chunk_symbols, st.est_bits) +
+                             8 + (C.tag_bits + 7) / 8;
+    const uint64_t rans_est = sv.eligible ? 4 + 4 + 8 + 8 +
+                                                rvc1_section_estimate(sv.count, sv.table_len,
+                                                                      rvc1_lanes_for(sv.count, lanes_max),
+                                                                      chunk_symbols, sv.est_bits)
+                                          : ~0ull;
+    const uint32_t mask = lch1_contenders(tag_est, fix_size, rans_est);
+    C.cand = mask;
+    st.encode = (mask & 1u) ? 1u : 0u;
+    sv.encode = (mask & 4u) ? 1u : 0u;
+}
+
+// the exact smallest contender; ties Tagged > Fixed > Raw-rANS
+__global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, uint32_t ninst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
+    CompInst& C = ci[i];
+    const Section& sv = sec[2 * i];
+    const Section& st = sec[2 * i + 1];
+    uint32_t kind = ~0u;
+    uint64_t best = 0;
+    if (C.cand & 1u) {
+        best = 4 + 4 + 8 + 1 + 8 + st.size + 8 + (C.tag_bits + 7) / 8;
+        kind = KIND_TAGR;
+    }
+    if (C.cand & 2u) {
+        uint64_t fix_size = 4 + 4 + 8 + (uint64_t)C.dims * 17;
+        for (uint32_t d = 0; d < C.dims; ++d) fix_size += C.fix_bytes[d];
+        if (kind == ~0u || fix_size < best) { best = fix_size; kind = KIND_FIXP; }
+    }
+    if (C.cand & 4u) {
         const uint64_t rans_size = 4 + 4 + 8 + 8 + sv.size;
-        if (rans_size < best) { best = rans_size; kind = KIND_RANS; }
+        if (kind == ~0u || rans_size < best) { best = rans_size; kind = KIND_RANS; }
     }
     C.kind = kind;
     C.size = best;

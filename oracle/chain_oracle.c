@@ -929,14 +929,112 @@ static int lch1_rans(octx* c, const mat* m, const oc_rans_params* rp, bb* out) {
     return 1;
 }
 
+/* The LCH1 chooser (this project's container): every candidate is first
+ * estimated in integer arithmetic, a RVC1 section as sum(count * (pb - log2 f))
+ * bits in 1/65536 (log2 from the exponent and a table of log2(1 + k/256) over
+ * the 8 bits below the leading one) plus its exact header, table and chunk
+ * words and the chunks' lane states; Fixed is exact. The candidates within
+ * 1/256 (+ 64 bytes) of the smallest estimate are encoded and the smallest of
+ * those wins, ties Tagged > Fixed > Raw-rANS. The GPU engine applies the same
+ * rule (chain_int.hpp: log2_q16, rvc1_section_estimate, lch1_contenders). */
+static const uint16_t LOG2_FRAC[256] = {
+        0,   369,   736,  1102,  1466,  1829,  2190,  2551,  2909,  3267,  3623,  3978,
+     4331,  4683,  5034,  5384,  5732,  6079,  6425,  6769,  7112,  7454,  7795,  8134,
+     8473,  8810,  9146,  9480,  9814, 10146, 10477, 10807, 11136, 11464, 11791, 12116,
+    12440, 12764, 13086, 13407, 13727, 14046, 14363, 14680, 14996, 15310, 15624, 15937,
+    16248, 16559, 16868, 17177, 17484, 17791, 18096, 18401, 18704, 19007, 19308, 19609,
+    19909, 20207, 20505, 20802, 21098, 21393, 21687, 21980, 22272, 22564, 22854, 23144,
+    23433, 23720, 24007, 24293, 24579, 24863, 25146, 25429, 25711, 25992, 26272, 26551,
+    26830, 27108, 27384, 27660, 27936, 28210, 28484, 28757, 29029, 29300, 29571, 29840,
+    30109, 30378, 30645, 30912, 31178, 31443, 31707, 31971, 32234, 32496, 32758, 33019,
+    33279, 33538, 33797, 34055, 34312, 34569, 34825, 35080, 35334, 35588, 35841, 36094,
+    36346, 36597, 36847, 37097, 37346, 37595, 37842, 38090, 38336, 38582, 38827, 39072,
+    39316, 39559, 39802, 40044, 40286, 40527, 40767, 41006, 41246, 41484, 41722, 41959,
+    42196, 42432, 42667, 42902, 43137, 43370, 43603, 43836, 44068, 44300, 44530, 44761,
+    44990, 45220, 45448, 45676, 45904, 46131, 46357, 46583, 46809, 47034, 47258, 47482,
+    47705, 47928, 48150, 48372, 48593, 48813, 49034, 49253, 49472, 49691, 49909, 50127,
+    50344, 50560, 50776, 50992, 51207, 51422, 51636, 51850, 52063, 52276, 52488, 52700,
+    52911, 53122, 53332, 53542, 53751, 53960, 54169, 54377, 54584, 54791, 54998, 55204,
+    55410, 55615, 55820, 56025, 56229, 56432, 56635, 56838, 57040, 57242, 57443, 57644,
+    57845, 58045, 58245, 58444, 58643, 58841, 59039, 59237, 59434, 59631, 59827, 60023,
+    60219, 60414, 60609, 60803, 60997, 61190, 61384, 61576, 61769, 61961, 62152, 62343,
+    62534, 62725, 62915, 63104, 63294, 63483, 63671, 63859, 64047, 64234, 64421, 64608,
+    64794, 64980, 65166, 65351,
+};
+
+static uint32_t log2_q16(uint32_t f) {
+    const int e = bitlen(f) - 1;
+    const uint32_t k = e >= 8 ? (f >> (e - 8)) & 255u : (f << (8 - e)) & 255u;
+    return ((uint32_t)e << 16) + LOG2_FRAC[k];
+}
+
+/* estimated bytes of the RVC1 section of values[0..n) with the given lanes */
+static uint64_t rvc1_estimate(octx* c, const uint64_t* values, uint64_t n, uint32_t L, uint32_t S) {
+    if (n == 0) return 30;
+    rtable t = rvn1_build_table(c, values, n);
+    bb table = bb_make(c);
+    rvn1_table_bytes(c, &t, &table);
+    uint64_t* hist = (uint64_t*)ax(c, (size_t)t.alphabet * sizeof(uint64_t));
+    memset(hist, 0, (size_t)t.alphabet * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) ++hist[values[i]];
+    uint64_t bits = 0;
+    for (uint32_t s = 0; s < t.alphabet; ++s)
+        if (hist[s]) bits += hist[s] * (uint64_t)(((uint32_t)t.precision_bits << 16) - log2_q16(t.prob[s]));
+    const uint64_t nchunks = (n + S - 1) / S;
+    return 30 + table.n + 4 * nchunks + 4ull * L * nchunks + ((bits + (8ull << 16) - 1) >> 19);
+}
+
+static uint32_t lch1_contenders(uint64_t tagr, uint64_t fixp, uint64_t rans) {
+    uint64_t best = tagr < fixp ? tagr : fixp;
+    best = rans < best ? rans : best;
+    const uint64_t lim = best + (best >> 8) + 64;
+    return (tagr <= lim ? 1u : 0u) | (fixp <= lim ? 2u : 0u) | (rans <= lim ? 4u : 0u);
+}
+
 static uint32_t choose_lch1(octx* c, const mat* m, const oc_rans_params* rp, bb* out) {
+    const uint64_t k = m->n * m->dims;
+    /* Tagged estimate: widths section + the packed bits */
+    uint64_t* tags = (uint64_t*)ax(c, (m->n ? m->n : 1) * sizeof(uint64_t));
+    uint64_t tag_bits = 0;
+    for (uint64_t i = 0; i < m->n; ++i) {
+        int tag = 1;
+        for (uint32_t d = 0; d < m->dims; ++d) {
+            const int b = m->v[i * m->dims + d] < 0 ? 64 : bitlen((uint64_t)m->v[i * m->dims + d]);
+            if (b > tag) tag = b;
+        }
+        tags[i] = (uint64_t)tag;
+        tag_bits += (uint64_t)tag * m->dims;
+    }
+    const uint64_t tagr = 4 + 4 + 8 + 1 + 8 + rvc1_estimate(c, tags, m->n, oc_rvc1_lanes_for(m->n, rp->lanes_max),
+                                                            rp->chunk_symbols) + 8 + (tag_bits + 7) / 8;
+    /* Fixed: exact */
+    uint64_t fixp = 4 + 4 + 8 + 17ull * m->dims;
+    for (uint32_t d = 0; d < m->dims; ++d) {
+        int64_t mn;
+        const int w = fixed_width(m->v, m->n, m->dims, d, &mn);
+        fixp += (m->n * (uint64_t)w + 7) / 8;
+    }
+    /* Raw-rANS estimate, only when every value is < 4096 */
+    int eligible = 1;
+    for (uint64_t i = 0; i < k; ++i) if (m->v[i] < 0 || (uint64_t)m->v[i] >= RANS_VALUE_LIMIT) { eligible = 0; break; }
+    uint64_t rans = UINT64_MAX;
+    if (eligible)
+        rans = 4 + 4 + 8 + 8 + rvc1_estimate(c, to_u64(c, m), k, oc_rvc1_lanes_for(k, rp->lanes_max), rp->chunk_symbols);
+    const uint32_t mask = lch1_contenders(tagr, fixp, rans);
+    /* the contenders, encoded exactly; ties Tagged > Fixed > Raw-rANS */
     bb best = bb_make(c), cand = bb_make(c);
-    uint32_t kind = K_TAGR;
-    lch1_tagr(c, m, rp, &best);
-    lch1_fixp(c, m, &cand);
-    if (cand.n < best.n) { bb t = best; best = cand; cand = t; kind = K_FIXP; }
-    cand.n = 0;
-    if (lch1_rans(c, m, rp, &cand) && cand.n < best.n) { best = cand; kind = K_RANS; }
+    uint32_t kind = UINT32_MAX;
+    if (mask & 1u) { lch1_tagr(c, m, rp, &best); kind = K_TAGR; }
+    if (mask & 2u) {
+        lch1_fixp(c, m, &cand);
+        if (kind == UINT32_MAX || cand.n < best.n) { bb t = best; best = cand; cand = t; kind = K_FIXP; }
+        cand.n = 0;
+    }
+    if (mask & 4u) {
+        cand.n = 0;
+        (void)lch1_rans(c, m, rp, &cand);
+        if (kind == UINT32_MAX || cand.n < best.n) { best = cand; kind = K_RANS; }
+    }
     out->n = 0;
     bb_put(out, best.p, best.n);
     return kind;

@@ -70,7 +70,7 @@ REGION_KERNELS = {
     "l2_counts": "chain::k_dec_tile_zero_counts + k_scan_u32",
     "l2_inverse": "chain::k_dec_l2",
     "chain_local": "chain::k_dec_chain_local + k_dec_check_counts",
-    "chain_jump": "chain::k_dec_chain_jump (ceil(log2 tiles)+2 launches)",
+    "chain_jump": "chain::k_dec_chain_jump (4 launches) + k_dec_chain_resolve",
     "dequantize": "chain::k_dec_dequant + k_dec_corr",
 }
 

@@ -1862,14 +1862,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     // write roots) is stated once, on the local pass; the jump rounds over
     // tile seams are extra work of this formulation
     tm.begin("chain_local", 8 * T);
-    uint32_t rounds = 1;
-    {
-        uint32_t maxt = 1;
-        for (uint32_t f = 0; f < F; ++f) maxt = std::max(maxt, ptb[f + 1] - ptb[f]);
-        while ((1u << (rounds - 1)) < maxt) ++rounds;   // ceil(log2(tiles)) + 1
-        ++rounds;                                        // hops through followers
-    }
-    fill32(st, X.jump_open, rounds + 1, 0);
+    fill32(st, X.jump_open, 8, 0);
     // the local pass ranks the explicit points itself: tiles by ticket, each
     // tile's first rank from a look-back over the tiles before it
     fill64(st, X.cstatus, npt, 0);
@@ -1882,13 +1875,21 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     hipLaunchKernelGGL(k_dec_check_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, X.fr, F, X.ptb, X.eoff,
                        &X.totals[1], X.err);
     tm.end();
+    // The exits: a few lock-step rounds of pointer jumping over the open list
+    // (each round halves the remaining hops of every chain and drops the
+    // settled exits), then one launch walks what is left, by then short.
     tm.begin("chain_jump");
-    for (uint32_t r = 0; r < rounds; ++r) {
-        uint32_t* lin = (r & 1) ? X.list_b : X.list_a;
-        uint32_t* lout = (r & 1) ? X.list_a : X.list_b;
-        hipLaunchKernelGGL(k_dec_chain_jump, dim3((unsigned)std::min<uint64_t>(cdiv(T, CH_TILE), 2048)), dim3(CH_WG), 0,
-                           st, X.root,
-                           lin, &X.jump_open[r], lout, &X.jump_open[r + 1]);
+    {
+        const uint32_t sync_rounds = 4;
+        const unsigned g = (unsigned)std::min<uint64_t>(cdiv(T, CH_TILE), 2048);
+        for (uint32_t r = 0; r < sync_rounds; ++r) {
+            uint32_t* lin = (r & 1) ? X.list_b : X.list_a;
+            uint32_t* lout = (r & 1) ? X.list_a : X.list_b;
+            hipLaunchKernelGGL(k_dec_chain_jump, dim3(g), dim3(CH_WG), 0, st, X.root, lin, &X.jump_open[r], lout,
+                               &X.jump_open[r + 1]);
+        }
+        hipLaunchKernelGGL(k_dec_chain_resolve, dim3(g), dim3(CH_WG), 0, st, X.root,
+                           (sync_rounds & 1) ? X.list_b : X.list_a, &X.jump_open[sync_rounds], X.err);
     }
     tm.end();
     tm.begin("dequantize", 4 * T + comp_es[4] * T * D + 8ull * D * P.R + T * D * es);

@@ -900,18 +900,27 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 }
 
 // ------------------------------------------------------------------------
-// Chain resolution: the explicit record of every point. A tile first
-// resolves the links that stay inside it (pointer jumping in LDS). The last
-// in-tile point of a chain segment that continues in a later tile (its exit)
-// keeps CH_EXT | its successor and joins the open list; the other points of
-// the segment keep CH_EXT | their exit. Rounds of global pointer jumping over
-// the open list resolve the exits (every hop goes to a later tile, through
-// at most one follower per tile, so ceil(log2(tiles)) + 2 rounds suffice);
-// dequantisation then reads a follower's value at its exit. No tile ever
-// waits for another.
-
-#define CH_LOCAL 0x80000000u
+// Chain resolution: the explicit record of every point (the backward chain
+// pass of serial.cpp:1277-1302, in parallel). Every successor lies at a larger
+// index, so a point's record is its successor's, down to an explicit point.
+//
+// In a tile of 4096 points, wavefront w owns the 1024 consecutive points
+// [1024 w, 1024 w + 1024) as 16 windows of 64 and walks them from the last to
+// the first: a successor in a later window of its own range is already
+// resolved in LDS (same wavefront, program order), one in the same window is
+// resolved by pointer jumping between lanes with shuffles (no barrier), one
+// in a later wavefront's range is left pending and resolved after one barrier
+// (at most three hops, every hop into a later range). A chain that leaves the
+// tile ends at its exit, the tile's last point on it: the exit keeps
+// CH_EXT | its successor and joins the open list, the points that reach it
+// keep CH_EXT | the exit. k_dec_chain_resolve then settles the exits, and
+// dequantisation reads a follower's value at its exit.
 #define CH_EXT 0x80000000u
+// in-tile states: resolved ordinal (< 4096), exit at local e, pending on local t,
+// in-window pointer to lane l
+#define CS_EXIT 0x10000u
+#define CS_PEND 0x20000u
+#define CS_WIN 0x40000u
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __restrict__ fr, uint32_t F,
                                                            const uint32_t* __restrict__ tile_begin,
@@ -921,10 +930,9 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
                                                            uint32_t* __restrict__ res, uint32_t* __restrict__ open_list,
                                                            uint32_t* __restrict__ open_count, uint32_t* err) {
     constexpr uint32_t W = CH_WG / 64;
+    constexpr uint32_t WIN = CH_TILE / CH_WG;   // windows per wavefront (16)
     __shared__ uint32_t s_val[CH_TILE];
-    __shared__ uint32_t s_ext[CH_TILE];
-    __shared__ uint32_t s_off[CH_ITEMS * W];
-    __shared__ uint32_t s_tile, s_rank0, s_tot;
+    __shared__ uint32_t s_wtot[W], s_tile, s_rank0, s_base;
     // tiles are taken by ticket, so every tile a look-back waits on has started
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
@@ -934,154 +942,142 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
     const uint64_t p0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
     const uint64_t p1 = p0 + CH_TILE < Fr.n ? p0 + CH_TILE : Fr.n;
     const uint32_t cnt = (uint32_t)(p1 - p0);
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
     const uint32_t* src = idrel + Fr.pt_off + p0;
-    // striped: item k of thread t is point k*CH_WG + t of the tile, so rows of
-    // 256 points are read, ranked and written together. Explicit ranks in
-    // index order: ballot prefix inside the wave + scanned (row, wave) counts.
-    uint32_t dv[CH_ITEMS], pre[CH_ITEMS];
-    // every load first (unconditional: a tile holds at least one point), then
-    // the ballots
+    const uint32_t wbase = w * (CH_TILE / W);
+    // every load first (unconditional: a tile holds at least one point)
+    uint32_t dv[WIN];
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint32_t j = k * CH_WG + threadIdx.x;
+    for (int k = 0; k < WIN; ++k) {
+        const uint32_t j = wbase + k * 64 + lane;
         dv[k] = src[j < cnt ? j : 0u];
     }
+    // explicit ordinals in index order: ballots inside the window, running
+    // counts over the wavefront's windows, wavefront offsets
+    uint32_t ord[WIN], run = 0, ebits = 0;
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint32_t j = k * CH_WG + threadIdx.x;
-        dv[k] = j < cnt ? dv[k] : 1u;
-        const uint64_t m = __ballot(j < cnt && dv[k] == 0);
-        pre[k] = __popcll(m & lt);
-        if (lane == 0) s_off[k * W + wave] = __popcll(m);
+    for (int k = 0; k < WIN; ++k) {
+        const uint32_t j = wbase + k * 64 + lane;
+        const bool ex = j < cnt && dv[k] == 0;
+        const uint64_t m = __ballot(ex);
+        ord[k] = run + (uint32_t)__popcll(m & lt);
+        run += (uint32_t)__popcll(m);
+        ebits |= (ex ? 1u : 0u) << k;
     }
+    if (lane == 0) s_wtot[w] = run;
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const uint32_t v = threadIdx.x < CH_ITEMS * W ? s_off[threadIdx.x] : 0u;
-        const uint32_t inc = wave_incl_scan(v);
-        if (threadIdx.x < CH_ITEMS * W) s_off[threadIdx.x] = inc - v;
-        // this tile's explicit count is published now; the ranks of the
-        // earlier tiles are waited for only when the results are written
-        s_tot = __shfl(inc, 63, 64);
-        if (threadIdx.x == 0) lb_store(&status[tile], (tile == 0 ? LB_INC : LB_AGG) | s_tot);
-    }
-    __syncthreads();
-    // s_val: LOCAL | j (pointer inside the tile) or the explicit point's
-    // ordinal inside the tile (its rank is that plus the tile's first rank);
-    // s_ext: global successor index for pointers leaving the tile (else ~0)
+    uint32_t woff = 0, tot = 0;
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint32_t j = k * CH_WG + threadIdx.x;
-        if (j >= cnt) continue;
-        s_ext[j] = 0xffffffffu;
-        if (dv[k] == 0) {
-            s_val[j] = s_off[k * W + wave] + pre[k];
-        } else {
-            const uint64_t t = p0 + j + dv[k];
-            if (t >= Fr.n) {
-                atomicOr(err, DERR_CHAIN);
-                s_val[j] = 0;
-            } else if (t < p1) {
-                s_val[j] = CH_LOCAL | (uint32_t)(t - p0);
+    for (uint32_t q = 0; q < W; ++q) {
+        woff += q < w ? s_wtot[q] : 0u;
+        tot += s_wtot[q];
+    }
+    // this tile's explicit count is published now; the ranks of the earlier
+    // tiles are waited for only when the results are written
+    if (threadIdx.x == 0) lb_store(&status[tile], (tile == 0 ? LB_INC : LB_AGG) | tot);
+    // windows from the last to the first
+    const uint32_t wend = wbase + CH_TILE / W;
+    for (int k = WIN - 1; k >= 0; --k) {
+        const uint32_t j = wbase + (uint32_t)k * 64 + lane;
+        const uint32_t lo = wbase + (uint32_t)k * 64, hi = lo + 64;
+        uint32_t v = 0;
+        if (j < cnt) {
+            if ((ebits >> k) & 1u) {
+                v = woff + ord[k];
             } else {
-                s_val[j] = CH_LOCAL | j;
-                s_ext[j] = (uint32_t)(Fr.pt_off + t);
+                const uint64_t t = (uint64_t)j + dv[k];
+                if (p0 + t >= Fr.n) {
+                    atomicOr(err, DERR_CHAIN);   // a successor outside the frame: the stream is rejected
+                    v = 0;
+                } else if (t >= cnt) {
+                    v = CS_EXIT | j;             // leaves the tile: j is an exit
+                } else if (t >= wend) {
+                    v = CS_PEND | (uint32_t)t;   // a later wavefront's range
+                } else if (t >= hi) {
+                    v = s_val[t];                // a later window of this wavefront: resolved
+                } else {
+                    v = CS_WIN | (uint32_t)(t - lo);
+                }
             }
         }
+        // pointer jumping inside the window: a lane pointing at a later lane
+        // takes that lane's state until none points inside the window
+        while (__ballot((v & CS_WIN) != 0)) {
+            const uint32_t src_lane = (v & CS_WIN) ? (v & 63u) : lane;
+            const uint32_t u = __shfl(v, (int)src_lane, 64);
+            if (v & CS_WIN) v = u;
+        }
+        if (j < cnt) s_val[j] = v;
     }
     __syncthreads();
-    // 12 synchronous rounds cover the 4096 links a tile can hold; an entry
-    // pointing to itself with s_ext set is an exit of the tile
-    for (int round = 0; round < 12; ++round) {
-        uint32_t nv[CH_ITEMS], ne[CH_ITEMS];
-        int open = 0;
+    // pending entries: follow into later ranges (resolved there, or pending
+    // further on); nothing is written until every wavefront has read
+    uint32_t fin[WIN];
 #pragma unroll
-        for (int k = 0; k < CH_ITEMS; ++k) {
-            const uint32_t j = k * CH_WG + threadIdx.x;
-            nv[k] = 0;
-            ne[k] = 0xffffffffu;
-            if (j >= cnt) continue;
-            nv[k] = s_val[j];
-            ne[k] = s_ext[j];
-            if ((nv[k] & CH_LOCAL) && ne[k] == 0xffffffffu) {
-                const uint32_t tj = nv[k] & ~CH_LOCAL;
-                nv[k] = s_val[tj];
-                ne[k] = s_ext[tj];
-                open |= (nv[k] & CH_LOCAL) && ne[k] == 0xffffffffu;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < CH_ITEMS; ++k) {
-            const uint32_t j = k * CH_WG + threadIdx.x;
-            if (j < cnt) {
-                s_val[j] = nv[k];
-                s_ext[j] = ne[k];
-            }
-        }
-        if (!__syncthreads_or(open)) break;   // every in-tile pointer resolved
+    for (int k = 0; k < WIN; ++k) {
+        const uint32_t j = wbase + k * 64 + lane;
+        uint32_t v = j < cnt ? s_val[j] : 0u;
+        while (v & CS_PEND) v = s_val[v & 0xffffu];
+        fin[k] = v;
     }
     // the tile's first rank: explicit points of all earlier tiles (every
     // frame's, in order)
     if (threadIdx.x < 64) {
-        const uint64_t excl = lookback(status, tile, s_tot, err);
+        const uint64_t excl = lookback(status, tile, tot, err);
         if (threadIdx.x == 0) {
             s_rank0 = (uint32_t)excl;
             tile_expl_off[tile] = (uint32_t)excl;
-            if (tile + 1 == gridDim.x) *expl_total = excl + s_tot;
+            if (tile + 1 == gridDim.x) *expl_total = excl + tot;
         }
     }
     __syncthreads();
     const uint32_t rank0 = s_rank0;
-    // results; points whose chain leaves the tile go to the open list, in
-    // tile order, with one reservation per tile
+    const uint32_t gbase = (uint32_t)(Fr.pt_off + p0);
+    // results; exits go to the open list, one reservation per tile
     uint32_t* dst = res + Fr.pt_off + p0;
-    uint32_t ext_bits = 0;
+    uint32_t xbits = 0, nx = 0;
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint32_t j = k * CH_WG + threadIdx.x;
-        bool ext = false;
-        if (j < cnt) {
-            const uint32_t v = s_val[j], g2 = s_ext[j];
-            uint32_t out;
-            if (g2 != 0xffffffffu) {
-                // the exit point of a chain segment (still pointing at itself)
-                // names its successor in a later tile and joins the open list;
-                // the points that reach it name it and follow it afterwards
-                ext = v == (CH_LOCAL | j);
-                out = CH_EXT | (ext ? g2 : (uint32_t)(Fr.pt_off + p0 + (v & ~CH_LOCAL)));
-            } else if (v & CH_LOCAL) {
-                atomicOr(err, DERR_CHAIN);
-                out = 0;
+    for (int k = 0; k < WIN; ++k) {
+        const uint32_t j = wbase + k * 64 + lane;
+        if (j >= cnt) continue;
+        const uint32_t v = fin[k];
+        uint32_t out;
+        if (v & CS_EXIT) {
+            const uint32_t e = v & 0xffffu;
+            if (e == j) {
+                out = CH_EXT | (gbase + j + dv[k]);   // the exit: its successor outside the tile
+                xbits |= 1u << k;
+                ++nx;
             } else {
-                out = rank0 + v;
+                out = CH_EXT | (gbase + e);           // a follower: its exit
             }
-            dst[j] = out;
+        } else {
+            out = rank0 + v;
         }
-        const uint64_t m = __ballot(ext);
-        pre[k] = __popcll(m & lt);
-        ext_bits |= (ext ? 1u : 0u) << k;
-        // s_off (the explicit-rank offsets) was last read before the rounds
-        if (lane == 0) s_off[k * W + wave] = __popcll(m);
+        dst[j] = out;
     }
-    __syncthreads();
-    __shared__ uint32_t s_base;
-    if (threadIdx.x < 64) {
-        const uint32_t v = threadIdx.x < CH_ITEMS * W ? s_off[threadIdx.x] : 0u;
-        const uint32_t inc = wave_incl_scan(v);
-        if (threadIdx.x < CH_ITEMS * W) s_off[threadIdx.x] = inc - v;
-        const uint32_t total = __shfl(inc, 63, 64);
-        if (threadIdx.x == 0) s_base = total ? atomicAdd(open_count, total) : 0u;
-    }
-    __syncthreads();
-    if (ext_bits) {
-        const uint32_t base = s_base;
+    {
+        uint32_t inc = nx;
 #pragma unroll
-        for (int k = 0; k < CH_ITEMS; ++k)
-            if (ext_bits & (1u << k))
-                open_list[base + s_off[k * W + wave] + pre[k]] =
-                    (uint32_t)(Fr.pt_off + p0 + k * CH_WG + threadIdx.x);
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= (uint32_t)o) inc += t;
+        }
+        if (lane == 63) s_wtot[w] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t all = s_wtot[0] + s_wtot[1] + s_wtot[2] + s_wtot[3];
+            s_base = all ? atomicAdd(open_count, all) : 0u;
+        }
+        __syncthreads();
+        if (xbits) {
+            uint32_t at = s_base + inc - nx;
+            for (uint32_t q = 0; q < w; ++q) at += s_wtot[q];
+#pragma unroll
+            for (int k = 0; k < WIN; ++k)
+                if ((xbits >> k) & 1u) open_list[at++] = gbase + wbase + k * 64 + lane;
+        }
     }
 }
 
@@ -1144,6 +1140,40 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_jump(uint32_t* __restrict__
                 if (open_bits & (1u << k)) list_out[base + s_off[k * W + wave] + pre[k]] = idx[k];
         }
         __syncthreads();   // s_off / s_base are reused by the next tile
+    }
+}
+
+// The exits of every tile, settled in one launch: an exit follows pointers
+// (a follower's points to its exit, an exit's to its successor) until it
+// reaches an explicit record, writing what it found back as it goes so that
+// the exits behind it take the longer jumps. The list (in tile order) is
+// taken from its end, so the exits nearest their records settle first and
+// a walk from an earlier exit mostly finds its target settled. Every pointer
+// goes to a larger index, so reading a value before or after another
+// thread's write-back is equally valid and every walk ends; the bound only
+// guards corrupt input.
+#define CH_WALK_LIMIT (1u << 22)
+__global__ __launch_bounds__(CH_WG) void k_dec_chain_resolve(uint32_t* __restrict__ res,
+                                                             const uint32_t* __restrict__ list,
+                                                             const uint32_t* __restrict__ count, uint32_t* err) {
+    const uint32_t n = *count;
+    for (uint32_t k = blockIdx.x * CH_WG + threadIdx.x; k < n; k += gridDim.x * CH_WG) {
+        const uint32_t idx = list[n - 1 - k];
+        uint32_t v = __hip_atomic_load(&res[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t steps = 0;
+        while (v & CH_EXT) {
+            const uint32_t u = __hip_atomic_load(&res[v & ~CH_EXT], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((u & CH_EXT) && (u & ~CH_EXT) <= (v & ~CH_EXT)) {   // never on valid input
+                atomicOr(err, DERR_CHAIN);
+                break;
+            }
+            v = u;
+            __hip_atomic_store(&res[idx], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (++steps > CH_WALK_LIMIT) {
+                atomicOr(err, DERR_CHAIN);
+                break;
+            }
+        }
     }
 }
 

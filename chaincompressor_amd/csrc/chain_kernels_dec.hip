@@ -568,12 +568,7 @@ __device__ inline void agg_combine(Agg<V>& left, const Agg<V>& right, V R) {   /
     }
 }
 
-// Striped layout: item k of thread t is record k*CH_WG + t of the tile, so
-// every row of 256 records is loaded with coalesced accesses. Pass 1 reduces
-// each (row, wave) segment of 64 records with an ordered shuffle tree; wave 0
-// combines the 64 segment aggregates into the tile aggregate and does the
-// cross-tile look-back; pass 2 recomputes the elements, scans each segment
-// with shuffles and writes prefix (+) element.
+// shuffles of a scan element (reset flag and D values)
 template <int D, typename V>
 __device__ inline void agg_shfl_up(uint32_t& r, V* v, uint32_t off) {
     r = __shfl_up(r, off, 64);
@@ -601,6 +596,112 @@ __device__ inline void agg_comb(uint32_t& lr, V* lv, uint32_t rr, const V* rv, V
     }
 }
 
+// Wave 0 of an L2 tile: publishes the tile aggregate, looks back over the
+// earlier tiles of the frame (64 predecessors at a time: lane l reads tile
+// t - l; the window up to the nearest inclusive or resetting aggregate is
+// combined in order, farther tiles on the left), publishes the inclusive
+// prefix and returns the exclusive one in every lane.
+// flags[t]: 1 = aggregate, 2 = inclusive prefix, bit 8 = the value resets.
+template <int D, typename V>
+__device__ inline Agg<V> l2_lookback(uint32_t tile, uint32_t seg, const Agg<V>& tot, V R, uint32_t* flags,
+                                     uint64_t* aggv, uint32_t* err) {
+    const uint32_t lane = threadIdx.x & 63u;
+    Agg<V> incoming;
+    incoming.reset = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) incoming.v[d] = 0;
+    if (tile != seg) {
+        if (lane == 0) {
+            for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + d] = tot.v[d];
+            __hip_atomic_store(&flags[tile], 1u | (tot.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // look-back over 64 predecessors at a time: lane l reads tile
+        // t - l; the window up to the nearest inclusive (or resetting)
+        // aggregate is combined in order, farther tiles on the left
+        uint32_t ar = 0;
+        V av[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) av[d] = 0;
+        int64_t t = (int64_t)tile - 1;
+        uint32_t spins = 0;
+        while (t >= (int64_t)seg) {
+            const int64_t mine = t - (int64_t)lane;
+            const bool beyond = mine < (int64_t)seg;
+            const uint32_t fl =
+                beyond ? 2u : __hip_atomic_load(&flags[mine], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t st = fl & 0xffu;
+            const uint64_t not_ready = __ballot(!beyond && st == 0);
+            const uint64_t stop = __ballot(st == 2 || (fl >> 8) != 0);
+            const uint64_t first_stop = stop ? (stop & (~stop + 1)) : 0ull;
+            const uint64_t need = first_stop ? (first_stop | (first_stop - 1)) : ~0ull;
+            if (not_ready & need) {
+                if (++spins > LB_SPIN_LIMIT) {
+                    if (lane == 0) atomicOr(err, DERR_WAIT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            uint32_t yr = 0;
+            V yv[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) yv[d] = 0;
+            if (((need >> lane) & 1ull) && !beyond) {
+                yr = fl >> 8;
+                const uint64_t slot = st == 2 ? 1 : 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) yv[d] = (V)aggv[((uint64_t)mine * 2 + slot) * CH_MAXD + d];
+            }
+            for (uint32_t off = 1; off < 64; off <<= 1) {
+                uint32_t orr = yr;
+                V ov[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) ov[d] = yv[d];
+                agg_shfl_down<D, V>(orr, ov, off);
+                if (lane + off < 64 && (lane & (2 * off - 1)) == 0) {
+                    agg_comb<D, V>(orr, ov, yr, yv, R);   // farther (+) nearer
+                    yr = orr;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) yv[d] = ov[d];
+                }
+            }
+            uint32_t wr = __shfl(yr, 0, 64);
+            V wv[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) wv[d] = __shfl(yv[d], 0, 64);
+            agg_comb<D, V>(wr, wv, ar, av, R);   // window (+) what was nearer
+            ar = wr;
+#pragma unroll
+            for (int d = 0; d < D; ++d) av[d] = wv[d];
+            if (first_stop) break;
+            t -= 64;
+        }
+        incoming.reset = ar;
+#pragma unroll
+        for (int d = 0; d < D; ++d) incoming.v[d] = av[d];
+    }
+    if (lane == 0) {
+        Agg<V> inc = incoming;
+        agg_combine<D, V>(inc, tot, R);
+        for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 1) * CH_MAXD + d] = inc.v[d];
+        __hip_atomic_store(&flags[tile], 2u | (inc.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    incoming.reset = __shfl(incoming.reset, 0, 64);
+#pragma unroll
+    for (int d = 0; d < D; ++d) incoming.v[d] = __shfl(incoming.v[d], 0, 64);
+    return incoming;
+}
+
+// L2 inverse of one tile of CH_TILE records, blocked: every record's element
+// is built from coalesced loads in striped order (record k*CH_WG + t by thread
+// t) and stored in LDS; thread t then scans the CH_ITEMS consecutive records
+// [CH_ITEMS*t, CH_ITEMS*t + CH_ITEMS) in registers, the threads' aggregates
+// are scanned with shuffles (then over the wavefronts, then across tiles by
+// look-back), and the tile's records leave LDS as one coalesced run of
+// n*D words. LDS rows are padded by one word per CH_ITEMS records, so the
+// blocked reads (stride CH_ITEMS+1 words) hit distinct banks.
+#define L2_PAD(r) ((r) + ((r) / CH_ITEMS))
+#define L2_LDS_WORDS (CH_TILE + CH_TILE / CH_ITEMS)
 template <int D, typename V>
 __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ fr, uint32_t F,
                                                   const uint32_t* __restrict__ rtile_begin,
@@ -610,11 +711,13 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
                                                   uint32_t* flags, uint64_t* aggv, uint32_t* ticket,
                                                   uint64_t* __restrict__ compact, uint32_t* err) {
     constexpr uint32_t W = CH_WG / 64;
-    constexpr uint32_t NS = CH_ITEMS * W;   // (row, wave) segments per tile = 64
+    constexpr uint32_t NS = CH_ITEMS * W;   // (row, wave) segments of the striped order
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_zoff[NS];         // keep records before each segment
-    __shared__ uint32_t s_sr[NS];           // segment aggregates, then exclusive prefixes
-    __shared__ V s_sv[NS][D];
+    __shared__ V s_v[D][L2_LDS_WORDS];
+    __shared__ uint8_t s_rs[L2_LDS_WORDS];
+    __shared__ uint32_t s_wr[W];
+    __shared__ V s_wv[W][D];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -624,24 +727,25 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     const uint64_t t0 = (uint64_t)(tile - rtile_begin[f]) * CH_TILE;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    // keep records (choice 0) before each record of the tile, within its
-    // (row, wave) segment (in LDS: the passes index items by a loop variable)
-    __shared__ uint16_t s_zpre[CH_ITEMS][CH_WG];
+    // keep records (choice 0): ballots per (row, wave) segment, in order
     uint32_t zbits = 0;
-    uint8_t chv[CH_ITEMS];
+    uint32_t zpre[CH_ITEMS];
+    {
+        uint8_t chv[CH_ITEMS];
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {   // loads first (a tile holds at least one record)
-        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        chv[k] = choice[Fr.rec_base + (r < Fr.nrec ? r : t0)];
-    }
+        for (int k = 0; k < CH_ITEMS; ++k) {   // loads first (a tile holds at least one record)
+            const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+            chv[k] = choice[Fr.rec_base + (r < Fr.nrec ? r : t0)];
+        }
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        const bool z = r < Fr.nrec && chv[k] == 0;
-        const uint64_t m = __ballot(z);
-        s_zpre[k][threadIdx.x] = (uint16_t)__popcll(m & lt);
-        zbits |= (z ? 1u : 0u) << k;
-        if (lane == 0) s_zoff[k * W + wave] = __popcll(m);
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+            const bool z = r < Fr.nrec && chv[k] == 0;
+            const uint64_t m = __ballot(z);
+            zpre[k] = (uint32_t)__popcll(m & lt);
+            zbits |= (z ? 1u : 0u) << k;
+            if (lane == 0) s_zoff[k * W + wave] = __popcll(m);
+        }
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -650,252 +754,167 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     }
     __syncthreads();
     const uint64_t kbase = tile_zero_off[tile];
-    // raw values of item k: the keep record or the diff symbol it reads
-    // (loaded for a group of items before any is used)
-    auto load_raw = [&](int k, uint64_t* raw) {
-        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        const uint64_t kix = kbase + s_zoff[k * W + wave] + s_zpre[k][threadIdx.x];   // global keep index (or keeps before)
-        const bool z = (zbits >> k) & 1u;
-        const uint64_t idx = z ? kix : Fr.rec_base + r - kix;
-        const bool inb = r < Fr.nrec && (z ? kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep
-                                           : idx >= Fr.diff_base && idx < Fr.diff_base + Fr.ndiff);
-        const uint64_t* src = z ? keep : diff;
-        // unconditional load (record 0 of the array, which always exists)
-        const uint64_t ic = inb ? idx : 0ull;
+    // striped: elements into LDS. keep -> (reset, value - wrap_min),
+    // diff -> (0, delta mod R); past the frame's end the identity (0, 0)
+#define L2_GROUP 4
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const uint64_t v = src[ic * D + d];
-            raw[d] = inb ? v : (z ? (uint64_t)Fr.wrap_min : 0ull);
-        }
-    };
-    // element of item k: keep -> (reset, value - wrap_min), diff -> (0, delta mod R)
-    auto element = [&](int k, bool check, const uint64_t* raw, V* v) -> uint32_t {
-        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        if (r >= Fr.nrec) {
+    for (int kg = 0; kg < CH_ITEMS; kg += L2_GROUP) {
+        uint64_t raw[L2_GROUP][D];
 #pragma unroll
-            for (int d = 0; d < D; ++d) v[d] = 0;
-            return 0;   // identity
-        }
-        const uint64_t grec = Fr.rec_base + r;
-        const uint64_t kix = kbase + s_zoff[k * W + wave] + s_zpre[k][threadIdx.x];
-        if ((zbits >> k) & 1u) {
-            const bool inb = kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep;
-            if (check && !inb) atomicOr(err, DERR_L2);
+        for (int j = 0; j < L2_GROUP; ++j) {   // the group's loads before any use
+            const int k = kg + j;
+            const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+            const uint64_t kix = kbase + s_zoff[k * W + wave] + zpre[k];   // global keep index (or keeps before)
+            const bool z = (zbits >> k) & 1u;
+            const uint64_t idx = z ? kix : Fr.rec_base + r - kix;
+            const bool inb = r < Fr.nrec && (z ? kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep
+                                               : idx >= Fr.diff_base && idx < Fr.diff_base + Fr.ndiff);
+            const uint64_t* src = z ? keep : diff;
+            const uint64_t ic = inb ? idx : 0ull;   // record 0 of the array always exists
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const uint64_t kv = raw[d];
-                if (check && ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max)) atomicOr(err, DERR_L2);
-                v[d] = (V)(kv - (uint64_t)Fr.wrap_min);
+                const uint64_t v = src[ic * D + d];
+                raw[j][d] = inb ? v : (z ? (uint64_t)Fr.wrap_min : 0ull);
             }
-            return 1;
         }
-        if (check && r == 0) atomicOr(err, DERR_L2);
-        const uint64_t didx = grec - kix;   // global diff index
-        const bool inb = didx >= Fr.diff_base && didx < Fr.diff_base + Fr.ndiff;
-        if (check && !inb) atomicOr(err, DERR_L2);
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const uint64_t sym = raw[d];
-            const int64_t corr = (sym & 1u) ? -(int64_t)(sym >> 1) - 1 : (int64_t)(sym >> 1);
-            if (check && ((int64_t)sym < 0 || corr < Fr.min_corr || corr > Fr.max_corr)) atomicOr(err, DERR_L2);
-            const uint64_t delta = corr > 0 ? (uint64_t)R - (uint64_t)corr : (uint64_t)(-corr);
-            v[d] = (V)(delta < (uint64_t)R ? delta : delta % (uint64_t)R);
+        for (int j = 0; j < L2_GROUP; ++j) {
+            const int k = kg + j;
+            const uint32_t rl = k * CH_WG + threadIdx.x;
+            const uint64_t r = t0 + rl;
+            uint32_t rs = 0;
+            V v[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) v[d] = 0;
+            if (r < Fr.nrec) {
+                const uint64_t kix = kbase + s_zoff[k * W + wave] + zpre[k];
+                if ((zbits >> k) & 1u) {
+                    if (kix < Fr.keep_base || kix >= Fr.keep_base + Fr.nkeep) atomicOr(err, DERR_L2);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const uint64_t kv = raw[j][d];
+                        if ((int64_t)kv < Fr.wrap_min || (int64_t)kv > Fr.wrap_max) atomicOr(err, DERR_L2);
+                        v[d] = (V)(kv - (uint64_t)Fr.wrap_min);
+                    }
+                    rs = 1;
+                } else {
+                    if (r == 0) atomicOr(err, DERR_L2);
+                    const uint64_t didx = Fr.rec_base + r - kix;   // global diff index
+                    if (didx < Fr.diff_base || didx >= Fr.diff_base + Fr.ndiff) atomicOr(err, DERR_L2);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const uint64_t sym = raw[j][d];
+                        const int64_t corr = (sym & 1u) ? -(int64_t)(sym >> 1) - 1 : (int64_t)(sym >> 1);
+                        if ((int64_t)sym < 0 || corr < Fr.min_corr || corr > Fr.max_corr) atomicOr(err, DERR_L2);
+                        const uint64_t delta = corr > 0 ? (uint64_t)R - (uint64_t)corr : (uint64_t)(-corr);
+                        v[d] = (V)(delta < (uint64_t)R ? delta : delta % (uint64_t)R);
+                    }
+                }
+            }
+            const uint32_t pos = L2_PAD(rl);
+#pragma unroll
+            for (int d = 0; d < D; ++d) s_v[d][pos] = v[d];
+            s_rs[pos] = (uint8_t)rs;
         }
-        return 0;
-    };
-#define L2_GROUP 4
-    // pass 1: ordered reduction of every (row, wave) segment
-#pragma unroll 1
-    for (int kg = 0; kg < CH_ITEMS; kg += L2_GROUP) {
-      uint64_t raw[L2_GROUP][D];
-#pragma unroll
-      for (int j = 0; j < L2_GROUP; ++j) load_raw(kg + j, raw[j]);
-#pragma unroll
-      for (int j = 0; j < L2_GROUP; ++j) {
-        const int k = kg + j;
-        V v[D];
-        uint32_t rs = element(k, true, raw[j], v);
-        for (uint32_t off = 1; off < 64; off <<= 1) {
-            uint32_t orr = rs;
-            V ov[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) ov[d] = v[d];
-            agg_shfl_down<D, V>(orr, ov, off);
-            if (lane + off < 64 && (lane & (2 * off - 1)) == 0) agg_comb<D, V>(rs, v, orr, ov, R);
-        }
-        if (lane == 0) {
-            s_sr[k * W + wave] = rs;
-#pragma unroll
-            for (int d = 0; d < D; ++d) s_sv[k * W + wave][d] = v[d];
-        }
-      }
     }
     __syncthreads();
-    // wave 0: exclusive scan of the 64 segment aggregates, tile aggregate,
-    // look-back with multi-word aggregates (flag 1 = tile aggregate, 2 = inclusive)
+    // blocked: inclusive scan of the thread's records in registers; rbits
+    // marks the records at or after the thread's first reset
+    const uint32_t pbase = L2_PAD(threadIdx.x * CH_ITEMS);
+    V iv[CH_ITEMS][D];
+    uint32_t ar = 0, rbits = 0;
+    V av[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) av[d] = 0;
+#pragma unroll
+    for (int j = 0; j < CH_ITEMS; ++j) {
+        const uint32_t rs = s_rs[pbase + j];
+        V v[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) v[d] = s_v[d][pbase + j];
+        agg_comb<D, V>(ar, av, rs, v, R);
+        rbits |= ar << j;
+#pragma unroll
+        for (int d = 0; d < D; ++d) iv[j][d] = av[d];
+    }
+    // exclusive scan of the threads' aggregates within the wavefront
+    uint32_t xr = ar;
+    V xv[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xv[d] = av[d];
+    for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan
+        uint32_t orr = xr;
+        V ov[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) ov[d] = xv[d];
+        agg_shfl_up<D, V>(orr, ov, off);
+        if (lane >= off) {
+            agg_comb<D, V>(orr, ov, xr, xv, R);   // earlier (+) mine
+            xr = orr;
+#pragma unroll
+            for (int d = 0; d < D; ++d) xv[d] = ov[d];
+        }
+    }
+    if (lane == 63) {
+        s_wr[wave] = xr;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s_wv[wave][d] = xv[d];
+    }
+    agg_shfl_up<D, V>(xr, xv, 1);   // inclusive of lane - 1 = exclusive of lane
+    if (lane == 0) {
+        xr = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) xv[d] = 0;
+    }
+    __syncthreads();
+    // wave 0: the tile aggregate (the wavefronts in order), the look-back,
+    // then every wavefront's exclusive prefix
     if (threadIdx.x < 64) {
-        uint32_t rs = s_sr[lane];
-        V v[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) v[d] = s_sv[lane][d];
-        const uint32_t r_self = rs;
-        V v_self[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) v_self[d] = v[d];
-        for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan
-            uint32_t orr = rs;
-            V ov[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) ov[d] = v[d];
-            agg_shfl_up<D, V>(orr, ov, off);
-            if (lane >= off) {
-                agg_comb<D, V>(orr, ov, rs, v, R);   // earlier (+) mine
-                rs = orr;
-#pragma unroll
-                for (int d = 0; d < D; ++d) v[d] = ov[d];
-            }
-        }
-        // tile aggregate = inclusive value of lane 63
         Agg<V> tot;
-        tot.reset = __shfl(rs, 63, 64);
+        tot.reset = 0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) tot.v[d] = __shfl(v[d], 63, 64);
-        Agg<V> incoming;
-        incoming.reset = 0;
+        for (int d = 0; d < D; ++d) tot.v[d] = 0;
+        Agg<V> wagg[W];
 #pragma unroll
-        for (int d = 0; d < D; ++d) incoming.v[d] = 0;
-        const uint32_t seg = rtile_begin[f];
-        if (tile != seg) {
-            if (lane == 0) {
-                for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 0) * CH_MAXD + d] = tot.v[d];
-                __hip_atomic_store(&flags[tile], 1u | (tot.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            // look-back over 64 predecessors at a time: lane l reads tile
-            // t - l; the window up to the nearest inclusive (or resetting)
-            // aggregate is combined in order, farther tiles on the left
-            uint32_t ar = 0;
-            V av[D];
+        for (uint32_t w = 0; w < W; ++w) {
+            wagg[w].reset = s_wr[w];
 #pragma unroll
-            for (int d = 0; d < D; ++d) av[d] = 0;
-            int64_t t = (int64_t)tile - 1;
-            uint32_t spins = 0;
-            while (t >= (int64_t)seg) {
-                const int64_t mine = t - (int64_t)lane;
-                const bool beyond = mine < (int64_t)seg;
-                const uint32_t fl =
-                    beyond ? 2u : __hip_atomic_load(&flags[mine], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t st = fl & 0xffu;
-                const uint64_t not_ready = __ballot(!beyond && st == 0);
-                const uint64_t stop = __ballot(st == 2 || (fl >> 8) != 0);
-                const uint64_t first_stop = stop ? (stop & (~stop + 1)) : 0ull;
-                const uint64_t need = first_stop ? (first_stop | (first_stop - 1)) : ~0ull;
-                if (not_ready & need) {
-                    if (++spins > LB_SPIN_LIMIT) {
-                        if (lane == 0) atomicOr(err, DERR_WAIT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                uint32_t yr = 0;
-                V yv[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) yv[d] = 0;
-                if (((need >> lane) & 1ull) && !beyond) {
-                    yr = fl >> 8;
-                    const uint64_t slot = st == 2 ? 1 : 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) yv[d] = (V)aggv[((uint64_t)mine * 2 + slot) * CH_MAXD + d];
-                }
-                for (uint32_t off = 1; off < 64; off <<= 1) {
-                    uint32_t orr = yr;
-                    V ov[D];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) ov[d] = yv[d];
-                    agg_shfl_down<D, V>(orr, ov, off);
-                    if (lane + off < 64 && (lane & (2 * off - 1)) == 0) {
-                        agg_comb<D, V>(orr, ov, yr, yv, R);   // farther (+) nearer
-                        yr = orr;
-#pragma unroll
-                        for (int d = 0; d < D; ++d) yv[d] = ov[d];
-                    }
-                }
-                uint32_t wr = __shfl(yr, 0, 64);
-                V wv[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) wv[d] = __shfl(yv[d], 0, 64);
-                agg_comb<D, V>(wr, wv, ar, av, R);   // window (+) what was nearer
-                ar = wr;
-#pragma unroll
-                for (int d = 0; d < D; ++d) av[d] = wv[d];
-                if (first_stop) break;
-                t -= 64;
-            }
-            incoming.reset = ar;
-#pragma unroll
-            for (int d = 0; d < D; ++d) incoming.v[d] = av[d];
+            for (int d = 0; d < D; ++d) wagg[w].v[d] = s_wv[w][d];
+            agg_combine<D, V>(tot, wagg[w], R);
         }
+        Agg<V> run = l2_lookback<D, V>(tile, rtile_begin[f], tot, R, flags, aggv, err);
         if (lane == 0) {
-            Agg<V> inc = incoming;
-            agg_combine<D, V>(inc, tot, R);
-            for (int d = 0; d < D; ++d) aggv[((uint64_t)tile * 2 + 1) * CH_MAXD + d] = inc.v[d];
-            __hip_atomic_store(&flags[tile], 2u | (inc.reset << 8), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (uint32_t w = 0; w < W; ++w) {
+                s_wr[w] = run.reset;
+#pragma unroll
+                for (int d = 0; d < D; ++d) s_wv[w][d] = run.v[d];
+                agg_combine<D, V>(run, wagg[w], R);
+            }
         }
-        incoming.reset = __shfl(incoming.reset, 0, 64);
-#pragma unroll
-        for (int d = 0; d < D; ++d) incoming.v[d] = __shfl(incoming.v[d], 0, 64);
-        // exclusive prefix of segment `lane` = incoming (+) inclusive of segment lane-1
-        uint32_t pr = incoming.reset;
-        V pv[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) pv[d] = incoming.v[d];
-        uint32_t br = __shfl_up(rs, 1, 64);
-        V bv[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) bv[d] = __shfl_up(v[d], 1, 64);
-        if (lane > 0) agg_comb<D, V>(pr, pv, br, bv, R);
-        (void)r_self;
-        (void)v_self;
-        s_sr[lane] = pr;
-#pragma unroll
-        for (int d = 0; d < D; ++d) s_sv[lane][d] = pv[d];
     }
     __syncthreads();
-    // pass 2: segment scan + prefix, output
-#pragma unroll 1
-    for (int kg = 0; kg < CH_ITEMS; kg += L2_GROUP) {
-      uint64_t raw[L2_GROUP][D];
+    // the thread's prefix = wavefront prefix (+) exclusive of the lane; a
+    // record at or after the thread's first reset keeps its own inclusive value
+    uint32_t pr = s_wr[wave];
+    V pv[D];
 #pragma unroll
-      for (int j = 0; j < L2_GROUP; ++j) load_raw(kg + j, raw[j]);
+    for (int d = 0; d < D; ++d) pv[d] = s_wv[wave][d];
+    agg_comb<D, V>(pr, pv, xr, xv, R);
 #pragma unroll
-      for (int j = 0; j < L2_GROUP; ++j) {
-        const int k = kg + j;
-        V v[D];
-        uint32_t rs = element(k, false, raw[j], v);
-        for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan in the wave
-            uint32_t orr = rs;
-            V ov[D];
+    for (int j = 0; j < CH_ITEMS; ++j) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) ov[d] = v[d];
-            agg_shfl_up<D, V>(orr, ov, off);
-            if (lane >= off) {
-                agg_comb<D, V>(orr, ov, rs, v, R);
-                rs = orr;
-#pragma unroll
-                for (int d = 0; d < D; ++d) v[d] = ov[d];
-            }
-        }
-        uint32_t pr = s_sr[k * W + wave];
-        V pv[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) pv[d] = s_sv[k * W + wave][d];
-        agg_comb<D, V>(pr, pv, rs, v, R);
-        const uint64_t r = t0 + k * CH_WG + threadIdx.x;
-        if (r < Fr.nrec) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) compact[(Fr.rec_base + r) * D + d] = (uint64_t)Fr.wrap_min + (uint64_t)pv[d];
-        }
-      }
+        for (int d = 0; d < D; ++d)
+            s_v[d][pbase + j] = ((rbits >> j) & 1u) ? iv[j][d] : addmod(pv[d], iv[j][d], R);
+    }
+    __syncthreads();
+    // one coalesced run of the tile's records
+    const uint64_t left = Fr.nrec - t0;
+    const uint32_t nt = left < CH_TILE ? (uint32_t)left : CH_TILE;
+    uint64_t* out = compact + (Fr.rec_base + t0) * D;
+    for (uint32_t e = threadIdx.x; e < nt * D; e += CH_WG) {
+        const uint32_t r = e / D, d = e - r * D;
+        out[e] = (uint64_t)Fr.wrap_min + (uint64_t)s_v[d][L2_PAD(r)];
     }
 }
 

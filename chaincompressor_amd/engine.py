@@ -448,6 +448,13 @@ class Engine:
         self._check(self._L.chain_copy(self._h, out, C.c_void_p(ptr), nbytes, 0))
         return bytes(out)[:nbytes]
 
+    def to_host_into(self, ptr: int, out: np.ndarray) -> np.ndarray:
+        """Copy out.nbytes bytes from the device straight into a contiguous host array."""
+        if not out.flags.c_contiguous:
+            raise ChainValueError("out must be C-contiguous")
+        self._check(self._L.chain_copy(self._h, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), out.nbytes, 0))
+        return out
+
     def stage_times(self, with_bytes=False):
         """[(stage, ms)] of the last call, or [(stage, ms, algorithmic bytes)]."""
         ms = (C.c_float * 128)()

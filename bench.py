@@ -515,8 +515,14 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
 
     # the batched path: entropy coding and L1 strategy tuned on this batch,
     # launches replayed from a captured HIP graph
-    best, trials = eng.autotune(ptrs, ns, dtype, 3, eb, p, thr)
-    tuned = dict(best, trials=[dict(t, ms=round(ms, 3), bytes=b) for t, ms, b in trials])
+    if args.tuning:
+        lanes, chunk, l1 = (int(x) for x in args.tuning.split(","))
+        eng.set_autotune(False)
+        eng.set_tuning(lanes, chunk, l1)
+        tuned = dict(eng.tuning(), trials="fixed by --tuning")
+    else:
+        best, trials = eng.autotune(ptrs, ns, dtype, 3, eb, p, thr)
+        tuned = dict(best, trials=[dict(t, ms=round(ms, 3), bytes=b) for t, ms, b in trials])
 
     def comp_batched(graph=True):
         eng.set_graphs(graph)
@@ -533,8 +539,8 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
     soffs = np.concatenate([[0], np.cumsum(csizes)[:-1]]).astype(np.int64).tolist()
 
     # the per-frame path (same tuned setting) writes the same containers
-    identical = True
-    for k in range(len(ns)):
+    identical = None if args.no_per_frame else True
+    for k in range(0 if args.no_per_frame else len(ns)):
         o, oo, ss, _ = eng.compress_batch_device([ptrs[k]], [ns[k]], dtype, 3, eb, p, thr)
         if ss[0] != csizes[k] or eng.to_host(o + oo[0], ss[0]) != streams[k]:
             identical = False
@@ -562,9 +568,9 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
 
     steps = args.batch_steps
     t_cb = timed(comp_batched, steps, 1)
-    t_cf = timed(comp_frames, steps, 1)
+    t_cf = None if args.no_per_frame else timed(comp_frames, steps, 1)
     t_db = timed(dec_batched, steps, 1)
-    t_df = timed(dec_frames, steps, 1)
+    t_df = None if args.no_per_frame else timed(dec_frames, steps, 1)
     # per-region device times of the batched calls (compression without the
     # graph, so that every region has its own events)
     c_steps, d_steps = [], []
@@ -589,11 +595,11 @@ def run_batched(args, eng, frames=None, eb=1e-3, dtype=np.float32, label="batche
     r = {
         "frames": len(ns), "points": total, "input_bytes": nbytes, "eb": eb, "dtype": np.dtype(dtype).name,
         "compress_gbps_batched": round(nbytes / t_cb / 1e9, 2),
-        "compress_gbps_per_frame": round(nbytes / t_cf / 1e9, 2),
-        "compress_speedup": round(t_cf / t_cb, 2),
+        "compress_gbps_per_frame": round(nbytes / t_cf / 1e9, 2) if t_cf else None,
+        "compress_speedup": round(t_cf / t_cb, 2) if t_cf else None,
         "decompress_gbps_batched": round(nbytes / t_db / 1e9, 2),
-        "decompress_gbps_per_frame": round(nbytes / t_df / 1e9, 2),
-        "decompress_speedup": round(t_df / t_db, 2),
+        "decompress_gbps_per_frame": round(nbytes / t_df / 1e9, 2) if t_df else None,
+        "decompress_speedup": round(t_df / t_db, 2) if t_df else None,
         "bits_per_particle": round(sum(csizes) * 8 / total, 3),
         "containers_identical_to_per_frame": identical,
         "max_abs_error": maxerr, "steps": steps,
@@ -766,6 +772,10 @@ def main():
     ap.add_argument("--no-search", action="store_true", help="skip the (p, threshold) search of the plain frame")
     ap.add_argument("--rans-chunk", type=int, default=0,
                     help="symbols per rANS chunk for the plain frame instead of the tuner's choice (profiling runs)")
+    ap.add_argument("--no-per-frame", action="store_true",
+                    help="batched/md: skip the one-call-per-frame comparator (profiling runs)")
+    ap.add_argument("--tuning", default="",
+                    help="batched/md: fixed lanes,chunk_symbols,l1_tiles instead of the tuner (profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 1 GiB device-to-device copy before the plain frame (profiler counter scale)")
     args = ap.parse_args()

@@ -535,7 +535,7 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                        x.tbytes, err);
     hipLaunchKernelGGL(k_prechoose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst, rc.lanes_max,
                        rc.chunk_symbols);
-    hipLaunchKernelGGL(k_rans_plan, dim3(1), dim3(1), 0, st, x.sec, nsec, F, rc.lanes_max, rc.chunk_symbols,
+    hipLaunchKernelGGL(k_rans_plan, dim3(1), dim3(CH_WG), 0, st, x.sec, nsec, F, rc.lanes_max, rc.chunk_symbols,
                        x.comp_chunk);
     tm.end();
     const uint64_t chunk_bound = comp_chunk_bound[0] + comp_chunk_bound[1] + comp_chunk_bound[2] +
@@ -559,7 +559,7 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
     et.assemble = tm.begin("assemble");
     hipLaunchKernelGGL(k_section_sizes, dim3(nsec), dim3(CH_WG), 0, st, x.sec, x.chunk_len, x.chunk_off);
     hipLaunchKernelGGL(k_choose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst);
-    hipLaunchKernelGGL(k_layout, dim3(1), dim3(1), 0, st, x.ci, x.sec, fo, F, D, x.out_off, x.out_size, x.total,
+    hipLaunchKernelGGL(k_layout, dim3(1), dim3(CH_WG), 0, st, x.ci, x.sec, fo, F, D, x.out_off, x.out_size, x.total,
                        x.tag_tiles);
     hipLaunchKernelGGL(k_write_frames, dim3(F), dim3(CH_WG), 0, st, fdev, fo, D, x.out_off, x.out_size, corr_idx,
                        corr_val, x.out);

@@ -587,33 +587,66 @@ __global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict
                 }
                 __syncthreads();
             }
-        if (threadIdx.x == 0) {
-            if (sum < M) {
-                s_prob[s_key[A - 1] & 0xfffu] += (uint32_t)(M - sum);
-            } else {
-                int64_t error = (int64_t)(sum - M);
-                uint64_t cur = sum;
-                while (error > 0) {
-                    const uint64_t sum_at_start = cur;
-                    bool changed = false;
-                    for (uint32_t j = A; j-- > 1 && error > 0;) {
-                        uint32_t& pr = s_prob[s_key[j] & 0xfffu];
-                        if (pr <= 1) {
-                            if (j == A - 1) { atomicOr(err, 4u); error = 0; }
-                            break;
-                        }
-                        const uint64_t np = rvn1_reduced_prob((int)pb, sum_at_start, pr);
-                        int64_t red = (int64_t)pr - (int64_t)np;
-                        if (red == 0) red = 1;
-                        if (red >= (int64_t)pr) red = (int64_t)pr - 1;
-                        if (red > error) red = error;
-                        pr -= (uint32_t)red;
-                        cur -= (uint64_t)red;
-                        error -= red;
-                        changed = true;
-                    }
-                    if (!changed && error > 0) { atomicOr(err, 4u); break; }
+        if (sum < M) {
+            if (threadIdx.x == 0) s_prob[s_key[A - 1] & 0xfffu] += (uint32_t)(M - sum);
+        } else {
+            // Passes over the symbols from the most probable down
+            // (lcps_cpp.cpp:626-672), position q = A - 1 - j of the sorted
+            // order. A pass stops at its first symbol of probability <= 1 and
+            // once the excess is gone. Inside a pass a symbol's reduction
+            // depends on its own probability and the sum at the pass's start
+            // only, so the pass is a block-wide prefix sum: symbol q gives up
+            // min(reduction_q, excess left after the symbols before it).
+            // Sums stay below 2^32: M <= 2^20 and A <= 4096.
+            __shared__ uint32_t s_qstop;
+            uint32_t cur = (uint32_t)sum, error = (uint32_t)(sum - M);
+            const uint32_t q0 = threadIdx.x * 16;
+            while (error > 0) {
+                if (threadIdx.x == 0) s_qstop = A - 1;   // positions 0 .. A-2; A-1 = no stop
+                __syncthreads();
+                uint32_t pq[16], qmin = A - 1;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const uint32_t q = q0 + k;
+                    pq[k] = q + 1 < A ? s_prob[s_key[A - 1 - q] & 0xfffu] : 0u;
+                    if (q + 1 < A && pq[k] <= 1 && q < qmin) qmin = q;
                 }
+                if (qmin < A - 1) atomicMin(&s_qstop, qmin);
+                __syncthreads();
+                const uint32_t qstop = s_qstop;
+                if (qstop == 0) {   // the most probable symbol cannot give up any
+                    if (threadIdx.x == 0) atomicOr(err, 4u);
+                    break;
+                }
+                uint32_t red[16], local = 0;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const uint32_t q = q0 + k;
+                    red[k] = 0;
+                    if (q < qstop) {   // pq >= 2: the reduction lies in [1, pq - 1]
+                        const uint64_t np = rvn1_reduced_prob((int)pb, cur, pq[k]);
+                        int64_t r = (int64_t)pq[k] - (int64_t)np;
+                        if (r == 0) r = 1;
+                        if (r >= (int64_t)pq[k]) r = (int64_t)pq[k] - 1;
+                        red[k] = (uint32_t)r;
+                        local += red[k];
+                    }
+                }
+                uint32_t tot;
+                uint32_t before = block_excl_scan(local, s_tmp, &tot);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    if (!red[k]) continue;
+                    if (before < error) {
+                        const uint32_t a = red[k] < error - before ? red[k] : error - before;
+                        s_prob[s_key[A - 1 - (q0 + k)] & 0xfffu] -= a;
+                    }
+                    before += red[k];
+                }
+                const uint32_t applied = tot < error ? tot : error;
+                error -= applied;
+                cur -= applied;
+                __syncthreads();   // this pass's probabilities before the next pass reads them
             }
         }
         __syncthreads();
@@ -727,21 +760,33 @@ __global__ __launch_bounds__(CH_WG) void k_rans_table(const CompInst* __restrict
 }
 
 // chunk plan of every section (one thread): lanes, chunk count, chunk bases
-__global__ void k_rans_plan(Section* sec, uint32_t nsec, uint32_t F, uint32_t lanes_max, uint32_t chunk_symbols,
-                            uint32_t* comp_chunk /* [6]: begin of component c, [5] = total */) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    uint32_t base = 0;
-    for (uint32_t si = 0; si < nsec; ++si) {
-        if (si % (2 * F) == 0) comp_chunk[si / (2 * F)] = base;
-        Section& S = sec[si];
-        S.chunk_symbols = chunk_symbols;
-        const bool used = S.eligible && S.count > 0 && S.encode;
-        S.lanes = used ? rvc1_lanes_for(S.count, lanes_max) : 0;
-        S.nchunks = used ? (uint32_t)((S.count + chunk_symbols - 1) / chunk_symbols) : 0;
-        S.chunk_base = base;
-        base += S.nchunks;
+// Chunk layout of every section: one workgroup scans the sections' chunk
+// counts in order (component c's sections start at 2cF).
+__global__ __launch_bounds__(CH_WG) void k_rans_plan(Section* sec, uint32_t nsec, uint32_t F, uint32_t lanes_max,
+                                                     uint32_t chunk_symbols,
+                                                     uint32_t* comp_chunk /* [6]: begin of component c, [5] = total */) {
+    __shared__ uint32_t s_tmp[4];
+    uint32_t run = 0;
+    for (uint32_t s0 = 0; s0 < nsec; s0 += CH_WG) {
+        const uint32_t si = s0 + threadIdx.x;
+        uint32_t nch = 0;
+        if (si < nsec) {
+            Section& S = sec[si];
+            S.chunk_symbols = chunk_symbols;
+            const bool used = S.eligible && S.count > 0 && S.encode;
+            S.lanes = used ? rvc1_lanes_for(S.count, lanes_max) : 0;
+            nch = used ? (uint32_t)((S.count + chunk_symbols - 1) / chunk_symbols) : 0;
+            S.nchunks = nch;
+        }
+        uint32_t tot;
+        const uint32_t pre = block_excl_scan(nch, s_tmp, &tot);
+        if (si < nsec) {
+            sec[si].chunk_base = run + pre;
+            if (si % (2 * F) == 0) comp_chunk[si / (2 * F)] = run + pre;
+        }
+        run += tot;
     }
-    comp_chunk[5] = base;
+    if (threadIdx.x == 0) comp_chunk[5] = run;
 }
 
 __device__ inline uint32_t find_section(const Section* sec, uint32_t lo, uint32_t hi, uint32_t chunk) {
@@ -1091,43 +1136,94 @@ __global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, u
 
 // output layout: one thread walks the frames in order
 #define TAG_TILE 1024
-__global__ void k_layout(CompInst* __restrict__ ci, Section* __restrict__ sec, const FrameOut* __restrict__ fo,
-                         uint32_t F, uint32_t D, uint64_t* __restrict__ out_off, uint64_t* __restrict__ out_size,
-                         uint64_t* __restrict__ total, uint32_t* __restrict__ tag_tiles /* [5][F+1] */) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int c = 0; c < 5; ++c) {
-        uint32_t run = 0;
-        for (uint32_t f = 0; f < F; ++f) {
-            tag_tiles[c * (F + 1) + f] = run;
-            const CompInst& C = ci[c * F + f];
-            if (C.kind == KIND_TAGR) run += (uint32_t)((C.n + TAG_TILE - 1) / TAG_TILE);
-        }
-        tag_tiles[c * (F + 1) + F] = run;
+// 64-bit exclusive scan over the workgroup (the container offsets of a batch
+// can pass 2^32)
+__device__ inline uint64_t block_excl_scan_u64(uint64_t v, uint64_t* s_tmp64, uint64_t* total) {
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    uint64_t inc = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint64_t t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
     }
-    uint64_t off = 0;
-    for (uint32_t f = 0; f < F; ++f) {
-        out_off[f] = off;
-        uint64_t p = off + 48 + 8ull * D;
-        for (int c = 0; c < 5; ++c) {
-            CompInst& C = ci[c * F + f];
-            p += 8;
-            C.off = p;
-            const uint32_t i = c * F + f;
-            if (C.kind == KIND_TAGR) {
-                sec[2 * i + 1].off = p + 4 + 4 + 8 + 1 + 8;
-                C.payload_off = sec[2 * i + 1].off + sec[2 * i + 1].size + 8;
-            } else if (C.kind == KIND_FIXP) {
-                C.payload_off = p + 16 + 17ull * C.dims;
-            } else {
-                sec[2 * i].off = p + 16 + 8;
+    if (lane == 63) s_tmp64[w] = inc;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < CH_WG / 64; ++k) {
+        const uint64_t t = s_tmp64[k];
+        before += k < w ? t : 0ull;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+// Output layout: one workgroup scans the frames' container sizes in order
+// (and, per component, the Tagged tiles of the frames); inside a frame the
+// five components follow the 48 + 8D byte header, each after its 8-byte
+// length, and the corrections close the frame (16 + 16 per correction).
+__global__ __launch_bounds__(CH_WG) void k_layout(CompInst* __restrict__ ci, Section* __restrict__ sec,
+                                                  const FrameOut* __restrict__ fo, uint32_t F, uint32_t D,
+                                                  uint64_t* __restrict__ out_off, uint64_t* __restrict__ out_size,
+                                                  uint64_t* __restrict__ total,
+                                                  uint32_t* __restrict__ tag_tiles /* [5][F+1] */) {
+    __shared__ uint32_t s_tmp[4];
+    __shared__ uint64_t s_tmp64[4];
+    uint64_t run = 0;
+    uint32_t trun[5] = {0, 0, 0, 0, 0};
+    for (uint32_t f0 = 0; f0 < F; f0 += CH_WG) {
+        const uint32_t f = f0 + threadIdx.x;
+        const bool live = f < F;
+        uint64_t size = 0;
+        uint32_t tt[5] = {0, 0, 0, 0, 0};
+        if (live) {
+            size = 48 + 8ull * D + 16 + 16ull * fo[f].ncorr;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const CompInst& C = ci[c * F + f];
+                size += 8 + C.size;
+                if (C.kind == KIND_TAGR) tt[c] = (uint32_t)((C.n + TAG_TILE - 1) / TAG_TILE);
             }
-            p += C.size;
         }
-        p += 16 + 16 * fo[f].ncorr;
-        out_size[f] = p - off;
-        off = p;
+        uint64_t tot;
+        const uint64_t off = run + block_excl_scan_u64(size, s_tmp64, &tot);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            uint32_t ctot;
+            const uint32_t pre = block_excl_scan(tt[c], s_tmp, &ctot);
+            if (live) tag_tiles[c * (F + 1) + f] = trun[c] + pre;
+            trun[c] += ctot;
+        }
+        if (live) {
+            out_off[f] = off;
+            out_size[f] = size;
+            uint64_t p = off + 48 + 8ull * D;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                CompInst& C = ci[c * F + f];
+                p += 8;
+                C.off = p;
+                const uint32_t i = c * F + f;
+                if (C.kind == KIND_TAGR) {
+                    sec[2 * i + 1].off = p + 4 + 4 + 8 + 1 + 8;
+                    C.payload_off = sec[2 * i + 1].off + sec[2 * i + 1].size + 8;
+                } else if (C.kind == KIND_FIXP) {
+                    C.payload_off = p + 16 + 17ull * C.dims;
+                } else {
+                    sec[2 * i].off = p + 16 + 8;
+                }
+                p += C.size;
+            }
+        }
+        run += tot;
     }
-    *total = off;
+    if (threadIdx.x == 0) {
+        *total = run;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) tag_tiles[c * (F + 1) + F] = trun[c];
+    }
 }
 
 // ------------------------------------------------------------------------

@@ -274,6 +274,8 @@ struct chain_engine {
     bool autotune = false;
     std::map<uint32_t, chain::Tuning> tuned;   // size bucket -> setting
     void* scratch = nullptr;   // this device's no-allocation scratch (device address)
+    uint64_t last_need_c = 0;        // arena bytes the last compression carved
+    uint64_t capacity_retries = 0;   // compressions run again with worst-case bounds
 };
 
 namespace chain {
@@ -282,6 +284,69 @@ static void ensure_arena(chain_engine* e, Arena& a, Arena& other, size_t bytes) 
     if (e->shared_budget && e->budget && bytes > a.cap && other.cap && bytes + other.cap > e->budget)
         other.release();
     a.ensure(bytes, e->budget && e->shared_budget ? e->budget - other.cap : e->budget);
+}
+
+// Grow the compression arena to `bytes` in the middle of a call, keeping its
+// first `keep` bytes (the call's phase A, complete at the planning sync);
+// true when the arena moved.
+static bool ensure_arena_keep(chain_engine* e, size_t bytes, size_t keep) {
+    Arena& a = e->arena;
+    if (bytes <= a.cap) return false;
+    if (e->shared_budget && e->budget && e->arena_d.cap && bytes + e->arena_d.cap > e->budget) e->arena_d.release();
+    const size_t budget = e->budget && e->shared_budget ? e->budget - e->arena_d.cap : e->budget;
+    size_t want = bytes + bytes / 8 + (32u << 20);
+    if (budget && want > budget) want = bytes;
+    if (budget && want > budget)
+        throw Error(CHAIN_ENOMEM, "device memory budget exceeded: need " + std::to_string(bytes) + " bytes");
+    char* nb = nullptr;
+    HIPCHK(hipMalloc(&nb, want));
+    if (keep && a.base) {
+        hipError_t rc = hipMemcpyAsync(nb, a.base, keep, hipMemcpyDeviceToDevice, e->stream);
+        if (rc == hipSuccess) rc = hipStreamSynchronize(e->stream);
+        if (rc != hipSuccess) {
+            (void)hipFree(nb);
+            HIPCHK(rc);
+        }
+    }
+    a.release();
+    a.base = nb;
+    a.cap = want;
+    ++a.allocs;
+    return true;
+}
+
+// Correction entries per tile in a first compression (a tile with more runs
+// the call again with room for every value).
+#define CORR_TILE_CAP 256u
+
+// Output bytes of one frame's LCH1 container, bounded by Fixed packing of
+// every component at widths the plan fixes: idrel < n, rel <= relmax, keep
+// records are block ids (<= max_bid), diff records zig-zag corrections
+// within the frame's wrap range (<= max_bid + 1), choice2 one bit; diff and
+// keep records together number at most n. The chooser keeps a rANS-coded or
+// Tagged candidate only when its estimate is smaller, with slack for the
+// estimate's error; the layout kernel checks the bound and the call runs
+// again with the worst case (8 bytes per value) if it is exceeded.
+// full: that worst case. Corrections: 16 bytes each, at most corr_cap per tile.
+static uint64_t out_bound_frame(uint64_t n, uint32_t D, uint32_t max_bid_bits, uint64_t relmax, uint32_t tiles,
+                                uint32_t corr_cap, bool full) {
+    const uint64_t hdr = 48 + 8ull * D + 16 + 16 * std::min<uint64_t>(n * D, (uint64_t)tiles * corr_cap);
+    uint64_t comp = 0;
+    if (full) {
+        for (int c = 0; c < 5; ++c) {
+            const uint32_t dc = (c == 0 || c == 3) ? 1u : D;
+            comp += 8 + 16 + 17ull * dc + 8ull * n * dc;
+        }
+        return hdr + comp;
+    }
+    auto bytes = [](uint64_t count, uint64_t bits) { return (count * std::max<uint64_t>(bits, 1) + 7) / 8; };
+    const uint64_t fix_hdr1 = 8 + 16 + 17, fix_hdrD = 8 + 16 + 17ull * D;
+    comp += fix_hdr1 + bytes(n, bitlen64(n ? n - 1 : 0));                        // idrel
+    comp += 2 * fix_hdrD + D * (bytes(n, max_bid_bits + 1ull) + 1);              // diff + keep
+    comp += fix_hdr1 + bytes(n, 1);                                              // choice2
+    comp += fix_hdrD + D * bytes(n, bitlen64(relmax));                           // rel
+    comp += comp / 32 + 5 * 4096;
+    return hdr + comp;
 }
 
 // ------------------------------------------------------------------------
@@ -487,7 +552,7 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                           const uint8_t* choice2, const void* rel, const uint64_t* corr_idx, const double* corr_val,
                           const uint32_t* tb, const uint32_t* tb_host, uint32_t ntiles,
                           uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const Tuning& rc, uint64_t stride,
-                          uint32_t* err, X& x) {
+                          uint32_t* err, uint64_t out_cap, uint32_t* cap_flag, X& x) {
     const uint32_t ninst = 5 * F, nsec = 10 * F;
     hipLaunchKernelGGL(k_comp_init, dim3(ninst), dim3(CH_WG), 0, st, fdev, fo, F, D, x.ci, x.vhist, x.thist);
     hipLaunchKernelGGL(k_comp_tiles, dim3(5), dim3(CH_WG), 0, st, x.ci, F, x.ctb);
@@ -560,14 +625,15 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
     hipLaunchKernelGGL(k_section_sizes, dim3(nsec), dim3(CH_WG), 0, st, x.sec, x.chunk_len, x.chunk_off);
     hipLaunchKernelGGL(k_choose, dim3(cdiv(ninst, 64)), dim3(64), 0, st, x.ci, x.sec, ninst);
     hipLaunchKernelGGL(k_layout, dim3(1), dim3(CH_WG), 0, st, x.ci, x.sec, fo, F, D, x.out_off, x.out_size, x.total,
-                       x.tag_tiles);
+                       x.tag_tiles, out_cap, cap_flag);
     hipLaunchKernelGGL(k_write_frames, dim3(F), dim3(CH_WG), 0, st, fdev, fo, D, x.out_off, x.out_size, corr_idx,
-                       corr_val, x.out);
+                       corr_val, x.total, out_cap, x.out);
     hipLaunchKernelGGL(k_write_components, dim3(ninst), dim3(CH_WG), 0, st, x.ci, x.sec, F, x.tbytes, x.chunk_len,
-                       x.out);
+                       x.total, out_cap, x.out);
     if (chunk_bound)
         hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunk_bound), dim3(CH_WG), 0, st, x.ci, x.sec, nsec,
-                           x.comp_chunk, x.scratch, stride, x.chunk_len, x.chunk_pos, x.chunk_off, x.out);
+                           x.comp_chunk, x.scratch, stride, x.chunk_len, x.chunk_pos, x.chunk_off, x.total, out_cap,
+                           x.out);
     tm.end();
     // every component's look-back words and ticket, cleared by two launches
     fill64(st, x.status1k, 5ull * (ntiles1k + 1), 0);
@@ -577,13 +643,13 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
         constexpr int DC = decltype(dd)::value;
         et.pack[c] = tm.begin("pack");
         hipLaunchKernelGGL((k_pack_fixed<V, DC>), dim3(ntiles), dim3(CH_WG), 0, st, (const V*)v, c, F, tb, x.ci,
-                           x.out);
+                           x.total, out_cap, x.out);
         hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(std::min<uint32_t>(ntiles1k, PACK_TAG_WGS)), dim3(CH_WG), 0, st,
                            (const V*)v, c, F,
                            x.tag_tiles, x.ci, x.status1k + (uint64_t)c * (ntiles1k + 1), x.tickets + c, err, x.head,
-                           x.tail, x.head_pos, x.out);
+                           x.tail, x.head_pos, x.total, out_cap, x.out);
         hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG)), dim3(CH_WG), 0, st, x.head, x.tail, x.head_pos,
-                           x.tag_tiles + c * (F + 1) + F, x.out);
+                           x.tag_tiles + c * (F + 1) + F, x.total, out_cap, x.out);
         tm.end();
     });
 }
@@ -648,8 +714,13 @@ static void run_graph(chain_engine* e, const std::string& key, Fn&& issue) {
     HIPCHK(hipGraphLaunch(G.exec, st));
 }
 
-static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
-                                uint32_t D, bool on_device, int format, CompressResult& res) {
+// One compression of a batch. full = false sizes the correction lists
+// (CORR_TILE_CAP values per tile) and the output (the Fixed-packing bound of
+// the plan, chain_out_bound) for the data; when either overflows, nothing
+// beyond the buffers is written, the call returns false and is run again
+// with full = true (every value corrected, worst-case widths).
+static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
+                               uint32_t D, bool on_device, int format, bool full, CompressResult& res) {
     require(F >= 1, CHAIN_EINVAL, "empty batch");
     require(D >= 1 && D <= CH_MAXD, CHAIN_EINVAL, "the GPU engine supports 1 to 4 dimensions");
     require(layout == CHAIN_AOS || layout == CHAIN_SOA, CHAIN_EINVAL, "layout must be AoS or SoA");
@@ -682,6 +753,9 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     const uint64_t T = pt_off[F];
     require(T < (1ull << 32), CHAIN_EINVAL, "a GPU batch holds fewer than 2^32 particles");
     const uint32_t ntiles = tile_begin[F];
+    // correction entries per tile: a few (corrections are rare), or every value
+    const uint32_t corr_cap = full ? CH_TILE * D : std::min<uint32_t>(CORR_TILE_CAP, CH_TILE * D);
+    const uint64_t corr_slots = (uint64_t)ntiles * corr_cap;
 
     // ---- phase A: bounding boxes
     bool contiguous = on_device;
@@ -762,12 +836,12 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         b.vb = c.take<uint32_t>(T);
         b.rel = c.take<char>(T * D * rel_bytes);
         b.idrel = c.take<uint32_t>(T);
-        b.corr_tmp_idx = c.take<uint64_t>((uint64_t)ntiles * CH_TILE * D);
-        b.corr_tmp_val = c.take<double>((uint64_t)ntiles * CH_TILE * D);
+        b.corr_tmp_idx = c.take<uint64_t>(corr_slots);
+        b.corr_tmp_val = c.take<double>(corr_slots);
         b.tile_ncorr = c.take<uint32_t>(ntiles);
         b.tile_corr_off = c.take<uint32_t>(ntiles);
-        b.corr_idx = c.take<uint64_t>(T * D);
-        b.corr_val = c.take<double>(T * D);
+        b.corr_idx = c.take<uint64_t>(std::min<uint64_t>(T * D, corr_slots));
+        b.corr_val = c.take<double>(std::min<uint64_t>(T * D, corr_slots));
         b.counts = c.take<uint32_t>(256 * sort_tiles);
         b.offsets = c.take<uint32_t>(256 * sort_tiles);
         // at most key_bytes radix passes per key word
@@ -791,11 +865,6 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         b.item_total = c.take<uint32_t>(1);
         return b;
     };
-    Carver cb0;
-    (void)carve_b(cb0, 8, 4, D);
-    Carver cb1;
-    (void)carve_b(cb1, 8, 4, 1);
-    const size_t size_b = std::max(cb0.off, cb1.off);
 
     // LCH1 entropy stage buffers, sized from the particle counts alone
     const Tuning rc = e->tuning;
@@ -803,22 +872,19 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     // that the encoder's aligned block stores stay aligned
     const uint64_t stride = CHUNK_STATE_BYTES + align256(3ull * rc.chunk_symbols);
     uint64_t comp_chunk_bound[5] = {0, 0, 0, 0, 0};
-    uint64_t out_bound = 0;
     std::vector<uint32_t> tile1k(F + 1, 0);
     for (uint32_t f = 0; f < F; ++f) {
         const uint64_t n = plan[f].n;
         for (int c = 0; c < 5; ++c) {
             const uint32_t dc = (c == 0 || c == 3) ? 1u : D;
             comp_chunk_bound[c] += cdiv(n * dc, rc.chunk_symbols) + cdiv(n, rc.chunk_symbols);
-            out_bound += 8 + 16 + 17ull * dc + 8ull * n * dc;
         }
-        out_bound += 48 + 8ull * D + 16 + 16ull * n * D;
         tile1k[f + 1] = tile1k[f] + (uint32_t)cdiv(n, TAG_TILE);
     }
     const uint64_t chunk_bound = comp_chunk_bound[0] + comp_chunk_bound[1] + comp_chunk_bound[2] +
                                  comp_chunk_bound[3] + comp_chunk_bound[4];
     const uint32_t ntiles1k = tile1k[F];
-    auto carve_c = [&](Carver& c) {
+    auto carve_c = [&](Carver& c, uint64_t out_bound) {
         struct {
             CompInst* ci;
             Section* sec;
@@ -870,13 +936,8 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         x.out = c.take<uint8_t>(out_bound);
         return x;
     };
-    size_t size_c = 0;
-    if (format == CHAIN_FMT_LCH1) {
-        Carver cc0;
-        (void)carve_c(cc0);
-        size_c = cc0.off;
-    }
-    ensure_arena(e, e->arena, e->arena_d, size_a + size_b + size_c);
+    // phase A now; phases B and C once the plan has fixed the widths
+    ensure_arena(e, e->arena, e->arena_d, size_a);
 
     Carver cA;
     cA.base = e->arena.base;
@@ -989,6 +1050,29 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     const bool item_link = nwords == 1 && e->tuning.l1_tiles != 0;
     const bool hist_q = !item_link && word_passes[0] >= 1;
 
+    // the plan fixes key, offset and value widths: phases B and C are sized
+    // now, and the arena grows (keeping phase A) when they do not fit
+    Carver cbx;
+    (void)carve_b(cbx, kbytes, rbytes, nwords);
+    const size_t size_b = cbx.off;
+    uint64_t out_bound = 0;
+    for (uint32_t f = 0; f < F; ++f)
+        out_bound += out_bound_frame(plan[f].n, D, fdev[f].max_bid_bits, plan[f].relmax,
+                                     tile_begin[f + 1] - tile_begin[f], corr_cap, full);
+    size_t size_c = 0;
+    if (format == CHAIN_FMT_LCH1) {
+        Carver cc0;
+        (void)carve_c(cc0, out_bound);
+        size_c = cc0.off;
+    }
+    e->last_need_c = size_a + size_b + size_c;
+    if (ensure_arena_keep(e, size_a + size_b + size_c, size_a)) {
+        Carver cA2;
+        cA2.base = e->arena.base;
+        A = carve_a(cA2);
+        dpts = contiguous ? frames[0].points : A.input;
+    }
+
     Carver cB;
     cB.base = e->arena.base + size_a;
     auto B = carve_b(cB, kbytes, rbytes, nwords);
@@ -998,7 +1082,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     EntropyTimes et{};
     Carver cC;
     cC.base = e->arena.base + size_a + size_b;
-    auto X = carve_c(cC);   // entropy-stage buffers (LCH1 only)
+    auto X = carve_c(cC, out_bound);   // entropy-stage buffers (LCH1 only)
     // every launch after the planning sync: issued directly, or captured
     // once into a HIP graph and replayed for batches of the same launch shape
     uint64_t* const lb_expl = B.status + (uint64_t)kbytes * nwords * lb_sort;
@@ -1023,12 +1107,14 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
                     if (layout == CHAIN_SOA)
                         hipLaunchKernelGGL((k_quantize<T_, DD, true, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
                                            (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T,
+                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, corr_cap, A.err + 1,
+                                           nwords, T,
                                            hist_q ? B.counts : nullptr, (uint32_t)sort_tiles);
                     else
                         hipLaunchKernelGGL((k_quantize<T_, DD, false, KeyT, RelT>), dim3(ntiles), dim3(CH_WG), 0, st,
                                            (const T_*)dpts, A.fdev, A.tb, F, (KeyT*)B.keys_in, (RelT*)B.rel,
-                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, nwords, T,
+                                           B.corr_tmp_idx, B.corr_tmp_val, B.tile_ncorr, corr_cap, A.err + 1,
+                                           nwords, T,
                                            hist_q ? B.counts : nullptr, (uint32_t)sort_tiles);
                 });
             });
@@ -1148,7 +1234,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
             hipLaunchKernelGGL((k_corr_gather<DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.corr_tmp_idx, B.corr_tmp_val,
-                               B.tile_ncorr, B.tile_corr_off, B.corr_idx, B.corr_val);
+                               B.tile_ncorr, B.tile_corr_off, corr_cap, B.corr_idx, B.corr_val);
         });
         hipLaunchKernelGGL(k_frame_corr, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.tb, B.tile_corr_off,
                            &B.totals[2]);
@@ -1156,13 +1242,13 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         if (format == CHAIN_FMT_LCH1)
             entropy_stage(st, tm, et, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
                           B.corr_val, A.tb, tile_begin.data(), ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err,
-                          X);
+                          out_bound, A.err + 1, X);
     };
     if (e->use_graphs) {
         GraphKey key;
         key.add(format).add(dtype).add(layout).add(D).add(F).add(T).add(nwords).add(key64).add(rel8);
         key.add(dpts).add(e->arena.base).add(ntiles).add(ntiles1k).add(rc.lanes_max).add(rc.chunk_symbols);
-        key.add(item_link);
+        key.add(item_link).add(corr_cap).add(out_bound);
         for (uint32_t w = 0; w < CH_MAXD; ++w) key.add(word_passes[w]);
         for (int c = 0; c < 5; ++c) key.add(comp_chunk_bound[c]);
         for (uint32_t f = 0; f <= F; ++f) key.add(pt_off[f]);
@@ -1257,6 +1343,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
             tm.set_bytes(et.encode, enc_all);
             tm.set_bytes(et.assemble, 2 * copied);
         }
+        if (errw[1]) {   // a correction list or the output overran its bound
+            require(!full, CHAIN_EDEVICE, "capacity exceeded at the worst-case bound");
+            return false;
+        }
         require(errw[0] == 0, CHAIN_EDEVICE, "device error word set in the entropy stage");
         require(total <= out_bound, CHAIN_EDEVICE, "container exceeds its bound");
         res.stats.resize(F);
@@ -1273,11 +1363,11 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         res.sizes = sizes;
         if (res.keep_on_device) {
             res.d_out = X.out;
-            return;
+            return true;
         }
         if (res.host_out) {
             device_to_bufs(st, X.out, offs.data(), sizes.data(), F, res.host_out);
-            return;
+            return true;
         }
         std::vector<uint8_t> all(total);
         if (total) HIPCHK(hipMemcpyAsync(all.data(), X.out, total, hipMemcpyDeviceToHost, st));
@@ -1285,7 +1375,7 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         res.containers.resize(F);
         for (uint32_t f = 0; f < F; ++f)
             res.containers[f].assign(all.begin() + offs[f], all.begin() + offs[f] + sizes[f]);
-        return;
+        return true;
     }
 
     // ---- results to the host
@@ -1298,6 +1388,10 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
     sg.d2h(st, &n_items, B.item_total, 4);
     sg.finish(st);
     set_item_bytes(n_items);
+    if (errw[1]) {   // a tile's corrections overran the list bound
+        require(!full, CHAIN_EDEVICE, "capacity exceeded at the worst-case bound");
+        return false;
+    }
     require(errw[0] == 0, CHAIN_EDEVICE, "device error word set (look-back wait limit or invalid block range)");
 
     const uint64_t R = totals[0], ND = totals[1], NC = totals[2];
@@ -1346,6 +1440,20 @@ static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint
         chain_stats& s = res.stats[f];
         write_lcp3(v, res.containers[f], s);
     }
+    return true;
+}
+
+static void compress_batch_impl(chain_engine* e, const chain_frame* frames, uint32_t F, int dtype, int layout,
+                                uint32_t D, bool on_device, int format, CompressResult& res) {
+    const bool keep = res.keep_on_device;
+    chain_buf* host_out = res.host_out;
+    if (compress_batch_try(e, frames, F, dtype, layout, D, on_device, format, false, res)) return;
+    res = CompressResult{};
+    res.keep_on_device = keep;
+    res.host_out = host_out;
+    ++e->capacity_retries;
+    require(compress_batch_try(e, frames, F, dtype, layout, D, on_device, format, true, res), CHAIN_EDEVICE,
+            "capacity exceeded at the worst-case bound");
 }
 
 

@@ -1168,7 +1168,8 @@ __global__ __launch_bounds__(CH_WG) void k_layout(CompInst* __restrict__ ci, Sec
                                                   const FrameOut* __restrict__ fo, uint32_t F, uint32_t D,
                                                   uint64_t* __restrict__ out_off, uint64_t* __restrict__ out_size,
                                                   uint64_t* __restrict__ total,
-                                                  uint32_t* __restrict__ tag_tiles /* [5][F+1] */) {
+                                                  uint32_t* __restrict__ tag_tiles /* [5][F+1] */, uint64_t out_cap,
+                                                  uint32_t* __restrict__ cap_flag) {
     __shared__ uint32_t s_tmp[4];
     __shared__ uint64_t s_tmp64[4];
     uint64_t run = 0;
@@ -1221,6 +1222,7 @@ __global__ __launch_bounds__(CH_WG) void k_layout(CompInst* __restrict__ ci, Sec
     }
     if (threadIdx.x == 0) {
         *total = run;
+        if (run > out_cap) atomicOr(cap_flag, 2u);   // the writers skip; the call runs again with the worst-case bound
 #pragma unroll
         for (int c = 0; c < 5; ++c) tag_tiles[c * (F + 1) + F] = trun[c];
     }
@@ -1255,7 +1257,9 @@ __global__ __launch_bounds__(CH_WG) void k_write_components(const CompInst* __re
                                                             const Section* __restrict__ sec, uint32_t F,
                                                             const uint8_t* __restrict__ tbytes,
                                                             const uint32_t* __restrict__ chunk_len,
+                                                            const uint64_t* __restrict__ out_total, uint64_t out_cap,
                                                             uint8_t* __restrict__ out) {
+    if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
     const uint32_t i = blockIdx.x;
     const CompInst& C = ci[i];
     const uint32_t tid = threadIdx.x;
@@ -1291,7 +1295,9 @@ __global__ __launch_bounds__(CH_WG) void k_write_frames(const FrameDev* __restri
                                                         const uint64_t* __restrict__ out_size,
                                                         const uint64_t* __restrict__ corr_idx,
                                                         const double* __restrict__ corr_val,
+                                                        const uint64_t* __restrict__ out_total, uint64_t out_cap,
                                                         uint8_t* __restrict__ out) {
+    if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
     const uint32_t f = blockIdx.x;
     const FrameDev& F = fr[f];
     const FrameOut& O = fo[f];
@@ -1324,7 +1330,9 @@ __global__ __launch_bounds__(CH_WG) void k_copy_chunks(const CompInst* __restric
                                                        const uint32_t* __restrict__ chunk_len,
                                                        const uint32_t* __restrict__ chunk_pos,
                                                        const uint32_t* __restrict__ chunk_off,
+                                                       const uint64_t* __restrict__ out_total, uint64_t out_cap,
                                                        uint8_t* __restrict__ out) {
+    if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
     const uint32_t chunk = blockIdx.x;
     if (chunk >= comp_chunk[5]) return;
     const uint32_t si = find_section(sec, 0, nsec, chunk);
@@ -1365,7 +1373,10 @@ __global__ __launch_bounds__(CH_WG) void k_copy_chunks(const CompInst* __restric
 template <typename ValT, int DIMS>
 __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ vals, int c, uint32_t F,
                                                       const uint32_t* __restrict__ tile_begin,
-                                                      const CompInst* __restrict__ ci, uint8_t* __restrict__ out) {
+                                                      const CompInst* __restrict__ ci,
+                                                      const uint64_t* __restrict__ out_total, uint64_t out_cap,
+                                                      uint8_t* __restrict__ out) {
+    if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
     const uint32_t tile = blockIdx.x;
     const uint32_t f = upper_find(tile_begin, F, tile);
     const CompInst& C = ci[c * F + f];
@@ -1478,7 +1489,10 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
                                                        const CompInst* __restrict__ ci, uint64_t* status,
                                                        uint32_t* ticket, uint32_t* err,
                                                        uint8_t* __restrict__ head, uint8_t* __restrict__ tail,
-                                                       uint64_t* __restrict__ head_pos, uint8_t* __restrict__ out) {
+                                                       uint64_t* __restrict__ head_pos,
+                                                       const uint64_t* __restrict__ out_total, uint64_t out_cap,
+                                                       uint8_t* __restrict__ out) {
+    if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_words[(TAG_TILE * 4 * 63) / 32 + 4];
@@ -1572,7 +1586,9 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
 
 __global__ void k_tag_seams(const uint8_t* __restrict__ head, const uint8_t* __restrict__ tail,
                             const uint64_t* __restrict__ head_pos, const uint32_t* __restrict__ ntiles_dev,
+                            const uint64_t* __restrict__ out_total, uint64_t out_cap,
                             uint8_t* __restrict__ out) {
+    if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= *ntiles_dev || head_pos[t] == ~0ull) return;
     out[head_pos[t]] = head[t] | tail[t - 1];

@@ -464,15 +464,18 @@ __device__ inline uint64_t quantize_items(const T* __restrict__ pts, const Frame
 }
 
 // Exact corrections of a tile, in element order (j, thread, d), into the
-// tile's slot of corr_idx / corr_val; rare, so the ordered write path runs
-// only when the tile has any. Called by the whole workgroup.
+// tile's slot of corr_idx / corr_val (cap entries per tile); rare, so the
+// ordered write path runs only when the tile has any. A tile with more than
+// cap corrections keeps its first cap, records cap as its count and raises
+// bit 0 of *cap_flag: the call is then run again with cap = CH_TILE * D.
+// Called by the whole workgroup.
 template <typename T, int D, bool SOA>
 __device__ inline void tile_corrections(const T* __restrict__ pts, const FrameDev& F, uint64_t start, uint32_t tile,
                                         uint64_t cmask, uint64_t* __restrict__ corr_idx,
                                         double* __restrict__ corr_val, uint32_t* __restrict__ tile_ncorr,
-                                        uint32_t* s_tmp) {
+                                        uint32_t cap, uint32_t* __restrict__ cap_flag, uint32_t* s_tmp) {
     uint32_t run = 0;
-    const uint64_t cbase = (uint64_t)tile * CH_TILE * D;
+    const uint64_t cbase = (uint64_t)tile * cap;
     if (__syncthreads_or(cmask != 0)) {
         for (int j = 0; j < CH_ITEMS; ++j) {
             const uint32_t bits = (uint32_t)(cmask >> (j * D)) & ((1u << D) - 1u);
@@ -482,14 +485,19 @@ __device__ inline void tile_corrections(const T* __restrict__ pts, const FrameDe
 #pragma unroll
             for (int d = 0; d < D; ++d)
                 if (bits & (1u << d)) {
-                    corr_idx[cbase + pos] = i * D + d;
-                    corr_val[cbase + pos] = load_coord<T, D, SOA>(pts, F, i, d);
+                    if (pos < cap) {
+                        corr_idx[cbase + pos] = i * D + d;
+                        corr_val[cbase + pos] = load_coord<T, D, SOA>(pts, F, i, d);
+                    }
                     ++pos;
                 }
             run += tot;
         }
     }
-    if (threadIdx.x == 0) tile_ncorr[tile] = run;
+    if (threadIdx.x == 0) {
+        if (run > cap) atomicOr(cap_flag, 1u);
+        tile_ncorr[tile] = run < cap ? run : cap;
+    }
 }
 
 template <typename T, int D, bool SOA, typename KeyT, typename RelT>
@@ -497,7 +505,8 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
                                                     const uint32_t* __restrict__ tile_begin, uint32_t nframes,
                                                     KeyT* __restrict__ key_out, RelT* __restrict__ rel_out,
                                                     uint64_t* __restrict__ corr_idx, double* __restrict__ corr_val,
-                                                    uint32_t* __restrict__ tile_ncorr, uint32_t nwords,
+                                                    uint32_t* __restrict__ tile_ncorr, uint32_t corr_cap,
+                                                    uint32_t* __restrict__ cap_flag, uint32_t nwords,
                                                     uint64_t total_pts, uint32_t* __restrict__ counts,
                                                     uint32_t sort_ntiles) {
     const uint32_t tile = blockIdx.x;
@@ -527,7 +536,7 @@ __global__ __launch_bounds__(CH_WG) void k_quantize(const T* __restrict__ pts, c
             if (v) atomicAdd(&counts[(uint64_t)threadIdx.x * sort_ntiles + st0 + s], v);
         }
     }
-    tile_corrections<T, D, SOA>(pts, F, start, tile, cmask, corr_idx, corr_val, tile_ncorr, s_tmp);
+    tile_corrections<T, D, SOA>(pts, F, start, tile, cmask, corr_idx, corr_val, tile_ncorr, corr_cap, cap_flag, s_tmp);
 }
 
 // gather per-tile correction lists into one ordered list
@@ -535,11 +544,11 @@ template <int D>
 __global__ __launch_bounds__(CH_WG) void k_corr_gather(const uint64_t* __restrict__ corr_idx,
                                                        const double* __restrict__ corr_val,
                                                        const uint32_t* __restrict__ tile_ncorr,
-                                                       const uint32_t* __restrict__ tile_corr_off,
+                                                       const uint32_t* __restrict__ tile_corr_off, uint32_t cap,
                                                        uint64_t* __restrict__ out_idx, double* __restrict__ out_val) {
     const uint32_t tile = blockIdx.x;
     const uint32_t c = tile_ncorr[tile];
-    const uint64_t src = (uint64_t)tile * CH_TILE * D;
+    const uint64_t src = (uint64_t)tile * cap;
     const uint64_t dst = tile_corr_off[tile];
     for (uint32_t k = threadIdx.x; k < c; k += CH_WG) {
         out_idx[dst + k] = corr_idx[src + k];

@@ -48,14 +48,40 @@ struct ServeDone {
     uint32_t batch = 0;
 };
 
-// Upper bounds of the arena bytes of one frame (chain_engine.hip: carve_a/b/c
-// of compress_batch_impl, carve of decompress_batch_impl).
-inline uint64_t serve_estimate_compress(uint64_t n, uint32_t D, uint32_t es, const Tuning& rc) {
-    const uint64_t per_point = (uint64_t)D * es + 76ull * D + 30 + 3ull * (3 * D + 7) + 16 + 40ull * D;
-    const uint64_t stride = CHUNK_STATE_BYTES + 3ull * rc.chunk_symbols;
-    const uint64_t per_frame = 10 * stride + 5ull * RANS_VALUE_LIMIT * 4 + 10ull * RANS_VALUE_LIMIT * 8 +
-                               10ull * TABLE_BYTES_MAX + (1u << 20);
-    return n * per_point + per_frame;
+// Upper bound of the arena bytes of one frame's compression in a first run
+// (compress_batch_try in chain_engine.hip, buffer by buffer): the host
+// frame's copy; keys as 8-byte words (D words, or one word and the
+// (tile, key) items), two sort buffers of keys and indices, rel (one byte
+// when p <= 256, else four), idrel, block records, choice2, diff, keep and
+// the radix counters; CORR_TILE_CAP corrections per tile twice; the rANS
+// chunk scratch of the 3D + 7 sections' symbols; the output bound of
+// out_bound_frame at the widest values a request allows (idrel 32 bits,
+// diff and keep 64, rel bitlen(p - 1)); the per-frame tables (histograms,
+// rANS tables, statistics slots) and the 256-byte rounding of every buffer.
+// A rare second run with worst-case bounds (a tile with more corrections,
+// an output over its bound) may need more; in a worker whose arena it
+// exceeds it fails with CHAIN_ENOMEM.
+inline uint64_t serve_estimate_compress(uint64_t n, uint32_t D, uint32_t es, int64_t p, uint32_t flags,
+                                        const Tuning& rc) {
+    const uint64_t pe = (flags & CHAIN_NO_BL) ? 1 : (uint64_t)p;
+    const uint64_t rbytes = pe <= 256 ? 1 : 4;
+    const uint64_t cs = rc.chunk_symbols;
+    const uint64_t stride = CHUNK_STATE_BYTES + ((3 * cs + 255) / 256) * 256;
+    const uint64_t tiles = (n + CH_TILE - 1) / CH_TILE;
+    uint64_t b = n * D * es;
+    b += n * (std::max<uint64_t>(24, 8ull * D) + 24 + D * rbytes + 4 + 24ull * D + 1) + n / 2 + n / 16;
+    b += 4 * tiles * CORR_TILE_CAP * 8;
+    const uint64_t chunks = ((3ull * D + 7) * n + cs - 1) / cs + 10;
+    b += chunks * (stride + 12);
+    const uint64_t relb = std::max<uint64_t>(1, bitlen64(pe - 1));
+    uint64_t out = 2 * (8 + 16 + 17) + 3 * (8 + 16 + 17ull * D) + 4 * n + D * (8 * n + 1) + (n + 7) / 8 +
+                   D * ((n * relb + 7) / 8);
+    out += out / 32 + 5 * 4096;
+    out += 48 + 8ull * D + 16 + 16 * std::min<uint64_t>(n * D, tiles * CORR_TILE_CAP);
+    b += out;
+    b += 5ull * RANS_VALUE_LIMIT * 4 + 5ull * RANS_TAG_LIMIT * 4 + 10ull * RANS_VALUE_LIMIT * 8 +
+         10ull * TABLE_BYTES_MAX + 5ull * (CS_GRID + 1) * sizeof(CompSlot) + (1u << 20);
+    return b;
 }
 
 // Decode carve per point, worst case: idrel, root, list_a, list_b (4 B each),
@@ -302,7 +328,8 @@ int chain_serve_submit(chain_server* s, const chain_request* r, uint64_t* ticket
         if (!(r->eb > 0.0) || !std::isfinite(2.0 * r->eb)) return fail(CHAIN_EINVAL, "eb must be positive with finite 2*eb");
         if (r->p <= 0) return fail(CHAIN_EINVAL, "p must be positive");
         it.dims = r->dims;
-        it.est = chain::serve_estimate_compress(r->n, r->dims, r->dtype == CHAIN_F32 ? 4 : 8, s->engines[0]->tuning);
+        it.est = chain::serve_estimate_compress(r->n, r->dims, r->dtype == CHAIN_F32 ? 4 : 8, r->p, r->flags,
+                                                s->engines[0]->tuning);
     } else if (r->kind == CHAIN_REQ_DECOMPRESS) {
         if (!r->bytes || r->len < 24) return fail(CHAIN_EINVAL, "decompress request without a stream");
         if (r->out_dtype != CHAIN_F32 && r->out_dtype != CHAIN_F64) return fail(CHAIN_EINVAL, "dtype must be float32 or float64");
@@ -353,6 +380,22 @@ int chain_serve_poll(chain_server* s, uint64_t ticket, int wait_ms, chain_result
 }
 
 double chain_server_clock(chain_server* s) { return s ? s->now() : 0.0; }
+
+uint64_t chain_serve_estimate_compress(uint64_t n, uint32_t dims, int dtype, int64_t p, uint32_t flags,
+                                       uint32_t chunk_symbols) {
+    if (n == 0 || dims < 1 || dims > CH_MAXD || p <= 0 || chunk_symbols == 0) return 0;
+    chain::Tuning rc;
+    rc.chunk_symbols = chunk_symbols;
+    return chain::serve_estimate_compress(n, dims, dtype == CHAIN_F64 ? 8 : 4, p, flags, rc);
+}
+
+int chain_engine_memory_stats(chain_engine* e, uint64_t* compress_need, uint64_t* capacity_retries) {
+    if (!e || !compress_need || !capacity_retries) return CHAIN_EINVAL;
+    std::lock_guard<std::mutex> lock(e->mu);
+    *compress_need = e->last_need_c;
+    *capacity_retries = e->capacity_retries;
+    return CHAIN_OK;
+}
 
 int chain_server_stats(chain_server* s, uint64_t* arena_allocations) {
     if (!s || !arena_allocations) return CHAIN_EINVAL;

@@ -57,7 +57,7 @@ EXPORTS = ["chain_engine_create", "chain_engine_destroy", "chain_last_error", "c
            "chain_engine_set_graphs", "chain_engine_graph_stats", "chain_engine_autotune",
            "chain_engine_set_autotune", "chain_compress_search", "chain_server_stats", "chain_engine_set_tuning",
            "chain_engine_get_tuning",
-           "chain_decompress_batch_noalloc"]
+           "chain_decompress_batch_noalloc", "chain_serve_estimate_compress", "chain_engine_memory_stats"]
 
 CHAIN_PENDING = 5
 REQ_COMPRESS, REQ_DECOMPRESS = 0, 1
@@ -146,8 +146,18 @@ def load_library():
     L.chain_server_clock.argtypes = [C.c_void_p]
     L.chain_server_clock.restype = C.c_double
     L.chain_server_stats.argtypes = [C.c_void_p, P(C.c_uint64)]
+    L.chain_serve_estimate_compress.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int64, C.c_uint32, C.c_uint32]
+    L.chain_serve_estimate_compress.restype = C.c_uint64
+    L.chain_engine_memory_stats.argtypes = [C.c_void_p, P(C.c_uint64), P(C.c_uint64)]
     _LIB = L
     return L
+
+
+def serve_estimate_compress(n, dims, dtype, p, flags=0, chunk_symbols=65536) -> int:
+    """Device bytes the server reserves for one compress request (an upper
+    bound of the first run's arena carve)."""
+    return int(load_library().chain_serve_estimate_compress(int(n), int(dims), _dtype_code(dtype), int(p),
+                                                            int(flags), int(chunk_symbols)))
 
 
 def _take(buf: _Buf) -> bytes:
@@ -290,6 +300,12 @@ class Engine:
         """Capture compression launches in a HIP graph and replay it for
         batches of the same launch shape."""
         self._check(self._L.chain_engine_set_graphs(self._h, 1 if on else 0))
+
+    def memory_stats(self):
+        """(arena bytes the last compression carved, compressions run again with worst-case bounds)."""
+        need, retries = C.c_uint64(), C.c_uint64()
+        self._check(self._L.chain_engine_memory_stats(self._h, C.byref(need), C.byref(retries)))
+        return int(need.value), int(retries.value)
 
     def graph_stats(self):
         c, r = C.c_uint64(), C.c_uint64()

@@ -268,6 +268,20 @@ double chain_server_clock(chain_server* s);
  * worker at creation; a larger count means a batch outgrew its arena) */
 int chain_server_stats(chain_server* s, uint64_t* arena_allocations);
 
+/* Memory diagnostics (no counterpart in the reference, which sizes nothing in
+ * advance; for callers that budget device memory).
+ * chain_serve_estimate_compress: the device bytes the server reserves for one
+ * compress request of n points (dtype CHAIN_F32 / CHAIN_F64, block size p,
+ * chain_flags) with the given rANS chunk length: an upper bound of what the
+ * compression carves in its first run. 0 for invalid arguments.
+ * chain_engine_memory_stats: arena bytes the last compression on e carved,
+ * and how many compressions on e ran a second time with worst-case bounds (a
+ * tile with more than 256 exact corrections, or an output over the bound of
+ * its plan). */
+uint64_t chain_serve_estimate_compress(uint64_t n, uint32_t dims, int dtype, int64_t p, uint32_t flags,
+                                       uint32_t chunk_symbols);
+int chain_engine_memory_stats(chain_engine* e, uint64_t* compress_need, uint64_t* capacity_retries);
+
 #ifdef __cplusplus
 }
 #endif

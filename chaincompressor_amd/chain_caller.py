@@ -5,9 +5,12 @@ the MI355X engine:
 
 * chain_gpu(coords (D, N), eb, p, threshold, diff) -> bytes (LCH1)
   replaces lcps_gpu_cpp.lcps_compress2_gpu (cu/chain/lcps_gpu_cpp.cu:1095)
-* gpu_decompress(bytes) -> float64 (N, D)
+* gpu_decompress(bytes) -> float64 (N, D); float32 for LCG2
   replaces lcps_gpu_decompress_ans.lcps_decompress_gpu
-  (cu/decompress/lcps_gpu_decompress_ans.cu:2898)
+  (cu/decompress/lcps_gpu_decompress_ans.cu:2898): LCG2 streams of the
+  reference GPU compressor decode bit for bit like its lcg2::decode when their
+  components use the Fixed encoding (Zstd sections, inflated on the host);
+  Tagged and Raw components are coded with nvCOMP ANS and are refused
 * chain(coords (N, D), eb, p, threshold, diff, bl, l1, l2, ec) -> [bytes, stats...]
   mirrors lcps_cpp.lcps_compress2 (cpp/chain/lcps_cpp.cpp:2340): ec=False
   returns the reference's LCP3 bytes exactly; ec=True returns the reference's
@@ -41,11 +44,15 @@ def chain(coords, eb, p, threshold, diff, bl=True, l1=True, l2=True, ec=True, *,
 
 def _device_stream(data):
     """LCP2 (the reference CPU compressor's container) becomes the LCP3 of the
-    same streams on the host; LCH1 and LCP3 go to the GPU as they are."""
+    same streams on the host, LCG2 (its GPU compressor's) gets its Zstd
+    sections inflated; LCH1 and LCP3 go to the GPU as they are."""
     b = bytes(data)
     if b[:4] == b"LCP2":
         from chaincompressor_amd import lcp2
         return lcp2.to_lcp3(b)
+    if b[:4] == b"LCG2":
+        from chaincompressor_amd import lcp2
+        return lcp2.inflate_lcg2(b)
     return b
 
 

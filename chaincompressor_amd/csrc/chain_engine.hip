@@ -1673,13 +1673,139 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
     }
 }
 
+// One component of an inflated LCG2 stream (lcps_gpu_decompress_ans.cu:
+// 2377-2424): Fixed fields (i32 minimum >= 0, u8 width 1..31, u64 bytes)
+// become a Fixed unpack job over the stored section in place; Tagged and Raw
+// components need nvCOMP ANS and are refused.
+static void parse_lcg2_component(HostReader r, int comp, uint32_t expected_dims, uint64_t max_records, uint64_t dst,
+                                 DecPlan& P, uint64_t* n_out) {
+    const uint8_t kind = r.u8();
+    require(kind <= 3, CHAIN_ERUNTIME, "unknown LCG2 encoding");
+    const uint32_t dims = r.u32();
+    const uint64_t n = r.u64();
+    require(dims == expected_dims && n <= max_records, CHAIN_ERUNTIME, "LCG2 component shape mismatch");
+    *n_out = n;
+    require(kind != 1 && kind != 3, CHAIN_ERUNTIME,
+            "LCG2 Tagged and Raw components are coded with nvCOMP ANS, whose format is not supported");
+    if (kind == 0) {
+        require(n == 0 && r.u8() == 0, CHAIN_ERUNTIME, "invalid empty LCG2 component");
+        r.finish();
+        return;
+    }
+    require(n > 0, CHAIN_ERUNTIME, "nonempty LCG2 encoding with no records");
+    DecFixed j;
+    std::memset(&j, 0, sizeof j);
+    uint64_t total = 0;
+    for (uint32_t d = 0; d < dims; ++d) {
+        const int32_t mn = (int32_t)r.u32();
+        j.width[d] = r.u8();
+        const uint64_t bytes = r.u64();
+        require(mn >= 0 && j.width[d] >= 1 && j.width[d] <= 31, CHAIN_ERUNTIME, "invalid LCG2 fixed field");
+        require(bytes == (n * j.width[d] + 7) / 8, CHAIN_ERUNTIME, "LCG2 fixed length mismatch");
+        j.minimum[d] = (uint64_t)mn;
+        j.byte_off[d] = total;
+        total += bytes;
+    }
+    require(r.u8() == 1, CHAIN_ERUNTIME, "LCG2 Fixed needs one section");
+    const uint8_t codec = r.u8();
+    require(codec != 2, CHAIN_ERUNTIME, "LCG2 Zstd sections are inflated on the host first (chain_lcg2_inflate)");
+    require(codec == 0, CHAIN_ERUNTIME, "unexpected LCG2 section codec");
+    require(r.u64() == total, CHAIN_ERUNTIME, "LCG2 section raw length mismatch");
+    const uint64_t at = r.dev + r.pos;
+    (void)r.bytes(total);
+    r.finish();
+    for (uint32_t d = 0; d < dims; ++d) {
+        j.byte_off[d] += at;
+        j.end[d] = j.byte_off[d] + (n * j.width[d] + 7) / 8;
+    }
+    j.n = n;
+    j.dims = dims;
+    j.dst = dst;
+    j.comp = comp;
+    j.tile0 = (uint32_t)P.fixed_tiles;
+    P.fixed_tiles += cdiv(n, DEC_FIX_RECS);
+    P.fixed.push_back(j);
+}
+
+// An LCG2 stream of the reference GPU compressor (cu/chain/lcps_gpu_cpp.cu:
+// 5-35) whose Zstd sections the host has inflated, read with the checks of
+// lcg2::parse (cu/decompress/lcps_gpu_decompress_ans.cu:2426-2452). Its
+// integer streams have the meaning of the LCP2 ones; the frame is rebuilt
+// in float32 (DecFrame.lcg2) and has no exact corrections.
+static void parse_lcg2(HostReader& r, DecPlan& P) {
+    require(r.u32() == 1, CHAIN_ERUNTIME, "unsupported LCG2 version");
+    DecFrame F;
+    std::memset(&F, 0, sizeof F);
+    const uint32_t D = r.u32();
+    F.n = r.u64();
+    const int32_t p = (int32_t)r.u32();
+    const uint32_t ebits = r.u32();
+    float eb;
+    std::memcpy(&eb, &ebits, 4);
+    const int32_t wmin = (int32_t)r.u32(), wmax = (int32_t)r.u32();
+    require(D > 0 && F.n > 0 && F.n <= (uint64_t)INT32_MAX && p > 0, CHAIN_ERUNTIME,
+            "invalid LCG2 shape or block size");
+    require(D <= CH_MAXD, CHAIN_EINVAL, "the GPU engine supports 1 to 4 dimensions");
+    require(P.D == 0 || P.D == D, CHAIN_EINVAL, "every stream of a batch must have the same dimensions");
+    P.D = D;
+    require(eb > 0.0f && std::isfinite(2.0f * eb), CHAIN_ERUNTIME, "invalid LCG2 error bound");
+    require(wmin >= 0 && wmax >= wmin, CHAIN_ERUNTIME, "invalid LCG2 wrap range");
+    F.p = (uint64_t)p;
+    F.eb = eb;
+    F.denom = 2.0 * eb;
+    F.eb_f = eb;
+    F.lcg2 = 1;
+    F.wrap_min = wmin;
+    F.wrap_max = wmax;
+    F.range = (uint64_t)(wmax - wmin) + 1;
+    F.max_corr = (int64_t)(F.range / 2);
+    F.min_corr = -F.max_corr;
+    if ((F.range & 1) == 0) --F.max_corr;
+    for (uint32_t d = 0; d < D; ++d) {
+        const uint32_t b = r.u32();
+        float m;
+        std::memcpy(&m, &b, 4);
+        require(std::isfinite(m), CHAIN_ERUNTIME, "nonfinite LCG2 minimum");
+        F.mins_f[d] = m;
+        F.mins[d] = m;
+    }
+    require(r.u32() == 5, CHAIN_ERUNTIME, "expected five LCG2 components");
+    F.pt_off = P.T;
+    F.out_off = P.T * D;
+    static const char* names[5] = {"IDR0", "DIF0", "KEP0", "CH20", "REL0"};
+    const uint32_t cd[5] = {1, D, D, 1, D};
+    uint64_t cn[5], dst[5];
+    dst[0] = F.pt_off;
+    dst[4] = F.pt_off * D;
+    for (int c = 0; c < 5; ++c) {
+        require(std::memcmp(r.bytes(4), names[c], 4) == 0, CHAIN_ERUNTIME, "unexpected LCG2 component name");
+        HostReader cr = r.sub(r.u64());
+        if (c == 1) dst[1] = P.ND * D;
+        if (c == 2) dst[2] = P.NK * D;
+        if (c == 3) dst[3] = P.R;
+        parse_lcg2_component(cr, c, cd[c], F.n, dst[c], P, &cn[c]);
+        if (c == 1) { F.diff_base = P.ND; F.ndiff = cn[1]; P.ND += cn[1]; }
+        if (c == 2) { F.keep_base = P.NK; F.nkeep = cn[2]; P.NK += cn[2]; }
+        if (c == 3) { F.rec_base = P.R; F.nrec = cn[3]; P.R += cn[3]; }
+    }
+    r.finish();
+    require(cn[0] == F.n && cn[4] == F.n, CHAIN_ERUNTIME, "LCG2 coordinate stream count mismatch");
+    require(cn[3] > 0 && cn[2] > 0 && cn[1] + cn[2] == cn[3], CHAIN_ERUNTIME, "LCG2 diff/keep count mismatch");
+    P.T += F.n;
+    P.frames.push_back(F);
+}
+
 static void parse_stream(const uint8_t* bytes, uint64_t dev_off, uint64_t len, DecPlan& P) {
     HostReader r{bytes, 0, len, dev_off};
     const uint8_t* magic = r.bytes(4);
+    if (std::memcmp(magic, "LCG2", 4) == 0) {
+        parse_lcg2(r, P);
+        return;
+    }
     const bool lch1 = std::memcmp(magic, "LCH1", 4) == 0, lcp3 = std::memcmp(magic, "LCP3", 4) == 0;
     require(lch1 || lcp3 || std::memcmp(magic, "LCP2", 4) != 0, CHAIN_ERUNTIME,
             "LCP2 streams carry Zstd sections; decode them with lcps_cpu_decompress");
-    require(lch1 || lcp3, CHAIN_ERUNTIME, "expected a self-contained LCH1 or LCP3 bitstream");
+    require(lch1 || lcp3, CHAIN_ERUNTIME, "expected a self-contained LCH1, LCP3 or inflated LCG2 bitstream");
     DecFrame F;
     std::memset(&F, 0, sizeof F);
     const uint32_t D = r.u32();

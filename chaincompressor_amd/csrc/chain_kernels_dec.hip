@@ -38,6 +38,11 @@ struct DecFrame {
     uint64_t out_off;           // element offset of the frame in the output
     uint64_t corr_off, ncorr;   // byte offset of the raw correction records in the input pool
     uint32_t tile0, rtile0;     // first point tile, first record tile
+    // LCG2 streams (the reference GPU compressor's container): float32
+    // reconstruction fmaf(float(q) + 0.5f, 2 eb_f, mins_f[d]), q <= INT32_MAX
+    uint32_t lcg2;
+    float eb_f;
+    float mins_f[CH_MAXD];
 };
 
 // destination of a decoded value: component c of frame f, value index k
@@ -1229,6 +1234,11 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
     double mins[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) mins[d] = Fr.mins[d];
+    const bool lcg2 = Fr.lcg2 != 0;
+    const float step_f = 2.0f * Fr.eb_f;
+    float mins_f[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) mins_f[d] = Fr.mins_f[d];
     const uint64_t i0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
     __shared__ OutT s_out[SOA ? 1 : DQ_STAGE * CH_WG * D];
     uint32_t bad = 0;
@@ -1289,7 +1299,15 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
                     if (good && (o >= p || __umul64hi(b, p) != 0 || b * p > (uint64_t)INT64_MAX - o))
                         bad |= DERR_QUANT;
                     const int64_t q = (int64_t)(b * p + o);
-                    const double x = fma((double)q + 0.5, step, mins[d]);
+                    double x;
+                    if (lcg2) {
+                        // cu/decompress/lcps_gpu_decompress_ans.cu:2605-2625
+                        const float v = __builtin_fmaf((float)(int32_t)q + 0.5f, step_f, mins_f[d]);
+                        if (good && (q > (int64_t)INT32_MAX || !isfinite(v))) bad |= DERR_QUANT;
+                        x = (double)v;
+                    } else {
+                        x = fma((double)q + 0.5, step, mins[d]);
+                    }
                     if (SOA) {
                         if (good) out[out_off + (uint64_t)d * n + i] = (OutT)x;
                     } else {

@@ -20,6 +20,7 @@
 #include <zstd.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -73,7 +74,7 @@ struct In {
     const uint8_t* p;
     size_t pos, end;
     const uint8_t* take(size_t n) {
-        need(n <= end - pos, "truncated LCP3 stream");
+        need(n <= end - pos, "truncated stream");
         const uint8_t* q = p + pos;
         pos += n;
         return q;
@@ -487,6 +488,151 @@ Mat read_component(In r, uint32_t expect_dims) {
     return m;
 }
 
+// ---------------------------------------------------------------------------
+// LCG2, the reference GPU compressor's container (cu/chain/lcps_gpu_cpp.cu:5-35;
+// read by lcg2::parse, cu/decompress/lcps_gpu_decompress_ans.cu:2345-2452),
+// with every check of that reader. Its Zstd sections become stored sections
+// (codec 0: u8 0, u64 raw bytes, the bytes), which the GPU engine reads in
+// place; the rest is copied. nvCOMP ANS sections (the tags of Tagged, the
+// values of Raw) are refused: their format is nvCOMP's own.
+
+uint32_t le32(const uint8_t* q) { uint32_t v = 0; for (int i = 0; i < 4; ++i) v |= (uint32_t)q[i] << (8 * i); return v; }
+uint64_t le64(const uint8_t* q) { uint64_t v = 0; for (int i = 0; i < 8; ++i) v |= (uint64_t)q[i] << (8 * i); return v; }
+
+// one section (lcps_gpu_decompress_ans.cu:2345-2375): codec, raw length,
+// chunk table (raw chunks of 65536 bytes for Zstd), chunks
+void lcg2_section(In& r, Out& o, uint8_t expected_codec, uint64_t expected_raw) {
+    const uint8_t codec = *r.take(1);
+    if (codec == 1)
+        throw Fail(CHAIN_LCP2_EUNSUPPORTED, "LCG2 section coded with nvCOMP ANS (Tagged tags or Raw values) is not "
+                                            "supported: its format is nvCOMP's own");
+    need(codec == expected_codec || codec == 0, "unexpected LCG2 section codec");
+    const uint64_t raw = r.u64();
+    need(raw == expected_raw && raw > 0, "LCG2 section raw length mismatch");
+    o.u8(0);
+    o.u64(raw);
+    if (codec == 0) {
+        o.raw(r.take(raw), raw);
+        return;
+    }
+    const uint64_t limit = 65536;
+    const uint32_t nchunks = r.u32();
+    need(nchunks == raw / limit + (raw % limit != 0), "LCG2 section chunk count mismatch");
+    need(nchunks <= (r.end - r.pos) / 16, "truncated LCG2 chunk table");
+    std::vector<uint64_t> ub(nchunks), cb(nchunks);
+    uint64_t remaining = raw, comp_total = 0;
+    for (uint32_t k = 0; k < nchunks; ++k) {
+        ub[k] = r.u64();
+        cb[k] = r.u64();
+        need(ub[k] == std::min(limit, remaining) && cb[k] > 0, "invalid LCG2 chunk lengths");
+        remaining -= ub[k];
+        need(cb[k] <= r.end - r.pos - comp_total, "truncated LCG2 section");
+        comp_total += cb[k];
+    }
+    need(remaining == 0, "incomplete LCG2 section");
+    for (uint32_t k = 0; k < nchunks; ++k) {
+        const std::vector<uint8_t> part = unzstd(r.take(cb[k]), cb[k], ub[k]);
+        o.raw(part.data(), part.size());
+    }
+}
+
+// one component (lcps_gpu_decompress_ans.cu:2377-2424)
+uint64_t lcg2_component(In& top, Out& o, const char* name, uint32_t dims, uint64_t max_n) {
+    need(std::memcmp(top.take(4), name, 4) == 0, "unexpected LCG2 component name");
+    const uint64_t length = top.u64();
+    need(length <= top.end - top.pos, "truncated LCG2 component");
+    In r{top.p, top.pos, top.pos + length};
+    top.pos += length;
+    Out c;
+    const uint8_t kind = *r.take(1);
+    need(kind <= 3, "unknown LCG2 encoding");
+    const uint32_t cd = r.u32();
+    const uint64_t n = r.u64();
+    need(cd == dims && n <= max_n, "LCG2 component shape mismatch");
+    c.u8(kind);
+    c.u32(cd);
+    c.u64(n);
+    uint64_t fixed_bytes = 0, tagged_bits = 0;
+    if (kind == 2) {
+        for (uint32_t d = 0; d < dims; ++d) {
+            const uint8_t* f = r.take(13);
+            const int32_t mn = (int32_t)le32(f);
+            const uint32_t width = f[4];
+            const uint64_t bytes = le64(f + 5);
+            need(mn >= 0 && width >= 1 && width <= 31, "invalid LCG2 fixed field");
+            need(bytes == (n * width + 7) / 8, "LCG2 fixed length mismatch");
+            fixed_bytes += bytes;
+            c.raw(f, 13);
+        }
+    } else if (kind == 1) {
+        tagged_bits = r.u64();
+        need(tagged_bits >= n * dims && tagged_bits <= n * dims * 31, "invalid LCG2 packed bit count");
+        c.u64(tagged_bits);
+    }
+    const uint8_t sections = *r.take(1);
+    c.u8(sections);
+    if (kind == 0) {
+        need(n == 0 && sections == 0, "invalid empty LCG2 component");
+    } else {
+        need(n > 0, "nonempty LCG2 encoding with no records");
+        if (kind == 1) {
+            need(sections == 2, "LCG2 Tagged needs ANS tags and Zstd values");
+            lcg2_section(r, c, 1, n);
+            lcg2_section(r, c, 2, (tagged_bits + 7) / 8);
+        } else if (kind == 2) {
+            need(sections == 1, "LCG2 Fixed needs one Zstd section");
+            lcg2_section(r, c, 2, fixed_bytes);
+        } else {
+            need(sections == 1, "LCG2 Raw needs one ANS section");
+            lcg2_section(r, c, 1, n * dims * 4);
+        }
+    }
+    need(r.pos == r.end, "trailing LCG2 component bytes");
+    o.raw(name, 4);
+    o.blob(c.b);
+    return n;
+}
+
+// the container (lcps_gpu_decompress_ans.cu:2426-2452)
+std::vector<uint8_t> lcg2_inflate(const uint8_t* data, size_t len) {
+    In r{data, 0, len};
+    need(std::memcmp(r.take(4), "LCG2", 4) == 0, "not an LCG2 stream");
+    need(r.u32() == 1, "unsupported LCG2 version");
+    const uint32_t D = r.u32();
+    const uint64_t n = r.u64();
+    const uint8_t* hdr = r.take(16);   // i32 p, f32 eb, i32 wrap_min, i32 wrap_max
+    const int32_t p = (int32_t)le32(hdr), wmin = (int32_t)le32(hdr + 8), wmax = (int32_t)le32(hdr + 12);
+    float eb;
+    const uint32_t ebits = le32(hdr + 4);
+    std::memcpy(&eb, &ebits, 4);
+    need(D >= 1 && D <= 64 && n > 0 && n <= 0x7fffffffull && p > 0, "invalid LCG2 shape or block size");
+    need(eb > 0 && std::isfinite(2.0f * eb), "invalid LCG2 error bound");
+    need(wmin >= 0 && wmax >= wmin, "invalid LCG2 wrap range");
+    const uint8_t* mins = r.take(4ull * D);
+    for (uint32_t d = 0; d < D; ++d) {
+        float m;
+        const uint32_t b = le32(mins + 4 * d);
+        std::memcpy(&m, &b, 4);
+        need(std::isfinite(m), "nonfinite LCG2 minimum");
+    }
+    need(r.u32() == 5, "expected five LCG2 components");
+    Out o;
+    o.raw("LCG2", 4);
+    o.u32(1);
+    o.u32(D);
+    o.u64(n);
+    o.raw(hdr, 16);
+    o.raw(mins, 4ull * D);
+    o.u32(5);
+    const char* names[5] = {"IDR0", "DIF0", "KEP0", "CH20", "REL0"};
+    uint64_t cn[5];
+    for (int c = 0; c < 5; ++c) cn[c] = lcg2_component(r, o, names[c], (c == 0 || c == 3) ? 1u : D, n);
+    need(r.pos == r.end, "trailing LCG2 bytes");
+    need(cn[0] == n && cn[4] == n, "LCG2 coordinate stream count mismatch");
+    need(cn[3] > 0 && cn[2] > 0 && cn[1] + cn[2] == cn[3], "LCG2 diff/keep count mismatch");
+    return o.b;
+}
+
 }  // namespace
 
 extern "C" {
@@ -589,6 +735,31 @@ int chain_lcp2_from_lcp3(const uint8_t* lcp3, size_t len, int zstd_level, uint8_
         need(*out != nullptr, "out of host memory", CHAIN_LCP2_ENOMEM);
         std::memcpy(*out, o.b.data(), o.b.size());
         *out_len = o.b.size();
+        return CHAIN_LCP2_OK;
+    } catch (const Fail& f) {
+        if (err && err_len) {
+            std::strncpy(err, f.what(), err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        return f.code;
+    } catch (const std::bad_alloc&) {
+        if (err && err_len) {
+            std::strncpy(err, "out of host memory", err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        return CHAIN_LCP2_ENOMEM;
+    }
+}
+
+int chain_lcg2_inflate(const uint8_t* lcg2, size_t len, uint8_t** out, size_t* out_len, char* err,
+                       size_t err_len) {
+    try {
+        need(lcg2 && out && out_len, "NULL argument", CHAIN_LCP2_EINVAL);
+        const std::vector<uint8_t> o = lcg2_inflate(lcg2, len);
+        *out = (uint8_t*)std::malloc(o.size() ? o.size() : 1);
+        need(*out != nullptr, "out of host memory", CHAIN_LCP2_ENOMEM);
+        std::memcpy(*out, o.data(), o.size());
+        *out_len = o.size();
         return CHAIN_LCP2_OK;
     } catch (const Fail& f) {
         if (err && err_len) {

@@ -8,17 +8,24 @@
 // GPU modules, kept on purpose for bit-exactness with the CPU path:
 //   * eb is taken as a Python float (double), not float32;
 //   * quantisation is FP64 with exact corrections (LCP2 semantics);
-//   * decompression returns float64 unless dtype="float32" is asked for.
+//   * decompression returns float64 unless dtype="float32" is asked for;
+//     LCG2 streams (the reference GPU compressor's) return float32, as the
+//     reference does, with their Zstd sections inflated on the host
+//     (libchain_lcp2, built with CHAIN_WITH_LCG2) and Fixed components only.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 
 #include "../../include/chain_engine.h"
+#ifdef CHAIN_WITH_LCG2
+#include "../../include/chain_lcp2.h"
+#endif
 
 namespace py = pybind11;
 
@@ -129,10 +136,30 @@ PYBIND11_MODULE(lcps_gpu_cpp, m) {
 
 #ifdef CHAIN_MODULE_DECOMPRESS
 
-// lcps_decompress_gpu(compressed_data, output_soa=False, dtype="float64")
-static py::array lcps_decompress_gpu(py::bytes compressed, bool output_soa, const std::string& dtype) {
+// lcps_decompress_gpu(compressed_data, output_soa=False, dtype=None)
+static py::array lcps_decompress_gpu(py::bytes compressed, bool output_soa, py::object dtype_arg) {
     std::string s = compressed;
+    const bool lcg2 = s.size() >= 4 && std::memcmp(s.data(), "LCG2", 4) == 0;
+    const std::string dtype = dtype_arg.is_none() ? (lcg2 ? "float32" : "float64") : py::str(dtype_arg).cast<std::string>();
     if (dtype != "float64" && dtype != "float32") throw py::value_error("dtype must be 'float64' or 'float32'");
+    if (lcg2) {
+#ifdef CHAIN_WITH_LCG2
+        uint8_t* inflated = nullptr;
+        size_t len = 0;
+        char err[256] = {0};
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = chain_lcg2_inflate(reinterpret_cast<const uint8_t*>(s.data()), s.size(), &inflated, &len, err,
+                                    sizeof err);
+        }
+        if (rc != CHAIN_LCP2_OK) throw std::runtime_error(std::string("LCG2: ") + err);
+        s.assign(reinterpret_cast<const char*>(inflated), len);
+        chain_lcp2_free(inflated);
+#else
+        throw std::runtime_error("LCG2 streams need libchain_lcp2 (libzstd) to inflate their Zstd sections");
+#endif
+    }
     const int odt = dtype == "float32" ? CHAIN_F32 : CHAIN_F64;
     chain_engine* e = engine();
     chain_buf out{nullptr, 0};
@@ -159,9 +186,9 @@ static py::array lcps_decompress_gpu(py::bytes compressed, bool output_soa, cons
 }
 
 PYBIND11_MODULE(lcps_gpu_decompress_ans, m) {
-    m.doc() = "MI355X Chain decompressor (HIP, gfx950) for LCH1 and LCP3 streams.";
+    m.doc() = "MI355X Chain decompressor (HIP, gfx950) for LCH1, LCP3 and LCG2 (Fixed components) streams.";
     m.def("lcps_decompress_gpu", &lcps_decompress_gpu, py::arg("compressed_data"), py::arg("output_soa") = false,
-          py::arg("dtype") = "float64");
+          py::arg("dtype") = py::none());
 }
 
 #endif

@@ -28,6 +28,7 @@ enum chain_lcp2_status {
     CHAIN_LCP2_EFORMAT = 2,   /* not a valid LCP3 stream */
     CHAIN_LCP2_EZSTD = 3,     /* libzstd error */
     CHAIN_LCP2_ENOMEM = 4,
+    CHAIN_LCP2_EUNSUPPORTED = 5,   /* an LCG2 section coded with nvCOMP ANS */
 };
 
 /* lcp3/len: one LCP3 stream. zstd_level: 22 for the reference's bytes.
@@ -42,6 +43,16 @@ int chain_lcp2_from_lcp3(const uint8_t* lcp3, size_t len, int zstd_level, uint8_
  * LCP3 of the same integer streams, which the GPU engine decodes. */
 int chain_lcp3_from_lcp2(const uint8_t* lcp2, size_t len, uint8_t** out, size_t* out_len, char* err,
                          size_t err_len);
+/* An LCG2 stream of the reference GPU compressor (cu/chain/lcps_gpu_cpp.cu:5-35;
+ * replaces the host parse of lcg2::decode, cu/decompress/
+ * lcps_gpu_decompress_ans.cu:2345-2452, with every one of its checks) with
+ * each Zstd section inflated to a stored section (codec 0: u8 0, u64 raw
+ * bytes, the bytes), the form the GPU engine decodes (chain_decompress*).
+ * Fixed and empty components are supported; a Tagged or Raw component,
+ * whose section is coded with nvCOMP ANS, returns CHAIN_LCP2_EUNSUPPORTED.
+ * A stream that is already inflated is returned unchanged. */
+int chain_lcg2_inflate(const uint8_t* lcg2, size_t len, uint8_t** out, size_t* out_len, char* err,
+                       size_t err_len);
 void chain_lcp2_free(uint8_t* p);
 /* version of the libzstd the library runs with */
 const char* chain_lcp2_zstd_version(void);

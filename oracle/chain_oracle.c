@@ -1226,40 +1226,11 @@ static int64_t* decode_component(octx* c, rd r, uint32_t expected_dims, uint64_t
 }
 
 /* serial.cpp:1182-1338 and lcps_cpp.cpp:1838-1995 */
-int oc_decode(const uint8_t* bytes, size_t len, double** out, uint64_t* n_out, uint32_t* dims_out) {
-    octx cx; cx.top = NULL; octx* c = &cx;
-    if (setjmp(cx.jb)) { arena_release(c); return -1; }
-    REQUIRE(c, fegetround() == FE_TONEAREST, "LCP2 decoding requires round-to-nearest floating-point mode");
-    rd r = {c, bytes, len};
-    const uint8_t* magic = rd_bytes(&r, 4);
-    const int lcp2 = !memcmp(magic, "LCP2", 4), lcp3 = !memcmp(magic, "LCP3", 4), lch1 = !memcmp(magic, "LCH1", 4);
-    REQUIRE(c, lcp2 || lcp3 || lch1, "expected a self-contained LCP2, LCP3 or LCH1 bitstream");
-    const uint32_t dims = rd_u32(&r);
-    const uint64_t n = rd_u64(&r);
-    REQUIRE(c, dims > 0 && n > 0, "invalid LCP2 shape");
-    const size_t nv = mul_chk(c, (size_t)n, dims);
-    const int64_t p = rd_i64(&r);
-    const double eb = rd_f64(&r);
-    REQUIRE(c, p > 0 && eb > 0.0 && isfinite(2.0 * eb), "invalid LCP2 quantization parameters");
-    const int64_t wmin = rd_i64(&r), wmax = rd_i64(&r);
-    REQUIRE(c, wmin >= 0, "invalid LCP2 block range");
-    const wrap_t wrap = make_wrap(c, wmin, wmax);
-    REQUIRE(c, dims <= r.n / 8, "truncated LCP2 minima");
-    double* mins = (double*)ax(c, dims * sizeof(double));
-    for (uint32_t d = 0; d < dims; ++d) { mins[d] = rd_f64(&r); REQUIRE(c, isfinite(mins[d]), "invalid LCP2 coordinate minimum"); }
-    int64_t* comp[5];
-    uint64_t cn[5];
-    const uint32_t cd[5] = {1, dims, dims, 1, dims};
-    for (int k = 0; k < 5; ++k) {
-        rd blob = rd_blob(&r);
-        if (lcp3) REQUIRE(c, blob.n >= 4 && !memcmp(blob.p, "RAW1", 4), "LCP3 requires raw component streams");
-        comp[k] = decode_component(c, blob, cd[k], n, &cn[k]);
-    }
-    const uint64_t exact_count = rd_u64(&r);
-    rd exact = rd_blob(&r);
-    rd_end(&r);
-    REQUIRE(c, cn[0] == n && cn[4] == n, "LCP2 point count mismatch");
-    REQUIRE(c, exact_count <= nv, "invalid LCP2 correction count");
+/* Integer decode shared by every container: L2 inverse (serial.cpp:1230-1275)
+ * and the backward chain pass (serial.cpp:1277-1302); comp[4] (block offsets)
+ * becomes the quantised coordinates q. */
+static void rebuild_q(octx* c, int64_t* comp[5], const uint64_t cn[5], uint32_t dims, uint64_t n, int64_t p,
+                      int64_t wmin, int64_t wmax, const wrap_t wrap) {
     const int64_t* idrel = comp[0];
     const int64_t* dif = comp[1];
     const int64_t* kept = comp[2];
@@ -1323,6 +1294,44 @@ int oc_decode(const uint8_t* bytes, size_t len, double** out, uint64_t* n_out, u
         }
     }
     REQUIRE(c, ci == 0, "unused explicit blocks");
+}
+
+int oc_decode(const uint8_t* bytes, size_t len, double** out, uint64_t* n_out, uint32_t* dims_out) {
+    octx cx; cx.top = NULL; octx* c = &cx;
+    if (setjmp(cx.jb)) { arena_release(c); return -1; }
+    REQUIRE(c, fegetround() == FE_TONEAREST, "LCP2 decoding requires round-to-nearest floating-point mode");
+    rd r = {c, bytes, len};
+    const uint8_t* magic = rd_bytes(&r, 4);
+    const int lcp2 = !memcmp(magic, "LCP2", 4), lcp3 = !memcmp(magic, "LCP3", 4), lch1 = !memcmp(magic, "LCH1", 4);
+    REQUIRE(c, lcp2 || lcp3 || lch1, "expected a self-contained LCP2, LCP3 or LCH1 bitstream");
+    const uint32_t dims = rd_u32(&r);
+    const uint64_t n = rd_u64(&r);
+    REQUIRE(c, dims > 0 && n > 0, "invalid LCP2 shape");
+    const size_t nv = mul_chk(c, (size_t)n, dims);
+    const int64_t p = rd_i64(&r);
+    const double eb = rd_f64(&r);
+    REQUIRE(c, p > 0 && eb > 0.0 && isfinite(2.0 * eb), "invalid LCP2 quantization parameters");
+    const int64_t wmin = rd_i64(&r), wmax = rd_i64(&r);
+    REQUIRE(c, wmin >= 0, "invalid LCP2 block range");
+    const wrap_t wrap = make_wrap(c, wmin, wmax);
+    REQUIRE(c, dims <= r.n / 8, "truncated LCP2 minima");
+    double* mins = (double*)ax(c, dims * sizeof(double));
+    for (uint32_t d = 0; d < dims; ++d) { mins[d] = rd_f64(&r); REQUIRE(c, isfinite(mins[d]), "invalid LCP2 coordinate minimum"); }
+    int64_t* comp[5];
+    uint64_t cn[5];
+    const uint32_t cd[5] = {1, dims, dims, 1, dims};
+    for (int k = 0; k < 5; ++k) {
+        rd blob = rd_blob(&r);
+        if (lcp3) REQUIRE(c, blob.n >= 4 && !memcmp(blob.p, "RAW1", 4), "LCP3 requires raw component streams");
+        comp[k] = decode_component(c, blob, cd[k], n, &cn[k]);
+    }
+    const uint64_t exact_count = rd_u64(&r);
+    rd exact = rd_blob(&r);
+    rd_end(&r);
+    REQUIRE(c, cn[0] == n && cn[4] == n, "LCP2 point count mismatch");
+    REQUIRE(c, exact_count <= nv, "invalid LCP2 correction count");
+    rebuild_q(c, comp, cn, dims, n, p, wmin, wmax, wrap);
+    const int64_t* qz = comp[4];
 
     double* vals = (double*)ax(c, nv * sizeof(double));
     for (uint64_t i = 0; i < n; ++i)
@@ -1347,6 +1356,202 @@ int oc_decode(const uint8_t* bytes, size_t len, double** out, uint64_t* n_out, u
         rd_end(&cr);
     }
     *out = (double*)dup_out(c, vals, nv * sizeof(double));
+    *n_out = n; *dims_out = dims;
+    arena_release(c);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LCG2, the reference GPU compressor's container (cu/chain/lcps_gpu_cpp.cu:5-35).
+ * The writer is test infrastructure for the reader: it stores given integer
+ * streams with the Fixed encoding only (one Zstd section per component, raw
+ * chunks of 65536 bytes, lcps_gpu_decompress_ans.cu:2356-2358) or as an empty
+ * component; the Tagged and Raw encodings need nvCOMP ANS, whose format is
+ * nvCOMP's own. The decoder restates lcg2::decode
+ * (cu/decompress/lcps_gpu_decompress_ans.cu:2345-2452, 2700-2900): the same
+ * integer streams as LCP2, float32 reconstruction
+ * fmaf(float(q) + 0.5f, 2.0f * eb, minimum) (:2605-2625), no corrections. Its
+ * sections may also be stored (codec 0, the inflated form of this project). */
+
+static void lcg2_fixed_component(octx* c, const mat* m, bb* out) {
+    bb comp = bb_make(c);
+    if (m->n == 0) {
+        bb_u8(&comp, 0); bb_u32(&comp, m->dims); bb_u64(&comp, 0); bb_u8(&comp, 0);
+        bb_blob(out, &comp);
+        return;
+    }
+    bb payload = bb_make(c);
+    bb_u8(&comp, 2); bb_u32(&comp, m->dims); bb_u64(&comp, m->n);
+    for (uint32_t d = 0; d < m->dims; ++d) {
+        int64_t mn; const int w = fixed_width(m->v, m->n, m->dims, d, &mn);
+        REQUIRE(c, mn >= 0 && mn <= INT32_MAX && w <= 31, "LCG2 values are non-negative int32 of at most 31 bits");
+        const size_t before = payload.n;
+        pack_fixed_dim(c, m->v, m->n, m->dims, d, mn, w, &payload);
+        bb_u32(&comp, (uint32_t)(int32_t)mn); bb_u8(&comp, (uint8_t)w); bb_u64(&comp, payload.n - before);
+    }
+    bb_u8(&comp, 1);                         /* one section */
+    bb_u8(&comp, 2);                         /* Zstd */
+    bb_u64(&comp, payload.n);
+    const uint64_t limit = 65536, nchunks = (payload.n + limit - 1) / limit;
+    bb_u32(&comp, (uint32_t)nchunks);
+    bb* frames = (bb*)ax(c, nchunks * sizeof(bb));
+    for (uint64_t k = 0; k < nchunks; ++k) {
+        const uint64_t ub = payload.n - k * limit < limit ? payload.n - k * limit : limit;
+        frames[k] = bb_make(c);
+        zstd_compress(c, payload.p + k * limit, ub, &frames[k]);
+        bb_u64(&comp, ub); bb_u64(&comp, frames[k].n);
+    }
+    for (uint64_t k = 0; k < nchunks; ++k) bb_put(&comp, frames[k].p, frames[k].n);
+    bb_blob(out, &comp);
+}
+
+/* one Zstd frame of the given bytes (for hand-built test containers) */
+int oc_zstd_frame(const uint8_t* src, size_t n, uint8_t** out, size_t* out_len) {
+    octx cx; cx.top = NULL; octx* c = &cx;
+    if (setjmp(cx.jb)) { arena_release(c); return -1; }
+    bb z = bb_make(c);
+    zstd_compress(c, src, n, &z);
+    *out = (uint8_t*)dup_out(c, z.p, z.n); *out_len = z.n;
+    arena_release(c);
+    return 0;
+}
+
+int oc_write_lcg2(const oc_streams* s, float eb, const float* mins, uint8_t** out, size_t* out_len) {
+    octx cx; cx.top = NULL; octx* c = &cx;
+    if (setjmp(cx.jb)) { arena_release(c); return -1; }
+    REQUIRE(c, s->n <= INT32_MAX && s->p_eff <= INT32_MAX && s->wrap_max <= INT32_MAX, "LCG2 holds int32 fields");
+    bb b = bb_make(c);
+    bb_put(&b, "LCG2", 4); bb_u32(&b, 1); bb_u32(&b, s->dims); bb_u64(&b, s->n);
+    uint32_t u;
+    bb_u32(&b, (uint32_t)(int32_t)s->p_eff);
+    memcpy(&u, &eb, 4); bb_u32(&b, u);
+    bb_u32(&b, (uint32_t)(int32_t)s->wrap_min); bb_u32(&b, (uint32_t)(int32_t)s->wrap_max);
+    for (uint32_t d = 0; d < s->dims; ++d) { memcpy(&u, &mins[d], 4); bb_u32(&b, u); }
+    bb_u32(&b, 5);
+    mat m[5]; components(s, m);
+    static const char* names[5] = {"IDR0", "DIF0", "KEP0", "CH20", "REL0"};
+    for (int k = 0; k < 5; ++k) { bb_put(&b, names[k], 4); lcg2_fixed_component(c, &m[k], &b); }
+    *out = (uint8_t*)dup_out(c, b.p, b.n); *out_len = b.n;
+    arena_release(c);
+    return 0;
+}
+
+/* a section (lcps_gpu_decompress_ans.cu:2345-2375), or the stored form */
+static const uint8_t* lcg2_section(octx* c, rd* r, uint64_t expected_raw) {
+    const uint8_t codec = rd_u8(r);
+    REQUIRE(c, codec != 1, "LCG2 nvCOMP ANS sections are not supported");
+    REQUIRE(c, codec == 2 || codec == 0, "unexpected LCG2 section codec");
+    const uint64_t raw = rd_u64(r);
+    REQUIRE(c, raw == expected_raw && raw > 0, "LCG2 section raw length mismatch");
+    if (codec == 0) return rd_bytes(r, (size_t)raw);
+    const uint32_t nchunks = rd_u32(r);
+    const uint64_t limit = 65536;
+    REQUIRE(c, nchunks == raw / limit + (raw % limit != 0), "LCG2 section chunk count mismatch");
+    uint64_t* ub = (uint64_t*)ax(c, (nchunks ? nchunks : 1) * 8);
+    uint64_t* cb = (uint64_t*)ax(c, (nchunks ? nchunks : 1) * 8);
+    uint64_t remaining = raw;
+    for (uint32_t k = 0; k < nchunks; ++k) {
+        ub[k] = rd_u64(r); cb[k] = rd_u64(r);
+        REQUIRE(c, ub[k] == (remaining < limit ? remaining : limit) && cb[k] > 0, "invalid LCG2 chunk lengths");
+        remaining -= ub[k];
+    }
+    uint8_t* outb = (uint8_t*)ax(c, (size_t)raw);
+    uint64_t at = 0;
+    for (uint32_t k = 0; k < nchunks; ++k) {
+        const uint8_t* chunk = rd_bytes(r, (size_t)cb[k]);
+        const uint8_t* part = zstd_decompress(c, chunk, (size_t)cb[k], (size_t)ub[k]);
+        memcpy(outb + at, part, (size_t)ub[k]);
+        at += ub[k];
+    }
+    return outb;
+}
+
+/* a component (lcps_gpu_decompress_ans.cu:2377-2424, 2484-2495) */
+static int64_t* lcg2_component(octx* c, rd* top, const char* name, uint32_t dims, uint64_t max_n, uint64_t* n_out) {
+    REQUIRE(c, rd_magic(top, name), "unexpected LCG2 component name");
+    rd r = rd_blob(top);
+    const uint8_t kind = rd_u8(&r);
+    REQUIRE(c, kind <= 3, "unknown LCG2 encoding");
+    REQUIRE(c, kind != 1 && kind != 3, "LCG2 nvCOMP ANS sections are not supported");
+    const uint32_t cd = rd_u32(&r);
+    const uint64_t n = rd_u64(&r);
+    REQUIRE(c, cd == dims && n <= max_n, "LCG2 component shape mismatch");
+    *n_out = n;
+    int64_t* res = (int64_t*)ax(c, ((n * dims) != 0 ? n * dims : 1) * sizeof(int64_t));
+    if (kind == 0) {
+        REQUIRE(c, n == 0 && rd_u8(&r) == 0, "invalid empty LCG2 component");
+        rd_end(&r);
+        return res;
+    }
+    REQUIRE(c, n > 0, "nonempty LCG2 encoding with no records");
+    int32_t mn[64]; int w[64]; uint64_t sz[64];
+    REQUIRE(c, dims <= 64, "too many LCG2 dimensions");
+    uint64_t total = 0;
+    for (uint32_t d = 0; d < dims; ++d) {
+        mn[d] = (int32_t)rd_u32(&r); w[d] = rd_u8(&r); sz[d] = rd_u64(&r);
+        REQUIRE(c, mn[d] >= 0 && w[d] >= 1 && w[d] <= 31, "invalid LCG2 fixed field");
+        REQUIRE(c, sz[d] == (n * (uint64_t)w[d] + 7) / 8, "LCG2 fixed length mismatch");
+        total += sz[d];
+    }
+    REQUIRE(c, rd_u8(&r) == 1, "LCG2 Fixed needs one Zstd section");
+    const uint8_t* raw = lcg2_section(c, &r, total);
+    rd_end(&r);
+    uint64_t off = 0;
+    for (uint32_t d = 0; d < dims; ++d) {
+        size_t pos = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t v = read_bits(c, raw + off, (size_t)sz[d], &pos, w[d]) + (uint64_t)mn[d];
+            REQUIRE(c, v <= INT32_MAX, "LCG2 value overflow");
+            res[i * dims + d] = (int64_t)v;
+        }
+        off += sz[d];
+    }
+    return res;
+}
+
+int oc_decode_lcg2(const uint8_t* bytes, size_t len, float** out, uint64_t* n_out, uint32_t* dims_out) {
+    octx cx; cx.top = NULL; octx* c = &cx;
+    if (setjmp(cx.jb)) { arena_release(c); return -1; }
+    REQUIRE(c, fegetround() == FE_TONEAREST, "LCG2 decoding requires round-to-nearest floating-point mode");
+    rd r = {c, bytes, len};
+    REQUIRE(c, rd_magic(&r, "LCG2"), "not an LCG2 stream");
+    REQUIRE(c, rd_u32(&r) == 1, "unsupported LCG2 version");
+    const uint32_t dims = rd_u32(&r);
+    const uint64_t n = rd_u64(&r);
+    const int32_t p = (int32_t)rd_u32(&r);
+    float eb; uint32_t u = rd_u32(&r); memcpy(&eb, &u, 4);
+    const int32_t wmin = (int32_t)rd_u32(&r), wmax = (int32_t)rd_u32(&r);
+    REQUIRE(c, dims > 0 && dims <= 64 && n > 0 && n <= INT32_MAX && p > 0, "invalid LCG2 shape or block size");
+    REQUIRE(c, eb > 0 && isfinite(2.0f * eb), "invalid LCG2 error bound");
+    REQUIRE(c, wmin >= 0 && wmax >= wmin, "invalid LCG2 wrap range");
+    float* mins = (float*)ax(c, dims * sizeof(float));
+    for (uint32_t d = 0; d < dims; ++d) {
+        u = rd_u32(&r); memcpy(&mins[d], &u, 4);
+        REQUIRE(c, isfinite(mins[d]), "nonfinite LCG2 minimum");
+    }
+    REQUIRE(c, rd_u32(&r) == 5, "expected five LCG2 components");
+    static const char* names[5] = {"IDR0", "DIF0", "KEP0", "CH20", "REL0"};
+    const uint32_t cd[5] = {1, dims, dims, 1, dims};
+    int64_t* comp[5];
+    uint64_t cn[5];
+    for (int k = 0; k < 5; ++k) comp[k] = lcg2_component(c, &r, names[k], cd[k], n, &cn[k]);
+    rd_end(&r);
+    REQUIRE(c, cn[0] == n && cn[4] == n, "LCG2 coordinate stream count mismatch");
+    REQUIRE(c, cn[3] > 0 && cn[2] > 0 && cn[1] + cn[2] == cn[3], "LCG2 diff/keep count mismatch");
+    const wrap_t wrap = make_wrap(c, wmin, wmax);
+    rebuild_q(c, comp, cn, dims, n, p, wmin, wmax, wrap);
+    const int64_t* qz = comp[4];
+    const size_t nv = mul_chk(c, (size_t)n, dims);
+    float* vals = (float*)ax(c, nv * sizeof(float));
+    const float step = 2.0f * eb;
+    for (size_t k = 0; k < nv; ++k) {
+        const int64_t q = qz[k];
+        REQUIRE(c, q >= 0 && q <= INT32_MAX, "LCG2 quantized coordinate outside int32");
+        const float mid = (float)(int32_t)q + 0.5f;
+        vals[k] = fmaf(mid, step, mins[k % dims]);
+        REQUIRE(c, isfinite(vals[k]), "nonfinite LCG2 coordinate");
+    }
+    *out = (float*)dup_out(c, vals, nv * sizeof(float));
     *n_out = n; *dims_out = dims;
     arena_release(c);
     return 0;

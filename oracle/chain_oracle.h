@@ -81,6 +81,16 @@ int oc_write_lch1(const oc_streams* s, const oc_rans_params* rp,
 /* Decoders: LCP2, LCP3 and LCH1 -> float64 AoS (n, dims), malloc'd. */
 int oc_decode(const uint8_t* bytes, size_t len, double** out, uint64_t* n, uint32_t* dims);
 
+/* LCG2 (the reference GPU compressor's container, cu/chain/lcps_gpu_cpp.cu:5-35):
+ * the writer stores given streams with Fixed (Zstd) components only, with the
+ * given float32 error bound and minima; the decoder restates lcg2::decode of
+ * cu/decompress/lcps_gpu_decompress_ans.cu (float32 AoS output, malloc'd) and
+ * also reads stored (codec 0) sections. */
+int oc_write_lcg2(const oc_streams* s, float eb, const float* mins, uint8_t** out, size_t* out_len);
+int oc_decode_lcg2(const uint8_t* bytes, size_t len, float** out, uint64_t* n, uint32_t* dims);
+/* one Zstd frame (level 22) of the given bytes, for hand-built containers */
+int oc_zstd_frame(const uint8_t* src, size_t n, uint8_t** out, size_t* out_len);
+
 /* Pre-entropy payloads of one component (matrix values[r*dims+d] >= 0). */
 int oc_pack_fixed(const int64_t* values, uint64_t n, uint32_t dims,
                   int64_t* mins_out, uint8_t* widths_out,

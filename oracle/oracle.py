@@ -57,6 +57,9 @@ def lib():
         L.oc_write_lcp2.argtypes = [P(_Streams), P(u8p), P(C.c_size_t), P(C.c_uint64)]
         L.oc_write_lch1.argtypes = [P(_Streams), P(_RansParams), P(u8p), P(C.c_size_t), P(C.c_uint32)]
         L.oc_decode.argtypes = [u8p, C.c_size_t, P(P(C.c_double)), P(C.c_uint64), P(C.c_uint32)]
+        L.oc_write_lcg2.argtypes = [P(_Streams), C.c_float, P(C.c_float), P(u8p), P(C.c_size_t)]
+        L.oc_decode_lcg2.argtypes = [u8p, C.c_size_t, P(P(C.c_float)), P(C.c_uint64), P(C.c_uint32)]
+        L.oc_zstd_frame.argtypes = [u8p, C.c_size_t, P(u8p), P(C.c_size_t)]
         L.oc_pack_fixed.argtypes = [P(C.c_int64), C.c_uint64, C.c_uint32, P(C.c_int64), u8p, P(u8p), P(C.c_size_t)]
         L.oc_pack_tagged.argtypes = [P(C.c_int64), C.c_uint64, C.c_uint32, P(u8p), P(u8p), P(C.c_size_t),
                                      P(C.c_uint32)]
@@ -136,6 +139,17 @@ class Streams:
         b = _take_bytes(out, n.value)
         return b, [int(st[1]), int(st[2]), int(st[3]), int(st[4])]
 
+    def lcg2(self, eb=None, mins=None) -> bytes:
+        """LCG2 with Fixed components (the reference GPU compressor's container):
+        these streams under a float32 error bound and float32 minima (by default
+        the frame's, rounded to float32)."""
+        eb32 = np.float32(self.eb if eb is None else eb)
+        m = np.asarray(self.mins if mins is None else mins, dtype=np.float32)
+        mp = (C.c_float * self.dims)(*[float(x) for x in m])
+        out = C.POINTER(C.c_uint8)(); n = C.c_size_t()
+        _check(lib().oc_write_lcg2(C.byref(self._raw), C.c_float(float(eb32)), mp, C.byref(out), C.byref(n)))
+        return _take_bytes(out, n.value)
+
     def lch1(self, lanes_max=64, chunk_symbols=65536):
         out = C.POINTER(C.c_uint8)(); n = C.c_size_t(); kinds = (C.c_uint32 * 5)()
         rp = _RansParams(lanes_max, chunk_symbols)
@@ -182,6 +196,24 @@ def decode(data: bytes) -> np.ndarray:
         return np.ctypeslib.as_array(out, shape=(n.value * d.value,)).copy().reshape(n.value, d.value)
     finally:
         lib().oc_free(out)
+
+
+def decode_lcg2(data: bytes) -> np.ndarray:
+    """lcg2::decode of the reference GPU decoder: float32 (N, D)."""
+    out = C.POINTER(C.c_float)(); n = C.c_uint64(); d = C.c_uint32()
+    buf = _u8(data)
+    _check(lib().oc_decode_lcg2(buf, len(data), C.byref(out), C.byref(n), C.byref(d)))
+    try:
+        return np.ctypeslib.as_array(out, shape=(n.value * d.value,)).copy().reshape(n.value, d.value)
+    finally:
+        lib().oc_free(out)
+
+
+def zstd_frame(data: bytes) -> bytes:
+    """One Zstd frame (level 22) of data."""
+    out = C.POINTER(C.c_uint8)(); n = C.c_size_t()
+    _check(lib().oc_zstd_frame(_u8(data), len(data), C.byref(out), C.byref(n)))
+    return _take_bytes(out, n.value)
 
 
 def pack_fixed(values: np.ndarray):

@@ -1083,6 +1083,11 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
     Carver cC;
     cC.base = e->arena.base + size_a + size_b;
     auto X = carve_c(cC, out_bound);   // entropy-stage buffers (LCH1 only)
+    // phases A, B and C are consecutive, disjoint ranges of the arena: each
+    // carve hands out exactly the bytes its sizing pass counted
+    require(cB.off == size_b && (format != CHAIN_FMT_LCH1 || cC.off == size_c) &&
+                size_a + size_b + size_c <= e->arena.cap,
+            CHAIN_ERUNTIME, "arena carve differs from its sizing pass");
     // every launch after the planning sync: issued directly, or captured
     // once into a HIP graph and replayed for batches of the same launch shape
     uint64_t* const lb_expl = B.status + (uint64_t)kbytes * nwords * lb_sort;

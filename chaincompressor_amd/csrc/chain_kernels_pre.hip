@@ -9,6 +9,8 @@
 //   k_l2_stats / k_l2_split  L2 diff/keep/choice2           :1305-1454, 1456-1478
 // All work is 64-wide-wavefront native: ballots for ranking/compaction,
 // LDS-staged tiles of 4096 items, single-pass decoupled look-back scans.
+#include <type_traits>
+
 #include "chain_common.hpp"
 
 #pragma clang fp contract(off)
@@ -816,13 +818,15 @@ __global__ __launch_bounds__(CH_WG) void k_link(const KeyT* __restrict__ skey, c
 // items are a small fraction of the points, and the global sort and its
 // scattered link writes shrink with them.
 #define QL_ITEMS (CH_TILE / CH_WG)   // positions per thread (16)
+#define QL_DIGIT_MAX 10u             // widest digit of a tile sort pass
 //
 // The sort runs on a tile key: each dimension's block id minus the tile's
 // smallest, packed into the bits the tile's range of that dimension needs.
 // It is one-to-one on the tile's keys, so it groups exactly like the full key,
 // and a tile of a compact cluster needs a single pass of a few bits where the
-// full key needs two or more passes of eight (the ranking is bound by its
-// ballots, four instructions per key bit and item).
+// full key needs two or more passes of eight. Ranking costs one ballot per key
+// bit and item, so the passes rank exactly the tile key's bits, spread evenly
+// over as few passes of at most QL_DIGIT_MAX bits as that width allows.
 template <typename KeyT, int D>
 __global__ __launch_bounds__(CH_WG) void k_tile_link(
     const KeyT* __restrict__ keys, const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_begin,
@@ -832,11 +836,19 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
     __shared__ KeyT s_key[CH_TILE];   // tile keys
     __shared__ __attribute__((aligned(16))) uint16_t s_ia[CH_TILE];
     __shared__ __attribute__((aligned(16))) uint16_t s_ib[CH_TILE];
-    __shared__ uint16_t s_link[CH_TILE];   // in-tile links by point (< 4096; exits 0)
-    __shared__ uint32_t s_wcnt[CH_WG / 64][256];
+    // per digit of a sort pass: the counts of wavefronts 0 | 1 << 16 and
+    // 2 | 3 << 16, then each wavefront's first position of the digit; after
+    // the sort, the in-tile links by point (< 4096; exits 0)
+    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[2][1u << QL_DIGIT_MAX];
+    static_assert(sizeof(s_cnt) >= CH_TILE * sizeof(uint16_t), "links alias the digit counters");
+    uint16_t* const s_link = reinterpret_cast<uint16_t*>(&s_cnt[0][0]);
+    // per point: e such that its successor's distance is stored iff it is
+    // below 2^e (the chain decision of its block); by bit length of the
+    // largest block coordinate, for the frame
+    __shared__ uint8_t s_exp[CH_TILE];
+    __shared__ uint8_t s_le[72];
     __shared__ uint32_t s_tmp[4], s_tile;
     __shared__ KeyT s_rng[CH_WG / 64][2 * CH_MAXD];
-    __shared__ uint64_t s_lim[65];   // distance limit by bit length of the largest block coordinate
     __shared__ uint64_t s_excl;
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
@@ -863,7 +875,7 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
         kmask[d] = wd >= 8 * sizeof(KeyT) ? ~KeyT(0) : (KeyT)((KeyT(1) << wd) - 1);
     }
     if (l1) {
-        if (tid < 65) s_lim[tid] = chain_limit<D>(F, (int)tid, 12);
+        if (tid < 65) s_le[tid] = (uint8_t)__builtin_ctzll(chain_limit<D>(F, (int)tid, 12));
         {
             // the tile's keys, coalesced, every load issued first (past the
             // end: the tile's first key, not counted)
@@ -920,13 +932,23 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
                 csh[d] = acc;
                 acc += cw[d];
             }
+            // tile key and chain limit of every point
 #pragma unroll
             for (int k = 0; k < QL_ITEMS; ++k) {
                 const uint32_t j = k * CH_WG + tid;
-                KeyT ck = 0;
+                KeyT ck = 0, mb = 0;
 #pragma unroll
-                for (int d = 0; d < D; ++d) ck |= (((kv[k] >> ksh[d]) & kmask[d]) - bmin[d]) << csh[d];
-                if (j < cnt) s_key[j] = ck;
+                for (int d = 0; d < D; ++d) {
+                    const KeyT b = (kv[k] >> ksh[d]) & kmask[d];
+                    ck |= (b - bmin[d]) << csh[d];
+                    mb = b > mb ? b : mb;
+                }
+                const uint32_t lb = sizeof(KeyT) == 4 ? 32u - (uint32_t)__clz((uint32_t)mb)
+                                                      : (uint32_t)bitlen64((uint64_t)mb);
+                if (j < cnt) {
+                    s_key[j] = ck;
+                    s_exp[j] = s_le[lb];
+                }
                 s_ia[j] = (uint16_t)j;
             }
         }
@@ -934,79 +956,110 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
         // (also when no pass runs: every key of the tile equal)
         __syncthreads();
         const uint32_t cbits = csh[D - 1] + cw[D - 1];
-        // Stable LSD passes of up to 8 bits. Wavefront w ranks the positions
-        // [w * 1024, (w + 1) * 1024), item r of lane l at w * 1024 + r * 64 + l,
-        // in order: ballots give each lane the lanes of its digit, and the
-        // lowest of them adds the group's size to the wavefront's counter of
-        // that digit with one LDS atomic that returns the count before it. A
-        // wavefront's LDS operations complete in order, so consecutive rounds'
-        // atomics are issued back to back and their results read afterwards.
-        for (uint32_t shift = 0; shift < cbits; shift += 8) {
-            const uint32_t nb = cbits - shift < 8u ? cbits - shift : 8u;
-#pragma unroll
-            for (int k = 0; k < CH_WG / 64; ++k) s_wcnt[k][tid] = 0;
+        // Stable LSD passes. Wavefront w ranks the positions [w * 1024,
+        // (w + 1) * 1024) of the current order, item r of lane l at
+        // w * 1024 + r * 64 + l, round by round: one ballot per digit bit
+        // narrows each lane's mask to the lanes of its digit (four rounds at
+        // a time, for independent work between the ballots), and the lowest
+        // lane of each group adds the group's size to the wavefront's counter
+        // of that digit with one LDS atomic that returns the count before it.
+        // A wavefront's LDS operations complete in order, so consecutive
+        // rounds' atomics are issued back to back and their results read
+        // afterwards.
+        const uint32_t npass = (cbits + QL_DIGIT_MAX - 1u) / QL_DIGIT_MAX;
+        const uint32_t pbits = npass ? (cbits + npass - 1u) / npass : 1u;
+        const uint32_t hsh = 16u * (w & 1u);
+        uint32_t* const wcnt = s_cnt[w >> 1];
+        for (uint32_t shift = 0; shift < cbits; shift += pbits) {
+            const uint32_t nb = cbits - shift < pbits ? cbits - shift : pbits;
+            const uint32_t ndig = 1u << nb;
+            for (uint32_t k = tid; k < ndig; k += CH_WG) {
+                s_cnt[0][k] = 0;
+                s_cnt[1][k] = 0;
+            }
             __syncthreads();
-            // per item one word: position (12 bits) | digit << 12 | rank in its
-            // group of the round << 20 | the group's leader lane << 26, then
-            // position | digit << 12 | rank in the wavefront's digit << 20
-            uint32_t it[QL_ITEMS];
+            // per item: its point (12 bits) | digit << 12; then its rank among
+            // the group's lanes | the group's leader lane << 6, and finally its
+            // rank among the wavefront's items of the digit
+            uint32_t it[QL_ITEMS], rk[QL_ITEMS];
 #pragma unroll
             for (int r = 0; r < QL_ITEMS; ++r) {
                 const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
                 it[r] = src[p < cnt ? p : 0u];
             }
 #pragma unroll
-            for (int r = 0; r < QL_ITEMS; ++r) it[r] |= ((uint32_t)(s_key[it[r]] >> shift) & ((1u << nb) - 1u)) << 12;
-            // four rounds' atomics in flight at a time (register pressure)
+            for (int r = 0; r < QL_ITEMS; ++r) it[r] |= ((uint32_t)(s_key[it[r]] >> shift) & (ndig - 1u)) << 12;
 #pragma unroll
-            for (int h = 0; h < QL_ITEMS; h += QL_ITEMS / 4) {
-                uint32_t bf[QL_ITEMS / 4];
+            for (int h = 0; h < QL_ITEMS; h += 4) {
+                // peer masks in 32-bit halves: bit b of the digit set, the
+                // lane keeps the ballot, else its complement (t = 0 / ~0)
+                uint32_t plo[4], phi[4], nd[4];
 #pragma unroll
-                for (int rr = 0; rr < QL_ITEMS / 4; ++rr) {
+                for (int rr = 0; rr < 4; ++rr) {
+                    const uint32_t p = w * (CH_TILE / 4) + (h + rr) * 64 + lane;
+                    const uint64_t v = __builtin_amdgcn_ballot_w64(p < cnt);
+                    plo[rr] = (uint32_t)v;
+                    phi[rr] = (uint32_t)(v >> 32);
+                    nd[rr] = ~(it[h + rr] >> 12);
+                }
+                for (uint32_t b = 0; b < nb; ++b) {
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) {
+                        const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int32_t)nd[rr], (int32_t)b, 1);
+                        const uint64_t bb = __builtin_amdgcn_ballot_w64(t == 0u);
+                        plo[rr] &= (uint32_t)bb ^ t;
+                        phi[rr] &= (uint32_t)(bb >> 32) ^ t;
+                    }
+                }
+                uint32_t bf[4];
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) {
                     const int r = h + rr;
                     const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
                     const bool valid = p < cnt;
-                    const uint32_t dg = (it[r] >> 12) & 0xffu;
-                    uint64_t peers = __ballot(valid);
-                    // all 8 bits, unrolled: a bit above the pass's width is 0 in
-                    // every lane's digit, so it leaves the mask as it is
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        const uint64_t bb = __ballot((dg >> b) & 1u);
-                        peers &= ((dg >> b) & 1u) ? bb : ~bb;
-                    }
-                    const uint32_t prank = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
-                    const uint32_t lead = valid ? (uint32_t)__ffsll((unsigned long long)peers) - 1u : lane;
-                    it[r] |= (prank << 20) | (lead << 26);
+                    const uint32_t prank = __builtin_amdgcn_mbcnt_hi(phi[rr], __builtin_amdgcn_mbcnt_lo(plo[rr], 0u));
+                    const uint32_t lead = !valid ? lane : plo[rr] ? (uint32_t)__builtin_ctz(plo[rr])
+                                                                   : 32u + (uint32_t)__builtin_ctz(phi[rr]);
+                    rk[r] = prank | (lead << 6);
                     bf[rr] = 0;
-                    if (valid && prank == 0) bf[rr] = atomicAdd(&s_wcnt[w][dg], (uint32_t)__popcll(peers));
+                    if (valid && prank == 0)
+                        bf[rr] = atomicAdd(&wcnt[it[r] >> 12], (uint32_t)(__popc(plo[rr]) + __popc(phi[rr])) << hsh);
                 }
-                // every lane: its group's count before this round (from the
-                // leader) + its rank in the group
+                // every lane: its group's count before this round (the
+                // leader's half of the counter word) + its rank in the group
 #pragma unroll
-                for (int rr = 0; rr < QL_ITEMS / 4; ++rr) {
+                for (int rr = 0; rr < 4; ++rr) {
                     const int r = h + rr;
-                    const uint32_t wr = __shfl(bf[rr], (int)(it[r] >> 26), 64) + ((it[r] >> 20) & 63u);
-                    it[r] = (it[r] & 0xfffffu) | (wr << 20);
+                    rk[r] = ((__shfl(bf[rr], (int)(rk[r] >> 6), 64) >> hsh) & 0xffffu) + (rk[r] & 63u);
                 }
             }
             __syncthreads();
             {
-                // digit tid: its first position in the tile, then every wavefront's
-                const uint32_t a0 = s_wcnt[0][tid], a1 = s_wcnt[1][tid], a2 = s_wcnt[2][tid], a3 = s_wcnt[3][tid];
+                // digits [tid * per, tid * per + per): their first positions in
+                // the new order, then every wavefront's (waves 0 and 1, 2 and 3
+                // packed in 16-bit halves; positions are below 4096)
+                const uint32_t per = ndig > CH_WG ? ndig / CH_WG : 1u;
+                const uint32_t d0 = tid * per, d1 = d0 + per < ndig ? d0 + per : ndig;
+                uint32_t sum = 0;
+                for (uint32_t d = d0; d < d1; ++d) {
+                    const uint32_t c01 = s_cnt[0][d], c23 = s_cnt[1][d];
+                    sum += (c01 & 0xffffu) + (c01 >> 16) + (c23 & 0xffffu) + (c23 >> 16);
+                }
                 uint32_t tot;
-                const uint32_t st = block_excl_scan(a0 + a1 + a2 + a3, s_tmp, &tot);
-                s_wcnt[0][tid] = st;
-                s_wcnt[1][tid] = st + a0;
-                s_wcnt[2][tid] = st + a0 + a1;
-                s_wcnt[3][tid] = st + a0 + a1 + a2;
+                uint32_t st = block_excl_scan(sum, s_tmp, &tot);
+                for (uint32_t d = d0; d < d1; ++d) {
+                    const uint32_t c01 = s_cnt[0][d], c23 = s_cnt[1][d];
+                    const uint32_t a0 = c01 & 0xffffu, a1 = c01 >> 16, a2 = c23 & 0xffffu;
+                    s_cnt[0][d] = st | ((st + a0) << 16);
+                    s_cnt[1][d] = (st + a0 + a1) | ((st + a0 + a1 + a2) << 16);
+                    st += a0 + a1 + a2 + (c23 >> 16);
+                }
             }
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < QL_ITEMS; ++r) {
                 const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
-                if (p < cnt) dst[s_wcnt[w][(it[r] >> 12) & 0xffu] + (it[r] >> 20)] = (uint16_t)(it[r] & 0xfffu);
+                if (p < cnt) dst[((wcnt[it[r] >> 12] >> hsh) & 0xffffu) + rk[r]] = (uint16_t)(it[r] & 0xfffu);
             }
             __syncthreads();
             uint16_t* t = src;
@@ -1015,10 +1068,12 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
         }
         // Sorted positions [tid * 16, tid * 16 + 16) of this thread: links to
         // the next position of the same key (into s_link by point), run
-        // starts counted for the items.
+        // starts counted for the items. A full tile needs no position checks.
         const uint32_t s0 = tid * QL_ITEMS;
         uint32_t first_bits = 0;
-        if (s0 < cnt) {
+        auto link = [&](auto full_tag) {
+            constexpr bool FULL = decltype(full_tag)::value;
+            if (!FULL && s0 >= cnt) return;
             // positions >= cnt of src hold no sorted entry: never used as an index
             uint32_t ix[QL_ITEMS + 1];
             const uint4* v = reinterpret_cast<const uint4*>(src + s0);
@@ -1030,8 +1085,10 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
                 ix[2 * q + 1] = wd[q] >> 16;
             }
             ix[QL_ITEMS] = src[s0 + QL_ITEMS < cnt ? s0 + QL_ITEMS : 0u];
+            if (!FULL) {
 #pragma unroll
-            for (int k = 0; k <= QL_ITEMS; ++k) ix[k] = s0 + k < cnt ? ix[k] : ix[0];   // a valid point
+                for (int k = 0; k <= QL_ITEMS; ++k) ix[k] = s0 + k < cnt ? ix[k] : ix[0];   // a valid point
+            }
             KeyT kp = s_key[src[s0 > 0 ? s0 - 1 : 0u]];
             KeyT kc = s_key[ix[0]];
 #pragma unroll
@@ -1039,33 +1096,17 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
                 const uint32_t sp = s0 + k;
                 const KeyT kn = s_key[ix[k + 1]];
                 const bool has_next = sp + 1 < cnt && kn == kc;
-                const bool first = sp < cnt && (sp == 0 || kp != kc);
+                const bool first = (FULL || sp < cnt) && (sp == 0 || kp != kc);
                 first_bits |= (first ? 1u : 0u) << k;
-                uint32_t lb;
-                if (sizeof(KeyT) == 4) {
-                    uint32_t mb = 0;   // largest block coordinate
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        const uint32_t b = (uint32_t)(bmin[d] + ((kc >> csh[d]) & cmask[d]));
-                        mb = b > mb ? b : mb;
-                    }
-                    lb = 32u - (uint32_t)__clz(mb);
-                } else {
-                    uint64_t mb = 0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        const uint64_t b = (uint64_t)(bmin[d] + ((kc >> csh[d]) & cmask[d]));
-                        mb = b > mb ? b : mb;
-                    }
-                    lb = (uint32_t)bitlen64(mb);
-                }
                 const uint32_t dist = ix[k + 1] - ix[k];   // > 0 when has_next: the sort is stable
-                const bool rel = has_next && (uint64_t)dist < s_lim[lb];
-                if (sp < cnt) s_link[ix[k]] = (uint16_t)(rel ? dist : 0u);
+                const bool rel = has_next && (dist >> s_exp[ix[k]]) == 0u;
+                if (FULL || sp < cnt) s_link[ix[k]] = (uint16_t)(rel ? dist : 0u);
                 kp = kc;
                 kc = kn;
             }
-        }
+        };
+        if (cnt == CH_TILE) link(std::true_type{});
+        else link(std::false_type{});
         const uint32_t myruns = (uint32_t)__popc(first_bits);
         const uint32_t run0 = block_excl_scan(myruns, s_tmp, &nruns);
         // run r starts at dst[r] (the other buffer is free now)

@@ -401,7 +401,12 @@ def run_plain(args, eng):
     def one_eb(eb, steps, warmup, timed_regions):
         # the engine's own tuner picks the entropy chunking for this frame;
         # repeated compressions of the same shape replay a captured HIP graph
-        if args.rans_chunk:
+        if args.tuning:
+            lanes, chunk, l1 = (int(x) for x in args.tuning.split(","))
+            eng.set_autotune(False)
+            eng.set_tuning(lanes, chunk, l1)
+            tuning = eng.tuning()
+        elif args.rans_chunk:
             eng.set_autotune(False)
             eng.set_tuning(64, args.rans_chunk, 1)
             tuning = eng.tuning()

@@ -824,16 +824,17 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
             FrameOut* fo;
             uint64_t* rec_base;
             uint64_t* totals;   // [0] rec total, [1] diff total, [2] corr total
-            void* item_key;     // one item per (tile, distinct key): key, first and last point
-            uint32_t* item_entry;
-            uint32_t* item_exit;
+            void* item_key;     // one item per (tile, distinct key): key, and last << 32 | first point
+            uint64_t* item_pay;
             uint32_t* item_total;
         } b;
         b.keys_in = c.take<char>(T * key_bytes * key_words);
         b.ka = c.take<char>(T * key_bytes);
         b.kb = c.take<char>(T * key_bytes);
-        b.va = c.take<uint32_t>(T);
-        b.vb = c.take<uint32_t>(T);
+        // sort values: point indices (u32 ping-pong), or on the in-tile link
+        // path the items' (last, first) point pairs (u64, with item_pay)
+        b.va = c.take<uint32_t>(2 * T);
+        b.vb = b.va + T;
         b.rel = c.take<char>(T * D * rel_bytes);
         b.idrel = c.take<uint32_t>(T);
         b.corr_tmp_idx = c.take<uint64_t>(corr_slots);
@@ -860,8 +861,7 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
         // single-word keys only (the in-tile link path)
         const bool items = key_words == 1;
         b.item_key = c.take<char>(items ? T * key_bytes : 0);
-        b.item_entry = c.take<uint32_t>(items ? T : 0);
-        b.item_exit = c.take<uint32_t>(items ? T : 0);
+        b.item_pay = c.take<uint64_t>(items ? T : 0);
         b.item_total = c.take<uint32_t>(1);
         return b;
     };
@@ -1131,7 +1131,7 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
                     constexpr int DD = decltype(dd)::value;
                     hipLaunchKernelGGL((k_tile_link<KeyT, DD>), dim3(ntiles), dim3(CH_WG), 0, st, (const KeyT*)B.keys_in,
                                        A.fdev, A.tb, F, B.idrel, tk_qlink, lb_qlink, A.err, ntiles,
-                                       (KeyT*)B.item_key, B.item_entry, B.item_exit, B.item_total);
+                                       (KeyT*)B.item_key, B.item_pay, B.item_total);
                 });
                 tm.end();
             }
@@ -1141,6 +1141,10 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
             const uint32_t* vin = nullptr;
             KeyT* kout = (KeyT*)B.ka;
             uint32_t* vout = B.va;
+            // the items' payloads ping-pong between item_pay and the u64 view
+            // of va/vb
+            const uint64_t* pin = B.item_pay;
+            uint64_t* pout = reinterpret_cast<uint64_t*>(B.va);
             uint32_t done_passes = 0;
             for (uint32_t w = 0; w < nwords; ++w)
             for (uint32_t ps = 0; ps < word_passes[w]; ++ps) {
@@ -1167,12 +1171,22 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
                 tm.end();
                 tm.begin("sort_scatter", ps_all == 0 ? T * sizeof(KeyT) + T * (sizeof(KeyT) + 4)
                                                      : 2 * T * (sizeof(KeyT) + 4));
-                if (ps_all == 0)
-                    hipLaunchKernelGGL((k_sort_scatter<KeyT, true>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
-                                       vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets, n_dev);
-                else
-                    hipLaunchKernelGGL((k_sort_scatter<KeyT, false>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st,
-                                       kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets, n_dev);
+                if (item_link) {
+                    hipLaunchKernelGGL((k_sort_scatter<KeyT, uint64_t, false>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0,
+                                       st, kin, pin, kout, pout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets,
+                                       n_dev);
+                    const uint64_t* t = pin;
+                    pin = pout;
+                    pout = const_cast<uint64_t*>(t);
+                } else if (ps_all == 0) {
+                    hipLaunchKernelGGL((k_sort_scatter<KeyT, uint32_t, true>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0,
+                                       st, kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets,
+                                       n_dev);
+                } else {
+                    hipLaunchKernelGGL((k_sort_scatter<KeyT, uint32_t, false>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0,
+                                       st, kin, vin, kout, vout, T, shift, (uint32_t)sort_tiles, B.counts, B.offsets,
+                                       n_dev);
+                }
                 tm.end();
                 kin = kout;
                 vin = vout;
@@ -1185,7 +1199,7 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
                 constexpr int DD = decltype(dd)::value;
                 if (item_link)
                     hipLaunchKernelGGL((k_link_items<KeyT, DD>), dim3((unsigned)sort_tiles), dim3(CH_WG), 0, st, kin,
-                                       vin, B.item_total, B.item_entry, B.item_exit, A.fdev, A.ptb, F, B.idrel);
+                                       pin, B.item_total, A.fdev, A.ptb, F, B.idrel);
                 else if (nwords == 1)
                     hipLaunchKernelGGL((k_link<KeyT, DD>), dim3(LINK_XCDS * (unsigned)cdiv(cdiv(T, CH_TILE), LINK_XCDS)),
                                        dim3(CH_WG), 0, st, kin, vin, T, A.fdev, A.ptb, F, B.idrel);
@@ -1274,15 +1288,14 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
 
     const size_t kbytes_all = kbytes * nwords;
     // on the in-tile link path the sort and link regions work on the M items
-    // (key, first point, last point) counted on the device
+    // (key, last << 32 | first point) counted on the device
     auto set_item_bytes = [&](uint64_t M) {
         if (!item_link || !tm.on) return;
-        uint32_t pass = 0;
         for (size_t i = 0; i < tm.used / 2; ++i) {
             const std::string& nm = tm.names[i];
             if (nm == "sort_hist") tm.set_bytes((int)i, M * kbytes);
-            else if (nm == "sort_scatter") tm.set_bytes((int)i, pass++ == 0 ? M * kbytes + M * (kbytes + 4) : 2 * M * (kbytes + 4));
-            else if (nm == "link") tm.set_bytes((int)i, M * (2 * kbytes + 8) + 8 * M);
+            else if (nm == "sort_scatter") tm.set_bytes((int)i, 2 * M * (kbytes + 8));
+            else if (nm == "link") tm.set_bytes((int)i, M * (kbytes + 8) + 4 * M);
         }
     };
     auto set_pre_bytes = [&](uint64_t R, uint64_t NC) {

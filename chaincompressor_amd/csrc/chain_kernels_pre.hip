@@ -602,9 +602,9 @@ __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ ke
     counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = c;
 }
 
-template <typename KeyT, bool FIRST>
-__global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                        KeyT* __restrict__ kout, uint32_t* __restrict__ vout,
+template <typename KeyT, typename ValT, bool FIRST>
+__global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__ kin, const ValT* __restrict__ vin,
+                                                        KeyT* __restrict__ kout, ValT* __restrict__ vout,
                                                         uint64_t T, uint32_t shift, uint32_t ntiles,
                                                         const uint32_t* __restrict__ counts,
                                                         const uint32_t* __restrict__ offsets,
@@ -614,18 +614,18 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
     __shared__ uint32_t s_lstart[256], s_goff[256], s_tmp[4];
     __shared__ uint32_t s_wcnt[4][256];
     __shared__ KeyT s_key[CH_TILE];
-    __shared__ uint32_t s_val[CH_TILE];
+    __shared__ ValT s_val[CH_TILE];
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
     const uint64_t base = (uint64_t)blockIdx.x * CH_TILE;
     const uint32_t cnt_tile = (uint32_t)(T - base < CH_TILE ? T - base : CH_TILE);
     KeyT key[SORT_WAVE_ITEMS];
-    uint32_t val[SORT_WAVE_ITEMS];
+    ValT val[SORT_WAVE_ITEMS];
 #pragma unroll
     for (int r = 0; r < SORT_WAVE_ITEMS; ++r) {
         const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
         const uint64_t pc = base + (p < cnt_tile ? p : 0u);   // unconditional (the tile's first key)
         key[r] = kin[pc];
-        val[r] = FIRST ? (uint32_t)(base + p) : vin[pc];
+        val[r] = FIRST ? (ValT)(base + p) : vin[pc];
     }
     {
         const uint32_t c = counts[(uint64_t)tid * ntiles + blockIdx.x];
@@ -642,14 +642,20 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
         const uint32_t p = w * (CH_TILE / 4) + r * 64 + lane;
         const bool valid = p < cnt_tile;
         const uint32_t digit = (uint32_t)(key[r] >> shift) & 0xffu;
-        uint64_t peers = __ballot(valid);
+        // peer mask in 32-bit halves: a lane whose digit bit b is set keeps
+        // the ballot, else its complement (t = 0 / ~0)
+        const uint64_t v0 = __builtin_amdgcn_ballot_w64(valid);
+        uint32_t plo = (uint32_t)v0, phi = (uint32_t)(v0 >> 32);
+        const uint32_t nd = ~digit;
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            const uint64_t bb = __ballot((digit >> b) & 1u);
-            peers &= ((digit >> b) & 1u) ? bb : ~bb;
+            const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int32_t)nd, b, 1);
+            const uint64_t bb = __builtin_amdgcn_ballot_w64(t == 0u);
+            plo &= (uint32_t)bb ^ t;
+            phi &= (uint32_t)(bb >> 32) ^ t;
         }
-        const uint32_t prank = __builtin_amdgcn_mbcnt_hi((uint32_t)(peers >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)peers, 0u));
+        const uint64_t peers = ((uint64_t)phi << 32) | plo;
+        const uint32_t prank = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
         uint32_t before = 0;
         if (valid) before = s_wcnt[w][digit];
         wr[r] = before + prank;
@@ -831,8 +837,8 @@ template <typename KeyT, int D>
 __global__ __launch_bounds__(CH_WG) void k_tile_link(
     const KeyT* __restrict__ keys, const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_begin,
     uint32_t nframes, uint32_t* __restrict__ idrel, uint32_t* ticket, uint64_t* status, uint32_t* err,
-    uint32_t ntiles, KeyT* __restrict__ item_key, uint32_t* __restrict__ item_entry,
-    uint32_t* __restrict__ item_exit, uint32_t* __restrict__ item_total) {
+    uint32_t ntiles, KeyT* __restrict__ item_key, uint64_t* __restrict__ item_pay,
+    uint32_t* __restrict__ item_total) {
     __shared__ KeyT s_key[CH_TILE];   // tile keys
     __shared__ __attribute__((aligned(16))) uint16_t s_ia[CH_TILE];
     __shared__ __attribute__((aligned(16))) uint16_t s_ib[CH_TILE];
@@ -1133,8 +1139,7 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
 #pragma unroll
                 for (int d = 0; d < D; ++d) key |= (bmin[d] + ((ck >> csh[d]) & cmask[d])) << ksh[d];
                 item_key[base + r] = key;
-                item_entry[base + r] = (uint32_t)(F.pt_off + start + en);
-                item_exit[base + r] = (uint32_t)(F.pt_off + start + ex);
+                item_pay[base + r] = ((uint64_t)(F.pt_off + start + ex) << 32) | (uint32_t)(F.pt_off + start + en);
             }
         }
         // the links in index order, written out coalesced (exits 0: they are
@@ -1158,15 +1163,14 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
     }
 }
 
-// The cross-tile links: items sorted by key (stable), so item s + 1 of the
-// same key is the next tile holding it; the last point of item s links to
-// the first point of item s + 1 when both lie in one frame. Grid sized for
-// the largest item count; positions past the actual count exit.
+// The cross-tile links: items sorted by key (stable), each carrying its last
+// and first point (exit << 32 | entry), so item s + 1 of the same key is the
+// next tile holding it and the last point of item s links to the first point
+// of item s + 1 when both lie in one frame. Every read is coalesced. Grid
+// sized for the largest item count; positions past the actual count exit.
 template <typename KeyT, int D>
-__global__ __launch_bounds__(CH_WG) void k_link_items(const KeyT* __restrict__ skey, const uint32_t* __restrict__ sval,
+__global__ __launch_bounds__(CH_WG) void k_link_items(const KeyT* __restrict__ skey, const uint64_t* __restrict__ spay,
                                                       const uint32_t* __restrict__ item_total,
-                                                      const uint32_t* __restrict__ item_entry,
-                                                      const uint32_t* __restrict__ item_exit,
                                                       const FrameDev* __restrict__ fr,
                                                       const uint32_t* __restrict__ frame_pt_begin, uint32_t nframes,
                                                       uint32_t* __restrict__ idrel) {
@@ -1180,23 +1184,23 @@ __global__ __launch_bounds__(CH_WG) void k_link_items(const KeyT* __restrict__ s
     const uint64_t M = *item_total;
     const uint64_t s0 = (uint64_t)blockIdx.x * CH_TILE;
     if (s0 >= M) return;
-    uint32_t av[CH_ITEMS], bv[CH_ITEMS];
+    uint64_t pa[CH_ITEMS], pb[CH_ITEMS];
     KeyT kv[CH_ITEMS], kn[CH_ITEMS];
 #pragma unroll
     for (int q = 0; q < CH_ITEMS; ++q) {
         const uint64_t s = s0 + (uint64_t)q * CH_WG + threadIdx.x;
         // unconditional loads at valid positions (the tile's first one)
         const uint64_t sc = s < M ? s : s0, nc = s + 1 < M ? s + 1 : s0;
-        av[q] = sval[sc];
+        pa[q] = spay[sc];
         kv[q] = skey[sc];
-        bv[q] = sval[nc];
+        pb[q] = spay[nc];
         kn[q] = skey[nc];
     }
 #pragma unroll
     for (int q = 0; q < CH_ITEMS; ++q) {
         const uint64_t s = s0 + (uint64_t)q * CH_WG + threadIdx.x;
         if (s + 1 >= M || kn[q] != kv[q]) continue;
-        const uint32_t ex = item_exit[av[q]], en = item_entry[bv[q]];
+        const uint32_t ex = (uint32_t)(pa[q] >> 32), en = (uint32_t)pb[q];
         const uint32_t f = upper_find(offs, nframes, ex);
         const FrameDev& F = fr[f];
         if (en < offs[f + 1] && chain_relative<D>(F, (uint64_t)kv[q], (uint64_t)(en - ex))) idrel[ex] = en - ex;

@@ -1239,7 +1239,8 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
             hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
             tm.end();
             t_l2p = tm.begin("l2_split");
-            hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)rec_tiles_max), dim3(CH_WG), 0, st, B.blocks,
+            hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)std::min<uint64_t>(rec_tiles_max, L2S_WGS)), dim3(CH_WG),
+                               0, st, B.blocks,
                                &B.totals[0], F, B.rec_base, A.fdev, B.fo, lb_l2, tk_expl + 1, A.err, B.choice2, B.diff,
                                B.keep, &B.totals[1]);
         });

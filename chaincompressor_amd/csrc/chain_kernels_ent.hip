@@ -1541,44 +1541,71 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
         const uint64_t e = lookback_seg(status, tile, tile_begin[f], tot, err);
         if (threadIdx.x == 0) s_excl = e;
     }
-    const uint32_t nwords = (tot + 8 + 31) / 32 + 1;
-    for (uint32_t k = threadIdx.x; k < nwords; k += CH_WG) s_words[k] = 0;
     __syncthreads();
     const uint64_t b0 = s_excl, b1 = b0 + tot;
     const uint32_t lb0 = (uint32_t)(b0 & 7);
-    uint32_t o = lb0 + pre;
+    const uint64_t gb0 = b0 >> 3;
+    // The tile's bits are staged in words aligned to the output: staged byte
+    // a + i is output byte payload_off + gb0 + i, a = its offset in a 4-byte
+    // word. Stream bit o (MSB first) is bit 7 - (o & 7) of staged byte o >> 3;
+    // the words hold their bytes little-endian, so a value is placed as a
+    // big-endian bit field, byte-swapped, with at most three word ORs.
+    const uint64_t gbyte0 = C.payload_off + gb0;
+    const uint32_t a = (uint32_t)(gbyte0 & 3u);
+    const uint32_t nwords = (8 * a + lb0 + tot + 31) / 32 + 2;
+    for (uint32_t k = threadIdx.x; k < nwords; k += CH_WG) s_words[k] = 0;
+    __syncthreads();
+    uint32_t o = 8 * a + lb0 + pre;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (!tg[k]) continue;
         const uint32_t w = tg[k];
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
-            const uint64_t v = xv[k][d];
-            const uint32_t fb = o >> 3, lb = (o + w - 1) >> 3;
-            for (uint32_t B = fb; B <= lb; ++B) {
-                const uint32_t sb = o > 8 * B ? o : 8 * B;
-                const uint32_t se = (o + w) < (8 * B + 8) ? (o + w) : (8 * B + 8);
-                const uint32_t chunkv = (uint32_t)((v >> (o + w - se)) & ((1u << (se - sb)) - 1u));
-                const uint32_t byte = chunkv << (8 * B + 8 - se);
-                atomicOr(&s_words[B >> 2], byte << (8 * (B & 3)));
+            const uint64_t v = xv[k][d];   // below 2^w
+            const uint32_t wi = o >> 5, sh = o & 31u;
+            if (sh + w <= 64u) {
+                const uint64_t x = v << (64u - sh - w);
+                atomicOr(&s_words[wi], __builtin_bswap32((uint32_t)(x >> 32)));
+                if (sh + w > 32u) atomicOr(&s_words[wi + 1], __builtin_bswap32((uint32_t)x));
+            } else {
+                // 64 - sh high bits fill the rest of words wi, wi + 1; the
+                // low r bits start word wi + 2
+                const uint32_t r = sh + w - 64u;
+                const uint64_t x = v >> r;
+                atomicOr(&s_words[wi], __builtin_bswap32((uint32_t)(x >> 32)));
+                atomicOr(&s_words[wi + 1], __builtin_bswap32((uint32_t)x));
+                atomicOr(&s_words[wi + 2], __builtin_bswap32((uint32_t)(v << (32u - r))));
             }
             o += w;
         }
     }
     __syncthreads();
-    const uint64_t gb0 = b0 >> 3;
+    // staged bytes [a, a + nbytes): a partial first byte goes to head (merged
+    // with the previous tile's tail by k_tag_seams), a partial last byte to
+    // tail unless the component ends here; the bytes between are written as
+    // aligned words where whole, byte by byte at the two ends
     const uint32_t nbytes = (uint32_t)(((b1 + 7) >> 3) - gb0);
     const bool last = r0 + TAG_TILE >= C.n;
-    for (uint32_t Lb = threadIdx.x; Lb < nbytes; Lb += CH_WG) {
-        const uint8_t byte = (uint8_t)(s_words[Lb >> 2] >> (8 * (Lb & 3)));
-        if (Lb == 0 && lb0 != 0) {
-            head[tile] = byte;
-            head_pos[tile] = C.payload_off + gb0;
-        } else if (Lb == nbytes - 1 && (b1 & 7) != 0 && !last) {
-            tail[tile] = byte;
-        } else {
-            out[C.payload_off + gb0 + Lb] = byte;
-        }
+    const bool has_head = lb0 != 0, has_tail = (b1 & 7) != 0 && !last;
+    const uint8_t* sb = reinterpret_cast<const uint8_t*>(s_words);
+    if (threadIdx.x == 0 && has_head) {
+        head[tile] = sb[a];
+        head_pos[tile] = gbyte0;
+    }
+    if (threadIdx.x == 1 && has_tail) tail[tile] = sb[a + nbytes - 1];
+    const uint32_t i0 = a + (has_head ? 1u : 0u), i1 = a + nbytes - (has_tail ? 1u : 0u);
+    uint8_t* const ob = out + (gbyte0 - a);   // 4-byte aligned: staged byte i -> ob[i]
+    const uint32_t k0 = (i0 + 3) / 4, k1 = i1 / 4;
+    if (k0 < k1) {
+        uint32_t* const ow = reinterpret_cast<uint32_t*>(ob);
+        for (uint32_t k = k0 + threadIdx.x; k < k1; k += CH_WG) ow[k] = s_words[k];
+        // edge bytes: [i0, 4 k0) and [4 k1, i1)
+        if (threadIdx.x < 3 && i0 + threadIdx.x < 4 * k0) ob[i0 + threadIdx.x] = sb[i0 + threadIdx.x];
+        if (threadIdx.x >= 4 && threadIdx.x < 7 && 4 * k1 + threadIdx.x - 4 < i1)
+            ob[4 * k1 + threadIdx.x - 4] = sb[4 * k1 + threadIdx.x - 4];
+    } else {
+        for (uint32_t i = i0 + threadIdx.x; i < i1; i += CH_WG) ob[i] = sb[i];
     }
     __syncthreads();   // LDS reused by the next tile
     }

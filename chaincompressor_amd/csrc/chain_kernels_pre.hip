@@ -1527,6 +1527,7 @@ __global__ void k_l2_wrap(FrameOut* fo, uint32_t nframes, uint32_t* err) {
 // decoupled look-back over tiles; coalesced loads and stores.
 #define L2S_GROUP 4
 #define L2S_MAXF_LDS 1024u   // frame bases kept in LDS up to this many frames
+#define L2S_WGS 2048u        // workgroups of a split launch at most
 template <int D>
 __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
@@ -1543,13 +1544,15 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
     const bool lds = nframes <= L2S_MAXF_LDS;
     if (lds)
         for (uint32_t k = threadIdx.x; k < nframes; k += CH_WG) s_base[k] = frame_rec_base[k];
+    const uint64_t* bases = lds ? s_base : frame_rec_base;
+    // At most L2S_WGS workgroups, each taking tiles by ticket until they run
+    // past the record count (known only on the device): a frame with few
+    // explicit records costs no workgroups for the tiles it does not have.
+    for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
-    const uint64_t* bases = lds ? s_base : frame_rec_base;
     const uint32_t tile = s_tile;
-    // the grid is sized for the largest possible record count; tiles past the
-    // end hold no records and no later tile waits on them
-    if ((uint64_t)tile * CH_TILE >= R) return;
+    if ((uint64_t)tile * CH_TILE >= R) break;
     const uint64_t t0 = (uint64_t)tile * CH_TILE;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
@@ -1682,6 +1685,8 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
             }
             if (r == R - 1) *diff_total = dpos + (c ? 1u : 0u);
         }
+    }
+    __syncthreads();   // LDS reused by the next tile
     }
 }
 

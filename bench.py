@@ -129,8 +129,10 @@ def agg_regions(steps):
     return out
 
 
-def roofline(step_lists, K):
-    """The region with the most device time per step over all step lists."""
+def roofline(step_lists, K, profiled=False):
+    """The region with the most device time per step over all step lists.
+    `traffic` comes from the committed counter profile, which covers the
+    plain frame only (profiled=True); other workloads report it as null."""
     regions = {}
     for steps in step_lists:
         for k, v in agg_regions(steps).items():
@@ -154,7 +156,7 @@ def roofline(step_lists, K):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5) if achieved else None,
-        "traffic": traffic_from_profiles(name),
+        "traffic": traffic_from_profiles(name) if profiled else None,
     }
 
 
@@ -833,7 +835,7 @@ def main():
                                   "decompress": pipeline_fraction(m["d_steps"], K)},
             "model": dict(m["model"], key_bits=m["key_bits"], explicit_records=m["explicit_records"],
                           pre_entropy_bytes_Y=m["pre_entropy_bytes"]),
-            "roofline": roofline([m["c_steps"], m["d_steps"]], K),
+            "roofline": roofline([m["c_steps"], m["d_steps"]], K, profiled=True),
         })
         log(f"plain done at {time.perf_counter() - t_start:.0f} s")
     if wl in ("default", "batched", "all"):

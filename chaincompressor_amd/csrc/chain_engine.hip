@@ -530,14 +530,17 @@ static void fill32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
 
 // status (cdiv(n, CH_TILE) words) and ticket start at zero: cleared here
 // unless the caller cleared them already (clear = false)
+// items_dev: scan 256 x the tiles of that device-side item count (at most n)
 static void scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint64_t n, uint64_t* status,
-                     uint32_t* ticket, uint32_t* err, uint64_t* total, bool clear = true) {
+                     uint32_t* ticket, uint32_t* err, uint64_t* total, bool clear = true,
+                     const uint32_t* items_dev = nullptr) {
     const uint64_t tiles = cdiv(n, CH_TILE);
     if (clear) {
         fill64(st, status, tiles, 0);
         fill32(st, ticket, 1, 0);
     }
-    hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)tiles), dim3(CH_WG), 0, st, in, out, n, status, ticket, err, total);
+    hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)tiles), dim3(CH_WG), 0, st, in, out, n, status, ticket, err, total,
+                       items_dev);
 }
 
 // timed regions of the entropy stage whose algorithmic bytes are known only
@@ -1167,7 +1170,7 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
                 }
                 tm.begin("sort_scan", 2ull * 256 * sort_tiles * 4);
                 scan_u32(st, B.counts, B.offsets, 256 * sort_tiles, B.status + (uint64_t)ps_all * lb_sort,
-                         B.ticket + ps_all, A.err, nullptr, false);
+                         B.ticket + ps_all, A.err, nullptr, false, n_dev);
                 tm.end();
                 tm.begin("sort_scatter", ps_all == 0 ? T * sizeof(KeyT) + T * (sizeof(KeyT) + 4)
                                                      : 2 * T * (sizeof(KeyT) + 4));
@@ -1295,6 +1298,7 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
         for (size_t i = 0; i < tm.used / 2; ++i) {
             const std::string& nm = tm.names[i];
             if (nm == "sort_hist") tm.set_bytes((int)i, M * kbytes);
+            else if (nm == "sort_scan") tm.set_bytes((int)i, 2ull * 256 * cdiv(M, CH_TILE) * 4);
             else if (nm == "sort_scatter") tm.set_bytes((int)i, 2 * M * (kbytes + 8));
             else if (nm == "link") tm.set_bytes((int)i, M * (kbytes + 8) + 4 * M);
         }

@@ -140,14 +140,18 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 // generic single-pass exclusive scan of u32 -> u32 (used for radix offsets
 // and per-tile counts)
 
+// items_dev: the length is 256 x the tiles of a count of items that lives on
+// the device (a radix histogram's digit-major counts); tiles past it exit
 __global__ __launch_bounds__(CH_WG) void k_scan_u32(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                     uint64_t n, uint64_t* status, uint32_t* ticket, uint32_t* err,
-                                                    uint64_t* total_out) {
+                                                    uint64_t* total_out, const uint32_t* __restrict__ items_dev) {
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
+    if (items_dev) n = 256ull * (((uint64_t)*items_dev + CH_TILE - 1) / CH_TILE);
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
+    if ((uint64_t)tile * CH_TILE >= n) return;
     const uint64_t base = (uint64_t)tile * CH_TILE + (uint64_t)threadIdx.x * CH_ITEMS;
     // a thread's 16 items in four 16-byte loads and stores when they are all
     // in range and both arrays are 16-byte aligned (the arena's are)
@@ -574,8 +578,14 @@ template <typename KeyT>
 __global__ __launch_bounds__(CH_WG) void k_sort_hist(const KeyT* __restrict__ keys, uint64_t T, uint32_t shift,
                                                      uint32_t ntiles, uint32_t* __restrict__ counts,
                                                      const uint32_t* __restrict__ n_dev) {
-    // n_dev: the item count lives on the device (tiles past it count nothing)
-    if (n_dev) T = *n_dev;
+    // n_dev: the item count lives on the device. The counts are digit-major
+    // with a stride of the tiles the count needs (not the launch's), so the
+    // scan and the scatter touch only those; tiles past it exit.
+    if (n_dev) {
+        T = *n_dev;
+        ntiles = (uint32_t)((T + CH_TILE - 1) / CH_TILE);
+        if (blockIdx.x >= ntiles) return;
+    }
     // one histogram per wavefront: atomics of different wavefronts never
     // meet on a word; the four are summed at the end
     __shared__ uint32_t s_h[CH_WG / 64][256];
@@ -609,7 +619,10 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
                                                         const uint32_t* __restrict__ counts,
                                                         const uint32_t* __restrict__ offsets,
                                                         const uint32_t* __restrict__ n_dev) {
-    if (n_dev) T = *n_dev;   // the item count lives on the device
+    if (n_dev) {   // the item count lives on the device; the counts' stride follows it
+        T = *n_dev;
+        ntiles = (uint32_t)((T + CH_TILE - 1) / CH_TILE);
+    }
     if ((uint64_t)blockIdx.x * CH_TILE >= T) return;
     __shared__ uint32_t s_lstart[256], s_goff[256], s_tmp[4];
     __shared__ uint32_t s_wcnt[4][256];

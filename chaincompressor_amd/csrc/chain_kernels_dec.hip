@@ -422,6 +422,10 @@ struct DecTagged {
 #define DEC_TAG_TILE 1024u
 #define DEC_TAG_ITEMS (DEC_TAG_TILE / CH_WG)
 #define DEC_TAG_LDS (DEC_TAG_TILE * CH_MAXD * 63u / 8u + 16u)   // widest tile's bytes
+// staging words: the widest tile's bytes plus alignment and the windows' two
+// words past the end, or the tile's values as u64, whichever is more
+#define DEC_TAG_WORDS ((DEC_TAG_TILE * CH_MAXD * 2u > DEC_TAG_LDS / 4u + 4u) ? DEC_TAG_TILE * CH_MAXD * 2u \
+                                                                          : DEC_TAG_LDS / 4u + 4u)
 __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict__ in,
                                                       const DecTagged* __restrict__ jobs, uint32_t njobs,
                                                       const uint8_t* __restrict__ tags, uint64_t* status,
@@ -430,7 +434,8 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
                                                       uint32_t* err) {
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
-    __shared__ uint8_t s_pay[DEC_TAG_LDS];
+    // staged payload words, then the tile's decoded values
+    __shared__ uint32_t s_pay[DEC_TAG_WORDS];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -473,42 +478,67 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
     if (want > have) {
         if (threadIdx.x == 0) atomicOr(err, DERR_VALUE);
     }
-    // stage [B0, B0 + nst) of the payload; 16 loads in flight per thread
-    for (uint32_t q0 = 0; q0 < nst; q0 += CH_WG * 16) {
-        uint8_t v[16];
+    // Stage payload bytes [B0, B0 + nst) as aligned words: staged byte
+    // a + i is payload byte B0 + i. The last word may reach up to three bytes
+    // past the payload, still inside its container (a payload is always
+    // followed by the correction count); bits past the tile are never used.
+    const uint64_t gb = J.byte_off + B0;
+    const uint32_t a = (uint32_t)(gb & 3u);
+    const uint32_t* gw = reinterpret_cast<const uint32_t*>(in + (gb - a));
+    const uint32_t nw = (a + nst + 3) / 4;
+    for (uint32_t q0 = 0; q0 < nw; q0 += CH_WG * 8) {
+        uint32_t v[8];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
+        for (int j = 0; j < 8; ++j) {
             const uint32_t q = q0 + j * CH_WG + threadIdx.x;
-            v[j] = in[J.byte_off + B0 + (q < nst ? q : 0u)];
+            v[j] = gw[q < nw ? q : 0u];   // unconditional (the tile's first word)
         }
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
+        for (int j = 0; j < 8; ++j) {
             const uint32_t q = q0 + j * CH_WG + threadIdx.x;
-            if (q < nst) s_pay[q] = v[j];
+            if (q < nw) s_pay[q] = v[j];
         }
     }
+    if (threadIdx.x < 2) s_pay[nw + threadIdx.x] = 0;   // read by the last windows
     __syncthreads();
-    uint32_t bit = (uint32_t)(bit0 & 7) + pre;   // bit offset inside s_pay
+    // Cut each value from a big-endian window of two staged words (a third
+    // for a value that reaches past them): stream bit o is bit 7 - (o & 7)
+    // of staged byte o >> 3, and the words hold their bytes little-endian.
+    uint32_t pos = 8 * a + (uint32_t)(bit0 & 7) + pre;
+    const uint32_t dims = J.dims;
+    uint64_t val[DEC_TAG_ITEMS][CH_MAXD];
 #pragma unroll
     for (int k = 0; k < DEC_TAG_ITEMS; ++k) {
         const uint32_t w = tg[k];
-        if (!w) continue;
-        for (uint32_t d = 0; d < J.dims; ++d) {
-            const uint32_t B = bit >> 3, sh = bit & 7;
-            const uint32_t nb = (sh + w + 7) >> 3;
-            uint64_t acc_hi = 0, acc = 0;
-            for (uint32_t q = 0; q < nb; ++q) {
-                const uint8_t by = B + q < nst ? s_pay[B + q] : 0;
-                acc_hi = (acc_hi << 8) | (acc >> 56);
-                acc = (acc << 8) | by;
-            }
-            const uint32_t drop = 8 * nb - sh - w;
-            uint64_t v = drop ? ((acc >> drop) | (drop < 64 ? (acc_hi << (64 - drop)) : 0)) : acc;
-            v &= (w >= 64 ? ~0ull : ((1ull << w) - 1ull));
-            store_value(J.comp, J.dst + (r0 + k) * J.dims + d, v, idrel, diff, keep, choice, rel, rel8, err);
-            bit += w;
+#pragma unroll
+        for (int d = 0; d < CH_MAXD; ++d) {
+            val[k][d] = 0;
+            if (!w || (uint32_t)d >= dims) continue;
+            const uint32_t wi = pos >> 5, sh = pos & 31u;
+            const uint64_t x = ((uint64_t)__builtin_bswap32(s_pay[wi]) << 32) | __builtin_bswap32(s_pay[wi + 1]);
+            uint64_t v = (x << sh) >> (64u - w);
+            if (sh + w > 64u) v |= (uint64_t)(__builtin_bswap32(s_pay[wi + 2]) >> (96u - sh - w));
+            val[k][d] = v;
+            pos += w;
         }
     }
+    // the tile's values in LDS (the staged payload is no longer needed), then
+    // written in index order by consecutive threads
+    __syncthreads();
+    uint64_t* const s_val = reinterpret_cast<uint64_t*>(s_pay);
+    const uint32_t nrec = J.n - rfirst < DEC_TAG_TILE ? (uint32_t)(J.n - rfirst) : DEC_TAG_TILE;
+#pragma unroll
+    for (int k = 0; k < DEC_TAG_ITEMS; ++k) {
+        const uint32_t lr = threadIdx.x * DEC_TAG_ITEMS + k;
+        if (lr >= nrec) continue;
+#pragma unroll
+        for (int d = 0; d < CH_MAXD; ++d)
+            if ((uint32_t)d < dims) s_val[lr * dims + d] = val[k][d];
+    }
+    __syncthreads();
+    const uint64_t g0 = J.dst + rfirst * dims;
+    for (uint32_t e = threadIdx.x; e < nrec * dims; e += CH_WG)
+        store_value(J.comp, g0 + e, s_val[e], idrel, diff, keep, choice, rel, rel8, err);
 }
 
 // ------------------------------------------------------------------------

@@ -271,6 +271,10 @@ __device__ __forceinline__ void fixed_tile(const uint8_t* __restrict__ in, const
                                           OutT* __restrict__ out, uint64_t vmax, uint32_t verr, uint32_t* err) {
     const uint32_t dims = J.dims;
     const uint64_t rbase = (uint64_t)tile_in_job * DEC_FIX_RECS;
+    // In-tile offsets are 32-bit lane values over per-tile scalar bases:
+    // record rbase + lr, lr < DEC_FIX_RECS. rbase is a multiple of 1024, so
+    // the tile's first value of every dimension starts on a byte.
+    const uint32_t nr = J.n - rbase < DEC_FIX_RECS ? (uint32_t)(J.n - rbase) : DEC_FIX_RECS;
     uint32_t q0[CH_MAXD][DEC_FIX_ITEMS], q1[CH_MAXD][DEC_FIX_ITEMS];
     uint32_t slow = 0;   // bit d * DEC_FIX_ITEMS + j
 #pragma unroll
@@ -278,16 +282,20 @@ __device__ __forceinline__ void fixed_tile(const uint8_t* __restrict__ in, const
         if ((uint32_t)d >= dims) break;
         const uint32_t w = J.width[d];
         const bool narrow = w <= 33;
+        const uint64_t tb = J.byte_off[d] + ((rbase * w) >> 3);
+        const uint64_t ta = tb & ~3ull;
+        const uint32_t ts = (uint32_t)(tb & 3ull) * 8;
+        const uint64_t tail = (J.end[d] - 8) & ~3ull;
 #pragma unroll
         for (int j = 0; j < DEC_FIX_ITEMS; ++j) {
-            const uint64_t r = rbase + (uint64_t)j * CH_WG + threadIdx.x;
-            const uint64_t a4 = (J.byte_off[d] + ((r * w) >> 3)) & ~3ull;
-            const bool live = r < J.n;
+            const uint32_t lr = (uint32_t)j * CH_WG + threadIdx.x;
+            const uint64_t a4 = ta + 4ull * ((ts + lr * w) >> 5);
+            const bool live = lr < nr;
             const bool fast = live && narrow && a4 + 8 <= J.end[d];
             // unconditional loads (a lane that does not use them reads the
             // payload's last aligned word pair, which a component header
             // always precedes) so that every load of the tile is in flight
-            const uint32_t* q = reinterpret_cast<const uint32_t*>(in + (fast ? a4 : ((J.end[d] - 8) & ~3ull)));
+            const uint32_t* q = reinterpret_cast<const uint32_t*>(in + (fast ? a4 : tail));
             q0[d][j] = q[0];
             q1[d][j] = q[1];
             if (live && !fast) slow |= 1u << (d * DEC_FIX_ITEMS + j);
@@ -299,18 +307,25 @@ __device__ __forceinline__ void fixed_tile(const uint8_t* __restrict__ in, const
         if (y > vmax) atomicOr(err, verr);
         out[J.dst + r * dims + d] = (OutT)y;
     };
+    OutT* const tout = out + J.dst + rbase * dims;
 #pragma unroll
     for (int d = 0; d < CH_MAXD; ++d) {
         if ((uint32_t)d >= dims) break;
         const uint32_t w = J.width[d];
+        const uint64_t mn = J.minimum[d];
+        // every value of width w lands in range: the per-value checks can go
+        const uint64_t vm = w >= 64 ? ~0ull : (1ull << w) - 1ull;
+        const bool checked = !(mn <= vmax && vm <= vmax - mn && vm <= (uint64_t)INT64_MAX - mn);
+        const uint32_t ts = (uint32_t)((J.byte_off[d] + ((rbase * w) >> 3)) & 3ull) * 8;
 #pragma unroll
         for (int j = 0; j < DEC_FIX_ITEMS; ++j) {
-            const uint64_t r = rbase + (uint64_t)j * CH_WG + threadIdx.x;
-            if (r >= J.n || ((slow >> (d * DEC_FIX_ITEMS + j)) & 1u)) continue;
-            const uint64_t bit = r * w;
-            const uint32_t s = (uint32_t)((J.byte_off[d] + (bit >> 3)) & 3ull) * 8 + (uint32_t)(bit & 7);
+            const uint32_t lr = (uint32_t)j * CH_WG + threadIdx.x;
+            if (lr >= nr || ((slow >> (d * DEC_FIX_ITEMS + j)) & 1u)) continue;
+            const uint32_t s = (ts + lr * w) & 31u;
             const uint64_t x = ((uint64_t)__builtin_bswap32(q0[d][j]) << 32) | __builtin_bswap32(q1[d][j]);
-            put(r, d, J.minimum[d], (x << s) >> (64 - w));
+            const uint64_t v = (x << s) >> (64 - w);
+            if (checked) put(rbase + lr, d, mn, v);
+            else tout[lr * dims + d] = (OutT)(mn + v);
         }
     }
     while (slow) {

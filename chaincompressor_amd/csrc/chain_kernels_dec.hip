@@ -182,57 +182,92 @@ __global__ __launch_bounds__(64 * DEC_WAVES, 4) void k_dec_rans(const uint8_t* _
     if (loaded < stream_len) fetch(loaded);
     uint64_t pos = 0;
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (uint64_t t = 0; t < T; ++t) {
-        // keep at least DEC_HALF staged bytes ahead of pos (a step takes <= 256)
-        if (loaded - pos < DEC_HALF && loaded < stream_len) {
-            commit(loaded);
-            loaded += DEC_HALF;
-            if (loaded < stream_len) fetch(loaded);
-        }
-        const uint64_t k = t * L + lane;
-        const bool active = lane < L && k < Sc;
-        if (active) {
-            const uint32_t slot = x & (M - 1);
-            const uint32_t b = slot >> sh;
-            uint32_t s = s_bucket[b];
-            if (sh) {
-                uint32_t hi = s_bucket[b + 1];
-                while (s < hi) {
-                    const uint32_t mid = (s + hi + 1) >> 1;
-                    if (s_cum[mid] <= slot) s = mid; else hi = mid - 1;
+    const uint64_t dst0 = S.dst + ch.first;   // first output index of the chunk
+    // The step loop, specialised for the section's output (widths, or one
+    // component's values of its element type): symbol k of the chunk goes
+    // to dst0 + k. Alphabets are below 4096, so only the one-bit choices
+    // and one-byte offsets can be out of range.
+    auto steps = [&](auto kind) {
+        constexpr int K = decltype(kind)::value;
+        for (uint32_t t = 0; t < (uint32_t)T; ++t) {
+            // keep at least DEC_HALF staged bytes ahead of pos (a step takes <= 256)
+            if (loaded - pos < DEC_HALF && loaded < stream_len) {
+                commit(loaded);
+                loaded += DEC_HALF;
+                if (loaded < stream_len) fetch(loaded);
+            }
+            const uint32_t k = t * L + lane;
+            const bool active = lane < L && k < Sc;
+            if (active) {
+                const uint32_t slot = x & (M - 1);
+                const uint32_t b = slot >> sh;
+                uint32_t s = s_bucket[b];
+                if (sh) {
+                    uint32_t hi = s_bucket[b + 1];
+                    while (s < hi) {
+                        const uint32_t mid = (s + hi + 1) >> 1;
+                        if (s_cum[mid] <= slot) s = mid; else hi = mid - 1;
+                    }
+                }
+                const uint32_t c0 = s_cum[s];
+                x = (s_cum[s + 1] - c0) * (x >> pb) + slot - c0;
+                const uint64_t g = dst0 + k;
+                if constexpr (K < 0) {
+                    tags[g] = (uint8_t)s;
+                } else if constexpr (K == 0) {
+                    idrel[g] = s;
+                } else if constexpr (K == 1) {
+                    diff[g] = s;
+                } else if constexpr (K == 2) {
+                    keep[g] = s;
+                } else if constexpr (K == 3) {
+                    if (s > 1) atomicOr(err, DERR_VALUE);
+                    choice[g] = (uint8_t)s;
+                } else if constexpr (K == 4) {
+                    if (s > 0xffu) atomicOr(err, DERR_QUANT);
+                    static_cast<uint8_t*>(rel)[g] = (uint8_t)s;
+                } else {
+                    static_cast<uint32_t*>(rel)[g] = s;
                 }
             }
-            const uint32_t c0 = s_cum[s];
-            x = (s_cum[s + 1] - c0) * (x >> pb) + slot - c0;
-            const uint64_t q = ch.first + k;
-            if (S.comp < 0) tags[S.dst + q] = (uint8_t)s;
-            else store_value(S.comp, S.dst + q, s, idrel, diff, keep, choice, rel, rel8, err);
+            // Bytes this step reads. lower = 2^(pb+2), pb >= 12: x << 8 | byte
+            // < lower exactly when x < lower >> 8, whatever the byte, and two
+            // bytes fall short exactly when x < lower >> 16 (pb >= 14; below,
+            // only x = 0 would, which no valid stream reaches: x >= 4 after a
+            // step). So the count is known before any byte is read and the (at
+            // most three) planes are read together; a state still below lower
+            // afterwards is a corrupt stream.
+            const uint32_t kb = active ? (uint32_t)(x < lower) + (uint32_t)(x < (lower >> 8)) +
+                                             (uint32_t)(x < (lower >> 16))
+                                       : 0u;
+            const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
+            const uint32_t n0 = __popcll(m0), n1 = __popcll(m1);
+            if (kb > 0) {
+                auto byte_at = [&](uint64_t at) -> uint32_t {
+                    return at < stream_len ? (uint32_t)ring[(uint32_t)at & (DEC_RING - 1)] : 0u;
+                };
+                uint32_t v = byte_at(pos + __popcll(m0 & lt));
+                if (kb > 1) v = (v << 8) | byte_at(pos + n0 + __popcll(m1 & lt));
+                if (kb > 2) v = (v << 8) | byte_at(pos + n0 + n1 + __popcll(m2 & lt));
+                x = (x << (8 * kb)) | v;
+            }
+            pos += n0 + n1 + __popcll(m2);
+            if (active && x < lower) {
+                atomicOr(err, DERR_RANS);
+                x = lower;
+            }
         }
-        // Bytes this step reads. lower = 2^(pb+2), pb >= 12: x << 8 | byte
-        // < lower exactly when x < lower >> 8, whatever the byte, and two
-        // bytes fall short exactly when x < lower >> 16 (pb >= 14; below,
-        // only x = 0 would, which no valid stream reaches: x >= 4 after a
-        // step). So the count is known before any byte is read and the (at
-        // most three) planes are read together; a state still below lower
-        // afterwards is a corrupt stream.
-        const uint32_t kb = active ? (uint32_t)(x < lower) + (uint32_t)(x < (lower >> 8)) + (uint32_t)(x < (lower >> 16))
-                                   : 0u;
-        const uint64_t m0 = __ballot(kb > 0), m1 = __ballot(kb > 1), m2 = __ballot(kb > 2);
-        const uint32_t n0 = __popcll(m0), n1 = __popcll(m1);
-        if (kb > 0) {
-            auto byte_at = [&](uint64_t at) -> uint32_t {
-                return at < stream_len ? (uint32_t)ring[(uint32_t)at & (DEC_RING - 1)] : 0u;
-            };
-            uint32_t v = byte_at(pos + __popcll(m0 & lt));
-            if (kb > 1) v = (v << 8) | byte_at(pos + n0 + __popcll(m1 & lt));
-            if (kb > 2) v = (v << 8) | byte_at(pos + n0 + n1 + __popcll(m2 & lt));
-            x = (x << (8 * kb)) | v;
-        }
-        pos += n0 + n1 + __popcll(m2);
-        if (active && x < lower) {
-            atomicOr(err, DERR_RANS);
-            x = lower;
-        }
+    };
+    switch (S.comp) {
+        case -1: steps(std::integral_constant<int, -1>{}); break;
+        case 0: steps(std::integral_constant<int, 0>{}); break;
+        case 1: steps(std::integral_constant<int, 1>{}); break;
+        case 2: steps(std::integral_constant<int, 2>{}); break;
+        case 3: steps(std::integral_constant<int, 3>{}); break;
+        default:
+            if (rel8) steps(std::integral_constant<int, 4>{});
+            else steps(std::integral_constant<int, 5>{});
+            break;
     }
     if (lane < L && x != lower) atomicOr(err, DERR_RANS);
     if (lane == 0 && pos != stream_len) atomicOr(err, DERR_RANS);

@@ -1215,7 +1215,8 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
             t_explicit = tm.begin("explicit");
             with_dims(D, [&](auto dd) {
                 constexpr int DD = decltype(dd)::value;
-                hipLaunchKernelGGL((k_explicit<KeyT, DD>), dim3(ntiles), dim3(CH_WG), 0, st, B.idrel,
+                hipLaunchKernelGGL((k_explicit<KeyT, DD, expl_tpw<KeyT>()>), dim3(cdiv(ntiles, expl_tpw<KeyT>())), dim3(CH_WG), 0, st,
+                                   B.idrel,
                                    (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, lb_expl, tk_expl, A.err, B.blocks,
                                    B.fo, &B.totals[0], nwords, T);
             });

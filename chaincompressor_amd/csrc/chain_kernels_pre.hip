@@ -1278,9 +1278,14 @@ __global__ __launch_bounds__(CH_WG) void k_link_wide(const uint64_t* __restrict_
 
 // ------------------------------------------------------------------------
 // K5: ordered compaction of the explicit points' block ids (index order),
-// per-frame point tiles, one decoupled look-back over all tiles.
+// per-frame point tiles, one decoupled look-back over groups of TPW
+// consecutive tiles (one ticket and one look-back step per group, which
+// halves the serial part per point against one tile per workgroup).
 
-template <typename KeyT, int D>
+// tiles per workgroup: 4 with one-word 32-bit keys, 2 with 64-bit keys (registers)
+template <typename KeyT> constexpr int expl_tpw() { return sizeof(KeyT) == 4 ? 4 : 2; }
+
+template <typename KeyT, int D, int TPW>
 __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__ idrel, const KeyT* __restrict__ keys,
                                                     const FrameDev* __restrict__ fr,
                                                     const uint32_t* __restrict__ tile_begin, uint32_t nframes,
@@ -1288,58 +1293,103 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
                                                     uint32_t* err, uint64_t* __restrict__ blocks,
                                                     FrameOut* __restrict__ fo, uint64_t* __restrict__ rec_total,
                                                     uint32_t nwords, uint64_t T) {
-    // striped: item k of thread t is point k*256 + t of the tile (coalesced);
-    // record positions from ballots, scanned (row, wave) counts and a
-    // decoupled look-back over tiles
+    // striped: item k of thread t is point k*256 + t of its tile (coalesced);
+    // record positions from ballots, scanned (tile, row, wave) counts and a
+    // decoupled look-back over the groups
     constexpr uint32_t W = CH_WG / 64;
-    __shared__ uint32_t s_tile, s_off[CH_ITEMS * W];
+    constexpr uint32_t NSEG = TPW * CH_ITEMS * W;   // (tile, row, wave) segments
+    constexpr uint32_t SPL = (NSEG + 63) / 64;       // segments scanned per lane
+    __shared__ uint32_t s_grp, s_off[NSEG];
     __shared__ uint64_t s_excl;
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    if (threadIdx.x == 0) s_grp = atomicAdd(ticket, 1u);
     __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t f = upper_find(tile_begin, nframes, tile);
-    const FrameDev& F = fr[f];
-    const uint64_t start = (uint64_t)(tile - F.tile0) * CH_TILE;
+    const uint32_t grp = s_grp;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t lt = lanemask_lt();
-    uint32_t flags = 0, pre[CH_ITEMS];
-    // one-word keys of the tile are loaded before the look-back waits
-    KeyT kpre[CH_ITEMS];
-    uint32_t idr[CH_ITEMS];
+    uint32_t tl[TPW], fl[TPW];
+    uint64_t st[TPW], nn[TPW], po[TPW];
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const uint64_t i = start + k * CH_WG + threadIdx.x;
-        const bool live = i < F.n;
-        // unconditional loads at a valid index (the tile's first point)
-        const uint64_t ic = F.pt_off + (live ? i : start);
-        const uint32_t v = idrel[ic];
-        const KeyT kk = keys[ic];
-        idr[k] = live ? v : 1u;
-        kpre[k] = live && nwords == 1 ? kk : KeyT(0);
+    for (int u = 0; u < TPW; ++u) {   // a tile past the end has no live points
+        tl[u] = grp * TPW + u;
+        const uint32_t tc = tl[u] < ntiles ? tl[u] : ntiles - 1;
+        fl[u] = upper_find(tile_begin, nframes, tc);
+        const FrameDev& F = fr[fl[u]];
+        st[u] = (uint64_t)(tc - F.tile0) * CH_TILE;
+        nn[u] = tl[u] < ntiles ? F.n : 0;
+        po[u] = F.pt_off;
+    }
+    uint32_t flags[TPW];
+    // one-word keys of the tiles are loaded before the look-back waits
+    KeyT kpre[TPW][CH_ITEMS];
+    uint32_t idr[TPW][CH_ITEMS];
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const uint64_t i = st[u] + k * CH_WG + threadIdx.x;
+            const bool live = i < nn[u];
+            // unconditional loads at a valid index (the tile's first point)
+            const uint64_t ic = po[u] + (live ? i : st[u]);
+            const uint32_t v = idrel[ic];
+            const KeyT kk = keys[ic];
+            idr[u][k] = live ? v : 1u;
+            kpre[u][k] = live && nwords == 1 ? kk : KeyT(0);
+        }
     }
 #pragma unroll
-    for (int k = 0; k < CH_ITEMS; ++k) {
-        const bool ex = idr[k] == 0;
-        const uint64_t m = __ballot(ex);
-        pre[k] = __popcll(m & lt);
-        flags |= (ex ? 1u : 0u) << k;
-        if (lane == 0) s_off[k * W + wave] = __popcll(m);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const uint32_t v = s_off[threadIdx.x];
-        const uint32_t inc = wave_incl_scan(v);
-        s_off[threadIdx.x] = inc - v;
-        const uint32_t tot = __shfl(inc, 63, 64);
-        const uint64_t e = lookback(status, tile, tot, err);
-        if (threadIdx.x == 0) {
-            s_excl = e;
-            if (tile == F.tile0) fo[f].rec_base = e;
-            if (tile == ntiles - 1) *rec_total = e + tot;
+    for (int u = 0; u < TPW; ++u) {
+        flags[u] = 0;
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const bool ex = idr[u][k] == 0;
+            const uint64_t m = __ballot(ex);
+            flags[u] |= (ex ? 1u : 0u) << k;
+            if (lane == 0) s_off[(u * CH_ITEMS + k) * W + wave] = __popcll(m);
         }
     }
     __syncthreads();
-    if (flags) {
+    if (threadIdx.x < 64) {
+        // lane l scans segments [SPL*l, SPL*l + SPL)
+        uint32_t loc[SPL], sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < SPL; ++j) {
+            const uint32_t ix = SPL * threadIdx.x + j;
+            loc[j] = ix < NSEG ? s_off[ix] : 0u;
+            sum += loc[j];
+        }
+        const uint32_t inc = wave_incl_scan(sum);
+        uint32_t run = inc - sum;
+        uint32_t mine[SPL];   // exclusive offset of each of the lane's segments
+#pragma unroll
+        for (uint32_t j = 0; j < SPL; ++j) {
+            const uint32_t ix = SPL * threadIdx.x + j;
+            mine[j] = run;
+            if (ix < NSEG) s_off[ix] = run;
+            run += loc[j];
+        }
+        const uint32_t tot = __shfl(inc, 63, 64);
+        const uint64_t e = lookback(status, grp, tot, err);
+        if (threadIdx.x == 0) s_excl = e;
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            // the tile's first segment is entry (u*CH_ITEMS*W) % SPL of lane (u*CH_ITEMS*W) / SPL
+            const uint32_t seg = u * CH_ITEMS * W;
+            uint32_t sel = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < SPL; ++j) sel = j == seg % SPL ? mine[j] : sel;
+            const uint32_t tile_pre = __shfl(sel, (int)(seg / SPL), 64);
+            if (threadIdx.x == 0 && tl[u] < ntiles) {
+                if (tl[u] == fr[fl[u]].tile0) fo[fl[u]].rec_base = e + tile_pre;
+                if (tl[u] == ntiles - 1) *rec_total = e + tot;
+            }
+        }
+    }
+    __syncthreads();
+    const uint64_t base = s_excl;
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+        if (!flags[u]) continue;
+        const FrameDev& F = fr[fl[u]];
         // the key layout in registers: the record stores then need no wait
         uint32_t sh[D], kw[D];
         uint64_t mk[D];
@@ -1350,24 +1400,26 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
             sh[d] = F.key_shift[d];
             kw[d] = F.key_word[d];
         }
-        const uint64_t base = s_excl;
         if (nwords == 1) {
 #pragma unroll
             for (int k = 0; k < CH_ITEMS; ++k) {
-                if (!(flags & (1u << k))) continue;
-                const uint64_t rec = base + s_off[k * W + wave] + pre[k];
+                if (!(flags[u] & (1u << k))) continue;
+                // the active lanes are the row's explicit points: rank = active lanes below
+                const uint32_t pre = __popcll(__ballot(1) & lt);
+                const uint64_t rec = base + s_off[(u * CH_ITEMS + k) * W + wave] + pre;
 #pragma unroll
-                for (int d = 0; d < D; ++d) blocks[rec * D + d] = ((uint64_t)kpre[k] >> sh[d]) & mk[d];
+                for (int d = 0; d < D; ++d) blocks[rec * D + d] = ((uint64_t)kpre[u][k] >> sh[d]) & mk[d];
             }
         } else {
 #pragma unroll
             for (int k = 0; k < CH_ITEMS; ++k) {
-                if (!(flags & (1u << k))) continue;
-                const uint64_t i = start + k * CH_WG + threadIdx.x;
-                const uint64_t rec = base + s_off[k * W + wave] + pre[k];
+                if (!(flags[u] & (1u << k))) continue;
+                const uint64_t i = st[u] + k * CH_WG + threadIdx.x;
+                const uint32_t pre = __popcll(__ballot(1) & lt);
+                const uint64_t rec = base + s_off[(u * CH_ITEMS + k) * W + wave] + pre;
                 uint64_t kd[D];
 #pragma unroll
-                for (int d = 0; d < D; ++d) kd[d] = (uint64_t)keys[(uint64_t)kw[d] * T + F.pt_off + i];
+                for (int d = 0; d < D; ++d) kd[d] = (uint64_t)keys[(uint64_t)kw[d] * T + po[u] + i];
 #pragma unroll
                 for (int d = 0; d < D; ++d) blocks[rec * D + d] = (kd[d] >> sh[d]) & mk[d];
             }

@@ -215,16 +215,13 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
     __syncthreads();
     const uint32_t f_first = upper_find(tile_begin, F, tb0);
     AccT mn[DIMS], mx[DIMS];
-    uint64_t bits = 0;
-    uint32_t maxtag = 0, ovf = 0;
+    uint32_t ovf = 0;
     auto reset = [&]() {
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) {
             mn[d] = (AccT)~0ull;
             mx[d] = 0;
         }
-        bits = 0;
-        maxtag = 0;
         ovf = 0;
     };
     reset();
@@ -271,20 +268,28 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
                 mx[d] = b > mx[d] ? b : mx[d];
             }
         }
-        for (int k = 32; k >= 1; k >>= 1) {
-            bits += __shfl_xor(bits, k, 64);
-            const uint32_t a = __shfl_xor(maxtag, k, 64);
-            maxtag = a > maxtag ? a : maxtag;
-            ovf |= __shfl_xor(ovf, k, 64);
-        }
+        for (int k = 32; k >= 1; k >>= 1) ovf |= __shfl_xor(ovf, k, 64);
         if (lane_id() == 0) {
             for (int d = 0; d < DIMS; ++d) {
-                // a wavefront without records keeps the 64-bit identity (~0, 0)
-                s_red[w][2 * d] = bits ? (uint64_t)mn[d] : ~0ull;
+                // a wavefront without records (minimum above maximum) keeps the 64-bit identity (~0, 0)
+                s_red[w][2 * d] = mn[d] > mx[d] ? ~0ull : (uint64_t)mn[d];
                 s_red[w][2 * d + 1] = mx[d];
             }
-            s_red[w][2 * CH_MAXD] = bits;
-            s_red[w][2 * CH_MAXD + 1] = ((uint64_t)maxtag << 32) | ovf;
+            s_red[w][2 * CH_MAXD + 1] = ovf;
+        }
+        // Tagged bits and the widest width from the width histogram (widths
+        // 1..CS_PRIV_BINS in s_st, wider ones in s_th): wavefront 0, lane = width
+        if (tid < 64) {
+            const uint32_t t = tid;
+            const uint32_t cnt = t >= 1 && t <= CS_PRIV_BINS ? s_st[t - 1] : (t > CS_PRIV_BINS ? s_th[t] : 0u);
+            uint64_t bsum = (uint64_t)t * cnt * DIMS;
+            uint32_t tmax = cnt ? t : 0u;
+            for (int k = 32; k >= 1; k >>= 1) {
+                bsum += __shfl_xor(bsum, k, 64);
+                const uint32_t o = __shfl_xor(tmax, k, 64);
+                tmax = o > tmax ? o : tmax;
+            }
+            if (tid == 0) s_red[0][2 * CH_MAXD] = ((uint64_t)tmax << 56) | bsum;
         }
         __syncthreads();
         CompSlot& S = slots[f == f_first ? blockIdx.x : gridDim.x + f];
@@ -295,14 +300,8 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
             s_st[tid] = 0;
         }
         if (tid == CS_PRIV_BINS) {
-            uint64_t tb = 0;
-            uint32_t mt = 0, of = 0;
-            for (int k = 0; k < 4; ++k) {
-                tb += s_red[k][2 * CH_MAXD];
-                const uint32_t a = (uint32_t)(s_red[k][2 * CH_MAXD + 1] >> 32);
-                mt = a > mt ? a : mt;
-                of |= (uint32_t)s_red[k][2 * CH_MAXD + 1];
-            }
+            uint32_t of = 0;
+            for (int k = 0; k < 4; ++k) of |= (uint32_t)s_red[k][2 * CH_MAXD + 1];
             for (int d = 0; d < CH_MAXD; ++d) {
                 uint64_t a = ~0ull, b = 0;
                 if (d < DIMS)
@@ -313,8 +312,9 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
                 S.dmin[d] = a;
                 S.dmax[d] = b;
             }
-            S.bits = tb;
-            S.max_tag = mt;
+            const uint64_t bt = s_red[0][2 * CH_MAXD];
+            S.bits = bt & ((1ull << 56) - 1ull);
+            S.max_tag = (uint32_t)(bt >> 56);
             S.overflow = of;
         }
         const uint32_t ci_idx = c * F + f;
@@ -330,23 +330,106 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         __syncthreads();
         reset();
     };
-    uint32_t cur = 0xffffffffu, since_fold = 0;
-    for (uint32_t tile = tb0; tile < tb1; ++tile) {
+    // one record's values; the value histogram is skipped once the
+    // wavefront has seen a value >= RANS_VALUE_LIMIT in this frame (the
+    // component then has no value-rANS candidate and the histogram is unused)
+    const uint32_t pt_base = tid;
+    auto record = [&](const AccT (&v)[DIMS], bool vals_on) {
+        AccT any = 0;   // bit length of the OR = widest bit length
+#pragma unroll
+        for (int d = 0; d < DIMS; ++d) {
+            mn[d] = v[d] < mn[d] ? v[d] : mn[d];
+            mx[d] = v[d] > mx[d] ? v[d] : mx[d];
+            any |= v[d];
+            if (vals_on) {
+                if (v[d] < CS_PRIV_BINS)
+                    atomicAdd(&s_pv[(uint32_t)v[d] >> 1][pt_base], 1u << (((uint32_t)v[d] & 1u) << 4));
+                else if (v[d] < RANS_VALUE_LIMIT)
+                    atomicAdd(&s_vh[(uint32_t)v[d]], 1u);
+                else
+                    ovf = 1u;
+            }
+        }
+        const uint32_t wb = sizeof(AccT) == 4 ? 32u - (uint32_t)__clz((uint32_t)any) : (uint32_t)bitlen64(any);
+        const uint32_t tag = wb > 1u ? wb : 1u;
+        if (tag <= CS_PRIV_BINS)
+            atomicAdd(&s_pt[(tag - 1) >> 1][pt_base], 1u << (((tag - 1) & 1u) << 4));
+        else
+            atomicAdd(&s_th[tag], 1u);
+    };
+    // a full tile of values of at most 32 bits: thread t reads the CH_ITEMS
+    // consecutive records [t*CH_ITEMS, t*CH_ITEMS + CH_ITEMS) of the tile as
+    // whole words (the statistics do not depend on the order)
+    constexpr uint32_t BPT = CH_ITEMS * DIMS * sizeof(ValT);
+    constexpr uint32_t NW = BPT / 4;
+    constexpr bool WORDS = sizeof(ValT) <= 4 && BPT % 4 == 0 && NW <= 48;
+    // loads the words of the thread's chunk of the full tile starting at record r
+    auto load_chunk = [&](const ValT* src, uint64_t r, uint32_t (&wv)[NW + 1]) {
+        const uint8_t* p = (const uint8_t*)(src + (r + (uint64_t)tid * CH_ITEMS) * DIMS);
+        const uint32_t al = (uint32_t)(uintptr_t)p & 3u;   // 0 unless values are narrower than 4 bytes
+        const uint32_t* pa = (const uint32_t*)(p - al);
+#pragma unroll
+        for (uint32_t i = 0; i < NW; ++i) wv[i] = pa[i];
+        // the word past the chunk holds chunk bytes only when the chunk is unaligned
+        wv[NW] = (sizeof(ValT) < 4 && al) ? pa[NW] : 0u;
+        return al;
+    };
+    auto count_chunk = [&](uint32_t (&wv)[NW + 1], uint32_t al, bool vals_on) {
+        if (sizeof(ValT) < 4) {
+#pragma unroll
+            for (uint32_t i = 0; i < NW; ++i) wv[i] = __builtin_amdgcn_alignbyte(wv[i + 1], wv[i], al);
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < CH_ITEMS; ++q) {
+            AccT v[DIMS];
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) {
+                const uint32_t e = q * DIMS + d;
+                if (sizeof(ValT) == 1) v[d] = (wv[e >> 2] >> ((e & 3u) * 8u)) & 0xffu;
+                else if (sizeof(ValT) == 2) v[d] = (wv[e >> 1] >> ((e & 1u) * 16u)) & 0xffffu;
+                else v[d] = wv[e];
+            }
+            record(v, vals_on);
+        }
+    };
+    uint32_t cur = 0xffffffffu, acc = 0;   // acc: tiles counted since the last fold
+    for (uint32_t tile = tb0; tile < tb1;) {
         const uint32_t f = upper_find(tile_begin, F, tile);
+        const CompInst& C = ci[c * F + f];
+        const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
+        const uint64_t n = C.n;
+        // two full tiles of one frame are loaded together
+        const bool pair = WORDS && tile + 1 < tb1 && r0 + 2ull * CH_TILE <= n;
+        const uint32_t g = pair ? 2u : 1u;
         // every frame the range touches is published, even with no records
         if (f != cur) {
             if (cur != 0xffffffffu) flush(cur);
             cur = f;
-            since_fold = 0;
-        } else if (++since_fold == CS_FOLD_TILES) {
+            acc = 0;
+        } else if (acc + g > CS_FOLD_TILES) {
             fold();
-            since_fold = 0;
+            acc = 0;
         }
-        const CompInst& C = ci[c * F + f];
-        const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
-        const uint64_t n = C.n;
+        acc += g;
+        tile += g;
         if (r0 >= n) continue;
         const ValT* src = vals + C.base * DIMS;
+        const bool vals_on = __ballot(ovf != 0) == 0;
+        if (pair) {
+            uint32_t w0[NW + 1], w1[NW + 1];
+            const uint32_t a0 = load_chunk(src, r0, w0);
+            const uint32_t a1 = load_chunk(src, r0 + CH_TILE, w1);
+            count_chunk(w0, a0, vals_on);
+            count_chunk(w1, a1, vals_on);
+            continue;
+        }
+        if (WORDS && r0 + CH_TILE <= n) {
+            uint32_t w0[NW + 1];
+            const uint32_t a0 = load_chunk(src, r0, w0);
+            count_chunk(w0, a0, vals_on);
+            continue;
+        }
+        // striped: item j of thread t is record j*256 + t of the tile;
         // every load of the tile is issued before the first is used
         ValT x[CH_ITEMS][DIMS];
 #pragma unroll
@@ -359,28 +442,10 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         for (int j = 0; j < CH_ITEMS; ++j) {
             const uint64_t r = r0 + (uint64_t)j * CH_WG + tid;
             if (r >= n) continue;
-            AccT any = 0;   // bit length of the OR = widest bit length
+            AccT v[DIMS];
 #pragma unroll
-            for (int d = 0; d < DIMS; ++d) {
-                const AccT v = (AccT)x[j][d];
-                mn[d] = v < mn[d] ? v : mn[d];
-                mx[d] = v > mx[d] ? v : mx[d];
-                any |= v;
-                if (v < CS_PRIV_BINS)
-                    atomicAdd(&s_pv[(uint32_t)v >> 1][tid], 1u << (((uint32_t)v & 1u) << 4));
-                else if (v < RANS_VALUE_LIMIT)
-                    atomicAdd(&s_vh[(uint32_t)v], 1u);
-                else
-                    ovf = 1u;
-            }
-            const uint32_t wb = sizeof(AccT) == 4 ? 32u - (uint32_t)__clz((uint32_t)any) : (uint32_t)bitlen64(any);
-            const uint32_t tag = wb > 1u ? wb : 1u;
-            if (tag <= CS_PRIV_BINS)
-                atomicAdd(&s_pt[(tag - 1) >> 1][tid], 1u << (((tag - 1) & 1u) << 4));
-            else
-                atomicAdd(&s_th[tag], 1u);
-            bits += (uint64_t)DIMS * tag;
-            maxtag = tag > maxtag ? tag : maxtag;
+            for (int d = 0; d < DIMS; ++d) v[d] = (AccT)x[j][d];
+            record(v, vals_on);
         }
     }
     if (cur != 0xffffffffu) flush(cur);

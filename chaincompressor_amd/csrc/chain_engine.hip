@@ -550,8 +550,9 @@ struct EntropyTimes {
 };
 
 template <class X>
-static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint32_t F, uint32_t D, bool rel8, const FrameDev* fdev,
-                          FrameOut* fo, const uint32_t* idrel, const uint64_t* diff, const uint64_t* keep,
+static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint32_t F, uint32_t D, bool rel8,
+                          bool dk32, const FrameDev* fdev, FrameOut* fo, const uint32_t* idrel, const void* diff,
+                          const void* keep,
                           const uint8_t* choice2, const void* rel, const uint64_t* corr_idx, const double* corr_val,
                           const uint32_t* tb, const uint32_t* tb_host, uint32_t ntiles,
                           uint32_t ntiles1k, const uint64_t* comp_chunk_bound, const Tuning& rc, uint64_t stride,
@@ -564,8 +565,13 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
             fn(0, TypeTag<uint32_t>{}, IC<1>{}, (const void*)idrel);
-            fn(1, TypeTag<uint64_t>{}, IC<DD>{}, (const void*)diff);
-            fn(2, TypeTag<uint64_t>{}, IC<DD>{}, (const void*)keep);
+            if (dk32) {
+                fn(1, TypeTag<uint32_t>{}, IC<DD>{}, diff);
+                fn(2, TypeTag<uint32_t>{}, IC<DD>{}, keep);
+            } else {
+                fn(1, TypeTag<uint64_t>{}, IC<DD>{}, diff);
+                fn(2, TypeTag<uint64_t>{}, IC<DD>{}, keep);
+            }
             fn(3, TypeTag<uint8_t>{}, IC<1>{}, (const void*)choice2);
             if (rel8) fn(4, TypeTag<uint8_t>{}, IC<DD>{}, rel);
             else fn(4, TypeTag<uint32_t>{}, IC<DD>{}, rel);
@@ -611,16 +617,22 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
     et.encode = -1;
     if (chunk_bound) {
         et.encode = tm.begin("rans_encode");
-        with_dims(D, [&](auto dd) {
+        auto encode = [&](auto dd, auto rt, auto dkt) {
             constexpr int DD = decltype(dd)::value;
-            if (rel8)
-                hipLaunchKernelGGL((k_rans_encode<DD, uint8_t>), dim3((unsigned)chunk_bound), dim3(64), 0, st, idrel,
-                                   diff, keep, choice2, (const uint8_t*)rel, F, x.ci, x.sec, x.comp_chunk, x.pc,
-                                   x.scratch, stride, x.chunk_len, x.chunk_pos);
-            else
-                hipLaunchKernelGGL((k_rans_encode<DD, uint32_t>), dim3((unsigned)chunk_bound), dim3(64), 0, st,
-                                   idrel, diff, keep, choice2, (const uint32_t*)rel, F, x.ci, x.sec, x.comp_chunk,
-                                   x.pc, x.scratch, stride, x.chunk_len, x.chunk_pos);
+            using RelT = typename decltype(rt)::type;
+            using DKT = typename decltype(dkt)::type;
+            hipLaunchKernelGGL((k_rans_encode<DD, RelT, DKT>), dim3((unsigned)chunk_bound), dim3(64), 0, st, idrel,
+                               (const DKT*)diff, (const DKT*)keep, choice2, (const RelT*)rel, F, x.ci, x.sec,
+                               x.comp_chunk, x.pc, x.scratch, stride, x.chunk_len, x.chunk_pos);
+        };
+        with_dims(D, [&](auto dd) {
+            if (rel8) {
+                if (dk32) encode(dd, TypeTag<uint8_t>{}, TypeTag<uint32_t>{});
+                else encode(dd, TypeTag<uint8_t>{}, TypeTag<uint64_t>{});
+            } else {
+                if (dk32) encode(dd, TypeTag<uint32_t>{}, TypeTag<uint32_t>{});
+                else encode(dd, TypeTag<uint32_t>{}, TypeTag<uint64_t>{});
+            }
         });
         tm.end();
     }
@@ -1043,6 +1055,11 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
     const bool key64 = key_bits_max > 32 || nwords > 1;
     const bool rel8 = relmax <= 0xff;
     const size_t kbytes = key64 ? 8 : 4, rbytes = rel8 ? 1 : 4;
+    // diff and keep values are at most the wrap range (<= largest block id + 1):
+    // 32-bit streams when every frame's block ids stay below 2^31
+    bool dk32 = true;
+    for (uint32_t f = 0; f < F; ++f) dk32 = dk32 && fdev[f].max_bid_bits <= 31;
+    const size_t dkbytes = dk32 ? 4 : 8;
     uint32_t word_passes[CH_MAXD];
     for (uint32_t w = 0; w < CH_MAXD; ++w) word_passes[w] = (word_bits[w] + 7) / 8;
     if (nwords == 1) word_passes[0] = std::max<uint32_t>(1, (key_bits_max + 7) / 8);
@@ -1243,10 +1260,14 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
             hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
             tm.end();
             t_l2p = tm.begin("l2_split");
-            hipLaunchKernelGGL((k_l2_split<DD>), dim3((unsigned)std::min<uint64_t>(rec_tiles_max, L2S_WGS)), dim3(CH_WG),
-                               0, st, B.blocks,
-                               &B.totals[0], F, B.rec_base, A.fdev, B.fo, lb_l2, tk_expl + 1, A.err, B.choice2, B.diff,
-                               B.keep, &B.totals[1]);
+            auto split = [&](auto dkt) {
+                using DKT = typename decltype(dkt)::type;
+                hipLaunchKernelGGL((k_l2_split<DD, DKT>), dim3((unsigned)std::min<uint64_t>(rec_tiles_max, L2S_WGS)),
+                                   dim3(CH_WG), 0, st, B.blocks, &B.totals[0], F, B.rec_base, A.fdev, B.fo, lb_l2,
+                                   tk_expl + 1, A.err, B.choice2, (DKT*)B.diff, (DKT*)B.keep, &B.totals[1]);
+            };
+            if (dk32) split(TypeTag<uint32_t>{});
+            else split(TypeTag<uint64_t>{});
         });
         hipLaunchKernelGGL(k_frame_split_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[1],
                            &B.totals[0]);
@@ -1264,13 +1285,14 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
                            &B.totals[2]);
         tm.end();
         if (format == CHAIN_FMT_LCH1)
-            entropy_stage(st, tm, et, F, D, rel8, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel, B.corr_idx,
+            entropy_stage(st, tm, et, F, D, rel8, dk32, A.fdev, B.fo, B.idrel, B.diff, B.keep, B.choice2, B.rel,
+                          B.corr_idx,
                           B.corr_val, A.tb, tile_begin.data(), ntiles, ntiles1k, comp_chunk_bound, rc, stride, A.err,
                           out_bound, A.err + 1, X);
     };
     if (e->use_graphs) {
         GraphKey key;
-        key.add(format).add(dtype).add(layout).add(D).add(F).add(T).add(nwords).add(key64).add(rel8);
+        key.add(format).add(dtype).add(layout).add(D).add(F).add(T).add(nwords).add(key64).add(rel8).add(dk32);
         key.add(dpts).add(e->arena.base).add(ntiles).add(ntiles1k).add(rc.lanes_max).add(rc.chunk_symbols);
         key.add(item_link).add(corr_cap).add(out_bound);
         for (uint32_t w = 0; w < CH_MAXD; ++w) key.add(word_passes[w]);
@@ -1307,7 +1329,7 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
     auto set_pre_bytes = [&](uint64_t R, uint64_t NC) {
         tm.set_bytes(t_explicit, T * (4 + kbytes_all) + 8ull * D * R);
         tm.set_bytes(t_l2s, 8ull * D * R);
-        tm.set_bytes(t_l2p, 16ull * D * R + R);
+        tm.set_bytes(t_l2p, (8ull + dkbytes) * D * R + R);
         tm.set_bytes(t_corr, 32ull * NC + 8ull * ntiles);
     };
     uint32_t n_items = 0;
@@ -1337,7 +1359,7 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
                 NC += fo[f].ncorr;
             }
             set_pre_bytes(R, NC);
-            const uint64_t esz[5] = {4, 8, 8, 1, rbytes};
+            const uint64_t esz[5] = {4, dkbytes, dkbytes, 1, rbytes};
             uint64_t copied = 0, enc_all = 0;
             for (int c = 0; c < 5; ++c) {
                 uint64_t in_b = 0, enc_b = 0, pack_b = 0;
@@ -1428,14 +1450,35 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
     std::vector<double> h_cval(NC);
     HIPCHK(hipMemcpyAsync(h_idrel.data(), B.idrel, T * 4, hipMemcpyDeviceToHost, st));
     if (R) HIPCHK(hipMemcpyAsync(h_choice.data(), B.choice2, R, hipMemcpyDeviceToHost, st));
-    if (ND) HIPCHK(hipMemcpyAsync(h_diff.data(), B.diff, ND * D * 8, hipMemcpyDeviceToHost, st));
-    if (R - ND) HIPCHK(hipMemcpyAsync(h_keep.data(), B.keep, (R - ND) * D * 8, hipMemcpyDeviceToHost, st));
+    // 32-bit streams are copied into the upper halves and widened in place
+    if (ND)
+        HIPCHK(hipMemcpyAsync((char*)h_diff.data() + (dk32 ? ND * D * 4 : 0), B.diff, ND * D * dkbytes,
+                              hipMemcpyDeviceToHost, st));
+    if (R - ND)
+        HIPCHK(hipMemcpyAsync((char*)h_keep.data() + (dk32 ? (R - ND) * D * 4 : 0), B.keep, (R - ND) * D * dkbytes,
+                              hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(h_rel.data(), B.rel, T * D * rbytes, hipMemcpyDeviceToHost, st));
     if (NC) {
         HIPCHK(hipMemcpyAsync(h_cidx.data(), B.corr_idx, NC * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(h_cval.data(), B.corr_val, NC * 8, hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
+    if (dk32) {
+        // 32-bit element i (bytes 4n + 4i of the n-element vector) becomes
+        // element i (bytes 8i .. 8i + 7): in ascending order a write ends at or
+        // before the next 32-bit element still to be read (8i + 8 <= 4n + 4i + 4)
+        auto widen = [](std::vector<uint64_t>& v) {
+            const uint64_t n = v.size();
+            const char* base = (const char*)v.data() + n * 4;
+            for (uint64_t i = 0; i < n; ++i) {
+                uint32_t x;
+                std::memcpy(&x, base + 4 * i, 4);
+                v[i] = x;
+            }
+        };
+        widen(h_diff);
+        widen(h_keep);
+    }
 
     res.containers.resize(F);
     res.stats.resize(F);

@@ -1069,10 +1069,10 @@ __device__ inline void enc_chunk_any(const ValT* __restrict__ vals, uint32_t chu
 
 // every component's chunks in one launch: block b codes chunk b of the
 // batch (the chunks of component c are [comp_chunk[c], comp_chunk[c+1]))
-template <int D, typename RelT>
+template <int D, typename RelT, typename DKT>
 __global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__ idrel,
-                                                    const uint64_t* __restrict__ diff,
-                                                    const uint64_t* __restrict__ keep,
+                                                    const DKT* __restrict__ diff,
+                                                    const DKT* __restrict__ keep,
                                                     const uint8_t* __restrict__ choice2,
                                                     const RelT* __restrict__ rel, uint32_t F,
                                                     const CompInst* __restrict__ ci, const Section* __restrict__ sec,
@@ -1092,8 +1092,8 @@ __global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__
     __shared__ __attribute__((aligned(16))) uint8_t ring[ENC_RING];
     switch (c) {
     case 0: enc_chunk_any<uint32_t, 1>(idrel, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
-    case 1: enc_chunk_any<uint64_t, D>(diff, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
-    case 2: enc_chunk_any<uint64_t, D>(keep, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
+    case 1: enc_chunk_any<DKT, D>(diff, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
+    case 2: enc_chunk_any<DKT, D>(keep, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     case 3: enc_chunk_any<uint8_t, 1>(choice2, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     default: enc_chunk_any<RelT, D>(rel, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     }

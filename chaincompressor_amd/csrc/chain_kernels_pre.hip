@@ -1593,14 +1593,16 @@ __global__ void k_l2_wrap(FrameOut* fo, uint32_t nframes, uint32_t* err) {
 #define L2S_GROUP 4
 #define L2S_MAXF_LDS 1024u   // frame bases kept in LDS up to this many frames
 #define L2S_WGS 2048u        // workgroups of a split launch at most
-template <int D>
+// DKT: element type of the diff and keep streams (32-bit when every frame's
+// block ids, and so every wrapped difference, stay below 2^31)
+template <int D, typename DKT>
 __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
                                                     const uint64_t* __restrict__ frame_rec_base,
                                                     const FrameDev* __restrict__ fr, FrameOut* __restrict__ fo,
                                                     uint64_t* status, uint32_t* ticket, uint32_t* err,
-                                                    uint8_t* __restrict__ choice2, uint64_t* __restrict__ diff,
-                                                    uint64_t* __restrict__ keep, uint64_t* __restrict__ diff_total) {
+                                                    uint8_t* __restrict__ choice2, DKT* __restrict__ diff,
+                                                    DKT* __restrict__ keep, uint64_t* __restrict__ diff_total) {
     constexpr uint32_t W = CH_WG / 64;
     __shared__ uint64_t s_base[L2S_MAXF_LDS];
     __shared__ uint32_t s_tile, s_off[CH_ITEMS * W];
@@ -1731,7 +1733,7 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
             if (!c) {
                 const uint64_t kpos = r - dpos;
 #pragma unroll
-                for (int d = 0; d < D; ++d) keep[kpos * D + d] = cur[j][d];
+                for (int d = 0; d < D; ++d) keep[kpos * D + d] = (DKT)cur[j][d];
             } else {
                 int64_t wminc = wminc_a, wmaxc = wmaxc_a, wr = wr_a;
                 if (!one) {
@@ -1745,7 +1747,8 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
                     int64_t corr = (int64_t)prv[j][d] - (int64_t)cur[j][d];
                     if (corr < wminc) corr += wr;
                     else if (corr > wmaxc) corr -= wr;
-                    diff[dpos * D + d] = corr >= 0 ? ((uint64_t)corr << 1) : (((uint64_t)(-(corr + 1)) << 1) | 1ull);
+                    diff[dpos * D + d] =
+                        (DKT)(corr >= 0 ? ((uint64_t)corr << 1) : (((uint64_t)(-(corr + 1)) << 1) | 1ull));
                 }
             }
             if (r == R - 1) *diff_total = dpos + (c ? 1u : 0u);

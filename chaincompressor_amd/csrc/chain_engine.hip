@@ -2077,7 +2077,12 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     if (res.d_pool) X.in = const_cast<uint8_t*>(res.d_pool);
     Staging& sg = e->stage;
     // algorithmic bytes of every timed region, from the plan
-    const uint64_t comp_es[5] = {4, 8, 8, 1, rel8 ? 1u : 4u};
+    // diff and keep as u32 when every frame's wrap range lies in [0, 2^31)
+    bool dk32 = true;
+    for (const DecFrame& fd : P.frames)
+        dk32 = dk32 && fd.wrap_min >= 0 && fd.wrap_max < (int64_t)0x80000000ll && fd.range <= (1ull << 31);
+    const uint64_t dkes = dk32 ? 4 : 8;
+    const uint64_t comp_es[5] = {4, dkes, dkes, 1, rel8 ? 1u : 4u};
     auto es_of = [&](int32_t comp) -> uint64_t { return comp < 0 ? 1 : comp_es[comp]; };
     uint64_t b_rans = 0, b_fixed = 0, b_raw = 0, b_tagged = 0;
     for (const DecChunk& ch : P.chunks) b_rans += ch.bytes;
@@ -2113,18 +2118,21 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     tm.begin("rans_decode", b_rans);
     if (!P.chunks.empty())
         hipLaunchKernelGGL(k_dec_rans, dim3((unsigned)P.jobs.size()), dim3(64 * DEC_WAVES), 0, st, X.in, X.secs,
-                           X.chunks, X.jobs, X.cum, X.tags, X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
+                           X.chunks, X.jobs, X.cum, X.tags, X.idrel, (void*)X.diff, (void*)X.keep, (uint32_t)dk32,
+                           X.choice, X.rel, rel8, X.err);
     tm.end();
     if (P.fixed_tiles) {
         tm.begin("unpack_fixed", b_fixed);
         hipLaunchKernelGGL(k_dec_fixed, dim3((unsigned)P.fixed_tiles), dim3(CH_WG), 0, st, X.in, X.fixed,
-                           (uint32_t)P.fixed.size(), X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
+                           (uint32_t)P.fixed.size(), X.idrel, (void*)X.diff, (void*)X.keep, (uint32_t)dk32, X.choice,
+                           X.rel, rel8, X.err);
         tm.end();
     }
     if (P.raw_total) {
         tm.begin("unpack_raw", b_raw);
         hipLaunchKernelGGL(k_dec_raw, dim3(grid_for(P.raw_total)), dim3(CH_WG), 0, st, X.in, X.raws,
-                           (uint32_t)P.raws.size(), P.raw_total, X.idrel, X.diff, X.keep, X.choice, X.rel, rel8, X.err);
+                           (uint32_t)P.raws.size(), P.raw_total, X.idrel, (void*)X.diff, (void*)X.keep, (uint32_t)dk32,
+                           X.choice, X.rel, rel8, X.err);
         tm.end();
     }
     if (P.tag_tiles) {
@@ -2132,8 +2140,8 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         fill64(st, X.status, P.tag_tiles, 0);
         fill32(st, X.ticket, 1, 0);
         hipLaunchKernelGGL(k_dec_tagged, dim3((unsigned)P.tag_tiles), dim3(CH_WG), 0, st, X.in, X.tagged,
-                           (uint32_t)P.tagged.size(), X.tags, X.status, X.ticket, X.idrel, X.diff, X.keep, X.choice,
-                           X.rel, rel8, X.err);
+                           (uint32_t)P.tagged.size(), X.tags, X.status, X.ticket, X.idrel, (void*)X.diff,
+                           (void*)X.keep, (uint32_t)dk32, X.choice, X.rel, rel8, X.err);
         tm.end();
     }
     if (nrt) {
@@ -2144,18 +2152,23 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         fill32(st, X.flags, nrt + 1, 0);
         fill32(st, X.ticket, 1, 0);
         tm.end();
-        tm.begin("l2_inverse", P.R + 8ull * D * (P.ND + P.NK) + 8ull * D * P.R);
+        tm.begin("l2_inverse", P.R + dkes * D * (P.ND + P.NK) + 8ull * D * P.R);
         // 32-bit scan arithmetic when every frame's wrap range is at most 2^31
         uint64_t max_range = 0;
         for (const DecFrame& fd : P.frames) max_range = std::max(max_range, fd.range);
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
-            if (max_range <= (1ull << 31))
-                hipLaunchKernelGGL((k_dec_l2<DD, uint32_t>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice,
-                                   X.diff, X.keep, X.zoff, X.flags, X.aggv, X.ticket, X.compact, X.err);
-            else
-                hipLaunchKernelGGL((k_dec_l2<DD, uint64_t>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice,
-                                   X.diff, X.keep, X.zoff, X.flags, X.aggv, X.ticket, X.compact, X.err);
+            auto inverse = [&](auto vt, auto dkt) {
+                using V = typename decltype(vt)::type;
+                using DKT = typename decltype(dkt)::type;
+                hipLaunchKernelGGL((k_dec_l2<DD, V, DKT>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice,
+                                   (const DKT*)X.diff, (const DKT*)X.keep, X.zoff, X.flags, X.aggv, X.ticket,
+                                   X.compact, X.err);
+            };
+            // dk32 implies a range of at most 2^31
+            if (dk32) inverse(TypeTag<uint32_t>{}, TypeTag<uint32_t>{});
+            else if (max_range <= (1ull << 31)) inverse(TypeTag<uint32_t>{}, TypeTag<uint64_t>{});
+            else inverse(TypeTag<uint64_t>{}, TypeTag<uint64_t>{});
         });
         tm.end();
     }

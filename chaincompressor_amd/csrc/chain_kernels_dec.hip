@@ -47,13 +47,25 @@ struct DecFrame {
 
 // destination of a decoded value: component c of frame f, value index k
 // in-block offsets are stored as u8 when every frame of the batch has
-// p <= 256 (rel8), else as u32; larger values are format errors either way
-__device__ inline void store_value(int c, uint64_t gidx, uint64_t v, uint32_t* idrel, uint64_t* diff, uint64_t* keep,
-                                   uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
+// p <= 256 (rel8), else as u32; larger values are format errors either way.
+// Diff and keep values are stored as u32 when every frame's wrap range lies
+// in [0, 2^31) (dk32): a valid keep is at most wrap_max and a valid diff at
+// most the range, so a value of 2^32 or more is the L2 error it would be
+// found to be later.
+__device__ inline void store_dk(void* dst, uint64_t gidx, uint64_t v, uint32_t dk32, uint32_t* err) {
+    if (dk32) {
+        if (v >> 32) atomicOr(err, DERR_L2);
+        static_cast<uint32_t*>(dst)[gidx] = (uint32_t)v;
+    } else {
+        static_cast<uint64_t*>(dst)[gidx] = v;
+    }
+}
+__device__ inline void store_value(int c, uint64_t gidx, uint64_t v, uint32_t* idrel, void* diff, void* keep,
+                                   uint32_t dk32, uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
     switch (c) {
         case 0: if (v >> 32) atomicOr(err, DERR_VALUE); idrel[gidx] = (uint32_t)v; break;
-        case 1: diff[gidx] = v; break;
-        case 2: keep[gidx] = v; break;
+        case 1: store_dk(diff, gidx, v, dk32, err); break;
+        case 2: store_dk(keep, gidx, v, dk32, err); break;
         case 3: if (v > 1) atomicOr(err, DERR_VALUE); choice[gidx] = (uint8_t)v; break;
         default:
             if (rel8) {
@@ -108,7 +120,7 @@ __global__ __launch_bounds__(64 * DEC_WAVES, 4) void k_dec_rans(const uint8_t* _
                                                              const DecJob* __restrict__ jobs,
                                                              const uint32_t* __restrict__ cum_pool,
                                                              uint8_t* __restrict__ tags, uint32_t* idrel,
-                                                             uint64_t* diff, uint64_t* keep, uint8_t* choice,
+                                                             void* diff, void* keep, uint32_t dk32, uint8_t* choice,
                                                              void* rel, uint32_t rel8, uint32_t* err) {
     __shared__ uint32_t s_cum[RANS_VALUE_LIMIT + 1];
     __shared__ uint16_t s_bucket[DEC_BUCKETS + 1];
@@ -217,9 +229,13 @@ __global__ __launch_bounds__(64 * DEC_WAVES, 4) void k_dec_rans(const uint8_t* _
                 } else if constexpr (K == 0) {
                     idrel[g] = s;
                 } else if constexpr (K == 1) {
-                    diff[g] = s;
+                    static_cast<uint64_t*>(diff)[g] = s;
                 } else if constexpr (K == 2) {
-                    keep[g] = s;
+                    static_cast<uint64_t*>(keep)[g] = s;
+                } else if constexpr (K == 6) {   // symbols < 4096: no range check
+                    static_cast<uint32_t*>(diff)[g] = s;
+                } else if constexpr (K == 7) {
+                    static_cast<uint32_t*>(keep)[g] = s;
                 } else if constexpr (K == 3) {
                     if (s > 1) atomicOr(err, DERR_VALUE);
                     choice[g] = (uint8_t)s;
@@ -261,8 +277,14 @@ __global__ __launch_bounds__(64 * DEC_WAVES, 4) void k_dec_rans(const uint8_t* _
     switch (S.comp) {
         case -1: steps(std::integral_constant<int, -1>{}); break;
         case 0: steps(std::integral_constant<int, 0>{}); break;
-        case 1: steps(std::integral_constant<int, 1>{}); break;
-        case 2: steps(std::integral_constant<int, 2>{}); break;
+        case 1:
+            if (dk32) steps(std::integral_constant<int, 6>{});
+            else steps(std::integral_constant<int, 1>{});
+            break;
+        case 2:
+            if (dk32) steps(std::integral_constant<int, 7>{});
+            else steps(std::integral_constant<int, 2>{});
+            break;
         case 3: steps(std::integral_constant<int, 3>{}); break;
         default:
             if (rel8) steps(std::integral_constant<int, 4>{});
@@ -386,8 +408,9 @@ __device__ __forceinline__ void fixed_tile(const uint8_t* __restrict__ in, const
 }
 
 __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__ in, const DecFixed* __restrict__ jobs,
-                                                     uint32_t njobs, uint32_t* idrel, uint64_t* diff,
-                                                     uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
+                                                     uint32_t njobs, uint32_t* idrel, void* diff, void* keep,
+                                                     uint32_t dk32, uint8_t* choice, void* rel, uint32_t rel8,
+                                                     uint32_t* err) {
     __shared__ uint32_t s_tile0[DEC_FIX_LDS];
     __shared__ uint32_t s_job;
     const uint32_t tile = blockIdx.x;
@@ -416,8 +439,13 @@ __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__
     const uint32_t t = tile - J.tile0;
     switch (J.comp) {
         case 0: fixed_tile<uint32_t>(in, J, Jg, t, idrel, 0xffffffffull, DERR_VALUE, err); break;
-        case 1: fixed_tile<uint64_t>(in, J, Jg, t, diff, ~0ull, 0, err); break;
-        case 2: fixed_tile<uint64_t>(in, J, Jg, t, keep, ~0ull, 0, err); break;
+        case 1:
+        case 2: {
+            void* dst = J.comp == 1 ? diff : keep;
+            if (dk32) fixed_tile<uint32_t>(in, J, Jg, t, static_cast<uint32_t*>(dst), 0xffffffffull, DERR_L2, err);
+            else fixed_tile<uint64_t>(in, J, Jg, t, static_cast<uint64_t*>(dst), ~0ull, 0, err);
+            break;
+        }
         case 3: fixed_tile<uint8_t>(in, J, Jg, t, choice, 1, DERR_VALUE, err); break;
         default:
             if (rel8) fixed_tile<uint8_t>(in, J, Jg, t, static_cast<uint8_t*>(rel), 0xffull, DERR_QUANT, err);
@@ -435,8 +463,9 @@ struct DecRaw {
 };
 
 __global__ __launch_bounds__(CH_WG) void k_dec_raw(const uint8_t* __restrict__ in, const DecRaw* __restrict__ jobs,
-                                                   uint32_t njobs, uint64_t total, uint32_t* idrel, uint64_t* diff,
-                                                   uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8, uint32_t* err) {
+                                                   uint32_t njobs, uint64_t total, uint32_t* idrel, void* diff,
+                                                   void* keep, uint32_t dk32, uint8_t* choice, void* rel, uint32_t rel8,
+                                                   uint32_t* err) {
     for (uint64_t g = blockIdx.x * (uint64_t)CH_WG + threadIdx.x; g < total; g += (uint64_t)gridDim.x * CH_WG) {
         uint32_t lo = 0, hi = njobs;
         while (hi - lo > 1) {
@@ -449,7 +478,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_raw(const uint8_t* __restrict__ i
         uint64_t v = 0;
         for (int b = 0; b < 8; ++b) v |= (uint64_t)p[b] << (8 * b);
         if ((int64_t)v < 0) atomicOr(err, DERR_VALUE);
-        store_value(J.comp, J.dst + k, v, idrel, diff, keep, choice, rel, rel8, err);
+        store_value(J.comp, J.dst + k, v, idrel, diff, keep, dk32, choice, rel, rel8, err);
     }
 }
 
@@ -479,8 +508,8 @@ struct DecTagged {
 __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict__ in,
                                                       const DecTagged* __restrict__ jobs, uint32_t njobs,
                                                       const uint8_t* __restrict__ tags, uint64_t* status,
-                                                      uint32_t* ticket, uint32_t* idrel, uint64_t* diff,
-                                                      uint64_t* keep, uint8_t* choice, void* rel, uint32_t rel8,
+                                                      uint32_t* ticket, uint32_t* idrel, void* diff, void* keep,
+                                                      uint32_t dk32, uint8_t* choice, void* rel, uint32_t rel8,
                                                       uint32_t* err) {
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
@@ -588,7 +617,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
     __syncthreads();
     const uint64_t g0 = J.dst + rfirst * dims;
     for (uint32_t e = threadIdx.x; e < nrec * dims; e += CH_WG)
-        store_value(J.comp, g0 + e, s_val[e], idrel, diff, keep, choice, rel, rel8, err);
+        store_value(J.comp, g0 + e, s_val[e], idrel, diff, keep, dk32, choice, rel, rel8, err);
 }
 
 // ------------------------------------------------------------------------
@@ -787,11 +816,11 @@ __device__ inline Agg<V> l2_lookback(uint32_t tile, uint32_t seg, const Agg<V>& 
 // blocked reads (stride CH_ITEMS+1 words) hit distinct banks.
 #define L2_PAD(r) ((r) + ((r) / CH_ITEMS))
 #define L2_LDS_WORDS (CH_TILE + CH_TILE / CH_ITEMS)
-template <int D, typename V>
+template <int D, typename V, typename DKT>
 __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ fr, uint32_t F,
                                                   const uint32_t* __restrict__ rtile_begin,
-                                                  const uint8_t* __restrict__ choice, const uint64_t* __restrict__ diff,
-                                                  const uint64_t* __restrict__ keep,
+                                                  const uint8_t* __restrict__ choice, const DKT* __restrict__ diff,
+                                                  const DKT* __restrict__ keep,
                                                   const uint32_t* __restrict__ tile_zero_off,
                                                   uint32_t* flags, uint64_t* aggv, uint32_t* ticket,
                                                   uint64_t* __restrict__ compact, uint32_t* err) {
@@ -854,11 +883,11 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             const uint64_t idx = z ? kix : Fr.rec_base + r - kix;
             const bool inb = r < Fr.nrec && (z ? kix >= Fr.keep_base && kix < Fr.keep_base + Fr.nkeep
                                                : idx >= Fr.diff_base && idx < Fr.diff_base + Fr.ndiff);
-            const uint64_t* src = z ? keep : diff;
+            const DKT* src = z ? keep : diff;
             const uint64_t ic = inb ? idx : 0ull;   // record 0 of the array always exists
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                const uint64_t v = src[ic * D + d];
+                const uint64_t v = (uint64_t)src[ic * D + d];
                 raw[j][d] = inb ? v : (z ? (uint64_t)Fr.wrap_min : 0ull);
             }
         }

@@ -1457,7 +1457,11 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
     // covering the tile's range are readable), then cut from LDS; the
     // workgroup branches on the array's alignment as a whole
     constexpr uint32_t STAGE_WORDS = sizeof(ValT) == 1 ? CH_TILE * DIMS / 4 + 2 : 1;
+    // byte values (width <= 8): every dimension's packed bytes of the tile
+    // (an even word count keeps every dimension's run 8-byte aligned)
+    constexpr uint32_t OUT_WORDS = sizeof(ValT) == 1 ? CH_TILE * 8 / 32 + 2 : 2;
     __shared__ uint32_t s_w[STAGE_WORDS];
+    __shared__ __attribute__((aligned(16))) uint32_t s_o[sizeof(ValT) == 1 ? DIMS : 1][OUT_WORDS];
     const uint8_t* s_b = nullptr;
     if constexpr (sizeof(ValT) == 1) {
         if ((reinterpret_cast<uintptr_t>(vals) & 3u) == 0) {
@@ -1478,6 +1482,94 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
             }
             __syncthreads();
             s_b = reinterpret_cast<const uint8_t*>(s_w) + (b0 - 4 * w0);
+            // Group g (records 8g .. 8g + 7 of the tile) reads its 8 * DIMS
+            // bytes as realigned LDS words; per dimension its eight values
+            // fill exactly w bytes (w <= 8), big-endian bit order, which go to
+            // that dimension's stream at byte g * w of the tile's run. Each
+            // run then leaves as whole 4-byte output words (consecutive LDS
+            // words joined at the run's byte offset in its first word), with
+            // the bytes of the two edge words, which neighbouring tiles also
+            // touch, written singly.
+            const uint32_t sb0 = (uint32_t)(b0 - 4 * w0);
+            constexpr uint32_t GW = (8 * DIMS + 3) / 4 + 1;   // words a group's bytes can span
+            uint32_t wd[DIMS];
+            uint64_t mnd[DIMS];
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) {
+                wd[d] = C.width[d];
+                mnd[d] = C.dmin[d];
+            }
+#pragma unroll
+            for (int h = 0; h < PACK_GROUPS / CH_WG; ++h) {
+                const uint32_t g = h * CH_WG + threadIdx.x;
+                const uint32_t la = 8u * g;
+                if (r0 + la >= r1) continue;
+                const uint32_t nvals = r1 - r0 - la < 8 ? (uint32_t)(r1 - r0 - la) : 8u;
+                const uint32_t q = sb0 + la * DIMS;
+                const uint32_t qs = q & 3u;
+                uint32_t rw[GW];
+#pragma unroll
+                for (uint32_t i = 0; i < GW; ++i) rw[i] = s_w[(q >> 2) + i];
+#pragma unroll
+                for (uint32_t i = 0; i + 1 < GW; ++i) rw[i] = __builtin_amdgcn_alignbyte(rw[i + 1], rw[i], qs);
+#pragma unroll
+                for (int d = 0; d < DIMS; ++d) {
+                    const uint32_t w = wd[d], m32 = (uint32_t)mnd[d];
+                    uint8_t* const so = reinterpret_cast<uint8_t*>(s_o[d]) + g * w;
+                    if (w <= 4) {
+                        uint32_t acc = 0;
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; ++k) {
+                            const uint32_t e = k * DIMS + d;
+                            const uint32_t v = k < nvals ? ((rw[e >> 2] >> (8 * (e & 3u))) & 0xffu) - m32 : 0u;
+                            acc = (acc << w) | v;
+                        }
+                        // 8 w bits, the first value in the top bits
+                        if (w == 4) *reinterpret_cast<uint32_t*>(so) = __builtin_bswap32(acc);
+                        else if (w == 2) *reinterpret_cast<uint16_t*>(so) = __builtin_bswap16((uint16_t)acc);
+                        else if (w == 1) *so = (uint8_t)acc;
+                        else
+                            for (uint32_t b = 0; b < w; ++b) so[b] = (uint8_t)(acc >> (8 * (w - 1 - b)));
+                    } else {
+                        uint64_t acc = 0;
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; ++k) {
+                            const uint32_t e = k * DIMS + d;
+                            const uint32_t v = k < nvals ? ((rw[e >> 2] >> (8 * (e & 3u))) & 0xffu) - m32 : 0u;
+                            acc = (acc << w) | v;
+                        }
+                        if (w == 8) *reinterpret_cast<uint64_t*>(so) = __builtin_bswap64(acc);
+                        else
+                            for (uint32_t b = 0; b < w; ++b) so[b] = (uint8_t)(acc >> (8 * (w - 1 - b)));
+                    }
+                }
+            }
+            __syncthreads();
+            uint64_t dof = doff;
+#pragma unroll
+            for (int d = 0; d < DIMS; ++d) {
+                const uint32_t w = wd[d];
+                const uint64_t o0 = dof + (r0 * w) / 8;   // r0 is a multiple of 8: a whole byte
+                const uint32_t nb = (uint32_t)(((r1 - r0) * w + 7) / 8);
+                const uint32_t a = (uint32_t)(o0 & 3u);
+                const uint32_t* const L = s_o[d];
+                const uint8_t* const Lb = reinterpret_cast<const uint8_t*>(L);
+                uint8_t* const ob = out + (o0 - a);   // 4-byte aligned: output byte 4k + i = run byte 4k + i - a
+                // whole words k in [k0, k1): run bytes [4k - a, 4k - a + 4) inside [0, nb)
+                const uint32_t k0 = a ? 1u : 0u, k1 = (nb + a) / 4;
+                if (k0 < k1) {
+                    uint32_t* const ow = reinterpret_cast<uint32_t*>(ob);
+                    for (uint32_t k = k0 + threadIdx.x; k < k1; k += CH_WG)
+                        ow[k] = a ? __builtin_amdgcn_alignbyte(L[k], L[k - 1], 4 - a) : L[k];
+                    if (threadIdx.x < 3 && threadIdx.x + a < 4 * k0) ob[a + threadIdx.x] = Lb[threadIdx.x];
+                    if (threadIdx.x >= 4 && threadIdx.x < 8 && 4 * k1 + threadIdx.x - 4 < nb + a)
+                        ob[4 * k1 + threadIdx.x - 4] = Lb[4 * k1 + threadIdx.x - 4 - a];
+                } else {
+                    for (uint32_t i = threadIdx.x; i < nb; i += CH_WG) ob[a + i] = Lb[i];
+                }
+                dof += C.fix_bytes[d];
+            }
+            return;
         }
     }
 #pragma unroll

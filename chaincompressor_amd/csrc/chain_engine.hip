@@ -1232,10 +1232,15 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
             t_explicit = tm.begin("explicit");
             with_dims(D, [&](auto dd) {
                 constexpr int DD = decltype(dd)::value;
-                hipLaunchKernelGGL((k_explicit<KeyT, DD, expl_tpw<KeyT>()>), dim3(cdiv(ntiles, expl_tpw<KeyT>())), dim3(CH_WG), 0, st,
-                                   B.idrel,
-                                   (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, lb_expl, tk_expl, A.err, B.blocks,
-                                   B.fo, &B.totals[0], nwords, T);
+                auto expl = [&](auto bt) {
+                    using BT = typename decltype(bt)::type;
+                    hipLaunchKernelGGL((k_explicit<KeyT, DD, expl_tpw<KeyT>(), BT>),
+                                       dim3(cdiv(ntiles, expl_tpw<KeyT>())), dim3(CH_WG), 0, st, B.idrel,
+                                       (const KeyT*)B.keys_in, A.fdev, A.tb, F, ntiles, lb_expl, tk_expl, A.err,
+                                       (BT*)B.blocks, B.fo, &B.totals[0], nwords, T);
+                };
+                if (dk32) expl(TypeTag<uint32_t>{});
+                else expl(TypeTag<uint64_t>{});
             });
             hipLaunchKernelGGL(k_frame_counts, dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, &B.totals[0]);
             tm.end();
@@ -1254,16 +1259,19 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
         const uint64_t rec_tiles_max = cdiv(T, CH_TILE);
         with_dims(D, [&](auto dd) {
             constexpr int DD = decltype(dd)::value;
-            hipLaunchKernelGGL((k_l2_stats<DD>), dim3(std::min<unsigned>(grid_for(T, CH_WG * CH_ITEMS), 1024)),
-                               dim3(CH_WG), 0, st, B.blocks,
-                               &B.totals[0], F, B.rec_base, B.fo);
+            if (dk32)
+                hipLaunchKernelGGL((k_l2_stats<DD, uint32_t>), dim3(std::min<unsigned>(grid_for(T, CH_WG * CH_ITEMS), 1024)),
+                                   dim3(CH_WG), 0, st, (const uint32_t*)B.blocks, &B.totals[0], F, B.rec_base, B.fo);
+            else
+                hipLaunchKernelGGL((k_l2_stats<DD, uint64_t>), dim3(std::min<unsigned>(grid_for(T, CH_WG * CH_ITEMS), 1024)),
+                                   dim3(CH_WG), 0, st, B.blocks, &B.totals[0], F, B.rec_base, B.fo);
             hipLaunchKernelGGL((k_l2_wrap<DD>), dim3(cdiv(F, CH_WG)), dim3(CH_WG), 0, st, B.fo, F, A.err);
             tm.end();
             t_l2p = tm.begin("l2_split");
             auto split = [&](auto dkt) {
                 using DKT = typename decltype(dkt)::type;
                 hipLaunchKernelGGL((k_l2_split<DD, DKT>), dim3((unsigned)std::min<uint64_t>(rec_tiles_max, L2S_WGS)),
-                                   dim3(CH_WG), 0, st, B.blocks, &B.totals[0], F, B.rec_base, A.fdev, B.fo, lb_l2,
+                                   dim3(CH_WG), 0, st, (const DKT*)B.blocks, &B.totals[0], F, B.rec_base, A.fdev, B.fo, lb_l2,
                                    tk_expl + 1, A.err, B.choice2, (DKT*)B.diff, (DKT*)B.keep, &B.totals[1]);
             };
             if (dk32) split(TypeTag<uint32_t>{});
@@ -1327,9 +1335,9 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
         }
     };
     auto set_pre_bytes = [&](uint64_t R, uint64_t NC) {
-        tm.set_bytes(t_explicit, T * (4 + kbytes_all) + 8ull * D * R);
-        tm.set_bytes(t_l2s, 8ull * D * R);
-        tm.set_bytes(t_l2p, (8ull + dkbytes) * D * R + R);
+        tm.set_bytes(t_explicit, T * (4 + kbytes_all) + dkbytes * D * R);
+        tm.set_bytes(t_l2s, dkbytes * D * R);
+        tm.set_bytes(t_l2p, 2ull * dkbytes * D * R + R);
         tm.set_bytes(t_corr, 32ull * NC + 8ull * ntiles);
     };
     uint32_t n_items = 0;

@@ -1285,12 +1285,14 @@ __global__ __launch_bounds__(CH_WG) void k_link_wide(const uint64_t* __restrict_
 // tiles per workgroup: 4 with one-word 32-bit keys, 2 with 64-bit keys (registers)
 template <typename KeyT> constexpr int expl_tpw() { return sizeof(KeyT) == 4 ? 4 : 2; }
 
-template <typename KeyT, int D, int TPW>
+// BT: element type of the block ids written (32-bit when every frame's block
+// ids stay below 2^31)
+template <typename KeyT, int D, int TPW, typename BT>
 __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__ idrel, const KeyT* __restrict__ keys,
                                                     const FrameDev* __restrict__ fr,
                                                     const uint32_t* __restrict__ tile_begin, uint32_t nframes,
                                                     uint32_t ntiles, uint64_t* status, uint32_t* ticket,
-                                                    uint32_t* err, uint64_t* __restrict__ blocks,
+                                                    uint32_t* err, BT* __restrict__ blocks,
                                                     FrameOut* __restrict__ fo, uint64_t* __restrict__ rec_total,
                                                     uint32_t nwords, uint64_t T) {
     // striped: item k of thread t is point k*256 + t of its tile (coalesced);
@@ -1408,7 +1410,7 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
                 const uint32_t pre = __popcll(__ballot(1) & lt);
                 const uint64_t rec = base + s_off[(u * CH_ITEMS + k) * W + wave] + pre;
 #pragma unroll
-                for (int d = 0; d < D; ++d) blocks[rec * D + d] = ((uint64_t)kpre[u][k] >> sh[d]) & mk[d];
+                for (int d = 0; d < D; ++d) blocks[rec * D + d] = (BT)(((uint64_t)kpre[u][k] >> sh[d]) & mk[d]);
             }
         } else {
 #pragma unroll
@@ -1421,7 +1423,7 @@ __global__ __launch_bounds__(CH_WG) void k_explicit(const uint32_t* __restrict__
 #pragma unroll
                 for (int d = 0; d < D; ++d) kd[d] = (uint64_t)keys[(uint64_t)kw[d] * T + po[u] + i];
 #pragma unroll
-                for (int d = 0; d < D; ++d) blocks[rec * D + d] = (kd[d] >> sh[d]) & mk[d];
+                for (int d = 0; d < D; ++d) blocks[rec * D + d] = (BT)((kd[d] >> sh[d]) & mk[d]);
             }
         }
     }
@@ -1457,8 +1459,8 @@ __device__ inline uint32_t frame_of_rec(const uint64_t* s_base, uint32_t nframes
 // (coalesced D x u64 loads); per-thread min / max / max |difference| per
 // frame, flushed when the frame changes and reduced per block at the end, so
 // a frame costs a few atomics per block, not per record.
-template <int D>
-__global__ __launch_bounds__(CH_WG) void k_l2_stats(const uint64_t* __restrict__ blocks,
+template <int D, typename BT>
+__global__ __launch_bounds__(CH_WG) void k_l2_stats(const BT* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
                                                     const uint64_t* __restrict__ frame_rec_base,
                                                     FrameOut* __restrict__ fo) {
@@ -1499,8 +1501,8 @@ __global__ __launch_bounds__(CH_WG) void k_l2_stats(const uint64_t* __restrict__
             const uint64_t rc = r < b1 ? r : rr, rp = rc ? rc - 1 : rc;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                cv[u][d] = blocks[rc * D + d];
-                pv[u][d] = blocks[rp * D + d];
+                cv[u][d] = (uint64_t)blocks[rc * D + d];
+                pv[u][d] = (uint64_t)blocks[rp * D + d];
             }
         }
 #pragma unroll
@@ -1596,7 +1598,7 @@ __global__ void k_l2_wrap(FrameOut* fo, uint32_t nframes, uint32_t* err) {
 // DKT: element type of the diff and keep streams (32-bit when every frame's
 // block ids, and so every wrapped difference, stay below 2^31)
 template <int D, typename DKT>
-__global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__ blocks,
+__global__ __launch_bounds__(CH_WG) void k_l2_split(const DKT* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
                                                     const uint64_t* __restrict__ frame_rec_base,
                                                     const FrameDev* __restrict__ fr, FrameOut* __restrict__ fo,
@@ -1632,6 +1634,120 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
     const bool l2_a = (fr[fa].flags & FL_L2) != 0;
     const uint64_t fvb_a = fo[fa].fixed_value_bits, fdb_a = fo[fa].fixed_diff_bits;
     const int64_t wminc_a = fo[fa].wrap_min_corr, wmaxc_a = fo[fa].wrap_max_corr, wr_a = fo[fa].wrap_range;
+    if constexpr (sizeof(DKT) == 4) {
+        // 32-bit streams (every block id below 2^31): the first pass decides
+        // each record's choice and computes the value it writes (its block, or
+        // the wrapped difference), kept in registers over the look-back, so the
+        // second pass only stores. Blocks are 32-bit values here.
+        uint32_t ov[CH_ITEMS][D];
+        uint32_t chosen = 0;
+#pragma unroll
+        for (int kg = 0; kg < CH_ITEMS; kg += L2S_GROUP) {
+            uint32_t cur[L2S_GROUP][D], prv[L2S_GROUP][D];
+#pragma unroll
+            for (int j = 0; j < L2S_GROUP; ++j) {
+                const uint64_t r = t0 + (kg + j) * CH_WG + threadIdx.x;
+                const uint64_t rc = r < R ? r : t0, rp = rc ? rc - 1 : rc;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    cur[j][d] = blocks[rc * D + d];
+                    prv[j][d] = blocks[rp * D + d];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < L2S_GROUP; ++j) {
+                const int k = kg + j;
+                const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+                bool c = false;
+                if (r < R) {
+                    uint32_t f = fa;
+                    uint64_t fbase = base_a, fvb = fvb_a, fdb = fdb_a;
+                    bool l2 = l2_a;
+                    int64_t wminc = wminc_a, wmaxc = wmaxc_a, wr = wr_a;
+                    if (!one) {
+                        f = frame_of_rec(bases, nframes, r);
+                        fbase = bases[f];
+                        l2 = (fr[f].flags & FL_L2) != 0;
+                        fvb = fo[f].fixed_value_bits;
+                        fdb = fo[f].fixed_diff_bits;
+                        wminc = fo[f].wrap_min_corr;
+                        wmaxc = fo[f].wrap_max_corr;
+                        wr = fo[f].wrap_range;
+                    }
+                    if (r != fbase && l2) {
+                        uint32_t mv = 0, mdf = 0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const uint32_t v = cur[j][d], u = prv[j][d];
+                            mv = v > mv ? v : mv;
+                            const uint32_t ad = u > v ? u - v : v - u;
+                            mdf = ad > mdf ? ad : mdf;
+                        }
+                        const uint64_t tvw = 32u - (uint32_t)__clz(mv), tdw = 32u - (uint32_t)__clz(mdf);
+                        const uint64_t tv = (uint64_t)D * tvw + bitlen64(tvw);
+                        const uint64_t td = (uint64_t)D * tdw + bitlen64(tdw);
+                        const uint64_t value_bits = tv < fvb ? tv : fvb;
+                        const uint64_t diff_bits = td > fdb ? td : fdb;
+                        c = value_bits >= diff_bits;
+                    }
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        if (c) {
+                            // prediction = previous record clamped to [wrap_min, wrap_max] (an identity here)
+                            int64_t corr = (int64_t)prv[j][d] - (int64_t)cur[j][d];
+                            if (corr < wminc) corr += wr;
+                            else if (corr > wmaxc) corr -= wr;
+                            ov[k][d] = (uint32_t)(corr >= 0 ? ((uint64_t)corr << 1)
+                                                            : (((uint64_t)(-(corr + 1)) << 1) | 1ull));
+                        } else {
+                            ov[k][d] = cur[j][d];
+                        }
+                    }
+                }
+                const uint64_t m = __ballot(c);
+                chosen |= (c ? 1u : 0u) << k;
+                if (lane == 0) s_off[k * W + wave] = __popcll(m);
+            }
+            // one group's loads in flight at a time (registers: the values of
+            // every item are held to the second pass)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const uint32_t v = s_off[threadIdx.x];
+            const uint32_t inc = wave_incl_scan(v);
+            s_off[threadIdx.x] = inc - v;
+            const uint32_t tot = __shfl(inc, 63, 64);
+            const uint64_t e = lookback(status, tile, tot, err);
+            if (threadIdx.x == 0) s_excl = e;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < CH_ITEMS; ++k) {
+            const uint64_t r = t0 + k * CH_WG + threadIdx.x;
+            const bool c = (chosen >> k) & 1u;
+            const uint32_t pre = __popcll(__ballot(c) & lt);   // diffs of the row before this lane
+            if (r >= R) continue;
+            const uint32_t f = one ? fa : frame_of_rec(bases, nframes, r);
+            const uint64_t dpos = s_excl + s_off[k * W + wave] + pre;   // diffs before record r
+            if (r == (one ? base_a : bases[f])) {
+                fo[f].diff_base = dpos;
+                fo[f].keep_base = r - dpos;
+            }
+            choice2[r] = (uint8_t)c;
+            if (!c) {
+                const uint64_t kpos = r - dpos;
+#pragma unroll
+                for (int d = 0; d < D; ++d) keep[kpos * D + d] = (DKT)ov[k][d];
+            } else {
+#pragma unroll
+                for (int d = 0; d < D; ++d) diff[dpos * D + d] = (DKT)ov[k][d];
+            }
+            if (r == R - 1) *diff_total = dpos + (c ? 1u : 0u);
+        }
+        __syncthreads();   // LDS reused by the next tile
+        continue;
+    } else {
     // Items are taken L2S_GROUP at a time: the blocks of the group's records
     // and of their predecessors are loaded unconditionally (clamped to the
     // tile's first record) before any is used.
@@ -1755,6 +1871,7 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const uint64_t* __restrict__
         }
     }
     __syncthreads();   // LDS reused by the next tile
+    }
     }
 }
 

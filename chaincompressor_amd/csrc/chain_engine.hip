@@ -2160,7 +2160,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
         fill32(st, X.flags, nrt + 1, 0);
         fill32(st, X.ticket, 1, 0);
         tm.end();
-        tm.begin("l2_inverse", P.R + dkes * D * (P.ND + P.NK) + 8ull * D * P.R);
+        tm.begin("l2_inverse", P.R + dkes * D * (P.ND + P.NK) + dkes * D * P.R);
         // 32-bit scan arithmetic when every frame's wrap range is at most 2^31
         uint64_t max_range = 0;
         for (const DecFrame& fd : P.frames) max_range = std::max(max_range, fd.range);
@@ -2171,7 +2171,7 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
                 using DKT = typename decltype(dkt)::type;
                 hipLaunchKernelGGL((k_dec_l2<DD, V, DKT>), dim3(nrt), dim3(CH_WG), 0, st, X.fr, F, X.rtb, X.choice,
                                    (const DKT*)X.diff, (const DKT*)X.keep, X.zoff, X.flags, X.aggv, X.ticket,
-                                   X.compact, X.err);
+                                   (DKT*)X.compact, X.err);
             };
             // dk32 implies a range of at most 2^31
             if (dk32) inverse(TypeTag<uint32_t>{}, TypeTag<uint32_t>{});
@@ -2214,23 +2214,28 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
                            (sync_rounds & 1) ? X.list_b : X.list_a, &X.jump_open[sync_rounds], X.err);
     }
     tm.end();
-    tm.begin("dequantize", 4 * T + comp_es[4] * T * D + 8ull * D * P.R + T * D * es);
+    tm.begin("dequantize", 4 * T + comp_es[4] * T * D + dkes * D * P.R + T * D * es);
     with_dims(D, [&](auto dd) {
         constexpr int DD = decltype(dd)::value;
         auto go = [&](auto ot, auto rt) {
             using O = typename decltype(ot)::type;
             using RT = typename decltype(rt)::type;
-            if (out_layout == CHAIN_SOA) {
-                hipLaunchKernelGGL((k_dec_dequant<DD, O, true, RT>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F,
-                                   X.ptb, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
-                hipLaunchKernelGGL((k_dec_corr<O, true>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
-                                   (O*)X.out, X.err);
-            } else {
-                hipLaunchKernelGGL((k_dec_dequant<DD, O, false, RT>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F,
-                                   X.ptb, X.root, X.compact, (const RT*)X.rel, (O*)X.out, X.err);
-                hipLaunchKernelGGL((k_dec_corr<O, false>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
-                                   (O*)X.out, X.err);
-            }
+            auto run = [&](auto bt) {
+                using BT = typename decltype(bt)::type;
+                if (out_layout == CHAIN_SOA) {
+                    hipLaunchKernelGGL((k_dec_dequant<DD, O, true, RT, BT>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F,
+                                       X.ptb, X.root, (const BT*)X.compact, (const RT*)X.rel, (O*)X.out, X.err);
+                    hipLaunchKernelGGL((k_dec_corr<O, true>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
+                                       (O*)X.out, X.err);
+                } else {
+                    hipLaunchKernelGGL((k_dec_dequant<DD, O, false, RT, BT>), dim3(npt), dim3(CH_WG), 0, st, X.fr, F,
+                                       X.ptb, X.root, (const BT*)X.compact, (const RT*)X.rel, (O*)X.out, X.err);
+                    hipLaunchKernelGGL((k_dec_corr<O, false>), dim3(64, F), dim3(CH_WG), 0, st, X.in, X.fr, F, D,
+                                       (O*)X.out, X.err);
+                }
+            };
+            if (dk32) run(TypeTag<uint32_t>{});
+            else run(TypeTag<uint64_t>{});
         };
         auto go_rel = [&](auto ot) {
             if (rel8) go(ot, TypeTag<uint8_t>{});

@@ -823,7 +823,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
                                                   const DKT* __restrict__ keep,
                                                   const uint32_t* __restrict__ tile_zero_off,
                                                   uint32_t* flags, uint64_t* aggv, uint32_t* ticket,
-                                                  uint64_t* __restrict__ compact, uint32_t* err) {
+                                                  DKT* __restrict__ compact, uint32_t* err) {
     constexpr uint32_t W = CH_WG / 64;
     constexpr uint32_t NS = CH_ITEMS * W;   // (row, wave) segments of the striped order
     __shared__ uint32_t s_tile;
@@ -1025,10 +1025,10 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     // one coalesced run of the tile's records
     const uint64_t left = Fr.nrec - t0;
     const uint32_t nt = left < CH_TILE ? (uint32_t)left : CH_TILE;
-    uint64_t* out = compact + (Fr.rec_base + t0) * D;
+    DKT* out = compact + (Fr.rec_base + t0) * D;   // 32-bit: every block id below 2^31
     for (uint32_t e = threadIdx.x; e < nt * D; e += CH_WG) {
         const uint32_t r = e / D, d = e - r * D;
-        out[e] = (uint64_t)Fr.wrap_min + (uint64_t)s_v[d][L2_PAD(r)];
+        out[e] = (DKT)((uint64_t)Fr.wrap_min + (uint64_t)s_v[d][L2_PAD(r)]);
     }
 }
 
@@ -1327,11 +1327,13 @@ __global__ void k_dec_check_counts(const DecFrame* __restrict__ fr, uint32_t F, 
 // every block and offset those roots name, then computes and stores.
 #define DQ_ITEMS 8
 #define DQ_STAGE 4
-template <int D, typename OutT, bool SOA, typename RelT>
+// BT: element type of the L2-inverted block ids (32-bit when every frame's
+// wrap range lies in [0, 2^31))
+template <int D, typename OutT, bool SOA, typename RelT, typename BT>
 __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restrict__ fr, uint32_t F,
                                                        const uint32_t* __restrict__ tile_begin,
                                                        const uint32_t* __restrict__ root,
-                                                       const uint64_t* __restrict__ compact,
+                                                       const BT* __restrict__ compact,
                                                        const RelT* __restrict__ rel, OutT* __restrict__ out,
                                                        uint32_t* err) {
     const uint32_t tile = blockIdx.x;
@@ -1387,7 +1389,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_dequant(const DecFrame* __restric
             const uint64_t rc = good ? rec : rec_base, ic = live ? i : i0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                blk[j][d] = compact[rc * D + d];
+                blk[j][d] = (uint64_t)compact[rc * D + d];
                 off[j][d] = rel[(pt_off + ic) * D + d];
             }
         }

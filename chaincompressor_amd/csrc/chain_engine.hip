@@ -2131,9 +2131,9 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     tm.end();
     if (P.fixed_tiles) {
         tm.begin("unpack_fixed", b_fixed);
-        hipLaunchKernelGGL(k_dec_fixed, dim3((unsigned)P.fixed_tiles), dim3(CH_WG), 0, st, X.in, X.fixed,
-                           (uint32_t)P.fixed.size(), X.idrel, (void*)X.diff, (void*)X.keep, (uint32_t)dk32, X.choice,
-                           X.rel, rel8, X.err);
+        hipLaunchKernelGGL(k_dec_fixed, dim3((unsigned)cdiv(P.fixed_tiles, DEC_FIX_TPW)), dim3(CH_WG), 0, st, X.in,
+                           X.fixed, (uint32_t)P.fixed.size(), (uint32_t)P.fixed_tiles, X.idrel, (void*)X.diff,
+                           (void*)X.keep, (uint32_t)dk32, X.choice, X.rel, rel8, X.err);
         tm.end();
     }
     if (P.raw_total) {

@@ -28,15 +28,42 @@ __device__ inline uint64_t okey(double x) {      // order-preserving key, -0 < +
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 __device__ inline uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
+// Cross-lane steps with DPP (no LDS round trip). row_shr:k gives each lane
+// the value of the lane k below it in its row of 16 (the identity where that
+// lane is outside the row); row_bcast:15 gives rows 1 and 3 the last lane of
+// the row before, row_bcast:31 gives rows 2 and 3 lane 31.
+#define DPP_ROW_SHR(k) (0x110 + (k))
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+template <int CTRL, int ROW_MASK>
+__device__ inline uint32_t dpp_u32(uint32_t ident, uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)x, CTRL, ROW_MASK, 0xf, false);
+}
+// inclusive combine over the wavefront in lane order (Hillis-Steele inside
+// each row, then the rows): every lane gets op over lanes 0..itself
+template <typename Op>
+__device__ inline uint32_t wave_incl_dpp(uint32_t x, uint32_t ident, Op op) {
+    x = op(x, dpp_u32<DPP_ROW_SHR(1), 0xf>(ident, x));
+    x = op(x, dpp_u32<DPP_ROW_SHR(2), 0xf>(ident, x));
+    x = op(x, dpp_u32<DPP_ROW_SHR(4), 0xf>(ident, x));
+    x = op(x, dpp_u32<DPP_ROW_SHR(8), 0xf>(ident, x));
+    x = op(x, dpp_u32<DPP_ROW_BCAST15, 0xa>(ident, x));
+    x = op(x, dpp_u32<DPP_ROW_BCAST31, 0xc>(ident, x));
+    return x;
+}
+
 // inclusive scan of a u32 inside one 64-lane wavefront
 __device__ inline uint32_t wave_incl_scan(uint32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int k = 1; k < 64; k <<= 1) {
-        uint32_t t = __shfl_up(v, k, 64);
-        if (lane >= (uint32_t)k) v += t;
-    }
-    return v;
+    return wave_incl_dpp(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+// minimum / maximum over the wavefront, in every lane
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+    const uint32_t r = wave_incl_dpp(v, 0xffffffffu, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)r, 63);
+}
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+    const uint32_t r = wave_incl_dpp(v, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)r, 63);
 }
 
 // exclusive scan over a 256-thread workgroup; returns the exclusive prefix,
@@ -897,13 +924,13 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
         if (tid < 65) s_le[tid] = (uint8_t)__builtin_ctzll(chain_limit<D>(F, (int)tid, 12));
         {
             // the tile's keys, coalesced, every load issued first (past the
-            // end: the tile's first key, not counted)
+            // end: the tile's last key again, which leaves the range as it is)
             const KeyT* kp = keys + F.pt_off + start;
             KeyT kv[QL_ITEMS];
 #pragma unroll
             for (int k = 0; k < QL_ITEMS; ++k) {
                 const uint32_t j = k * CH_WG + tid;
-                kv[k] = kp[j < cnt ? j : 0u];
+                kv[k] = kp[j < cnt - 1 ? j : cnt - 1];
             }
             // per-dimension range of the tile's block ids
             KeyT mn[D], mx[D];
@@ -912,8 +939,7 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
                 mn[d] = (kv[0] >> ksh[d]) & kmask[d];
                 mx[d] = mn[d];
             }
-            // (a position past the end loaded the tile's first key: it leaves
-            // the range as it is)
+            // (a position past the end loaded the tile's last key again)
 #pragma unroll
             for (int k = 1; k < QL_ITEMS; ++k) {
 #pragma unroll
@@ -925,10 +951,15 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
             }
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                for (int o = 32; o >= 1; o >>= 1) {
-                    const KeyT a = __shfl_xor(mn[d], o, 64), c = __shfl_xor(mx[d], o, 64);
-                    mn[d] = a < mn[d] ? a : mn[d];
-                    mx[d] = c > mx[d] ? c : mx[d];
+                if constexpr (sizeof(KeyT) == 4) {
+                    mn[d] = wave_min_u32(mn[d]);
+                    mx[d] = wave_max_u32(mx[d]);
+                } else {
+                    for (int o = 32; o >= 1; o >>= 1) {
+                        const KeyT a = __shfl_xor(mn[d], o, 64), c = __shfl_xor(mx[d], o, 64);
+                        mn[d] = a < mn[d] ? a : mn[d];
+                        mx[d] = c > mx[d] ? c : mx[d];
+                    }
                 }
                 if (lane == 0) {
                     s_rng[w][2 * d] = mn[d];
@@ -964,10 +995,9 @@ __global__ __launch_bounds__(CH_WG) void k_tile_link(
                 }
                 const uint32_t lb = sizeof(KeyT) == 4 ? 32u - (uint32_t)__clz((uint32_t)mb)
                                                       : (uint32_t)bitlen64((uint64_t)mb);
-                if (j < cnt) {
-                    s_key[j] = ck;
-                    s_exp[j] = s_le[lb];
-                }
+                // positions past the end are never read: no bound test
+                s_key[j] = ck;
+                s_exp[j] = s_le[lb];
                 s_ia[j] = (uint16_t)j;
             }
         }

@@ -790,6 +790,31 @@ __device__ inline void agg_comb(uint32_t& lr, V* lv, uint32_t rr, const V* rv, V
     }
 }
 
+// One DPP step of the wavefront scan of scan elements: every lane combines
+// the element CTRL names (an earlier lane; the identity (0, 0) where there is
+// none or its row is masked off) with its own, earlier on the left.
+template <int CTRL, int ROW_MASK, typename V>
+__device__ inline V dpp_val(V x) {
+    if constexpr (sizeof(V) == 4) {
+        return (V)dpp_u32<CTRL, ROW_MASK>(0u, (uint32_t)x);
+    } else {
+        const uint32_t lo = dpp_u32<CTRL, ROW_MASK>(0u, (uint32_t)x);
+        const uint32_t hi = dpp_u32<CTRL, ROW_MASK>(0u, (uint32_t)((uint64_t)x >> 32));
+        return (V)(((uint64_t)hi << 32) | lo);
+    }
+}
+template <int CTRL, int ROW_MASK, int D, typename V>
+__device__ inline void agg_dpp_step(uint32_t& r, V* v, V R) {
+    uint32_t er = dpp_u32<CTRL, ROW_MASK>(0u, r);
+    V ev[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) ev[d] = dpp_val<CTRL, ROW_MASK, V>(v[d]);
+    agg_comb<D, V>(er, ev, r, v, R);   // earlier (+) mine
+    r = er;
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[d] = ev[d];
+}
+
 // Wave 0 of an L2 tile: publishes the tile aggregate, looks back over the
 // earlier tiles of the frame (64 predecessors at a time: lane l reads tile
 // t - l; the window up to the nearest inclusive or resetting aggregate is
@@ -1036,25 +1061,23 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     V xv[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xv[d] = av[d];
-    for (uint32_t off = 1; off < 64; off <<= 1) {   // inclusive segmented scan
-        uint32_t orr = xr;
-        V ov[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) ov[d] = xv[d];
-        agg_shfl_up<D, V>(orr, ov, off);
-        if (lane >= off) {
-            agg_comb<D, V>(orr, ov, xr, xv, R);   // earlier (+) mine
-            xr = orr;
-#pragma unroll
-            for (int d = 0; d < D; ++d) xv[d] = ov[d];
-        }
-    }
+    // inclusive segmented scan: inside each row of 16 lanes, then rows 1 and
+    // 3 take the end of the row before, rows 2 and 3 the end of row 1
+    agg_dpp_step<DPP_ROW_SHR(1), 0xf, D, V>(xr, xv, R);
+    agg_dpp_step<DPP_ROW_SHR(2), 0xf, D, V>(xr, xv, R);
+    agg_dpp_step<DPP_ROW_SHR(4), 0xf, D, V>(xr, xv, R);
+    agg_dpp_step<DPP_ROW_SHR(8), 0xf, D, V>(xr, xv, R);
+    agg_dpp_step<DPP_ROW_BCAST15, 0xa, D, V>(xr, xv, R);
+    agg_dpp_step<DPP_ROW_BCAST31, 0xc, D, V>(xr, xv, R);
     if (lane == 63) {
         s_wr[wave] = xr;
 #pragma unroll
         for (int d = 0; d < D; ++d) s_wv[wave][d] = xv[d];
     }
-    agg_shfl_up<D, V>(xr, xv, 1);   // inclusive of lane - 1 = exclusive of lane
+    // inclusive of lane - 1 = exclusive of lane (wave_shr:1; lane 0 gets (0, 0))
+    xr = dpp_u32<0x138, 0xf>(0u, xr);
+#pragma unroll
+    for (int d = 0; d < D; ++d) xv[d] = dpp_val<0x138, 0xf, V>(xv[d]);
     if (lane == 0) {
         xr = 0;
 #pragma unroll

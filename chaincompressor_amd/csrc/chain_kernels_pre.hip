@@ -295,25 +295,41 @@ __global__ __launch_bounds__(CH_WG) void k_bbox(const T* __restrict__ pts, const
         __builtin_memcpy(&x, &b, sizeof(x));
         return okey((double)x);
     };
-    uint64_t mn[D], mx[D], zp[D], zn[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        mn[d] = kmn[d] == BK_MAX ? ~0ull : widen(kmn[d]);
-        mx[d] = kmx[d] == 0 ? 0ull : widen(kmx[d]);
-        zp[d] = rzp[d] == ~0u ? ~0ull : start + rzp[d];
-        zn[d] = rzn[d] == ~0u ? ~0ull : start + rzn[d];
-    }
     __shared__ uint64_t s_red[4][CH_MAXD][4];
     const uint32_t w = threadIdx.x >> 6;
+    if constexpr (sizeof(BK) == 4) {
+        // 32-bit keys and tile offsets reduce across the wavefront with DPP,
+        // then only the results are widened
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        uint64_t a = mn[d], b = mx[d], c = zp[d], e = zn[d];
-        for (int k = 32; k >= 1; k >>= 1) {
-            uint64_t ta = __shfl_xor(a, k, 64), tb = __shfl_xor(b, k, 64);
-            uint64_t tc = __shfl_xor(c, k, 64), te = __shfl_xor(e, k, 64);
-            a = ta < a ? ta : a; b = tb > b ? tb : b; c = tc < c ? tc : c; e = te < e ? te : e;
+        for (int d = 0; d < D; ++d) {
+            const uint32_t a = wave_min_u32(kmn[d]), b = wave_max_u32(kmx[d]);
+            const uint32_t c = wave_min_u32(rzp[d]), e = wave_min_u32(rzn[d]);
+            if (lane_id() == 0) {
+                s_red[w][d][0] = a == BK_MAX ? ~0ull : widen(a);
+                s_red[w][d][1] = b == 0 ? 0ull : widen(b);
+                s_red[w][d][2] = c == ~0u ? ~0ull : start + c;
+                s_red[w][d][3] = e == ~0u ? ~0ull : start + e;
+            }
         }
-        if (lane_id() == 0) { s_red[w][d][0] = a; s_red[w][d][1] = b; s_red[w][d][2] = c; s_red[w][d][3] = e; }
+    } else {
+        uint64_t mn[D], mx[D], zp[D], zn[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            mn[d] = kmn[d] == BK_MAX ? ~0ull : widen(kmn[d]);
+            mx[d] = kmx[d] == 0 ? 0ull : widen(kmx[d]);
+            zp[d] = rzp[d] == ~0u ? ~0ull : start + rzp[d];
+            zn[d] = rzn[d] == ~0u ? ~0ull : start + rzn[d];
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            uint64_t a = mn[d], b = mx[d], c = zp[d], e = zn[d];
+            for (int k = 32; k >= 1; k >>= 1) {
+                uint64_t ta = __shfl_xor(a, k, 64), tb = __shfl_xor(b, k, 64);
+                uint64_t tc = __shfl_xor(c, k, 64), te = __shfl_xor(e, k, 64);
+                a = ta < a ? ta : a; b = tb > b ? tb : b; c = tc < c ? tc : c; e = te < e ? te : e;
+            }
+            if (lane_id() == 0) { s_red[w][d][0] = a; s_red[w][d][1] = b; s_red[w][d][2] = c; s_red[w][d][3] = e; }
+        }
     }
     if (nonfinite) atomicOr(&bad[f], 1u);
     __syncthreads();

@@ -134,20 +134,19 @@ __global__ __launch_bounds__(CH_WG) void k_comp_init(const FrameDev* __restrict_
 // One workgroup walks a contiguous range of tiles and keeps its histograms
 // and range statistics in LDS / registers until the frame changes.
 // Small values and widths (< CS_PRIV_BINS, the common case: in-block offsets,
-// choice flags, short links) are counted in per-thread LDS columns of 16-bit
-// pairs: bin b of thread t is half (b & 1) of word [b >> 1][t], so the 64
+// choice flags, short links) are counted in per-thread LDS columns of 8-bit
+// counters: bin b of thread t is byte (b & 3) of word [b >> 2][t], so the 64
 // lanes of a wavefront add to 64 distinct words and never serialise on a
-// shared bin. Larger values use a shared LDS histogram. A fold moves the
-// columns into 32-bit totals before a 16-bit half can overflow (at most
-// CH_ITEMS * DIMS additions per tile).
+// shared bin. Larger values use a shared LDS histogram (256 bins for byte
+// values). A fold moves the columns into 32-bit totals before a byte can
+// overflow (at most CH_ITEMS * DIMS additions per tile).
 // Publishing: the small bins and the range statistics of a (workgroup, frame)
 // pair go to a slot of their own with plain stores (k_comp_reduce combines
 // them), because thousands of workgroups adding into the same few words
 // serialise on the device's atomic unit; larger bins are spread over many
 // words and are added with global atomics directly.
 #define CS_PRIV_BINS 32u
-#define CS_PRIV_ROWS (CS_PRIV_BINS / 2)
-#define CS_FOLD_TILES 1000u
+#define CS_PRIV_ROWS (CS_PRIV_BINS / 4)   // four 8-bit counters per word
 #define CS_GRID 2048u
 #define CS_REDUCE_SLOTS 64u
 
@@ -190,10 +189,14 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
                                                       const uint32_t* __restrict__ tile_begin,
                                                       const CompInst* __restrict__ ci, CompSlot* __restrict__ slots,
                                                       uint32_t* __restrict__ vhist, uint32_t* __restrict__ thist) {
-    static_assert(CH_ITEMS * 4 * CS_FOLD_TILES < 65536u, "16-bit private counters overflow");
+    // tiles between folds: a thread adds at most CH_ITEMS * DIMS to a byte per tile
+    constexpr uint32_t FOLD_TILES = 255u / (CH_ITEMS * DIMS);
+    static_assert(FOLD_TILES >= 2, "two tiles are counted between folds");
+    // byte values need 256 value bins
+    constexpr uint32_t VH = sizeof(ValT) == 1 ? 256u : RANS_VALUE_LIMIT;
     // values of at most 32 bits are compared and measured in 32-bit registers
     using AccT = typename std::conditional<(sizeof(ValT) <= 4), uint32_t, uint64_t>::type;
-    __shared__ uint32_t s_vh[RANS_VALUE_LIMIT], s_th[RANS_TAG_LIMIT];
+    __shared__ uint32_t s_vh[VH], s_th[RANS_TAG_LIMIT];
     __shared__ uint32_t s_sv[CS_PRIV_BINS], s_st[CS_PRIV_BINS];
     __shared__ uint32_t s_pv[CS_PRIV_ROWS][CH_WG], s_pt[CS_PRIV_ROWS][CH_WG];
     __shared__ uint64_t s_red[4][2 * CH_MAXD + 2];
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t tb0 = blockIdx.x * per, tb1 = tb0 + per < ntiles ? tb0 + per : ntiles;
     if (tb0 >= tb1) return;
-    for (uint32_t k = tid; k < RANS_VALUE_LIMIT; k += CH_WG) s_vh[k] = 0;
+    for (uint32_t k = tid; k < VH; k += CH_WG) s_vh[k] = 0;
     if (tid < RANS_TAG_LIMIT) s_th[tid] = 0;
     if (tid < CS_PRIV_BINS) {
         s_sv[tid] = 0;
@@ -227,33 +230,36 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
     reset();
     const uint32_t w = tid >> 6;
     // private columns -> 32-bit totals (whole workgroup; ends synchronised).
-    // Thread (row, g) sums 16 words of its row; 16 lanes reduce the row.
+    // Thread (row, g) sums 8 words of its row, four bytes each; the 32
+    // lanes of the row reduce them.
     auto fold = [&]() {
         __syncthreads();
-        const uint32_t row = tid >> 4, g = tid & 15u;
-        uint32_t vl = 0, vh = 0, tl = 0, th = 0;
+        const uint32_t row = tid >> 5, g = tid & 31u;
+        uint32_t vs[4] = {0, 0, 0, 0}, ts[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (uint32_t i = 0; i < 16; ++i) {
-            const uint32_t col = g + 16u * i;
+        for (uint32_t i = 0; i < CH_WG / 32; ++i) {
+            const uint32_t col = g + 32u * i;
             const uint32_t a = s_pv[row][col], b = s_pt[row][col];
             s_pv[row][col] = 0;
             s_pt[row][col] = 0;
-            vl += a & 0xffffu;
-            vh += a >> 16;
-            tl += b & 0xffffu;
-            th += b >> 16;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                vs[q] += (a >> (8 * q)) & 0xffu;
+                ts[q] += (b >> (8 * q)) & 0xffu;
+            }
         }
-        for (int k = 8; k >= 1; k >>= 1) {
-            vl += __shfl_xor(vl, k, 64);
-            vh += __shfl_xor(vh, k, 64);
-            tl += __shfl_xor(tl, k, 64);
-            th += __shfl_xor(th, k, 64);
-        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            for (int k = 16; k >= 1; k >>= 1) {
+                vs[q] += __shfl_xor(vs[q], k, 64);
+                ts[q] += __shfl_xor(ts[q], k, 64);
+            }
         if (g == 0) {
-            s_sv[2 * row] += vl;
-            s_sv[2 * row + 1] += vh;
-            s_st[2 * row] += tl;
-            s_st[2 * row + 1] += th;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                s_sv[4 * row + q] += vs[q];
+                s_st[4 * row + q] += ts[q];
+            }
         }
         __syncthreads();
     };
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
             S.overflow = of;
         }
         const uint32_t ci_idx = c * F + f;
-        for (uint32_t k = CS_PRIV_BINS + tid; k < RANS_VALUE_LIMIT; k += CH_WG)
+        for (uint32_t k = CS_PRIV_BINS + tid; k < VH; k += CH_WG)
             if (s_vh[k]) {
                 atomicAdd(&vhist[(uint64_t)ci_idx * RANS_VALUE_LIMIT + k], s_vh[k]);
                 s_vh[k] = 0;
@@ -343,8 +349,8 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
             any |= v[d];
             if (vals_on) {
                 if (v[d] < CS_PRIV_BINS)
-                    atomicAdd(&s_pv[(uint32_t)v[d] >> 1][pt_base], 1u << (((uint32_t)v[d] & 1u) << 4));
-                else if (v[d] < RANS_VALUE_LIMIT)
+                    atomicAdd(&s_pv[(uint32_t)v[d] >> 2][pt_base], 1u << (((uint32_t)v[d] & 3u) << 3));
+                else if (v[d] < VH)
                     atomicAdd(&s_vh[(uint32_t)v[d]], 1u);
                 else
                     ovf = 1u;
@@ -353,7 +359,7 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         const uint32_t wb = sizeof(AccT) == 4 ? 32u - (uint32_t)__clz((uint32_t)any) : (uint32_t)bitlen64(any);
         const uint32_t tag = wb > 1u ? wb : 1u;
         if (tag <= CS_PRIV_BINS)
-            atomicAdd(&s_pt[(tag - 1) >> 1][pt_base], 1u << (((tag - 1) & 1u) << 4));
+            atomicAdd(&s_pt[(tag - 1) >> 2][pt_base], 1u << (((tag - 1) & 3u) << 3));
         else
             atomicAdd(&s_th[tag], 1u);
     };
@@ -398,15 +404,16 @@ __global__ __launch_bounds__(CH_WG) void k_comp_stats(const ValT* __restrict__ v
         const CompInst& C = ci[c * F + f];
         const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * CH_TILE;
         const uint64_t n = C.n;
-        // two full tiles of one frame are loaded together
-        const bool pair = WORDS && tile + 1 < tb1 && r0 + 2ull * CH_TILE <= n;
+        // two full tiles of one frame are loaded together (one-dimensional
+        // components: wider ones would pass 128 registers, fewer waves per SIMD)
+        const bool pair = WORDS && DIMS == 1 && tile + 1 < tb1 && r0 + 2ull * CH_TILE <= n;
         const uint32_t g = pair ? 2u : 1u;
         // every frame the range touches is published, even with no records
         if (f != cur) {
             if (cur != 0xffffffffu) flush(cur);
             cur = f;
             acc = 0;
-        } else if (acc + g > CS_FOLD_TILES) {
+        } else if (acc + g > FOLD_TILES) {
             fold();
             acc = 0;
         }

@@ -1710,11 +1710,7 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
         j.dims = dims;
         j.dst = dst;
         j.comp = comp;
-        if (n) {
-            j.tile0 = (uint32_t)P.fixed_tiles;
-            P.fixed_tiles += cdiv(n, DEC_FIX_RECS);
-            P.fixed.push_back(j);
-        }
+        if (n) P.fixed.push_back(j);   // tiles are assigned once the batch is parsed
         return;
     }
     if (std::memcmp(magic, "RANS", 4) == 0) {
@@ -1797,9 +1793,7 @@ static void parse_lcg2_component(HostReader r, int comp, uint32_t expected_dims,
     j.dims = dims;
     j.dst = dst;
     j.comp = comp;
-    j.tile0 = (uint32_t)P.fixed_tiles;
-    P.fixed_tiles += cdiv(n, DEC_FIX_RECS);
-    P.fixed.push_back(j);
+    P.fixed.push_back(j);   // tiles are assigned once the batch is parsed
 }
 
 // An LCG2 stream of the reference GPU compressor (cu/chain/lcps_gpu_cpp.cu:
@@ -1974,6 +1968,20 @@ static void decompress_batch_impl(chain_engine* e, const uint8_t* const* bytes, 
     // in-block offsets fit a byte when every frame's p is at most 256
     uint32_t rel8 = 1;
     for (const DecFrame& fd : P.frames) rel8 &= fd.p <= 256 ? 1u : 0u;
+    // Fixed jobs' tiles: byte outputs (choice flags; offsets when they are u8)
+    // whose every value lands in range take the byte path in tiles of
+    // DEC_FIX_BRECS records, the others tiles of DEC_FIX_RECS
+    P.fixed_tiles = 0;
+    for (DecFixed& j : P.fixed) {
+        const uint64_t vmax = j.comp == 3 ? 1u : 0xffu;
+        bool bytes = j.comp == 3 || (j.comp == 4 && rel8);
+        for (uint32_t d = 0; d < j.dims; ++d)
+            bytes = bytes && j.width[d] <= 8 && j.minimum[d] <= vmax &&
+                    (1ull << j.width[d]) - 1ull <= vmax - j.minimum[d];
+        j.recs = bytes ? DEC_FIX_BRECS : DEC_FIX_RECS;
+        j.tile0 = (uint32_t)P.fixed_tiles;
+        P.fixed_tiles += cdiv(j.n, j.recs);
+    }
     // tiles: points (chain resolution) and records (L2 inverse), per frame
     std::vector<uint32_t> ptb(F + 1, 0), rtb(F + 1, 0), ptb32(F + 1, 0);
     for (uint32_t f = 0; f < F; ++f) {

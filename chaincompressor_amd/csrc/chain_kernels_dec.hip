@@ -305,8 +305,8 @@ struct DecFixed {
     uint64_t dst;               // first value index (record r -> dst + r*dims + d)
     uint32_t dims;
     int32_t comp;
-    uint32_t tile0;             // first tile of DEC_FIX_RECS records (prefix over jobs)
-    uint32_t pad;
+    uint32_t tile0;             // first tile of the job (prefix over jobs)
+    uint32_t recs;              // records per tile: DEC_FIX_RECS, or DEC_FIX_BRECS on the byte path
 };
 
 // One workgroup per tile of DEC_FIX_RECS records of one job: every dimension
@@ -319,6 +319,7 @@ struct DecFixed {
 #define DEC_FIX_LDS 1024u
 #define DEC_FIX_ITEMS 4
 #define DEC_FIX_RECS (CH_WG * DEC_FIX_ITEMS)
+#define DEC_FIX_BRECS 4096u   // records per tile of a byte-path job
 
 // J is a register copy of the job (uniform, indexed only by unrolled
 // constants); Jg is the job in memory, read by the rare byte-by-byte path
@@ -417,15 +418,15 @@ __device__ __forceinline__ void fixed_tile(const uint8_t* __restrict__ in, const
 __device__ __forceinline__ void fixed_tile_bytes(const uint8_t* __restrict__ in, const DecFixed* __restrict__ Jg,
                                                 uint32_t dims, uint64_t n, uint64_t dst, uint32_t tile_in_job,
                                                 uint8_t* __restrict__ out, uint8_t* __restrict__ s_out) {
-    const uint64_t rbase = (uint64_t)tile_in_job * DEC_FIX_RECS;
-    const uint32_t nr = n - rbase < DEC_FIX_RECS ? (uint32_t)(n - rbase) : DEC_FIX_RECS;
+    const uint64_t rbase = (uint64_t)tile_in_job * DEC_FIX_BRECS;
+    const uint32_t nr = n - rbase < DEC_FIX_BRECS ? (uint32_t)(n - rbase) : DEC_FIX_BRECS;
     const uint64_t o0 = dst + rbase * dims;   // first output byte of the tile
     const uint32_t a2 = (uint32_t)(o0 & 3u);  // LDS byte a2 + i holds output byte o0 + i
-    static_assert(DEC_FIX_RECS / 8 == 128, "two wavefronts per dimension");
-    // wavefronts 0-1 take dimensions 0 and 2, wavefronts 2-3 dimensions 1 and 3
-    const uint32_t g = threadIdx.x & 127u;
-    for (uint32_t dv = threadIdx.x >> 7; dv < dims; dv += 2) {
-        const uint32_t d = __builtin_amdgcn_readfirstlane(dv);
+    constexpr uint32_t NG = DEC_FIX_BRECS / 8;   // groups per dimension (a power of two)
+    static_assert((NG & (NG - 1)) == 0 && NG % 64 == 0, "a wavefront's groups share a dimension");
+    for (uint32_t pi = threadIdx.x; pi < dims * NG; pi += CH_WG) {
+        const uint32_t d = __builtin_amdgcn_readfirstlane(pi / NG);
+        const uint32_t g = pi & (NG - 1);
         if (8 * g >= nr) continue;
         const uint32_t w = Jg->width[d];
         const uint32_t mn = (uint32_t)Jg->minimum[d];
@@ -482,7 +483,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__
                                                      void* keep, uint32_t dk32, uint8_t* choice, void* rel,
                                                      uint32_t rel8, uint32_t* err) {
     __shared__ uint32_t s_tile0[DEC_FIX_LDS];
-    __shared__ __attribute__((aligned(16))) uint8_t s_out[DEC_FIX_RECS * CH_MAXD + 8];
+    __shared__ __attribute__((aligned(16))) uint8_t s_out[DEC_FIX_BRECS * CH_MAXD + 8];
     const bool lds = njobs <= DEC_FIX_LDS;
     if (lds) {
         for (uint32_t k = threadIdx.x; k < njobs; k += CH_WG) s_tile0[k] = jobs[k].tile0;
@@ -500,21 +501,11 @@ __global__ __launch_bounds__(CH_WG) void k_dec_fixed(const uint8_t* __restrict__
         const DecFixed* Jg = jobs + __builtin_amdgcn_readfirstlane(lo);
         const DecFixed J = *Jg;
         const uint32_t t = tile - J.tile0;
-        if (J.comp == 3 || (J.comp == 4 && rel8)) {
-            // byte outputs: the group path when no value needs a range check
-            const uint64_t vmax = J.comp == 3 ? 1u : 0xffu;
-            bool bytes = true;
-#pragma unroll
-            for (int d = 0; d < CH_MAXD; ++d)
-                if ((uint32_t)d < J.dims)
-                    bytes = bytes && J.width[d] <= 8 && J.minimum[d] <= vmax &&
-                            (1ull << J.width[d]) - 1ull <= vmax - J.minimum[d];
-            if (bytes) {
-                fixed_tile_bytes(in, Jg, J.dims, J.n, J.dst, t, J.comp == 3 ? choice : static_cast<uint8_t*>(rel),
-                                 s_out);
-                __syncthreads();   // s_out is reused by the next tile
-                continue;
-            }
+        if (J.recs == DEC_FIX_BRECS) {
+            // byte outputs whose values need no range check (decided by the host)
+            fixed_tile_bytes(in, Jg, J.dims, J.n, J.dst, t, J.comp == 3 ? choice : static_cast<uint8_t*>(rel), s_out);
+            __syncthreads();   // s_out is reused by the next tile
+            continue;
         }
         switch (J.comp) {
             case 0: fixed_tile<uint32_t>(in, J, Jg, t, idrel, 0xffffffffull, DERR_VALUE, err); break;

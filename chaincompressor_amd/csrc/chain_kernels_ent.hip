@@ -1207,7 +1207,7 @@ __global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, u
 }
 
 // output layout: one thread walks the frames in order
-#define TAG_TILE 1024
+#define TAG_TILE 2048   // records of a Tagged packing tile (8 per thread)
 // 64-bit exclusive scan over the workgroup (the container offsets of a batch
 // can pass 2^32)
 __device__ inline uint64_t block_excl_scan_u64(uint64_t v, uint64_t* s_tmp64, uint64_t* total) {
@@ -1641,7 +1641,7 @@ __global__ __launch_bounds__(CH_WG) void k_pack_fixed(const ValT* __restrict__ v
     }
 }
 
-// Tagged payloads: 1024 records per tile, bit offsets by a segmented look-back
+// Tagged payloads: TAG_TILE records per tile, bit offsets by a segmented look-back
 // over each frame's tiles, bits assembled in LDS; the partial bytes at the
 // two ends of a tile go to side arrays and are merged by k_tag_seams.
 // tag_tiles[f] (F+1 entries) counts the tiles of the frames whose component
@@ -1659,7 +1659,10 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
     __shared__ uint32_t s_tile, s_tmp[4];
     __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_words[(TAG_TILE * 4 * 63) / 32 + 4];
+    // widest value: 63 bits (values below 2^63), else the type's width
+    constexpr uint32_t MAXW = sizeof(ValT) == 8 ? 63u : 8u * (uint32_t)sizeof(ValT);
+    constexpr uint32_t RPT = TAG_TILE / CH_WG;   // records per thread
+    __shared__ uint32_t s_words[(TAG_TILE * DIMS * MAXW) / 32 + 4];
     const uint32_t* tile_begin = tag_tiles + c * (F + 1);
     // The grid is sized for the largest possible tile count, capped at
     // PACK_TAG_WGS workgroups; each workgroup takes tiles by ticket until
@@ -1676,18 +1679,18 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     if (threadIdx.x == 0) head_pos[tile] = ~0ull;
     const uint64_t r0 = (uint64_t)(tile - tile_begin[f]) * TAG_TILE;
     const ValT* src = vals + C.base * DIMS;
-    const uint64_t rt = r0 + threadIdx.x * 4;
-    // the thread's 4 records, loaded once and unconditionally (the tile's
+    const uint64_t rt = r0 + threadIdx.x * RPT;
+    // the thread's RPT records, loaded once and unconditionally (the tile's
     // first record past the end)
-    uint64_t xv[4][DIMS];
+    uint64_t xv[RPT][DIMS];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < (int)RPT; ++k)
 #pragma unroll
         for (int d = 0; d < DIMS; ++d) xv[k][d] = (uint64_t)src[(rt + k < C.n ? rt + k : r0) * DIMS + d];
-    uint32_t tg[4];
+    uint32_t tg[RPT];
     uint32_t bits = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < (int)RPT; ++k) {
         tg[k] = 0;
         if (rt + k < C.n) {
             uint64_t any = xv[k][0];   // bit length of the OR = widest bit length
@@ -1721,7 +1724,7 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     __syncthreads();
     uint32_t o = 8 * a + lb0 + pre;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < (int)RPT; ++k) {
         if (!tg[k]) continue;
         const uint32_t w = tg[k];
 #pragma unroll

@@ -1207,7 +1207,7 @@ __global__ void k_choose(CompInst* __restrict__ ci, Section* __restrict__ sec, u
 }
 
 // output layout: one thread walks the frames in order
-#define TAG_TILE 2048   // records of a Tagged packing tile (8 per thread)
+#define TAG_TILE 4096   // records of a Tagged packing tile (16 per thread)
 // 64-bit exclusive scan over the workgroup (the container offsets of a batch
 // can pass 2^32)
 __device__ inline uint64_t block_excl_scan_u64(uint64_t v, uint64_t* s_tmp64, uint64_t* total) {

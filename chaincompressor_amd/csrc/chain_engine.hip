@@ -621,7 +621,8 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
             constexpr int DD = decltype(dd)::value;
             using RelT = typename decltype(rt)::type;
             using DKT = typename decltype(dkt)::type;
-            hipLaunchKernelGGL((k_rans_encode<DD, RelT, DKT>), dim3((unsigned)chunk_bound), dim3(64), 0, st, idrel,
+            hipLaunchKernelGGL((k_rans_encode<DD, RelT, DKT>), dim3((unsigned)std::min<uint64_t>(chunk_bound, ENC_WGS)),
+                               dim3(64), 0, st, idrel,
                                (const DKT*)diff, (const DKT*)keep, choice2, (const RelT*)rel, F, x.ci, x.sec,
                                x.comp_chunk, x.pc, x.scratch, stride, x.chunk_len, x.chunk_pos);
         };
@@ -646,7 +647,8 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
     hipLaunchKernelGGL(k_write_components, dim3(ninst), dim3(CH_WG), 0, st, x.ci, x.sec, F, x.tbytes, x.chunk_len,
                        x.total, out_cap, x.out);
     if (chunk_bound)
-        hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)chunk_bound), dim3(CH_WG), 0, st, x.ci, x.sec, nsec,
+        hipLaunchKernelGGL(k_copy_chunks, dim3((unsigned)std::min<uint64_t>(chunk_bound, COPY_WGS)), dim3(CH_WG), 0,
+                           st, x.ci, x.sec, nsec,
                            x.comp_chunk, x.scratch, stride, x.chunk_len, x.chunk_pos, x.chunk_off, x.total, out_cap,
                            x.out);
     tm.end();

@@ -1076,6 +1076,8 @@ __device__ inline void enc_chunk_any(const ValT* __restrict__ vals, uint32_t chu
 
 // every component's chunks in one launch: block b codes chunk b of the
 // batch (the chunks of component c are [comp_chunk[c], comp_chunk[c+1]))
+#define ENC_WGS 8192u    // rANS coding workgroups (one wavefront each) at most
+#define COPY_WGS 4096u   // chunk copy workgroups at most
 template <int D, typename RelT, typename DKT>
 __global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__ idrel,
                                                     const DKT* __restrict__ diff,
@@ -1087,22 +1089,26 @@ __global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__
                                                     const uint64_t* __restrict__ pc_tab, uint8_t* __restrict__ scratch,
                                                     uint64_t stride, uint32_t* __restrict__ chunk_len,
                                                     uint32_t* __restrict__ chunk_pos) {
-    const uint32_t chunk = blockIdx.x;
-    if (chunk >= comp_chunk[5]) return;
+    __shared__ EncLds lds;
+    __shared__ __attribute__((aligned(16))) uint8_t ring[ENC_RING];
+    // the grid is bounded by the most chunks the batch can have; workgroups
+    // stride over the chunks the plan made (known on the device only)
+    const uint32_t nchunks = comp_chunk[5];
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     int c = 0;
     while (c < 4 && chunk >= comp_chunk[c + 1]) ++c;
     const uint32_t si = find_section(sec, 2 * F * c, 2 * F * (c + 1), chunk);
     const Section& S = sec[si];
     const CompInst& C = ci[si >> 1];
     const uint64_t* gpc = pc_tab + (uint64_t)si * RANS_VALUE_LIMIT;
-    __shared__ EncLds lds;
-    __shared__ __attribute__((aligned(16))) uint8_t ring[ENC_RING];
     switch (c) {
     case 0: enc_chunk_any<uint32_t, 1>(idrel, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     case 1: enc_chunk_any<DKT, D>(diff, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     case 2: enc_chunk_any<DKT, D>(keep, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     case 3: enc_chunk_any<uint8_t, 1>(choice2, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
     default: enc_chunk_any<RelT, D>(rel, chunk, si, C, S, gpc, lds, ring, scratch, stride, chunk_len, chunk_pos); break;
+    }
+    __syncthreads();   // LDS tables and ring are reloaded for the next chunk
     }
 }
 
@@ -1405,14 +1411,14 @@ __global__ __launch_bounds__(CH_WG) void k_copy_chunks(const CompInst* __restric
                                                        const uint64_t* __restrict__ out_total, uint64_t out_cap,
                                                        uint8_t* __restrict__ out) {
     if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
-    const uint32_t chunk = blockIdx.x;
-    if (chunk >= comp_chunk[5]) return;
+    const uint32_t nchunks = comp_chunk[5];
+    for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const uint32_t si = find_section(sec, 0, nsec, chunk);
     const Section& S = sec[si];
     const CompInst& C = ci[si >> 1];
     const bool tags = (si & 1) != 0;
     const bool used = tags ? C.kind == KIND_TAGR : C.kind == KIND_RANS;
-    if (!used) return;
+    if (!used) continue;
     uint8_t* dst = out + S.off + RVC1_HDR_FIXED + S.table_len + 4ull * S.nchunks + chunk_off[chunk];
     const uint8_t* src = scratch + (uint64_t)chunk * stride;
     const uint32_t nstate = 4u * S.lanes;
@@ -1433,6 +1439,7 @@ __global__ __launch_bounds__(CH_WG) void k_copy_chunks(const CompInst* __restric
             const uint32_t i = k + u * CH_WG;
             if (i < len) dst[nstate + i] = b[u];
         }
+    }
     }
 }
 

@@ -427,7 +427,7 @@ __device__ inline bool within_eb(double restored, double original, double eb) {
     return difference >= 0.0 ? remainder <= 0.0 : remainder >= 0.0;
 }
 
-#define QZ_GROUP 4   // points per thread whose loads are in flight together
+#define QZ_GROUP 8   // points per thread whose loads are in flight together (16: 120 VGPRs, slower)
 
 // Quantise the tile [start, end) of frame F: rel offsets and block keys to
 // global memory (and, with S_KEY, every key also to s_key[point - start]);

@@ -663,12 +663,17 @@ static void entropy_stage(hipStream_t st, StageTimer& tm, EntropyTimes& et, uint
                            x.total, out_cap, x.out);
         hipLaunchKernelGGL((k_pack_tagged<V, DC>), dim3(std::min<uint32_t>(ntiles1k, PACK_TAG_WGS)), dim3(CH_WG), 0, st,
                            (const V*)v, c, F,
-                           x.tag_tiles, x.ci, x.status1k + (uint64_t)c * (ntiles1k + 1), x.tickets + c, err, x.head,
-                           x.tail, x.head_pos, x.total, out_cap, x.out);
-        hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG)), dim3(CH_WG), 0, st, x.head, x.tail, x.head_pos,
-                           x.tag_tiles + c * (F + 1) + F, x.total, out_cap, x.out);
+                           x.tag_tiles, x.ci, x.status1k + (uint64_t)c * (ntiles1k + 1), x.tickets + c, err,
+                           x.head + (uint64_t)c * ntiles1k, x.tail + (uint64_t)c * ntiles1k,
+                           x.head_pos + (uint64_t)c * ntiles1k, x.total, out_cap, x.out);
         tm.end();
     });
+    // every component's seams in one launch
+    tm.begin("pack");
+    if (ntiles1k)
+        hipLaunchKernelGGL(k_tag_seams, dim3(cdiv(ntiles1k, CH_WG), 5), dim3(CH_WG), 0, st, x.head, x.tail,
+                           x.head_pos, ntiles1k, x.tag_tiles, F, x.total, out_cap, x.out);
+    tm.end();
 }
 
 // bytes of every value that shapes the launch sequence of a call
@@ -944,9 +949,10 @@ static bool compress_batch_try(chain_engine* e, const chain_frame* frames, uint3
         x.chunk_len = c.take<uint32_t>(chunk_bound);
         x.chunk_pos = c.take<uint32_t>(chunk_bound);
         x.chunk_off = c.take<uint32_t>(chunk_bound);
-        x.head = c.take<uint8_t>(ntiles1k);
-        x.tail = c.take<uint8_t>(ntiles1k);
-        x.head_pos = c.take<uint64_t>(ntiles1k);
+        // seam bytes of the Tagged tiles, a slice of ntiles1k per component
+        x.head = c.take<uint8_t>(5ull * ntiles1k);
+        x.tail = c.take<uint8_t>(5ull * ntiles1k);
+        x.head_pos = c.take<uint64_t>(5ull * ntiles1k);
         x.status1k = c.take<uint64_t>(5ull * (ntiles1k + 1));
         x.tickets = c.take<uint32_t>(5);
         x.tag_tiles = c.take<uint32_t>(5ull * (F + 1));

@@ -1785,14 +1785,19 @@ __global__ __launch_bounds__(CH_WG) void k_pack_tagged(const ValT* __restrict__ 
     }
 }
 
+// seams of every component (blockIdx.y): slice c of the seam arrays holds
+// its tiles; tag_tiles[c * (F + 1) + F] is its tile count
 __global__ void k_tag_seams(const uint8_t* __restrict__ head, const uint8_t* __restrict__ tail,
-                            const uint64_t* __restrict__ head_pos, const uint32_t* __restrict__ ntiles_dev,
+                            const uint64_t* __restrict__ head_pos, uint32_t stride,
+                            const uint32_t* __restrict__ tag_tiles, uint32_t F,
                             const uint64_t* __restrict__ out_total, uint64_t out_cap,
                             uint8_t* __restrict__ out) {
     if (*out_total > out_cap) return;   // the layout overran the output bound: nothing is written
+    const uint32_t c = blockIdx.y;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *ntiles_dev || head_pos[t] == ~0ull) return;
-    out[head_pos[t]] = head[t] | tail[t - 1];
+    const uint64_t o = (uint64_t)c * stride;
+    if (t >= tag_tiles[c * (F + 1) + F] || head_pos[o + t] == ~0ull) return;
+    out[head_pos[o + t]] = head[o + t] | tail[o + t - 1];
 }
 
 }  // namespace chain

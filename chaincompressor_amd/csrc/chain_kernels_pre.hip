@@ -1754,9 +1754,6 @@ __global__ __launch_bounds__(CH_WG) void k_l2_split(const DKT* __restrict__ bloc
                 chosen |= (c ? 1u : 0u) << k;
                 if (lane == 0) s_off[k * W + wave] = __popcll(m);
             }
-            // one group's loads in flight at a time (registers: the values of
-            // every item are held to the second pass)
-            __builtin_amdgcn_sched_barrier(0);
         }
         __syncthreads();
         if (threadIdx.x < 64) {

@@ -1419,7 +1419,7 @@ __global__ void k_dec_check_counts(const DecFrame* __restrict__ fr, uint32_t F, 
 // One workgroup per point tile of a frame (the frame's fields are uniform).
 // Each pass over DQ_ITEMS points per thread loads every root first, then
 // every block and offset those roots name, then computes and stores.
-#define DQ_ITEMS 8
+#define DQ_ITEMS 16
 #define DQ_STAGE 4
 // BT: element type of the L2-inverted block ids (32-bit when every frame's
 // wrap range lies in [0, 2^31))

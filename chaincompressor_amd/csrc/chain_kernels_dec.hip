@@ -1285,12 +1285,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __res
         dst[j] = out;
     }
     {
-        uint32_t inc = nx;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(inc, o, 64);
-            if (lane >= (uint32_t)o) inc += t;
-        }
+        const uint32_t inc = wave_incl_scan(nx);
         if (lane == 63) s_wtot[w] = inc;
         __syncthreads();
         if (threadIdx.x == 0) {

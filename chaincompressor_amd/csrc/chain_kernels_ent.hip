@@ -555,18 +555,12 @@ __global__ __launch_bounds__(CH_WG) void k_comp_reduce(const CompSlot* __restric
 // thread 0); s4 holds 4 entries
 __device__ inline uint32_t block_excl_max(uint32_t v, uint32_t* s4) {
     const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    uint32_t inc = v;
-#pragma unroll
-    for (int k = 1; k < 64; k <<= 1) {
-        const uint32_t t = __shfl_up(inc, k, 64);
-        if (lane >= (uint32_t)k) inc = t > inc ? t : inc;
-    }
+    const uint32_t inc = wave_incl_dpp(v, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
     if (lane == 63) s4[w] = inc;
     __syncthreads();
     uint32_t before = 0;
     for (uint32_t k = 0; k < w; ++k) before = s4[k] > before ? s4[k] : before;
-    uint32_t ex = __shfl_up(inc, 1, 64);
-    if (lane == 0) ex = 0;
+    const uint32_t ex = dpp_u32<0x138, 0xf>(0u, inc);   // wave_shr:1 (lane 0: 0)
     __syncthreads();
     return ex > before ? ex : before;
 }

@@ -1747,7 +1747,10 @@ static void parse_component(HostReader r, int comp, uint32_t expected_dims, uint
         j.dims = dims;
         j.tile0 = (uint32_t)P.tag_tiles;
         j.comp = comp;
-        P.tag_tiles += cdiv(n, DEC_TAG_TILE);
+        j.maxw = maxw;
+        // narrow: values fit the u32 staging, payload and values fit the LDS
+        j.recs = maxw <= 32 && maxw * dims <= 96 && dims <= 3 ? DEC_TAG_TILE2 : DEC_TAG_TILE;
+        P.tag_tiles += cdiv(n, j.recs);
         P.tagged.push_back(j);
     }
 }

@@ -562,6 +562,8 @@ struct DecTagged {
     uint64_t dst;               // first value index
     uint32_t dims, tile0;       // first tile of this job
     int32_t comp;
+    uint32_t maxw;              // the section's widest width (every tag is at most this)
+    uint32_t recs;              // records per tile: DEC_TAG_TILE, or DEC_TAG_TILE2 for narrow jobs
     uint32_t pad;
 };
 
@@ -571,43 +573,38 @@ struct DecTagged {
 // is cut from LDS.
 #define DEC_TAG_TILE 1024u
 #define DEC_TAG_ITEMS (DEC_TAG_TILE / CH_WG)
+// Narrow jobs (maxw <= 32, maxw * dims <= 96 bits a record, dims <= 3) take tiles of
+// DEC_TAG_TILE2 records: their payload (<= 24 KB) and their values staged as
+// u32 (<= 24 KB) fit the same LDS (half the tiles, look-back steps and
+// barriers per record).
+#define DEC_TAG_TILE2 2048u
 #define DEC_TAG_LDS (DEC_TAG_TILE * CH_MAXD * 63u / 8u + 16u)   // widest tile's bytes
 // staging words: the widest tile's bytes plus alignment and the windows' two
 // words past the end, or the tile's values as u64, whichever is more
 #define DEC_TAG_WORDS ((DEC_TAG_TILE * CH_MAXD * 2u > DEC_TAG_LDS / 4u + 4u) ? DEC_TAG_TILE * CH_MAXD * 2u \
                                                                           : DEC_TAG_LDS / 4u + 4u)
-__global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict__ in,
-                                                      const DecTagged* __restrict__ jobs, uint32_t njobs,
-                                                      const uint8_t* __restrict__ tags, uint64_t* status,
-                                                      uint32_t* ticket, uint32_t* idrel, void* diff, void* keep,
-                                                      uint32_t dk32, uint8_t* choice, void* rel, uint32_t rel8,
-                                                      uint32_t* err) {
-    __shared__ uint32_t s_tile, s_tmp[4];
-    __shared__ uint64_t s_excl;
-    // staged payload words, then the tile's decoded values
-    __shared__ uint32_t s_pay[DEC_TAG_WORDS];
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    uint32_t lo = 0, hi = njobs;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].tile0 <= tile) lo = mid; else hi = mid;
-    }
-    const DecTagged J = jobs[lo];
-    const uint64_t rfirst = (uint64_t)(tile - J.tile0) * DEC_TAG_TILE;   // < J.n
-    const uint64_t r0 = rfirst + (uint64_t)threadIdx.x * DEC_TAG_ITEMS;
-    uint32_t tg[DEC_TAG_ITEMS];
+// One tile of job J: ITEMS consecutive records per thread, values staged in
+// LDS as VT before the coalesced stores.
+template <uint32_t ITEMS, typename VT>
+__device__ __forceinline__ void tagged_tile(const uint8_t* __restrict__ in, const DecTagged& J, uint32_t tile,
+                                            const uint8_t* __restrict__ tags, uint64_t* status, uint32_t* s_tmp,
+                                            uint64_t* s_excl, uint32_t* s_pay, uint32_t* idrel, void* diff,
+                                            void* keep, uint32_t dk32, uint8_t* choice, void* rel, uint32_t rel8,
+                                            uint32_t* err) {
+    constexpr uint32_t RECS = ITEMS * CH_WG;
+    const uint64_t rfirst = (uint64_t)(tile - J.tile0) * RECS;   // < J.n
+    const uint64_t r0 = rfirst + (uint64_t)threadIdx.x * ITEMS;
+    uint32_t tg[ITEMS];
 #pragma unroll
-    for (int k = 0; k < DEC_TAG_ITEMS; ++k) tg[k] = tags[J.tag_off + (r0 + k < J.n ? r0 + k : rfirst)];
+    for (int k = 0; k < (int)ITEMS; ++k) tg[k] = tags[J.tag_off + (r0 + k < J.n ? r0 + k : rfirst)];
     uint32_t bits = 0;
 #pragma unroll
-    for (int k = 0; k < DEC_TAG_ITEMS; ++k) {
+    for (int k = 0; k < (int)ITEMS; ++k) {
         if (r0 + k >= J.n) {
             tg[k] = 0;
             continue;
         }
-        if (tg[k] < 1 || tg[k] > 63) {
+        if (tg[k] < 1 || tg[k] > J.maxw) {
             atomicOr(err, DERR_VALUE);
             tg[k] = 1;   // keep the work bounded; the stream is rejected
         }
@@ -617,12 +614,12 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
     const uint32_t pre = block_excl_scan(bits, s_tmp, &tot);
     if (threadIdx.x < 64) {
         const uint64_t e = lookback_seg(status, tile, J.tile0, tot, err);
-        if (threadIdx.x == 0) s_excl = e;
+        if (threadIdx.x == 0) *s_excl = e;
     }
     __syncthreads();
-    const uint64_t bit0 = s_excl;
+    const uint64_t bit0 = *s_excl;
     const uint64_t B0 = bit0 >> 3;
-    const uint64_t want = ((bit0 + tot + 7) >> 3) - B0;   // <= DEC_TAG_LDS
+    const uint64_t want = ((bit0 + tot + 7) >> 3) - B0;   // <= the staging bytes (widths <= maxw)
     const uint64_t have = J.bytes > B0 ? J.bytes - B0 : 0;
     const uint32_t nst = (uint32_t)(want < have ? want : have);
     if (want > have) {
@@ -656,9 +653,9 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
     // of staged byte o >> 3, and the words hold their bytes little-endian.
     uint32_t pos = 8 * a + (uint32_t)(bit0 & 7) + pre;
     const uint32_t dims = J.dims;
-    uint64_t val[DEC_TAG_ITEMS][CH_MAXD];
+    VT val[ITEMS][CH_MAXD];
 #pragma unroll
-    for (int k = 0; k < DEC_TAG_ITEMS; ++k) {
+    for (int k = 0; k < (int)ITEMS; ++k) {
         const uint32_t w = tg[k];
 #pragma unroll
         for (int d = 0; d < CH_MAXD; ++d) {
@@ -668,18 +665,18 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
             const uint64_t x = ((uint64_t)__builtin_bswap32(s_pay[wi]) << 32) | __builtin_bswap32(s_pay[wi + 1]);
             uint64_t v = (x << sh) >> (64u - w);
             if (sh + w > 64u) v |= (uint64_t)(__builtin_bswap32(s_pay[wi + 2]) >> (96u - sh - w));
-            val[k][d] = v;
+            val[k][d] = (VT)v;   // below 2^maxw: fits VT
             pos += w;
         }
     }
     // the tile's values in LDS (the staged payload is no longer needed), then
     // written in index order by consecutive threads
     __syncthreads();
-    uint64_t* const s_val = reinterpret_cast<uint64_t*>(s_pay);
-    const uint32_t nrec = J.n - rfirst < DEC_TAG_TILE ? (uint32_t)(J.n - rfirst) : DEC_TAG_TILE;
+    VT* const s_val = reinterpret_cast<VT*>(s_pay);
+    const uint32_t nrec = J.n - rfirst < RECS ? (uint32_t)(J.n - rfirst) : RECS;
 #pragma unroll
-    for (int k = 0; k < DEC_TAG_ITEMS; ++k) {
-        const uint32_t lr = threadIdx.x * DEC_TAG_ITEMS + k;
+    for (int k = 0; k < (int)ITEMS; ++k) {
+        const uint32_t lr = threadIdx.x * ITEMS + k;
         if (lr >= nrec) continue;
 #pragma unroll
         for (int d = 0; d < CH_MAXD; ++d)
@@ -688,7 +685,36 @@ __global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict_
     __syncthreads();
     const uint64_t g0 = J.dst + rfirst * dims;
     for (uint32_t e = threadIdx.x; e < nrec * dims; e += CH_WG)
-        store_value(J.comp, g0 + e, s_val[e], idrel, diff, keep, dk32, choice, rel, rel8, err);
+        store_value(J.comp, g0 + e, (uint64_t)s_val[e], idrel, diff, keep, dk32, choice, rel, rel8, err);
+}
+
+__global__ __launch_bounds__(CH_WG) void k_dec_tagged(const uint8_t* __restrict__ in,
+                                                      const DecTagged* __restrict__ jobs, uint32_t njobs,
+                                                      const uint8_t* __restrict__ tags, uint64_t* status,
+                                                      uint32_t* ticket, uint32_t* idrel, void* diff, void* keep,
+                                                      uint32_t dk32, uint8_t* choice, void* rel, uint32_t rel8,
+                                                      uint32_t* err) {
+    __shared__ uint32_t s_tile, s_tmp[4];
+    __shared__ uint64_t s_excl;
+    // staged payload words, then the tile's decoded values
+    __shared__ uint32_t s_pay[DEC_TAG_WORDS];
+    static_assert(DEC_TAG_TILE2 * 3 * 4 / 4 <= DEC_TAG_WORDS && DEC_TAG_TILE2 * 96 / 32 + 4 <= DEC_TAG_WORDS,
+                  "a narrow tile's payload and values fit the staging words");
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].tile0 <= tile) lo = mid; else hi = mid;
+    }
+    const DecTagged J = jobs[lo];
+    if (J.recs == DEC_TAG_TILE2)
+        tagged_tile<DEC_TAG_TILE2 / CH_WG, uint32_t>(in, J, tile, tags, status, s_tmp, &s_excl, s_pay, idrel, diff,
+                                                     keep, dk32, choice, rel, rel8, err);
+    else
+        tagged_tile<DEC_TAG_ITEMS, uint64_t>(in, J, tile, tags, status, s_tmp, &s_excl, s_pay, idrel, diff, keep,
+                                             dk32, choice, rel, rel8, err);
 }
 
 // ------------------------------------------------------------------------

@@ -1174,8 +1174,10 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
 #define CS_EXIT 0x10000u
 #define CS_PEND 0x20000u
 #define CS_WIN 0x40000u
+// Occupancy: held to 80 VGPRs for six waves per SIMD (81 unconstrained, five
+// waves; no spill at 80): 0.219 vs 0.237 ms on a 50M-point frame.
 template <int D>
-__global__ __launch_bounds__(CH_WG) void k_dec_chain_local(const DecFrame* __restrict__ fr, uint32_t F,
+__global__ __launch_bounds__(CH_WG) __attribute__((amdgpu_waves_per_eu(6))) void k_dec_chain_local(const DecFrame* __restrict__ fr, uint32_t F,
                                                            const uint32_t* __restrict__ tile_begin,
                                                            const uint32_t* __restrict__ idrel,
                                                            uint32_t* __restrict__ tile_expl_off, uint64_t* status,

@@ -1072,8 +1072,10 @@ __device__ inline void enc_chunk_any(const ValT* __restrict__ vals, uint32_t chu
 // batch (the chunks of component c are [comp_chunk[c], comp_chunk[c+1]))
 #define ENC_WGS 8192u    // rANS coding workgroups (one wavefront each) at most
 #define COPY_WGS 4096u   // chunk copy workgroups at most
+// Occupancy: held to 96 VGPRs for five single-wavefront workgroups per SIMD
+// (113 unconstrained; 88 and no spill under the bound).
 template <int D, typename RelT, typename DKT>
-__global__ __launch_bounds__(64) void k_rans_encode(const uint32_t* __restrict__ idrel,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_rans_encode(const uint32_t* __restrict__ idrel,
                                                     const DKT* __restrict__ diff,
                                                     const DKT* __restrict__ keep,
                                                     const uint8_t* __restrict__ choice2,

@@ -1643,8 +1643,12 @@ __global__ void k_l2_wrap(FrameOut* fo, uint32_t nframes, uint32_t* err) {
 #define L2S_WGS 2048u        // workgroups of a split launch at most
 // DKT: element type of the diff and keep streams (32-bit when every frame's
 // block ids, and so every wrapped difference, stay below 2^31)
+// Occupancy: up to three dimensions the kernel is held to 168 VGPRs for three
+// waves per SIMD (the 32-bit, three-dimension variant needs 178 unconstrained
+// and spills 12 at 168: 0.091 vs 0.110 ms on a 50M-point frame); at four
+// waves it would spill 66.
 template <int D, typename DKT>
-__global__ __launch_bounds__(CH_WG) void k_l2_split(const DKT* __restrict__ blocks,
+__global__ __launch_bounds__(CH_WG) __attribute__((amdgpu_waves_per_eu(D <= 3 ? 3 : 1))) void k_l2_split(const DKT* __restrict__ blocks,
                                                     const uint64_t* __restrict__ rec_total, uint32_t nframes,
                                                     const uint64_t* __restrict__ frame_rec_base,
                                                     const FrameDev* __restrict__ fr, FrameOut* __restrict__ fo,

@@ -935,7 +935,9 @@ __device__ inline Agg<V> l2_lookback(uint32_t tile, uint32_t seg, const Agg<V>& 
 // are scanned with shuffles (then over the wavefronts, then across tiles by
 // look-back), and the tile's records leave LDS as one coalesced run of
 // n*D words. LDS rows are padded by one word per CH_ITEMS records, so the
-// blocked reads (stride CH_ITEMS+1 words) hit distinct banks.
+// blocked reads (stride CH_ITEMS+1 words) hit distinct banks. The reset flags
+// are kept as one ballot word per 64 records, which holds the kernel's LDS at
+// 53 KB for three dimensions of 32-bit values (three workgroups per CU).
 #define L2_PAD(r) ((r) + ((r) / CH_ITEMS))
 #define L2_LDS_WORDS (CH_TILE + CH_TILE / CH_ITEMS)
 template <int D, typename V, typename DKT>
@@ -951,7 +953,8 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_zoff[NS];         // keep records before each segment
     __shared__ V s_v[D][L2_LDS_WORDS];
-    __shared__ uint8_t s_rs[L2_LDS_WORDS];
+    __shared__ uint64_t s_rsb[CH_TILE / 64];   // reset flags, bit r % 64 of word r / 64
+    static_assert(64 % CH_ITEMS == 0 && CH_WG % 64 == 0, "a thread's records share one flag word");
     __shared__ uint32_t s_wr[W];
     __shared__ V s_wv[W][D];
     if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
@@ -1050,13 +1053,16 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
             const uint32_t pos = L2_PAD(rl);
 #pragma unroll
             for (int d = 0; d < D; ++d) s_v[d][pos] = v[d];
-            s_rs[pos] = (uint8_t)rs;
+            const uint64_t rb = __ballot(rs != 0);   // records k*CH_WG + 64*wave + lane
+            if (lane == 0) s_rsb[k * W + wave] = rb;
         }
     }
     __syncthreads();
     // blocked: inclusive scan of the thread's records in registers; rbits
     // marks the records at or after the thread's first reset
     const uint32_t pbase = L2_PAD(threadIdx.x * CH_ITEMS);
+    // the thread's CH_ITEMS records share one flag word
+    const uint32_t rsm = (uint32_t)(s_rsb[(threadIdx.x * CH_ITEMS) / 64] >> ((threadIdx.x * CH_ITEMS) % 64));
     V iv[CH_ITEMS][D];
     uint32_t ar = 0, rbits = 0;
     V av[D];
@@ -1064,7 +1070,7 @@ __global__ __launch_bounds__(CH_WG) void k_dec_l2(const DecFrame* __restrict__ f
     for (int d = 0; d < D; ++d) av[d] = 0;
 #pragma unroll
     for (int j = 0; j < CH_ITEMS; ++j) {
-        const uint32_t rs = s_rs[pbase + j];
+        const uint32_t rs = (rsm >> j) & 1u;
         V v[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) v[d] = s_v[d][pbase + j];

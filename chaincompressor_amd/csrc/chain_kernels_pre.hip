@@ -668,7 +668,10 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
     }
     if ((uint64_t)blockIdx.x * CH_TILE >= T) return;
     __shared__ uint32_t s_lstart[256], s_goff[256], s_tmp[4];
-    __shared__ uint32_t s_wcnt[4][256];
+    // per-wavefront digit counts, then run starts: at most CH_TILE, so 16 bits
+    // (2 KB less LDS: three workgroups per CU with 32-bit keys and 64-bit values)
+    __shared__ uint16_t s_wcnt[4][256];
+    static_assert(CH_TILE <= 65535, "run starts fit 16 bits");
     __shared__ KeyT s_key[CH_TILE];
     __shared__ ValT s_val[CH_TILE];
     const uint32_t tid = threadIdx.x, w = tid >> 6, lane = tid & 63u;
@@ -717,7 +720,7 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
         wr[r] = before + prank;
         // lanes of one digit read the count before its leader updates it
         __builtin_amdgcn_wave_barrier();
-        if (valid && prank == 0) s_wcnt[w][digit] = before + (uint32_t)__popcll(peers);
+        if (valid && prank == 0) s_wcnt[w][digit] = (uint16_t)(before + (uint32_t)__popcll(peers));
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
@@ -725,10 +728,10 @@ __global__ __launch_bounds__(CH_WG) void k_sort_scatter(const KeyT* __restrict__
         // digit tid: start of every wavefront's run in the tile
         const uint32_t a0 = s_wcnt[0][tid], a1 = s_wcnt[1][tid], a2 = s_wcnt[2][tid];
         const uint32_t st = s_lstart[tid];
-        s_wcnt[0][tid] = st;
-        s_wcnt[1][tid] = st + a0;
-        s_wcnt[2][tid] = st + a0 + a1;
-        s_wcnt[3][tid] = st + a0 + a1 + a2;
+        s_wcnt[0][tid] = (uint16_t)st;
+        s_wcnt[1][tid] = (uint16_t)(st + a0);
+        s_wcnt[2][tid] = (uint16_t)(st + a0 + a1);
+        s_wcnt[3][tid] = (uint16_t)(st + a0 + a1 + a2);
     }
     __syncthreads();
 #pragma unroll
